@@ -1,0 +1,906 @@
+// Schur-complement elimination on gfx950: reset_schur_{sparse,dense}!, reduce_schur_{sparse,dense}!,
+// backsubstitute! and the copyto! gather, behind the C ABI of include/bls_amd.h.
+//
+// Phase 1 (factor_eliminated!, src/schur_reduction.jl:121-162): one lane per eliminated block of a run of equal
+// d_e; closed-form inverse for d_e <= 3 and a register Cholesky for 4..8 (inv_sym_block, src/kernels.jl:14-83),
+// written without contraction so the acceptance test makes the reference's decisions; caches E_inv and z.
+//
+// Phase 2 (accumulate_*_columns!, src/schur_reduction.jl:262-380) is owner-computes, like the reference's
+// col_chunks: a workgroup owns an "item" -- a run of consecutive block rows ("slots") of one block column r2 of S
+// -- and each lane owns one (slot, row) of that column, holding the d_r2 entries of that row in registers. The
+// item's landmarks (the couplings of r2 in ascending eliminated block, the reference's order) are streamed in
+// batches: their E_inv, z and coupling blocks are staged in LDS with coalesced loads, then every lane applies,
+// in order, the landmarks that touch its slot -- W_c1[:,row] = E_inv C_c1[:,row] recomputed in registers and
+// S[row, col] -= W_c1[:,row] . C_c2[:,col]. Each S entry therefore receives its updates in the reference's
+// sequence from one lane: no float atomics, and the result is bitwise reproducible run to run.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bls_internal.h"
+
+using namespace bls;
+
+namespace {
+
+constexpr int P2_THREADS = 256;
+constexpr int P2_MAXE = 64;      // entries (landmarks) per staged batch
+constexpr int P2_LDS = 4096;     // doubles of staged data per batch (32 KB)
+constexpr int MAXDR = 16;        // largest retained block handled
+
+struct P2Item {
+    int64_t base;        // S index of (row 0, col 0) of the column, before the slot's row base
+    int64_t colstride;   // distance between scalar columns of the block column
+    int32_t r2, dr2, ro2;
+    int32_t slot_lo, nslots, ntasks;
+    int32_t has_diag, uniform_dr;
+    int32_t batch_lo, batch_hi;
+};
+
+struct P2Entry {
+    int32_t e, c_first, c2, pad;
+    uint64_t mask;
+};
+
+struct P2Args {
+    const P2Item *items;
+    int64_t nitems;
+    const P2Entry *entries;
+    const int32_t *batch_ptr;
+    const int64_t *slot_rowbase;
+    const int32_t *slot_dr;
+    const int64_t *eoffsets, *boffsets_e, *coffsets;
+    const int32_t *e_dims;
+    const double *E_inv, *z, *C;
+    const double *b_r, *s_r;
+    double *S, *g;
+    const unsigned long long *flag;
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// inv_sym_block (src/kernels.jl:14-83). H: DE x DE column-major (scaled), damp: lambda*scale (already formed).
+// ---------------------------------------------------------------------------------------------------------
+#pragma clang fp contract(off)
+template <int DE>
+__device__ inline bool inv_sym(const double *H, const double *damp, double *inv) {
+    const double tol = 4504.0 * 2.220446049250313e-16;
+    if constexpr (DE == 1) {
+        const double h11 = H[0] + damp[0];
+        if (!(h11 > 0) || !isfinite(h11)) return false;
+        inv[0] = 1.0 / h11;
+        return true;
+    } else if constexpr (DE == 2) {
+        const double H11 = H[0] + damp[0], H22 = H[3] + damp[1], H12 = H[2];
+        const double det = H11 * H22 - H12 * H12;
+        if (!(H11 > 0) || !(det > tol * (H11 * H22)) || !isfinite(det)) return false;
+        const double id = 1.0 / det;
+        inv[0] = H22 * id; inv[1] = -H12 * id; inv[2] = -H12 * id; inv[3] = H11 * id;
+        return true;
+    } else if constexpr (DE == 3) {
+        const double H11 = H[0] + damp[0], H22 = H[4] + damp[1], H33 = H[8] + damp[2];
+        const double H12 = H[3], H13 = H[6], H23 = H[7];
+        const double c11 = H22 * H33 - H23 * H23;
+        const double c12 = H13 * H23 - H12 * H33;
+        const double c13 = H12 * H23 - H13 * H22;
+        const double c22 = H11 * H33 - H13 * H13;
+        const double c23 = H12 * H13 - H11 * H23;
+        const double c33 = H11 * H22 - H12 * H12;
+        const double det = H11 * c11 + H12 * c12 + H13 * c13;
+        if (!(H11 > 0) || !(c33 > tol * (H11 * H22)) || !(det > tol * (c33 * H33)) || !isfinite(det)) return false;
+        const double id = 1.0 / det;
+        inv[0] = c11 * id; inv[1] = c12 * id; inv[2] = c13 * id;
+        inv[3] = c12 * id; inv[4] = c22 * id; inv[5] = c23 * id;
+        inv[6] = c13 * id; inv[7] = c23 * id; inv[8] = c33 * id;
+        return true;
+    } else {
+        // Cholesky of Symmetric(H + diag(damp), :U): U'U, every pivot ratio |U_ii|^2 / Hd_ii > tol; inv = U^-1 U^-T
+        double U[DE * DE];
+#pragma unroll
+        for (int q = 0; q < DE * DE; q++) U[q] = 0.0;
+#pragma unroll
+        for (int j = 0; j < DE; j++) {
+#pragma unroll
+            for (int i = 0; i <= j; i++) {
+                double a = H[j * DE + i] + (i == j ? damp[i] : 0.0);
+                for (int k = 0; k < i; k++) a -= U[i * DE + k] * U[j * DE + k];
+                if (i == j) {
+                    if (!(a > 0) || !isfinite(a)) return false;
+                    U[j * DE + j] = sqrt(a);
+                } else {
+                    U[j * DE + i] = a / U[i * DE + i];
+                }
+            }
+        }
+        double min_ratio = INFINITY;
+        bool nan_ratio = false;
+#pragma unroll
+        for (int i = 0; i < DE; i++) {
+            const double r = (U[i * DE + i] * U[i * DE + i]) / (H[i * DE + i] + damp[i]);
+            if (isnan(r)) nan_ratio = true;
+            min_ratio = r < min_ratio ? r : min_ratio;
+        }
+        if (nan_ratio || !(min_ratio > tol && isfinite(min_ratio))) return false;
+        double Ui[DE * DE];
+#pragma unroll
+        for (int q = 0; q < DE * DE; q++) Ui[q] = 0.0;
+#pragma unroll
+        for (int j = 0; j < DE; j++) {
+            Ui[j * DE + j] = 1.0 / U[j * DE + j];
+            for (int i = j - 1; i >= 0; i--) {
+                double s = 0.0;
+                for (int k = i + 1; k <= j; k++) s += U[k * DE + i] * Ui[j * DE + k];
+                Ui[j * DE + i] = -s / U[i * DE + i];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < DE; j++)
+#pragma unroll
+            for (int i = 0; i < DE; i++) {
+                double s = 0.0;
+                for (int k = (i > j ? i : j); k < DE; k++) s += Ui[k * DE + i] * Ui[k * DE + j];
+                inv[j * DE + i] = s;
+            }
+        return true;
+    }
+}
+
+// Phase 1 over one run of equal d_e: blocks first..first+count-1 (0-based), E/E_inv at eoff0 + i*DE^2,
+// b_e/z/scaling at boff0 + i*DE.
+template <int DE>
+__global__ __launch_bounds__(256) void k_phase1(int64_t first, int64_t count, int64_t eoff0, int64_t boff0,
+                                                const double *__restrict__ E, const double *__restrict__ b_e,
+                                                const double *__restrict__ scaling_e, double lambda,
+                                                const double *__restrict__ s_e, double *__restrict__ E_inv,
+                                                double *__restrict__ z, unsigned long long *flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int64_t eo = eoff0 + i * DE * DE, bo = boff0 + i * DE;
+    double se[DE], damp[DE], H[DE * DE], inv[DE * DE];
+#pragma unroll
+    for (int k = 0; k < DE; k++) {
+        se[k] = s_e ? s_e[bo + k] : 1.0;
+        damp[k] = (lambda * scaling_e[bo + k]) * (se[k] * se[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < DE * DE; k++) H[k] = s_e ? E[eo + k] * (se[k % DE] * se[k / DE]) : E[eo + k];
+    if (!inv_sym<DE>(H, damp, inv)) {
+        atomicMin(flag, (unsigned long long)(first + i + 1));
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < DE * DE; k++) E_inv[eo + k] = inv[k];
+#pragma unroll
+    for (int r = 0; r < DE; r++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < DE; j++) acc += inv[j * DE + r] * (b_e[bo + j] * se[j]);
+        z[bo + r] = acc;
+    }
+}
+#pragma clang fp contract(fast)
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+__global__ void k_finalize_flag(const unsigned long long *flag, int64_t *out) {
+    const unsigned long long f = *flag;
+    *out = f == (unsigned long long)NO_FAILURE ? 0 : (int64_t)f;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Phase 2
+// ---------------------------------------------------------------------------------------------------------
+template <int DE>
+__global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
+    __shared__ double lds[P2_LDS];
+    __shared__ int s_off[P2_MAXE + 1];
+    __shared__ int s_c1off[P2_MAXE], s_c2off[P2_MAXE], s_size[P2_MAXE], s_de[P2_MAXE];
+    __shared__ unsigned long long s_mask[P2_MAXE];
+    __shared__ int64_t s_src_e[P2_MAXE], s_src_z[P2_MAXE], s_src_c1[P2_MAXE], s_src_c2[P2_MAXE];
+    __shared__ int s_slot_dr[64], s_cum[65];
+
+    if (*a.flag != (unsigned long long)NO_FAILURE) return;   // phase 1 failed: S and g_S stay untouched
+    const int64_t idx = xcd_remap(blockIdx.x, gridDim.x);
+    if (idx >= a.nitems) return;
+    const P2Item it = a.items[idx];
+    const int t = threadIdx.x;
+    if (t < it.nslots) s_slot_dr[t] = a.slot_dr[it.slot_lo + t];
+    __syncthreads();
+    if (t == 0) {
+        int c = 0;
+        for (int s = 0; s < it.nslots; s++) { s_cum[s] = c; c += s_slot_dr[s]; }
+        s_cum[it.nslots] = c;
+    }
+    __syncthreads();
+    int my_slot = -1, my_row = 0;
+    if (t < it.ntasks) {
+        int s = 0;
+        while (s_cum[s + 1] <= t) s++;
+        my_slot = s;
+        my_row = t - s_cum[s];
+    }
+    const bool diag = it.has_diag && my_slot == it.nslots - 1;
+    const int dr2 = it.dr2;
+    double acc[MAXDR];
+    double gacc = 0.0;
+    int64_t rowaddr = 0;
+    if (my_slot >= 0) {
+        rowaddr = it.base + a.slot_rowbase[it.slot_lo + my_slot] + my_row;
+#pragma unroll
+        for (int c = 0; c < MAXDR; c++)
+            if (c < dr2 && (!diag || c >= my_row)) acc[c] = a.S[rowaddr + c * it.colstride];
+        if (diag) gacc = a.b_r[it.ro2 + my_row] * (a.s_r ? a.s_r[it.ro2 + my_row] : 1.0);
+    }
+    const unsigned long long lowmask = my_slot > 0 ? ((1ull << my_slot) - 1ull) : 0ull;
+    const int udr = it.uniform_dr;
+
+    for (int b = it.batch_lo; b < it.batch_hi; b++) {
+        const int j0 = a.batch_ptr[b];
+        const int nb = a.batch_ptr[b + 1] - j0;
+        if (t < nb) {
+            const P2Entry en = a.entries[j0 + t];
+            const int de = DE ? DE : a.e_dims[en.e];
+            int len1;
+            if (udr) {
+                len1 = __popcll(en.mask) * de * udr;
+            } else {
+                len1 = 0;
+                unsigned long long m = en.mask;
+                while (m) { const int s = __builtin_ctzll(m); m &= m - 1; len1 += de * s_slot_dr[s]; }
+            }
+            s_mask[t] = en.mask;
+            s_de[t] = de;
+            s_src_e[t] = a.eoffsets[en.e];
+            s_src_z[t] = a.boffsets_e[en.e];
+            s_src_c1[t] = a.coffsets[en.c_first];
+            s_src_c2[t] = a.coffsets[en.c2];
+            s_size[t] = de * de + de + (it.has_diag ? 0 : de * dr2) + len1;
+            s_c1off[t] = len1;   // temporarily the range length
+        }
+        __syncthreads();
+        if (t == 0) {
+            int o = 0;
+            for (int j = 0; j < nb; j++) {
+                const int de = s_de[j];
+                s_off[j] = o;
+                const int len1 = s_c1off[j];
+                const int c1 = o + de * de + de + (it.has_diag ? 0 : de * dr2);
+                s_c1off[j] = c1;
+                s_c2off[j] = it.has_diag ? c1 + len1 - de * dr2 : o + de * de + de;
+                o += s_size[j];
+            }
+            s_off[nb] = o;
+        }
+        __syncthreads();
+        const int L = s_off[nb];
+        for (int p = t; p < L; p += P2_THREADS) {
+            int lo = 0, hi = nb;   // entry j with s_off[j] <= p < s_off[j+1]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= p) lo = mid; else hi = mid;
+            }
+            const int j = lo;
+            const int de = DE ? DE : s_de[j];
+            int q = p - s_off[j];
+            double v;
+            if (q < de * de) {
+                v = a.E_inv[s_src_e[j] + q];
+            } else if ((q -= de * de) < de) {
+                v = a.z[s_src_z[j] + q];
+            } else {
+                q -= de;
+                if (!it.has_diag && q < de * dr2) v = a.C[s_src_c2[j] + q];
+                else v = a.C[s_src_c1[j] + (it.has_diag ? q : q - de * dr2)];
+            }
+            lds[p] = v;
+        }
+        __syncthreads();
+        if (my_slot >= 0) {
+            unsigned long long bits = 0;
+            for (int j = 0; j < nb; j++) bits |= ((s_mask[j] >> my_slot) & 1ull) << j;
+            while (bits) {
+                const int j = __builtin_ctzll(bits);
+                bits &= bits - 1;
+                const int de = DE ? DE : s_de[j];
+                const unsigned long long m = s_mask[j];
+                int pre;
+                if (udr) {
+                    pre = __popcll(m & lowmask) * de * udr;
+                } else {
+                    pre = 0;
+                    unsigned long long mm = m & lowmask;
+                    while (mm) { const int s = __builtin_ctzll(mm); mm &= mm - 1; pre += de * s_slot_dr[s]; }
+                }
+                const double *Ei = lds + s_off[j];
+                const double *zz = Ei + de * de;
+                const double *C2 = lds + s_c2off[j];
+                const double *C1 = lds + s_c1off[j] + pre + my_row * de;
+                double w[DE ? DE : 8];
+                if constexpr (DE > 0) {
+#pragma unroll
+                    for (int i = 0; i < DE; i++) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int k = 0; k < DE; k++) s += Ei[i + k * DE] * C1[k];
+                        w[i] = s;
+                    }
+#pragma unroll
+                    for (int c = 0; c < MAXDR; c++) {
+                        if (c < dr2 && (!diag || c >= my_row)) {
+                            double s = 0.0;
+#pragma unroll
+                            for (int i = 0; i < DE; i++) s += w[i] * C2[c * DE + i];
+                            acc[c] -= s;
+                        }
+                    }
+                    if (diag) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int i = 0; i < DE; i++) s += C2[my_row * DE + i] * zz[i];
+                        gacc -= s;
+                    }
+                } else {
+                    for (int i = 0; i < de; i++) {
+                        double s = 0.0;
+                        for (int k = 0; k < de; k++) s += Ei[i + k * de] * C1[k];
+                        w[i] = s;
+                    }
+#pragma unroll
+                    for (int c = 0; c < MAXDR; c++) {
+                        if (c < dr2 && (!diag || c >= my_row)) {
+                            double s = 0.0;
+                            for (int i = 0; i < de; i++) s += w[i] * C2[c * de + i];
+                            acc[c] -= s;
+                        }
+                    }
+                    if (diag) {
+                        double s = 0.0;
+                        for (int i = 0; i < de; i++) s += C2[my_row * de + i] * zz[i];
+                        gacc -= s;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (my_slot >= 0) {
+#pragma unroll
+        for (int c = 0; c < MAXDR; c++)
+            if (c < dr2 && (!diag || c >= my_row)) a.S[rowaddr + c * it.colstride] = acc[c];
+        if (diag) a.g[it.ro2 + my_row] = gacc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// reset_schur_* and copyto!
+// ---------------------------------------------------------------------------------------------------------
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_scatter_r(double *nzval, const double *R, const int64_t *r_to_s, int64_t nR) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t idx = r_to_s[i];
+        if (idx > 0) nzval[idx - 1] = R[i];
+    }
+}
+
+__global__ void k_add_damping(double *nzval, const int64_t *diag_inds, int64_t nr, const double *scaling_r,
+                              double lambda, const double *s_r) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nr) return;
+    {
+#pragma clang fp contract(off)
+        const double si = s_r ? s_r[i] : 1.0;
+        const int64_t k = diag_inds[i] - 1;
+        nzval[k] = nzval[k] + lambda * scaling_r[i] * (si * si);
+    }
+}
+
+__global__ void k_scale_sparse(double *nzval, const int64_t *S_colptr, const int64_t *S_rowval, int64_t nr,
+                               const double *s_r) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nr) return;
+    for (int64_t k = S_colptr[j] - 1; k < S_colptr[j + 1] - 1; k++) nzval[k] = nzval[k] * (s_r[S_rowval[k] - 1] * s_r[j]);
+}
+
+__global__ void k_reset_dense(double *S, const double *R, int64_t nr, const double *scaling_r, double lambda,
+                              const double *s_r) {
+    const int64_t n2 = nr * nr;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = k % nr, j = k / nr;
+        double v = s_r ? R[k] * (s_r[i] * s_r[j]) : R[k];
+        if (i == j) {
+#pragma clang fp contract(off)
+            const double si = s_r ? s_r[i] : 1.0;
+            v = v + lambda * scaling_r[i] * (si * si);
+        }
+        S[k] = v;
+    }
+}
+
+__global__ void k_gather(double *dest, const double *src, const int64_t *map, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dest[i] = src[map[i] - 1];
+}
+
+// backsubstitute! (src/schur_reduction.jl:541-576): one lane per eliminated block
+__global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eoffsets, const int64_t *boffsets_e,
+                          const int64_t *eliminated_ptr, const int32_t *entry_retained, const int64_t *coffsets,
+                          const int32_t *r_dims, const int64_t *boffsets_r, const double *C, const double *E_inv,
+                          const double *z, const double *x_r, const double *s_r, const double *s_e, double *x_e) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= neb) return;
+    const int de = e_dims[e];
+    const int64_t eo = eoffsets[e], bo = boffsets_e[e];
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t c = eliminated_ptr[e]; c < eliminated_ptr[e + 1]; c++) {
+        const int r = entry_retained[c];
+        const int dr = r_dims[r];
+        const int64_t ro = boffsets_r[r], co = coffsets[c];
+        for (int ir = 0; ir < dr; ir++) {
+            const double xc = x_r[ro + ir];
+            const double f = s_r ? s_r[ro + ir] : 1.0;
+            for (int i = 0; i < de; i++) {
+                const double cv = (s_r || s_e) ? C[co + ir * de + i] * ((s_e ? s_e[bo + i] : 1.0) * f) : C[co + ir * de + i];
+                t[i] += cv * xc;
+            }
+        }
+    }
+    for (int i = 0; i < de; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < de; j++) acc += E_inv[eo + j * de + i] * t[j];
+        x_e[bo + i] = z[bo + i] - acc;
+    }
+}
+
+struct Run {
+    int64_t first, count, de, eoff0, boff0;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------
+// The handle
+// ---------------------------------------------------------------------------------------------------------
+struct bls_schur {
+    int64_t nr = 0, nrb = 0, neb = 0, ncoup = 0;
+    std::vector<int64_t> r_dims, e_dims, boffsets_r, boffsets_e, eoffsets, eliminated_ptr, entry_retained, coffsets,
+        retained_ptr, retained_entry;   // host, 0-based values
+    std::vector<Run> runs;
+    int uniform_de = 0;
+    int mode = 0;   // 1 sparse, 2 dense
+    int64_t nnz = 0, nR = 0;
+    // device
+    DevArray<int64_t> d_eoffsets, d_boffsets_e, d_coffsets, d_eliminated_ptr, d_boffsets_r;
+    DevArray<int32_t> d_e_dims, d_r_dims, d_entry_retained;
+    DevArray<int64_t> d_r_to_s, d_diag_inds, d_S_colptr, d_S_rowval;
+    DevArray<P2Item> d_items;
+    DevArray<P2Entry> d_entries;
+    DevArray<int32_t> d_batch_ptr, d_slot_dr;
+    DevArray<int64_t> d_slot_rowbase;
+    DevArray<unsigned long long> d_flag;
+    DevArray<int64_t> d_failed_out;
+    int64_t nitems = 0, nentries = 0;
+    Timing timing;
+};
+
+namespace {
+
+int check_structure(const bls_schur *h) {
+    for (int64_t e = 0; e < h->neb; e++)
+        if (h->e_dims[e] < 1 || h->e_dims[e] > 8)
+            return fail(BLS_ERR_UNSUPPORTED, "eliminated block " + std::to_string(e + 1) + " has dimension " +
+                                                 std::to_string(h->e_dims[e]) + "; the device path supports 1 to 8");
+    for (int64_t r = 0; r < h->nrb; r++)
+        if (h->r_dims[r] < 1 || h->r_dims[r] > MAXDR)
+            return fail(BLS_ERR_UNSUPPORTED, "retained block " + std::to_string(r + 1) + " has dimension " +
+                                                 std::to_string(h->r_dims[r]) + "; the device path supports 1 to 16");
+    return BLS_OK;
+}
+
+// Build the phase-2 work: items (slot runs of block columns), their landmark entries, and the staging batches.
+// `slots_of(r2)` gives the block rows of column r2 (ascending, last == r2), `rowbase_of(r2, k)` the row base of
+// its k-th block, and base/colstride the column's addressing.
+template <typename Slots, typename RowBase, typename Base>
+int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of) {
+    std::vector<P2Item> items;
+    std::vector<P2Entry> entries;
+    std::vector<int32_t> batch_ptr{0};
+    std::vector<int64_t> slot_rowbase;
+    std::vector<int32_t> slot_dr;
+    std::vector<int64_t> owner(h->ncoup);
+    for (int64_t e = 0; e < h->neb; e++)
+        for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++) owner[c] = e;
+    std::vector<int64_t> col;
+    for (int64_t r2 = 0; r2 < h->nrb; r2++) {
+        col.clear();
+        slots_of(r2, col);
+        if (col.empty() || col.back() != r2)
+            return fail(BLS_ERR_ARG, "the reduced-system pattern has no diagonal block " + std::to_string(r2 + 1));
+        const int dr2 = (int)h->r_dims[r2];
+        int64_t base, colstride;
+        base_of(r2, base, colstride);
+        size_t k0 = 0;
+        while (k0 < col.size()) {
+            // slot run: at most 64 slots and 256 (slot, row) lanes
+            size_t k1 = k0;
+            int rows = 0;
+            while (k1 < col.size() && k1 - k0 < 64 && rows + h->r_dims[col[k1]] <= P2_THREADS) rows += (int)h->r_dims[col[k1++]];
+            P2Item it{};
+            it.base = base;
+            it.colstride = colstride;
+            it.r2 = (int32_t)r2;
+            it.dr2 = dr2;
+            it.ro2 = (int32_t)h->boffsets_r[r2];
+            it.slot_lo = (int32_t)slot_dr.size();
+            it.nslots = (int32_t)(k1 - k0);
+            it.ntasks = rows;
+            it.has_diag = k1 == col.size();
+            int udr = (int)h->r_dims[col[k0]];
+            for (size_t k = k0; k < k1; k++) {
+                slot_dr.push_back((int32_t)h->r_dims[col[k]]);
+                slot_rowbase.push_back(rowbase_of(r2, k, col[k]));
+                if (h->r_dims[col[k]] != udr) udr = 0;
+            }
+            it.uniform_dr = udr;
+            const int64_t r1_lo = col[k0], r1_hi = col[k1 - 1];
+            it.batch_lo = (int32_t)batch_ptr.size() - 1;
+            int nb = 0, lds = 0;
+            for (int64_t kk = h->retained_ptr[r2]; kk < h->retained_ptr[r2 + 1]; kk++) {
+                const int64_t c2 = h->retained_entry[kk];
+                const int64_t e = owner[c2];
+                const int64_t cs = h->eliminated_ptr[e];
+                // couplings of e with retained block in [r1_lo, r1_hi] (entries ascending, all <= r2)
+                const int64_t *er = h->entry_retained.data();
+                const int64_t ca = std::lower_bound(er + cs, er + c2 + 1, r1_lo) - er;
+                const int64_t cb = std::upper_bound(er + cs, er + c2 + 1, r1_hi) - er;   // exclusive
+                if (ca >= cb) continue;
+                unsigned long long mask = 0;
+                size_t k = k0;
+                int len1 = 0;
+                for (int64_t c = ca; c < cb; c++) {
+                    while (k < k1 && col[k] < er[c]) k++;
+                    if (k == k1 || col[k] != er[c])
+                        return fail(BLS_ERR_ARG, "The sparse pattern has no block (" + std::to_string(er[c] + 1) + ", " +
+                                                     std::to_string(r2 + 1) + ")");
+                    mask |= 1ull << (k - k0);
+                    len1 += (int)(h->e_dims[e] * h->r_dims[er[c]]);
+                }
+                const int de = (int)h->e_dims[e];
+                const int size = de * de + de + (it.has_diag ? 0 : de * dr2) + len1;
+                if (size > P2_LDS) return fail(BLS_ERR_UNSUPPORTED, "landmark data exceeds the staging buffer");
+                if (nb == P2_MAXE || lds + size > P2_LDS) {
+                    batch_ptr.push_back((int32_t)entries.size());
+                    nb = 0;
+                    lds = 0;
+                }
+                P2Entry en{};
+                en.e = (int32_t)e;
+                en.c_first = (int32_t)ca;
+                en.c2 = (int32_t)c2;
+                en.mask = mask;
+                entries.push_back(en);
+                nb++;
+                lds += size;
+            }
+            if (nb > 0) batch_ptr.push_back((int32_t)entries.size());
+            it.batch_hi = (int32_t)batch_ptr.size() - 1;
+            items.push_back(it);
+            k0 = k1;
+        }
+    }
+    if (entries.size() > (size_t)INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "too many phase-2 entries");
+    int rc;
+    if ((rc = h->d_items.upload(items)) || (rc = h->d_entries.upload(entries)) || (rc = h->d_batch_ptr.upload(batch_ptr)) ||
+        (rc = h->d_slot_rowbase.upload(slot_rowbase)) || (rc = h->d_slot_dr.upload(slot_dr)))
+        return rc;
+    h->nitems = (int64_t)items.size();
+    h->nentries = (int64_t)entries.size();
+    return BLS_OK;
+}
+
+template <int DE> void launch_phase1(const Run &r, const double *E, const double *b_e, const double *sc, double lambda,
+                                     const double *s_e, double *E_inv, double *z, unsigned long long *flag,
+                                     hipStream_t s) {
+    k_phase1<DE><<<grid_for(r.count, 256), 256, 0, s>>>(r.first, r.count, r.eoff0, r.boff0, E, b_e, sc, lambda, s_e,
+                                                         E_inv, z, flag);
+}
+
+int phase1(bls_schur *h, const double *E, const double *b_e, const double *sc, double lambda, const double *s_e,
+           double *E_inv, double *z, hipStream_t s) {
+    for (const Run &r : h->runs) {
+        if (r.count == 0) continue;
+        switch (r.de) {
+        case 1: launch_phase1<1>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 2: launch_phase1<2>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 3: launch_phase1<3>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 4: launch_phase1<4>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 5: launch_phase1<5>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 6: launch_phase1<6>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 7: launch_phase1<7>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 8: launch_phase1<8>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        default: return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension above 8");
+        }
+    }
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double *b_r, const double *b_e,
+                 const double *scaling_e, double lambda, const double *E, const double *C, double *E_inv, double *z,
+                 const double *s_r, const double *s_e, int64_t *d_failed, hipStream_t s) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (h->mode != want_mode)
+        return fail(BLS_ERR_STATE, want_mode == 1 ? "handle has no sparse layout (bls_schur_set_sparse)"
+                                                  : "handle is not dense (bls_schur_set_dense)");
+    if (!(std::isfinite(lambda) && lambda >= 0))
+        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
+    if (s_r || s_e) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
+    if (!S || !g || !b_r || !scaling_e || !E || !C || !E_inv || !z || !d_failed || (h->neb && !b_e))
+        return fail(BLS_ERR_ARG, "reduce_schur: a required array is NULL");
+    h->timing.mark(1, s);
+    k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
+    int rc = phase1(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, s);
+    if (rc) return rc;
+    h->timing.mark(2, s);
+    P2Args a{};
+    a.items = h->d_items.ptr;
+    a.nitems = h->nitems;
+    a.entries = h->d_entries.ptr;
+    a.batch_ptr = h->d_batch_ptr.ptr;
+    a.slot_rowbase = h->d_slot_rowbase.ptr;
+    a.slot_dr = h->d_slot_dr.ptr;
+    a.eoffsets = h->d_eoffsets.ptr;
+    a.boffsets_e = h->d_boffsets_e.ptr;
+    a.coffsets = h->d_coffsets.ptr;
+    a.e_dims = h->d_e_dims.ptr;
+    a.E_inv = E_inv;
+    a.z = z;
+    a.C = C;
+    a.b_r = b_r;
+    a.s_r = s_r;
+    a.S = S;
+    a.g = g;
+    a.flag = h->d_flag.ptr;
+    if (h->nitems > 0) {
+        const unsigned grid = (unsigned)(8 * ((h->nitems + 7) / 8));
+        switch (h->uniform_de) {
+        case 1: k_phase2<1><<<grid, P2_THREADS, 0, s>>>(a); break;
+        case 2: k_phase2<2><<<grid, P2_THREADS, 0, s>>>(a); break;
+        case 3: k_phase2<3><<<grid, P2_THREADS, 0, s>>>(a); break;
+        default: k_phase2<0><<<grid, P2_THREADS, 0, s>>>(a); break;
+        }
+    }
+    h->timing.mark(3, s);
+    k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+int reduce_sync(bls_schur *h, int mode, double *S, double *g, const double *b_r, const double *b_e,
+                const double *sc, double lambda, const double *E, const double *C, double *E_inv, double *z,
+                const double *s_r, const double *s_e, int64_t *failed_block, hipStream_t s) {
+    int rc = reduce_async(h, mode, S, g, b_r, b_e, sc, lambda, E, C, E_inv, z, s_r, s_e, h ? h->d_failed_out.ptr : nullptr, s);
+    if (rc) return rc;
+    int64_t fb = 0;
+    BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    BLS_HIP(hipStreamSynchronize(s));
+    if (h->timing.enabled)
+        for (int i = 1; i < 3; i++) (void)hipEventElapsedTime(&h->timing.ms[i], h->timing.ev[i], h->timing.ev[i + 1]);
+    if (failed_block) *failed_block = fb;
+    return fb ? BLS_FAILED : BLS_OK;
+}
+
+std::vector<int64_t> to0(const int64_t *p, int64_t n) {
+    std::vector<int64_t> v(p, p + n);
+    for (auto &x : v) x -= 1;
+    return v;
+}
+
+std::vector<int32_t> narrow32(const std::vector<int64_t> &v) { return std::vector<int32_t>(v.begin(), v.end()); }
+
+}  // namespace
+
+extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, int64_t ncoup, const int64_t *r_dims,
+                                const int64_t *e_dims, const int64_t *boffsets_r, const int64_t *boffsets_e,
+                                const int64_t *eoffsets, const int64_t *eliminated_ptr, const int64_t *entry_retained,
+                                const int64_t *coffsets, const int64_t *retained_ptr, const int64_t *retained_entry,
+                                bls_schur **out) {
+    if (!out || nr < 0 || nrblocks < 0 || neblocks < 0 || ncoup < 0) return fail(BLS_ERR_ARG, "bls_schur_create: bad arguments");
+    *out = nullptr;
+    if (ncoup > INT32_MAX || neblocks > INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "more than 2^31 couplings");
+    auto *h = new bls_schur();
+    h->nr = nr;
+    h->nrb = nrblocks;
+    h->neb = neblocks;
+    h->ncoup = ncoup;
+    h->r_dims.assign(r_dims, r_dims + nrblocks);
+    h->e_dims.assign(e_dims, e_dims + neblocks);
+    h->boffsets_r = to0(boffsets_r, nrblocks);
+    h->boffsets_e = to0(boffsets_e, neblocks);
+    h->eoffsets = to0(eoffsets, neblocks);
+    h->eliminated_ptr = to0(eliminated_ptr, neblocks + 1);
+    h->entry_retained = to0(entry_retained, ncoup);
+    h->coffsets = to0(coffsets, ncoup);
+    h->retained_ptr = to0(retained_ptr, nrblocks + 1);
+    h->retained_entry = to0(retained_entry, ncoup);
+    int rc = check_structure(h);
+    if (rc) { delete h; return rc; }
+    for (int64_t c = 0; c < ncoup; c++)
+        if (h->entry_retained[c] < 0 || h->entry_retained[c] >= nrblocks) { delete h; return fail(BLS_ERR_ARG, "entry_retained out of range"); }
+    // runs of equal d_e, with contiguous E / b offsets inside each
+    h->uniform_de = neblocks ? (int)h->e_dims[0] : 3;
+    for (int64_t e = 0; e < neblocks;) {
+        int64_t f = e;
+        while (e < neblocks && h->e_dims[e] == h->e_dims[f]) e++;
+        h->runs.push_back(Run{f, e - f, h->e_dims[f], h->eoffsets[f], h->boffsets_e[f]});
+        if (h->e_dims[f] != h->uniform_de) h->uniform_de = 0;
+    }
+    if (h->uniform_de > 3) h->uniform_de = 0;
+    if ((rc = h->d_eoffsets.upload(h->eoffsets)) || (rc = h->d_boffsets_e.upload(h->boffsets_e)) ||
+        (rc = h->d_coffsets.upload(h->coffsets)) || (rc = h->d_eliminated_ptr.upload(h->eliminated_ptr)) ||
+        (rc = h->d_boffsets_r.upload(h->boffsets_r)) || (rc = h->d_e_dims.upload(narrow32(h->e_dims))) ||
+        (rc = h->d_r_dims.upload(narrow32(h->r_dims))) || (rc = h->d_entry_retained.upload(narrow32(h->entry_retained))) ||
+        (rc = h->d_flag.alloc(1)) || (rc = h->d_failed_out.alloc(1))) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return BLS_OK;
+}
+
+extern "C" void bls_schur_destroy(bls_schur *h) { delete h; }
+
+extern "C" int bls_schur_set_sparse(bls_schur *h, int64_t nnz, const int64_t *S_colptr, int64_t nnz_blocks,
+                                    const int64_t *block_colptr, const int64_t *block_rowval,
+                                    const int64_t *block_row_offsets, const int64_t *col_heights, int64_t nR,
+                                    const int64_t *r_to_s_map, const int64_t *diag_inds) {
+    if (!h || nnz < 0 || nnz_blocks < 0 || nR < 0) return fail(BLS_ERR_ARG, "bls_schur_set_sparse: bad arguments");
+    std::vector<int64_t> bcp = to0(block_colptr, h->nrb + 1), brv = to0(block_rowval, nnz_blocks);
+    std::vector<int64_t> Scp(S_colptr, S_colptr + h->nr + 1);
+    for (int64_t r = 0; r < h->nrb; r++)
+        for (int64_t k = bcp[r]; k < bcp[r + 1]; k++)
+            if (brv[k] < 0 || brv[k] > r) return fail(BLS_ERR_ARG, "block pattern is not upper triangular");
+    int rc = build_phase2(
+        h, [&](int64_t r2, std::vector<int64_t> &col) { col.assign(brv.begin() + bcp[r2], brv.begin() + bcp[r2 + 1]); },
+        [&](int64_t r2, size_t k, int64_t) { return block_row_offsets[bcp[r2] + (int64_t)k]; },
+        [&](int64_t r2, int64_t &base, int64_t &cs) {
+            base = Scp[h->boffsets_r[r2]] - 1;
+            cs = col_heights[r2];
+        });
+    if (rc) return rc;
+    std::vector<int64_t> rts(r_to_s_map, r_to_s_map + nR), di(diag_inds, diag_inds + h->nr);
+    if ((rc = h->d_r_to_s.upload(rts)) || (rc = h->d_diag_inds.upload(di)) || (rc = h->d_S_colptr.upload(Scp))) return rc;
+    h->nnz = nnz;
+    h->nR = nR;
+    h->mode = 1;
+    return BLS_OK;
+}
+
+extern "C" int bls_schur_set_dense(bls_schur *h) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    int rc = build_phase2(
+        h, [&](int64_t r2, std::vector<int64_t> &col) { for (int64_t r = 0; r <= r2; r++) col.push_back(r); },
+        [&](int64_t, size_t, int64_t r1) { return h->boffsets_r[r1]; },
+        [&](int64_t r2, int64_t &base, int64_t &cs) {
+            base = h->boffsets_r[r2] * h->nr;
+            cs = h->nr;
+        });
+    if (rc) return rc;
+    h->mode = 2;
+    return BLS_OK;
+}
+
+extern "C" int bls_reset_schur_sparse_f64(bls_schur *h, double *nzval, const double *R_nzval, const double *scaling_r,
+                                          double lambda, const double *s_r, void *stream) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (h->mode != 1) return fail(BLS_ERR_STATE, "handle has no sparse layout (bls_schur_set_sparse)");
+    if (!(std::isfinite(lambda) && lambda >= 0))
+        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
+    if (s_r) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
+    if (!nzval || !scaling_r || (h->nR && !R_nzval)) return fail(BLS_ERR_ARG, "reset_schur_sparse: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    h->timing.mark(0, s);
+    k_fill_f64<<<std::min<int64_t>(grid_for(h->nnz, 256), 4096), 256, 0, s>>>(nzval, h->nnz, 0.0);
+    if (h->nR) k_scatter_r<<<std::min<int64_t>(grid_for(h->nR, 256), 4096), 256, 0, s>>>(nzval, R_nzval, h->d_r_to_s.ptr, h->nR);
+    if (h->nr) k_add_damping<<<grid_for(h->nr, 256), 256, 0, s>>>(nzval, h->d_diag_inds.ptr, h->nr, scaling_r, lambda, s_r);
+    h->timing.mark(1, s);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_reset_schur_dense_f64(bls_schur *h, double *S, const double *R, const double *scaling_r,
+                                         double lambda, const double *s_r, void *stream) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (!(std::isfinite(lambda) && lambda >= 0))
+        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
+    if (h->nr && (!S || !R || !scaling_r)) return fail(BLS_ERR_ARG, "reset_schur_dense: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    h->timing.mark(0, s);
+    if (h->nr) k_reset_dense<<<std::min<int64_t>(grid_for(h->nr * h->nr, 256), 4096), 256, 0, s>>>(S, R, h->nr, scaling_r, lambda, s_r);
+    h->timing.mark(1, s);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_reduce_schur_sparse_f64(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                           const double *b_e, const double *scaling_e, double lambda,
+                                           const double *E_data, const double *C_data, double *E_inv, double *z,
+                                           const double *s_r, const double *s_e, int64_t *failed_block, void *stream) {
+    return reduce_sync(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e, failed_block,
+                       (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_dense_f64(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
+                                          const double *scaling_e, double lambda, const double *E_data,
+                                          const double *C_data, double *E_inv, double *z, const double *s_r,
+                                          const double *s_e, int64_t *failed_block, void *stream) {
+    return reduce_sync(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e, failed_block,
+                       (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_sparse_f64_async(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                                 const double *b_e, const double *scaling_e, double lambda,
+                                                 const double *E_data, const double *C_data, double *E_inv, double *z,
+                                                 const double *s_r, const double *s_e, int64_t *d_failed_block,
+                                                 void *stream) {
+    return reduce_async(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                        d_failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const double *b_r,
+                                                const double *b_e, const double *scaling_e, double lambda,
+                                                const double *E_data, const double *C_data, double *E_inv, double *z,
+                                                const double *s_r, const double *s_e, int64_t *d_failed_block,
+                                                void *stream) {
+    return reduce_async(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e, d_failed_block,
+                        (hipStream_t)stream);
+}
+
+extern "C" int bls_backsubstitute_schur_f64(bls_schur *h, double *x_e, const double *x_r, const double *C_data,
+                                            const double *E_inv, const double *z, const double *s_r,
+                                            const double *s_e, void *stream) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (h->neb && (!x_e || !x_r || !C_data || !E_inv || !z)) return fail(BLS_ERR_ARG, "backsubstitute: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    if (h->neb)
+        k_backsub<<<grid_for(h->neb, 256), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr, h->d_boffsets_e.ptr,
+                                                          h->d_eliminated_ptr.ptr, h->d_entry_retained.ptr, h->d_coffsets.ptr,
+                                                          h->d_r_dims.ptr, h->d_boffsets_r.ptr, C_data, E_inv, z, x_r, s_r,
+                                                          s_e, x_e);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_copyto_csc_f64(int64_t n, double *dest, const double *src, const int64_t *map, void *stream) {
+    if (n < 0) return fail(BLS_ERR_ARG, "bls_copyto_csc_f64: negative length");
+    if (n == 0) return BLS_OK;
+    if (!dest || !src || !map) return fail(BLS_ERR_ARG, "bls_copyto_csc_f64: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    k_gather<<<std::min<int64_t>(grid_for(n, 256), 8192), 256, 0, s>>>(dest, src, map, n);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_map) {
+    if (n < 0 || !d_map) return fail(BLS_ERR_ARG, "bls_scatter_map_create: bad arguments");
+    *d_map = nullptr;
+    if (n == 0) return BLS_OK;
+    BLS_HIP(hipMalloc((void **)d_map, n * sizeof(int64_t)));
+    BLS_HIP(hipMemcpy(*d_map, scatter_map, n * sizeof(int64_t), hipMemcpyHostToDevice));
+    return BLS_OK;
+}
+
+extern "C" int bls_schur_set_timing(bls_schur *h, int enable) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (enable) return h->timing.enable();
+    h->timing.enabled = false;
+    return BLS_OK;
+}
+
+extern "C" int bls_schur_kernel_times(bls_schur *h, float *times, int n) {
+    if (!h || !times) return fail(BLS_ERR_ARG, "bad arguments");
+    if (h->timing.enabled) (void)hipEventElapsedTime(&h->timing.ms[0], h->timing.ev[0], h->timing.ev[1]);
+    for (int i = 0; i < n && i < 3; i++) times[i] = h->timing.ms[i];
+    return BLS_OK;
+}

@@ -1,0 +1,241 @@
+"""ctypes binding of libblsamd.so (the HIP kernels) with the reference's entry-point names.
+
+Julia's ``reduce_schur_sparse!`` is ``reduce_schur_sparse_`` here (a trailing underscore for the bang); arguments keep
+the reference's order and meaning (src/schur_reduction.jl:465-507, src/SparseBlockLinearSystem.jl:356-362). Value
+arrays are torch tensors on the GPU (float64), index arrays are the host mirror's 1-based Int64 numpy arrays.
+There is no CPU fallback: if the library is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libblsamd.so")
+P64 = ct.POINTER(ct.c_int64)
+VP = ct.c_void_p
+
+BLS_OK, BLS_FAILED = 0, 1
+
+
+class BLSError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# exported symbols and their signatures (restype, argtypes)
+SIGNATURES = {
+    "bls_last_error": (ct.c_char_p, []),
+    "bls_version": (ct.c_char_p, []),
+    "bls_alloc": (ct.c_int, [ct.POINTER(VP), ct.c_int64]),
+    "bls_free": (ct.c_int, [VP]),
+    "bls_upload": (ct.c_int, [VP, VP, ct.c_int64, VP]),
+    "bls_download": (ct.c_int, [VP, VP, ct.c_int64, VP]),
+    "bls_stream_synchronize": (ct.c_int, [VP]),
+    "bls_schur_create": (ct.c_int, [ct.c_int64] * 4 + [P64] * 10 + [ct.POINTER(VP)]),
+    "bls_schur_destroy": (None, [VP]),
+    "bls_schur_set_sparse": (ct.c_int, [VP, ct.c_int64, P64, ct.c_int64, P64, P64, P64, P64, ct.c_int64, P64, P64]),
+    "bls_schur_set_dense": (ct.c_int, [VP]),
+    "bls_reset_schur_sparse_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, VP, VP]),
+    "bls_reset_schur_dense_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, VP, VP]),
+    "bls_reduce_schur_sparse_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, P64, VP]),
+    "bls_reduce_schur_dense_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, P64, VP]),
+    "bls_reduce_schur_sparse_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_reduce_schur_dense_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_backsubstitute_schur_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_copyto_csc_f64": (ct.c_int, [ct.c_int64, VP, VP, VP, VP]),
+    "bls_scatter_map_create": (ct.c_int, [ct.c_int64, P64, ct.POINTER(VP)]),
+    "bls_schur_set_timing": (ct.c_int, [VP, ct.c_int]),
+    "bls_schur_kernel_times": (ct.c_int, [VP, ct.POINTER(ct.c_float), ct.c_int]),
+}
+
+
+def lib():
+    """Load libblsamd.so (built by __graft_entry__.build() / csrc/Makefile). Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise BLSError(f"{LIBPATH} is missing: build it with `make -C blocklinearsystems.jl_amd/csrc`")
+        L = ct.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise BLSError(lib().bls_last_error().decode())
+    return rc
+
+
+def _p64(a, keep):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    keep.append(a)
+    return a.ctypes.data_as(P64)
+
+
+def _dp(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+        raise BLSError("expected a contiguous GPU tensor")
+    return ct.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ct.c_void_p(stream)
+
+
+class SchurHandle:
+    """A registered SchurStructure (+ its workspace index and reduced-system layout) on the device:
+    the C ABI's bls_schur. ``pattern`` (build_schur_pattern's result) selects the sparse S; ``dense=True`` the dense
+    nr x nr S. ``r_to_s`` is build_r_to_s_map's result (sparse only)."""
+
+    def __init__(self, st, ws_index, pattern=None, r_to_s=None, dense=False):
+        keep = []
+        h = VP()
+        _check(lib().bls_schur_create(st.nr, st.nrblocks, st.neblocks, st.ncoup, _p64(st.r_dims, keep),
+                                      _p64(st.e_dims, keep), _p64(st.boffsets_r, keep), _p64(st.boffsets_e, keep),
+                                      _p64(st.eoffsets, keep), _p64(st.eliminated_ptr, keep),
+                                      _p64(st.entry_retained, keep), _p64(st.coffsets, keep),
+                                      _p64(ws_index.retained_ptr, keep), _p64(ws_index.retained_entry, keep), ct.byref(h)))
+        self.h = h
+        self.st = st
+        self.dense = dense
+        try:
+            if dense:
+                _check(lib().bls_schur_set_dense(h))
+            else:
+                if pattern is None:
+                    raise BLSError("a sparse handle needs the pattern of build_schur_pattern")
+                rts = np.zeros(0, np.int64) if r_to_s is None else r_to_s
+                _check(lib().bls_schur_set_sparse(h, pattern.nnz, _p64(pattern.S_colptr, keep), len(pattern.block_rowval),
+                                                  _p64(pattern.block_colptr, keep), _p64(pattern.block_rowval, keep),
+                                                  _p64(pattern.block_row_offsets, keep), _p64(pattern.col_heights, keep),
+                                                  len(rts), _p64(rts, keep), _p64(pattern.diag_inds, keep)))
+                self.nnz = pattern.nnz
+        except Exception:
+            lib().bls_schur_destroy(h)
+            self.h = None
+            raise
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_schur_destroy(self.h)
+            self.h = None
+
+    def set_timing(self, on=True):
+        _check(lib().bls_schur_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        arr = (ct.c_float * 3)()
+        _check(lib().bls_schur_kernel_times(self.h, arr, 3))
+        return list(arr)
+
+
+class SchurReductionWorkspace:
+    """Device counterpart of SchurReductionWorkspace{Float64} (src/schur_reduction.jl:16-27): E_inv, z and the
+    failure flag of the last reduction."""
+
+    def __init__(self, handle):
+        import torch
+        st = handle.st
+        self.E_inv_data = torch.empty(st.eliminated_storage_length(), dtype=torch.float64, device="cuda")
+        self.z_data = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+        self.d_failed = torch.zeros(1, dtype=torch.int64, device="cuda")
+        self.failed_block = 0
+
+
+def reset_schur_sparse_(nzval, R_nzval, handle, scaling_r, lam, s_r=None, stream=None, check=True):
+    """reset_schur_sparse! (src/schur_reduction.jl:416-436). check=False skips the host-side check_damping of
+    scaling_r (a device reduction and a synchronization) for a caller that validated it already."""
+    if check:
+        _check_damping(lam, scaling_r, "scaling_r")
+    _check(lib().bls_reset_schur_sparse_f64(handle.h, _dp(nzval), _dp(R_nzval), _dp(scaling_r), float(lam), _dp(s_r),
+                                            _stream(stream)))
+
+
+def reset_schur_dense_(S, R, handle, scaling_r, lam, s_r=None, stream=None, check=True):
+    """reset_schur_dense! (src/schur_reduction.jl:389-407); S, R column-major flat (nr*nr)."""
+    if check:
+        _check_damping(lam, scaling_r, "scaling_r")
+    _check(lib().bls_reset_schur_dense_f64(handle.h, _dp(S), _dp(R), _dp(scaling_r), float(lam), _dp(s_r), _stream(stream)))
+
+
+def _check_damping(lam, scaling, name):
+    """check_damping (src/utils.jl:353-360), on the host before any work."""
+    import math
+    if not (math.isfinite(lam) and lam >= 0):
+        raise ValueError(f"lambda must be finite and non-negative, got {lam}")
+    if scaling is not None:
+        import torch
+        bad = ~(torch.isfinite(scaling) & (scaling >= 0))
+        if bool(bad.any()):
+            i = int(torch.nonzero(bad)[0])
+            raise ValueError(f"{name} must be finite and non-negative, but entry {i + 1} is {float(scaling[i])}")
+
+
+def reduce_schur_sparse_(nzval, g_S, b_r, b_e, scaling_e, lam, handle, E_data, C_data, ws, s_r=None, s_e=None,
+                         stream=None, sync=True, check=True):
+    """reduce_schur_sparse! (src/schur_reduction.jl:489-507). Returns True, or False with ws.failed_block set.
+    With sync=False the status is left on the device (ws.d_failed) and None is returned."""
+    if check:
+        _check_damping(lam, scaling_e, "scaling_e")
+    return _reduce(handle, False, nzval, g_S, b_r, b_e, scaling_e, lam, E_data, C_data, ws, s_r, s_e, stream, sync)
+
+
+def reduce_schur_dense_(S, g_S, b_r, b_e, scaling_e, lam, handle, E_data, C_data, ws, s_r=None, s_e=None,
+                        stream=None, sync=True, check=True):
+    """reduce_schur_dense! (src/schur_reduction.jl:465-479); S column-major flat (nr*nr), upper triangle written."""
+    if check:
+        _check_damping(lam, scaling_e, "scaling_e")
+    return _reduce(handle, True, S, g_S, b_r, b_e, scaling_e, lam, E_data, C_data, ws, s_r, s_e, stream, sync)
+
+
+def _reduce(handle, dense, S, g, b_r, b_e, sc, lam, E, C, ws, s_r, s_e, stream, sync):
+    L = lib()
+    if sync:
+        fb = ct.c_int64(0)
+        f = L.bls_reduce_schur_dense_f64 if dense else L.bls_reduce_schur_sparse_f64
+        rc = _check(f(handle.h, _dp(S), _dp(g), _dp(b_r), _dp(b_e), _dp(sc), float(lam), _dp(E), _dp(C),
+                      _dp(ws.E_inv_data), _dp(ws.z_data), _dp(s_r), _dp(s_e), ct.byref(fb), _stream(stream)))
+        ws.failed_block = int(fb.value)
+        return rc == BLS_OK
+    f = L.bls_reduce_schur_dense_f64_async if dense else L.bls_reduce_schur_sparse_f64_async
+    _check(f(handle.h, _dp(S), _dp(g), _dp(b_r), _dp(b_e), _dp(sc), float(lam), _dp(E), _dp(C), _dp(ws.E_inv_data),
+             _dp(ws.z_data), _dp(s_r), _dp(s_e), _dp(ws.d_failed), _stream(stream)))
+    return None
+
+
+def backsubstitute_(x_e, x_r, handle, C_data, ws, s_r=None, s_e=None, stream=None):
+    """backsubstitute! (src/schur_reduction.jl:519-576)."""
+    _check(lib().bls_backsubstitute_schur_f64(handle.h, _dp(x_e), _dp(x_r), _dp(C_data), _dp(ws.E_inv_data),
+                                              _dp(ws.z_data), _dp(s_r), _dp(s_e), _stream(stream)))
+
+
+def copyto_(dest, src, scatter_map, stream=None):
+    """copyto!(dest, src, scatter_map) (src/SparseBlockLinearSystem.jl:356-362): dest[i] = src[scatter_map[i]].
+    ``scatter_map``: a GPU int64 tensor of 1-based indices (see scatter_map_to_device)."""
+    import torch
+    if dest.numel() != scatter_map.numel():
+        raise ValueError(f"Destination nzval length ({dest.numel()}) does not match scatter map length ({scatter_map.numel()})")
+    if scatter_map.dtype != torch.int64:
+        raise BLSError("scatter map must be int64")
+    _check(lib().bls_copyto_csc_f64(dest.numel(), _dp(dest), _dp(src), _dp(scatter_map), _stream(stream)))
+    return dest
+
+
+def scatter_map_to_device(scatter_map):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(scatter_map, dtype=np.int64)).cuda()

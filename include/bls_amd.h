@@ -1,0 +1,127 @@
+/*
+ * bls_amd.h -- C ABI of the MI355X elimination library (libblsamd.so).
+ *
+ * Each entry point replaces the reference function cited beside it (BlockLinearSystems.jl, file:line) and takes
+ * the reference's own arrays verbatim: index arrays are Int64 holding the 1-based values the Julia structures
+ * store, blocks are column-major, C_er is stored d_e x d_r (src/schur_assembly.jl:236-246).
+ *
+ * Memory: integer maps are copied to the device once, when a handle is created (the structure is immutable).
+ * Value arrays (E_data, C_data, R, b, S, g_S, tables, ...) are DEVICE pointers; callers with host arrays stage
+ * them with bls_alloc / bls_upload / bls_download. All work of a call is enqueued on `stream` (a hipStream_t;
+ * NULL = the null stream). A handle may be used from one host thread at a time; distinct handles may be used
+ * concurrently on distinct streams.
+ *
+ * Return codes: BLS_OK (0) success; BLS_FAILED (1) numerical failure, the reference's `false`, with the lowest
+ * failing eliminated block (1-based) in *failed_block; negative values are errors, described by bls_last_error().
+ * Argument checks the reference makes before any work (check_damping, src/utils.jl:353-360; reduce_qr!'s
+ * structure and length checks, src/qr_reduction.jl:166-169) are made here on scalars and sizes; checks over
+ * device arrays (finite, non-negative scaling vectors) are the caller's, made on its host copies before the call.
+ */
+#ifndef BLS_AMD_H
+#define BLS_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLS_OK 0
+#define BLS_FAILED 1
+#define BLS_ERR_ARG (-1)
+#define BLS_ERR_HIP (-2)
+#define BLS_ERR_UNSUPPORTED (-3)
+#define BLS_ERR_STATE (-4)
+
+typedef struct bls_schur bls_schur;       /* a registered SchurStructure + its reduced-system layout */
+typedef struct bls_qr bls_qr;             /* a registered SchurJacobianStructure */
+
+/* Message of the last error on this host thread ("" if none). */
+const char *bls_last_error(void);
+/* Library version string. */
+const char *bls_version(void);
+
+/* ---- device memory helpers (for callers holding host arrays) ---------------------------------------- */
+int bls_alloc(void **ptr, int64_t bytes);
+int bls_free(void *ptr);
+int bls_upload(void *dst_device, const void *src_host, int64_t bytes, void *stream);
+int bls_download(void *dst_host, const void *src_device, int64_t bytes, void *stream);
+int bls_stream_synchronize(void *stream);
+
+/* ---- Schur path ------------------------------------------------------------------------------------- */
+
+/* Register a SchurStructure (src/schur_structure.jl:50-68) with the retained-block-major index of its
+ * SchurReductionWorkspace (src/schur_reduction.jl:51-92). r_dims / e_dims: the block sizes (collect of the
+ * BlockRLEs); boffsets_r, boffsets_e, eoffsets, eliminated_ptr, entry_retained, coffsets: the structure's fields;
+ * retained_ptr, retained_entry: the workspace's. Eliminated blocks must have d_e <= 8 (the reference's static
+ * path, src/kernels.jl:88) and retained blocks d_r <= 16; other sizes return BLS_ERR_UNSUPPORTED. */
+int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, int64_t ncoup, const int64_t *r_dims,
+                     const int64_t *e_dims, const int64_t *boffsets_r, const int64_t *boffsets_e,
+                     const int64_t *eoffsets, const int64_t *eliminated_ptr, const int64_t *entry_retained,
+                     const int64_t *coffsets, const int64_t *retained_ptr, const int64_t *retained_entry,
+                     bls_schur **out);
+void bls_schur_destroy(bls_schur *h);
+
+/* Attach the sparse reduced-system layout of build_schur_pattern (src/schur_structure.jl:241-323): S.colptr
+ * (nr + 1), the block pattern (block_colptr nrblocks + 1, block_rowval nnz_blocks, block_row_offsets
+ * nnz_blocks, col_heights nrblocks), and for reset_schur_sparse! the r_to_s_map of build_r_to_s_map
+ * (src/schur_structure.jl:406-431, length nR) and diag_inds (nr). */
+int bls_schur_set_sparse(bls_schur *h, int64_t nnz, const int64_t *S_colptr, int64_t nnz_blocks,
+                         const int64_t *block_colptr, const int64_t *block_rowval, const int64_t *block_row_offsets,
+                         const int64_t *col_heights, int64_t nR, const int64_t *r_to_s_map, const int64_t *diag_inds);
+/* Use a dense nr x nr column-major S (reduce_schur_dense!). */
+int bls_schur_set_dense(bls_schur *h);
+
+/* reset_schur_sparse! (src/schur_reduction.jl:416-436): nzval = 0; nzval[r_to_s_map] = R_nzval; scaled by s_r
+ * (NULL = UnitScale); nzval[diag_inds] += lambda * scaling_r * s_r^2. */
+int bls_reset_schur_sparse_f64(bls_schur *h, double *nzval, const double *R_nzval, const double *scaling_r,
+                               double lambda, const double *s_r, void *stream);
+/* reset_schur_dense! (src/schur_reduction.jl:389-407). */
+int bls_reset_schur_dense_f64(bls_schur *h, double *S, const double *R, const double *scaling_r, double lambda,
+                              const double *s_r, void *stream);
+
+/* reduce_schur_sparse! (src/schur_reduction.jl:489-507) / reduce_schur_dense! (:465-479): phase 1 caches
+ * E_inv (sum d_e^2) and z (sum d_e) -- the workspace's E_inv_data and z_data --, then g_S = b_r .* s_r and the
+ * phase-2 updates of S (upper triangle) and g_S, in the reference's order (per S entry, ascending eliminated
+ * block), bitwise reproducible. Synchronizes `stream` to return the status. */
+int bls_reduce_schur_sparse_f64(bls_schur *h, double *nzval, double *g_S, const double *b_r, const double *b_e,
+                                const double *scaling_e, double lambda, const double *E_data, const double *C_data,
+                                double *E_inv, double *z, const double *s_r, const double *s_e,
+                                int64_t *failed_block, void *stream);
+int bls_reduce_schur_dense_f64(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
+                               const double *scaling_e, double lambda, const double *E_data, const double *C_data,
+                               double *E_inv, double *z, const double *s_r, const double *s_e,
+                               int64_t *failed_block, void *stream);
+/* The same without the synchronization: the lowest failing block (0 = none) is written to the device int64
+ * *d_failed_block when the stream reaches it. For pipelines and graph capture. */
+int bls_reduce_schur_sparse_f64_async(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                      const double *b_e, const double *scaling_e, double lambda,
+                                      const double *E_data, const double *C_data, double *E_inv, double *z,
+                                      const double *s_r, const double *s_e, int64_t *d_failed_block, void *stream);
+int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
+                                     const double *scaling_e, double lambda, const double *E_data,
+                                     const double *C_data, double *E_inv, double *z, const double *s_r,
+                                     const double *s_e, int64_t *d_failed_block, void *stream);
+
+/* backsubstitute! (src/schur_reduction.jl:519-576): x_e = z_e - E_inv sum_r C_er x_r, from the E_inv and z of
+ * the last reduction. */
+int bls_backsubstitute_schur_f64(bls_schur *h, double *x_e, const double *x_r, const double *C_data,
+                                 const double *E_inv, const double *z, const double *s_r, const double *s_e,
+                                 void *stream);
+
+/* copyto!(dest::SparseMatrixCSC, src, scatter_map) (src/SparseBlockLinearSystem.jl:356-362): the gather
+ * dest[i] = src[scatter_map[i]] (1-based map, host array, uploaded at each call unless a device copy is given
+ * by bls_scatter_map_create). */
+int bls_copyto_csc_f64(int64_t n, double *dest, const double *src, const int64_t *scatter_map_device_i64,
+                       void *stream);
+/* A device copy of a scatter map (1-based Int64, host) for repeated copyto!: returns a device pointer. */
+int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_map);
+
+/* Optional per-kernel timing of the last call on a handle (HIP events on the launch stream), in ms:
+ * times[0] reset, [1] phase 1, [2] phase 2. Enabled by bls_schur_set_timing(h, 1). */
+int bls_schur_set_timing(bls_schur *h, int enable);
+int bls_schur_kernel_times(bls_schur *h, float *times, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+"""Shared test helpers: block-wise relative errors and problem builders (the reference's fixtures restated)."""
+import numpy as np
+
+
+def dense_upper_blocks(st, S_flat):
+    """Upper blocks (r1 <= r2) of a dense column-major flat S as a dict (r1, r2) -> block (rows r1, cols r2)."""
+    nr = st.nr
+    S = S_flat.reshape(nr, nr).T
+    out = {}
+    for r2 in range(st.nrblocks):
+        c0, d2 = st.boffsets_r[r2] - 1, st.r_dims[r2]
+        for r1 in range(r2 + 1):
+            o1, d1 = st.boffsets_r[r1] - 1, st.r_dims[r1]
+            blk = S[o1:o1 + d1, c0:c0 + d2].copy()
+            if r1 == r2:
+                blk = np.triu(blk)
+            out[(r1, r2)] = blk
+    return out
+
+
+def sparse_upper_blocks(st, pattern, nzval):
+    out = {}
+    for r2 in range(st.nrblocks):
+        d2 = st.r_dims[r2]
+        base = pattern.S_colptr[st.boffsets_r[r2] - 1] - 1
+        h = pattern.col_heights[r2]
+        for k in range(pattern.block_colptr[r2] - 1, pattern.block_colptr[r2 + 1] - 1):
+            r1 = pattern.block_rowval[k] - 1
+            d1 = st.r_dims[r1]
+            ro = pattern.block_row_offsets[k]
+            blk = np.empty((d1, d2))
+            for c in range(d2):
+                blk[:, c] = nzval[base + c * h + ro: base + c * h + ro + d1]
+            if r1 == r2:
+                blk = np.triu(blk)
+            out[(r1, r2)] = blk
+    return out
+
+
+def max_block_rel(A, B):
+    """max over blocks of ||A_b - B_b||_F / ||B_b||_F (absolute where B_b = 0)."""
+    worst = 0.0
+    for k, b in B.items():
+        nb = np.linalg.norm(b)
+        d = np.linalg.norm(A[k] - b)
+        worst = max(worst, d / nb if nb > 0 else d)
+    return worst
+
+
+def schur_test_problem(rng, r_dims, e_dims, couplings, rr_pairs=()):
+    """SchurTestProblem (test/fixtures/SchurFixtures.jl:26-54): a random strictly diagonally dominant bipartite H,
+    retained first, and b. Returns (H, b)."""
+    r_dims, e_dims = list(r_dims), list(e_dims)
+    nr, ne = sum(r_dims), sum(e_dims)
+    n = nr + ne
+    roff = np.cumsum([0] + r_dims[:-1])
+    eoff = nr + np.cumsum([0] + e_dims[:-1])
+    H = np.zeros((n, n))
+
+    def fill(r0, nrow, c0, ncol):
+        H[r0:r0 + nrow, c0:c0 + ncol] = 2 * rng.random((nrow, ncol)) - 1
+
+    for r, d in enumerate(r_dims):
+        fill(roff[r], d, roff[r], d)
+    for e, d in enumerate(e_dims):
+        fill(eoff[e], d, eoff[e], d)
+    for e, cpl in enumerate(couplings):
+        for r in cpl:
+            fill(roff[r - 1], r_dims[r - 1], eoff[e], e_dims[e])
+    for (r1, r2) in rr_pairs:
+        fill(roff[r1 - 1], r_dims[r1 - 1], roff[r2 - 1], r_dims[r2 - 1])
+    H = np.triu(H) + np.triu(H, 1).T
+    for i in range(n):
+        H[i, i] = np.abs(H[i]).sum() - abs(H[i, i]) + 0.5 + rng.random()
+    return H, rng.standard_normal(n)
+
+
+class Problem:
+    pass
+
+
+def problem_from_dense(blsamd, H, b, r_dims, e_dims, couplings, lam, dense, nchunks=1, rr_pairs=(), scaling=None):
+    """Lay a dense H out as a SchurBlockHessian (fill_ce!, test/fixtures/SchurFixtures.jl:59-79) and build the
+    pieces the kernels take: structure, workspace index, pattern, R storage, r_to_s, E_data, C_data, b_r, b_e."""
+    S_ = blsamd.structure
+    st = S_.SchurStructure(r_dims, e_dims, couplings, nchunks=nchunks, retained_pairs=[tuple(p) for p in rr_pairs])
+    nr = st.nr
+    E_data = np.zeros(st.eliminated_storage_length())
+    C_data = np.zeros(st.coupling_storage_length())
+    for e in range(st.neblocks):
+        de = st.e_dims[e]
+        eo = st.eoffsets[e] - 1
+        eo_h = nr + st.boffsets_e[e] - 1
+        E_data[eo:eo + de * de] = H[eo_h:eo_h + de, eo_h:eo_h + de].T.reshape(-1)
+        for c in range(st.eliminated_ptr[e] - 1, st.eliminated_ptr[e + 1] - 1):
+            r = st.entry_retained[c] - 1
+            dr = st.r_dims[r]
+            ro = st.boffsets_r[r] - 1
+            co = st.coffsets[c] - 1
+            C_data[co:co + dr * de] = H[ro:ro + dr, eo_h:eo_h + de].reshape(-1)   # (d_e x d_r) column-major
+    p = Problem()
+    p.structure, p.lam, p.dense = st, lam, dense
+    p.E_data, p.C_data = E_data, C_data
+    p.b_r, p.b_e = b[:nr].copy(), b[nr:].copy()
+    p.scaling_r = np.ones(nr) if scaling is None else scaling[:nr].copy()
+    p.scaling_e = np.ones(st.ne) if scaling is None else scaling[nr:].copy()
+    p.ws_index = S_.SchurReductionWorkspaceIndex(st, nchunks)
+    p.H, p.b = H, b
+    Rfull = H[:nr, :nr]
+    if dense:
+        p.R = np.triu(Rfull) + np.tril(np.zeros_like(Rfull))
+        # the dense retained part stores the upper triangle of off-diagonal blocks and full diagonal blocks
+        for r in range(st.nrblocks):
+            o, d = st.boffsets_r[r] - 1, st.r_dims[r]
+            p.R[o:o + d, o:o + d] = Rfull[o:o + d, o:o + d]
+        p.pattern = None
+    else:
+        pat = S_.build_schur_pattern(st)
+        Rs = S_.hessian_retained_storage(st, pat)
+        Rdata = np.zeros(Rs.nnzvals)
+        for brow, bcol, s0 in zip(Rs.brow, Rs.bcol, Rs.start):
+            ro, rd_ = st.boffsets_r[brow - 1] - 1, st.r_dims[brow - 1]
+            co, cd = st.boffsets_r[bcol - 1] - 1, st.r_dims[bcol - 1]
+            Rdata[s0 - 1:s0 - 1 + rd_ * cd] = Rfull[ro:ro + rd_, co:co + cd].T.reshape(-1)
+        p.pattern, p.Rsys, p.R_data = pat, Rs, Rdata
+        p.r_to_s = S_.build_r_to_s_map(Rs, pat, st.boffsets_r)
+    return p
+
+
+def standard_problem_dims():
+    """standard_problem (test/fixtures/SchurFixtures.jl:119-130)-like: retained 4 x 6, 20 eliminated of dims 3/2/1."""
+    return [6, 6, 6, 6], [3] * 10 + [2] * 6 + [1] * 4
+
+
+def reference_schur_problem():
+    """The hand-built structure of test/schur.jl:12-66: r_dims [6,3,6,9], e_dims 8 x 3 + 2 x 1, couplings."""
+    couplings = [[1, 2], [1, 3], [2, 3, 4], [1, 4], [1, 2, 3], [2, 4], [3, 4], [1, 2, 3, 4], [1, 3], [2, 4]]
+    return [6, 3, 6, 9], [3] * 8 + [1, 1], couplings

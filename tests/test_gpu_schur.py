@@ -1,0 +1,212 @@
+"""GPU parity of the Schur path: the HIP kernels through the C ABI against the CPU oracle on the same inputs.
+
+Tolerance (floating point, FP64): 1e-12 relative Frobenius per block of S (upper blocks), 1e-12 relative on
+g_S / E_inv / z / x_e. Integer outputs (failed_block, status) exact; repeated runs bitwise identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import (dense_upper_blocks, max_block_rel, problem_from_dense, reference_schur_problem,
+                     schur_test_problem, sparse_upper_blocks, standard_problem_dims)
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def run_gpu(blsamd, p, lam=None, repeat=1):
+    """reset + reduce on the device; returns host copies of S, g, E_inv, z, status, failed_block."""
+    torch = _torch()
+    dev = blsamd.device
+    st = p.structure
+    lam = p.lam if lam is None else lam
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=getattr(p, "r_to_s", None), dense=p.dense)
+    ws = dev.SchurReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    E, C, b_r, b_e = T(p.E_data), T(p.C_data), T(p.b_r), T(p.b_e)
+    sr, se = T(p.scaling_r), T(p.scaling_e)
+    outs = []
+    for _ in range(repeat):
+        g = torch.full((st.nr,), float("nan"), dtype=torch.float64, device="cuda")
+        if p.dense:
+            S = torch.empty(st.nr * st.nr, dtype=torch.float64, device="cuda")
+            dev.reset_schur_dense_(S, T(p.R.T.reshape(-1)), h, sr, lam)
+            ok = dev.reduce_schur_dense_(S, g, b_r, b_e, se, lam, h, E, C, ws)
+        else:
+            S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
+            dev.reset_schur_sparse_(S, T(p.R_data), h, sr, lam)
+            ok = dev.reduce_schur_sparse_(S, g, b_r, b_e, se, lam, h, E, C, ws)
+        torch.cuda.synchronize()
+        outs.append(dict(S=S.cpu().numpy(), g=g.cpu().numpy(), E_inv=ws.E_inv_data.cpu().numpy(),
+                         z=ws.z_data.cpu().numpy(), ok=ok, failed_block=ws.failed_block, h=h, ws=ws, C=C))
+    return outs if repeat > 1 else outs[0]
+
+
+def oracle_run(p, lam=None):
+    return oracle.schur_reduce_problem(p, lam=lam)
+
+
+def compare(p, gpu, orc):
+    st = p.structure
+    assert gpu["ok"] == orc["ok"]
+    if p.dense:
+        A, B = dense_upper_blocks(st, gpu["S"]), dense_upper_blocks(st, orc["S"])
+    else:
+        A, B = sparse_upper_blocks(st, p.pattern, gpu["S"]), sparse_upper_blocks(st, p.pattern, orc["S"])
+    assert max_block_rel(A, B) <= TOL
+    for k in ("g", "E_inv", "z"):
+        ref = orc[k]
+        assert np.abs(gpu[k] - ref).max() <= TOL * max(1.0, np.abs(ref).max()), k
+
+
+def test_config1_dense(blsamd):
+    p = blsamd.generators.config1()
+    compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
+def test_config2_small_sparse(blsamd):
+    p = blsamd.generators.config2(scale=0.01)
+    compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
+def test_config2_small_dense(blsamd):
+    p = blsamd.generators.config2(scale=0.002, dense=True)
+    compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
+@pytest.mark.parametrize("dense", [True, False])
+def test_reference_structure_mixed_dims(blsamd, dense):
+    """The hand-built structure of test/schur.jl:12-66 (retained dims 6/3/6/9, eliminated 3 and 1)."""
+    rng = np.random.default_rng(11)
+    r_dims, e_dims, cpl = reference_schur_problem()
+    H, b = schur_test_problem(rng, r_dims, e_dims, cpl, rr_pairs=[(1, 2), (2, 3), (3, 4)])
+    p = problem_from_dense(blsamd, H, b, r_dims, e_dims, cpl, 0.3, dense, rr_pairs=[(1, 2), (2, 3), (3, 4)])
+    compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
+@pytest.mark.parametrize("de", [1, 2, 3, 4, 5, 7, 8])
+def test_eliminated_dims(blsamd, de):
+    """test/schur_problems.jl:13-41's sweep over eliminated block sizes (device path: d_e <= 8)."""
+    rng = np.random.default_rng(100 + de)
+    nrb, neb = 5, 12
+    cpl = [sorted(rng.choice(np.arange(1, nrb + 1), size=rng.integers(1, 4), replace=False).tolist()) for _ in range(neb)]
+    H, b = schur_test_problem(rng, [6] * nrb, [de] * neb, cpl)
+    for dense in (True, False):
+        p = problem_from_dense(blsamd, H, b, [6] * nrb, [de] * neb, cpl, 0.0, dense)
+        compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
+def test_standard_problem_mixed_eliminated(blsamd):
+    rng = np.random.default_rng(5)
+    r_dims, e_dims = standard_problem_dims()
+    cpl = [sorted(rng.choice(np.arange(1, 5), size=rng.integers(1, 5), replace=False).tolist()) for _ in e_dims]
+    H, b = schur_test_problem(rng, r_dims, e_dims, cpl)
+    for dense in (True, False):
+        p = problem_from_dense(blsamd, H, b, r_dims, e_dims, cpl, 1e-3, dense)
+        compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
+def test_bitwise_reproducible(blsamd):
+    p = blsamd.generators.config2(scale=0.02)
+    a, b = run_gpu(blsamd, p, repeat=2)
+    assert np.array_equal(a["S"], b["S"]) and np.array_equal(a["g"], b["g"])
+
+
+def test_singular_block_reports_lowest(blsamd):
+    """Rank-1 E at lambda = 0 fails with the lowest failing block (test/schur.jl:405-425)."""
+    p = blsamd.generators.config2(scale=0.002)
+    st = p.structure
+    E = p.E_data.copy()
+    bad = [7, 3, 12]   # 1-based eliminated blocks made rank 1
+    for e in bad:
+        v = np.arange(1.0, 4.0)
+        eo = st.eoffsets[e - 1] - 1
+        E[eo:eo + 9] = np.outer(v, v).T.reshape(-1)
+    p.E_data = E
+    gpu = run_gpu(blsamd, p, lam=0.0)
+    orc = oracle_run(p, lam=0.0)
+    assert orc["ok"] is False and orc["failed_block"] == 3
+    assert gpu["ok"] is False and gpu["failed_block"] == 3
+    assert np.isnan(gpu["g"]).all()   # g_S is not written when phase 1 fails
+
+
+def test_backsubstitute(blsamd):
+    torch = _torch()
+    p = blsamd.generators.config2(scale=0.005)
+    st = p.structure
+    gpu = run_gpu(blsamd, p)
+    orc = oracle_run(p)
+    x_r = np.random.default_rng(0).standard_normal(st.nr)
+    xe_o = np.zeros(st.ne)
+    oracle.backsubstitute_schur(orc["args"], xe_o, x_r, p.C_data, orc["E_inv"], orc["z"])
+    xe_g = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    blsamd.device.backsubstitute_(xe_g, torch.as_tensor(x_r).cuda(), gpu["h"], gpu["C"], gpu["ws"])
+    xe_g = xe_g.cpu().numpy()
+    assert np.abs(xe_g - xe_o).max() <= TOL * np.abs(xe_o).max()
+
+
+def test_full_solve_matches_dense_cholesky(blsamd):
+    """reset -> reduce -> dense reduced solve -> backsubstitute equals the dense solve of the damped H
+    (reference_solve, test/fixtures/SchurFixtures.jl:114-117)."""
+    torch = _torch()
+    rng = np.random.default_rng(3)
+    r_dims, e_dims, cpl = reference_schur_problem()
+    H, b = schur_test_problem(rng, r_dims, e_dims, cpl)
+    lam = 0.3
+    p = problem_from_dense(blsamd, H, b, r_dims, e_dims, cpl, lam, True)
+    gpu = run_gpu(blsamd, p)
+    st = p.structure
+    S = gpu["S"].reshape(st.nr, st.nr).T
+    S = np.triu(S) + np.triu(S, 1).T
+    x_r = np.linalg.solve(S, gpu["g"])
+    xe = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    blsamd.device.backsubstitute_(xe, torch.as_tensor(x_r).cuda(), gpu["h"], gpu["C"], gpu["ws"])
+    x = np.concatenate([x_r, xe.cpu().numpy()])
+    x_ref = np.linalg.solve(H + lam * np.eye(len(b)), b)
+    assert np.linalg.norm(x - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+
+
+def test_copyto_symmetric_csc(blsamd):
+    """copyto!(csc, S, scatter_map) of the reduced system into its full symmetric CSC equals the oracle's S
+    symmetrised, bit for bit (a gather)."""
+    torch = _torch()
+    import scipy.sparse as sps
+    p = blsamd.generators.config2(scale=0.005)
+    st = p.structure
+    gpu = run_gpu(blsamd, p)
+    colptr, rowval, smap = blsamd.structure.reduced_symmetric_scatter(st, p.pattern)
+    dmap = blsamd.device.scatter_map_to_device(smap)
+    dest = torch.empty(len(smap), dtype=torch.float64, device="cuda")
+    blsamd.device.copyto_(dest, torch.as_tensor(gpu["S"]).cuda(), dmap)
+    vals = dest.cpu().numpy()
+    assert np.array_equal(vals, gpu["S"][smap - 1])
+    A = sps.csc_matrix((vals, rowval - 1, colptr - 1), shape=(st.nr, st.nr)).toarray()
+    assert np.array_equal(A, A.T)
+    orc = oracle_run(p)
+    B = sps.csc_matrix((orc["S"][smap - 1], rowval - 1, colptr - 1), shape=(st.nr, st.nr)).toarray()
+    assert np.abs(A - B).max() <= TOL * np.abs(B).max()
+
+
+def test_null_required_arrays_rejected(blsamd):
+    """A NULL scaling vector (or any required array) is an argument error raised on the host, before any launch."""
+    torch = _torch()
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.001)
+    st = p.structure
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=p.r_to_s)
+    ws = dev.SchurReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    S = torch.zeros(p.pattern.nnz, dtype=torch.float64, device="cuda")
+    g = torch.zeros(st.nr, dtype=torch.float64, device="cuda")
+    with pytest.raises(dev.BLSError):
+        dev.reset_schur_sparse_(S, T(p.R_data), h, None, p.lam, check=False)
+    with pytest.raises(dev.BLSError):
+        dev.reduce_schur_sparse_(S, g, T(p.b_r), T(p.b_e), None, p.lam, h, T(p.E_data), T(p.C_data), ws, check=False)
+    with pytest.raises(ValueError):
+        dev.reduce_schur_sparse_(S, g, T(p.b_r), T(p.b_e), T(p.scaling_e), -1.0, h, T(p.E_data), T(p.C_data), ws)
