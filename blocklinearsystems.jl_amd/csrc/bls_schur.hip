@@ -34,7 +34,7 @@ struct P2Item {
     int32_t r2, dr2, ro2;
     int32_t slot_lo, nslots, ntasks;
     int32_t has_diag, uniform_dr;
-    int32_t batch_lo, batch_hi;
+    int64_t entry_lo, entry_hi;
 };
 
 struct P2Entry {
@@ -46,7 +46,6 @@ struct P2Args {
     const P2Item *items;
     int64_t nitems;
     const P2Entry *entries;
-    const int32_t *batch_ptr;
     const int64_t *slot_rowbase;
     const int32_t *slot_dr;
     const int64_t *eoffsets, *boffsets_e, *coffsets;
@@ -55,6 +54,8 @@ struct P2Args {
     const double *b_r, *s_r;
     double *S, *g;
     const unsigned long long *flag;
+    int uniform_offsets;   // one run of d_e == DE and one d_r: E/z/C offsets are e*DE^2, e*DE, c*cstride
+    int64_t cstride;
 };
 
 // ---------------------------------------------------------------------------------------------------------
@@ -188,14 +189,132 @@ __global__ void k_finalize_flag(const unsigned long long *flag, int64_t *out) {
 
 // ---------------------------------------------------------------------------------------------------------
 // Phase 2
+//
+// Software pipeline per workgroup: while batch b is applied from LDS, the coupling data of batch b+1 is in flight
+// into registers and the landmark entries of batch b+2 into wave 0's lanes. Batches are formed in-kernel with the
+// host's greedy rule (at most P2_MAXE entries and P2_LDS staged doubles), from a ballot over a wave scan of the
+// entries' staged sizes.
 // ---------------------------------------------------------------------------------------------------------
+constexpr int P2_Q = P2_LDS / P2_THREADS;   // staged doubles per lane per batch
+
+struct P2Meta {
+    int nb;
+    int off[P2_MAXE + 1];
+    int c1off[P2_MAXE], c2off[P2_MAXE], de[P2_MAXE];
+    unsigned long long mask[P2_MAXE];
+    int64_t src_e[P2_MAXE], src_z[P2_MAXE], src_c1[P2_MAXE], src_c2[P2_MAXE];
+};
+
+// Wave 0: load up to 64 entries starting at `start`.
+__device__ inline bool p2_load_entries(const P2Args &a, int64_t start, int64_t end, int lane, P2Entry &en) {
+    if (start + lane < end) {
+        en = a.entries[start + lane];
+        return true;
+    }
+    return false;
+}
+
+// Wave 0: form the batch from the loaded entries and describe it in `m`; returns its entry count.
+template <int DE>
+__device__ inline int p2_process(const P2Args &a, const P2Item &it, const int *slot_dr, P2Meta &m, bool valid,
+                                 const P2Entry &en, int lane) {
+    int de = DE;
+    if (DE == 0) de = valid ? a.e_dims[en.e] : 1;
+    int64_t se = 0, sz = 0, sc1 = 0, sc2 = 0;
+    int len1 = 0;
+    if (valid) {
+        if (a.uniform_offsets) {
+            se = (int64_t)en.e * (DE * DE);
+            sz = (int64_t)en.e * DE;
+            sc1 = (int64_t)en.c_first * a.cstride;
+            sc2 = (int64_t)en.c2 * a.cstride;
+        } else {
+            se = a.eoffsets[en.e];
+            sz = a.boffsets_e[en.e];
+            sc1 = a.coffsets[en.c_first];
+            sc2 = a.coffsets[en.c2];
+        }
+        if (it.uniform_dr) {
+            len1 = __popcll(en.mask) * de * it.uniform_dr;
+        } else {
+            unsigned long long mm = en.mask;
+            while (mm) { const int sl = __builtin_ctzll(mm); mm &= mm - 1; len1 += de * slot_dr[sl]; }
+        }
+    }
+    const int size = valid ? de * de + de + (it.has_diag ? 0 : de * it.dr2) + len1 : 0;
+    int incl = size;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const int excl = incl - size;
+    const unsigned long long take = __ballot(valid && incl <= P2_LDS);
+    const int nb = __popcll(take);
+    if (lane < nb) {
+        m.off[lane] = excl;
+        const int c1 = excl + de * de + de + (it.has_diag ? 0 : de * it.dr2);
+        m.c1off[lane] = c1;
+        m.c2off[lane] = it.has_diag ? c1 + len1 - de * it.dr2 : excl + de * de + de;
+        m.de[lane] = de;
+        m.mask[lane] = en.mask;
+        m.src_e[lane] = se;
+        m.src_z[lane] = sz;
+        m.src_c1[lane] = sc1;
+        m.src_c2[lane] = sc2;
+        if (lane == nb - 1) m.off[nb] = incl;
+    }
+    if (lane == 0) {
+        m.nb = nb;
+        if (nb == 0) m.off[0] = 0;
+    }
+    return nb;
+}
+
+// All lanes: issue the loads of a batch's staged data into registers.
+template <int DE>
+__device__ inline void p2_issue(const P2Args &a, const P2Item &it, const P2Meta &m, int t, double *v) {
+    const int nb = m.nb;
+    const int L = m.off[nb];
+#pragma unroll
+    for (int q = 0; q < P2_Q; q++) {
+        const int p = t + q * P2_THREADS;
+        if (p < L) {
+            int lo = 0, hi = nb;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (m.off[mid] <= p) lo = mid; else hi = mid;
+            }
+            const int de = DE ? DE : m.de[lo];
+            int r = p - m.off[lo];
+            const double *src;
+            if (r < de * de) {
+                src = a.E_inv + m.src_e[lo] + r;
+            } else if ((r -= de * de) < de) {
+                src = a.z + m.src_z[lo] + r;
+            } else {
+                r -= de;
+                if (!it.has_diag && r < de * it.dr2) src = a.C + m.src_c2[lo] + r;
+                else src = a.C + m.src_c1[lo] + (it.has_diag ? r : r - de * it.dr2);
+            }
+            v[q] = *src;
+        }
+    }
+}
+
+__device__ inline void p2_store(const P2Meta &m, int t, const double *v, double *lds) {
+    const int L = m.off[m.nb];
+#pragma unroll
+    for (int q = 0; q < P2_Q; q++) {
+        const int p = t + q * P2_THREADS;
+        if (p < L) lds[p] = v[q];
+    }
+}
+
 template <int DE>
 __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
     __shared__ double lds[P2_LDS];
-    __shared__ int s_off[P2_MAXE + 1];
-    __shared__ int s_c1off[P2_MAXE], s_c2off[P2_MAXE], s_size[P2_MAXE], s_de[P2_MAXE];
-    __shared__ unsigned long long s_mask[P2_MAXE];
-    __shared__ int64_t s_src_e[P2_MAXE], s_src_z[P2_MAXE], s_src_c1[P2_MAXE], s_src_c2[P2_MAXE];
+    __shared__ P2Meta meta[2];
     __shared__ int s_slot_dr[64], s_cum[65];
 
     if (*a.flag != (unsigned long long)NO_FAILURE) return;   // phase 1 failed: S and g_S stay untouched
@@ -203,20 +322,22 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
     if (idx >= a.nitems) return;
     const P2Item it = a.items[idx];
     const int t = threadIdx.x;
+    const int lane = t & 63;
+    const bool wave0 = t < 64;
     if (t < it.nslots) s_slot_dr[t] = a.slot_dr[it.slot_lo + t];
     __syncthreads();
     if (t == 0) {
         int c = 0;
-        for (int s = 0; s < it.nslots; s++) { s_cum[s] = c; c += s_slot_dr[s]; }
+        for (int sl = 0; sl < it.nslots; sl++) { s_cum[sl] = c; c += s_slot_dr[sl]; }
         s_cum[it.nslots] = c;
     }
     __syncthreads();
     int my_slot = -1, my_row = 0;
     if (t < it.ntasks) {
-        int s = 0;
-        while (s_cum[s + 1] <= t) s++;
-        my_slot = s;
-        my_row = t - s_cum[s];
+        int sl = 0;
+        while (s_cum[sl + 1] <= t) sl++;
+        my_slot = sl;
+        my_row = t - s_cum[sl];
     }
     const bool diag = it.has_diag && my_slot == it.nslots - 1;
     const int dr2 = it.dr2;
@@ -233,89 +354,53 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
     const unsigned long long lowmask = my_slot > 0 ? ((1ull << my_slot) - 1ull) : 0ull;
     const int udr = it.uniform_dr;
 
-    for (int b = it.batch_lo; b < it.batch_hi; b++) {
-        const int j0 = a.batch_ptr[b];
-        const int nb = a.batch_ptr[b + 1] - j0;
-        if (t < nb) {
-            const P2Entry en = a.entries[j0 + t];
-            const int de = DE ? DE : a.e_dims[en.e];
-            int len1;
-            if (udr) {
-                len1 = __popcll(en.mask) * de * udr;
-            } else {
-                len1 = 0;
-                unsigned long long m = en.mask;
-                while (m) { const int s = __builtin_ctzll(m); m &= m - 1; len1 += de * s_slot_dr[s]; }
-            }
-            s_mask[t] = en.mask;
-            s_de[t] = de;
-            s_src_e[t] = a.eoffsets[en.e];
-            s_src_z[t] = a.boffsets_e[en.e];
-            s_src_c1[t] = a.coffsets[en.c_first];
-            s_src_c2[t] = a.coffsets[en.c2];
-            s_size[t] = de * de + de + (it.has_diag ? 0 : de * dr2) + len1;
-            s_c1off[t] = len1;   // temporarily the range length
-        }
-        __syncthreads();
-        if (t == 0) {
-            int o = 0;
-            for (int j = 0; j < nb; j++) {
-                const int de = s_de[j];
-                s_off[j] = o;
-                const int len1 = s_c1off[j];
-                const int c1 = o + de * de + de + (it.has_diag ? 0 : de * dr2);
-                s_c1off[j] = c1;
-                s_c2off[j] = it.has_diag ? c1 + len1 - de * dr2 : o + de * de + de;
-                o += s_size[j];
-            }
-            s_off[nb] = o;
-        }
-        __syncthreads();
-        const int L = s_off[nb];
-        for (int p = t; p < L; p += P2_THREADS) {
-            int lo = 0, hi = nb;   // entry j with s_off[j] <= p < s_off[j+1]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= p) lo = mid; else hi = mid;
-            }
-            const int j = lo;
-            const int de = DE ? DE : s_de[j];
-            int q = p - s_off[j];
-            double v;
-            if (q < de * de) {
-                v = a.E_inv[s_src_e[j] + q];
-            } else if ((q -= de * de) < de) {
-                v = a.z[s_src_z[j] + q];
-            } else {
-                q -= de;
-                if (!it.has_diag && q < de * dr2) v = a.C[s_src_c2[j] + q];
-                else v = a.C[s_src_c1[j] + (it.has_diag ? q : q - de * dr2)];
-            }
-            lds[p] = v;
-        }
-        __syncthreads();
+    // prologue: describe the first two batches, stage the first
+    int64_t start = it.entry_lo;
+    P2Entry en{};
+    bool ev = false;
+    if (wave0) {
+        ev = p2_load_entries(a, start, it.entry_hi, lane, en);
+        start += p2_process<DE>(a, it, s_slot_dr, meta[0], ev, en, lane);
+        ev = p2_load_entries(a, start, it.entry_hi, lane, en);
+        start += p2_process<DE>(a, it, s_slot_dr, meta[1], ev, en, lane);
+    }
+    __syncthreads();
+    double v[P2_Q];
+    if (meta[0].nb > 0) {
+        p2_issue<DE>(a, it, meta[0], t, v);
+        p2_store(meta[0], t, v, lds);
+    }
+    __syncthreads();
+    int cur = 0;
+    while (meta[cur].nb > 0) {
+        const int nxt = cur ^ 1;
+        const bool have_next = meta[nxt].nb > 0;
+        if (have_next) p2_issue<DE>(a, it, meta[nxt], t, v);
+        if (wave0) ev = p2_load_entries(a, start, it.entry_hi, lane, en);
+        const P2Meta &m = meta[cur];
         if (my_slot >= 0) {
+            const int nb = m.nb;
             unsigned long long bits = 0;
-            for (int j = 0; j < nb; j++) bits |= ((s_mask[j] >> my_slot) & 1ull) << j;
+            for (int j = 0; j < nb; j++) bits |= ((m.mask[j] >> my_slot) & 1ull) << j;
             while (bits) {
                 const int j = __builtin_ctzll(bits);
                 bits &= bits - 1;
-                const int de = DE ? DE : s_de[j];
-                const unsigned long long m = s_mask[j];
+                const int de = DE ? DE : m.de[j];
+                const unsigned long long mk = m.mask[j];
                 int pre;
                 if (udr) {
-                    pre = __popcll(m & lowmask) * de * udr;
+                    pre = __popcll(mk & lowmask) * de * udr;
                 } else {
                     pre = 0;
-                    unsigned long long mm = m & lowmask;
-                    while (mm) { const int s = __builtin_ctzll(mm); mm &= mm - 1; pre += de * s_slot_dr[s]; }
+                    unsigned long long mm = mk & lowmask;
+                    while (mm) { const int sl = __builtin_ctzll(mm); mm &= mm - 1; pre += de * s_slot_dr[sl]; }
                 }
-                const double *Ei = lds + s_off[j];
+                const double *Ei = lds + m.off[j];
                 const double *zz = Ei + de * de;
-                const double *C2 = lds + s_c2off[j];
-                const double *C1 = lds + s_c1off[j] + pre + my_row * de;
-                double w[DE ? DE : 8];
+                const double *C2 = lds + m.c2off[j];
+                const double *C1 = lds + m.c1off[j] + pre + my_row * de;
                 if constexpr (DE > 0) {
+                    double w[DE];
 #pragma unroll
                     for (int i = 0; i < DE; i++) {
                         double s = 0.0;
@@ -339,6 +424,7 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
                         gacc -= s;
                     }
                 } else {
+                    double w[8];
                     for (int i = 0; i < de; i++) {
                         double s = 0.0;
                         for (int k = 0; k < de; k++) s += Ei[i + k * de] * C1[k];
@@ -361,6 +447,10 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
             }
         }
         __syncthreads();
+        if (have_next) p2_store(meta[nxt], t, v, lds);
+        if (wave0) start += p2_process<DE>(a, it, s_slot_dr, meta[cur], ev, en, lane);
+        __syncthreads();
+        cur = nxt;
     }
     if (my_slot >= 0) {
 #pragma unroll
@@ -476,7 +566,8 @@ struct bls_schur {
     DevArray<int64_t> d_r_to_s, d_diag_inds, d_S_colptr, d_S_rowval;
     DevArray<P2Item> d_items;
     DevArray<P2Entry> d_entries;
-    DevArray<int32_t> d_batch_ptr, d_slot_dr;
+    DevArray<int32_t> d_slot_dr;
+    int uniform_dr_all = 0;
     DevArray<int64_t> d_slot_rowbase;
     DevArray<unsigned long long> d_flag;
     DevArray<int64_t> d_failed_out;
@@ -505,7 +596,6 @@ template <typename Slots, typename RowBase, typename Base>
 int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of) {
     std::vector<P2Item> items;
     std::vector<P2Entry> entries;
-    std::vector<int32_t> batch_ptr{0};
     std::vector<int64_t> slot_rowbase;
     std::vector<int32_t> slot_dr;
     std::vector<int64_t> owner(h->ncoup);
@@ -544,8 +634,7 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
             }
             it.uniform_dr = udr;
             const int64_t r1_lo = col[k0], r1_hi = col[k1 - 1];
-            it.batch_lo = (int32_t)batch_ptr.size() - 1;
-            int nb = 0, lds = 0;
+            it.entry_lo = (int64_t)entries.size();
             for (int64_t kk = h->retained_ptr[r2]; kk < h->retained_ptr[r2 + 1]; kk++) {
                 const int64_t c2 = h->retained_entry[kk];
                 const int64_t e = owner[c2];
@@ -569,29 +658,21 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
                 const int de = (int)h->e_dims[e];
                 const int size = de * de + de + (it.has_diag ? 0 : de * dr2) + len1;
                 if (size > P2_LDS) return fail(BLS_ERR_UNSUPPORTED, "landmark data exceeds the staging buffer");
-                if (nb == P2_MAXE || lds + size > P2_LDS) {
-                    batch_ptr.push_back((int32_t)entries.size());
-                    nb = 0;
-                    lds = 0;
-                }
                 P2Entry en{};
                 en.e = (int32_t)e;
                 en.c_first = (int32_t)ca;
                 en.c2 = (int32_t)c2;
                 en.mask = mask;
                 entries.push_back(en);
-                nb++;
-                lds += size;
             }
-            if (nb > 0) batch_ptr.push_back((int32_t)entries.size());
-            it.batch_hi = (int32_t)batch_ptr.size() - 1;
+            it.entry_hi = (int64_t)entries.size();
             items.push_back(it);
             k0 = k1;
         }
     }
     if (entries.size() > (size_t)INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "too many phase-2 entries");
     int rc;
-    if ((rc = h->d_items.upload(items)) || (rc = h->d_entries.upload(entries)) || (rc = h->d_batch_ptr.upload(batch_ptr)) ||
+    if ((rc = h->d_items.upload(items)) || (rc = h->d_entries.upload(entries)) ||
         (rc = h->d_slot_rowbase.upload(slot_rowbase)) || (rc = h->d_slot_dr.upload(slot_dr)))
         return rc;
     h->nitems = (int64_t)items.size();
@@ -647,7 +728,6 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.items = h->d_items.ptr;
     a.nitems = h->nitems;
     a.entries = h->d_entries.ptr;
-    a.batch_ptr = h->d_batch_ptr.ptr;
     a.slot_rowbase = h->d_slot_rowbase.ptr;
     a.slot_dr = h->d_slot_dr.ptr;
     a.eoffsets = h->d_eoffsets.ptr;
@@ -662,6 +742,8 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.S = S;
     a.g = g;
     a.flag = h->d_flag.ptr;
+    a.uniform_offsets = h->uniform_de > 0 && h->uniform_dr_all > 0 && h->runs.size() == 1;
+    a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
     if (h->nitems > 0) {
         const unsigned grid = (unsigned)(8 * ((h->nitems + 7) / 8));
         switch (h->uniform_de) {
@@ -737,6 +819,9 @@ extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, 
         if (h->e_dims[f] != h->uniform_de) h->uniform_de = 0;
     }
     if (h->uniform_de > 3) h->uniform_de = 0;
+    h->uniform_dr_all = nrblocks ? (int)h->r_dims[0] : 0;
+    for (int64_t r = 0; r < nrblocks; r++)
+        if (h->r_dims[r] != h->uniform_dr_all) h->uniform_dr_all = 0;
     if ((rc = h->d_eoffsets.upload(h->eoffsets)) || (rc = h->d_boffsets_e.upload(h->boffsets_e)) ||
         (rc = h->d_coffsets.upload(h->coffsets)) || (rc = h->d_eliminated_ptr.upload(h->eliminated_ptr)) ||
         (rc = h->d_boffsets_r.upload(h->boffsets_r)) || (rc = h->d_e_dims.upload(narrow32(h->e_dims))) ||

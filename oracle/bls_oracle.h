@@ -57,6 +57,7 @@ typedef struct {
 
 #define ORC_DECL(T, S)                                                                                          \
     int orc_inv_sym_block_##S(int64_t d, const T *H, const T *damp, T *invH);                                  \
+    int orc_inv_sym_block_chol_##S(int64_t d, const T *H, const T *damp, T *invH);                             \
     int orc_reset_schur_dense_##S(int64_t nr, T *S, const T *R, const double *scaling_r, double lambda,        \
                                   const double *s_r);                                                           \
     int orc_reset_schur_sparse_##S(int64_t nnz, T *nzval, int64_t nR, const T *R_nzval,                       \

@@ -60,7 +60,15 @@ int FN(orc_inv_sym_block)(int64_t d, const T *H, const T *damp, T *invH)
         invH[6] = c13 * inv_det; invH[7] = c23 * inv_det; invH[8] = c33 * inv_det;
         return 1;
     }
-    /* kernels.jl:69-83: Cholesky of Symmetric(Hd, :U), every pivot ratio |U_ii|^2 / Hd_ii > tol, inv = U^-1 U^-T */
+    return FN(orc_inv_sym_block_chol)(d, H, damp, invH);
+}
+
+/* The generic method of inv_sym_block (src/kernels.jl:69-83), callable at any size 1..8 (test/kernels.jl:11-49
+ * calls it at sizes 1-3 to pin the closed forms): Cholesky of Symmetric(Hd, :U), every pivot ratio
+ * |U_ii|^2 / Hd_ii > tol, inv = U^-1 U^-T. */
+int FN(orc_inv_sym_block_chol)(int64_t d, const T *H, const T *damp, T *invH)
+{
+    const T tol = (T)4504 * (sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07);
     T U[64], Ui[64];
     if (d > 8) return 0;
     for (int64_t j = 0; j < d; j++)

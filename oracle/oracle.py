@@ -57,6 +57,7 @@ def lib():
             getattr(_lib, f"orc_reduce_schur_{s}").restype = ct.c_int
             getattr(_lib, f"orc_reduce_qr_{s}").restype = ct.c_int
             getattr(_lib, f"orc_inv_sym_block_{s}").restype = ct.c_int
+            getattr(_lib, f"orc_inv_sym_block_chol_{s}").restype = ct.c_int
             getattr(_lib, f"orc_factor_diagonal_blocks_{s}").restype = ct.c_int
         _lib.orc_pcg_f64.restype = ct.c_int64
     return _lib
@@ -123,13 +124,15 @@ def _sfx(dtype):
     return "f64" if np.dtype(dtype) == np.float64 else "f32"
 
 
-def inv_sym_block(H, damp):
-    H = np.asfortranarray(H)
+def inv_sym_block(H, damp, chol=False):
+    """inv_sym_block(H, one(T), damp): closed forms for d <= 3 (chol=False) or the generic Cholesky method."""
+    H = np.asarray(H)
     d = H.shape[0]
     dt = H.dtype
     Hc = np.ascontiguousarray(H.T.reshape(-1))   # column-major flat
     out = np.zeros(d * d, dtype=dt)
-    ok = getattr(lib(), f"orc_inv_sym_block_{_sfx(dt)}")(ct.c_int64(d), _ptr(Hc, dt), _ptr(np.ascontiguousarray(damp, dtype=dt), dt), _ptr(out, dt))
+    f = getattr(lib(), f"orc_inv_sym_block{'_chol' if chol else ''}_{_sfx(dt)}")
+    ok = f(ct.c_int64(d), _ptr(Hc, dt), _ptr(np.ascontiguousarray(damp, dtype=dt), dt), _ptr(out, dt))
     return bool(ok), out.reshape(d, d).T.copy()
 
 
