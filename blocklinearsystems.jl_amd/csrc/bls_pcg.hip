@@ -1,0 +1,447 @@
+// Block-Jacobi preconditioned CG on the reduced camera system(s), batched over independent problems.
+//
+// The reference has no PCG (its Krylov extension runs LSMR on the QR path with the BlockJacobi preconditioner of
+// ext/KrylovExt.jl:88-95, from factor_diagonal_blocks! / solve_diagonal_blocks!, src/diagonal_blocks.jl:6-61);
+// this is textbook PCG (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 9.1) with that
+// preconditioner, judged by the true residual ||S x - g|| / ||g|| in FP64.
+//
+// Layout: S arrives in the upper-block CSC of build_schur_pattern; at each solve it is gathered once (the copyto!
+// kernel) into a symmetric block-row layout (BSR: every block (i, j) of the full matrix, row-major, blocks of a
+// block row contiguous). An iteration is two kernels: the BSR product q = S p, one lane per scalar row, and an
+// update kernel with one workgroup per problem (dot products reduced in a fixed order: deterministic). A fixed
+// number of iterations is captured once in a hipGraph on the handle's own stream and replayed until every problem
+// has converged.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bls_internal.h"
+
+using namespace bls;
+
+namespace {
+
+constexpr int UPD_THREADS = 512;
+constexpr int ITERS_PER_GRAPH = 8;
+constexpr int MAXD = 16;
+
+struct ProbState {
+    double rz, gnorm, relres;
+    int64_t iters;
+    int32_t active, failed;
+};
+
+struct PcgArgs {
+    int64_t nr;
+    const int32_t *row_block;          // block row of each scalar row
+    const int32_t *bsr_rowptr, *bsr_col;
+    const int64_t *bsr_voff;
+    const int32_t *dims;
+    const int64_t *bo;                 // 0-based scalar offset of each block
+    const int32_t *prob_of_block;
+    const int64_t *prob_bptr;          // block range of each problem (0-based, nprob + 1)
+    const double *vals;                // BSR values
+    const double *L;                   // preconditioner factors (lower, column-major d x d per block)
+    const int64_t *loff;               // offset of each block's factor
+    double *x, *r, *z, *p, *q, *g;
+    ProbState *st;
+    int *nactive;
+    double tol;
+    int64_t maxit;
+};
+
+__global__ void k_bsr_spmv(PcgArgs a, const double *xin, double *yout, int only_active) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nr) return;
+    const int i = a.row_block[s];
+    if (only_active && !a.st[a.prob_of_block[i]].active) return;
+    const int di = a.dims[i];
+    const int row = (int)(s - a.bo[i]);
+    double acc = 0.0;
+    for (int k = a.bsr_rowptr[i]; k < a.bsr_rowptr[i + 1]; k++) {
+        const int j = a.bsr_col[k];
+        const int dj = a.dims[j];
+        const double *B = a.vals + a.bsr_voff[k] + (int64_t)row * dj;
+        const double *xj = xin + a.bo[j];
+        for (int c = 0; c < dj; c++) acc += B[c] * xj[c];
+    }
+    yout[s] = acc;
+    (void)di;
+}
+
+// z_b = (L L')^-1 r_b for one block (solve_diagonal_blocks!, src/diagonal_blocks.jl:33-61)
+__device__ inline void block_solve(const double *L, int d, const double *rb, double *zb) {
+    double y[MAXD];
+    for (int i = 0; i < d; i++) y[i] = rb[i];
+    for (int j = 0; j < d; j++) {
+        const double yj = y[j] / L[j * d + j];
+        y[j] = yj;
+        for (int i = j + 1; i < d; i++) y[i] -= L[j * d + i] * yj;
+    }
+    for (int j = d - 1; j >= 0; j--) {
+        double t = y[j];
+        for (int i = j + 1; i < d; i++) t -= L[j * d + i] * y[i];
+        y[j] = t / L[j * d + j];
+    }
+    for (int i = 0; i < d; i++) zb[i] = y[i];
+}
+
+// fixed-order workgroup sum (deterministic)
+__device__ inline double wg_sum(double v, double *red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = UPD_THREADS / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    const double out = red[0];
+    __syncthreads();
+    return out;
+}
+
+// Problem-wise (re)start of CG from the current x: r = g - S x (q holds S x; first = 1: x = 0, r = g),
+// z = M^-1 r, p = z, rz; the true relative residual decides whether the problem (re)enters the iteration.
+__global__ __launch_bounds__(UPD_THREADS) void k_pcg_restart(PcgArgs a, int first) {
+    __shared__ double red[UPD_THREADS];
+    const int pb = blockIdx.x;
+    const int64_t b0 = a.prob_bptr[pb], b1 = a.prob_bptr[pb + 1];
+    const int64_t s0 = a.bo[b0], s1 = a.bo[b1 - 1] + a.dims[b1 - 1];
+    ProbState &ps = a.st[pb];
+    double gg = 0.0, rr = 0.0;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
+        const double gv = a.g[s];
+        if (first) a.x[s] = 0.0;
+        const double rv = first ? gv : gv - a.q[s];
+        a.r[s] = rv;
+        gg += gv * gv;
+        rr += rv * rv;
+    }
+    gg = wg_sum(gg, red);
+    rr = wg_sum(rr, red);
+    const bool ok = !ps.failed;
+    if (ok)
+        for (int64_t b = b0 + threadIdx.x; b < b1; b += UPD_THREADS)
+            block_solve(a.L + a.loff[b], a.dims[b], a.r + a.bo[b], a.z + a.bo[b]);
+    __syncthreads();
+    double rz = 0.0;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
+        const double zv = ok ? a.z[s] : 0.0;
+        a.p[s] = zv;
+        rz += a.r[s] * zv;
+    }
+    rz = wg_sum(rz, red);
+    if (threadIdx.x == 0) {
+        if (first) { ps.gnorm = sqrt(gg); ps.iters = 0; }
+        ps.rz = rz;
+        ps.relres = ps.gnorm > 0 ? sqrt(rr) / ps.gnorm : 0.0;
+        const int was = first ? 0 : ps.active;
+        ps.active = (ok && ps.relres > a.tol && ps.iters < a.maxit && isfinite(ps.relres)) ? 1 : 0;
+        if (ps.active && !was) atomicAdd(a.nactive, 1);
+        if (!ps.active && was) atomicSub(a.nactive, 1);
+    }
+}
+
+// one PCG iteration after q = S p (Saad Alg. 9.1, lines 3-7)
+__global__ __launch_bounds__(UPD_THREADS) void k_pcg_update(PcgArgs a) {
+    __shared__ double red[UPD_THREADS];
+    const int pb = blockIdx.x;
+    ProbState &ps = a.st[pb];
+    if (!ps.active) return;
+    const int64_t b0 = a.prob_bptr[pb], b1 = a.prob_bptr[pb + 1];
+    const int64_t s0 = a.bo[b0], s1 = a.bo[b1 - 1] + a.dims[b1 - 1];
+    double pq = 0.0;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) pq += a.p[s] * a.q[s];
+    pq = wg_sum(pq, red);
+    const double alpha = ps.rz / pq;
+    double rr = 0.0;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
+        a.x[s] += alpha * a.p[s];
+        const double rv = a.r[s] - alpha * a.q[s];
+        a.r[s] = rv;
+        rr += rv * rv;
+    }
+    rr = wg_sum(rr, red);
+    for (int64_t b = b0 + threadIdx.x; b < b1; b += UPD_THREADS)
+        block_solve(a.L + a.loff[b], a.dims[b], a.r + a.bo[b], a.z + a.bo[b]);
+    __syncthreads();
+    double rz = 0.0;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) rz += a.r[s] * a.z[s];
+    rz = wg_sum(rz, red);
+    const double beta = rz / ps.rz;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) a.p[s] = a.z[s] + beta * a.p[s];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ps.rz = rz;
+        ps.iters += 1;
+        ps.relres = ps.gnorm > 0 ? sqrt(rr) / ps.gnorm : 0.0;
+        if (!(ps.relres > a.tol) || ps.iters >= a.maxit || !isfinite(ps.relres)) {
+            ps.active = 0;
+            atomicSub(a.nactive, 1);
+        }
+    }
+}
+
+// Preconditioner blocks: the diagonal blocks of S (upper triangle mirrored), optionally replaced by a corrected
+// block, then factored in place (factor_diagonal_blocks!, src/diagonal_blocks.jl:6-30). A block that is not
+// positive definite marks its problem failed.
+__global__ void k_pcg_factor(PcgArgs a, int64_t nblocks, const int64_t *diag_voff, double *Lout,
+                             const double *Mblocks) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const int d = a.dims[b];
+    double *L = Lout + a.loff[b];
+    if (Mblocks) {
+        for (int q = 0; q < d * d; q++) L[q] = Mblocks[a.loff[b] + q];
+    } else {
+        const double *B = a.vals + diag_voff[b];   // row-major full block
+        for (int c = 0; c < d; c++)
+            for (int r = 0; r < d; r++) L[c * d + r] = B[r * d + c];
+    }
+    for (int j = 0; j < d; j++) {
+        double s = L[j * d + j];
+        for (int k = 0; k < j; k++) s -= L[k * d + j] * L[k * d + j];
+        if (!(s > 0.0) || !isfinite(s)) {
+            a.st[a.prob_of_block[b]].failed = 1;
+            return;
+        }
+        const double ljj = sqrt(s);
+        L[j * d + j] = ljj;
+        for (int i = j + 1; i < d; i++) {
+            double t = L[j * d + i];
+            for (int k = 0; k < j; k++) t -= L[k * d + i] * L[k * d + j];
+            L[j * d + i] = t / ljj;
+        }
+    }
+}
+
+__global__ void k_gather_f64(double *dest, const double *src, const int64_t *map, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dest[i] = src[map[i] - 1];
+}
+
+__global__ void k_copy_f64(double *dst, const double *src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_pcg_init_state(ProbState *st, int nprob, int *nactive) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *nactive = 0;
+    if (i < nprob) {
+        st[i].rz = 0; st[i].gnorm = 0; st[i].relres = 0; st[i].iters = 0; st[i].active = 0; st[i].failed = 0;
+    }
+}
+
+}  // namespace
+
+struct bls_pcg {
+    int64_t nr = 0, nrb = 0, nprob = 0, nvals = 0, nbsr = 0, nL = 0;
+    std::vector<int64_t> prob_bptr;
+    DevArray<int32_t> d_row_block, d_rowptr, d_col, d_dims, d_prob_of_block;
+    DevArray<int64_t> d_voff, d_bo, d_prob_bptr, d_map, d_loff, d_diag_voff;
+    DevArray<double> d_vals, d_L, d_x, d_r, d_z, d_p, d_q, d_g;
+    DevArray<ProbState> d_st;
+    DevArray<int> d_nactive;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double graph_tol = -1;
+    int64_t graph_maxit = -1;
+    ~bls_pcg() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    PcgArgs args(double tol, int64_t maxit) {
+        PcgArgs a{};
+        a.nr = nr;
+        a.row_block = d_row_block.ptr;
+        a.bsr_rowptr = d_rowptr.ptr;
+        a.bsr_col = d_col.ptr;
+        a.bsr_voff = d_voff.ptr;
+        a.dims = d_dims.ptr;
+        a.bo = d_bo.ptr;
+        a.prob_of_block = d_prob_of_block.ptr;
+        a.prob_bptr = d_prob_bptr.ptr;
+        a.vals = d_vals.ptr;
+        a.L = d_L.ptr;
+        a.loff = d_loff.ptr;
+        a.x = d_x.ptr; a.r = d_r.ptr; a.z = d_z.ptr; a.p = d_p.ptr; a.q = d_q.ptr; a.g = d_g.ptr;
+        a.st = d_st.ptr;
+        a.nactive = d_nactive.ptr;
+        a.tol = tol;
+        a.maxit = maxit;
+        return a;
+    }
+};
+
+extern "C" int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dims, const int64_t *boffsets_r,
+                              int64_t nnz, const int64_t *S_colptr, int64_t nnz_blocks, const int64_t *block_colptr,
+                              const int64_t *block_rowval, const int64_t *block_row_offsets,
+                              const int64_t *col_heights, int64_t nprob, const int64_t *prob_block_ptr,
+                              bls_pcg **out) {
+    if (!out || nr < 0 || nrblocks < 0 || nprob < 1) return fail(BLS_ERR_ARG, "bls_pcg_create: bad arguments");
+    *out = nullptr;
+    for (int64_t r = 0; r < nrblocks; r++)
+        if (r_dims[r] < 1 || r_dims[r] > MAXD) return fail(BLS_ERR_UNSUPPORTED, "retained block dimension outside 1..16");
+    if (prob_block_ptr[0] != 1 || prob_block_ptr[nprob] != nrblocks + 1)
+        return fail(BLS_ERR_ARG, "prob_block_ptr must run from 1 to nrblocks + 1");
+    for (int64_t p = 0; p < nprob; p++)
+        if (prob_block_ptr[p + 1] <= prob_block_ptr[p]) return fail(BLS_ERR_ARG, "every problem needs a retained block");
+    auto *h = new bls_pcg();
+    h->nr = nr;
+    h->nrb = nrblocks;
+    h->nprob = nprob;
+    // full symmetric block-row lists: (col, slot, transposed)
+    struct Ent { int64_t col, slot; bool tr; };
+    std::vector<std::vector<Ent>> rows(nrblocks);
+    for (int64_t j = 0; j < nrblocks; j++)
+        for (int64_t s = block_colptr[j] - 1; s < block_colptr[j + 1] - 1; s++) {
+            const int64_t i = block_rowval[s] - 1;
+            rows[i].push_back({j, s, false});
+            if (i < j) rows[j].push_back({i, s, true});
+        }
+    std::vector<int32_t> rowptr{0}, col, dims(nrblocks), row_block(nr), prob_of_block(nrblocks);
+    std::vector<int64_t> voff, map, bo(nrblocks), loff(nrblocks), diag_voff(nrblocks, -1), pbp(nprob + 1);
+    for (int64_t r = 0; r < nrblocks; r++) {
+        dims[r] = (int32_t)r_dims[r];
+        bo[r] = boffsets_r[r] - 1;
+        for (int64_t a = 0; a < r_dims[r]; a++) row_block[bo[r] + a] = (int32_t)r;
+    }
+    for (int64_t p = 0; p <= nprob; p++) pbp[p] = prob_block_ptr[p] - 1;
+    for (int64_t p = 0; p < nprob; p++)
+        for (int64_t b = pbp[p]; b < pbp[p + 1]; b++) prob_of_block[b] = (int32_t)p;
+    int64_t nv = 0, nl = 0;
+    for (int64_t i = 0; i < nrblocks; i++) {
+        std::sort(rows[i].begin(), rows[i].end(), [](const Ent &x, const Ent &y) { return x.col < y.col; });
+        const int64_t di = r_dims[i];
+        for (const Ent &en : rows[i]) {
+            const int64_t j = en.col, dj = r_dims[j];
+            if (prob_of_block[j] != prob_of_block[i]) { delete h; return fail(BLS_ERR_ARG, "a block couples two problems"); }
+            col.push_back((int32_t)j);
+            voff.push_back(nv);
+            if (j == i) diag_voff[i] = nv;
+            for (int64_t a = 0; a < di; a++)
+                for (int64_t c = 0; c < dj; c++) {
+                    int64_t idx;   // 1-based index into S.nzval of S[(i,a), (j,c)]
+                    const int64_t ro = block_row_offsets[en.slot];
+                    if (!en.tr) {   // stored block (i, j), i <= j, in column j
+                        if (i == j && a > c) idx = S_colptr[bo[i] + a] + ro + c;
+                        else idx = S_colptr[bo[j] + c] + ro + a;
+                    } else {        // i > j: stored (j, i) in column i; element (c, a)
+                        idx = S_colptr[bo[i] + a] + ro + c;
+                    }
+                    map.push_back(idx);
+                    nv++;
+                }
+        }
+        rowptr.push_back((int32_t)col.size());
+        loff[i] = nl;
+        nl += di * di;
+        if (diag_voff[i] < 0) { delete h; return fail(BLS_ERR_ARG, "the pattern has no diagonal block " + std::to_string(i + 1)); }
+    }
+    (void)nnz; (void)nnz_blocks; (void)col_heights;
+    h->nvals = nv;
+    h->nbsr = (int64_t)col.size();
+    h->nL = nl;
+    h->prob_bptr = pbp;
+    int rc;
+    if ((rc = h->d_row_block.upload(row_block)) || (rc = h->d_rowptr.upload(rowptr)) || (rc = h->d_col.upload(col)) ||
+        (rc = h->d_dims.upload(dims)) || (rc = h->d_prob_of_block.upload(prob_of_block)) || (rc = h->d_voff.upload(voff)) ||
+        (rc = h->d_bo.upload(bo)) || (rc = h->d_prob_bptr.upload(pbp)) || (rc = h->d_map.upload(map)) ||
+        (rc = h->d_loff.upload(loff)) || (rc = h->d_diag_voff.upload(diag_voff)) || (rc = h->d_vals.alloc(nv)) ||
+        (rc = h->d_L.alloc(nl)) || (rc = h->d_x.alloc(nr)) || (rc = h->d_r.alloc(nr)) || (rc = h->d_z.alloc(nr)) ||
+        (rc = h->d_p.alloc(nr)) || (rc = h->d_q.alloc(nr)) || (rc = h->d_g.alloc(nr)) || (rc = h->d_st.alloc(nprob)) ||
+        (rc = h->d_nactive.alloc(1))) {
+        delete h;
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming) != hipSuccess) {
+        delete h;
+        return fail(BLS_ERR_HIP, "stream / event creation failed");
+    }
+    *out = h;
+    return BLS_OK;
+}
+
+extern "C" void bls_pcg_destroy(bls_pcg *h) { delete h; }
+
+extern "C" int bls_pcg_values_size(bls_pcg *h, int64_t *nvals, int64_t *nfactor) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (nvals) *nvals = h->nvals;
+    if (nfactor) *nfactor = h->nL;
+    return BLS_OK;
+}
+
+extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol,
+                                 int64_t maxit, const double *precond_blocks, int64_t *iters, double *relres,
+                                 void *stream) {
+    if (!h || !S_nzval || !g || !x) return fail(BLS_ERR_ARG, "bls_pcg_solve_f64: bad arguments");
+    if (!(tol > 0) || maxit < 0) return fail(BLS_ERR_ARG, "tol must be positive and maxit non-negative");
+    hipStream_t cs = (hipStream_t)stream;
+    hipStream_t s = h->stream;
+    BLS_HIP(hipEventRecord(h->ev_in, cs));
+    BLS_HIP(hipStreamWaitEvent(s, h->ev_in, 0));
+    PcgArgs a = h->args(tol, maxit);
+    const int nprob = (int)h->nprob;
+    const unsigned rows_grid = grid_for(h->nr, 256);
+    k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
+    k_copy_f64<<<std::min<int64_t>(grid_for(h->nr, 256), 4096), 256, 0, s>>>(h->d_g.ptr, g, h->nr);
+    k_pcg_init_state<<<grid_for(nprob, 256), 256, 0, s>>>(h->d_st.ptr, nprob, h->d_nactive.ptr);
+    k_pcg_factor<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_diag_voff.ptr, h->d_L.ptr, precond_blocks);
+    k_pcg_restart<<<nprob, UPD_THREADS, 0, s>>>(a, 1);
+    BLS_HIP(hipGetLastError());
+    // the iteration graph (handle-owned buffers only; rebuilt when tol / maxit change)
+    if (!h->exec || h->graph_tol != tol || h->graph_maxit != maxit) {
+        if (h->exec) { (void)hipGraphExecDestroy(h->exec); h->exec = nullptr; }
+        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+        BLS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < ITERS_PER_GRAPH; it++) {
+            k_bsr_spmv<<<rows_grid, 256, 0, s>>>(a, h->d_p.ptr, h->d_q.ptr, 1);
+            k_pcg_update<<<nprob, UPD_THREADS, 0, s>>>(a);
+        }
+        hipError_t e = hipStreamEndCapture(s, &h->graph);
+        if (e != hipSuccess) return fail(BLS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        BLS_HIP(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+        h->graph_tol = tol;
+        h->graph_maxit = maxit;
+    }
+    // iterate; then confirm every problem with its true residual, restarting those still above tol
+    int nact = 0;
+    for (int round = 0; round < 4; round++) {
+        BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        BLS_HIP(hipStreamSynchronize(s));
+        int64_t launched = 0;
+        while (nact > 0 && launched <= maxit + ITERS_PER_GRAPH) {
+            BLS_HIP(hipGraphLaunch(h->exec, s));
+            launched += ITERS_PER_GRAPH;
+            BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+            BLS_HIP(hipStreamSynchronize(s));
+        }
+        k_bsr_spmv<<<rows_grid, 256, 0, s>>>(a, h->d_x.ptr, h->d_q.ptr, 0);
+        k_pcg_restart<<<nprob, UPD_THREADS, 0, s>>>(a, 0);
+        BLS_HIP(hipGetLastError());
+        BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        BLS_HIP(hipStreamSynchronize(s));
+        if (nact == 0) break;
+    }
+    std::vector<ProbState> stv(nprob);
+    BLS_HIP(hipMemcpyAsync(stv.data(), h->d_st.ptr, sizeof(ProbState) * nprob, hipMemcpyDeviceToHost, s));
+    BLS_HIP(hipMemcpyAsync(x, h->d_x.ptr, sizeof(double) * h->nr, hipMemcpyDeviceToDevice, s));
+    BLS_HIP(hipEventRecord(h->ev_out, s));
+    BLS_HIP(hipStreamWaitEvent(cs, h->ev_out, 0));
+    BLS_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int p = 0; p < nprob; p++) {
+        if (relres) relres[p] = stv[p].relres;
+        if (iters) iters[p] = stv[p].failed ? -1 : stv[p].iters;
+        if (stv[p].failed || !(stv[p].relres <= tol)) all = false;
+    }
+    return all ? BLS_OK : BLS_FAILED;
+}

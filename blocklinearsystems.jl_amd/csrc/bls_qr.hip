@@ -1,0 +1,306 @@
+// QR elimination on gfx950: reduce_qr! (src/qr_reduction.jl:162-187) behind the C ABI of include/bls_amd.h.
+//
+// One wavefront per table T_e = [J_r | J_e] (M = m_e + d_e rows, column-major; residual in its own Float64
+// vector). Lane i holds row i of the d_e eliminated columns; the wave computes the d_e Householder reflectors
+// of qr_table! (src/qr_reduction.jl:50-131) cooperatively -- norms by wave reduction, the singularity test
+// |beta| > M eps ||col|| -- and leaves them in LDS. Each lane then streams whole retained columns through
+// registers (at most 64 rows), applies H_1 ... H_de to each in the reference's sequence, and writes it back; one
+// lane does the same for the residual, in Float64. Tables are bucketed by row count (16 / 32 / 64) so that a
+// column fits the registers of its bucket. The raw tables are never modified (retry semantics of the reference).
+#include <algorithm>
+#include <cmath>
+
+#include "bls_internal.h"
+
+using namespace bls;
+
+namespace {
+
+constexpr int QR_WAVES = 4;   // tables per workgroup (one wavefront each)
+constexpr int QR_MAXDE = 8;
+
+struct QRArgs {
+    const int32_t *tables;    // tables of this bucket (0-based)
+    int64_t ntab;
+    const int32_t *nrows, *ncols, *e_dims;   // per table: M, raw columns (table_ncols - 1); e_dims per table (0 = free)
+    const int64_t *raw_off, *res_off, *boff_e;
+    const double *Rw, *Rr, *scaling_e;
+    double lambda;
+    double *F, *Fr;
+    unsigned long long *flag;
+};
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+template <int MB>
+__global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
+    __shared__ double sV[QR_WAVES][QR_MAXDE][64];
+    __shared__ double sTau[QR_WAVES][QR_MAXDE];
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t ti = (int64_t)blockIdx.x * QR_WAVES + w;
+    if (ti >= a.ntab) return;
+    const int t = a.tables[ti];
+    const int M = a.nrows[t];
+    const int C = a.ncols[t];
+    const int de = a.e_dims[t];
+    const int64_t off = a.raw_off[t];
+    const int64_t rq = a.res_off[t];
+    double bad = 0.0;
+    if (de == 0) {   // a free table: copied (copy_table!, its result unused by reduce_qr!)
+        for (int64_t i = lane; i < (int64_t)M * C; i += 64) a.F[off + i] = a.Rw[off + i];
+        for (int i = lane; i < M; i += 64) a.Fr[rq + i] = a.Rr[rq + i];
+        return;
+    }
+    const int me = M - de, ce = C - de;
+    const int64_t so = a.boff_e[t];
+    // panel: lane i holds row i of the eliminated columns (damping rows zero, sqrt(lambda sc) on their diagonal)
+    double P[QR_MAXDE];
+#pragma unroll
+    for (int k = 0; k < QR_MAXDE; k++) {
+        P[k] = 0.0;
+        if (k < de) {
+            if (lane < me) {
+                const double x = a.Rw[off + (int64_t)(ce + k) * M + lane];
+                P[k] = x;
+                bad += x - x;
+            } else if (lane == me + k) {
+                P[k] = sqrt(a.lambda * a.scaling_e[so + k]);
+            }
+        }
+    }
+    const double tol = (double)M * 2.220446049250313e-16;
+    bool singular = false;
+#pragma unroll
+    for (int k = 0; k < QR_MAXDE; k++) {
+        if (k >= de) break;
+        const double x = lane < M ? P[k] : 0.0;
+        const double s_top = wave_sum(lane < k ? x * x : 0.0);
+        const double s_sub = wave_sum(lane >= k ? x * x : 0.0);
+        const double nrm = sqrt(s_sub);
+        const double full = sqrt(s_top + s_sub);
+        const double akk = __shfl(x, k, 64);
+        const double beta = akk >= 0.0 ? -nrm : nrm;
+        if (!(isfinite(beta) && fabs(beta) > tol * full)) { singular = true; break; }
+        const double scale = 1.0 / (akk - beta);
+        const double tau = (beta - akk) / beta;
+        const double v = lane > k ? x * scale : (lane == k ? 1.0 : 0.0);
+        sV[w][k][lane] = v;
+        if (lane == 0) sTau[w][k] = tau;
+        P[k] = lane == k ? beta : (lane < k ? x : 0.0);
+        // apply H_k to the eliminated columns not yet triangularised
+#pragma unroll
+        for (int j = 0; j < QR_MAXDE; j++) {
+            if (j > k && j < de) {
+                const double y = lane < M ? P[j] : 0.0;
+                double s = wave_sum(lane > k ? v * y : 0.0) + __shfl(y, k, 64);
+                s *= tau;
+                if (lane == k) P[j] = y - s;
+                else if (lane > k) P[j] = y - s * v;
+            }
+        }
+    }
+    const bool any_bad_panel = __any(isnan(bad));
+    if (singular || any_bad_panel) {
+        if (lane == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
+        return;
+    }
+    // eliminated columns of F: [U; 0]
+    for (int k = 0; k < de; k++)
+        if (lane < M) a.F[off + (int64_t)(ce + k) * M + lane] = P[k];
+    // sV / sTau are private to this wave: a wavefront-scope fence and wave barrier order its LDS writes before
+    // the reads below (no workgroup barrier: the other waves of the block may have finished already)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // retained columns and the residual: lane-per-column through registers
+    for (int j = lane; j <= ce; j += 64) {
+        const bool res = j == ce;
+        double col[MB];
+#pragma unroll
+        for (int i = 0; i < MB; i++) {
+            double x = 0.0;
+            if (i < me) x = res ? a.Rr[rq + i] : a.Rw[off + (int64_t)j * M + i];
+            bad += x - x;
+            col[i] = x;
+        }
+#pragma unroll
+        for (int k = 0; k < QR_MAXDE; k++) {
+            if (k >= de) break;
+            const double tau = sTau[w][k];
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+                if (i >= k && i < M) s += (i == k ? 1.0 : sV[w][k][i]) * col[i];
+            s *= tau;
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+                if (i >= k && i < M) col[i] -= s * (i == k ? 1.0 : sV[w][k][i]);
+        }
+        if (res) {
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+                if (i < M) a.Fr[rq + i] = col[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+                if (i < M) a.F[off + (int64_t)j * M + i] = col[i];
+        }
+    }
+    if (__any(isnan(bad)) && lane == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
+}
+
+__global__ void k_qr_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+__global__ void k_qr_finalize(const unsigned long long *flag, int64_t *out) {
+    const unsigned long long f = *flag;
+    *out = f == (unsigned long long)NO_FAILURE ? 0 : (int64_t)f;
+}
+
+}  // namespace
+
+struct bls_qr {
+    int64_t nr = 0, nrb = 0, neb = 0, ntables = 0, raw_len = 0, res_len = 0;
+    std::vector<int32_t> bucket_tables[3];
+    DevArray<int32_t> d_bucket[3];
+    DevArray<int32_t> d_nrows, d_ncols, d_edims;
+    DevArray<int64_t> d_raw_off, d_res_off, d_boff_e;
+    DevArray<unsigned long long> d_flag;
+    DevArray<int64_t> d_failed_out;
+    Timing timing;
+};
+
+namespace {
+std::vector<int64_t> sub1(const int64_t *p, int64_t n) {
+    std::vector<int64_t> v(p, p + n);
+    for (auto &x : v) x -= 1;
+    return v;
+}
+
+int reduce_qr_async(bls_qr *h, double *F, double *Fr, const double *Rw, const double *Rr, const double *scaling_e,
+                    double lambda, int64_t *d_failed, hipStream_t s) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (!(std::isfinite(lambda) && lambda >= 0))
+        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
+    if ((h->raw_len && (!F || !Rw)) || (h->res_len && (!Fr || !Rr)) || (h->neb && !scaling_e) || !d_failed)
+        return fail(BLS_ERR_ARG, "reduce_qr: a required array is NULL");
+    h->timing.mark(0, s);
+    k_qr_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
+    QRArgs a{};
+    a.nrows = h->d_nrows.ptr;
+    a.ncols = h->d_ncols.ptr;
+    a.e_dims = h->d_edims.ptr;
+    a.raw_off = h->d_raw_off.ptr;
+    a.res_off = h->d_res_off.ptr;
+    a.boff_e = h->d_boff_e.ptr;
+    a.Rw = Rw;
+    a.Rr = Rr;
+    a.scaling_e = scaling_e;
+    a.lambda = lambda;
+    a.F = F;
+    a.Fr = Fr;
+    a.flag = h->d_flag.ptr;
+    for (int b = 0; b < 3; b++) {
+        a.tables = h->d_bucket[b].ptr;
+        a.ntab = (int64_t)h->bucket_tables[b].size();
+        if (!a.ntab) continue;
+        const unsigned grid = (unsigned)((a.ntab + QR_WAVES - 1) / QR_WAVES);
+        if (b == 0) k_qr_tables<16><<<grid, 64 * QR_WAVES, 0, s>>>(a);
+        else if (b == 1) k_qr_tables<32><<<grid, 64 * QR_WAVES, 0, s>>>(a);
+        else k_qr_tables<64><<<grid, 64 * QR_WAVES, 0, s>>>(a);
+    }
+    h->timing.mark(1, s);
+    k_qr_finalize<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, const int64_t *e_dims,
+                             const int64_t *boffsets_e, int64_t ntables, const int64_t *table_nrows,
+                             const int64_t *table_ncols, const int64_t *raw_offsets, const int64_t *residual_offsets,
+                             bls_qr **out) {
+    if (!out || nr < 0 || nrblocks < 0 || neblocks < 0 || ntables < neblocks) return fail(BLS_ERR_ARG, "bls_qr_create: bad arguments");
+    *out = nullptr;
+    if (ntables > INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "more than 2^31 tables");
+    auto *h = new bls_qr();
+    h->nr = nr;
+    h->nrb = nrblocks;
+    h->neb = neblocks;
+    h->ntables = ntables;
+    std::vector<int32_t> nrows(ntables), ncols(ntables), edims(ntables, 0);
+    std::vector<int64_t> boff(ntables, 0);
+    for (int64_t t = 0; t < ntables; t++) {
+        nrows[t] = (int32_t)table_nrows[t];
+        ncols[t] = (int32_t)(table_ncols[t] - 1);
+        if (t < neblocks) {
+            edims[t] = (int32_t)e_dims[t];
+            boff[t] = boffsets_e[t] - 1;
+            if (e_dims[t] < 1 || e_dims[t] > QR_MAXDE) {
+                delete h;
+                return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension outside 1..8 on the device QR path");
+            }
+        }
+        if (table_nrows[t] > 64 && t < neblocks) {
+            delete h;
+            return fail(BLS_ERR_UNSUPPORTED, "table " + std::to_string(t + 1) + " has " + std::to_string(table_nrows[t]) +
+                                                 " rows; the device QR path supports up to 64");
+        }
+        const int b = t >= neblocks ? 0 : (table_nrows[t] <= 16 ? 0 : table_nrows[t] <= 32 ? 1 : 2);
+        h->bucket_tables[b].push_back((int32_t)t);
+    }
+    h->raw_len = raw_offsets[ntables] - 1;
+    h->res_len = residual_offsets[ntables] - 1;
+    int rc;
+    if ((rc = h->d_nrows.upload(nrows)) || (rc = h->d_ncols.upload(ncols)) || (rc = h->d_edims.upload(edims)) ||
+        (rc = h->d_raw_off.upload(sub1(raw_offsets, ntables + 1))) ||
+        (rc = h->d_res_off.upload(sub1(residual_offsets, ntables + 1))) || (rc = h->d_boff_e.upload(boff)) ||
+        (rc = h->d_flag.alloc(1)) || (rc = h->d_failed_out.alloc(1))) {
+        delete h;
+        return rc;
+    }
+    for (int b = 0; b < 3; b++)
+        if ((rc = h->d_bucket[b].upload(h->bucket_tables[b]))) { delete h; return rc; }
+    *out = h;
+    return BLS_OK;
+}
+
+extern "C" void bls_qr_destroy(bls_qr *h) { delete h; }
+
+extern "C" int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
+                                 const double *scaling_e, double lambda, const double *s, int64_t *failed_block,
+                                 void *stream) {
+    if (s) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, h ? h->d_failed_out.ptr : nullptr, st);
+    if (rc) return rc;
+    int64_t fb = 0;
+    BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    BLS_HIP(hipStreamSynchronize(st));
+    if (h->timing.enabled) (void)hipEventElapsedTime(&h->timing.ms[0], h->timing.ev[0], h->timing.ev[1]);
+    if (failed_block) *failed_block = fb;
+    return fb ? BLS_FAILED : BLS_OK;
+}
+
+extern "C" int bls_reduce_qr_f64_async(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
+                                       const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
+                                       void *stream) {
+    if (s) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
+    return reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, d_failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_qr_set_timing(bls_qr *h, int enable) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (enable) return h->timing.enable();
+    h->timing.enabled = false;
+    return BLS_OK;
+}
+
+extern "C" int bls_qr_kernel_time(bls_qr *h, float *ms) {
+    if (!h || !ms) return fail(BLS_ERR_ARG, "bad arguments");
+    *ms = h->timing.ms[0];
+    return BLS_OK;
+}

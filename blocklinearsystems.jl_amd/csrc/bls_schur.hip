@@ -9,10 +9,11 @@
 // col_chunks: a workgroup owns an "item" -- a run of consecutive block rows ("slots") of one block column r2 of S
 // -- and each lane owns one (slot, row) of that column, holding the d_r2 entries of that row in registers. The
 // item's landmarks (the couplings of r2 in ascending eliminated block, the reference's order) are streamed in
-// batches: their E_inv, z and coupling blocks are staged in LDS with coalesced loads, then every lane applies,
-// in order, the landmarks that touch its slot -- W_c1[:,row] = E_inv C_c1[:,row] recomputed in registers and
-// S[row, col] -= W_c1[:,row] . C_c2[:,col]. Each S entry therefore receives its updates in the reference's
-// sequence from one lane: no float atomics, and the result is bitwise reproducible run to run.
+// batches: their E_inv, z and C_c2 blocks are staged in LDS with coalesced loads, then every lane applies, in
+// order, the landmarks that touch its slot -- W_c1[:,row] = E_inv C_c1[:,row] formed in registers from its own
+// column of C_c1 and S[row, col] -= W_c1[:,row] . C_c2[:,col]. Each S entry receives its updates from one lane
+// (or from a few lanes whose partial sums are merged in a fixed order): no float atomics, and the result is
+// bitwise reproducible run to run.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,17 +25,17 @@ using namespace bls;
 namespace {
 
 constexpr int P2_THREADS = 256;
-constexpr int P2_MAXE = 64;      // entries (landmarks) per staged batch
-constexpr int P2_LDS = 4096;     // doubles of staged data per batch (32 KB)
-constexpr int MAXDR = 16;        // largest retained block handled
+constexpr int P2_MAXE = 64;        // entries (landmarks) per staged batch
+constexpr int P2_BASE_LANES = 96;  // (slot, row) lanes of one item before its heavy slots are split into chunks
+constexpr int MAXDR = 16;          // largest retained block handled
 
 struct P2Item {
     int64_t base;        // S index of (row 0, col 0) of the column, before the slot's row base
     int64_t colstride;   // distance between scalar columns of the block column
-    int32_t r2, dr2, ro2;
-    int32_t slot_lo, nslots, ntasks;
-    int32_t has_diag, uniform_dr;
     int64_t entry_lo, entry_hi;
+    int32_t r2, dr2, ro2;
+    int32_t slot_lo, nslots, ntasks, task_lo;
+    int32_t has_diag, uniform_dr;
 };
 
 struct P2Entry {
@@ -42,10 +43,18 @@ struct P2Entry {
     uint64_t mask;
 };
 
+// A lane's share of an item: one row of one slot. A slot with many landmarks is split over `nchunk` lane groups
+// (tasks slot_base + chunk * d_r1 + row); group `chunk` takes that share of the slot's landmarks in every batch,
+// and the groups' partial sums are merged in chunk order at the end.
+struct P2Task {
+    int16_t slot, row, chunk, nchunk;
+};
+
 struct P2Args {
     const P2Item *items;
     int64_t nitems;
     const P2Entry *entries;
+    const P2Task *tasks;
     const int64_t *slot_rowbase;
     const int32_t *slot_dr;
     const int64_t *eoffsets, *boffsets_e, *coffsets;
@@ -190,38 +199,63 @@ __global__ void k_finalize_flag(const unsigned long long *flag, int64_t *out) {
 // ---------------------------------------------------------------------------------------------------------
 // Phase 2
 //
-// Software pipeline per workgroup: while batch b is applied from LDS, the coupling data of batch b+1 is in flight
-// into registers and the landmark entries of batch b+2 into wave 0's lanes. Batches are formed in-kernel with the
-// host's greedy rule (at most P2_MAXE entries and P2_LDS staged doubles), from a ballot over a wave scan of the
-// entries' staged sizes.
+// A workgroup streams its item's landmark entries in batches of up to P2_MAXE. Per batch, each landmark's shared
+// data [E_inv | z | C_c2] (d_e^2 + d_e + d_e d_r2 doubles) is staged in LDS with coalesced loads. Every lane then
+// applies the batch's landmarks that touch its slot, in order, loading its own column C_c1[:, row] (d_e
+// contiguous doubles) from global memory, P2_G landmarks at a time so that their loads are in flight together:
+//   w = E_inv C_c1[:, row]   (a column of the reference's W_data, src/schur_reduction.jl:149-160)
+//   S[row, c] -= w . C_c2[:, c]   and, on the diagonal slot, g[row] -= C_c2[:, row] . z
+// Software pipeline: while batch b is applied, the staged data of batch b+1 is in flight into registers and
+// wave 0 describes batch b+2 (descriptions triple-buffered, staging double-buffered: one barrier per batch).
+// A slot touched by many landmarks is split over `nchunk` lane groups; in every batch, group k takes the k-th of
+// nchunk equal contiguous shares of the slot's landmarks, so the groups stay balanced batch by batch. Partial
+// sums are merged in group order at the end: deterministic, no atomics.
 // ---------------------------------------------------------------------------------------------------------
-constexpr int P2_Q = P2_LDS / P2_THREADS;   // staged doubles per lane per batch
+constexpr int P2_STAGE = 2048;                 // doubles per staging buffer (16 KB)
+constexpr int P2_Q = P2_STAGE / P2_THREADS;    // staged doubles per lane per batch
+constexpr int P2_G = 4;                        // landmarks whose C_c1 columns a lane loads together
+constexpr int P2_WORDS = P2_STAGE / 64;
 
 struct P2Meta {
     int nb;
-    int off[P2_MAXE + 1];
-    int c1off[P2_MAXE], c2off[P2_MAXE], de[P2_MAXE];
+    int off[P2_MAXE + 1];       // staged offset of each entry's [E_inv | z | C_c2]
+    int de[P2_MAXE];
     unsigned long long mask[P2_MAXE];
     int64_t src_e[P2_MAXE], src_z[P2_MAXE], src_c1[P2_MAXE], src_c2[P2_MAXE];
+    unsigned long long slotbits[64];   // per slot of the item: which entries of the batch touch it
+    // staged positions where an entry starts, one bit per position, with the number of starts before each
+    // 64-position word: the entry holding position p is found with one popcount
+    unsigned long long startbits[P2_WORDS];
+    int startcnt[P2_WORDS];
 };
 
-// Wave 0: load up to 64 entries starting at `start`.
-__device__ inline bool p2_load_entries(const P2Args &a, int64_t start, int64_t end, int lane, P2Entry &en) {
-    if (start + lane < end) {
-        en = a.entries[start + lane];
-        return true;
-    }
-    return false;
+// Load up to 64 entries starting at `start` (lane i takes entry start + i). The load is unconditional, from a
+// clamped index (the entry array ends with an unused record, so entry_lo is always a valid index), so that no
+// branch joins on its result and waiting for it never waits for loads issued after it.
+__device__ inline bool p2_load_entries(const P2Args &a, int64_t start, int64_t lo, int64_t end, int lane,
+                                       P2Entry &en) {
+    const bool valid = start + lane < end;
+    en = a.entries[valid ? start + lane : lo];
+    return valid;
 }
 
-// Wave 0: form the batch from the loaded entries and describe it in `m`; returns its entry count.
+__device__ inline int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(v, d, 64);
+        if (lane >= d) v += y;
+    }
+    return v;
+}
+
+// Wave 0: form the batch from the loaded entries (the longest prefix whose staged data fits P2_STAGE) and
+// describe it in `m`; returns its entry count.
 template <int DE>
-__device__ inline int p2_process(const P2Args &a, const P2Item &it, const int *slot_dr, P2Meta &m, bool valid,
-                                 const P2Entry &en, int lane) {
+__device__ inline int p2_process(const P2Args &a, const P2Item &it, P2Meta &m, bool valid, const P2Entry &en,
+                                 int lane) {
     int de = DE;
     if (DE == 0) de = valid ? a.e_dims[en.e] : 1;
     int64_t se = 0, sz = 0, sc1 = 0, sc2 = 0;
-    int len1 = 0;
     if (valid) {
         if (a.uniform_offsets) {
             se = (int64_t)en.e * (DE * DE);
@@ -234,28 +268,14 @@ __device__ inline int p2_process(const P2Args &a, const P2Item &it, const int *s
             sc1 = a.coffsets[en.c_first];
             sc2 = a.coffsets[en.c2];
         }
-        if (it.uniform_dr) {
-            len1 = __popcll(en.mask) * de * it.uniform_dr;
-        } else {
-            unsigned long long mm = en.mask;
-            while (mm) { const int sl = __builtin_ctzll(mm); mm &= mm - 1; len1 += de * slot_dr[sl]; }
-        }
     }
-    const int size = valid ? de * de + de + (it.has_diag ? 0 : de * it.dr2) + len1 : 0;
-    int incl = size;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
+    const int size = valid ? de * de + de + de * it.dr2 : 0;
+    const int incl = wave_incl_scan(size, lane);
     const int excl = incl - size;
-    const unsigned long long take = __ballot(valid && incl <= P2_LDS);
+    const unsigned long long take = __ballot(valid && incl <= P2_STAGE);
     const int nb = __popcll(take);
     if (lane < nb) {
         m.off[lane] = excl;
-        const int c1 = excl + de * de + de + (it.has_diag ? 0 : de * it.dr2);
-        m.c1off[lane] = c1;
-        m.c2off[lane] = it.has_diag ? c1 + len1 - de * it.dr2 : excl + de * de + de;
         m.de[lane] = de;
         m.mask[lane] = en.mask;
         m.src_e[lane] = se;
@@ -268,54 +288,89 @@ __device__ inline int p2_process(const P2Args &a, const P2Item &it, const int *s
         m.nb = nb;
         if (nb == 0) m.off[0] = 0;
     }
+    const unsigned long long mk = lane < nb ? en.mask : 0ull;
+    for (int sl = 0; sl < it.nslots; sl++) {
+        const unsigned long long b = __ballot((mk >> sl) & 1ull);
+        if (lane == 0) m.slotbits[sl] = b;
+    }
+    if (lane < P2_WORDS) m.startbits[lane] = 0ull;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nb) atomicOr(&m.startbits[excl >> 6], 1ull << (excl & 63));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int ps = lane < P2_WORDS ? __popcll(m.startbits[lane]) : 0;
+    const int ips = wave_incl_scan(ps, lane);
+    if (lane < P2_WORDS) m.startcnt[lane] = ips - ps;
     return nb;
 }
 
-// All lanes: issue the loads of a batch's staged data into registers.
+// Index of the entry holding staged position p (0 <= p < P2_STAGE): starts at or before p, less one. Past the
+// last entry it returns the last one.
+__device__ inline int seg_lookup(const P2Meta &m, int p) {
+    const int w = p >> 6;
+    const unsigned long long upto = (2ull << (p & 63)) - 1ull;   // bits 0..p%64 (all 64 when p%64 == 63)
+    return m.startcnt[w] + __popcll(m.startbits[w] & upto) - 1;
+}
+
+// All lanes: issue the loads of a batch's staged data into registers (addresses first, then the loads).
 template <int DE>
-__device__ inline void p2_issue(const P2Args &a, const P2Item &it, const P2Meta &m, int t, double *v) {
-    const int nb = m.nb;
-    const int L = m.off[nb];
+__device__ inline void p2_issue(const P2Args &a, const P2Meta &m, int t, double *v) {
+    const int L = m.off[m.nb];
+    const double *src[P2_Q];
 #pragma unroll
     for (int q = 0; q < P2_Q; q++) {
         const int p = t + q * P2_THREADS;
-        if (p < L) {
-            int lo = 0, hi = nb;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (m.off[mid] <= p) lo = mid; else hi = mid;
-            }
-            const int de = DE ? DE : m.de[lo];
-            int r = p - m.off[lo];
-            const double *src;
-            if (r < de * de) {
-                src = a.E_inv + m.src_e[lo] + r;
-            } else if ((r -= de * de) < de) {
-                src = a.z + m.src_z[lo] + r;
-            } else {
-                r -= de;
-                if (!it.has_diag && r < de * it.dr2) src = a.C + m.src_c2[lo] + r;
-                else src = a.C + m.src_c1[lo] + (it.has_diag ? r : r - de * it.dr2);
-            }
-            v[q] = *src;
-        }
+        const int j = seg_lookup(m, p);
+        const int de = DE ? DE : m.de[j];
+        const int r = p - m.off[j];
+        const int d2 = de * de, d3 = d2 + de;
+        const double *s1 = a.E_inv + m.src_e[j] + r;
+        const double *s2 = a.z + m.src_z[j] + (r - d2);
+        const double *s3 = a.C + m.src_c2[j] + (r - d3);
+        const double *sp = r < d2 ? s1 : (r < d3 ? s2 : s3);
+        src[q] = p < L ? sp : a.C;
     }
+#pragma unroll
+    for (int q = 0; q < P2_Q; q++) v[q] = *src[q];
 }
 
-__device__ inline void p2_store(const P2Meta &m, int t, const double *v, double *lds) {
+__device__ inline void p2_store(const P2Meta &m, int t, const double *v, double *stage) {
     const int L = m.off[m.nb];
 #pragma unroll
     for (int q = 0; q < P2_Q; q++) {
         const int p = t + q * P2_THREADS;
-        if (p < L) lds[p] = v[q];
+        if (p < L) stage[p] = v[q];
     }
 }
 
-template <int DE>
+// Position of the r-th (0-based) set bit of x; 64 when x has at most r set bits.
+__device__ inline int nth_set_bit(unsigned long long x, int r) {
+    if (r >= __popcll(x)) return 64;
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll((x >> pos) & ((1ull << w) - 1ull));
+        if (r >= c) {
+            r -= c;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+__device__ inline unsigned long long bits_below(int p) { return p >= 64 ? ~0ull : ((1ull << p) - 1ull); }
+
+template <int DE, int DRM>
 __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
-    __shared__ double lds[P2_LDS];
-    __shared__ P2Meta meta[2];
-    __shared__ int s_slot_dr[64], s_cum[65];
+    constexpr int MERGE = P2_THREADS * (DRM + 1);
+    // the update reads DE*DRM doubles of C_c2 whatever d_r2 is: pad each staging buffer by the largest such read
+    constexpr int STAGE = P2_STAGE + 8 * DRM;
+    constexpr int LDSN = 2 * STAGE > MERGE ? 2 * STAGE : MERGE;
+    constexpr int DEC = DE ? DE : 8;   // register width of a C_c1 column
+    __shared__ double lds[LDSN];
+    __shared__ P2Meta meta[3];
+    __shared__ int s_slot_dr[64];
 
     if (*a.flag != (unsigned long long)NO_FAILURE) return;   // phase 1 failed: S and g_S stay untouched
     const int64_t idx = xcd_remap(blockIdx.x, gridDim.x);
@@ -325,113 +380,153 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
     const int lane = t & 63;
     const bool wave0 = t < 64;
     if (t < it.nslots) s_slot_dr[t] = a.slot_dr[it.slot_lo + t];
-    __syncthreads();
-    if (t == 0) {
-        int c = 0;
-        for (int sl = 0; sl < it.nslots; sl++) { s_cum[sl] = c; c += s_slot_dr[sl]; }
-        s_cum[it.nslots] = c;
-    }
-    __syncthreads();
-    int my_slot = -1, my_row = 0;
+    int my_slot = -1, my_row = 0, chunk = 0, nchunk = 1;
     if (t < it.ntasks) {
-        int sl = 0;
-        while (s_cum[sl + 1] <= t) sl++;
-        my_slot = sl;
-        my_row = t - s_cum[sl];
+        const P2Task tk = a.tasks[it.task_lo + t];
+        my_slot = tk.slot;
+        my_row = tk.row;
+        chunk = tk.chunk;
+        nchunk = tk.nchunk;
     }
     const bool diag = it.has_diag && my_slot == it.nslots - 1;
     const int dr2 = it.dr2;
-    double acc[MAXDR];
+    double acc[DRM];
     double gacc = 0.0;
     int64_t rowaddr = 0;
+#pragma unroll
+    for (int c = 0; c < DRM; c++) acc[c] = 0.0;
     if (my_slot >= 0) {
         rowaddr = it.base + a.slot_rowbase[it.slot_lo + my_slot] + my_row;
+        if (chunk == 0) {
 #pragma unroll
-        for (int c = 0; c < MAXDR; c++)
-            if (c < dr2 && (!diag || c >= my_row)) acc[c] = a.S[rowaddr + c * it.colstride];
-        if (diag) gacc = a.b_r[it.ro2 + my_row] * (a.s_r ? a.s_r[it.ro2 + my_row] : 1.0);
+            for (int c = 0; c < DRM; c++)
+                if (c < dr2 && (!diag || c >= my_row)) acc[c] = a.S[rowaddr + c * it.colstride];
+            if (diag) gacc = a.b_r[it.ro2 + my_row] * (a.s_r ? a.s_r[it.ro2 + my_row] : 1.0);
+        }
     }
     const unsigned long long lowmask = my_slot > 0 ? ((1ull << my_slot) - 1ull) : 0ull;
     const int udr = it.uniform_dr;
 
     // prologue: describe the first two batches, stage the first
     int64_t start = it.entry_lo;
-    P2Entry en{};
-    bool ev = false;
-    if (wave0) {
-        ev = p2_load_entries(a, start, it.entry_hi, lane, en);
-        start += p2_process<DE>(a, it, s_slot_dr, meta[0], ev, en, lane);
-        ev = p2_load_entries(a, start, it.entry_hi, lane, en);
-        start += p2_process<DE>(a, it, s_slot_dr, meta[1], ev, en, lane);
-    }
+    P2Entry en;
+    bool ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);
+    if (wave0) start += p2_process<DE>(a, it, meta[0], ev, en, lane);
+    ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);
+    if (wave0) start += p2_process<DE>(a, it, meta[1], ev, en, lane);
     __syncthreads();
     double v[P2_Q];
     if (meta[0].nb > 0) {
-        p2_issue<DE>(a, it, meta[0], t, v);
+        p2_issue<DE>(a, meta[0], t, v);
         p2_store(meta[0], t, v, lds);
     }
     __syncthreads();
-    int cur = 0;
-    while (meta[cur].nb > 0) {
-        const int nxt = cur ^ 1;
-        const bool have_next = meta[nxt].nb > 0;
-        if (have_next) p2_issue<DE>(a, it, meta[nxt], t, v);
-        if (wave0) ev = p2_load_entries(a, start, it.entry_hi, lane, en);
-        const P2Meta &m = meta[cur];
+    int mc = 0, sb = 0;
+    while (meta[mc].nb > 0) {
+        const int mn = mc == 2 ? 0 : mc + 1;
+        const int mnn = mn == 2 ? 0 : mn + 1;
+        const bool have_next = meta[mn].nb > 0;
+        ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);   // wave 0 describes the batch after next
+        if (have_next) p2_issue<DE>(a, meta[mn], t, v);
+        const P2Meta &m = meta[mc];
+        const double *stg = lds + sb * STAGE;
+        unsigned long long bits = 0;
         if (my_slot >= 0) {
-            const int nb = m.nb;
-            unsigned long long bits = 0;
-            for (int j = 0; j < nb; j++) bits |= ((m.mask[j] >> my_slot) & 1ull) << j;
-            while (bits) {
-                const int j = __builtin_ctzll(bits);
-                bits &= bits - 1;
+            const unsigned long long sbits = m.slotbits[my_slot];
+            if (nchunk == 1) {
+                bits = sbits;
+            } else {
+                const int h = __popcll(sbits);
+                const int lo = chunk * h / nchunk, hi = (chunk + 1) * h / nchunk;
+                bits = sbits & ~bits_below(nth_set_bit(sbits, lo)) & bits_below(nth_set_bit(sbits, hi));
+            }
+        }
+        while (bits) {
+            int jj[P2_G];
+            const double *cp[P2_G];
+#pragma unroll
+            for (int gi = 0; gi < P2_G; gi++) {
+                const bool hv = bits != 0;
+                const int j = hv ? __builtin_ctzll(bits) : 0;
+                bits &= bits - 1ull;
                 const int de = DE ? DE : m.de[j];
-                const unsigned long long mk = m.mask[j];
                 int pre;
                 if (udr) {
-                    pre = __popcll(mk & lowmask) * de * udr;
+                    pre = __popcll(m.mask[j] & lowmask) * de * udr;
                 } else {
                     pre = 0;
-                    unsigned long long mm = mk & lowmask;
-                    while (mm) { const int sl = __builtin_ctzll(mm); mm &= mm - 1; pre += de * s_slot_dr[sl]; }
+                    unsigned long long mm = m.mask[j] & lowmask;
+                    while (mm) {
+                        const int sl = __builtin_ctzll(mm);
+                        mm &= mm - 1;
+                        pre += de * s_slot_dr[sl];
+                    }
                 }
-                const double *Ei = lds + m.off[j];
-                const double *zz = Ei + de * de;
-                const double *C2 = lds + m.c2off[j];
-                const double *C1 = lds + m.c1off[j] + pre + my_row * de;
+                jj[gi] = hv ? j : -1;
+                cp[gi] = hv ? a.C + m.src_c1[j] + pre + my_row * de : a.C;
+            }
+            double cv[P2_G][DEC];
+#pragma unroll
+            for (int gi = 0; gi < P2_G; gi++) {
                 if constexpr (DE > 0) {
-                    double w[DE];
+#pragma unroll
+                    for (int k = 0; k < DE; k++) cv[gi][k] = cp[gi][k];
+                } else {
+                    const int de = jj[gi] >= 0 ? m.de[jj[gi]] : 1;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) cv[gi][k] = cp[gi][k < de ? k : 0];
+                }
+            }
+#pragma unroll
+            for (int gi = 0; gi < P2_G; gi++) {
+                const int j = jj[gi];
+                if (j < 0) continue;
+                const double *Ei = stg + m.off[j];
+                if constexpr (DE > 0) {
+                    const double *zz = Ei + DE * DE;
+                    const double *C2 = zz + DE;
+                    double w[DE], c2[DE * DRM];
 #pragma unroll
                     for (int i = 0; i < DE; i++) {
                         double s = 0.0;
 #pragma unroll
-                        for (int k = 0; k < DE; k++) s += Ei[i + k * DE] * C1[k];
+                        for (int k = 0; k < DE; k++) s += Ei[i + k * DE] * cv[gi][k];
                         w[i] = s;
                     }
 #pragma unroll
-                    for (int c = 0; c < MAXDR; c++) {
-                        if (c < dr2 && (!diag || c >= my_row)) {
-                            double s = 0.0;
+                    for (int q = 0; q < DE * DRM; q++) c2[q] = C2[q];   // columns past d_r2 are read but never used
 #pragma unroll
-                            for (int i = 0; i < DE; i++) s += w[i] * C2[c * DE + i];
-                            acc[c] -= s;
-                        }
+                    for (int c = 0; c < DRM; c++) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int i = 0; i < DE; i++) s += w[i] * c2[c * DE + i];
+                        const bool ok = c < dr2 && (!diag || c >= my_row);
+                        acc[c] = ok ? acc[c] - s : acc[c];
                     }
                     if (diag) {
-                        double s = 0.0;
+                        double sg = 0.0;
 #pragma unroll
-                        for (int i = 0; i < DE; i++) s += C2[my_row * DE + i] * zz[i];
-                        gacc -= s;
+                        for (int c = 0; c < DRM; c++) {
+                            if (c == my_row) {
+#pragma unroll
+                                for (int i = 0; i < DE; i++) sg += c2[c * DE + i] * zz[i];
+                            }
+                        }
+                        gacc -= sg;
                     }
                 } else {
+                    const int de = m.de[j];
+                    const double *zz = Ei + de * de;
+                    const double *C2 = zz + de;
                     double w[8];
-                    for (int i = 0; i < de; i++) {
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
                         double s = 0.0;
-                        for (int k = 0; k < de; k++) s += Ei[i + k * de] * C1[k];
+                        for (int k = 0; k < de; k++) s += Ei[i + k * de] * cv[gi][k];
                         w[i] = s;
                     }
 #pragma unroll
-                    for (int c = 0; c < MAXDR; c++) {
+                    for (int c = 0; c < DRM; c++) {
                         if (c < dr2 && (!diag || c >= my_row)) {
                             double s = 0.0;
                             for (int i = 0; i < de; i++) s += w[i] * C2[c * de + i];
@@ -446,15 +541,29 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
                 }
             }
         }
+        if (have_next) p2_store(meta[mn], t, v, lds + (sb ^ 1) * STAGE);
+        if (wave0) start += p2_process<DE>(a, it, meta[mnn], ev, en, lane);
         __syncthreads();
-        if (have_next) p2_store(meta[nxt], t, v, lds);
-        if (wave0) start += p2_process<DE>(a, it, s_slot_dr, meta[cur], ev, en, lane);
-        __syncthreads();
-        cur = nxt;
+        mc = mn;
+        sb ^= 1;
     }
+    // merge the chunks of each (slot, row) in chunk order, then write S (upper triangle) and g
     if (my_slot >= 0) {
 #pragma unroll
-        for (int c = 0; c < MAXDR; c++)
+        for (int c = 0; c < DRM; c++) lds[t * (DRM + 1) + c] = acc[c];
+        lds[t * (DRM + 1) + DRM] = gacc;
+    }
+    __syncthreads();
+    if (my_slot >= 0 && chunk == 0) {
+        const int dr1 = s_slot_dr[my_slot];
+        for (int k = 1; k < nchunk; k++) {
+            const int sib = t + k * dr1;
+#pragma unroll
+            for (int c = 0; c < DRM; c++) acc[c] += lds[sib * (DRM + 1) + c];
+            gacc += lds[sib * (DRM + 1) + DRM];
+        }
+#pragma unroll
+        for (int c = 0; c < DRM; c++)
             if (c < dr2 && (!diag || c >= my_row)) a.S[rowaddr + c * it.colstride] = acc[c];
         if (diag) a.g[it.ro2 + my_row] = gacc;
     }
@@ -566,8 +675,10 @@ struct bls_schur {
     DevArray<int64_t> d_r_to_s, d_diag_inds, d_S_colptr, d_S_rowval;
     DevArray<P2Item> d_items;
     DevArray<P2Entry> d_entries;
+    DevArray<P2Task> d_tasks;
     DevArray<int32_t> d_slot_dr;
     int uniform_dr_all = 0;
+    int max_dr = 0;
     DevArray<int64_t> d_slot_rowbase;
     DevArray<unsigned long long> d_flag;
     DevArray<int64_t> d_failed_out;
@@ -589,19 +700,23 @@ int check_structure(const bls_schur *h) {
     return BLS_OK;
 }
 
-// Build the phase-2 work: items (slot runs of block columns), their landmark entries, and the staging batches.
-// `slots_of(r2)` gives the block rows of column r2 (ascending, last == r2), `rowbase_of(r2, k)` the row base of
-// its k-th block, and base/colstride the column's addressing.
+// Build the phase-2 work: items (slot runs of block columns), their landmark entries, and the lanes' tasks.
+// `slots_of(r2)` gives the block rows of column r2 (ascending, last == r2), `rowbase_of(r2, k, r1)` the row base of
+// its k-th block, and base_of the column's addressing. A slot touched by many landmarks gets several lanes per row,
+// each owning a contiguous range of the item's entries (balanced by hit count); their partial sums are merged in
+// range order, so every S entry still receives its updates in ascending eliminated block, deterministically.
 template <typename Slots, typename RowBase, typename Base>
 int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of) {
     std::vector<P2Item> items;
     std::vector<P2Entry> entries;
+    std::vector<P2Task> tasks;
     std::vector<int64_t> slot_rowbase;
     std::vector<int32_t> slot_dr;
     std::vector<int64_t> owner(h->ncoup);
     for (int64_t e = 0; e < h->neb; e++)
         for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++) owner[c] = e;
     std::vector<int64_t> col;
+    std::vector<int64_t> hits(64);   // landmarks touching each slot of the current item
     for (int64_t r2 = 0; r2 < h->nrb; r2++) {
         col.clear();
         slots_of(r2, col);
@@ -612,10 +727,10 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
         base_of(r2, base, colstride);
         size_t k0 = 0;
         while (k0 < col.size()) {
-            // slot run: at most 64 slots and 256 (slot, row) lanes
             size_t k1 = k0;
             int rows = 0;
-            while (k1 < col.size() && k1 - k0 < 64 && rows + h->r_dims[col[k1]] <= P2_THREADS) rows += (int)h->r_dims[col[k1++]];
+            while (k1 < col.size() && k1 - k0 < 64 && rows + h->r_dims[col[k1]] <= P2_BASE_LANES)
+                rows += (int)h->r_dims[col[k1++]];
             P2Item it{};
             it.base = base;
             it.colstride = colstride;
@@ -624,7 +739,6 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
             it.ro2 = (int32_t)h->boffsets_r[r2];
             it.slot_lo = (int32_t)slot_dr.size();
             it.nslots = (int32_t)(k1 - k0);
-            it.ntasks = rows;
             it.has_diag = k1 == col.size();
             int udr = (int)h->r_dims[col[k0]];
             for (size_t k = k0; k < k1; k++) {
@@ -635,6 +749,7 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
             it.uniform_dr = udr;
             const int64_t r1_lo = col[k0], r1_hi = col[k1 - 1];
             it.entry_lo = (int64_t)entries.size();
+            std::fill(hits.begin(), hits.end(), 0);
             for (int64_t kk = h->retained_ptr[r2]; kk < h->retained_ptr[r2 + 1]; kk++) {
                 const int64_t c2 = h->retained_entry[kk];
                 const int64_t e = owner[c2];
@@ -646,18 +761,14 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
                 if (ca >= cb) continue;
                 unsigned long long mask = 0;
                 size_t k = k0;
-                int len1 = 0;
                 for (int64_t c = ca; c < cb; c++) {
                     while (k < k1 && col[k] < er[c]) k++;
                     if (k == k1 || col[k] != er[c])
                         return fail(BLS_ERR_ARG, "The sparse pattern has no block (" + std::to_string(er[c] + 1) + ", " +
                                                      std::to_string(r2 + 1) + ")");
                     mask |= 1ull << (k - k0);
-                    len1 += (int)(h->e_dims[e] * h->r_dims[er[c]]);
+                    hits[k - k0]++;
                 }
-                const int de = (int)h->e_dims[e];
-                const int size = de * de + de + (it.has_diag ? 0 : de * dr2) + len1;
-                if (size > P2_LDS) return fail(BLS_ERR_UNSUPPORTED, "landmark data exceeds the staging buffer");
                 P2Entry en{};
                 en.e = (int32_t)e;
                 en.c_first = (int32_t)ca;
@@ -666,17 +777,52 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
                 entries.push_back(en);
             }
             it.entry_hi = (int64_t)entries.size();
+            // chunks per slot: greedily give another lane group to the slot with the most hits per chunk
+            const int ns = it.nslots;
+            std::vector<int> nch(ns, 1);
+            int lanes = rows;
+            while (true) {
+                int best = -1;
+                double bw = 0.0;
+                for (int sl = 0; sl < ns; sl++) {
+                    const double w = (double)hits[sl] / nch[sl];
+                    if (w > bw) { bw = w; best = sl; }
+                }
+                if (best < 0 || bw < 2.0) break;
+                if (lanes + slot_dr[it.slot_lo + best] > P2_THREADS || nch[best] >= 32767) break;
+                nch[best]++;
+                lanes += slot_dr[it.slot_lo + best];
+            }
+            it.task_lo = (int32_t)tasks.size();
+            for (int sl = 0; sl < ns; sl++) {
+                const int dr1 = slot_dr[it.slot_lo + sl];
+                for (int ch = 0; ch < nch[sl]; ch++) {
+                    for (int row = 0; row < dr1; row++) {
+                        P2Task tk{};
+                        tk.slot = (int16_t)sl;
+                        tk.row = (int16_t)row;
+                        tk.chunk = (int16_t)ch;
+                        tk.nchunk = (int16_t)nch[sl];
+                        tasks.push_back(tk);
+                    }
+                }
+            }
+            it.ntasks = (int32_t)(tasks.size() - it.task_lo);
             items.push_back(it);
             k0 = k1;
         }
     }
     if (entries.size() > (size_t)INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "too many phase-2 entries");
+    const int64_t nentries = (int64_t)entries.size();
+    // one unused record past the end: the kernel reads entries[entry_lo] unconditionally, also for an item without
+    // entries (a block column no landmark touches), whose entry_lo may equal the entry count
+    entries.push_back(P2Entry{});
     int rc;
-    if ((rc = h->d_items.upload(items)) || (rc = h->d_entries.upload(entries)) ||
+    if ((rc = h->d_items.upload(items)) || (rc = h->d_entries.upload(entries)) || (rc = h->d_tasks.upload(tasks)) ||
         (rc = h->d_slot_rowbase.upload(slot_rowbase)) || (rc = h->d_slot_dr.upload(slot_dr)))
         return rc;
     h->nitems = (int64_t)items.size();
-    h->nentries = (int64_t)entries.size();
+    h->nentries = nentries;
     return BLS_OK;
 }
 
@@ -707,6 +853,21 @@ int phase1(bls_schur *h, const double *E, const double *b_e, const double *sc, d
     return BLS_OK;
 }
 
+template <int DRM> void launch_phase2_dr(int de, unsigned grid, hipStream_t s, const P2Args &a) {
+    switch (de) {
+    case 1: k_phase2<1, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
+    case 2: k_phase2<2, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
+    case 3: k_phase2<3, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
+    default: k_phase2<0, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
+    }
+}
+
+void launch_phase2(int de, int max_dr, unsigned grid, hipStream_t s, const P2Args &a) {
+    if (max_dr <= 6) launch_phase2_dr<6>(de, grid, s, a);
+    else if (max_dr <= 9) launch_phase2_dr<9>(de, grid, s, a);
+    else launch_phase2_dr<16>(de, grid, s, a);
+}
+
 int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double *b_r, const double *b_e,
                  const double *scaling_e, double lambda, const double *E, const double *C, double *E_inv, double *z,
                  const double *s_r, const double *s_e, int64_t *d_failed, hipStream_t s) {
@@ -728,6 +889,7 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.items = h->d_items.ptr;
     a.nitems = h->nitems;
     a.entries = h->d_entries.ptr;
+    a.tasks = h->d_tasks.ptr;
     a.slot_rowbase = h->d_slot_rowbase.ptr;
     a.slot_dr = h->d_slot_dr.ptr;
     a.eoffsets = h->d_eoffsets.ptr;
@@ -746,12 +908,7 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
     if (h->nitems > 0) {
         const unsigned grid = (unsigned)(8 * ((h->nitems + 7) / 8));
-        switch (h->uniform_de) {
-        case 1: k_phase2<1><<<grid, P2_THREADS, 0, s>>>(a); break;
-        case 2: k_phase2<2><<<grid, P2_THREADS, 0, s>>>(a); break;
-        case 3: k_phase2<3><<<grid, P2_THREADS, 0, s>>>(a); break;
-        default: k_phase2<0><<<grid, P2_THREADS, 0, s>>>(a); break;
-        }
+        launch_phase2(h->uniform_de, h->max_dr, grid, s, a);
     }
     h->timing.mark(3, s);
     k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
@@ -820,6 +977,7 @@ extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, 
     }
     if (h->uniform_de > 3) h->uniform_de = 0;
     h->uniform_dr_all = nrblocks ? (int)h->r_dims[0] : 0;
+    for (int64_t r = 0; r < nrblocks; r++) h->max_dr = std::max(h->max_dr, (int)h->r_dims[r]);
     for (int64_t r = 0; r < nrblocks; r++)
         if (h->r_dims[r] != h->uniform_dr_all) h->uniform_dr_all = 0;
     if ((rc = h->d_eoffsets.upload(h->eoffsets)) || (rc = h->d_boffsets_e.upload(h->boffsets_e)) ||

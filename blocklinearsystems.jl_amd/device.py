@@ -49,6 +49,17 @@ SIGNATURES = {
     "bls_copyto_csc_f64": (ct.c_int, [ct.c_int64, VP, VP, VP, VP]),
     "bls_scatter_map_create": (ct.c_int, [ct.c_int64, P64, ct.POINTER(VP)]),
     "bls_schur_set_timing": (ct.c_int, [VP, ct.c_int]),
+    "bls_pcg_create": (ct.c_int, [ct.c_int64, ct.c_int64, P64, P64, ct.c_int64, P64, ct.c_int64, P64, P64, P64, P64,
+                                  ct.c_int64, P64, ct.POINTER(VP)]),
+    "bls_pcg_destroy": (None, [VP]),
+    "bls_pcg_values_size": (ct.c_int, [VP, P64, P64]),
+    "bls_pcg_solve_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, ct.c_int64, VP, P64, ct.POINTER(ct.c_double), VP]),
+    "bls_qr_create": (ct.c_int, [ct.c_int64] * 3 + [P64] * 2 + [ct.c_int64] + [P64] * 4 + [ct.POINTER(VP)]),
+    "bls_qr_destroy": (None, [VP]),
+    "bls_reduce_qr_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, P64, VP]),
+    "bls_reduce_qr_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP]),
+    "bls_qr_set_timing": (ct.c_int, [VP, ct.c_int]),
+    "bls_qr_kernel_time": (ct.c_int, [VP, ct.POINTER(ct.c_float)]),
     "bls_schur_kernel_times": (ct.c_int, [VP, ct.POINTER(ct.c_float), ct.c_int]),
 }
 
@@ -239,3 +250,102 @@ def copyto_(dest, src, scatter_map, stream=None):
 def scatter_map_to_device(scatter_map):
     import torch
     return torch.as_tensor(np.ascontiguousarray(scatter_map, dtype=np.int64)).cuda()
+
+
+class QRHandle:
+    """A registered SchurJacobianStructure on the device (the C ABI's bls_qr)."""
+
+    def __init__(self, jst):
+        keep = []
+        st = jst.schur
+        h = VP()
+        _check(lib().bls_qr_create(st.nr, st.nrblocks, st.neblocks, _p64(st.e_dims, keep), _p64(st.boffsets_e, keep),
+                                   jst.ntables, _p64(jst.table_nrows, keep), _p64(jst.table_ncols, keep),
+                                   _p64(jst.raw_offsets, keep), _p64(jst.residual_offsets, keep), ct.byref(h)))
+        self.h = h
+        self.jst = jst
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_qr_destroy(self.h)
+            self.h = None
+
+    def set_timing(self, on=True):
+        _check(lib().bls_qr_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self):
+        v = ct.c_float(0)
+        _check(lib().bls_qr_kernel_time(self.h, ct.byref(v)))
+        return v.value
+
+
+class QRReductionWorkspace:
+    """Device counterpart of QRReductionWorkspace{Float64} (src/qr_reduction.jl:11-23): the factored tables, the
+    Float64 factored residual and the failure flag."""
+
+    def __init__(self, handle):
+        import torch
+        jst = handle.jst
+        self.structure = jst
+        self.factored = torch.empty(jst.raw_length(), dtype=torch.float64, device="cuda")
+        self.fres = torch.empty(jst.residual_length(), dtype=torch.float64, device="cuda")
+        self.d_failed = torch.zeros(1, dtype=torch.int64, device="cuda")
+        self.failed_block = 0
+
+
+def reduce_qr_(ws, handle, J_data, J_r, scaling_e, lam, s=None, stream=None, sync=True, check=True):
+    """reduce_qr!(ws, J, scaling_e, lambda, s) (src/qr_reduction.jl:162-187); J given as its raw tables and Float64
+    residual (SchurBlockJacobian's data and r)."""
+    jst = handle.jst
+    if ws.structure is not jst:
+        raise ValueError("The Jacobian and the workspace were built for different structures")
+    if scaling_e.numel() != jst.schur.ne:
+        raise ValueError(f"scaling_e has length {scaling_e.numel()}, expected {jst.schur.ne}")
+    if check:
+        _check_damping(lam, scaling_e, "scaling_e")
+    L = lib()
+    if sync:
+        fb = ct.c_int64(0)
+        rc = _check(L.bls_reduce_qr_f64(handle.h, _dp(ws.factored), _dp(ws.fres), _dp(J_data), _dp(J_r), _dp(scaling_e),
+                                        float(lam), _dp(s), ct.byref(fb), _stream(stream)))
+        ws.failed_block = int(fb.value)
+        return rc == BLS_OK
+    _check(L.bls_reduce_qr_f64_async(handle.h, _dp(ws.factored), _dp(ws.fres), _dp(J_data), _dp(J_r), _dp(scaling_e),
+                                     float(lam), _dp(s), _dp(ws.d_failed), _stream(stream)))
+    return None
+
+
+class PCGHandle:
+    """Block-Jacobi PCG on reduced systems in build_schur_pattern's layout, batched over independent problems
+    (prob_block_ptr: 1-based first retained block of each problem, plus nrblocks + 1)."""
+
+    def __init__(self, st, pattern, prob_block_ptr=None):
+        keep = []
+        pbp = np.array([1, st.nrblocks + 1], dtype=np.int64) if prob_block_ptr is None else np.asarray(prob_block_ptr, np.int64)
+        h = VP()
+        _check(lib().bls_pcg_create(st.nr, st.nrblocks, _p64(st.r_dims, keep), _p64(st.boffsets_r, keep), pattern.nnz,
+                                    _p64(pattern.S_colptr, keep), len(pattern.block_rowval), _p64(pattern.block_colptr, keep),
+                                    _p64(pattern.block_rowval, keep), _p64(pattern.block_row_offsets, keep),
+                                    _p64(pattern.col_heights, keep), len(pbp) - 1, _p64(pbp, keep), ct.byref(h)))
+        self.h = h
+        self.nprob = len(pbp) - 1
+        self.st = st
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_pcg_destroy(self.h)
+            self.h = None
+
+    def factor_size(self):
+        nv, nf = ct.c_int64(0), ct.c_int64(0)
+        _check(lib().bls_pcg_values_size(self.h, ct.byref(nv), ct.byref(nf)))
+        return int(nf.value)
+
+    def solve(self, S_nzval, g, x, tol=1e-8, maxit=10000, precond_blocks=None, stream=None):
+        """x = S^-1 g per problem to ||S x - g|| <= tol ||g||. Returns (all_converged, iters[nprob], relres[nprob])."""
+        iters = np.zeros(self.nprob, dtype=np.int64)
+        rel = np.zeros(self.nprob)
+        rc = _check(lib().bls_pcg_solve_f64(self.h, _dp(S_nzval), _dp(g), _dp(x), float(tol), int(maxit),
+                                            _dp(precond_blocks), iters.ctypes.data_as(P64),
+                                            rel.ctypes.data_as(ct.POINTER(ct.c_double)), _stream(stream)))
+        return rc == BLS_OK, iters, rel

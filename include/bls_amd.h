@@ -121,6 +121,52 @@ int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_ma
 int bls_schur_set_timing(bls_schur *h, int enable);
 int bls_schur_kernel_times(bls_schur *h, float *times, int n);
 
+/* ---- QR path (the Jacobian form) ---------------------------------------------------------------------- */
+
+/* Register a SchurJacobianStructure (src/schur_jacobian.jl:30-57): the eliminated blocks' sizes and offsets
+ * (e_dims, boffsets_e of its SchurStructure) and the table layout (table_nrows, table_ncols, raw_offsets,
+ * residual_offsets; ntables + 1 offsets). Eliminated tables must have at most 64 rows and d_e <= 8 on the
+ * device path (BLS_ERR_UNSUPPORTED otherwise). */
+int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, const int64_t *e_dims, const int64_t *boffsets_e,
+                  int64_t ntables, const int64_t *table_nrows, const int64_t *table_ncols, const int64_t *raw_offsets,
+                  const int64_t *residual_offsets, bls_qr **out);
+void bls_qr_destroy(bls_qr *h);
+/* reduce_qr!(ws, J, scaling_e, lambda, s) (src/qr_reduction.jl:162-187): factored = Q' [J_r | J_e] per table with
+ * the damping rows sqrt(lambda scaling_e) written, fres = Q' r in Float64, free tables copied; raw tables and r
+ * are read only. BLS_FAILED with the lowest failing eliminated block for a non-finite table or a numerically
+ * singular U (|beta| <= M eps ||col||). `s` (conditioning scale) must be NULL on the device path. */
+int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
+                      const double *scaling_e, double lambda, const double *s, int64_t *failed_block, void *stream);
+int bls_reduce_qr_f64_async(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
+                            const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
+                            void *stream);
+/* Kernel time of the last synchronous reduce_qr (HIP events, ms), when enabled. */
+int bls_qr_set_timing(bls_qr *h, int enable);
+int bls_qr_kernel_time(bls_qr *h, float *ms);
+
+/* ---- block-Jacobi PCG on the reduced system (no reference counterpart) -------------------------------- */
+
+typedef struct bls_pcg bls_pcg;
+/* A PCG solver for reduced systems in the layout of build_schur_pattern (S_colptr, block pattern; as for
+ * bls_schur_set_sparse), batched over `nprob` independent problems whose retained blocks are the 1-based ranges
+ * prob_block_ptr[p] .. prob_block_ptr[p+1]-1 (no block of S may couple two problems). The preconditioner is the
+ * reference's BlockJacobi (ext/KrylovExt.jl:88-95: factor_diagonal_blocks! / solve_diagonal_blocks!,
+ * src/diagonal_blocks.jl:6-61) on S's diagonal blocks. */
+int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dims, const int64_t *boffsets_r, int64_t nnz,
+                   const int64_t *S_colptr, int64_t nnz_blocks, const int64_t *block_colptr,
+                   const int64_t *block_rowval, const int64_t *block_row_offsets, const int64_t *col_heights,
+                   int64_t nprob, const int64_t *prob_block_ptr, bls_pcg **out);
+void bls_pcg_destroy(bls_pcg *h);
+/* Sizes of the handle's symmetric block-row copy of S and of the preconditioner blocks (sum d_r^2). */
+int bls_pcg_values_size(bls_pcg *h, int64_t *nvals, int64_t *nfactor);
+/* Solve S x = g (device arrays; S in the upper-block CSC of build_schur_pattern) per problem until the true
+ * residual ||S x - g|| / ||g|| <= tol (FP64), or maxit iterations. precond_blocks: NULL for the plain
+ * block-Jacobi preconditioner, else full d x d column-major SPD blocks (sum d_r^2 values) that replace S's
+ * diagonal blocks in it (the learned correction). Writes iters[p] (-1: preconditioner block not positive
+ * definite) and relres[p] (host arrays of nprob). BLS_OK when every problem converged, else BLS_FAILED. */
+int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol, int64_t maxit,
+                      const double *precond_blocks, int64_t *iters, double *relres, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

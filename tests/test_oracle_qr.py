@@ -1,0 +1,174 @@
+"""CPU tests pinning the oracle's QR elimination (reduce_qr!) and its reduced-problem products to what the
+mathematics fixes: U'U = E + lambda diag(scaling), U'B = C, U'q = b_e (test/qr_reduction.jl:20-58), reduced normal
+equations == the Schur reduction (test/schur_qr.jl:13-57), dense least squares (test/fixtures/QRFixtures.jl:164-169)."""
+import numpy as np
+import pytest
+
+import oracle
+
+
+def _small_jacobian(blsamd, seed=0, nland=40, ncam=6, mixed=True, kmax=4):
+    rng = np.random.default_rng(seed)
+    cam_dims = np.where(rng.random(ncam) < 0.5, 6, 9) if mixed else np.full(ncam, 6)
+    k = rng.integers(2, kmax + 1, size=nland)
+    ptr = np.concatenate([[0], np.cumsum(k)])
+    cams = np.concatenate([rng.choice(ncam, size=kk, replace=False) for kk in k])
+    return blsamd.generators.jacobian_from_tracks(rng, ncam, cam_dims.astype(np.int64), nland, k, ptr, cams, 0.1)
+
+
+def _dense_J(p):
+    """Dense J (rows of all tables' data rows, columns [x_r; x_e]) and r, from the raw tables."""
+    jst, st = p.jst, p.structure
+    nr, ne = st.nr, st.ne
+    rows, res = [], []
+    for t in range(jst.ntables):
+        M = jst.table_nrows[t]
+        top = jst.table_top[t]
+        me = M - top if t < st.neblocks else M
+        off = jst.raw_offsets[t] - 1
+        C = jst.table_ncols[t] - 1
+        T = p.data[off:off + M * C].reshape(C, M).T
+        for i in range(me):
+            row = np.zeros(nr + ne)
+            for u in range(jst.table_ptr[t] - 1, jst.table_ptr[t + 1] - 1):
+                r = jst.entry_retained[u] - 1
+                ro, dr = st.boffsets_r[r] - 1, st.r_dims[r]
+                c0 = jst.entry_col[u] - 1
+                row[ro:ro + dr] = T[i, c0:c0 + dr]
+            if t < st.neblocks:
+                de = st.e_dims[t]
+                eo = nr + st.boffsets_e[t] - 1
+                row[eo:eo + de] = T[i, C - de:C]
+            rows.append(row)
+            res.append(p.r[jst.residual_offsets[t] - 1 + i])
+    return np.array(rows), np.array(res)
+
+
+def _qr(p, lam, nthreads=1):
+    jst, st = p.jst, p.structure
+    from blsload import load
+    ws = load().structure.SchurReductionWorkspaceIndex(st, 1)
+    sargs = oracle.SchurArgs(st, ws)
+    jargs = oracle.JacobianArgs(jst)
+    F = np.zeros(jst.raw_length())
+    Fr = np.zeros(jst.residual_length())
+    ok, fb = oracle.reduce_qr(sargs, jargs, F, Fr, p.data, p.r, p.scaling_e, lam, nthreads=nthreads)
+    return ok, fb, F, Fr, sargs, jargs
+
+
+def test_qr_factors_match_normal_equations(blsamd):
+    """U'U = E + lambda diag(scaling_e), U'B = C (the coupling block), U'q = b_e, per eliminated block."""
+    p = _small_jacobian(blsamd, 1)
+    lam = 0.1
+    ok, fb, F, Fr, _, _ = _qr(p, lam)
+    assert ok and fb == 0
+    jst, st = p.jst, p.structure
+    J, r = _dense_J(p)
+    nr = st.nr
+    for e in range(st.neblocks):
+        de = st.e_dims[e]
+        M = jst.table_nrows[e]
+        C = jst.table_ncols[e] - 1
+        off = jst.raw_offsets[e] - 1
+        T = F[off:off + M * C].reshape(C, M).T
+        U = T[:de, C - de:C]
+        assert np.allclose(np.tril(U, -1), 0)
+        eo = nr + st.boffsets_e[e] - 1
+        Je = J[:, eo:eo + de]
+        E = Je.T @ Je + lam * np.diag(p.scaling_e[st.boffsets_e[e] - 1: st.boffsets_e[e] - 1 + de])
+        assert np.allclose(U.T @ U, E, rtol=1e-11, atol=1e-11 * np.abs(E).max())
+        B = T[:de, :C - de]
+        Cr = np.zeros((de, C - de))
+        for u in range(jst.table_ptr[e] - 1, jst.table_ptr[e + 1] - 1):
+            rr = jst.entry_retained[u] - 1
+            ro, dr = st.boffsets_r[rr] - 1, st.r_dims[rr]
+            c0 = jst.entry_col[u] - 1
+            Cr[:, c0:c0 + dr] = Je.T @ J[:, ro:ro + dr]
+        assert np.allclose(U.T @ B, Cr, rtol=1e-11, atol=1e-11 * max(1, np.abs(Cr).max()))
+        q = Fr[jst.residual_offsets[e] - 1: jst.residual_offsets[e] - 1 + de]
+        be = Je.T @ r
+        assert np.allclose(U.T @ q, be, rtol=1e-11, atol=1e-11 * max(1, np.abs(be).max()))
+
+
+def test_reduced_normal_equals_schur_and_solves_least_squares(blsamd):
+    """J~'J~ + lambda D_r and J~'r~ equal the Schur complement of the damped J'J (test/schur_qr.jl:13-57); the
+    full step equals the dense damped least-squares solution."""
+    p = _small_jacobian(blsamd, 2)
+    lam = 0.05
+    ok, fb, F, Fr, sargs, jargs = _qr(p, lam)
+    assert ok
+    jst, st = p.jst, p.structure
+    nr = st.nr
+    S = np.zeros(nr * nr)
+    g = np.zeros(nr)
+    oracle.reduced_normal(False, sargs, jargs, S, g, F, Fr)
+    S = S.reshape(nr, nr).T
+    S = np.triu(S) + np.triu(S, 1).T
+    J, r = _dense_J(p)
+    H = J.T @ J
+    b = J.T @ r
+    n = H.shape[0]
+    D = np.zeros(n)
+    D[nr:] = lam * p.scaling_e
+    Hd = H + np.diag(D)
+    Sch = Hd[:nr, :nr] - Hd[:nr, nr:] @ np.linalg.solve(Hd[nr:, nr:], Hd[nr:, :nr])
+    gs = b[:nr] - Hd[:nr, nr:] @ np.linalg.solve(Hd[nr:, nr:], b[nr:])
+    assert np.allclose(S, Sch, rtol=1e-10, atol=1e-10 * np.abs(Sch).max())
+    assert np.allclose(g, gs, rtol=1e-10, atol=1e-10 * np.abs(gs).max())
+    # full solve with retained damping lambda too, vs dense damped least squares
+    Sd = S + lam * np.eye(nr)
+    x_r = np.linalg.solve(Sd, g)
+    x_e = np.zeros(st.ne)
+    oracle.qr_backsubstitute(sargs, jargs, x_e, x_r, F, Fr)
+    x = np.concatenate([x_r, x_e])
+    x_ref = np.linalg.solve(H + lam * np.eye(n), b)
+    assert np.linalg.norm(x - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+    # the diagonal blocks and the products
+    Dblk = np.zeros(int((st.r_dims ** 2).sum()))
+    oracle.reduced_diagonal_blocks(sargs, jargs, Dblk, F)
+    o = 0
+    for rr in range(st.nrblocks):
+        ro, d = st.boffsets_r[rr] - 1, st.r_dims[rr]
+        assert np.allclose(Dblk[o:o + d * d].reshape(d, d).T, S[ro:ro + d, ro:ro + d], rtol=1e-10, atol=1e-10 * np.abs(S).max())
+        o += d * d
+    v = np.random.default_rng(0).standard_normal(nr)
+    y = np.zeros(jst.reduced_nrows())
+    oracle.reduced_mul(sargs, jargs, y, F, v)
+    gt = np.zeros(nr)
+    oracle.reduced_tmul(sargs, jargs, gt, F, y)
+    assert np.allclose(gt, S @ v, rtol=1e-10, atol=1e-10 * np.abs(S @ v).max())
+    rt = np.zeros(jst.reduced_nrows())
+    oracle.reduced_residual(jargs, rt, Fr)
+    gt2 = np.zeros(nr)
+    oracle.reduced_tmul(sargs, jargs, gt2, F, rt)
+    assert np.allclose(gt2, g, rtol=1e-12, atol=1e-12 * np.abs(g).max())
+
+
+def test_qr_failures_lowest_block(blsamd):
+    """test/qr_reduction.jl:60-92: an all-zero eliminated block at lambda = 0 is singular, a non-finite entry
+    fails the table; the lowest failing block is reported, whatever the threading."""
+    p = _small_jacobian(blsamd, 3)
+    jst, st = p.jst, p.structure
+    for e in (7, 11):   # zero the eliminated columns of tables 7 and 11 (1-based)
+        M = jst.table_nrows[e - 1]
+        C = jst.table_ncols[e - 1] - 1
+        off = jst.raw_offsets[e - 1] - 1
+        de = st.e_dims[e - 1]
+        p.data[off + (C - de) * M: off + C * M] = 0.0
+    off = jst.raw_offsets[20] - 1
+    p.data[off] = np.nan    # table 21: a NaN in a retained column
+    for nt in (1, 4):
+        ok, fb, *_ = _qr(p, 0.0, nthreads=nt)
+        assert not ok and fb == 7
+    ok, fb, *_ = _qr(p, 0.5)      # damping makes the zero blocks regular; the NaN still fails table 21
+    assert not ok and fb == 21
+
+
+def test_qr_re_reduction_exact(blsamd):
+    """test/qr_reduction.jl:116-142: raw tables are read only, so reducing twice gives identical results."""
+    p = _small_jacobian(blsamd, 4)
+    raw = p.data.copy()
+    a = _qr(p, 0.2)
+    b = _qr(p, 0.2)
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    assert np.array_equal(p.data, raw)
