@@ -33,9 +33,17 @@ struct P2Item {
     int64_t base;        // S index of (row 0, col 0) of the column, before the slot's row base
     int64_t colstride;   // distance between scalar columns of the block column
     int64_t entry_lo, entry_hi;
+    int64_t pbase;       // split items: this part's rows in the partial buffer
     int32_t r2, dr2, ro2;
     int32_t slot_lo, nslots, ntasks, task_lo;
     int32_t has_diag, uniform_dr;
+    int32_t part, nparts;
+};
+
+// A split item: its parts' row sums (nparts x rows x (d_r2 + 1), the last column g) are added in part order.
+struct P2Merge {
+    int64_t base, colstride, pbase;
+    int32_t slot_lo, nslots, rows, dr2, ro2, has_diag, nparts;
 };
 
 struct P2Entry {
@@ -56,12 +64,13 @@ struct P2Args {
     const P2Entry *entries;
     const P2Task *tasks;
     const int64_t *slot_rowbase;
-    const int32_t *slot_dr;
+    const int32_t *slot_dr, *slot_roff;
     const int64_t *eoffsets, *boffsets_e, *coffsets;
     const int32_t *e_dims;
     const double *E_inv, *z, *C;
     const double *b_r, *s_r;
     double *S, *g;
+    double *partial;
     const unsigned long long *flag;
     int uniform_offsets;   // one run of d_e == DE and one d_r: E/z/C offsets are e*DE^2, e*DE, c*cstride
     int64_t cstride;
@@ -208,13 +217,25 @@ __global__ void k_finalize_flag(const unsigned long long *flag, int64_t *out) {
 // Software pipeline: while batch b is applied, the staged data of batch b+1 is in flight into registers and
 // wave 0 describes batch b+2 (descriptions triple-buffered, staging double-buffered: one barrier per batch).
 // A slot touched by many landmarks is split over `nchunk` lane groups; in every batch, group k takes the k-th of
-// nchunk equal contiguous shares of the slot's landmarks, so the groups stay balanced batch by batch. Partial
-// sums are merged in group order at the end: deterministic, no atomics.
+// nchunk equal contiguous shares of the slot's landmarks, so the groups stay balanced batch by batch. An item
+// with many landmarks is split into `nparts` workgroups over consecutive landmark ranges, whose row sums go to a
+// partial buffer and are added in part order by k_phase2_merge. Every sum is taken in a fixed order:
+// deterministic, no atomics.
+//
+// With one landmark dimension DE (the common case) a staged record has a fixed size SZ, record j sits at j*SZ
+// and four lanes stage each record. Mixed landmark dimensions (DE == 0) pack records of varying size; a lane
+// finds the record holding a staged position from bit masks of the records' starts.
 // ---------------------------------------------------------------------------------------------------------
 constexpr int P2_STAGE = 2048;                 // doubles per staging buffer (16 KB)
-constexpr int P2_Q = P2_STAGE / P2_THREADS;    // staged doubles per lane per batch
+constexpr int P2_Q = P2_STAGE / P2_THREADS;    // staged doubles per lane per batch (mixed dimensions)
 constexpr int P2_G = 4;                        // landmarks whose C_c1 columns a lane loads together
 constexpr int P2_WORDS = P2_STAGE / 64;
+constexpr int P2_PART = 384;                   // landmarks per workgroup before an item is split into parts
+
+// staged doubles per lane per batch: four lanes per record of at most DE^2 + DE + DE*DRM doubles, or P2_Q
+template <int DE, int DRM> struct P2Width {
+    static constexpr int value = DE > 0 ? (DE * DE + DE + DE * DRM + 3) / 4 : P2_Q;
+};
 
 struct P2Meta {
     int nb;
@@ -223,8 +244,8 @@ struct P2Meta {
     unsigned long long mask[P2_MAXE];
     int64_t src_e[P2_MAXE], src_z[P2_MAXE], src_c1[P2_MAXE], src_c2[P2_MAXE];
     unsigned long long slotbits[64];   // per slot of the item: which entries of the batch touch it
-    // staged positions where an entry starts, one bit per position, with the number of starts before each
-    // 64-position word: the entry holding position p is found with one popcount
+    // mixed dimensions: staged positions where an entry starts, one bit per position, with the number of starts
+    // before each 64-position word: the entry holding position p is found with one popcount
     unsigned long long startbits[P2_WORDS];
     int startcnt[P2_WORDS];
 };
@@ -248,11 +269,12 @@ __device__ inline int wave_incl_scan(int v, int lane) {
     return v;
 }
 
-// Wave 0: form the batch from the loaded entries (the longest prefix whose staged data fits P2_STAGE) and
-// describe it in `m`; returns its entry count.
+// Wave 0: form the batch from the loaded entries and describe it in `m`; returns its entry count. With one
+// landmark dimension the batch holds up to `nbmax` records of SZ doubles; otherwise it is the longest prefix
+// whose records fit P2_STAGE.
 template <int DE>
 __device__ inline int p2_process(const P2Args &a, const P2Item &it, P2Meta &m, bool valid, const P2Entry &en,
-                                 int lane) {
+                                 int lane, int nbmax) {
     int de = DE;
     if (DE == 0) de = valid ? a.e_dims[en.e] : 1;
     int64_t se = 0, sz = 0, sc1 = 0, sc2 = 0;
@@ -269,11 +291,17 @@ __device__ inline int p2_process(const P2Args &a, const P2Item &it, P2Meta &m, b
             sc2 = a.coffsets[en.c2];
         }
     }
-    const int size = valid ? de * de + de + de * it.dr2 : 0;
-    const int incl = wave_incl_scan(size, lane);
-    const int excl = incl - size;
-    const unsigned long long take = __ballot(valid && incl <= P2_STAGE);
-    const int nb = __popcll(take);
+    const int size = de * de + de + de * it.dr2;
+    int nb, excl;
+    if constexpr (DE > 0) {
+        nb = __popcll(__ballot(valid && lane < nbmax));
+        excl = lane * size;
+    } else {
+        const int sz_l = valid ? size : 0;
+        const int incl = wave_incl_scan(sz_l, lane);
+        excl = incl - sz_l;
+        nb = __popcll(__ballot(valid && incl <= P2_STAGE));
+    }
     if (lane < nb) {
         m.off[lane] = excl;
         m.de[lane] = de;
@@ -282,7 +310,7 @@ __device__ inline int p2_process(const P2Args &a, const P2Item &it, P2Meta &m, b
         m.src_z[lane] = sz;
         m.src_c1[lane] = sc1;
         m.src_c2[lane] = sc2;
-        if (lane == nb - 1) m.off[nb] = incl;
+        if (lane == nb - 1) m.off[nb] = excl + size;
     }
     if (lane == 0) {
         m.nb = nb;
@@ -293,20 +321,22 @@ __device__ inline int p2_process(const P2Args &a, const P2Item &it, P2Meta &m, b
         const unsigned long long b = __ballot((mk >> sl) & 1ull);
         if (lane == 0) m.slotbits[sl] = b;
     }
-    if (lane < P2_WORDS) m.startbits[lane] = 0ull;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (lane < nb) atomicOr(&m.startbits[excl >> 6], 1ull << (excl & 63));
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const int ps = lane < P2_WORDS ? __popcll(m.startbits[lane]) : 0;
-    const int ips = wave_incl_scan(ps, lane);
-    if (lane < P2_WORDS) m.startcnt[lane] = ips - ps;
+    if constexpr (DE == 0) {
+        if (lane < P2_WORDS) m.startbits[lane] = 0ull;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane < nb) atomicOr(&m.startbits[excl >> 6], 1ull << (excl & 63));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int ps = lane < P2_WORDS ? __popcll(m.startbits[lane]) : 0;
+        const int ips = wave_incl_scan(ps, lane);
+        if (lane < P2_WORDS) m.startcnt[lane] = ips - ps;
+    }
     return nb;
 }
 
-// Index of the entry holding staged position p (0 <= p < P2_STAGE): starts at or before p, less one. Past the
-// last entry it returns the last one.
+// Mixed dimensions: index of the entry holding staged position p (0 <= p < P2_STAGE): starts at or before p,
+// less one. Past the last entry it returns the last one.
 __device__ inline int seg_lookup(const P2Meta &m, int p) {
     const int w = p >> 6;
     const unsigned long long upto = (2ull << (p & 63)) - 1ull;   // bits 0..p%64 (all 64 when p%64 == 63)
@@ -314,33 +344,64 @@ __device__ inline int seg_lookup(const P2Meta &m, int p) {
 }
 
 // All lanes: issue the loads of a batch's staged data into registers (addresses first, then the loads).
-template <int DE>
-__device__ inline void p2_issue(const P2Args &a, const P2Meta &m, int t, double *v) {
-    const int L = m.off[m.nb];
-    const double *src[P2_Q];
+template <int DE, int DRM>
+__device__ inline void p2_issue(const P2Args &a, const P2Meta &m, int t, int dr2, double *v) {
+    constexpr int QN = P2Width<DE, DRM>::value;
+    const double *src[QN];
+    if constexpr (DE > 0) {
+        // lanes 4j..4j+3 stage record j: lane part takes doubles part, part + 4, ...
+        const int size = DE * DE + DE + DE * dr2;
+        const int j = t >> 2, part = t & 3;
+        const bool has = j < m.nb;
+        const int jj = has ? j : 0;
+        const int64_t se = m.src_e[jj], sz = m.src_z[jj], sc = m.src_c2[jj];
 #pragma unroll
-    for (int q = 0; q < P2_Q; q++) {
-        const int p = t + q * P2_THREADS;
-        const int j = seg_lookup(m, p);
-        const int de = DE ? DE : m.de[j];
-        const int r = p - m.off[j];
-        const int d2 = de * de, d3 = d2 + de;
-        const double *s1 = a.E_inv + m.src_e[j] + r;
-        const double *s2 = a.z + m.src_z[j] + (r - d2);
-        const double *s3 = a.C + m.src_c2[j] + (r - d3);
-        const double *sp = r < d2 ? s1 : (r < d3 ? s2 : s3);
-        src[q] = p < L ? sp : a.C;
+        for (int k = 0; k < QN; k++) {
+            const int r = part + 4 * k;
+            const double *sp = r < DE * DE ? a.E_inv + se + r
+                                           : (r < DE * DE + DE ? a.z + sz + (r - DE * DE) : a.C + sc + (r - DE * DE - DE));
+            src[k] = has && r < size ? sp : a.C;
+        }
+    } else {
+        const int L = m.off[m.nb];
+#pragma unroll
+        for (int q = 0; q < QN; q++) {
+            const int p = t + q * P2_THREADS;
+            const int j = seg_lookup(m, p);
+            const int de = m.de[j];
+            const int r = p - m.off[j];
+            const int d2 = de * de, d3 = d2 + de;
+            const double *s1 = a.E_inv + m.src_e[j] + r;
+            const double *s2 = a.z + m.src_z[j] + (r - d2);
+            const double *s3 = a.C + m.src_c2[j] + (r - d3);
+            const double *sp = r < d2 ? s1 : (r < d3 ? s2 : s3);
+            src[q] = p < L ? sp : a.C;
+        }
     }
 #pragma unroll
-    for (int q = 0; q < P2_Q; q++) v[q] = *src[q];
+    for (int q = 0; q < QN; q++) v[q] = *src[q];
 }
 
-__device__ inline void p2_store(const P2Meta &m, int t, const double *v, double *stage) {
-    const int L = m.off[m.nb];
+template <int DE, int DRM>
+__device__ inline void p2_store(const P2Meta &m, int t, int dr2, const double *v, double *stage) {
+    constexpr int QN = P2Width<DE, DRM>::value;
+    if constexpr (DE > 0) {
+        const int size = DE * DE + DE + DE * dr2;
+        const int j = t >> 2, part = t & 3;
+        if (j < m.nb) {
 #pragma unroll
-    for (int q = 0; q < P2_Q; q++) {
-        const int p = t + q * P2_THREADS;
-        if (p < L) stage[p] = v[q];
+            for (int k = 0; k < QN; k++) {
+                const int r = part + 4 * k;
+                if (r < size) stage[j * size + r] = v[k];
+            }
+        }
+    } else {
+        const int L = m.off[m.nb];
+#pragma unroll
+        for (int q = 0; q < QN; q++) {
+            const int p = t + q * P2_THREADS;
+            if (p < L) stage[p] = v[q];
+        }
     }
 }
 
@@ -361,19 +422,24 @@ __device__ inline int nth_set_bit(unsigned long long x, int r) {
 
 __device__ inline unsigned long long bits_below(int p) { return p >= 64 ? ~0ull : ((1ull << p) - 1ull); }
 
+// at least 3 waves per SIMD (3 workgroups per CU) where the row registers allow it
 template <int DE, int DRM>
-__global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
+__global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args a) {
     constexpr int MERGE = P2_THREADS * (DRM + 1);
-    // the update reads DE*DRM doubles of C_c2 whatever d_r2 is: pad each staging buffer by the largest such read
-    constexpr int STAGE = P2_STAGE + 8 * DRM;
+    // the update reads DE*DRM doubles of C_c2 (and a column at a row up to MAXDR) whatever d_r2 is: pad each
+    // staging buffer by the largest such read
+    constexpr int STAGE = P2_STAGE + 8 * (DRM > MAXDR ? DRM : MAXDR);
     constexpr int LDSN = 2 * STAGE > MERGE ? 2 * STAGE : MERGE;
     constexpr int DEC = DE ? DE : 8;   // register width of a C_c1 column
+    constexpr int QN = P2Width<DE, DRM>::value;
     __shared__ double lds[LDSN];
     __shared__ P2Meta meta[3];
     __shared__ int s_slot_dr[64];
 
     if (*a.flag != (unsigned long long)NO_FAILURE) return;   // phase 1 failed: S and g_S stay untouched
-    const int64_t idx = xcd_remap(blockIdx.x, gridDim.x);
+    // the dispatcher deals workgroups to the 8 XCDs round robin, so with items longest first every XCD gets a
+    // longest-first share of the work
+    const int64_t idx = blockIdx.x;
     if (idx >= a.nitems) return;
     const P2Item it = a.items[idx];
     const int t = threadIdx.x;
@@ -390,6 +456,8 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
     }
     const bool diag = it.has_diag && my_slot == it.nslots - 1;
     const int dr2 = it.dr2;
+    const int size = DE * DE + DE + DE * dr2;   // staged record (one landmark dimension)
+    const int nbmax = DE > 0 ? (P2_STAGE / size < P2_MAXE ? P2_STAGE / size : P2_MAXE) : P2_MAXE;
     double acc[DRM];
     double gacc = 0.0;
     int64_t rowaddr = 0;
@@ -397,7 +465,7 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
     for (int c = 0; c < DRM; c++) acc[c] = 0.0;
     if (my_slot >= 0) {
         rowaddr = it.base + a.slot_rowbase[it.slot_lo + my_slot] + my_row;
-        if (chunk == 0) {
+        if (chunk == 0 && it.part == 0) {
 #pragma unroll
             for (int c = 0; c < DRM; c++)
                 if (c < dr2 && (!diag || c >= my_row)) acc[c] = a.S[rowaddr + c * it.colstride];
@@ -411,14 +479,14 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
     int64_t start = it.entry_lo;
     P2Entry en;
     bool ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);
-    if (wave0) start += p2_process<DE>(a, it, meta[0], ev, en, lane);
+    if (wave0) start += p2_process<DE>(a, it, meta[0], ev, en, lane, nbmax);
     ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);
-    if (wave0) start += p2_process<DE>(a, it, meta[1], ev, en, lane);
+    if (wave0) start += p2_process<DE>(a, it, meta[1], ev, en, lane, nbmax);
     __syncthreads();
-    double v[P2_Q];
+    double v[QN];
     if (meta[0].nb > 0) {
-        p2_issue<DE>(a, meta[0], t, v);
-        p2_store(meta[0], t, v, lds);
+        p2_issue<DE, DRM>(a, meta[0], t, dr2, v);
+        p2_store<DE, DRM>(meta[0], t, dr2, v, lds);
     }
     __syncthreads();
     int mc = 0, sb = 0;
@@ -427,7 +495,7 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
         const int mnn = mn == 2 ? 0 : mn + 1;
         const bool have_next = meta[mn].nb > 0;
         ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);   // wave 0 describes the batch after next
-        if (have_next) p2_issue<DE>(a, meta[mn], t, v);
+        if (have_next) p2_issue<DE, DRM>(a, meta[mn], t, dr2, v);
         const P2Meta &m = meta[mc];
         const double *stg = lds + sb * STAGE;
         unsigned long long bits = 0;
@@ -481,11 +549,14 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
             for (int gi = 0; gi < P2_G; gi++) {
                 const int j = jj[gi];
                 if (j < 0) continue;
-                const double *Ei = stg + m.off[j];
                 if constexpr (DE > 0) {
+                    // every column is accumulated (a column past d_r2, or below the diagonal of the diagonal
+                    // block, collects values that are never written) and the g term is formed on every lane
+                    // (only the diagonal slot's is kept): no per-column selects
+                    const double *Ei = stg + j * size;
                     const double *zz = Ei + DE * DE;
                     const double *C2 = zz + DE;
-                    double w[DE], c2[DE * DRM];
+                    double w[DE];
 #pragma unroll
                     for (int i = 0; i < DE; i++) {
                         double s = 0.0;
@@ -494,27 +565,18 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
                         w[i] = s;
                     }
 #pragma unroll
-                    for (int q = 0; q < DE * DRM; q++) c2[q] = C2[q];   // columns past d_r2 are read but never used
-#pragma unroll
                     for (int c = 0; c < DRM; c++) {
                         double s = 0.0;
 #pragma unroll
-                        for (int i = 0; i < DE; i++) s += w[i] * c2[c * DE + i];
-                        const bool ok = c < dr2 && (!diag || c >= my_row);
-                        acc[c] = ok ? acc[c] - s : acc[c];
+                        for (int i = 0; i < DE; i++) s += w[i] * C2[c * DE + i];
+                        acc[c] -= s;
                     }
-                    if (diag) {
-                        double sg = 0.0;
+                    double sg = 0.0;
 #pragma unroll
-                        for (int c = 0; c < DRM; c++) {
-                            if (c == my_row) {
-#pragma unroll
-                                for (int i = 0; i < DE; i++) sg += c2[c * DE + i] * zz[i];
-                            }
-                        }
-                        gacc -= sg;
-                    }
+                    for (int i = 0; i < DE; i++) sg += C2[my_row * DE + i] * zz[i];
+                    gacc -= sg;
                 } else {
+                    const double *Ei = stg + m.off[j];
                     const int de = m.de[j];
                     const double *zz = Ei + de * de;
                     const double *C2 = zz + de;
@@ -541,13 +603,14 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
                 }
             }
         }
-        if (have_next) p2_store(meta[mn], t, v, lds + (sb ^ 1) * STAGE);
-        if (wave0) start += p2_process<DE>(a, it, meta[mnn], ev, en, lane);
+        if (have_next) p2_store<DE, DRM>(meta[mn], t, dr2, v, lds + (sb ^ 1) * STAGE);
+        if (wave0) start += p2_process<DE>(a, it, meta[mnn], ev, en, lane, nbmax);
         __syncthreads();
         mc = mn;
         sb ^= 1;
     }
-    // merge the chunks of each (slot, row) in chunk order, then write S (upper triangle) and g
+    // merge the chunks of each (slot, row) in chunk order, then write S (upper triangle) and g, or this part's
+    // row sums to the partial buffer
     if (my_slot >= 0) {
 #pragma unroll
         for (int c = 0; c < DRM; c++) lds[t * (DRM + 1) + c] = acc[c];
@@ -562,10 +625,47 @@ __global__ __launch_bounds__(P2_THREADS) void k_phase2(P2Args a) {
             for (int c = 0; c < DRM; c++) acc[c] += lds[sib * (DRM + 1) + c];
             gacc += lds[sib * (DRM + 1) + DRM];
         }
+        if (it.nparts == 1) {
 #pragma unroll
-        for (int c = 0; c < DRM; c++)
-            if (c < dr2 && (!diag || c >= my_row)) a.S[rowaddr + c * it.colstride] = acc[c];
-        if (diag) a.g[it.ro2 + my_row] = gacc;
+            for (int c = 0; c < DRM; c++)
+                if (c < dr2 && (!diag || c >= my_row)) a.S[rowaddr + c * it.colstride] = acc[c];
+            if (diag) a.g[it.ro2 + my_row] = gacc;
+        } else {
+            double *P = a.partial + it.pbase + (int64_t)(a.slot_roff[it.slot_lo + my_slot] + my_row) * (dr2 + 1);
+#pragma unroll
+            for (int c = 0; c < DRM; c++)
+                if (c < dr2) P[c] = acc[c];
+            P[dr2] = gacc;
+        }
+    }
+}
+
+// Sum the parts of split items in part order and write S (upper triangle) and g: one workgroup per split item,
+// one lane per row of its slots.
+__global__ void k_phase2_merge(const P2Merge *merges, const int64_t *slot_rowbase, const int32_t *slot_dr,
+                               const double *partial, double *S, double *g, const unsigned long long *flag) {
+    if (*flag != (unsigned long long)NO_FAILURE) return;
+    const P2Merge mg = merges[blockIdx.x];
+    const int t = threadIdx.x;
+    if (t >= mg.rows) return;
+    int sl = 0, r0 = 0;
+    while (r0 + slot_dr[mg.slot_lo + sl] <= t) r0 += slot_dr[mg.slot_lo + sl++];
+    const int row = t - r0;
+    const bool diag = mg.has_diag && sl == mg.nslots - 1;
+    const int W = mg.dr2 + 1;
+    const int64_t rowaddr = mg.base + slot_rowbase[mg.slot_lo + sl] + row;
+    const int64_t pstride = (int64_t)mg.rows * W;
+    for (int c = diag ? row : 0; c < mg.dr2; c++) {
+        const double *P = partial + mg.pbase + (int64_t)t * W + c;
+        double s = P[0];
+        for (int p = 1; p < mg.nparts; p++) s += P[p * pstride];
+        S[rowaddr + c * mg.colstride] = s;
+    }
+    if (diag) {
+        const double *P = partial + mg.pbase + (int64_t)t * W + mg.dr2;
+        double s = P[0];
+        for (int p = 1; p < mg.nparts; p++) s += P[p * pstride];
+        g[mg.ro2 + row] = s;
     }
 }
 
@@ -676,7 +776,10 @@ struct bls_schur {
     DevArray<P2Item> d_items;
     DevArray<P2Entry> d_entries;
     DevArray<P2Task> d_tasks;
-    DevArray<int32_t> d_slot_dr;
+    DevArray<int32_t> d_slot_dr, d_slot_roff;
+    DevArray<P2Merge> d_merges;
+    DevArray<double> d_partial;   // row sums of the parts of split items
+    int64_t nmerges = 0;
     int uniform_dr_all = 0;
     int max_dr = 0;
     DevArray<int64_t> d_slot_rowbase;
@@ -702,16 +805,19 @@ int check_structure(const bls_schur *h) {
 
 // Build the phase-2 work: items (slot runs of block columns), their landmark entries, and the lanes' tasks.
 // `slots_of(r2)` gives the block rows of column r2 (ascending, last == r2), `rowbase_of(r2, k, r1)` the row base of
-// its k-th block, and base_of the column's addressing. A slot touched by many landmarks gets several lanes per row,
-// each owning a contiguous range of the item's entries (balanced by hit count); their partial sums are merged in
-// range order, so every S entry still receives its updates in ascending eliminated block, deterministically.
+// its k-th block, and base_of the column's addressing. An item with more than P2_PART landmarks is split into
+// parts over consecutive landmark ranges (one workgroup each, summed in part order by k_phase2_merge). Within a
+// part, a slot touched by many landmarks gets several lane groups (chunks), which share each batch's landmarks of
+// the slot in contiguous runs and are merged in chunk order: every sum is taken in a fixed order.
 template <typename Slots, typename RowBase, typename Base>
 int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of) {
     std::vector<P2Item> items;
     std::vector<P2Entry> entries;
     std::vector<P2Task> tasks;
     std::vector<int64_t> slot_rowbase;
-    std::vector<int32_t> slot_dr;
+    std::vector<int32_t> slot_dr, slot_roff;
+    std::vector<P2Merge> merges;
+    int64_t npartial = 0;
     std::vector<int64_t> owner(h->ncoup);
     for (int64_t e = 0; e < h->neb; e++)
         for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++) owner[c] = e;
@@ -741,15 +847,17 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
             it.nslots = (int32_t)(k1 - k0);
             it.has_diag = k1 == col.size();
             int udr = (int)h->r_dims[col[k0]];
+            int roff = 0;
             for (size_t k = k0; k < k1; k++) {
                 slot_dr.push_back((int32_t)h->r_dims[col[k]]);
+                slot_roff.push_back(roff);
                 slot_rowbase.push_back(rowbase_of(r2, k, col[k]));
+                roff += (int)h->r_dims[col[k]];
                 if (h->r_dims[col[k]] != udr) udr = 0;
             }
             it.uniform_dr = udr;
             const int64_t r1_lo = col[k0], r1_hi = col[k1 - 1];
-            it.entry_lo = (int64_t)entries.size();
-            std::fill(hits.begin(), hits.end(), 0);
+            const int64_t e_begin = (int64_t)entries.size();
             for (int64_t kk = h->retained_ptr[r2]; kk < h->retained_ptr[r2 + 1]; kk++) {
                 const int64_t c2 = h->retained_entry[kk];
                 const int64_t e = owner[c2];
@@ -767,7 +875,6 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
                         return fail(BLS_ERR_ARG, "The sparse pattern has no block (" + std::to_string(er[c] + 1) + ", " +
                                                      std::to_string(r2 + 1) + ")");
                     mask |= 1ull << (k - k0);
-                    hits[k - k0]++;
                 }
                 P2Entry en{};
                 en.e = (int32_t)e;
@@ -776,52 +883,88 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
                 en.mask = mask;
                 entries.push_back(en);
             }
-            it.entry_hi = (int64_t)entries.size();
-            // chunks per slot: greedily give another lane group to the slot with the most hits per chunk
-            const int ns = it.nslots;
-            std::vector<int> nch(ns, 1);
-            int lanes = rows;
-            while (true) {
-                int best = -1;
-                double bw = 0.0;
-                for (int sl = 0; sl < ns; sl++) {
-                    const double w = (double)hits[sl] / nch[sl];
-                    if (w > bw) { bw = w; best = sl; }
-                }
-                if (best < 0 || bw < 2.0) break;
-                if (lanes + slot_dr[it.slot_lo + best] > P2_THREADS || nch[best] >= 32767) break;
-                nch[best]++;
-                lanes += slot_dr[it.slot_lo + best];
+            const int64_t e_end = (int64_t)entries.size();
+            const int64_t ne = e_end - e_begin;
+            const int np = ne > P2_PART ? (int)((ne + P2_PART - 1) / P2_PART) : 1;
+            it.nparts = np;
+            if (np > 1) {
+                P2Merge mg{};
+                mg.base = base;
+                mg.colstride = colstride;
+                mg.pbase = npartial;
+                mg.slot_lo = it.slot_lo;
+                mg.nslots = it.nslots;
+                mg.rows = rows;
+                mg.dr2 = dr2;
+                mg.ro2 = it.ro2;
+                mg.has_diag = it.has_diag;
+                mg.nparts = np;
+                merges.push_back(mg);
             }
-            it.task_lo = (int32_t)tasks.size();
-            for (int sl = 0; sl < ns; sl++) {
-                const int dr1 = slot_dr[it.slot_lo + sl];
-                for (int ch = 0; ch < nch[sl]; ch++) {
-                    for (int row = 0; row < dr1; row++) {
-                        P2Task tk{};
-                        tk.slot = (int16_t)sl;
-                        tk.row = (int16_t)row;
-                        tk.chunk = (int16_t)ch;
-                        tk.nchunk = (int16_t)nch[sl];
-                        tasks.push_back(tk);
+            for (int part = 0; part < np; part++) {
+                P2Item pt = it;
+                pt.part = part;
+                pt.entry_lo = e_begin + part * ne / np;
+                pt.entry_hi = e_begin + (part + 1) * ne / np;
+                pt.pbase = np > 1 ? npartial + (int64_t)part * rows * (dr2 + 1) : 0;
+                std::fill(hits.begin(), hits.end(), 0);
+                for (int64_t x = pt.entry_lo; x < pt.entry_hi; x++)
+                    for (unsigned long long mm = entries[x].mask; mm; mm &= mm - 1) hits[__builtin_ctzll(mm)]++;
+                // chunks per slot: greedily give another lane group to the slot with the most hits per chunk
+                const int ns = pt.nslots;
+                std::vector<int> nch(ns, 1);
+                int lanes = rows;
+                while (true) {
+                    int best = -1;
+                    double bw = 0.0;
+                    for (int sl = 0; sl < ns; sl++) {
+                        const double w = (double)hits[sl] / nch[sl];
+                        if (w > bw) { bw = w; best = sl; }
+                    }
+                    if (best < 0 || bw < 2.0) break;
+                    if (lanes + slot_dr[pt.slot_lo + best] > P2_THREADS || nch[best] >= 32767) break;
+                    nch[best]++;
+                    lanes += slot_dr[pt.slot_lo + best];
+                }
+                pt.task_lo = (int32_t)tasks.size();
+                for (int sl = 0; sl < ns; sl++) {
+                    const int dr1 = slot_dr[pt.slot_lo + sl];
+                    for (int ch = 0; ch < nch[sl]; ch++) {
+                        for (int row = 0; row < dr1; row++) {
+                            P2Task tk{};
+                            tk.slot = (int16_t)sl;
+                            tk.row = (int16_t)row;
+                            tk.chunk = (int16_t)ch;
+                            tk.nchunk = (int16_t)nch[sl];
+                            tasks.push_back(tk);
+                        }
                     }
                 }
+                pt.ntasks = (int32_t)(tasks.size() - pt.task_lo);
+                items.push_back(pt);
             }
-            it.ntasks = (int32_t)(tasks.size() - it.task_lo);
-            items.push_back(it);
+            if (np > 1) npartial += (int64_t)np * rows * (dr2 + 1);
             k0 = k1;
         }
     }
     if (entries.size() > (size_t)INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "too many phase-2 entries");
+    // longest items first (their landmark count sets the workgroup's run time): the dispatcher then fills the
+    // tail with short items. Items are independent, so the order does not change any result.
+    std::stable_sort(items.begin(), items.end(), [](const P2Item &x, const P2Item &y) {
+        return x.entry_hi - x.entry_lo > y.entry_hi - y.entry_lo;
+    });
     const int64_t nentries = (int64_t)entries.size();
     // one unused record past the end: the kernel reads entries[entry_lo] unconditionally, also for an item without
     // entries (a block column no landmark touches), whose entry_lo may equal the entry count
     entries.push_back(P2Entry{});
     int rc;
     if ((rc = h->d_items.upload(items)) || (rc = h->d_entries.upload(entries)) || (rc = h->d_tasks.upload(tasks)) ||
-        (rc = h->d_slot_rowbase.upload(slot_rowbase)) || (rc = h->d_slot_dr.upload(slot_dr)))
+        (rc = h->d_slot_rowbase.upload(slot_rowbase)) || (rc = h->d_slot_dr.upload(slot_dr)) ||
+        (rc = h->d_slot_roff.upload(slot_roff)) || (rc = h->d_merges.upload(merges)) ||
+        (rc = h->d_partial.alloc(npartial)))
         return rc;
     h->nitems = (int64_t)items.size();
+    h->nmerges = (int64_t)merges.size();
     h->nentries = nentries;
     return BLS_OK;
 }
@@ -892,6 +1035,8 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.tasks = h->d_tasks.ptr;
     a.slot_rowbase = h->d_slot_rowbase.ptr;
     a.slot_dr = h->d_slot_dr.ptr;
+    a.slot_roff = h->d_slot_roff.ptr;
+    a.partial = h->d_partial.ptr;
     a.eoffsets = h->d_eoffsets.ptr;
     a.boffsets_e = h->d_boffsets_e.ptr;
     a.coffsets = h->d_coffsets.ptr;
@@ -907,8 +1052,11 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.uniform_offsets = h->uniform_de > 0 && h->uniform_dr_all > 0 && h->runs.size() == 1;
     a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
     if (h->nitems > 0) {
-        const unsigned grid = (unsigned)(8 * ((h->nitems + 7) / 8));
-        launch_phase2(h->uniform_de, h->max_dr, grid, s, a);
+        launch_phase2(h->uniform_de, h->max_dr, (unsigned)h->nitems, s, a);
+        if (h->nmerges > 0)
+            k_phase2_merge<<<(unsigned)h->nmerges, P2_THREADS, 0, s>>>(h->d_merges.ptr, h->d_slot_rowbase.ptr,
+                                                                      h->d_slot_dr.ptr, h->d_partial.ptr, S, g,
+                                                                      h->d_flag.ptr);
     }
     h->timing.mark(3, s);
     k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);

@@ -118,6 +118,36 @@ def test_bitwise_reproducible(blsamd):
     assert np.array_equal(a["S"], b["S"]) and np.array_equal(a["g"], b["g"])
 
 
+# Columns with more landmarks than one workgroup takes (384) are split into parts summed in part order; these
+# workloads put 500-1000 landmarks on every camera so that every diagonal item is split.
+SPLIT = 384
+
+
+def _max_column_landmarks(p):
+    return int(np.diff(p.ws_index.retained_ptr).max())
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_split_columns(blsamd, dense):
+    p = blsamd.generators.config2(scale=0.01 if not dense else 0.006, ncam=10 if not dense else 6, dense=dense)
+    assert _max_column_landmarks(p) > 2 * SPLIT
+    compare(p, run_gpu(blsamd, p), oracle_run(p))
+    a, b = run_gpu(blsamd, p, repeat=2)
+    assert np.array_equal(a["S"], b["S"]) and np.array_equal(a["g"], b["g"])
+
+
+def test_split_columns_mixed_eliminated_dims(blsamd):
+    """Landmark dimensions 1-4 mixed (the packed-record path) with 500+ landmarks per camera."""
+    rng = np.random.default_rng(17)
+    r_dims = [6, 9, 6, 9]
+    e_dims = rng.integers(1, 5, size=900).tolist()
+    cpl = [sorted(rng.choice(np.arange(1, 5), size=rng.integers(2, 5), replace=False).tolist()) for _ in e_dims]
+    H, b = schur_test_problem(rng, r_dims, e_dims, cpl)
+    p = problem_from_dense(blsamd, H, b, r_dims, e_dims, cpl, 1e-3, False)
+    assert _max_column_landmarks(p) > SPLIT
+    compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
 def test_singular_block_reports_lowest(blsamd):
     """Rank-1 E at lambda = 0 fails with the lowest failing block (test/schur.jl:405-425)."""
     p = blsamd.generators.config2(scale=0.002)
