@@ -5,7 +5,11 @@ clip(Geometric(0.2), 2, 40) over a contiguous camera window, N(0,1) Jacobian blo
 reset_schur_sparse! + reduce_schur_sparse! + copyto! of the reduced system into its full symmetric CSC.
 A step is one such pass with every input resident in HBM. Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calibration-copy]
+
+--calibration-copy first runs one device copy of 256 MiB (a kernel of known traffic) so that a hardware-counter
+run of this command can convert FETCH_SIZE / WRITE_SIZE into bytes. roofline.traffic is read from the committed
+counter summary profiles/k_phase2_slam_traffic.md when present.
 """
 import argparse
 import json
@@ -19,6 +23,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 METRIC = "landmarks eliminated/s + time-to-1e-8 PCG residual, FP64 plain vs mixed/learned"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_phase2_slam_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_phase2 launch from the counter summary (its line `traffic_bytes_per_launch: N`)."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -32,6 +49,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibration-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no multi-GPU path; "
@@ -44,6 +62,12 @@ def main():
     b = blsload.load()
     dev = b.device
     torch.cuda.set_device(0)
+    if args.calibration_copy:
+        src = torch.ones(256 << 17, dtype=torch.float64, device="cuda")   # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     p = b.generators.config2()
     st = p.structure
@@ -88,7 +112,7 @@ def main():
 
     # per-kernel durations (HIP events recorded by the library on the launch stream)
     h.set_timing(True)
-    kt = np.zeros(3)
+    kt = np.zeros(4)
     for _ in range(args.steps):
         dev.reset_schur_sparse_(S, Rd, h, sr, lam, check=False)
         dev.reduce_schur_sparse_(S, g, b_r, b_e, se, lam, h, E, C, ws, check=False)
@@ -104,7 +128,10 @@ def main():
     whole_bytes = float(np.sum(216 + 144 * k))   # the per-landmark figure 216 + 144 k over the instance
     p2_gbs = p2_bytes / (kt[2] * 1e-3) / 1e9
     log(f"kernels: reset {kt[0]:.4f} ms, phase1 {kt[1]:.4f} ms ({p1_bytes / (kt[1] * 1e-3) / 1e9:.0f} GB/s), "
-        f"phase2 {kt[2]:.4f} ms ({p2_gbs:.0f} GB/s)")
+        f"phase2 {kt[2]:.4f} ms ({p2_gbs:.0f} GB/s), merge {kt[3]:.4f} ms")
+    pcie = pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, max(3, args.steps // 4))
+    log(f"PCIe-inclusive (host E, C, b, R in; S, g, CSC out): {pcie['ms_per_step']:.3f} ms/step, "
+        f"{pcie['value']:.4g} landmarks/s")
     result = {
         "metric": METRIC, "value": value, "unit": "landmarks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -112,14 +139,50 @@ def main():
         "config": {"workload": "reduce_schur_sparse! + copyto!, sequential SLAM, 1000 cams (d6) x 200000 landmarks (d3), "
                                "FP64", "landmarks": int(st.neblocks), "cameras": int(st.nrblocks),
                    "couplings": int(st.ncoup), "S_nnz": int(nnz), "csc_nnz": int(len(smap))},
-        "kernel_ms": {"reset": kt[0], "phase1": kt[1], "phase2": kt[2]},
+        "kernel_ms": {"reset": kt[0], "phase1": kt[1], "phase2": kt[2], "phase2_merge": kt[3]},
         "roofline": {"bound": "hbm", "kernel": "k_phase2", "achieved": p2_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": p2_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": p2_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "step_achieved": whole_bytes / (ms * 1e-3) / 1e9},
+        "pcie_inclusive": pcie,
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)
     print(json.dumps(result), flush=True)
+
+
+def pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, steps):
+    """The same step with the inputs in pinned host memory: copy E, C, b_r, b_e, R in, reduce, copy S, g and the
+    CSC values out (the cost a caller holding host arrays pays; not the metric's value)."""
+    st = p.structure
+    pin = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).pin_memory()
+    hE, hC, hbr, hbe, hR = pin(p.E_data), pin(p.C_data), pin(p.b_r), pin(p.b_e), pin(p.R_data)
+    oS = torch.empty(S.numel(), dtype=torch.float64).pin_memory()
+    og = torch.empty(g.numel(), dtype=torch.float64).pin_memory()
+    ocsc = torch.empty(csc.numel(), dtype=torch.float64).pin_memory()
+    dE, dC, dbr, dbe, dR = (torch.empty(x.numel(), dtype=torch.float64, device="cuda") for x in (hE, hC, hbr, hbe, hR))
+    sr = torch.as_tensor(np.ascontiguousarray(p.scaling_r)).cuda()
+    se = torch.as_tensor(np.ascontiguousarray(p.scaling_e)).cuda()
+
+    def run():
+        for d, x in ((dE, hE), (dC, hC), (dbr, hbr), (dbe, hbe), (dR, hR)):
+            d.copy_(x, non_blocking=True)
+        dev.reset_schur_sparse_(S, dR, h, sr, p.lam, check=False)
+        dev.reduce_schur_sparse_(S, g, dbr, dbe, se, p.lam, h, dE, dC, ws, sync=False, check=False)
+        dev.copyto_(csc, S, dmap)
+        oS.copy_(S, non_blocking=True)
+        og.copy_(g, non_blocking=True)
+        ocsc.copy_(csc, non_blocking=True)
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    moved = sum(x.numel() for x in (hE, hC, hbr, hbe, hR, oS, og, ocsc)) * 8
+    return {"value": st.neblocks / (ms * 1e-3), "unit": "landmarks/s", "ms_per_step": ms,
+            "host_bytes_per_step": moved}
 
 
 def cpu_baseline(p, threads):

@@ -1053,12 +1053,14 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
     if (h->nitems > 0) {
         launch_phase2(h->uniform_de, h->max_dr, (unsigned)h->nitems, s, a);
+        h->timing.mark(3, s);
         if (h->nmerges > 0)
             k_phase2_merge<<<(unsigned)h->nmerges, P2_THREADS, 0, s>>>(h->d_merges.ptr, h->d_slot_rowbase.ptr,
                                                                       h->d_slot_dr.ptr, h->d_partial.ptr, S, g,
                                                                       h->d_flag.ptr);
     }
-    h->timing.mark(3, s);
+    if (h->nitems == 0) h->timing.mark(3, s);
+    h->timing.mark(4, s);
     k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
@@ -1073,7 +1075,7 @@ int reduce_sync(bls_schur *h, int mode, double *S, double *g, const double *b_r,
     BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     BLS_HIP(hipStreamSynchronize(s));
     if (h->timing.enabled)
-        for (int i = 1; i < 3; i++) (void)hipEventElapsedTime(&h->timing.ms[i], h->timing.ev[i], h->timing.ev[i + 1]);
+        for (int i = 1; i < 4; i++) (void)hipEventElapsedTime(&h->timing.ms[i], h->timing.ev[i], h->timing.ev[i + 1]);
     if (failed_block) *failed_block = fb;
     return fb ? BLS_FAILED : BLS_OK;
 }
@@ -1292,6 +1294,6 @@ extern "C" int bls_schur_set_timing(bls_schur *h, int enable) {
 extern "C" int bls_schur_kernel_times(bls_schur *h, float *times, int n) {
     if (!h || !times) return fail(BLS_ERR_ARG, "bad arguments");
     if (h->timing.enabled) (void)hipEventElapsedTime(&h->timing.ms[0], h->timing.ev[0], h->timing.ev[1]);
-    for (int i = 0; i < n && i < 3; i++) times[i] = h->timing.ms[i];
+    for (int i = 0; i < n && i < 4; i++) times[i] = h->timing.ms[i];
     return BLS_OK;
 }

@@ -150,8 +150,8 @@ class SchurHandle:
         _check(lib().bls_schur_set_timing(self.h, 1 if on else 0))
 
     def kernel_times(self):
-        arr = (ct.c_float * 3)()
-        _check(lib().bls_schur_kernel_times(self.h, arr, 3))
+        arr = (ct.c_float * 4)()
+        _check(lib().bls_schur_kernel_times(self.h, arr, 4))
         return list(arr)
 
 

@@ -117,7 +117,8 @@ int bls_copyto_csc_f64(int64_t n, double *dest, const double *src, const int64_t
 int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_map);
 
 /* Optional per-kernel timing of the last call on a handle (HIP events on the launch stream), in ms:
- * times[0] reset, [1] phase 1, [2] phase 2. Enabled by bls_schur_set_timing(h, 1). */
+ * times[0] reset, [1] phase 1, [2] phase 2 (k_phase2), [3] the merge of split block columns. Enabled by
+ * bls_schur_set_timing(h, 1). */
 int bls_schur_set_timing(bls_schur *h, int enable);
 int bls_schur_kernel_times(bls_schur *h, float *times, int n);
 
