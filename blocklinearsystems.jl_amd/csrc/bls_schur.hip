@@ -752,6 +752,27 @@ __global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eof
     }
 }
 
+// Conditioning scales (src/conditioning.jl:99-141): Cs = the value the reference reads for every coupling entry,
+// scaled(C_er[i, ir], s_e[i] s_r[ir]) = C * (s_e * s_r) in Float64, for phase 2 to use in place of C (phase 1 scales
+// E and b_e itself). One lane per eliminated block.
+__global__ void k_scale_c(int64_t neb, const int32_t *e_dims, const int64_t *boffsets_e, const int64_t *eliminated_ptr,
+                          const int32_t *entry_retained, const int64_t *coffsets, const int32_t *r_dims,
+                          const int64_t *boffsets_r, const double *C, const double *s_r, const double *s_e, double *Cs) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= neb) return;
+    const int de = e_dims[e];
+    const int64_t bo = boffsets_e[e];
+    for (int64_t c = eliminated_ptr[e]; c < eliminated_ptr[e + 1]; c++) {
+        const int r = entry_retained[c];
+        const int dr = r_dims[r];
+        const int64_t ro = boffsets_r[r], co = coffsets[c];
+        for (int ir = 0; ir < dr; ir++) {
+            const double f = s_r ? s_r[ro + ir] : 1.0;
+            for (int i = 0; i < de; i++) Cs[co + ir * de + i] = C[co + ir * de + i] * ((s_e ? s_e[bo + i] : 1.0) * f);
+        }
+    }
+}
+
 struct Run {
     int64_t first, count, de, eoff0, boff0;
 };
@@ -779,6 +800,8 @@ struct bls_schur {
     DevArray<int32_t> d_slot_dr, d_slot_roff;
     DevArray<P2Merge> d_merges;
     DevArray<double> d_partial;   // row sums of the parts of split items
+    DevArray<double> d_Cs;        // scaled coupling blocks, when conditioning scales are given
+    int64_t clen = 0;             // length of C_data
     int64_t nmerges = 0;
     int uniform_dr_all = 0;
     int max_dr = 0;
@@ -1020,13 +1043,23 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
                                                   : "handle is not dense (bls_schur_set_dense)");
     if (!(std::isfinite(lambda) && lambda >= 0))
         return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
-    if (s_r || s_e) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
     if (!S || !g || !b_r || !scaling_e || !E || !C || !E_inv || !z || !d_failed || (h->neb && !b_e))
         return fail(BLS_ERR_ARG, "reduce_schur: a required array is NULL");
+    if ((s_r || s_e) && h->clen && !h->d_Cs.ptr) {
+        int rc = h->d_Cs.alloc(h->clen);
+        if (rc) return rc;
+    }
     h->timing.mark(1, s);
     k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
     int rc = phase1(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, s);
     if (rc) return rc;
+    if ((s_r || s_e) && h->neb) {
+        k_scale_c<<<grid_for(h->neb, 256), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_boffsets_e.ptr,
+                                                         h->d_eliminated_ptr.ptr, h->d_entry_retained.ptr,
+                                                         h->d_coffsets.ptr, h->d_r_dims.ptr, h->d_boffsets_r.ptr, C,
+                                                         s_r, s_e, h->d_Cs.ptr);
+        C = h->d_Cs.ptr;
+    }
     h->timing.mark(2, s);
     P2Args a{};
     a.items = h->d_items.ptr;
@@ -1117,6 +1150,9 @@ extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, 
     if (rc) { delete h; return rc; }
     for (int64_t c = 0; c < ncoup; c++)
         if (h->entry_retained[c] < 0 || h->entry_retained[c] >= nrblocks) { delete h; return fail(BLS_ERR_ARG, "entry_retained out of range"); }
+    for (int64_t e = 0; e < neblocks; e++)
+        for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++)
+            h->clen = std::max(h->clen, h->coffsets[c] + h->e_dims[e] * h->r_dims[h->entry_retained[c]]);
     // runs of equal d_e, with contiguous E / b offsets inside each
     h->uniform_de = neblocks ? (int)h->e_dims[0] : 3;
     for (int64_t e = 0; e < neblocks;) {
@@ -1163,7 +1199,23 @@ extern "C" int bls_schur_set_sparse(bls_schur *h, int64_t nnz, const int64_t *S_
         });
     if (rc) return rc;
     std::vector<int64_t> rts(r_to_s_map, r_to_s_map + nR), di(diag_inds, diag_inds + h->nr);
-    if ((rc = h->d_r_to_s.upload(rts)) || (rc = h->d_diag_inds.upload(di)) || (rc = h->d_S_colptr.upload(Scp))) return rc;
+    // scalar row of every stored entry (1-based), for the conditioning scale of reset_schur_sparse!: a scalar column
+    // of block column r2 holds, in order, the rows of the blocks (r1, r2) of the pattern
+    std::vector<int64_t> Srv((size_t)nnz);
+    for (int64_t r2 = 0; r2 < h->nrb; r2++) {
+        std::vector<int64_t> rows;
+        for (int64_t k = bcp[r2]; k < bcp[r2 + 1]; k++)
+            for (int64_t i = 0; i < h->r_dims[brv[k]]; i++) rows.push_back(h->boffsets_r[brv[k]] + i + 1);
+        if ((int64_t)rows.size() != col_heights[r2]) return fail(BLS_ERR_ARG, "col_heights do not match the block pattern");
+        for (int64_t j = h->boffsets_r[r2]; j < h->boffsets_r[r2] + h->r_dims[r2]; j++) {
+            if (Scp[j + 1] - Scp[j] != col_heights[r2] || Scp[j + 1] - 1 > nnz)
+                return fail(BLS_ERR_ARG, "S_colptr does not match the block pattern");
+            std::copy(rows.begin(), rows.end(), Srv.begin() + (Scp[j] - 1));
+        }
+    }
+    if ((rc = h->d_r_to_s.upload(rts)) || (rc = h->d_diag_inds.upload(di)) || (rc = h->d_S_colptr.upload(Scp)) ||
+        (rc = h->d_S_rowval.upload(Srv)))
+        return rc;
     h->nnz = nnz;
     h->nR = nR;
     h->mode = 1;
@@ -1190,12 +1242,13 @@ extern "C" int bls_reset_schur_sparse_f64(bls_schur *h, double *nzval, const dou
     if (h->mode != 1) return fail(BLS_ERR_STATE, "handle has no sparse layout (bls_schur_set_sparse)");
     if (!(std::isfinite(lambda) && lambda >= 0))
         return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
-    if (s_r) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
     if (!nzval || !scaling_r || (h->nR && !R_nzval)) return fail(BLS_ERR_ARG, "reset_schur_sparse: a required array is NULL");
     hipStream_t s = (hipStream_t)stream;
     h->timing.mark(0, s);
     k_fill_f64<<<std::min<int64_t>(grid_for(h->nnz, 256), 4096), 256, 0, s>>>(nzval, h->nnz, 0.0);
     if (h->nR) k_scatter_r<<<std::min<int64_t>(grid_for(h->nR, 256), 4096), 256, 0, s>>>(nzval, R_nzval, h->d_r_to_s.ptr, h->nR);
+    if (s_r && h->nr)   // scale_sparse! (src/conditioning.jl:247-253)
+        k_scale_sparse<<<grid_for(h->nr, 256), 256, 0, s>>>(nzval, h->d_S_colptr.ptr, h->d_S_rowval.ptr, h->nr, s_r);
     if (h->nr) k_add_damping<<<grid_for(h->nr, 256), 256, 0, s>>>(nzval, h->d_diag_inds.ptr, h->nr, scaling_r, lambda, s_r);
     h->timing.mark(1, s);
     BLS_HIP(hipGetLastError());

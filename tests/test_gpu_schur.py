@@ -240,3 +240,63 @@ def test_null_required_arrays_rejected(blsamd):
         dev.reduce_schur_sparse_(S, g, T(p.b_r), T(p.b_e), None, p.lam, h, T(p.E_data), T(p.C_data), ws, check=False)
     with pytest.raises(ValueError):
         dev.reduce_schur_sparse_(S, g, T(p.b_r), T(p.b_e), T(p.scaling_e), -1.0, h, T(p.E_data), T(p.C_data), ws)
+
+
+def _scales(p, seed):
+    rng = np.random.default_rng(seed)
+    st = p.structure
+    return 10.0 ** rng.uniform(-1, 1, st.nr), 10.0 ** rng.uniform(-1, 1, st.ne)
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_conditioning_scales(blsamd, dense):
+    """Conditioning scales s_r, s_e (src/conditioning.jl:99-141): every stored value is read as
+    scaled(a, s_i s_j); reset scales R and the damping by s_r^2; phase 1 scales E and b_e; phase 2 reads
+    scaled C; backsubstitute! unscales nothing (x is in the scaled variables)."""
+    torch = _torch()
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.01 if not dense else 0.003, dense=dense)
+    st = p.structure
+    s_r, s_e = _scales(p, 3)
+    # oracle
+    args = oracle.SchurArgs(st, p.ws_index, p.pattern)
+    E_inv_o = np.zeros(len(p.E_data)); z_o = np.zeros(st.ne); W = np.zeros(len(p.C_data)); g_o = np.zeros(st.nr)
+    if dense:
+        S_o = np.zeros(st.nr * st.nr)
+        oracle.reset_schur_dense(S_o, p.R.T.reshape(-1).copy(), p.scaling_r, p.lam, s_r=s_r)
+    else:
+        S_o = np.zeros(p.pattern.nnz)
+        oracle.reset_schur_sparse(S_o, p.R_data, p.r_to_s, p.pattern, p.scaling_r, p.lam, s_r=s_r)
+    ok_o, fb_o = oracle.reduce_schur(not dense, args, S_o, g_o, p.b_r, p.b_e, p.scaling_e, p.lam, p.E_data,
+                                     p.C_data, E_inv_o, z_o, W, s_r=s_r, s_e=s_e)
+    assert ok_o
+    # device
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=getattr(p, "r_to_s", None), dense=dense)
+    ws = dev.SchurReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    E, C, b_r, b_e, sr, se = T(p.E_data), T(p.C_data), T(p.b_r), T(p.b_e), T(p.scaling_r), T(p.scaling_e)
+    dsr, dse = T(s_r), T(s_e)
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    if dense:
+        S = torch.empty(st.nr * st.nr, dtype=torch.float64, device="cuda")
+        dev.reset_schur_dense_(S, T(p.R.T.reshape(-1)), h, sr, p.lam, s_r=dsr)
+        ok = dev.reduce_schur_dense_(S, g, b_r, b_e, se, p.lam, h, E, C, ws, s_r=dsr, s_e=dse)
+    else:
+        S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
+        dev.reset_schur_sparse_(S, T(p.R_data), h, sr, p.lam, s_r=dsr)
+        ok = dev.reduce_schur_sparse_(S, g, b_r, b_e, se, p.lam, h, E, C, ws, s_r=dsr, s_e=dse)
+    torch.cuda.synchronize()
+    gpu = dict(S=S.cpu().numpy(), g=g.cpu().numpy(), E_inv=ws.E_inv_data.cpu().numpy(), z=ws.z_data.cpu().numpy(), ok=ok)
+    orc = dict(S=S_o, g=g_o, E_inv=E_inv_o, z=z_o, ok=ok_o)
+    compare(p, gpu, orc)
+    # the unscaled reduction differs: the scales are really applied
+    base = oracle_run(p)
+    assert not np.allclose(base["g"], g_o)
+    # backsubstitute with the same scales
+    x_r = np.random.default_rng(1).standard_normal(st.nr)
+    xe_o = np.zeros(st.ne)
+    oracle.backsubstitute_schur(args, xe_o, x_r, p.C_data, E_inv_o, z_o, s_r=s_r, s_e=s_e)
+    xe = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    dev.backsubstitute_(xe, T(x_r), h, C, ws, s_r=dsr, s_e=dse)
+    torch.cuda.synchronize()
+    assert np.abs(xe.cpu().numpy() - xe_o).max() <= TOL * max(1.0, np.abs(xe_o).max())
