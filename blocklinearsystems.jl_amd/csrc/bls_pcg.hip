@@ -7,10 +7,17 @@
 //
 // Layout: S arrives in the upper-block CSC of build_schur_pattern; at each solve it is gathered once (the copyto!
 // kernel) into a symmetric block-row layout (BSR: every block (i, j) of the full matrix, row-major, blocks of a
-// block row contiguous). An iteration is two kernels: the BSR product q = S p, one lane per scalar row, and an
-// update kernel with one workgroup per problem (dot products reduced in a fixed order: deterministic). A fixed
-// number of iterations is captured once in a hipGraph on the handle's own stream and replayed until every problem
-// has converged.
+// block row contiguous). An iteration is three kernels, each skipping converged problems:
+//   k_bsr_spmv_w  q = S p, one wavefront per block row (lanes over its blocks, a fixed-order wave reduction), and
+//                 the block row's partial p.q;
+//   k_pcg_step1   per group of up to 256 blocks of one problem: alpha = rz / (sum of the problem's p.q partials,
+//                 in a fixed order), x += alpha p, r -= alpha q, z = M^-1 r per block, partials r.z and r.r;
+//   k_pcg_step2   beta = rz_new / rz from the same fixed-order sums, p = z + beta p; the problem's first group
+//                 records rz_new, the iteration count and the convergence test.
+// Every group of a problem forms the same sums in the same order, so the iteration is deterministic. rz is kept
+// per iteration parity (rz2[it & 1] read, rz2[(it + 1) & 1] written) so that no group overwrites a value another
+// still reads. A fixed even number of iterations is captured once in a hipGraph on the handle's own stream and
+// replayed until every problem has converged.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,11 +29,12 @@ using namespace bls;
 namespace {
 
 constexpr int UPD_THREADS = 512;
-constexpr int ITERS_PER_GRAPH = 8;
+constexpr int GRP_BLOCKS = 256;     // blocks per group of k_pcg_step1 / k_pcg_step2 (one thread each)
+constexpr int ITERS_PER_GRAPH = 8;  // even: the rz parity of every problem is 0 at each replay's start
 constexpr int MAXD = 16;
 
 struct ProbState {
-    double rz, gnorm, relres;
+    double rz2[2], gnorm, relres;
     int64_t iters;
     int32_t active, failed;
 };
@@ -44,29 +52,69 @@ struct PcgArgs {
     const double *L;                   // preconditioner factors (lower, column-major d x d per block)
     const int64_t *loff;               // offset of each block's factor
     double *x, *r, *z, *p, *q, *g;
+    double *pq_part, *rz_part, *rr_part;   // per block
+    const int32_t *grp_prob;           // problem of each group
+    const int64_t *grp_b0, *grp_b1;    // block range of each group
+    int64_t nrb;
     ProbState *st;
     int *nactive;
     double tol;
     int64_t maxit;
 };
 
-__global__ void k_bsr_spmv(PcgArgs a, const double *xin, double *yout, int only_active) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.nr) return;
-    const int i = a.row_block[s];
+// sum over the 64 lanes; lane 0's result is formed in a fixed order (the only one used)
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// yout = S xin for the block rows of active problems (all rows when only_active = 0); D >= every block dimension.
+// One wavefront per block row: lane l takes blocks l, l + 64, ... of the row. With pq_part, the block row's
+// partial sum of p . yout (p = xin) is stored too.
+template <int D>
+__global__ __launch_bounds__(256) void k_bsr_spmv_w(PcgArgs a, const double *xin, double *yout, double *pq_part,
+                                                     int only_active) {
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= a.nrb) return;
     if (only_active && !a.st[a.prob_of_block[i]].active) return;
     const int di = a.dims[i];
-    const int row = (int)(s - a.bo[i]);
-    double acc = 0.0;
-    for (int k = a.bsr_rowptr[i]; k < a.bsr_rowptr[i + 1]; k++) {
+    double y[D];
+#pragma unroll
+    for (int r = 0; r < D; r++) y[r] = 0.0;
+    for (int k = a.bsr_rowptr[i] + lane; k < a.bsr_rowptr[i + 1]; k += 64) {
         const int j = a.bsr_col[k];
         const int dj = a.dims[j];
-        const double *B = a.vals + a.bsr_voff[k] + (int64_t)row * dj;
+        const double *B = a.vals + a.bsr_voff[k];
         const double *xj = xin + a.bo[j];
-        for (int c = 0; c < dj; c++) acc += B[c] * xj[c];
+        double xv[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) xv[c] = c < dj ? xj[c] : 0.0;
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            if (r < di) {
+                double t = 0.0;
+#pragma unroll
+                for (int c = 0; c < D; c++)
+                    if (c < dj) t += B[r * dj + c] * xv[c];
+                y[r] += t;
+            }
+        }
     }
-    yout[s] = acc;
-    (void)di;
+#pragma unroll
+    for (int r = 0; r < D; r++) y[r] = wave_sum64(y[r]);
+    if (lane == 0) {
+        const int64_t b = a.bo[i];
+        double pq = 0.0;
+#pragma unroll
+        for (int r = 0; r < D; r++)
+            if (r < di) {
+                yout[b + r] = y[r];
+                pq += xin[b + r] * y[r];
+            }
+        if (pq_part) pq_part[i] = pq;
+    }
 }
 
 // z_b = (L L')^-1 r_b for one block (solve_diagonal_blocks!, src/diagonal_blocks.jl:33-61)
@@ -133,7 +181,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_pcg_restart(PcgArgs a, int firs
     rz = wg_sum(rz, red);
     if (threadIdx.x == 0) {
         if (first) { ps.gnorm = sqrt(gg); ps.iters = 0; }
-        ps.rz = rz;
+        ps.rz2[0] = rz;
         ps.relres = ps.gnorm > 0 ? sqrt(rr) / ps.gnorm : 0.0;
         const int was = first ? 0 : ps.active;
         ps.active = (ok && ps.relres > a.tol && ps.iters < a.maxit && isfinite(ps.relres)) ? 1 : 0;
@@ -142,37 +190,72 @@ __global__ __launch_bounds__(UPD_THREADS) void k_pcg_restart(PcgArgs a, int firs
     }
 }
 
-// one PCG iteration after q = S p (Saad Alg. 9.1, lines 3-7)
-__global__ __launch_bounds__(UPD_THREADS) void k_pcg_update(PcgArgs a) {
-    __shared__ double red[UPD_THREADS];
-    const int pb = blockIdx.x;
+// fixed-order sum over the 256 threads of a group
+__device__ inline double grp_sum(double v, double *red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = GRP_BLOCKS / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    const double out = red[0];
+    __syncthreads();
+    return out;
+}
+
+// sum of a per-block partial over the blocks [b0, b1) of one problem, in a fixed order
+__device__ inline double problem_sum(const double *part, int64_t b0, int64_t b1, double *red) {
+    double s = 0.0;
+    for (int64_t b = b0 + threadIdx.x; b < b1; b += GRP_BLOCKS) s += part[b];
+    return grp_sum(s, red);
+}
+
+// PCG iteration, after q = S p (Saad Alg. 9.1, lines 3-5): alpha, x, r, z = M^-1 r, partial r.z and r.r
+__global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_step1(PcgArgs a, int parity) {
+    __shared__ double red[GRP_BLOCKS];
+    const int pb = a.grp_prob[blockIdx.x];
+    const ProbState &ps = a.st[pb];
+    if (!ps.active) return;
+    const double pq = problem_sum(a.pq_part, a.prob_bptr[pb], a.prob_bptr[pb + 1], red);
+    const double alpha = ps.rz2[parity] / pq;
+    const int64_t b = a.grp_b0[blockIdx.x] + threadIdx.x;
+    if (b >= a.grp_b1[blockIdx.x]) return;
+    const int d = a.dims[b];
+    const int64_t o = a.bo[b];
+    double rr = 0.0;
+    for (int i = 0; i < d; i++) {
+        a.x[o + i] += alpha * a.p[o + i];
+        const double rv = a.r[o + i] - alpha * a.q[o + i];
+        a.r[o + i] = rv;
+        rr += rv * rv;
+    }
+    block_solve(a.L + a.loff[b], d, a.r + o, a.z + o);
+    double rz = 0.0;
+    for (int i = 0; i < d; i++) rz += a.r[o + i] * a.z[o + i];
+    a.rz_part[b] = rz;
+    a.rr_part[b] = rr;
+}
+
+// PCG iteration (lines 6-7): beta, p = z + beta p; the problem's first group records rz, the iteration count and
+// the convergence test on the recurrence residual (confirmed with the true residual after the iterations)
+__global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_step2(PcgArgs a, int parity) {
+    __shared__ double red[GRP_BLOCKS];
+    const int pb = a.grp_prob[blockIdx.x];
     ProbState &ps = a.st[pb];
     if (!ps.active) return;
     const int64_t b0 = a.prob_bptr[pb], b1 = a.prob_bptr[pb + 1];
-    const int64_t s0 = a.bo[b0], s1 = a.bo[b1 - 1] + a.dims[b1 - 1];
-    double pq = 0.0;
-    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) pq += a.p[s] * a.q[s];
-    pq = wg_sum(pq, red);
-    const double alpha = ps.rz / pq;
-    double rr = 0.0;
-    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
-        a.x[s] += alpha * a.p[s];
-        const double rv = a.r[s] - alpha * a.q[s];
-        a.r[s] = rv;
-        rr += rv * rv;
+    const double rz = problem_sum(a.rz_part, b0, b1, red);
+    const double rr = problem_sum(a.rr_part, b0, b1, red);
+    const double beta = rz / ps.rz2[parity];
+    const int64_t b = a.grp_b0[blockIdx.x] + threadIdx.x;
+    if (b < a.grp_b1[blockIdx.x]) {
+        const int d = a.dims[b];
+        const int64_t o = a.bo[b];
+        for (int i = 0; i < d; i++) a.p[o + i] = a.z[o + i] + beta * a.p[o + i];
     }
-    rr = wg_sum(rr, red);
-    for (int64_t b = b0 + threadIdx.x; b < b1; b += UPD_THREADS)
-        block_solve(a.L + a.loff[b], a.dims[b], a.r + a.bo[b], a.z + a.bo[b]);
-    __syncthreads();
-    double rz = 0.0;
-    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) rz += a.r[s] * a.z[s];
-    rz = wg_sum(rz, red);
-    const double beta = rz / ps.rz;
-    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) a.p[s] = a.z[s] + beta * a.p[s];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ps.rz = rz;
+    if (a.grp_b0[blockIdx.x] == b0 && threadIdx.x == 0) {
+        ps.rz2[parity ^ 1] = rz;
         ps.iters += 1;
         ps.relres = ps.gnorm > 0 ? sqrt(rr) / ps.gnorm : 0.0;
         if (!(ps.relres > a.tol) || ps.iters >= a.maxit || !isfinite(ps.relres)) {
@@ -229,7 +312,7 @@ __global__ void k_pcg_init_state(ProbState *st, int nprob, int *nactive) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *nactive = 0;
     if (i < nprob) {
-        st[i].rz = 0; st[i].gnorm = 0; st[i].relres = 0; st[i].iters = 0; st[i].active = 0; st[i].failed = 0;
+        st[i].rz2[0] = 0; st[i].rz2[1] = 0; st[i].gnorm = 0; st[i].relres = 0; st[i].iters = 0; st[i].active = 0; st[i].failed = 0;
     }
 }
 
@@ -240,7 +323,11 @@ struct bls_pcg {
     std::vector<int64_t> prob_bptr;
     DevArray<int32_t> d_row_block, d_rowptr, d_col, d_dims, d_prob_of_block;
     DevArray<int64_t> d_voff, d_bo, d_prob_bptr, d_map, d_loff, d_diag_voff;
-    DevArray<double> d_vals, d_L, d_x, d_r, d_z, d_p, d_q, d_g;
+    DevArray<double> d_vals, d_L, d_x, d_r, d_z, d_p, d_q, d_g, d_pq, d_rzp, d_rrp;
+    DevArray<int32_t> d_grp_prob;
+    DevArray<int64_t> d_grp_b0, d_grp_b1;
+    int64_t ngroups = 0;
+    int maxd = 1;
     DevArray<ProbState> d_st;
     DevArray<int> d_nactive;
     hipStream_t stream = nullptr;
@@ -271,6 +358,9 @@ struct bls_pcg {
         a.L = d_L.ptr;
         a.loff = d_loff.ptr;
         a.x = d_x.ptr; a.r = d_r.ptr; a.z = d_z.ptr; a.p = d_p.ptr; a.q = d_q.ptr; a.g = d_g.ptr;
+        a.pq_part = d_pq.ptr; a.rz_part = d_rzp.ptr; a.rr_part = d_rrp.ptr;
+        a.grp_prob = d_grp_prob.ptr; a.grp_b0 = d_grp_b0.ptr; a.grp_b1 = d_grp_b1.ptr;
+        a.nrb = nrb;
         a.st = d_st.ptr;
         a.nactive = d_nactive.ptr;
         a.tol = tol;
@@ -345,6 +435,17 @@ extern "C" int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dim
         if (diag_voff[i] < 0) { delete h; return fail(BLS_ERR_ARG, "the pattern has no diagonal block " + std::to_string(i + 1)); }
     }
     (void)nnz; (void)nnz_blocks; (void)col_heights;
+    // update groups: up to GRP_BLOCKS consecutive blocks of one problem each
+    std::vector<int32_t> grp_prob;
+    std::vector<int64_t> grp_b0, grp_b1;
+    for (int64_t p = 0; p < nprob; p++)
+        for (int64_t b = pbp[p]; b < pbp[p + 1]; b += GRP_BLOCKS) {
+            grp_prob.push_back((int32_t)p);
+            grp_b0.push_back(b);
+            grp_b1.push_back(std::min<int64_t>(b + GRP_BLOCKS, pbp[p + 1]));
+        }
+    h->ngroups = (int64_t)grp_prob.size();
+    for (int64_t r = 0; r < nrblocks; r++) h->maxd = std::max(h->maxd, (int)r_dims[r]);
     h->nvals = nv;
     h->nbsr = (int64_t)col.size();
     h->nL = nl;
@@ -356,7 +457,9 @@ extern "C" int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dim
         (rc = h->d_loff.upload(loff)) || (rc = h->d_diag_voff.upload(diag_voff)) || (rc = h->d_vals.alloc(nv)) ||
         (rc = h->d_L.alloc(nl)) || (rc = h->d_x.alloc(nr)) || (rc = h->d_r.alloc(nr)) || (rc = h->d_z.alloc(nr)) ||
         (rc = h->d_p.alloc(nr)) || (rc = h->d_q.alloc(nr)) || (rc = h->d_g.alloc(nr)) || (rc = h->d_st.alloc(nprob)) ||
-        (rc = h->d_nactive.alloc(1))) {
+        (rc = h->d_nactive.alloc(1)) || (rc = h->d_pq.alloc(nrblocks)) || (rc = h->d_rzp.alloc(nrblocks)) ||
+        (rc = h->d_rrp.alloc(nrblocks)) || (rc = h->d_grp_prob.upload(grp_prob)) || (rc = h->d_grp_b0.upload(grp_b0)) ||
+        (rc = h->d_grp_b1.upload(grp_b1))) {
         delete h;
         return rc;
     }
@@ -390,7 +493,13 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
     BLS_HIP(hipStreamWaitEvent(s, h->ev_in, 0));
     PcgArgs a = h->args(tol, maxit);
     const int nprob = (int)h->nprob;
-    const unsigned rows_grid = grid_for(h->nr, 256);
+    auto spmv = [&](const double *xin, double *yout, double *pq, int only_active) {
+        const unsigned grid = (unsigned)((h->nrb + 3) / 4);
+        if (h->maxd <= 6) k_bsr_spmv_w<6><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+        else if (h->maxd <= 9) k_bsr_spmv_w<9><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+        else k_bsr_spmv_w<16><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    };
+    const unsigned ngroups = (unsigned)h->ngroups;
     k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
     k_copy_f64<<<std::min<int64_t>(grid_for(h->nr, 256), 4096), 256, 0, s>>>(h->d_g.ptr, g, h->nr);
     k_pcg_init_state<<<grid_for(nprob, 256), 256, 0, s>>>(h->d_st.ptr, nprob, h->d_nactive.ptr);
@@ -403,8 +512,9 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
         if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
         BLS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int it = 0; it < ITERS_PER_GRAPH; it++) {
-            k_bsr_spmv<<<rows_grid, 256, 0, s>>>(a, h->d_p.ptr, h->d_q.ptr, 1);
-            k_pcg_update<<<nprob, UPD_THREADS, 0, s>>>(a);
+            spmv(h->d_p.ptr, h->d_q.ptr, h->d_pq.ptr, 1);
+            k_pcg_step1<<<ngroups, GRP_BLOCKS, 0, s>>>(a, it & 1);
+            k_pcg_step2<<<ngroups, GRP_BLOCKS, 0, s>>>(a, it & 1);
         }
         hipError_t e = hipStreamEndCapture(s, &h->graph);
         if (e != hipSuccess) return fail(BLS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -424,7 +534,7 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
             BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
             BLS_HIP(hipStreamSynchronize(s));
         }
-        k_bsr_spmv<<<rows_grid, 256, 0, s>>>(a, h->d_x.ptr, h->d_q.ptr, 0);
+        spmv(h->d_x.ptr, h->d_q.ptr, nullptr, 0);
         k_pcg_restart<<<nprob, UPD_THREADS, 0, s>>>(a, 0);
         BLS_HIP(hipGetLastError());
         BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -85,31 +85,34 @@ def _hessian_from_tracks(rng, ncam, cam_dims, nland, k, ptr, cams, lam, dense, c
         Jc = Jc * f
         Jl = Jl * f
         rr = rr * f[:, :, 0]
-    # E_data: per landmark sum of J_l'J_l, column-major de x de at eoffsets
-    JlTJl = np.einsum("oki,okj->oij", Jl, Jl)          # (nobs, de, de), [i, j]
+    # E_data: per landmark sum of J_l'J_l, column-major de x de at eoffsets (a landmark's observations are
+    # consecutive, so the per-landmark sums are segment sums in observation order)
+    JlTJl = Jl[:, 0, :, None] * Jl[:, 0, None, :] + Jl[:, 1, :, None] * Jl[:, 1, None, :]   # (nobs, de, de), [i, j]
     E = np.zeros((st.neblocks, de, de))
-    np.add.at(E, e_of - 1, JlTJl)
+    first = ptr[:-1]
+    E[e_of[first] - 1] = np.add.reduceat(JlTJl, first, axis=0)
     E_data = E.transpose(0, 2, 1).reshape(-1).copy()    # column-major per block
     # C_data: per coupling J_l'J_c, de x dr column-major: C[co + (ir-1)*de + i-1] = (J_l'J_c)[i, ir]
     C_data = np.zeros(st.coupling_storage_length())
     dr_c = st.r_dims[st.entry_retained - 1]
     for d in np.unique(dr_c):
         m = dcam == d
-        blk = np.einsum("oki,okj->oji", Jl[m], Jc[m][:, :, :d])   # (n, dr, de): [ir, i]
+        jl, jc = Jl[m], Jc[m][:, :, :d]
+        blk = jc[:, 0, :, None] * jl[:, 0, None, :] + jc[:, 1, :, None] * jl[:, 1, None, :]   # (n, dr, de): [ir, i]
         co = st.coffsets[c_of[m] - 1] - 1
         C_data[(co[:, None] + np.arange(d * de)[None, :]).ravel()] = blk.reshape(len(co), -1).ravel()
     # b = J'r in layout order
     b_r = np.zeros(st.nr)
     b_e = np.zeros(st.ne)
-    be = np.einsum("oki,ok->oi", Jl, rr)
-    np.add.at(b_e, (st.boffsets_e[e_of - 1] - 1)[:, None] + np.arange(de)[None, :], be)
-    bc = np.einsum("oki,ok->oi", Jc, rr)
+    be = Jl[:, 0, :] * rr[:, 0, None] + Jl[:, 1, :] * rr[:, 1, None]
+    b_e[(st.boffsets_e[e_of[first] - 1] - 1)[:, None] + np.arange(de)[None, :]] = np.add.reduceat(be, first, axis=0)
+    bc = Jc[:, 0, :] * rr[:, 0, None] + Jc[:, 1, :] * rr[:, 1, None]
     ro = st.boffsets_r[r_of - 1] - 1
     for d in np.unique(dr_c):
         m = dcam == d
         np.add.at(b_r, ro[m][:, None] + np.arange(d)[None, :], bc[m][:, :d])
     # R: diagonal blocks sum J_c'J_c
-    JcTJc = np.einsum("oki,okj->oij", Jc, Jc)
+    JcTJc = Jc[:, 0, :, None] * Jc[:, 0, None, :] + Jc[:, 1, :, None] * Jc[:, 1, None, :]
     Rblk = np.zeros((st.nrblocks, dmax, dmax))
     np.add.at(Rblk, r_of - 1, JcTJc)
     p = HessianProblem()
@@ -181,6 +184,31 @@ def config5_problem(i, seed=5, scale=1.0, ncam=None, nland=None):
     rng = np.random.default_rng([seed, i])
     k, ptr, cams = _tracks_window(rng, ncam, nland)
     return _hessian_from_tracks(rng, ncam, np.full(ncam, 6, dtype=I64), nland, k, ptr, cams, 1e-4, False)
+
+
+def config5_union(seed=5, scale=1.0, nprob=64):
+    """The batched-serving configuration as one structure: the nprob problems' cameras and landmarks side by side,
+    so that S is block diagonal over problems and one reduction (one launch per kernel) and one batched PCG cover
+    them all. Problem i has config5_sizes' camera count and scale x its landmark count; its tracks come from the
+    stream [seed, i] as in config5_problem, the Jacobian values from one stream [seed, nprob]. Adds
+    `prob_block_ptr`: the 1-based first camera block of each problem, then nrblocks + 1."""
+    ncams, nlands = config5_sizes(seed, nprob)
+    ks, cams_all, coff = [], [], 0
+    for i in range(nprob):
+        rng = np.random.default_rng([seed, i])
+        k, _, cams = _tracks_window(rng, int(ncams[i]), max(4, int(round(int(nlands[i]) * scale))))
+        ks.append(k)
+        cams_all.append(cams + coff)
+        coff += int(ncams[i])
+    k = np.concatenate(ks)
+    ptr = np.zeros(len(k) + 1, dtype=I64)
+    np.cumsum(k, out=ptr[1:])
+    rng = np.random.default_rng([seed, nprob])
+    p = _hessian_from_tracks(rng, coff, np.full(coff, 6, dtype=I64), len(k), k, ptr, np.concatenate(cams_all),
+                             1e-4, False)
+    p.prob_block_ptr = np.cumsum([1] + [int(c) for c in ncams]).astype(I64)
+    p.nprob = nprob
+    return p
 
 
 class JacobianProblem:

@@ -549,28 +549,43 @@ def sparse_with_map(bsm, symmetric=False):
 
 def build_r_to_s_map(R, pattern, boffsets_r):
     """build_r_to_s_map(R_sys, pattern, boffsets_r) (src/schur_structure.jl:406-431): for each entry of R's data,
-    its 1-based index in S.nzval (0 for the strict lower triangle of diagonal blocks)."""
+    its 1-based index in S.nzval (0 for the strict lower triangle of diagonal blocks). Vectorized over the stored
+    blocks of R, grouped by block shape."""
     out = np.zeros(R.nnzvals, dtype=I64)
-    S_colptr = pattern.S_colptr
-    for brow, bcol, ds in zip(R.brow, R.bcol, R.start):
-        if brow < bcol:
-            continue
-        drow, dcol = int(R.rowblocksizes[brow - 1]), int(R.columnblocksizes[bcol - 1])
-        lo, hi = pattern.block_colptr[brow - 1] - 1, pattern.block_colptr[brow] - 1
-        k = np.searchsorted(pattern.block_rowval[lo:hi], bcol)
-        if not (k < hi - lo and pattern.block_rowval[lo + k] == bcol):
-            raise ValueError(f"Block ({bcol}, {brow}) of R is not present in the pattern of S")
-        row_offset = pattern.block_row_offsets[lo + k]
-        base = S_colptr[boffsets_r[brow - 1] - 1] + row_offset
-        stride = pattern.col_heights[brow - 1]
-        col, row = np.meshgrid(np.arange(1, dcol + 1), np.arange(1, drow + 1), indexing="ij")
-        col, row = col.ravel(), row.ravel()
-        pos = ds + (col - 1) * drow + row - 1
-        if brow == bcol:
-            m = row <= col
-            out[pos[m] - 1] = base + (col[m] - 1) * stride + row[m] - 1
-        else:
-            out[pos - 1] = base + (row - 1) * stride + col - 1
+    brow, bcol, ds = np.asarray(R.brow, I64), np.asarray(R.bcol, I64), np.asarray(R.start, I64)
+    keep = brow >= bcol
+    brow, bcol, ds = brow[keep], bcol[keep], ds[keep]
+    if len(brow) == 0:
+        return out
+    nrb = len(pattern.block_colptr) - 1
+    bc = np.asarray(pattern.block_colptr, I64)
+    rv = np.asarray(pattern.block_rowval, I64)
+    col_of_slot = np.repeat(np.arange(1, nrb + 1, dtype=I64), np.diff(bc))
+    keys = col_of_slot * (nrb + 1) + rv                       # ascending: slots are column-major, rows sorted
+    want = brow * (nrb + 1) + bcol                            # block (bcol, brow) of the upper pattern
+    k = np.searchsorted(keys, want)
+    bad = (k >= len(keys)) | (keys[np.minimum(k, len(keys) - 1)] != want)
+    if bad.any():
+        i = int(np.argmax(bad))
+        raise ValueError(f"Block ({int(bcol[i])}, {int(brow[i])}) of R is not present in the pattern of S")
+    base = np.asarray(pattern.S_colptr, I64)[np.asarray(boffsets_r, I64)[brow - 1] - 1] + \
+        np.asarray(pattern.block_row_offsets, I64)[k]
+    stride = np.asarray(pattern.col_heights, I64)[brow - 1]
+    drow = np.asarray(R.rowblocksizes, I64)[brow - 1]
+    dcol = np.asarray(R.columnblocksizes, I64)[bcol - 1]
+    for dr, dc in set(zip(drow.tolist(), dcol.tolist())):
+        col, row = np.meshgrid(np.arange(1, dc + 1, dtype=I64), np.arange(1, dr + 1, dtype=I64), indexing="ij")
+        col, row = col.ravel()[None, :], row.ravel()[None, :]
+        idx = np.flatnonzero((drow == dr) & (dcol == dc))
+        for c0 in range(0, len(idx), 1 << 16):      # bounded temporaries
+            m = idx[c0:c0 + (1 << 16)]
+            pos = ds[m][:, None] + (col - 1) * dr + row - 1
+            diag = (brow[m] == bcol[m])[:, None]
+            upper = base[m][:, None] + (col - 1) * stride[m][:, None] + row - 1   # diagonal block: (row, col)
+            lower = base[m][:, None] + (row - 1) * stride[m][:, None] + col - 1   # lower block (brow, bcol): transposed
+            val = np.where(diag, upper, lower)
+            sel = ~diag | (row <= col)
+            out[pos[sel] - 1] = val[sel]
     return out
 
 
