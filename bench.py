@@ -200,11 +200,31 @@ def batched_pcg_leg(torch, dev, b, scale, reps=3):
     red, sol = float(np.median(t_red)) * 1e3, float(np.median(t_pcg)) * 1e3
     log(f"batched leg: reduce {red:.3f} ms ({st.neblocks / red * 1e3:.4g} landmarks/s), PCG {sol:.3f} ms "
         f"(iterations max {int(iters.max())}, mean {iters.mean():.1f}), time to 1e-8 {red + sol:.3f} ms")
+    learned = None
+    P = b.precond
+    if os.path.exists(P.WEIGHTS):
+        # the learned correction: features, MLP, corrected blocks and PCG, all on the device (falls back to plain
+        # block-Jacobi if its solve fails)
+        model = P.MLP.load()
+        P.solve(pcg, S, g, x, p.lam, model=model)
+        torch.cuda.synchronize()
+        t_l = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            okl, itl, rell, used = P.solve(pcg, S, g, x, p.lam, model=model)
+            torch.cuda.synchronize()
+            t_l.append(time.perf_counter() - t0)
+        lms = float(np.median(t_l)) * 1e3
+        learned = {"pcg_ms": lms, "time_to_1e-8_ms": red + lms, "pcg_iters_max": int(itl.max()),
+                   "pcg_iters_mean": float(itl.mean()), "relres_max": float(rell.max()), "used_learned": bool(used),
+                   "output_scale": model.alpha}
+        log(f"batched leg, learned correction (output scale {model.alpha}): PCG {lms:.3f} ms (iterations max "
+            f"{int(itl.max())}, mean {itl.mean():.1f}), time to 1e-8 {red + lms:.3f} ms")
     return {"problems": int(p.nprob), "landmark_scale": scale, "cameras": int(st.nrblocks),
             "landmarks": int(st.neblocks), "S_nnz": int(p.pattern.nnz), "reduce_ms": red,
             "landmarks_per_s": st.neblocks / (red * 1e-3), "pcg_ms": sol, "time_to_1e-8_ms": red + sol,
             "pcg_iters_max": int(iters.max()), "pcg_iters_mean": float(iters.mean()),
-            "relres_max": float(rel.max()), "preconditioner": "block-Jacobi (FP64)"}
+            "relres_max": float(rel.max()), "preconditioner": "block-Jacobi (FP64)", "learned": learned}
 
 
 def pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, steps):

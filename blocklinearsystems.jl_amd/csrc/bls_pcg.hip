@@ -298,6 +298,71 @@ __global__ void k_pcg_factor(PcgArgs a, int64_t nblocks, const int64_t *diag_vof
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// The learned correction of the block-Jacobi preconditioner (no reference counterpart): per retained block,
+// features for a small MLP, and the corrected blocks M_b = D_b + mu_b m_b I (D_b the diagonal block of S, m_b the
+// mean of its diagonal), read from S through the handle's BSR map.
+// ---------------------------------------------------------------------------------------------------------
+constexpr int PC_NF = 26;
+
+// Features of block row i (dimension 6; other dimensions get zeros): the 21 upper entries of D / m, log10 m,
+// log(1 + number of off-diagonal blocks), log10 lambda, log10(m / lambda), log10(off-diagonal Frobenius mass of the
+// block row / ||D||_F). One wavefront per block row; the off-diagonal sum is a fixed-order wave reduction.
+__global__ __launch_bounds__(256) void k_pcg_features(PcgArgs a, const double *S, const int64_t *map,
+                                                       const int64_t *diag_voff, double lambda, float *F) {
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= a.nrb) return;
+    const int di = a.dims[i];
+    double off = 0.0;
+    for (int k = a.bsr_rowptr[i] + lane; k < a.bsr_rowptr[i + 1]; k += 64) {
+        const int j = a.bsr_col[k];
+        if (j == i) continue;
+        const int64_t v0 = a.bsr_voff[k];
+        const int n = di * a.dims[j];
+        for (int q = 0; q < n; q++) {
+            const double x = S[map[v0 + q] - 1];
+            off += x * x;
+        }
+    }
+    off = wave_sum64(off);
+    if (lane != 0) return;
+    float *f = F + i * PC_NF;
+    for (int q = 0; q < PC_NF; q++) f[q] = 0.0f;
+    if (di != 6) return;
+    double D[36], m = 0.0, dd = 0.0;
+    for (int q = 0; q < 36; q++) {
+        D[q] = S[map[diag_voff[i] + q] - 1];   // row-major
+        dd += D[q] * D[q];
+    }
+    for (int r = 0; r < 6; r++) m += D[r * 6 + r];
+    m /= 6.0;
+    const double mm = m > 0 ? m : 1e-300;
+    int q = 0;
+    for (int c = 0; c < 6; c++)
+        for (int r = 0; r <= c; r++) f[q++] = (float)(D[r * 6 + c] / mm);
+    f[21] = (float)log10(mm);
+    f[22] = (float)log(1.0 + (double)(a.bsr_rowptr[i + 1] - a.bsr_rowptr[i] - 1));
+    f[23] = (float)log10(lambda > 0 ? lambda : 1e-300);
+    f[24] = (float)log10(mm / (lambda > 0 ? lambda : 1e-300));
+    f[25] = (float)log10((sqrt(off) + 1e-300) / (sqrt(dd) + 1e-300));
+}
+
+// M_b = D_b + mu_b m_b I, full d x d column-major at the preconditioner offsets (the precond_blocks of the solve)
+__global__ void k_pcg_shift_blocks(PcgArgs a, const double *S, const int64_t *map, const int64_t *diag_voff,
+                                   const float *mu, double *M) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.nrb) return;
+    const int d = a.dims[b];
+    double m = 0.0;
+    for (int r = 0; r < d; r++) m += S[map[diag_voff[b] + r * d + r] - 1];
+    m /= d;
+    const double shift = (double)mu[b] * m;
+    double *out = M + a.loff[b];
+    for (int c = 0; c < d; c++)
+        for (int r = 0; r < d; r++) out[c * d + r] = S[map[diag_voff[b] + r * d + c] - 1] + (r == c ? shift : 0.0);
+}
+
 __global__ void k_gather_f64(double *dest, const double *src, const int64_t *map, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dest[i] = src[map[i] - 1];
@@ -474,6 +539,26 @@ extern "C" int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dim
 }
 
 extern "C" void bls_pcg_destroy(bls_pcg *h) { delete h; }
+
+extern "C" int bls_pcg_features_f32(bls_pcg *h, const double *S_nzval, double lambda, float *features,
+                                    void *stream) {
+    if (!h || !S_nzval || !features) return fail(BLS_ERR_ARG, "bls_pcg_features_f32: bad arguments");
+    PcgArgs a = h->args(1.0, 0);
+    k_pcg_features<<<(unsigned)((h->nrb + 3) / 4), 256, 0, (hipStream_t)stream>>>(a, S_nzval, h->d_map.ptr,
+                                                                                 h->d_diag_voff.ptr, lambda, features);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_pcg_shift_blocks_f64(bls_pcg *h, const double *S_nzval, const float *mu, double *blocks,
+                                        void *stream) {
+    if (!h || !S_nzval || !mu || !blocks) return fail(BLS_ERR_ARG, "bls_pcg_shift_blocks_f64: bad arguments");
+    PcgArgs a = h->args(1.0, 0);
+    k_pcg_shift_blocks<<<grid_for(h->nrb, 256), 256, 0, (hipStream_t)stream>>>(a, S_nzval, h->d_map.ptr,
+                                                                            h->d_diag_voff.ptr, mu, blocks);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
 
 extern "C" int bls_pcg_values_size(bls_pcg *h, int64_t *nvals, int64_t *nfactor) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");

@@ -60,6 +60,12 @@ SIGNATURES = {
     "bls_reduce_qr_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP]),
     "bls_qr_set_timing": (ct.c_int, [VP, ct.c_int]),
     "bls_qr_kernel_time": (ct.c_int, [VP, ct.POINTER(ct.c_float)]),
+    "bls_pcg_features_f32": (ct.c_int, [VP, VP, ct.c_double, VP, VP]),
+    "bls_pcg_shift_blocks_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_mlp_dims": (ct.c_int, [ct.POINTER(ct.c_int)] * 4),
+    "bls_mlp_forward_f32": (ct.c_int, [VP, VP, VP, VP, ct.c_int64, VP, VP]),
+    "bls_mlp_grad_f32": (ct.c_int, [VP, VP, VP, VP, VP, ct.c_int64, VP, VP, VP, VP]),
+    "bls_mlp_adam_f32": (ct.c_int, [VP, VP, VP, VP, ct.c_int64, ct.c_float, VP]),
     "bls_schur_kernel_times": (ct.c_int, [VP, ct.POINTER(ct.c_float), ct.c_int]),
 }
 

@@ -186,13 +186,17 @@ def config5_problem(i, seed=5, scale=1.0, ncam=None, nland=None):
     return _hessian_from_tracks(rng, ncam, np.full(ncam, 6, dtype=I64), nland, k, ptr, cams, 1e-4, False)
 
 
-def config5_union(seed=5, scale=1.0, nprob=64):
+def config5_union(seed=5, scale=1.0, nprob=64, ncams=None, nlands=None):
     """The batched-serving configuration as one structure: the nprob problems' cameras and landmarks side by side,
     so that S is block diagonal over problems and one reduction (one launch per kernel) and one batched PCG cover
     them all. Problem i has config5_sizes' camera count and scale x its landmark count; its tracks come from the
     stream [seed, i] as in config5_problem, the Jacobian values from one stream [seed, nprob]. Adds
-    `prob_block_ptr`: the 1-based first camera block of each problem, then nrblocks + 1."""
-    ncams, nlands = config5_sizes(seed, nprob)
+    `prob_block_ptr`: the 1-based first camera block of each problem, then nrblocks + 1. `ncams` / `nlands`
+    replace the generated sizes (small test instances)."""
+    c5, l5 = config5_sizes(seed, nprob)
+    ncams = c5 if ncams is None else np.asarray(ncams)
+    nlands = l5 if nlands is None else np.asarray(nlands)
+    nprob = len(ncams)
     ks, cams_all, coff = [], [], 0
     for i in range(nprob):
         rng = np.random.default_rng([seed, i])

@@ -168,6 +168,30 @@ int bls_pcg_values_size(bls_pcg *h, int64_t *nvals, int64_t *nfactor);
 int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol, int64_t maxit,
                       const double *precond_blocks, int64_t *iters, double *relres, void *stream);
 
+/* ---- learned correction of the block-Jacobi preconditioner (no reference counterpart) ------------------ */
+
+/* Per retained block of a PCG handle's reduced system (device S in build_schur_pattern's layout), 26 FP32
+ * features (nrblocks x 26, row-major) for the MLP: the 21 upper entries of the diagonal block D divided by the mean
+ * m of its diagonal, log10 m, log(1 + off-diagonal blocks in the block row), log10 lambda, log10(m / lambda),
+ * log10(off-diagonal Frobenius mass of the block row / ||D||_F). Blocks of dimension other than 6 get zeros. */
+int bls_pcg_features_f32(bls_pcg *h, const double *S_nzval, double lambda, float *features, void *stream);
+/* The corrected preconditioner blocks M_b = D_b + mu_b m_b I (mu: nrblocks FP32 shifts, >= 0 keeps M_b SPD when D_b
+ * is), full d x d column-major (sum d^2 values): the precond_blocks argument of bls_pcg_solve_f64. */
+int bls_pcg_shift_blocks_f64(bls_pcg *h, const double *S_nzval, const float *mu, double *blocks, void *stream);
+
+/* The MLP (26 -> 64 -> 64 -> 1, ReLU, FP32): sizes of its feature vector, hidden layers, flat parameter vector
+ * (W1 [26][64], b1, W2 [64][64], b2, W3 [64], b3) and per-sample work space of bls_mlp_grad_f32. */
+int bls_mlp_dims(int *nfeatures, int *nhidden, int *nparams, int *work_per_sample);
+/* out[s] = MLP((X[s] - fmean) / fstd) for n samples (X: n x 26 row-major). */
+int bls_mlp_forward_f32(const float *params, const float *fmean, const float *fstd, const float *X, int64_t n,
+                        float *out, void *stream);
+/* Outputs and the gradient of L = mean_s (out[s] - y[s])^2 / 2 with respect to the parameters (deterministic:
+ * every gradient entry is summed in sample order). work: n * work_per_sample floats of scratch. */
+int bls_mlp_grad_f32(const float *params, const float *fmean, const float *fstd, const float *X, const float *y,
+                     int64_t n, float *out, float *grad, float *work, void *stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected for step >= 1) on the parameters. */
+int bls_mlp_adam_f32(float *params, const float *grad, float *m, float *v, int64_t step, float lr, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

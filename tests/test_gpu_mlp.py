@@ -1,0 +1,172 @@
+"""GPU tests of the learned block-Jacobi correction: the HIP MLP forward/backward/Adam against the CPU oracle
+(oracle/mlp_ref.py, itself pinned by finite differences), deterministic gradients, training that reduces the loss,
+features and corrected blocks against their definitions on a dense S, the learned solve with its plain fallback,
+and what the trained network gains over plain block-Jacobi on held-out problems.
+
+Tolerances: FP32 network against the float64 oracle, 1e-4 relative (outputs, gradients per parameter group)."""
+import numpy as np
+import pytest
+
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(blsamd, seed=0):
+    P = blsamd.precond
+    rng = np.random.default_rng(seed)
+    nf = mlp_ref.NF
+    return P.MLP.initial(rng.standard_normal(nf) * 0.1, 0.5 + rng.random(nf), seed=seed)
+
+
+def test_forward_and_gradient_match_oracle(blsamd):
+    import torch
+    m = _model(blsamd)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((777, mlp_ref.NF)).astype(np.float32)
+    y = rng.standard_normal(777).astype(np.float32)
+    Xd, yd = torch.as_tensor(X).cuda(), torch.as_tensor(y).cuda()
+    p, fm, fs = (t.cpu().numpy().astype(np.float64) for t in (m.params, m.fmean, m.fstd))
+    o_ref, _ = mlp_ref.forward(p, fm, fs, X)
+    o = m.forward(Xd).cpu().numpy()
+    assert np.abs(o - o_ref).max() <= 1e-4 * max(1.0, np.abs(o_ref).max())
+    out, G = m.grad(Xd, yd)
+    g_ref = mlp_ref.grad(p, fm, fs, X, y)
+    G = G.cpu().numpy()
+    o1 = mlp_ref.NF * mlp_ref.NH
+    o2 = o1 + mlp_ref.NH
+    o3 = o2 + mlp_ref.NH * mlp_ref.NH
+    for a, b in [(0, o1), (o1, o2), (o2, o3), (o3, len(G))]:
+        assert np.linalg.norm(G[a:b] - g_ref[a:b]) <= 1e-4 * np.linalg.norm(g_ref[a:b]), (a, b)
+    # deterministic
+    _, G2 = m.grad(Xd, yd)
+    assert np.array_equal(G2.cpu().numpy(), G)
+
+
+def test_adam_step_matches_oracle(blsamd):
+    import torch
+    import ctypes as ct
+    dev = blsamd.device
+    m = _model(blsamd, 1)
+    rng = np.random.default_rng(4)
+    X = torch.as_tensor(rng.standard_normal((300, mlp_ref.NF)).astype(np.float32)).cuda()
+    y = torch.as_tensor(rng.standard_normal(300).astype(np.float32)).cuda()
+    _, G = m.grad(X, y)
+    p0, g = m.params.cpu().numpy().astype(np.float64), G.cpu().numpy().astype(np.float64)
+    mm = torch.zeros_like(m.params)
+    vv = torch.zeros_like(m.params)
+    for step in (1, 2):
+        dev._check(dev.lib().bls_mlp_adam_f32(dev._dp(m.params), dev._dp(G), dev._dp(mm), dev._dp(vv), step,
+                                              ct.c_float(1e-3), None))
+    pr, mr, vr = mlp_ref.adam(p0, g, np.zeros_like(p0), np.zeros_like(p0), 1, 1e-3)
+    pr, mr, vr = mlp_ref.adam(pr, g, mr, vr, 2, 1e-3)
+    assert np.abs(m.params.cpu().numpy() - pr).max() <= 1e-6
+
+
+def test_training_reduces_loss(blsamd):
+    import torch
+    P = blsamd.precond
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((20000, mlp_ref.NF)).astype(np.float32)
+    y = (np.tanh(X[:, 0]) + 0.5 * X[:, 3] * X[:, 7]).astype(np.float32)
+    m = P.MLP.initial(X.mean(0), X.std(0), seed=2)
+    losses = P.train(m, torch.as_tensor(X).cuda(), torch.as_tensor(y).cuda(), steps=300, batch=4096, lr=3e-3)
+    assert losses[-1] < 0.1 * losses[0]
+
+
+def _small_reduced(blsamd, seed=9):
+    import torch
+    p = blsamd.generators.config5_union(seed=seed, ncams=[40, 75, 52], nlands=[900, 300, 2000])
+    dev = blsamd.device
+    st = p.structure
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=p.r_to_s)
+    ws = dev.SchurReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    dev.reset_schur_sparse_(S, T(p.R_data), h, T(p.scaling_r), p.lam)
+    assert dev.reduce_schur_sparse_(S, g, T(p.b_r), T(p.b_e), T(p.scaling_e), p.lam, h, T(p.E_data), T(p.C_data), ws)
+    pcg = dev.PCGHandle(st, p.pattern, p.prob_block_ptr)
+    return p, S, g, pcg
+
+
+def _dense(p, S):
+    st, pat = p.structure, p.pattern
+    M = np.zeros((st.nr, st.nr))
+    for j in range(st.nr):
+        for k in range(pat.S_colptr[j] - 1, pat.S_colptr[j + 1] - 1):
+            M[pat.S_rowval[k] - 1, j] = S[k]
+    return np.triu(M) + np.triu(M, 1).T
+
+
+def test_features_and_shift_blocks_match_definitions(blsamd):
+    import torch
+    P = blsamd.precond
+    p, S, g, pcg = _small_reduced(blsamd)
+    st = p.structure
+    Sd = _dense(p, S.cpu().numpy())
+    bc = p.pattern.block_colptr
+    nblk = np.zeros(st.nrblocks, dtype=np.int64)
+    for c in range(st.nrblocks):           # stored blocks of each block row of the full symmetric matrix
+        for k in range(bc[c] - 1, bc[c + 1] - 1):
+            r = p.pattern.block_rowval[k] - 1
+            nblk[c] += 1
+            if r != c:
+                nblk[r] += 1
+    F = P.features(pcg, S, p.lam).cpu().numpy()
+    Fr = mlp_ref.features(Sd, st.r_dims, st.boffsets_r, nblk, p.lam)
+    assert np.allclose(F, Fr, rtol=1e-5, atol=1e-5)
+    mu = torch.as_tensor(np.random.default_rng(0).random(st.nrblocks).astype(np.float32) * 0.2).cuda()
+    M = P.shift_blocks(pcg, S, mu).cpu().numpy()
+    o = 0
+    for b in range(st.nrblocks):
+        d, ro = st.r_dims[b], st.boffsets_r[b] - 1
+        ref = mlp_ref.shift_block(Sd[ro:ro + d, ro:ro + d], float(mu[b]))
+        assert np.allclose(M[o:o + d * d].reshape(d, d).T, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+        o += d * d
+
+
+def test_learned_solve_and_fallback(blsamd):
+    import torch
+    P = blsamd.precond
+    p, S, g, pcg = _small_reduced(blsamd)
+    x = torch.zeros_like(g)
+    ok, iters, rel, used = P.solve(pcg, S, g, x, p.lam, model=_model(blsamd))
+    assert ok and used and (rel <= 1e-8).all()
+    Sd = _dense(p, S.cpu().numpy())
+    gn = g.cpu().numpy()
+    assert np.linalg.norm(Sd @ x.cpu().numpy() - gn) <= 1.0001e-8 * np.linalg.norm(gn) * np.sqrt(pcg.nprob)
+    # an iteration cap the learned solve cannot meet: it falls back to plain block-Jacobi (which fails too here)
+    ok2, _, _, used2 = P.solve(pcg, S, g, x, p.lam, model=_model(blsamd), maxit=1)
+    assert not ok2 and not used2
+
+
+def test_trained_network_against_plain(blsamd):
+    """What the committed network gains over plain block-Jacobi on problems it was not trained or calibrated on (the
+    64-problem generator's own seed, landmarks at 1/20 so that block-Jacobi needs hundreds of iterations): max and
+    mean PCG iterations to 1e-8, printed. The network's output scale was chosen on held-out problems so that it does
+    not lose to plain block-Jacobi there; here it must stay within 3 % of plain on the mean."""
+    import os
+    import torch
+    P = blsamd.precond
+    if not os.path.exists(P.WEIGHTS):
+        pytest.skip("no trained weights (blocklinearsystems.jl_amd/train_precond.py writes them)")
+    model = P.MLP.load()
+    p = blsamd.generators.config5_union(seed=5, scale=0.05, nprob=16)
+    dev = blsamd.device
+    st = p.structure
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=p.r_to_s)
+    ws = dev.SchurReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    dev.reset_schur_sparse_(S, T(p.R_data), h, T(p.scaling_r), p.lam)
+    assert dev.reduce_schur_sparse_(S, g, T(p.b_r), T(p.b_e), T(p.scaling_e), p.lam, h, T(p.E_data), T(p.C_data), ws)
+    pcg = dev.PCGHandle(st, p.pattern, p.prob_block_ptr)
+    x = torch.zeros_like(g)
+    ok0, it0, _, _ = P.solve(pcg, S, g, x, p.lam, model=None, maxit=20000)
+    ok1, it1, _, used = P.solve(pcg, S, g, x, p.lam, model=model, maxit=20000)
+    assert ok0 and ok1 and used
+    print(f"plain block-Jacobi iterations max {it0.max()} mean {it0.mean():.1f}; learned max {it1.max()} "
+          f"mean {it1.mean():.1f}")
+    assert it1.mean() <= 1.03 * it0.mean()
