@@ -232,10 +232,18 @@ constexpr int P2_G = 4;                        // landmarks whose C_c1 columns a
 constexpr int P2_WORDS = P2_STAGE / 64;
 constexpr int P2_PART = 384;                   // landmarks per workgroup before an item is split into parts
 
-// staged doubles per lane per batch: four lanes per record of at most DE^2 + DE + DE*DRM doubles, or P2_Q
+// Loaded doubles per lane per batch. One landmark dimension: four lanes per landmark, lane 0 loading E_inv and z
+// (DE^2 + DE), lanes 1-3 every third column of C_c2 (DE each, up to DRM columns); mixed dimensions: P2_Q.
 template <int DE, int DRM> struct P2Width {
-    static constexpr int value = DE > 0 ? (DE * DE + DE + DE * DRM + 3) / 4 : P2_Q;
+    static constexpr int c2cols = (DRM + 2) / 3;
+    static constexpr int value = DE > 0 ? (DE * DE + DE > DE * c2cols ? DE * DE + DE : DE * c2cols) : P2_Q;
 };
+
+// Doubles of a landmark's staged record: with one landmark dimension [W2 = E_inv C_c2 (DE x d_r2) | gz = C_c2' z
+// (d_r2)], formed by the staging lanes; with mixed dimensions the raw [E_inv | z | C_c2].
+template <int DE> __device__ __host__ inline int p2_record(int de, int dr2) {
+    return DE > 0 ? (DE + 1) * dr2 : de * de + de + de * dr2;
+}
 
 struct P2Meta {
     int nb;
@@ -291,7 +299,7 @@ __device__ inline int p2_process(const P2Args &a, const P2Item &it, P2Meta &m, b
             sc2 = a.coffsets[en.c2];
         }
     }
-    const int size = de * de + de + de * it.dr2;
+    const int size = p2_record<DE>(de, it.dr2);
     int nb, excl;
     if constexpr (DE > 0) {
         nb = __popcll(__ballot(valid && lane < nbmax));
@@ -349,18 +357,20 @@ __device__ inline void p2_issue(const P2Args &a, const P2Meta &m, int t, int dr2
     constexpr int QN = P2Width<DE, DRM>::value;
     const double *src[QN];
     if constexpr (DE > 0) {
-        // lanes 4j..4j+3 stage record j: lane part takes doubles part, part + 4, ...
-        const int size = DE * DE + DE + DE * dr2;
+        // lanes 4j..4j+3 load landmark j: lane 4j its E_inv then z, lane 4j+p (p = 1..3) the C_c2 columns
+        // p-1, p+2, ... (DE doubles each)
         const int j = t >> 2, part = t & 3;
         const bool has = j < m.nb;
         const int jj = has ? j : 0;
         const int64_t se = m.src_e[jj], sz = m.src_z[jj], sc = m.src_c2[jj];
 #pragma unroll
         for (int k = 0; k < QN; k++) {
-            const int r = part + 4 * k;
-            const double *sp = r < DE * DE ? a.E_inv + se + r
-                                           : (r < DE * DE + DE ? a.z + sz + (r - DE * DE) : a.C + sc + (r - DE * DE - DE));
-            src[k] = has && r < size ? sp : a.C;
+            const int c = (part - 1) + 3 * (k / DE);   // C_c2 column of this double (lanes 1-3)
+            const double *s0 = k < DE * DE ? a.E_inv + se + k : a.z + sz + (k - DE * DE);
+            const double *s1 = a.C + sc + c * DE + (k % DE);
+            const bool ok0 = part == 0 && k < DE * DE + DE;
+            const bool ok1 = part != 0 && c < dr2;
+            src[k] = has && (ok0 || ok1) ? (part == 0 ? s0 : s1) : a.C;
         }
     } else {
         const int L = m.off[m.nb];
@@ -386,13 +396,35 @@ template <int DE, int DRM>
 __device__ inline void p2_store(const P2Meta &m, int t, int dr2, const double *v, double *stage) {
     constexpr int QN = P2Width<DE, DRM>::value;
     if constexpr (DE > 0) {
-        const int size = DE * DE + DE + DE * dr2;
+        // lanes 4j+1..4j+3 take E_inv and z from lane 4j and write W2 = E_inv C_c2 and gz = C_c2' z for their
+        // columns of C_c2 into record j (the reference's W_data for the coupling of the block column,
+        // src/schur_reduction.jl:149-160, and its g term)
+        constexpr int CC = P2Width<DE, DRM>::c2cols;
         const int j = t >> 2, part = t & 3;
-        if (j < m.nb) {
+        const int src_lane = (t & 63) & ~3;
+        double ez[DE * DE + DE];
 #pragma unroll
-            for (int k = 0; k < QN; k++) {
-                const int r = part + 4 * k;
-                if (r < size) stage[j * size + r] = v[k];
+        for (int q = 0; q < DE * DE + DE; q++) ez[q] = __shfl(v[q], src_lane, 64);
+        if (j < m.nb && part != 0) {
+            const int size = (DE + 1) * dr2;
+            double *rec = stage + j * size;
+#pragma unroll
+            for (int cc = 0; cc < CC; cc++) {
+                const int c = (part - 1) + 3 * cc;
+                if (c < dr2) {
+                    const double *col = v + cc * DE;
+                    double gz = 0.0;
+#pragma unroll
+                    for (int i = 0; i < DE; i++) gz += col[i] * ez[DE * DE + i];
+#pragma unroll
+                    for (int k = 0; k < DE; k++) {
+                        double w = 0.0;
+#pragma unroll
+                        for (int i = 0; i < DE; i++) w += ez[k + i * DE] * col[i];
+                        rec[c * DE + k] = w;
+                    }
+                    rec[DE * dr2 + c] = gz;
+                }
             }
         }
     } else {
@@ -456,7 +488,7 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
     }
     const bool diag = it.has_diag && my_slot == it.nslots - 1;
     const int dr2 = it.dr2;
-    const int size = DE * DE + DE + DE * dr2;   // staged record (one landmark dimension)
+    const int size = p2_record<DE>(DE, dr2);   // staged record (one landmark dimension)
     const int nbmax = DE > 0 ? (P2_STAGE / size < P2_MAXE ? P2_STAGE / size : P2_MAXE) : P2_MAXE;
     double acc[DRM];
     double gacc = 0.0;
@@ -550,31 +582,18 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
                 const int j = jj[gi];
                 if (j < 0) continue;
                 if constexpr (DE > 0) {
-                    // every column is accumulated (a column past d_r2, or below the diagonal of the diagonal
-                    // block, collects values that are never written) and the g term is formed on every lane
-                    // (only the diagonal slot's is kept): no per-column selects
-                    const double *Ei = stg + j * size;
-                    const double *zz = Ei + DE * DE;
-                    const double *C2 = zz + DE;
-                    double w[DE];
-#pragma unroll
-                    for (int i = 0; i < DE; i++) {
-                        double s = 0.0;
-#pragma unroll
-                        for (int k = 0; k < DE; k++) s += Ei[i + k * DE] * cv[gi][k];
-                        w[i] = s;
-                    }
+                    // S[row, c] -= C_c1[:, row] . W2[:, c] for every column (a column past d_r2, or below the
+                    // diagonal of the diagonal block, collects values that are never written); g[row] -= gz[row]
+                    // on the diagonal slot
+                    const double *W2 = stg + j * size;
 #pragma unroll
                     for (int c = 0; c < DRM; c++) {
                         double s = 0.0;
 #pragma unroll
-                        for (int i = 0; i < DE; i++) s += w[i] * C2[c * DE + i];
+                        for (int k = 0; k < DE; k++) s += cv[gi][k] * W2[c * DE + k];
                         acc[c] -= s;
                     }
-                    double sg = 0.0;
-#pragma unroll
-                    for (int i = 0; i < DE; i++) sg += C2[my_row * DE + i] * zz[i];
-                    gacc -= sg;
+                    if (diag) gacc -= W2[DE * dr2 + my_row];
                 } else {
                     const double *Ei = stg + m.off[j];
                     const int de = m.de[j];
