@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibration-copy", action="store_true")
+    ap.add_argument("--qr-scale", type=float, default=0.2,
+                    help="landmark scale of the reduce_qr! leg (500 cameras d 6/9, 500000 landmarks x scale); 0 skips")
     ap.add_argument("--config5-scale", type=float, default=1.0,
                     help="landmark scale of the batched 64-problem leg (cameras at full size); 0 skips it")
     args = ap.parse_args()
@@ -147,13 +149,51 @@ def main():
                      "step_achieved": whole_bytes / (ms * 1e-3) / 1e9},
         "pcie_inclusive": pcie,
     }
-    if args.config5_scale > 0:
-        del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
+    del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
+    torch.cuda.empty_cache()
+    if args.qr_scale > 0:
+        result["reduce_qr"] = qr_leg(torch, dev, b, args.qr_scale)
         torch.cuda.empty_cache()
+    if args.config5_scale > 0:
         result["batched_pcg"] = batched_pcg_leg(torch, dev, b, args.config5_scale)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)
     print(json.dumps(result), flush=True)
+
+
+def qr_leg(torch, dev, b, scale, reps=10):
+    """reduce_qr! on the Jacobian workload: 500 cameras (d 6 or 9, 50/50), 500000 x scale landmarks (d 3), track
+    length U{2..30}, 2 x d residual blocks N(0,1), lambda 1e-4, FP64. Time of the table kernel (HIP events) and its
+    HBM figure: per table 2 * 8 M C bytes (raw in, factored out) + 16 M (r in, fres out)."""
+    t0 = time.time()
+    p = b.generators.config4(scale=scale)
+    jst, st = p.jst, p.structure
+    h = dev.QRHandle(jst)
+    ws = dev.QRReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    raw, r, sc = T(p.data), T(p.r), T(p.scaling_e)
+    log(f"QR leg setup: {jst.ntables} tables ({st.neblocks} landmarks), {st.nrblocks} cameras, raw "
+        f"{jst.raw_length() * 8 / 1e9:.2f} GB ({time.time() - t0:.1f}s)")
+    lam = 1e-4
+    assert dev.reduce_qr_(ws, h, raw, r, sc, lam)
+    h.set_timing(True)
+    times = []
+    for _ in range(reps):
+        assert dev.reduce_qr_(ws, h, raw, r, sc, lam, check=False)
+        times.append(h.kernel_time())
+    h.set_timing(False)
+    ms = float(np.median(times))
+    M = jst.table_nrows.astype(np.float64)
+    Cc = (jst.table_ncols - 1).astype(np.float64)
+    nbytes = float(np.sum(16 * M * Cc + 16 * M))
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"QR leg: {ms:.3f} ms, {st.neblocks / (ms * 1e-3):.4g} landmarks/s, {gbs:.0f} GB/s "
+        f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM)")
+    return {"landmark_scale": scale, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
+            "cameras": int(st.nrblocks), "raw_GB": jst.raw_length() * 8 / 1e9, "ms": ms,
+            "landmarks_per_s": st.neblocks / (ms * 1e-3),
+            "roofline": {"bound": "hbm", "kernel": "k_qr_tables", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
 
 
 def batched_pcg_leg(torch, dev, b, scale, reps=3):
