@@ -154,6 +154,114 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
     if (__any(isnan(bad)) && lane == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Products with the reduced Jacobian J~ (the rows below U in every factored table; src/qr_reduced.jl:161-243,
+// :484-512). Table t contributes n_t = M_t - top_t reduced rows at reduced_offsets[t]; its entry u holds retained
+// block entry_retained[u] at table column entry_col[u].
+// ---------------------------------------------------------------------------------------------------------
+struct RedArgs {
+    int64_t ntables, nrb;
+    const int32_t *nrows, *top;
+    const int64_t *raw_off, *res_off, *red_off;
+    const int32_t *table_ptr, *entry_retained, *entry_col, *entry_table;
+    const int32_t *retained_ptr, *retained_entry;
+    const int32_t *r_dims;
+    const int64_t *boff_r, *doff;
+};
+
+// y = J~ v (reduced_mul!, src/qr_reduced.jl:161-193): one wavefront per table, lane i its reduced row i; sums over
+// the table's entries and their columns in the reference's order
+__global__ __launch_bounds__(256) void k_red_mul(RedArgs a, const double *F, const double *v, double *y) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= a.ntables) return;
+    const int M = a.nrows[t], top = a.top[t], n = M - top;
+    for (int i = lane; i < n; i += 64) {
+        double acc = 0.0;
+        for (int u = a.table_ptr[t]; u < a.table_ptr[t + 1]; u++) {
+            const int r = a.entry_retained[u];
+            const int64_t cb = a.raw_off[t] + (int64_t)a.entry_col[u] * M + top + i;
+            const double *vr = v + a.boff_r[r];
+            for (int j = 0; j < a.r_dims[r]; j++) acc += F[cb + (int64_t)j * M] * vr[j];
+        }
+        y[a.red_off[t] + i] = acc;
+    }
+}
+
+// r~ (reduced_residual!, src/qr_reduced.jl:227-243): the residual rows below the top of each table
+__global__ __launch_bounds__(256) void k_red_residual(RedArgs a, const double *Fr, double *rt) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= a.ntables) return;
+    const int top = a.top[t], n = a.nrows[t] - top;
+    for (int i = lane; i < n; i += 64) rt[a.red_off[t] + i] = Fr[a.res_off[t] + top + i];
+}
+
+// One workgroup per retained block r: `npair` quantities, each a sum over the tables coupling r. Thread t takes
+// quantity t % npair and every nch-th coupling from t / npair (nch = 256 / npair); the nch partial sums are then
+// added in order, so the result is deterministic.
+constexpr int RED_THREADS = 256;
+
+// g = J~' w (reduced_tmul!, src/qr_reduced.jl:195-223)
+__global__ __launch_bounds__(RED_THREADS) void k_red_tmul(RedArgs a, const double *F, const double *w, double *g) {
+    __shared__ double part[RED_THREADS];
+    const int r = blockIdx.x;
+    const int dr = a.r_dims[r];
+    const int nch = RED_THREADS / dr;
+    const int j = threadIdx.x % dr, ch = threadIdx.x / dr;
+    double acc = 0.0;
+    if (ch < nch) {
+        for (int k = a.retained_ptr[r] + ch; k < a.retained_ptr[r + 1]; k += nch) {
+            const int u = a.retained_entry[k];
+            const int t = a.entry_table[u];
+            const int M = a.nrows[t], top = a.top[t], n = M - top;
+            const double *col = F + a.raw_off[t] + (int64_t)(a.entry_col[u] + j) * M + top;
+            const double *wt = w + a.red_off[t];
+            for (int i = 0; i < n; i++) acc += col[i] * wt[i];
+        }
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < dr) {
+        double s = 0.0;
+        for (int c = 0; c < nch; c++) s += part[c * dr + threadIdx.x];
+        g[a.boff_r[r] + threadIdx.x] = s;
+    }
+}
+
+// diagonal blocks of J~'J~ (reduced_diagonal_blocks!, src/qr_reduced.jl:484-512), full d x d column-major at doff
+__global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const double *F, double *D) {
+    __shared__ double part[RED_THREADS];
+    const int r = blockIdx.x;
+    const int dr = a.r_dims[r];
+    const int npair = dr * (dr + 1) / 2;
+    const int nch = RED_THREADS / npair;
+    const int pr = threadIdx.x % npair, ch = threadIdx.x / npair;
+    int col = 0;
+    while ((col + 1) * (col + 2) / 2 <= pr) col++;
+    const int row = pr - col * (col + 1) / 2;     // row <= col
+    double acc = 0.0;
+    if (ch < nch) {
+        for (int k = a.retained_ptr[r] + ch; k < a.retained_ptr[r + 1]; k += nch) {
+            const int u = a.retained_entry[k];
+            const int t = a.entry_table[u];
+            const int M = a.nrows[t], top = a.top[t], n = M - top;
+            const double *cb = F + a.raw_off[t] + (int64_t)a.entry_col[u] * M + top;
+            const double *x = cb + (int64_t)row * M, *y = cb + (int64_t)col * M;
+            for (int i = 0; i < n; i++) acc += x[i] * y[i];
+        }
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < npair) {
+        double s = 0.0;
+        for (int c = 0; c < nch; c++) s += part[c * npair + threadIdx.x];
+        double *Db = D + a.doff[r];
+        Db[col * dr + row] = s;
+        Db[row * dr + col] = s;
+    }
+}
+
 __global__ void k_qr_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 __global__ void k_qr_finalize(const unsigned long long *flag, int64_t *out) {
     const unsigned long long f = *flag;
@@ -171,6 +279,32 @@ struct bls_qr {
     DevArray<unsigned long long> d_flag;
     DevArray<int64_t> d_failed_out;
     Timing timing;
+    // reduced-Jacobian structure (bls_qr_set_reduced)
+    bool has_reduced = false;
+    int64_t red_len = 0, dlen = 0;
+    DevArray<int32_t> d_top, d_table_ptr, d_entry_retained, d_entry_col, d_entry_table, d_retained_ptr,
+        d_retained_entry, d_r_dims;
+    DevArray<int64_t> d_red_off, d_boff_r, d_doff;
+    RedArgs red() const {
+        RedArgs a{};
+        a.ntables = ntables;
+        a.nrb = nrb;
+        a.nrows = d_nrows.ptr;
+        a.top = d_top.ptr;
+        a.raw_off = d_raw_off.ptr;
+        a.res_off = d_res_off.ptr;
+        a.red_off = d_red_off.ptr;
+        a.table_ptr = d_table_ptr.ptr;
+        a.entry_retained = d_entry_retained.ptr;
+        a.entry_col = d_entry_col.ptr;
+        a.entry_table = d_entry_table.ptr;
+        a.retained_ptr = d_retained_ptr.ptr;
+        a.retained_entry = d_retained_entry.ptr;
+        a.r_dims = d_r_dims.ptr;
+        a.boff_r = d_boff_r.ptr;
+        a.doff = d_doff.ptr;
+        return a;
+    }
 };
 
 namespace {
@@ -269,6 +403,92 @@ extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, con
 }
 
 extern "C" void bls_qr_destroy(bls_qr *h) { delete h; }
+
+extern "C" int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int64_t *reduced_offsets,
+                                  const int64_t *table_ptr, int64_t nentries, const int64_t *entry_retained,
+                                  const int64_t *entry_col, const int64_t *entry_table, const int64_t *retained_ptr,
+                                  const int64_t *retained_entry, const int64_t *r_dims, const int64_t *boffsets_r,
+                                  const int64_t *doffsets) {
+    if (!h || !table_top || !reduced_offsets || !table_ptr || nentries < 0 || !r_dims || !boffsets_r || !doffsets ||
+        (nentries && (!entry_retained || !entry_col || !entry_table || !retained_ptr || !retained_entry)))
+        return fail(BLS_ERR_ARG, "bls_qr_set_reduced: bad arguments");
+    if (nentries > INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "more than 2^31 table entries");
+    const int64_t nt = h->ntables, nrb = h->nrb;
+    std::vector<int32_t> top(nt), tp(nt + 1), er(nentries), ec(nentries), et(nentries), rp(nrb + 1), re(nentries),
+        rd(nrb);
+    std::vector<int64_t> ro(nt + 1), bo(nrb), dof(nrb);
+    for (int64_t t = 0; t < nt; t++) top[t] = (int32_t)table_top[t];
+    for (int64_t t = 0; t <= nt; t++) { tp[t] = (int32_t)(table_ptr[t] - 1); ro[t] = reduced_offsets[t] - 1; }
+    for (int64_t u = 0; u < nentries; u++) {
+        er[u] = (int32_t)(entry_retained[u] - 1);
+        ec[u] = (int32_t)(entry_col[u] - 1);
+        et[u] = (int32_t)(entry_table[u] - 1);
+        re[u] = (int32_t)(retained_entry[u] - 1);
+        if (er[u] < 0 || er[u] >= nrb || et[u] < 0 || et[u] >= nt || re[u] < 0 || re[u] >= nentries)
+            return fail(BLS_ERR_ARG, "bls_qr_set_reduced: entry index out of range");
+    }
+    for (int64_t r = 0; r <= nrb; r++) rp[r] = (int32_t)(retained_ptr[r] - 1);
+    int64_t dl = 0;
+    for (int64_t r = 0; r < nrb; r++) {
+        rd[r] = (int32_t)r_dims[r];
+        bo[r] = boffsets_r[r] - 1;
+        dof[r] = doffsets[r] - 1;
+        if (rd[r] < 1 || rd[r] > 16) return fail(BLS_ERR_UNSUPPORTED, "retained block dimension outside 1..16");
+        dl = std::max<int64_t>(dl, dof[r] + (int64_t)rd[r] * rd[r]);
+    }
+    int rc;
+    if ((rc = h->d_top.upload(top)) || (rc = h->d_table_ptr.upload(tp)) || (rc = h->d_entry_retained.upload(er)) ||
+        (rc = h->d_entry_col.upload(ec)) || (rc = h->d_entry_table.upload(et)) || (rc = h->d_retained_ptr.upload(rp)) ||
+        (rc = h->d_retained_entry.upload(re)) || (rc = h->d_r_dims.upload(rd)) || (rc = h->d_red_off.upload(ro)) ||
+        (rc = h->d_boff_r.upload(bo)) || (rc = h->d_doff.upload(dof)))
+        return rc;
+    h->red_len = ro[nt];
+    h->dlen = dl;
+    h->has_reduced = true;
+    return BLS_OK;
+}
+
+static int reduced_ready(bls_qr *h) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (!h->has_reduced) return fail(BLS_ERR_STATE, "no reduced-Jacobian structure (bls_qr_set_reduced)");
+    return BLS_OK;
+}
+
+extern "C" int bls_reduced_mul_f64(bls_qr *h, double *y, const double *F, const double *v, void *stream) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!y || !F || !v) return fail(BLS_ERR_ARG, "bls_reduced_mul_f64: a required array is NULL");
+    if (h->ntables) k_red_mul<<<(unsigned)((h->ntables + 3) / 4), 256, 0, (hipStream_t)stream>>>(h->red(), F, v, y);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_reduced_tmul_f64(bls_qr *h, double *g, const double *F, const double *w, void *stream) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!g || !F || !w) return fail(BLS_ERR_ARG, "bls_reduced_tmul_f64: a required array is NULL");
+    if (h->nrb) k_red_tmul<<<(unsigned)h->nrb, RED_THREADS, 0, (hipStream_t)stream>>>(h->red(), F, w, g);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *Fr, void *stream) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!rt || !Fr) return fail(BLS_ERR_ARG, "bls_reduced_residual_f64: a required array is NULL");
+    if (h->ntables) k_red_residual<<<(unsigned)((h->ntables + 3) / 4), 256, 0, (hipStream_t)stream>>>(h->red(), Fr, rt);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_reduced_diagonal_blocks_f64(bls_qr *h, double *D, const double *F, void *stream) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!D || !F) return fail(BLS_ERR_ARG, "bls_reduced_diagonal_blocks_f64: a required array is NULL");
+    if (h->nrb) k_red_diag<<<(unsigned)h->nrb, RED_THREADS, 0, (hipStream_t)stream>>>(h->red(), F, D);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
 
 extern "C" int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                                  const double *scaling_e, double lambda, const double *s, int64_t *failed_block,

@@ -60,6 +60,11 @@ SIGNATURES = {
     "bls_reduce_qr_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP]),
     "bls_qr_set_timing": (ct.c_int, [VP, ct.c_int]),
     "bls_qr_kernel_time": (ct.c_int, [VP, ct.POINTER(ct.c_float)]),
+    "bls_qr_set_reduced": (ct.c_int, [VP, P64, P64, P64, ct.c_int64] + [P64] * 8),
+    "bls_reduced_mul_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_reduced_tmul_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_reduced_residual_f64": (ct.c_int, [VP, VP, VP, VP]),
+    "bls_reduced_diagonal_blocks_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_pcg_features_f32": (ct.c_int, [VP, VP, ct.c_double, VP, VP]),
     "bls_pcg_shift_blocks_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_mlp_dims": (ct.c_int, [ct.POINTER(ct.c_int)] * 4),
@@ -270,6 +275,12 @@ class QRHandle:
                                    _p64(jst.raw_offsets, keep), _p64(jst.residual_offsets, keep), ct.byref(h)))
         self.h = h
         self.jst = jst
+        _check(lib().bls_qr_set_reduced(h, _p64(jst.table_top, keep), _p64(jst.reduced_offsets, keep),
+                                        _p64(jst.table_ptr, keep), len(jst.entry_retained),
+                                        _p64(jst.entry_retained, keep), _p64(jst.entry_col, keep),
+                                        _p64(jst.entry_table, keep), _p64(jst.retained_ptr, keep),
+                                        _p64(jst.retained_entry, keep), _p64(st.r_dims, keep),
+                                        _p64(st.boffsets_r, keep), _p64(jst.doffsets, keep)))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -355,3 +366,24 @@ class PCGHandle:
                                             _dp(precond_blocks), iters.ctypes.data_as(P64),
                                             rel.ctypes.data_as(ct.POINTER(ct.c_double)), _stream(stream)))
         return rc == BLS_OK, iters, rel
+
+
+# ---- products with the reduced Jacobian J~ (src/qr_reduced.jl:161-243, :484-512) ------------------------------
+def reduced_mul_(y, handle, factored, v, stream=None):
+    """reduced_mul!: y = J~ v."""
+    _check(lib().bls_reduced_mul_f64(handle.h, _dp(y), _dp(factored), _dp(v), _stream(stream)))
+
+
+def reduced_tmul_(g, handle, factored, w, stream=None):
+    """reduced_tmul!: g = J~' w."""
+    _check(lib().bls_reduced_tmul_f64(handle.h, _dp(g), _dp(factored), _dp(w), _stream(stream)))
+
+
+def reduced_residual_(rt, handle, fres, stream=None):
+    """reduced_residual!: r~."""
+    _check(lib().bls_reduced_residual_f64(handle.h, _dp(rt), _dp(fres), _stream(stream)))
+
+
+def reduced_diagonal_blocks_(D, handle, factored, stream=None):
+    """reduced_diagonal_blocks!: the diagonal blocks of J~'J~ (full, column-major, at doffsets)."""
+    _check(lib().bls_reduced_diagonal_blocks_f64(handle.h, _dp(D), _dp(factored), _stream(stream)))

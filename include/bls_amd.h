@@ -141,6 +141,23 @@ int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *r
 int bls_reduce_qr_f64_async(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                             const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
                             void *stream);
+/* The reduced-Jacobian structure of the registered SchurJacobianStructure (src/schur_jacobian.jl:30-160), for
+ * the products below: table_top (ntables), reduced_offsets (ntables + 1), table_ptr (ntables + 1), and per table
+ * entry entry_retained, entry_col, entry_table (nentries), retained_ptr (nrblocks + 1), retained_entry (nentries),
+ * r_dims / boffsets_r of its SchurStructure, doffsets (nrblocks + 1). All 1-based as stored. */
+int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int64_t *reduced_offsets, const int64_t *table_ptr,
+                       int64_t nentries, const int64_t *entry_retained, const int64_t *entry_col,
+                       const int64_t *entry_table, const int64_t *retained_ptr, const int64_t *retained_entry,
+                       const int64_t *r_dims, const int64_t *boffsets_r, const int64_t *doffsets);
+/* reduced_mul! (src/qr_reduced.jl:161-193): y = J~ v, J~ the rows of the factored tables below U. */
+int bls_reduced_mul_f64(bls_qr *h, double *y, const double *factored, const double *v, void *stream);
+/* reduced_tmul! (src/qr_reduced.jl:195-223): g = J~' w (deterministic: fixed-order partial sums). */
+int bls_reduced_tmul_f64(bls_qr *h, double *g, const double *factored, const double *w, void *stream);
+/* reduced_residual! (src/qr_reduced.jl:227-243): r~, the factored residual rows below U. */
+int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *fres, void *stream);
+/* reduced_diagonal_blocks! (src/qr_reduced.jl:484-512): the diagonal blocks of J~'J~, full d x d column-major at
+ * doffsets (the block-Jacobi blocks of the QR path). */
+int bls_reduced_diagonal_blocks_f64(bls_qr *h, double *D, const double *factored, void *stream);
 /* Kernel time of the last synchronous reduce_qr (HIP events, ms), when enabled. */
 int bls_qr_set_timing(bls_qr *h, int enable);
 int bls_qr_kernel_time(bls_qr *h, float *ms);

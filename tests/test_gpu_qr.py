@@ -121,3 +121,45 @@ def test_qr_bitwise_reproducible(blsamd):
     p = _small_jacobian(blsamd, 6, nland=200, ncam=12, kmax=8)
     (a, _, Fa, Fra), (b, _, Fb, Frb) = run_gpu(blsamd, p, 0.1, repeat=2)
     assert np.array_equal(Fa, Fb) and np.array_equal(Fra, Frb)
+
+
+@pytest.mark.parametrize("free", [False, True])
+def test_reduced_products_match_oracle(blsamd, free):
+    """reduced_mul!, reduced_tmul!, reduced_residual!, reduced_diagonal_blocks! (src/qr_reduced.jl:161-243,
+    :484-512) on the device, from the device-factored tables, against the oracle on the same factored tables."""
+    import torch
+    dev = blsamd.device
+    p = _jac_with_free_rows(blsamd) if free else _small_jacobian(blsamd, 7, nland=90, ncam=9, kmax=6)
+    ok, fb, Fg, Frg = run_gpu(blsamd, p, 0.2)
+    assert ok
+    jst, st = p.jst, p.structure
+    sargs = oracle.SchurArgs(st, blsamd.structure.SchurReductionWorkspaceIndex(st, 1))
+    jargs = oracle.JacobianArgs(jst)
+    h = dev.QRHandle(jst)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    F, Fr = T(Fg), T(Frg)
+    nred = jst.reduced_nrows()
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(st.nr)
+    w = rng.standard_normal(nred)
+    y = torch.empty(nred, dtype=torch.float64, device="cuda")
+    dev.reduced_mul_(y, h, F, T(v))
+    yo = np.zeros(nred)
+    oracle.reduced_mul(sargs, jargs, yo, Fg, v)
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    dev.reduced_tmul_(g, h, F, T(w))
+    go = np.zeros(st.nr)
+    oracle.reduced_tmul(sargs, jargs, go, Fg, w)
+    rt = torch.empty(nred, dtype=torch.float64, device="cuda")
+    dev.reduced_residual_(rt, h, Fr)
+    rto = np.zeros(nred)
+    oracle.reduced_residual(jargs, rto, Frg)
+    D = torch.empty(int((st.r_dims ** 2).sum()), dtype=torch.float64, device="cuda")
+    dev.reduced_diagonal_blocks_(D, h, F)
+    Do = np.zeros(int((st.r_dims ** 2).sum()))
+    oracle.reduced_diagonal_blocks(sargs, jargs, Do, Fg)
+    torch.cuda.synchronize()
+    for name, a, b in (("mul", y, yo), ("tmul", g, go), ("residual", rt, rto), ("diag", D, Do)):
+        a = a.cpu().numpy()
+        assert np.abs(a - b).max() <= TOL * max(1.0, np.abs(b).max()), name
+    assert np.array_equal(rt.cpu().numpy(), rto)
