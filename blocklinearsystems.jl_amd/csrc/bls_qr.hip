@@ -28,7 +28,22 @@ struct QRArgs {
     double lambda;
     double *F, *Fr;
     unsigned long long *flag;
+    // conditioning (scale_table_columns!, src/qr_reduction.jl:236-255): cs over all nr + ne variables, NULL =
+    // UnitScale; a retained column finds its variable through the table's entries
+    const double *cs;
+    int64_t nr;
+    const int32_t *table_ptr, *entry_retained, *entry_col, *r_dims;
+    const int64_t *boff_r;
 };
+
+// The conditioning factor of retained column j (0-based) of table t: s of the variable it holds.
+__device__ inline double column_scale(const QRArgs &a, int t, int j) {
+    for (int u = a.table_ptr[t]; u < a.table_ptr[t + 1]; u++) {
+        const int c0 = a.entry_col[u], r = a.entry_retained[u];
+        if (j >= c0 && j < c0 + a.r_dims[r]) return a.cs[a.boff_r[r] + (j - c0)];
+    }
+    return 1.0;   // not reached for a column of the structure
+}
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -51,8 +66,15 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
     const int64_t off = a.raw_off[t];
     const int64_t rq = a.res_off[t];
     double bad = 0.0;
-    if (de == 0) {   // a free table: copied (copy_table!, its result unused by reduce_qr!)
-        for (int64_t i = lane; i < (int64_t)M * C; i += 64) a.F[off + i] = a.Rw[off + i];
+    if (de == 0) {   // a free table: copied (copy_table!, its result unused by reduce_qr!), columns scaled
+        if (!a.cs) {
+            for (int64_t i = lane; i < (int64_t)M * C; i += 64) a.F[off + i] = a.Rw[off + i];
+        } else {
+            for (int j = 0; j < C; j++) {
+                const double f = column_scale(a, t, j);
+                for (int i = lane; i < M; i += 64) a.F[off + (int64_t)j * M + i] = a.Rw[off + (int64_t)j * M + i] * f;
+            }
+        }
         for (int i = lane; i < M; i += 64) a.Fr[rq + i] = a.Rr[rq + i];
         return;
     }
@@ -64,12 +86,13 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
     for (int k = 0; k < QR_MAXDE; k++) {
         P[k] = 0.0;
         if (k < de) {
+            const double f = a.cs ? a.cs[a.nr + so + k] : 1.0;
             if (lane < me) {
                 const double x = a.Rw[off + (int64_t)(ce + k) * M + lane];
-                P[k] = x;
+                P[k] = x * f;
                 bad += x - x;
             } else if (lane == me + k) {
-                P[k] = sqrt(a.lambda * a.scaling_e[so + k]);
+                P[k] = sqrt(a.lambda * a.scaling_e[so + k]) * f;
             }
         }
     }
@@ -120,13 +143,14 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
     // retained columns and the residual: lane-per-column through registers
     for (int j = lane; j <= ce; j += 64) {
         const bool res = j == ce;
+        const double f = (a.cs && !res) ? column_scale(a, t, j) : 1.0;
         double col[MB];
 #pragma unroll
         for (int i = 0; i < MB; i++) {
             double x = 0.0;
             if (i < me) x = res ? a.Rr[rq + i] : a.Rw[off + (int64_t)j * M + i];
             bad += x - x;
-            col[i] = x;
+            col[i] = x * f;
         }
 #pragma unroll
         for (int k = 0; k < QR_MAXDE; k++) {
@@ -315,8 +339,10 @@ std::vector<int64_t> sub1(const int64_t *p, int64_t n) {
 }
 
 int reduce_qr_async(bls_qr *h, double *F, double *Fr, const double *Rw, const double *Rr, const double *scaling_e,
-                    double lambda, int64_t *d_failed, hipStream_t s) {
+                    double lambda, const double *cs, int64_t *d_failed, hipStream_t s) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (cs && !h->has_reduced)
+        return fail(BLS_ERR_STATE, "reduce_qr with a conditioning scale needs the table entries (bls_qr_set_reduced)");
     if (!(std::isfinite(lambda) && lambda >= 0))
         return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
     if ((h->raw_len && (!F || !Rw)) || (h->res_len && (!Fr || !Rr)) || (h->neb && !scaling_e) || !d_failed)
@@ -337,6 +363,13 @@ int reduce_qr_async(bls_qr *h, double *F, double *Fr, const double *Rw, const do
     a.F = F;
     a.Fr = Fr;
     a.flag = h->d_flag.ptr;
+    a.cs = cs;
+    a.nr = h->nr;
+    a.table_ptr = h->d_table_ptr.ptr;
+    a.entry_retained = h->d_entry_retained.ptr;
+    a.entry_col = h->d_entry_col.ptr;
+    a.r_dims = h->d_r_dims.ptr;
+    a.boff_r = h->d_boff_r.ptr;
     for (int b = 0; b < 3; b++) {
         a.tables = h->d_bucket[b].ptr;
         a.ntab = (int64_t)h->bucket_tables[b].size();
@@ -493,9 +526,8 @@ extern "C" int bls_reduced_diagonal_blocks_f64(bls_qr *h, double *D, const doubl
 extern "C" int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                                  const double *scaling_e, double lambda, const double *s, int64_t *failed_block,
                                  void *stream) {
-    if (s) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
     hipStream_t st = (hipStream_t)stream;
-    int rc = reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, h ? h->d_failed_out.ptr : nullptr, st);
+    int rc = reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, s, h ? h->d_failed_out.ptr : nullptr, st);
     if (rc) return rc;
     int64_t fb = 0;
     BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -508,8 +540,7 @@ extern "C" int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, cons
 extern "C" int bls_reduce_qr_f64_async(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                                        const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
                                        void *stream) {
-    if (s) return fail(BLS_ERR_UNSUPPORTED, "conditioning scales are not supported on the device path yet");
-    return reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, d_failed_block, (hipStream_t)stream);
+    return reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, s, d_failed_block, (hipStream_t)stream);
 }
 
 extern "C" int bls_qr_set_timing(bls_qr *h, int enable) {

@@ -135,14 +135,16 @@ void bls_qr_destroy(bls_qr *h);
 /* reduce_qr!(ws, J, scaling_e, lambda, s) (src/qr_reduction.jl:162-187): factored = Q' [J_r | J_e] per table with
  * the damping rows sqrt(lambda scaling_e) written, fres = Q' r in Float64, free tables copied; raw tables and r
  * are read only. BLS_FAILED with the lowest failing eliminated block for a non-finite table or a numerically
- * singular U (|beta| <= M eps ||col||). `s` (conditioning scale) must be NULL on the device path. */
+ * singular U (|beta| <= M eps ||col||). `s` is the conditioning scale over all nr + ne variables (device
+ * Float64, NULL = UnitScale; scale_table_columns!, src/qr_reduction.jl:236-255): each column is scaled by its
+ * variable's s as it is read and the damping rows by the eliminated variable's; it needs bls_qr_set_reduced. */
 int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                       const double *scaling_e, double lambda, const double *s, int64_t *failed_block, void *stream);
 int bls_reduce_qr_f64_async(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                             const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
                             void *stream);
-/* The reduced-Jacobian structure of the registered SchurJacobianStructure (src/schur_jacobian.jl:30-160), for
- * the products below: table_top (ntables), reduced_offsets (ntables + 1), table_ptr (ntables + 1), and per table
+/* The table entries and reduced-Jacobian structure of the registered SchurJacobianStructure
+ * (src/schur_jacobian.jl:30-160), for the products below and for reduce_qr's conditioning scale: table_top (ntables), reduced_offsets (ntables + 1), table_ptr (ntables + 1), and per table
  * entry entry_retained, entry_col, entry_table (nentries), retained_ptr (nrblocks + 1), retained_entry (nentries),
  * r_dims / boffsets_r of its SchurStructure, doffsets (nrblocks + 1). All 1-based as stored. */
 int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int64_t *reduced_offsets, const int64_t *table_ptr,

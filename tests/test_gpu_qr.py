@@ -163,3 +163,25 @@ def test_reduced_products_match_oracle(blsamd, free):
         a = a.cpu().numpy()
         assert np.abs(a - b).max() <= TOL * max(1.0, np.abs(b).max()), name
     assert np.array_equal(rt.cpu().numpy(), rto)
+
+
+def test_qr_conditioning_scale(blsamd):
+    """reduce_qr! with a conditioning scale s (src/qr_reduction.jl:162-255) on the device against the oracle, on
+    eliminated and free tables."""
+    import torch
+    from test_oracle_qr import _pow2_scale
+    dev = blsamd.device
+    rng = np.random.default_rng(11)
+    for p in (_small_jacobian(blsamd, 8, nland=70, ncam=9, kmax=6), _jac_with_free_rows(blsamd)):
+        st, jst = p.structure, p.jst
+        s = _pow2_scale(rng, st.nr + st.ne) * rng.uniform(0.5, 2.0, st.nr + st.ne)
+        h = dev.QRHandle(jst)
+        ws = dev.QRReductionWorkspace(h)
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+        ok = dev.reduce_qr_(ws, h, T(p.data), T(p.r), T(p.scaling_e), 0.25, s=T(s))
+        torch.cuda.synchronize()
+        sargs = oracle.SchurArgs(st, blsamd.structure.SchurReductionWorkspaceIndex(st, 1))
+        Fo = np.zeros(jst.raw_length()); Fro = np.zeros(jst.residual_length())
+        oko, fbo = oracle.reduce_qr(sargs, oracle.JacobianArgs(jst), Fo, Fro, p.data, p.r, p.scaling_e, 0.25, cs=s)
+        assert ok and oko
+        compare_tables(p, ws.factored.cpu().numpy(), ws.fres.cpu().numpy(), Fo, Fro)

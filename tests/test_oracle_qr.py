@@ -172,3 +172,55 @@ def test_qr_re_reduction_exact(blsamd):
     b = _qr(p, 0.2)
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
     assert np.array_equal(p.data, raw)
+
+
+def _column_scaled(p, s):
+    """The raw tables with every column multiplied by its variable's s (the conditioned Jacobian J S), built from
+    the table layout: entry u of table t holds retained block entry_retained[u] at columns entry_col[u] ..;
+    an eliminated table's last d_e columns hold its eliminated variables."""
+    jst, st = p.jst, p.structure
+    data = p.data.copy()
+    for t in range(jst.ntables):
+        M, C, off = jst.table_nrows[t], jst.table_ncols[t] - 1, jst.raw_offsets[t] - 1
+        T = data[off:off + M * C].reshape(C, M)
+        for u in range(jst.table_ptr[t] - 1, jst.table_ptr[t + 1] - 1):
+            r = jst.entry_retained[u] - 1
+            for k in range(st.r_dims[r]):
+                T[jst.entry_col[u] - 1 + k] *= s[st.boffsets_r[r] - 1 + k]
+        if t < st.neblocks:
+            de = st.e_dims[t]
+            for k in range(de):
+                T[C - de + k] *= s[st.nr + st.boffsets_e[t] - 1 + k]
+    return data
+
+
+def _pow2_scale(rng, n):
+    return 2.0 ** rng.integers(-6, 7, n).astype(np.float64)
+
+
+def test_qr_conditioning_scale_equals_prescaled_columns(blsamd):
+    """reduce_qr!(ws, J, sc, lambda, s) factors J S with damping rows sqrt(lambda sc) s_e (src/qr_reduction.jl:
+    162-255): with powers of two for s, the same tables as the unconditioned QR of the column-scaled raw tables with
+    scaling_e multiplied by s_e^2, bit for bit."""
+    import copy
+    rng = np.random.default_rng(9)
+    for p in (_small_jacobian(blsamd, 4), _jac_with_free_rows_cpu(blsamd)):
+        st = p.structure
+        s = _pow2_scale(rng, st.nr + st.ne)
+        jst = p.jst
+        from blsload import load
+        sargs = oracle.SchurArgs(st, load().structure.SchurReductionWorkspaceIndex(st, 1))
+        jargs = oracle.JacobianArgs(jst)
+        F = np.zeros(jst.raw_length()); Fr = np.zeros(jst.residual_length())
+        ok, fb = oracle.reduce_qr(sargs, jargs, F, Fr, p.data, p.r, p.scaling_e, 0.2, cs=s)
+        q = copy.copy(p)
+        q.data = _column_scaled(p, s)
+        q.scaling_e = p.scaling_e * s[st.nr:] ** 2
+        ok2, fb2, F2, Fr2, _, _ = _qr(q, 0.2)
+        assert ok and ok2
+        assert np.array_equal(F, F2) and np.array_equal(Fr, Fr2)
+
+
+def _jac_with_free_rows_cpu(blsamd):
+    from test_gpu_qr import _jac_with_free_rows
+    return _jac_with_free_rows(blsamd)
