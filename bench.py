@@ -133,6 +133,7 @@ def main():
     p2_gbs = p2_bytes / (kt[2] * 1e-3) / 1e9
     log(f"kernels: reset {kt[0]:.4f} ms, phase1 {kt[1]:.4f} ms ({p1_bytes / (kt[1] * 1e-3) / 1e9:.0f} GB/s), "
         f"phase2 {kt[2]:.4f} ms ({p2_gbs:.0f} GB/s), merge {kt[3]:.4f} ms")
+    mixed = mixed_leg(torch, dev, p, h, S, csc, dmap, args.steps, args.warmup)
     pcie = pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, max(3, args.steps // 4))
     log(f"PCIe-inclusive (host E, C, b, R in; S, g, CSC out): {pcie['ms_per_step']:.3f} ms/step, "
         f"{pcie['value']:.4g} landmarks/s")
@@ -148,6 +149,7 @@ def main():
                      "frac": p2_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "step_achieved": whole_bytes / (ms * 1e-3) / 1e9},
         "pcie_inclusive": pcie,
+        "mixed": mixed,
     }
     del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
     torch.cuda.empty_cache()
@@ -159,6 +161,57 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)
     print(json.dumps(result), flush=True)
+
+
+def mixed_leg(torch, dev, p, h, S64, csc, dmap, steps, warmup):
+    """The same step with the blocks E, C, R stored in Float32 (the mixed entry points: Float64 landmark
+    factorisation, S and g_S; the reduced system of the Float32 blocks to Float64 rounding). Reports its rate, its
+    kernel times and how far its S is from the Float64 step's (the rounding of the inputs to Float32)."""
+    st = p.structure
+    ws = dev.SchurReductionWorkspace(h, "mixed")
+    T32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    T64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    E, C, Rd = T32(p.E_data), T32(p.C_data), T32(p.R_data)
+    b_r, b_e, sr, se = T64(p.b_r), T64(p.b_e), T64(p.scaling_r), T64(p.scaling_e)
+    S = torch.empty_like(S64)
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    ref = S64.clone()
+
+    def step():
+        dev.reset_schur_sparse_(S, Rd, h, sr, p.lam, check=False)
+        dev.reduce_schur_sparse_(S, g, b_r, b_e, se, p.lam, h, E, C, ws, sync=False, check=False)
+        dev.copyto_(csc, S, dmap)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    assert int(ws.d_failed.item()) == 0
+    h.set_timing(True)
+    kt = np.zeros(4)
+    for _ in range(steps):
+        dev.reset_schur_sparse_(S, Rd, h, sr, p.lam, check=False)
+        dev.reduce_schur_sparse_(S, g, b_r, b_e, se, p.lam, h, E, C, ws, check=False)
+        kt += np.array(h.kernel_times())
+    h.set_timing(False)
+    kt /= steps
+    k = np.diff(st.eliminated_ptr)
+    # phase 2 reads E_inv (Float64, 72 B), z (24 B) and C (Float32, 72 B per coupling), reads and writes S
+    p2_bytes = float(np.sum(96 + 72 * k) + 16 * p.pattern.nnz + 16 * st.nr)
+    rel = float((S - ref).norm() / ref.norm())
+    log(f"mixed (Float32 blocks, Float64 system): {ms:.4f} ms/step, {st.neblocks / (ms * 1e-3):.4g} landmarks/s; "
+        f"phase1 {kt[1]:.4f} ms, phase2 {kt[2]:.4f} ms ({p2_bytes / (kt[2] * 1e-3) / 1e9:.0f} GB/s); "
+        f"||S_mixed - S_f64|| / ||S_f64|| = {rel:.3g}")
+    return {"ms_per_step": ms, "landmarks_per_s": st.neblocks / (ms * 1e-3),
+            "kernel_ms": {"reset": kt[0], "phase1": kt[1], "phase2": kt[2], "phase2_merge": kt[3]},
+            "phase2_GBs": p2_bytes / (kt[2] * 1e-3) / 1e9, "rel_diff_S_vs_f64_inputs": rel,
+            "dtype": "E, C, R Float32; E_inv, S, g Float64"}
 
 
 def qr_leg(torch, dev, b, scale, reps=10):
@@ -189,7 +242,27 @@ def qr_leg(torch, dev, b, scale, reps=10):
     gbs = nbytes / (ms * 1e-3) / 1e9
     log(f"QR leg: {ms:.3f} ms, {st.neblocks / (ms * 1e-3):.4g} landmarks/s, {gbs:.0f} GB/s "
         f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM)")
-    return {"landmark_scale": scale, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
+    # SchurBlockJacobian{Float32}: Float32 tables, Float64 residual
+    F64 = ws.factored
+    del raw
+    ws32 = dev.QRReductionWorkspace(h, dtype=torch.float32)
+    raw32 = torch.as_tensor(np.ascontiguousarray(p.data, dtype=np.float32)).cuda()
+    assert dev.reduce_qr_(ws32, h, raw32, r, sc, lam)
+    h.set_timing(True)
+    times = []
+    for _ in range(reps):
+        assert dev.reduce_qr_(ws32, h, raw32, r, sc, lam, check=False)
+        times.append(h.kernel_time())
+    h.set_timing(False)
+    ms32 = float(np.median(times))
+    nbytes32 = float(np.sum(8 * M * Cc + 16 * M))
+    gbs32 = nbytes32 / (ms32 * 1e-3) / 1e9
+    rel32 = float((ws32.factored.double() - F64).norm() / F64.norm())
+    log(f"QR leg, Float32 tables: {ms32:.3f} ms, {st.neblocks / (ms32 * 1e-3):.4g} landmarks/s, {gbs32:.0f} GB/s; "
+        f"||F32 - F64|| / ||F64|| = {rel32:.3g}")
+    f32 = {"ms": ms32, "landmarks_per_s": st.neblocks / (ms32 * 1e-3), "achieved_GBs": gbs32,
+           "frac": gbs32 / HBM_PEAK_GBS, "rel_diff_vs_f64": rel32}
+    return {"landmark_scale": scale, "f32": f32, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
             "cameras": int(st.nrblocks), "raw_GB": jst.raw_length() * 8 / 1e9, "ms": ms,
             "landmarks_per_s": st.neblocks / (ms * 1e-3),
             "roofline": {"bound": "hbm", "kernel": "k_qr_tables", "achieved": gbs, "peak": HBM_PEAK_GBS,
@@ -260,11 +333,78 @@ def batched_pcg_leg(torch, dev, b, scale, reps=3):
                    "output_scale": model.alpha}
         log(f"batched leg, learned correction (output scale {model.alpha}): PCG {lms:.3f} ms (iterations max "
             f"{int(itl.max())}, mean {itl.mean():.1f}), time to 1e-8 {red + lms:.3f} ms")
+    mixed = batched_mixed(torch, dev, b, p, h, pcg, S, g, reps)
     return {"problems": int(p.nprob), "landmark_scale": scale, "cameras": int(st.nrblocks),
             "landmarks": int(st.neblocks), "S_nnz": int(p.pattern.nnz), "reduce_ms": red,
             "landmarks_per_s": st.neblocks / (red * 1e-3), "pcg_ms": sol, "time_to_1e-8_ms": red + sol,
             "pcg_iters_max": int(iters.max()), "pcg_iters_mean": float(iters.mean()),
-            "relres_max": float(rel.max()), "preconditioner": "block-Jacobi (FP64)", "learned": learned}
+            "relres_max": float(rel.max()), "preconditioner": "block-Jacobi (FP64)", "learned": learned,
+            "mixed": mixed}
+
+
+def batched_mixed(torch, dev, b, p, h, pcg, S64, g64, reps):
+    """The 64 problems with their blocks E, C, R stored in Float32: the mixed reduction (Float64 factorisation and
+    reduced system of the Float32 blocks), then the same PCG (plain and learned) to 1e-8 on that system. Also the
+    residual of its solution against the Float64-stored problem's system, ||S64 x - g64|| / ||g64|| per problem:
+    the cost of the Float32 inputs."""
+    st = p.structure
+    ws = dev.SchurReductionWorkspace(h, "mixed")
+    T32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    T64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    E, C, Rd = T32(p.E_data), T32(p.C_data), T32(p.R_data)
+    b_r, b_e, sr, se = T64(p.b_r), T64(p.b_e), T64(p.scaling_r), T64(p.scaling_e)
+    S = torch.empty_like(S64)
+    g = torch.empty_like(g64)
+    x = torch.zeros_like(g64)
+
+    def reduce():
+        dev.reset_schur_sparse_(S, Rd, h, sr, p.lam, check=False)
+        dev.reduce_schur_sparse_(S, g, b_r, b_e, se, p.lam, h, E, C, ws, sync=False, check=False)
+
+    reduce()
+    x.zero_()
+    pcg.solve(S, g, x, tol=1e-8)
+    torch.cuda.synchronize()
+    t_red, t_pcg = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        reduce()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x.zero_()
+        ok, iters, rel = pcg.solve(S, g, x, tol=1e-8)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        assert ok and int(ws.d_failed.item()) == 0
+        t_red.append(t1 - t0)
+        t_pcg.append(t2 - t1)
+    red, sol = float(np.median(t_red)) * 1e3, float(np.median(t_pcg)) * 1e3
+    # residual against the Float64-stored problem's reduced system (a check, untimed)
+    cross = float(pcg.relres(S64, g64, x).max())
+    own = float(pcg.relres(S, g, x).max())
+    out = {"reduce_ms": red, "landmarks_per_s": st.neblocks / (red * 1e-3), "pcg_ms": sol,
+           "time_to_1e-8_ms": red + sol, "pcg_iters_max": int(iters.max()), "pcg_iters_mean": float(iters.mean()),
+           "relres_max": float(rel.max()), "relres_recomputed_max": own, "relres_vs_f64_inputs_max": cross,
+           "dtype": "E, C, R Float32; E_inv, S, g, PCG Float64"}
+    log(f"batched leg, mixed: reduce {red:.3f} ms ({st.neblocks / red * 1e3:.4g} landmarks/s), PCG {sol:.3f} ms "
+        f"(iterations max {int(iters.max())}), time to 1e-8 {red + sol:.3f} ms; residual against the Float64-stored "
+        f"system {cross:.3g}")
+    P = b.precond
+    if os.path.exists(P.WEIGHTS):
+        model = P.MLP.load()
+        P.solve(pcg, S, g, x, p.lam, model=model)
+        torch.cuda.synchronize()
+        t_l = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            okl, itl, rell, used = P.solve(pcg, S, g, x, p.lam, model=model)
+            torch.cuda.synchronize()
+            t_l.append(time.perf_counter() - t0)
+        lms = float(np.median(t_l)) * 1e3
+        out["learned"] = {"pcg_ms": lms, "time_to_1e-8_ms": red + lms, "pcg_iters_max": int(itl.max()),
+                          "relres_max": float(rell.max()), "used_learned": bool(used), "output_scale": model.alpha}
+        log(f"batched leg, mixed + learned: time to 1e-8 {red + lms:.3f} ms")
+    return out
 
 
 def pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, steps):

@@ -373,6 +373,23 @@ __global__ void k_copy_f64(double *dst, const double *src, int64_t n) {
         dst[i] = src[i];
 }
 
+// ||g - q|| / ||g|| per problem (q = S x), one workgroup per problem, fixed-order sums
+__global__ __launch_bounds__(UPD_THREADS) void k_pcg_relres(PcgArgs a, const double *g, double *out) {
+    __shared__ double red[UPD_THREADS];
+    const int pb = blockIdx.x;
+    const int64_t b0 = a.prob_bptr[pb], b1 = a.prob_bptr[pb + 1];
+    const int64_t s0 = a.bo[b0], s1 = a.bo[b1 - 1] + a.dims[b1 - 1];
+    double gg = 0.0, rr = 0.0;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
+        const double gv = g[s], rv = gv - a.q[s];
+        gg += gv * gv;
+        rr += rv * rv;
+    }
+    gg = wg_sum(gg, red);
+    rr = wg_sum(rr, red);
+    if (threadIdx.x == 0) out[pb] = gg > 0 ? sqrt(rr) / sqrt(gg) : sqrt(rr);
+}
+
 __global__ void k_pcg_init_state(ProbState *st, int nprob, int *nactive) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *nactive = 0;
@@ -639,4 +656,21 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
         if (stv[p].failed || !(stv[p].relres <= tol)) all = false;
     }
     return all ? BLS_OK : BLS_FAILED;
+}
+
+extern "C" int bls_pcg_relres_f64(bls_pcg *h, const double *S_nzval, const double *g, const double *x,
+                                  double *relres, void *stream) {
+    if (!h || !S_nzval || !g || !x || !relres) return fail(BLS_ERR_ARG, "bls_pcg_relres_f64: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    PcgArgs a = h->args(1.0, 0);
+    k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
+    const unsigned grid = (unsigned)((h->nrb + 3) / 4);
+    if (h->maxd <= 6) k_bsr_spmv_w<6><<<grid, 256, 0, s>>>(a, x, h->d_q.ptr, nullptr, 0);
+    else if (h->maxd <= 9) k_bsr_spmv_w<9><<<grid, 256, 0, s>>>(a, x, h->d_q.ptr, nullptr, 0);
+    else k_bsr_spmv_w<16><<<grid, 256, 0, s>>>(a, x, h->d_q.ptr, nullptr, 0);
+    k_pcg_relres<<<(unsigned)h->nprob, UPD_THREADS, 0, s>>>(a, g, h->d_pq.ptr);
+    BLS_HIP(hipGetLastError());
+    BLS_HIP(hipMemcpyAsync(relres, h->d_pq.ptr, sizeof(double) * h->nprob, hipMemcpyDeviceToHost, s));
+    BLS_HIP(hipStreamSynchronize(s));
+    return BLS_OK;
 }

@@ -19,14 +19,19 @@ namespace {
 constexpr int QR_WAVES = 4;   // tables per workgroup (one wavefront each)
 constexpr int QR_MAXDE = 8;
 
-struct QRArgs {
+// T: element type of the raw and factored tables (SchurBlockJacobian{T}); the residual is Float64 and the
+// reflectors are formed and applied in Float64 registers for either T, with the reference's singularity test
+// |beta| > M eps(T) ||col||.
+template <class T> struct QRArgs {
     const int32_t *tables;    // tables of this bucket (0-based)
     int64_t ntab;
     const int32_t *nrows, *ncols, *e_dims;   // per table: M, raw columns (table_ncols - 1); e_dims per table (0 = free)
     const int64_t *raw_off, *res_off, *boff_e;
-    const double *Rw, *Rr, *scaling_e;
+    const T *Rw;
+    const double *Rr, *scaling_e;
     double lambda;
-    double *F, *Fr;
+    T *F;
+    double *Fr;
     unsigned long long *flag;
     // conditioning (scale_table_columns!, src/qr_reduction.jl:236-255): cs over all nr + ne variables, NULL =
     // UnitScale; a retained column finds its variable through the table's entries
@@ -37,7 +42,7 @@ struct QRArgs {
 };
 
 // The conditioning factor of retained column j (0-based) of table t: s of the variable it holds.
-__device__ inline double column_scale(const QRArgs &a, int t, int j) {
+template <class T> __device__ inline double column_scale(const QRArgs<T> &a, int t, int j) {
     for (int u = a.table_ptr[t]; u < a.table_ptr[t + 1]; u++) {
         const int c0 = a.entry_col[u], r = a.entry_retained[u];
         if (j >= c0 && j < c0 + a.r_dims[r]) return a.cs[a.boff_r[r] + (j - c0)];
@@ -51,8 +56,8 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-template <int MB>
-__global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
+template <int MB, class T>
+__global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs<T> a) {
     __shared__ double sV[QR_WAVES][QR_MAXDE][64];
     __shared__ double sTau[QR_WAVES][QR_MAXDE];
     const int w = threadIdx.x >> 6;
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
         } else {
             for (int j = 0; j < C; j++) {
                 const double f = column_scale(a, t, j);
-                for (int i = lane; i < M; i += 64) a.F[off + (int64_t)j * M + i] = a.Rw[off + (int64_t)j * M + i] * f;
+                for (int i = lane; i < M; i += 64) a.F[off + (int64_t)j * M + i] = (T)((double)a.Rw[off + (int64_t)j * M + i] * f);
             }
         }
         for (int i = lane; i < M; i += 64) a.Fr[rq + i] = a.Rr[rq + i];
@@ -88,15 +93,15 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
         if (k < de) {
             const double f = a.cs ? a.cs[a.nr + so + k] : 1.0;
             if (lane < me) {
-                const double x = a.Rw[off + (int64_t)(ce + k) * M + lane];
-                P[k] = x * f;
+                const double x = (double)a.Rw[off + (int64_t)(ce + k) * M + lane];
+                P[k] = (double)(T)(x * f);
                 bad += x - x;
             } else if (lane == me + k) {
-                P[k] = sqrt(a.lambda * a.scaling_e[so + k]) * f;
+                P[k] = (double)(T)(sqrt(a.lambda * a.scaling_e[so + k]) * f);
             }
         }
     }
-    const double tol = (double)M * 2.220446049250313e-16;
+    const double tol = (double)M * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07);
     bool singular = false;
 #pragma unroll
     for (int k = 0; k < QR_MAXDE; k++) {
@@ -134,7 +139,7 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
     }
     // eliminated columns of F: [U; 0]
     for (int k = 0; k < de; k++)
-        if (lane < M) a.F[off + (int64_t)(ce + k) * M + lane] = P[k];
+        if (lane < M) a.F[off + (int64_t)(ce + k) * M + lane] = (T)P[k];
     // sV / sTau are private to this wave: a wavefront-scope fence and wave barrier order its LDS writes before
     // the reads below (no workgroup barrier: the other waves of the block may have finished already)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -148,9 +153,9 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
 #pragma unroll
         for (int i = 0; i < MB; i++) {
             double x = 0.0;
-            if (i < me) x = res ? a.Rr[rq + i] : a.Rw[off + (int64_t)j * M + i];
+            if (i < me) x = res ? a.Rr[rq + i] : (double)a.Rw[off + (int64_t)j * M + i];
             bad += x - x;
-            col[i] = x * f;
+            col[i] = res ? x : (double)(T)(x * f);
         }
 #pragma unroll
         for (int k = 0; k < QR_MAXDE; k++) {
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs a) {
         } else {
 #pragma unroll
             for (int i = 0; i < MB; i++)
-                if (i < M) a.F[off + (int64_t)j * M + i] = col[i];
+                if (i < M) a.F[off + (int64_t)j * M + i] = (T)col[i];
         }
     }
     if (__any(isnan(bad)) && lane == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
@@ -195,7 +200,8 @@ struct RedArgs {
 
 // y = J~ v (reduced_mul!, src/qr_reduced.jl:161-193): one wavefront per table, lane i its reduced row i; sums over
 // the table's entries and their columns in the reference's order
-__global__ __launch_bounds__(256) void k_red_mul(RedArgs a, const double *F, const double *v, double *y) {
+template <class T>
+__global__ __launch_bounds__(256) void k_red_mul(RedArgs a, const T *F, const double *v, double *y) {
     const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (t >= a.ntables) return;
@@ -206,7 +212,7 @@ __global__ __launch_bounds__(256) void k_red_mul(RedArgs a, const double *F, con
             const int r = a.entry_retained[u];
             const int64_t cb = a.raw_off[t] + (int64_t)a.entry_col[u] * M + top + i;
             const double *vr = v + a.boff_r[r];
-            for (int j = 0; j < a.r_dims[r]; j++) acc += F[cb + (int64_t)j * M] * vr[j];
+            for (int j = 0; j < a.r_dims[r]; j++) acc += (double)F[cb + (int64_t)j * M] * vr[j];
         }
         y[a.red_off[t] + i] = acc;
     }
@@ -227,7 +233,8 @@ __global__ __launch_bounds__(256) void k_red_residual(RedArgs a, const double *F
 constexpr int RED_THREADS = 256;
 
 // g = J~' w (reduced_tmul!, src/qr_reduced.jl:195-223)
-__global__ __launch_bounds__(RED_THREADS) void k_red_tmul(RedArgs a, const double *F, const double *w, double *g) {
+template <class T>
+__global__ __launch_bounds__(RED_THREADS) void k_red_tmul(RedArgs a, const T *F, const double *w, double *g) {
     __shared__ double part[RED_THREADS];
     const int r = blockIdx.x;
     const int dr = a.r_dims[r];
@@ -239,9 +246,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_tmul(RedArgs a, const doubl
             const int u = a.retained_entry[k];
             const int t = a.entry_table[u];
             const int M = a.nrows[t], top = a.top[t], n = M - top;
-            const double *col = F + a.raw_off[t] + (int64_t)(a.entry_col[u] + j) * M + top;
+            const T *col = F + a.raw_off[t] + (int64_t)(a.entry_col[u] + j) * M + top;
             const double *wt = w + a.red_off[t];
-            for (int i = 0; i < n; i++) acc += col[i] * wt[i];
+            for (int i = 0; i < n; i++) acc += (double)col[i] * wt[i];
         }
     }
     part[threadIdx.x] = acc;
@@ -254,7 +261,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_tmul(RedArgs a, const doubl
 }
 
 // diagonal blocks of J~'J~ (reduced_diagonal_blocks!, src/qr_reduced.jl:484-512), full d x d column-major at doff
-__global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const double *F, double *D) {
+template <class T>
+__global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const T *F, T *D) {
     __shared__ double part[RED_THREADS];
     const int r = blockIdx.x;
     const int dr = a.r_dims[r];
@@ -270,9 +278,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const doubl
             const int u = a.retained_entry[k];
             const int t = a.entry_table[u];
             const int M = a.nrows[t], top = a.top[t], n = M - top;
-            const double *cb = F + a.raw_off[t] + (int64_t)a.entry_col[u] * M + top;
-            const double *x = cb + (int64_t)row * M, *y = cb + (int64_t)col * M;
-            for (int i = 0; i < n; i++) acc += x[i] * y[i];
+            const T *cb = F + a.raw_off[t] + (int64_t)a.entry_col[u] * M + top;
+            const T *x = cb + (int64_t)row * M, *y = cb + (int64_t)col * M;
+            for (int i = 0; i < n; i++) acc += (double)x[i] * (double)y[i];
         }
     }
     part[threadIdx.x] = acc;
@@ -280,9 +288,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const doubl
     if (threadIdx.x < npair) {
         double s = 0.0;
         for (int c = 0; c < nch; c++) s += part[c * npair + threadIdx.x];
-        double *Db = D + a.doff[r];
-        Db[col * dr + row] = s;
-        Db[row * dr + col] = s;
+        T *Db = D + a.doff[r];
+        Db[col * dr + row] = (T)s;
+        Db[row * dr + col] = (T)s;
     }
 }
 
@@ -338,7 +346,8 @@ std::vector<int64_t> sub1(const int64_t *p, int64_t n) {
     return v;
 }
 
-int reduce_qr_async(bls_qr *h, double *F, double *Fr, const double *Rw, const double *Rr, const double *scaling_e,
+template <class T>
+int reduce_qr_async(bls_qr *h, T *F, double *Fr, const T *Rw, const double *Rr, const double *scaling_e,
                     double lambda, const double *cs, int64_t *d_failed, hipStream_t s) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (cs && !h->has_reduced)
@@ -349,7 +358,7 @@ int reduce_qr_async(bls_qr *h, double *F, double *Fr, const double *Rw, const do
         return fail(BLS_ERR_ARG, "reduce_qr: a required array is NULL");
     h->timing.mark(0, s);
     k_qr_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
-    QRArgs a{};
+    QRArgs<T> a{};
     a.nrows = h->d_nrows.ptr;
     a.ncols = h->d_ncols.ptr;
     a.e_dims = h->d_edims.ptr;
@@ -375,9 +384,9 @@ int reduce_qr_async(bls_qr *h, double *F, double *Fr, const double *Rw, const do
         a.ntab = (int64_t)h->bucket_tables[b].size();
         if (!a.ntab) continue;
         const unsigned grid = (unsigned)((a.ntab + QR_WAVES - 1) / QR_WAVES);
-        if (b == 0) k_qr_tables<16><<<grid, 64 * QR_WAVES, 0, s>>>(a);
-        else if (b == 1) k_qr_tables<32><<<grid, 64 * QR_WAVES, 0, s>>>(a);
-        else k_qr_tables<64><<<grid, 64 * QR_WAVES, 0, s>>>(a);
+        if (b == 0) k_qr_tables<16, T><<<grid, 64 * QR_WAVES, 0, s>>>(a);
+        else if (b == 1) k_qr_tables<32, T><<<grid, 64 * QR_WAVES, 0, s>>>(a);
+        else k_qr_tables<64, T><<<grid, 64 * QR_WAVES, 0, s>>>(a);
     }
     h->timing.mark(1, s);
     k_qr_finalize<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
@@ -487,22 +496,59 @@ static int reduced_ready(bls_qr *h) {
     return BLS_OK;
 }
 
-extern "C" int bls_reduced_mul_f64(bls_qr *h, double *y, const double *F, const double *v, void *stream) {
+namespace {
+template <class T> int reduced_mul(bls_qr *h, double *y, const T *F, const double *v, hipStream_t s) {
     int rc = reduced_ready(h);
     if (rc) return rc;
-    if (!y || !F || !v) return fail(BLS_ERR_ARG, "bls_reduced_mul_f64: a required array is NULL");
-    if (h->ntables) k_red_mul<<<(unsigned)((h->ntables + 3) / 4), 256, 0, (hipStream_t)stream>>>(h->red(), F, v, y);
+    if (!y || !F || !v) return fail(BLS_ERR_ARG, "reduced_mul: a required array is NULL");
+    if (h->ntables) k_red_mul<T><<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), F, v, y);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
 
-extern "C" int bls_reduced_tmul_f64(bls_qr *h, double *g, const double *F, const double *w, void *stream) {
+template <class T> int reduced_tmul(bls_qr *h, double *g, const T *F, const double *w, hipStream_t s) {
     int rc = reduced_ready(h);
     if (rc) return rc;
-    if (!g || !F || !w) return fail(BLS_ERR_ARG, "bls_reduced_tmul_f64: a required array is NULL");
-    if (h->nrb) k_red_tmul<<<(unsigned)h->nrb, RED_THREADS, 0, (hipStream_t)stream>>>(h->red(), F, w, g);
+    if (!g || !F || !w) return fail(BLS_ERR_ARG, "reduced_tmul: a required array is NULL");
+    if (h->nrb) k_red_tmul<T><<<(unsigned)h->nrb, RED_THREADS, 0, s>>>(h->red(), F, w, g);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
+}
+
+template <class T> int reduced_diag(bls_qr *h, T *D, const T *F, hipStream_t s) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!D || !F) return fail(BLS_ERR_ARG, "reduced_diagonal_blocks: a required array is NULL");
+    if (h->nrb) k_red_diag<T><<<(unsigned)h->nrb, RED_THREADS, 0, s>>>(h->red(), F, D);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+template <class T>
+int reduce_qr_sync(bls_qr *h, T *factored, double *fres, const T *raw, const double *r, const double *scaling_e,
+                   double lambda, const double *s, int64_t *failed_block, hipStream_t st) {
+    int rc = reduce_qr_async<T>(h, factored, fres, raw, r, scaling_e, lambda, s, h ? h->d_failed_out.ptr : nullptr, st);
+    if (rc) return rc;
+    int64_t fb = 0;
+    BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    BLS_HIP(hipStreamSynchronize(st));
+    if (h->timing.enabled) (void)hipEventElapsedTime(&h->timing.ms[0], h->timing.ev[0], h->timing.ev[1]);
+    if (failed_block) *failed_block = fb;
+    return fb ? BLS_FAILED : BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_reduced_mul_f64(bls_qr *h, double *y, const double *F, const double *v, void *stream) {
+    return reduced_mul<double>(h, y, F, v, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_mul_f32(bls_qr *h, double *y, const float *F, const double *v, void *stream) {
+    return reduced_mul<float>(h, y, F, v, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_tmul_f64(bls_qr *h, double *g, const double *F, const double *w, void *stream) {
+    return reduced_tmul<double>(h, g, F, w, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_tmul_f32(bls_qr *h, double *g, const float *F, const double *w, void *stream) {
+    return reduced_tmul<float>(h, g, F, w, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *Fr, void *stream) {
@@ -515,32 +561,34 @@ extern "C" int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *Fr,
 }
 
 extern "C" int bls_reduced_diagonal_blocks_f64(bls_qr *h, double *D, const double *F, void *stream) {
-    int rc = reduced_ready(h);
-    if (rc) return rc;
-    if (!D || !F) return fail(BLS_ERR_ARG, "bls_reduced_diagonal_blocks_f64: a required array is NULL");
-    if (h->nrb) k_red_diag<<<(unsigned)h->nrb, RED_THREADS, 0, (hipStream_t)stream>>>(h->red(), F, D);
-    BLS_HIP(hipGetLastError());
-    return BLS_OK;
+    return reduced_diag<double>(h, D, F, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_diagonal_blocks_f32(bls_qr *h, float *D, const float *F, void *stream) {
+    return reduced_diag<float>(h, D, F, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                                  const double *scaling_e, double lambda, const double *s, int64_t *failed_block,
                                  void *stream) {
-    hipStream_t st = (hipStream_t)stream;
-    int rc = reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, s, h ? h->d_failed_out.ptr : nullptr, st);
-    if (rc) return rc;
-    int64_t fb = 0;
-    BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    BLS_HIP(hipStreamSynchronize(st));
-    if (h->timing.enabled) (void)hipEventElapsedTime(&h->timing.ms[0], h->timing.ev[0], h->timing.ev[1]);
-    if (failed_block) *failed_block = fb;
-    return fb ? BLS_FAILED : BLS_OK;
+    return reduce_qr_sync<double>(h, factored, fres, raw, r, scaling_e, lambda, s, failed_block, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduce_qr_f64_async(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                                        const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
                                        void *stream) {
-    return reduce_qr_async(h, factored, fres, raw, r, scaling_e, lambda, s, d_failed_block, (hipStream_t)stream);
+    return reduce_qr_async<double>(h, factored, fres, raw, r, scaling_e, lambda, s, d_failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_qr_f32(bls_qr *h, float *factored, double *fres, const float *raw, const double *r,
+                                 const double *scaling_e, double lambda, const double *s, int64_t *failed_block,
+                                 void *stream) {
+    return reduce_qr_sync<float>(h, factored, fres, raw, r, scaling_e, lambda, s, failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_qr_f32_async(bls_qr *h, float *factored, double *fres, const float *raw, const double *r,
+                                       const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
+                                       void *stream) {
+    return reduce_qr_async<float>(h, factored, fres, raw, r, scaling_e, lambda, s, d_failed_block, (hipStream_t)stream);
 }
 
 extern "C" int bls_qr_set_timing(bls_qr *h, int enable) {

@@ -17,12 +17,48 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "bls_internal.h"
 
 using namespace bls;
 
 namespace {
+
+// Element types of a precision variant: E and C as the caller stores them, E_inv as cached, R and S of the
+// reduced system. z, g_S, b and the damping are Float64 in all of them (src/schur_reduction.jl:16-92).
+//   f64    SchurBlockHessian{Float64}
+//   f32    SchurBlockHessian{Float32}: the landmark factorisation in Float32 (inv_sym_block in T, so that blocks are
+//          accepted and rejected as the reference's Float32 path decides), E_inv and S stored in Float32
+//   mixed  Float32 blocks E, C, R in; the landmark factorisation, E_inv, S and g_S in Float64 (no reference
+//          counterpart: halves the bytes of the blocks and leaves the reduced system exact for the stored blocks)
+// Products are formed and accumulated in Float64 registers in every variant: without packed FP32 math a Float32
+// FMA issues at the Float64 rate on this hardware, so Float32 pays only through the bytes it saves.
+// TSC: the type a conditioned coupling value scaled(C, s_e s_r) is held in -- the element type T for f64 and f32,
+// as the reference narrows it (src/conditioning.jl:202); Float64 for mixed, whose reduced system is the Float64
+// one of the stored blocks. With conditioning scales phase 2 reads that scaled copy, with the policy `Scaled`.
+struct PrecF64 {
+    using TE = double; using TEI = double; using TC = double; using TS = double; using TR = double;
+    using TSC = double;
+    static constexpr bool F32_FACTOR = false;
+};
+struct PrecF32 {
+    using TE = float; using TEI = float; using TC = float; using TS = float; using TR = float;
+    using TSC = float;
+    static constexpr bool F32_FACTOR = true;
+};
+struct PrecMixed {
+    using TE = float; using TEI = double; using TC = float; using TS = double; using TR = float;
+    using TSC = double;
+    static constexpr bool F32_FACTOR = false;
+};
+// the policy phase 2 runs with when it reads the scaled coupling copy
+template <class P> struct ScaledPolicy { using type = P; };
+template <> struct ScaledPolicy<PrecMixed> { using type = PrecF64; };
+template <class P> constexpr bool all_f64() {
+    return sizeof(typename P::TE) == 8 && sizeof(typename P::TEI) == 8 && sizeof(typename P::TC) == 8 &&
+           sizeof(typename P::TS) == 8;
+}
 
 constexpr int P2_THREADS = 256;
 constexpr int P2_MAXE = 64;        // entries (landmarks) per staged batch
@@ -58,7 +94,10 @@ struct P2Task {
     int16_t slot, row, chunk, nchunk;
 };
 
-struct P2Args {
+template <class P> struct P2Args {
+    using TEI = typename P::TEI;
+    using TC = typename P::TC;
+    using TS = typename P::TS;
     const P2Item *items;
     int64_t nitems;
     const P2Entry *entries;
@@ -67,9 +106,12 @@ struct P2Args {
     const int32_t *slot_dr, *slot_roff;
     const int64_t *eoffsets, *boffsets_e, *coffsets;
     const int32_t *e_dims;
-    const double *E_inv, *z, *C;
+    const TEI *E_inv;
+    const double *z;
+    const TC *C;
     const double *b_r, *s_r;
-    double *S, *g;
+    TS *S;
+    double *g;
     double *partial;
     const unsigned long long *flag;
     int uniform_offsets;   // one run of d_e == DE and one d_r: E/z/C offsets are e*DE^2, e*DE, c*cstride
@@ -80,47 +122,47 @@ struct P2Args {
 // inv_sym_block (src/kernels.jl:14-83). H: DE x DE column-major (scaled), damp: lambda*scale (already formed).
 // ---------------------------------------------------------------------------------------------------------
 #pragma clang fp contract(off)
-template <int DE>
-__device__ inline bool inv_sym(const double *H, const double *damp, double *inv) {
-    const double tol = 4504.0 * 2.220446049250313e-16;
+template <int DE, class F>
+__device__ inline bool inv_sym(const F *H, const F *damp, F *inv) {
+    const F tol = (F)4504 * (sizeof(F) == 8 ? (F)2.220446049250313e-16 : (F)1.1920928955078125e-07);
     if constexpr (DE == 1) {
-        const double h11 = H[0] + damp[0];
+        const F h11 = H[0] + damp[0];
         if (!(h11 > 0) || !isfinite(h11)) return false;
-        inv[0] = 1.0 / h11;
+        inv[0] = (F)1 / h11;
         return true;
     } else if constexpr (DE == 2) {
-        const double H11 = H[0] + damp[0], H22 = H[3] + damp[1], H12 = H[2];
-        const double det = H11 * H22 - H12 * H12;
+        const F H11 = H[0] + damp[0], H22 = H[3] + damp[1], H12 = H[2];
+        const F det = H11 * H22 - H12 * H12;
         if (!(H11 > 0) || !(det > tol * (H11 * H22)) || !isfinite(det)) return false;
-        const double id = 1.0 / det;
+        const F id = (F)1 / det;
         inv[0] = H22 * id; inv[1] = -H12 * id; inv[2] = -H12 * id; inv[3] = H11 * id;
         return true;
     } else if constexpr (DE == 3) {
-        const double H11 = H[0] + damp[0], H22 = H[4] + damp[1], H33 = H[8] + damp[2];
-        const double H12 = H[3], H13 = H[6], H23 = H[7];
-        const double c11 = H22 * H33 - H23 * H23;
-        const double c12 = H13 * H23 - H12 * H33;
-        const double c13 = H12 * H23 - H13 * H22;
-        const double c22 = H11 * H33 - H13 * H13;
-        const double c23 = H12 * H13 - H11 * H23;
-        const double c33 = H11 * H22 - H12 * H12;
-        const double det = H11 * c11 + H12 * c12 + H13 * c13;
+        const F H11 = H[0] + damp[0], H22 = H[4] + damp[1], H33 = H[8] + damp[2];
+        const F H12 = H[3], H13 = H[6], H23 = H[7];
+        const F c11 = H22 * H33 - H23 * H23;
+        const F c12 = H13 * H23 - H12 * H33;
+        const F c13 = H12 * H23 - H13 * H22;
+        const F c22 = H11 * H33 - H13 * H13;
+        const F c23 = H12 * H13 - H11 * H23;
+        const F c33 = H11 * H22 - H12 * H12;
+        const F det = H11 * c11 + H12 * c12 + H13 * c13;
         if (!(H11 > 0) || !(c33 > tol * (H11 * H22)) || !(det > tol * (c33 * H33)) || !isfinite(det)) return false;
-        const double id = 1.0 / det;
+        const F id = (F)1 / det;
         inv[0] = c11 * id; inv[1] = c12 * id; inv[2] = c13 * id;
         inv[3] = c12 * id; inv[4] = c22 * id; inv[5] = c23 * id;
         inv[6] = c13 * id; inv[7] = c23 * id; inv[8] = c33 * id;
         return true;
     } else {
         // Cholesky of Symmetric(H + diag(damp), :U): U'U, every pivot ratio |U_ii|^2 / Hd_ii > tol; inv = U^-1 U^-T
-        double U[DE * DE];
+        F U[DE * DE];
 #pragma unroll
-        for (int q = 0; q < DE * DE; q++) U[q] = 0.0;
+        for (int q = 0; q < DE * DE; q++) U[q] = 0;
 #pragma unroll
         for (int j = 0; j < DE; j++) {
 #pragma unroll
             for (int i = 0; i <= j; i++) {
-                double a = H[j * DE + i] + (i == j ? damp[i] : 0.0);
+                F a = H[j * DE + i] + (i == j ? damp[i] : (F)0);
                 for (int k = 0; k < i; k++) a -= U[i * DE + k] * U[j * DE + k];
                 if (i == j) {
                     if (!(a > 0) || !isfinite(a)) return false;
@@ -130,23 +172,23 @@ __device__ inline bool inv_sym(const double *H, const double *damp, double *inv)
                 }
             }
         }
-        double min_ratio = INFINITY;
+        F min_ratio = (F)INFINITY;
         bool nan_ratio = false;
 #pragma unroll
         for (int i = 0; i < DE; i++) {
-            const double r = (U[i * DE + i] * U[i * DE + i]) / (H[i * DE + i] + damp[i]);
+            const F r = (U[i * DE + i] * U[i * DE + i]) / (H[i * DE + i] + damp[i]);
             if (isnan(r)) nan_ratio = true;
             min_ratio = r < min_ratio ? r : min_ratio;
         }
         if (nan_ratio || !(min_ratio > tol && isfinite(min_ratio))) return false;
-        double Ui[DE * DE];
+        F Ui[DE * DE];
 #pragma unroll
-        for (int q = 0; q < DE * DE; q++) Ui[q] = 0.0;
+        for (int q = 0; q < DE * DE; q++) Ui[q] = 0;
 #pragma unroll
         for (int j = 0; j < DE; j++) {
-            Ui[j * DE + j] = 1.0 / U[j * DE + j];
+            Ui[j * DE + j] = (F)1 / U[j * DE + j];
             for (int i = j - 1; i >= 0; i--) {
-                double s = 0.0;
+                F s = 0;
                 for (int k = i + 1; k <= j; k++) s += U[k * DE + i] * Ui[j * DE + k];
                 Ui[j * DE + i] = -s / U[i * DE + i];
             }
@@ -155,7 +197,7 @@ __device__ inline bool inv_sym(const double *H, const double *damp, double *inv)
         for (int j = 0; j < DE; j++)
 #pragma unroll
             for (int i = 0; i < DE; i++) {
-                double s = 0.0;
+                F s = 0;
                 for (int k = (i > j ? i : j); k < DE; k++) s += Ui[k * DE + i] * Ui[k * DE + j];
                 inv[j * DE + i] = s;
             }
@@ -165,34 +207,39 @@ __device__ inline bool inv_sym(const double *H, const double *damp, double *inv)
 
 // Phase 1 over one run of equal d_e: blocks first..first+count-1 (0-based), E/E_inv at eoff0 + i*DE^2,
 // b_e/z/scaling at boff0 + i*DE.
-template <int DE>
+// The damped, scaled block is formed as the reference forms it (scaled(E, s_i s_j) and T((lambda sc) s^2), in
+// Float64 and narrowed once) and inverted in Float32 for the f32 variant, Float64 otherwise; z in Float64.
+template <int DE, class P>
 __global__ __launch_bounds__(256) void k_phase1(int64_t first, int64_t count, int64_t eoff0, int64_t boff0,
-                                                const double *__restrict__ E, const double *__restrict__ b_e,
+                                                const typename P::TE *__restrict__ E, const double *__restrict__ b_e,
                                                 const double *__restrict__ scaling_e, double lambda,
-                                                const double *__restrict__ s_e, double *__restrict__ E_inv,
+                                                const double *__restrict__ s_e, typename P::TEI *__restrict__ E_inv,
                                                 double *__restrict__ z, unsigned long long *flag) {
+    using F = typename std::conditional<P::F32_FACTOR, float, double>::type;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const int64_t eo = eoff0 + i * DE * DE, bo = boff0 + i * DE;
-    double se[DE], damp[DE], H[DE * DE], inv[DE * DE];
+    double se[DE];
+    F damp[DE], H[DE * DE], inv[DE * DE];
 #pragma unroll
     for (int k = 0; k < DE; k++) {
         se[k] = s_e ? s_e[bo + k] : 1.0;
-        damp[k] = (lambda * scaling_e[bo + k]) * (se[k] * se[k]);
+        damp[k] = (F)((lambda * scaling_e[bo + k]) * (se[k] * se[k]));
     }
 #pragma unroll
-    for (int k = 0; k < DE * DE; k++) H[k] = s_e ? E[eo + k] * (se[k % DE] * se[k / DE]) : E[eo + k];
-    if (!inv_sym<DE>(H, damp, inv)) {
+    for (int k = 0; k < DE * DE; k++)
+        H[k] = s_e ? (F)((double)E[eo + k] * (se[k % DE] * se[k / DE])) : (F)E[eo + k];
+    if (!inv_sym<DE, F>(H, damp, inv)) {
         atomicMin(flag, (unsigned long long)(first + i + 1));
         return;
     }
 #pragma unroll
-    for (int k = 0; k < DE * DE; k++) E_inv[eo + k] = inv[k];
+    for (int k = 0; k < DE * DE; k++) E_inv[eo + k] = (typename P::TEI)inv[k];
 #pragma unroll
     for (int r = 0; r < DE; r++) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < DE; j++) acc += inv[j * DE + r] * (b_e[bo + j] * se[j]);
+        for (int j = 0; j < DE; j++) acc += (double)inv[j * DE + r] * (b_e[bo + j] * se[j]);
         z[bo + r] = acc;
     }
 }
@@ -261,7 +308,8 @@ struct P2Meta {
 // Load up to 64 entries starting at `start` (lane i takes entry start + i). The load is unconditional, from a
 // clamped index (the entry array ends with an unused record, so entry_lo is always a valid index), so that no
 // branch joins on its result and waiting for it never waits for loads issued after it.
-__device__ inline bool p2_load_entries(const P2Args &a, int64_t start, int64_t lo, int64_t end, int lane,
+template <class P>
+__device__ inline bool p2_load_entries(const P2Args<P> &a, int64_t start, int64_t lo, int64_t end, int lane,
                                        P2Entry &en) {
     const bool valid = start + lane < end;
     en = a.entries[valid ? start + lane : lo];
@@ -280,8 +328,8 @@ __device__ inline int wave_incl_scan(int v, int lane) {
 // Wave 0: form the batch from the loaded entries and describe it in `m`; returns its entry count. With one
 // landmark dimension the batch holds up to `nbmax` records of SZ doubles; otherwise it is the longest prefix
 // whose records fit P2_STAGE.
-template <int DE>
-__device__ inline int p2_process(const P2Args &a, const P2Item &it, P2Meta &m, bool valid, const P2Entry &en,
+template <int DE, class P>
+__device__ inline int p2_process(const P2Args<P> &a, const P2Item &it, P2Meta &m, bool valid, const P2Entry &en,
                                  int lane, int nbmax) {
     int de = DE;
     if (DE == 0) de = valid ? a.e_dims[en.e] : 1;
@@ -352,9 +400,45 @@ __device__ inline int seg_lookup(const P2Meta &m, int p) {
 }
 
 // All lanes: issue the loads of a batch's staged data into registers (addresses first, then the loads).
-template <int DE, int DRM>
-__device__ inline void p2_issue(const P2Args &a, const P2Meta &m, int t, int dr2, double *v) {
+template <int DE, int DRM, class P>
+__device__ inline void p2_issue(const P2Args<P> &a, const P2Meta &m, int t, int dr2, double *v) {
     constexpr int QN = P2Width<DE, DRM>::value;
+    if constexpr (!all_f64<P>()) {
+        // typed loads: E_inv (TEI), z (Float64) and C (TC), each promoted to Float64
+        if constexpr (DE > 0) {
+            const int j = t >> 2, part = t & 3;
+            const bool has = j < m.nb;
+            const int jj = has ? j : 0;
+            const int64_t se = m.src_e[jj], sz = m.src_z[jj], sc = m.src_c2[jj];
+#pragma unroll
+            for (int k = 0; k < QN; k++) {
+                const int c = (part - 1) + 3 * (k / DE);
+                const bool ok0 = has && part == 0 && k < DE * DE + DE;
+                const bool ok1 = has && part != 0 && c < dr2;
+                double x = 0.0;
+                if (ok0) x = k < DE * DE ? (double)a.E_inv[se + k] : a.z[sz + (k - DE * DE)];
+                else if (ok1) x = (double)a.C[sc + c * DE + (k % DE)];
+                v[k] = x;
+            }
+        } else {
+            const int L = m.off[m.nb];
+#pragma unroll
+            for (int q = 0; q < QN; q++) {
+                const int p = t + q * P2_THREADS;
+                double x = 0.0;
+                if (p < L) {
+                    const int j = seg_lookup(m, p);
+                    const int de = m.de[j];
+                    const int r = p - m.off[j];
+                    const int d2 = de * de, d3 = d2 + de;
+                    x = r < d2 ? (double)a.E_inv[m.src_e[j] + r]
+                               : (r < d3 ? a.z[m.src_z[j] + (r - d2)] : (double)a.C[m.src_c2[j] + (r - d3)]);
+                }
+                v[q] = x;
+            }
+        }
+        return;
+    } else {
     const double *src[QN];
     if constexpr (DE > 0) {
         // lanes 4j..4j+3 load landmark j: lane 4j its E_inv then z, lane 4j+p (p = 1..3) the C_c2 columns
@@ -390,6 +474,7 @@ __device__ inline void p2_issue(const P2Args &a, const P2Meta &m, int t, int dr2
     }
 #pragma unroll
     for (int q = 0; q < QN; q++) v[q] = *src[q];
+    }
 }
 
 template <int DE, int DRM>
@@ -455,8 +540,10 @@ __device__ inline int nth_set_bit(unsigned long long x, int r) {
 __device__ inline unsigned long long bits_below(int p) { return p >= 64 ? ~0ull : ((1ull << p) - 1ull); }
 
 // at least 3 waves per SIMD (3 workgroups per CU) where the row registers allow it
-template <int DE, int DRM>
-__global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args a) {
+template <int DE, int DRM, class P>
+__global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args<P> a) {
+    using TC = typename P::TC;
+    using TS = typename P::TS;
     constexpr int MERGE = P2_THREADS * (DRM + 1);
     // the update reads DE*DRM doubles of C_c2 (and a column at a row up to MAXDR) whatever d_r2 is: pad each
     // staging buffer by the largest such read
@@ -500,7 +587,7 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
         if (chunk == 0 && it.part == 0) {
 #pragma unroll
             for (int c = 0; c < DRM; c++)
-                if (c < dr2 && (!diag || c >= my_row)) acc[c] = a.S[rowaddr + c * it.colstride];
+                if (c < dr2 && (!diag || c >= my_row)) acc[c] = (double)a.S[rowaddr + c * it.colstride];
             if (diag) gacc = a.b_r[it.ro2 + my_row] * (a.s_r ? a.s_r[it.ro2 + my_row] : 1.0);
         }
     }
@@ -511,13 +598,13 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
     int64_t start = it.entry_lo;
     P2Entry en;
     bool ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);
-    if (wave0) start += p2_process<DE>(a, it, meta[0], ev, en, lane, nbmax);
+    if (wave0) start += p2_process<DE, P>(a, it, meta[0], ev, en, lane, nbmax);
     ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);
-    if (wave0) start += p2_process<DE>(a, it, meta[1], ev, en, lane, nbmax);
+    if (wave0) start += p2_process<DE, P>(a, it, meta[1], ev, en, lane, nbmax);
     __syncthreads();
     double v[QN];
     if (meta[0].nb > 0) {
-        p2_issue<DE, DRM>(a, meta[0], t, dr2, v);
+        p2_issue<DE, DRM, P>(a, meta[0], t, dr2, v);
         p2_store<DE, DRM>(meta[0], t, dr2, v, lds);
     }
     __syncthreads();
@@ -527,7 +614,7 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
         const int mnn = mn == 2 ? 0 : mn + 1;
         const bool have_next = meta[mn].nb > 0;
         ev = p2_load_entries(a, start, it.entry_lo, it.entry_hi, lane, en);   // wave 0 describes the batch after next
-        if (have_next) p2_issue<DE, DRM>(a, meta[mn], t, dr2, v);
+        if (have_next) p2_issue<DE, DRM, P>(a, meta[mn], t, dr2, v);
         const P2Meta &m = meta[mc];
         const double *stg = lds + sb * STAGE;
         unsigned long long bits = 0;
@@ -543,7 +630,7 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
         }
         while (bits) {
             int jj[P2_G];
-            const double *cp[P2_G];
+            const TC *cp[P2_G];
 #pragma unroll
             for (int gi = 0; gi < P2_G; gi++) {
                 const bool hv = bits != 0;
@@ -570,11 +657,11 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
             for (int gi = 0; gi < P2_G; gi++) {
                 if constexpr (DE > 0) {
 #pragma unroll
-                    for (int k = 0; k < DE; k++) cv[gi][k] = cp[gi][k];
+                    for (int k = 0; k < DE; k++) cv[gi][k] = (double)cp[gi][k];
                 } else {
                     const int de = jj[gi] >= 0 ? m.de[jj[gi]] : 1;
 #pragma unroll
-                    for (int k = 0; k < 8; k++) cv[gi][k] = cp[gi][k < de ? k : 0];
+                    for (int k = 0; k < 8; k++) cv[gi][k] = (double)cp[gi][k < de ? k : 0];
                 }
             }
 #pragma unroll
@@ -623,7 +710,7 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
             }
         }
         if (have_next) p2_store<DE, DRM>(meta[mn], t, dr2, v, lds + (sb ^ 1) * STAGE);
-        if (wave0) start += p2_process<DE>(a, it, meta[mnn], ev, en, lane, nbmax);
+        if (wave0) start += p2_process<DE, P>(a, it, meta[mnn], ev, en, lane, nbmax);
         __syncthreads();
         mc = mn;
         sb ^= 1;
@@ -647,22 +734,23 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args 
         if (it.nparts == 1) {
 #pragma unroll
             for (int c = 0; c < DRM; c++)
-                if (c < dr2 && (!diag || c >= my_row)) a.S[rowaddr + c * it.colstride] = acc[c];
+                if (c < dr2 && (!diag || c >= my_row)) a.S[rowaddr + c * it.colstride] = (TS)acc[c];
             if (diag) a.g[it.ro2 + my_row] = gacc;
         } else {
-            double *P = a.partial + it.pbase + (int64_t)(a.slot_roff[it.slot_lo + my_slot] + my_row) * (dr2 + 1);
+            double *prow = a.partial + it.pbase + (int64_t)(a.slot_roff[it.slot_lo + my_slot] + my_row) * (dr2 + 1);
 #pragma unroll
             for (int c = 0; c < DRM; c++)
-                if (c < dr2) P[c] = acc[c];
-            P[dr2] = gacc;
+                if (c < dr2) prow[c] = acc[c];
+            prow[dr2] = gacc;
         }
     }
 }
 
 // Sum the parts of split items in part order and write S (upper triangle) and g: one workgroup per split item,
 // one lane per row of its slots.
+template <class TS>
 __global__ void k_phase2_merge(const P2Merge *merges, const int64_t *slot_rowbase, const int32_t *slot_dr,
-                               const double *partial, double *S, double *g, const unsigned long long *flag) {
+                               const double *partial, TS *S, double *g, const unsigned long long *flag) {
     if (*flag != (unsigned long long)NO_FAILURE) return;
     const P2Merge mg = merges[blockIdx.x];
     const int t = threadIdx.x;
@@ -678,7 +766,7 @@ __global__ void k_phase2_merge(const P2Merge *merges, const int64_t *slot_rowbas
         const double *P = partial + mg.pbase + (int64_t)t * W + c;
         double s = P[0];
         for (int p = 1; p < mg.nparts; p++) s += P[p * pstride];
-        S[rowaddr + c * mg.colstride] = s;
+        S[rowaddr + c * mg.colstride] = (TS)s;
     }
     if (diag) {
         const double *P = partial + mg.pbase + (int64_t)t * W + mg.dr2;
@@ -691,18 +779,21 @@ __global__ void k_phase2_merge(const P2Merge *merges, const int64_t *slot_rowbas
 // ---------------------------------------------------------------------------------------------------------
 // reset_schur_* and copyto!
 // ---------------------------------------------------------------------------------------------------------
-__global__ void k_fill_f64(double *p, int64_t n, double v) {
+template <class T>
+__global__ void k_fill(T *p, int64_t n, T v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-__global__ void k_scatter_r(double *nzval, const double *R, const int64_t *r_to_s, int64_t nR) {
+template <class TS, class TR>
+__global__ void k_scatter_r(TS *nzval, const TR *R, const int64_t *r_to_s, int64_t nR) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t idx = r_to_s[i];
-        if (idx > 0) nzval[idx - 1] = R[i];
+        if (idx > 0) nzval[idx - 1] = (TS)R[i];
     }
 }
 
-__global__ void k_add_damping(double *nzval, const int64_t *diag_inds, int64_t nr, const double *scaling_r,
+template <class TS>
+__global__ void k_add_damping(TS *nzval, const int64_t *diag_inds, int64_t nr, const double *scaling_r,
                               double lambda, const double *s_r) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nr) return;
@@ -710,41 +801,45 @@ __global__ void k_add_damping(double *nzval, const int64_t *diag_inds, int64_t n
 #pragma clang fp contract(off)
         const double si = s_r ? s_r[i] : 1.0;
         const int64_t k = diag_inds[i] - 1;
-        nzval[k] = nzval[k] + lambda * scaling_r[i] * (si * si);
+        nzval[k] = (TS)((double)nzval[k] + lambda * scaling_r[i] * (si * si));
     }
 }
 
-__global__ void k_scale_sparse(double *nzval, const int64_t *S_colptr, const int64_t *S_rowval, int64_t nr,
+template <class TS>
+__global__ void k_scale_sparse(TS *nzval, const int64_t *S_colptr, const int64_t *S_rowval, int64_t nr,
                                const double *s_r) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nr) return;
-    for (int64_t k = S_colptr[j] - 1; k < S_colptr[j + 1] - 1; k++) nzval[k] = nzval[k] * (s_r[S_rowval[k] - 1] * s_r[j]);
+    for (int64_t k = S_colptr[j] - 1; k < S_colptr[j + 1] - 1; k++) nzval[k] = (TS)((double)nzval[k] * (s_r[S_rowval[k] - 1] * s_r[j]));
 }
 
-__global__ void k_reset_dense(double *S, const double *R, int64_t nr, const double *scaling_r, double lambda,
+template <class TS, class TR>
+__global__ void k_reset_dense(TS *S, const TR *R, int64_t nr, const double *scaling_r, double lambda,
                               const double *s_r) {
     const int64_t n2 = nr * nr;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = k % nr, j = k / nr;
-        double v = s_r ? R[k] * (s_r[i] * s_r[j]) : R[k];
+        TS v = s_r ? (TS)((double)R[k] * (s_r[i] * s_r[j])) : (TS)R[k];
         if (i == j) {
 #pragma clang fp contract(off)
             const double si = s_r ? s_r[i] : 1.0;
-            v = v + lambda * scaling_r[i] * (si * si);
+            v = (TS)((double)v + lambda * scaling_r[i] * (si * si));
         }
         S[k] = v;
     }
 }
 
-__global__ void k_gather(double *dest, const double *src, const int64_t *map, int64_t n) {
+template <class T>
+__global__ void k_gather(T *dest, const T *src, const int64_t *map, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dest[i] = src[map[i] - 1];
 }
 
 // backsubstitute! (src/schur_reduction.jl:541-576): one lane per eliminated block
+template <class TC, class TEI, class TSC>
 __global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eoffsets, const int64_t *boffsets_e,
                           const int64_t *eliminated_ptr, const int32_t *entry_retained, const int64_t *coffsets,
-                          const int32_t *r_dims, const int64_t *boffsets_r, const double *C, const double *E_inv,
+                          const int32_t *r_dims, const int64_t *boffsets_r, const TC *C, const TEI *E_inv,
                           const double *z, const double *x_r, const double *s_r, const double *s_e, double *x_e) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= neb) return;
@@ -759,14 +854,16 @@ __global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eof
             const double xc = x_r[ro + ir];
             const double f = s_r ? s_r[ro + ir] : 1.0;
             for (int i = 0; i < de; i++) {
-                const double cv = (s_r || s_e) ? C[co + ir * de + i] * ((s_e ? s_e[bo + i] : 1.0) * f) : C[co + ir * de + i];
+                // scaled(T, C, s_e s_r) is held in TSC before it is promoted (src/conditioning.jl:222-235)
+                const double cv = (s_r || s_e) ? (double)(TSC)((double)C[co + ir * de + i] * ((s_e ? s_e[bo + i] : 1.0) * f))
+                                               : (double)C[co + ir * de + i];
                 t[i] += cv * xc;
             }
         }
     }
     for (int i = 0; i < de; i++) {
         double acc = 0.0;
-        for (int j = 0; j < de; j++) acc += E_inv[eo + j * de + i] * t[j];
+        for (int j = 0; j < de; j++) acc += (double)E_inv[eo + j * de + i] * t[j];
         x_e[bo + i] = z[bo + i] - acc;
     }
 }
@@ -774,9 +871,10 @@ __global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eof
 // Conditioning scales (src/conditioning.jl:99-141): Cs = the value the reference reads for every coupling entry,
 // scaled(C_er[i, ir], s_e[i] s_r[ir]) = C * (s_e * s_r) in Float64, for phase 2 to use in place of C (phase 1 scales
 // E and b_e itself). One lane per eliminated block.
+template <class TC, class TSC>
 __global__ void k_scale_c(int64_t neb, const int32_t *e_dims, const int64_t *boffsets_e, const int64_t *eliminated_ptr,
                           const int32_t *entry_retained, const int64_t *coffsets, const int32_t *r_dims,
-                          const int64_t *boffsets_r, const double *C, const double *s_r, const double *s_e, double *Cs) {
+                          const int64_t *boffsets_r, const TC *C, const double *s_r, const double *s_e, TSC *Cs) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= neb) return;
     const int de = e_dims[e];
@@ -787,7 +885,7 @@ __global__ void k_scale_c(int64_t neb, const int32_t *e_dims, const int64_t *bof
         const int64_t ro = boffsets_r[r], co = coffsets[c];
         for (int ir = 0; ir < dr; ir++) {
             const double f = s_r ? s_r[ro + ir] : 1.0;
-            for (int i = 0; i < de; i++) Cs[co + ir * de + i] = C[co + ir * de + i] * ((s_e ? s_e[bo + i] : 1.0) * f);
+            for (int i = 0; i < de; i++) Cs[co + ir * de + i] = (TSC)((double)C[co + ir * de + i] * ((s_e ? s_e[bo + i] : 1.0) * f));
         }
     }
 }
@@ -819,7 +917,7 @@ struct bls_schur {
     DevArray<int32_t> d_slot_dr, d_slot_roff;
     DevArray<P2Merge> d_merges;
     DevArray<double> d_partial;   // row sums of the parts of split items
-    DevArray<double> d_Cs;        // scaled coupling blocks, when conditioning scales are given
+    DevArray<double> d_Cs;        // scaled coupling blocks (in the variant's C type), when conditioning scales are given
     int64_t clen = 0;             // length of C_data
     int64_t nmerges = 0;
     int uniform_dr_all = 0;
@@ -1011,26 +1109,27 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
     return BLS_OK;
 }
 
-template <int DE> void launch_phase1(const Run &r, const double *E, const double *b_e, const double *sc, double lambda,
-                                     const double *s_e, double *E_inv, double *z, unsigned long long *flag,
-                                     hipStream_t s) {
-    k_phase1<DE><<<grid_for(r.count, 256), 256, 0, s>>>(r.first, r.count, r.eoff0, r.boff0, E, b_e, sc, lambda, s_e,
-                                                         E_inv, z, flag);
+template <int DE, class P>
+void launch_phase1(const Run &r, const typename P::TE *E, const double *b_e, const double *sc, double lambda,
+                   const double *s_e, typename P::TEI *E_inv, double *z, unsigned long long *flag, hipStream_t s) {
+    k_phase1<DE, P><<<grid_for(r.count, 256), 256, 0, s>>>(r.first, r.count, r.eoff0, r.boff0, E, b_e, sc, lambda,
+                                                            s_e, E_inv, z, flag);
 }
 
-int phase1(bls_schur *h, const double *E, const double *b_e, const double *sc, double lambda, const double *s_e,
-           double *E_inv, double *z, hipStream_t s) {
+template <class P>
+int phase1(bls_schur *h, const typename P::TE *E, const double *b_e, const double *sc, double lambda,
+           const double *s_e, typename P::TEI *E_inv, double *z, hipStream_t s) {
     for (const Run &r : h->runs) {
         if (r.count == 0) continue;
         switch (r.de) {
-        case 1: launch_phase1<1>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 2: launch_phase1<2>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 3: launch_phase1<3>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 4: launch_phase1<4>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 5: launch_phase1<5>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 6: launch_phase1<6>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 7: launch_phase1<7>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 8: launch_phase1<8>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 1: launch_phase1<1, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 2: launch_phase1<2, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 3: launch_phase1<3, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 4: launch_phase1<4, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 5: launch_phase1<5, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 6: launch_phase1<6, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 7: launch_phase1<7, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 8: launch_phase1<8, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
         default: return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension above 8");
         }
     }
@@ -1038,49 +1137,25 @@ int phase1(bls_schur *h, const double *E, const double *b_e, const double *sc, d
     return BLS_OK;
 }
 
-template <int DRM> void launch_phase2_dr(int de, unsigned grid, hipStream_t s, const P2Args &a) {
+template <int DRM, class P> void launch_phase2_dr(int de, unsigned grid, hipStream_t s, const P2Args<P> &a) {
     switch (de) {
-    case 1: k_phase2<1, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
-    case 2: k_phase2<2, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
-    case 3: k_phase2<3, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
-    default: k_phase2<0, DRM><<<grid, P2_THREADS, 0, s>>>(a); break;
+    case 1: k_phase2<1, DRM, P><<<grid, P2_THREADS, 0, s>>>(a); break;
+    case 2: k_phase2<2, DRM, P><<<grid, P2_THREADS, 0, s>>>(a); break;
+    case 3: k_phase2<3, DRM, P><<<grid, P2_THREADS, 0, s>>>(a); break;
+    default: k_phase2<0, DRM, P><<<grid, P2_THREADS, 0, s>>>(a); break;
     }
 }
 
-void launch_phase2(int de, int max_dr, unsigned grid, hipStream_t s, const P2Args &a) {
-    if (max_dr <= 6) launch_phase2_dr<6>(de, grid, s, a);
-    else if (max_dr <= 9) launch_phase2_dr<9>(de, grid, s, a);
-    else launch_phase2_dr<16>(de, grid, s, a);
+template <class P> void launch_phase2(int de, int max_dr, unsigned grid, hipStream_t s, const P2Args<P> &a) {
+    if (max_dr <= 6) launch_phase2_dr<6, P>(de, grid, s, a);
+    else if (max_dr <= 9) launch_phase2_dr<9, P>(de, grid, s, a);
+    else launch_phase2_dr<16, P>(de, grid, s, a);
 }
 
-int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double *b_r, const double *b_e,
-                 const double *scaling_e, double lambda, const double *E, const double *C, double *E_inv, double *z,
-                 const double *s_r, const double *s_e, int64_t *d_failed, hipStream_t s) {
-    if (!h) return fail(BLS_ERR_ARG, "null handle");
-    if (h->mode != want_mode)
-        return fail(BLS_ERR_STATE, want_mode == 1 ? "handle has no sparse layout (bls_schur_set_sparse)"
-                                                  : "handle is not dense (bls_schur_set_dense)");
-    if (!(std::isfinite(lambda) && lambda >= 0))
-        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
-    if (!S || !g || !b_r || !scaling_e || !E || !C || !E_inv || !z || !d_failed || (h->neb && !b_e))
-        return fail(BLS_ERR_ARG, "reduce_schur: a required array is NULL");
-    if ((s_r || s_e) && h->clen && !h->d_Cs.ptr) {
-        int rc = h->d_Cs.alloc(h->clen);
-        if (rc) return rc;
-    }
-    h->timing.mark(1, s);
-    k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
-    int rc = phase1(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, s);
-    if (rc) return rc;
-    if ((s_r || s_e) && h->neb) {
-        k_scale_c<<<grid_for(h->neb, 256), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_boffsets_e.ptr,
-                                                         h->d_eliminated_ptr.ptr, h->d_entry_retained.ptr,
-                                                         h->d_coffsets.ptr, h->d_r_dims.ptr, h->d_boffsets_r.ptr, C,
-                                                         s_r, s_e, h->d_Cs.ptr);
-        C = h->d_Cs.ptr;
-    }
-    h->timing.mark(2, s);
-    P2Args a{};
+template <class Q>
+void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, const typename Q::TC *C,
+                const double *b_r, const double *s_r, typename Q::TS *S, double *g, hipStream_t s) {
+    P2Args<Q> a{};
     a.items = h->d_items.ptr;
     a.nitems = h->nitems;
     a.entries = h->d_entries.ptr;
@@ -1104,24 +1179,63 @@ int reduce_async(bls_schur *h, int want_mode, double *S, double *g, const double
     a.uniform_offsets = h->uniform_de > 0 && h->uniform_dr_all > 0 && h->runs.size() == 1;
     a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
     if (h->nitems > 0) {
-        launch_phase2(h->uniform_de, h->max_dr, (unsigned)h->nitems, s, a);
+        launch_phase2<Q>(h->uniform_de, h->max_dr, (unsigned)h->nitems, s, a);
         h->timing.mark(3, s);
         if (h->nmerges > 0)
-            k_phase2_merge<<<(unsigned)h->nmerges, P2_THREADS, 0, s>>>(h->d_merges.ptr, h->d_slot_rowbase.ptr,
-                                                                      h->d_slot_dr.ptr, h->d_partial.ptr, S, g,
-                                                                      h->d_flag.ptr);
+            k_phase2_merge<typename Q::TS><<<(unsigned)h->nmerges, P2_THREADS, 0, s>>>(
+                h->d_merges.ptr, h->d_slot_rowbase.ptr, h->d_slot_dr.ptr, h->d_partial.ptr, S, g, h->d_flag.ptr);
     }
     if (h->nitems == 0) h->timing.mark(3, s);
     h->timing.mark(4, s);
+}
+
+template <class P>
+int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, const double *b_r, const double *b_e,
+                 const double *scaling_e, double lambda, const typename P::TE *E, const typename P::TC *C,
+                 typename P::TEI *E_inv, double *z, const double *s_r, const double *s_e, int64_t *d_failed,
+                 hipStream_t s) {
+    using TC = typename P::TC;
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (h->mode != want_mode)
+        return fail(BLS_ERR_STATE, want_mode == 1 ? "handle has no sparse layout (bls_schur_set_sparse)"
+                                                  : "handle is not dense (bls_schur_set_dense)");
+    if (!(std::isfinite(lambda) && lambda >= 0))
+        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
+    if (!S || !g || !b_r || !scaling_e || !E || !C || !E_inv || !z || !d_failed || (h->neb && !b_e))
+        return fail(BLS_ERR_ARG, "reduce_schur: a required array is NULL");
+    if ((s_r || s_e) && h->clen && !h->d_Cs.ptr) {
+        int rc = h->d_Cs.alloc(h->clen);
+        if (rc) return rc;
+    }
+    h->timing.mark(1, s);
+    k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
+    int rc = phase1<P>(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, s);
+    if (rc) return rc;
+    if ((s_r || s_e) && h->neb) {
+        using Q = typename ScaledPolicy<P>::type;
+        using TSC = typename P::TSC;
+        TSC *Cs = reinterpret_cast<TSC *>(h->d_Cs.ptr);
+        k_scale_c<TC, TSC><<<grid_for(h->neb, 256), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_boffsets_e.ptr,
+                                                                  h->d_eliminated_ptr.ptr, h->d_entry_retained.ptr,
+                                                                  h->d_coffsets.ptr, h->d_r_dims.ptr,
+                                                                  h->d_boffsets_r.ptr, C, s_r, s_e, Cs);
+        h->timing.mark(2, s);
+        run_phase2<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, s);
+    } else {
+        h->timing.mark(2, s);
+        run_phase2<P>(h, E_inv, z, C, b_r, s_r, S, g, s);
+    }
     k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
 
-int reduce_sync(bls_schur *h, int mode, double *S, double *g, const double *b_r, const double *b_e,
-                const double *sc, double lambda, const double *E, const double *C, double *E_inv, double *z,
-                const double *s_r, const double *s_e, int64_t *failed_block, hipStream_t s) {
-    int rc = reduce_async(h, mode, S, g, b_r, b_e, sc, lambda, E, C, E_inv, z, s_r, s_e, h ? h->d_failed_out.ptr : nullptr, s);
+template <class P>
+int reduce_sync(bls_schur *h, int mode, typename P::TS *S, double *g, const double *b_r, const double *b_e,
+                const double *sc, double lambda, const typename P::TE *E, const typename P::TC *C,
+                typename P::TEI *E_inv, double *z, const double *s_r, const double *s_e, int64_t *failed_block,
+                hipStream_t s) {
+    int rc = reduce_async<P>(h, mode, S, g, b_r, b_e, sc, lambda, E, C, E_inv, z, s_r, s_e, h ? h->d_failed_out.ptr : nullptr, s);
     if (rc) return rc;
     int64_t fb = 0;
     BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -1255,53 +1369,97 @@ extern "C" int bls_schur_set_dense(bls_schur *h) {
     return BLS_OK;
 }
 
-extern "C" int bls_reset_schur_sparse_f64(bls_schur *h, double *nzval, const double *R_nzval, const double *scaling_r,
-                                          double lambda, const double *s_r, void *stream) {
+namespace {
+
+template <class TS, class TR>
+int reset_sparse(bls_schur *h, TS *nzval, const TR *R_nzval, const double *scaling_r, double lambda,
+                 const double *s_r, hipStream_t s) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (h->mode != 1) return fail(BLS_ERR_STATE, "handle has no sparse layout (bls_schur_set_sparse)");
     if (!(std::isfinite(lambda) && lambda >= 0))
         return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
     if (!nzval || !scaling_r || (h->nR && !R_nzval)) return fail(BLS_ERR_ARG, "reset_schur_sparse: a required array is NULL");
-    hipStream_t s = (hipStream_t)stream;
     h->timing.mark(0, s);
-    k_fill_f64<<<std::min<int64_t>(grid_for(h->nnz, 256), 4096), 256, 0, s>>>(nzval, h->nnz, 0.0);
-    if (h->nR) k_scatter_r<<<std::min<int64_t>(grid_for(h->nR, 256), 4096), 256, 0, s>>>(nzval, R_nzval, h->d_r_to_s.ptr, h->nR);
+    k_fill<TS><<<std::min<int64_t>(grid_for(h->nnz, 256), 4096), 256, 0, s>>>(nzval, h->nnz, (TS)0);
+    if (h->nR)
+        k_scatter_r<TS, TR><<<std::min<int64_t>(grid_for(h->nR, 256), 4096), 256, 0, s>>>(nzval, R_nzval, h->d_r_to_s.ptr,
+                                                                                          h->nR);
     if (s_r && h->nr)   // scale_sparse! (src/conditioning.jl:247-253)
-        k_scale_sparse<<<grid_for(h->nr, 256), 256, 0, s>>>(nzval, h->d_S_colptr.ptr, h->d_S_rowval.ptr, h->nr, s_r);
-    if (h->nr) k_add_damping<<<grid_for(h->nr, 256), 256, 0, s>>>(nzval, h->d_diag_inds.ptr, h->nr, scaling_r, lambda, s_r);
+        k_scale_sparse<TS><<<grid_for(h->nr, 256), 256, 0, s>>>(nzval, h->d_S_colptr.ptr, h->d_S_rowval.ptr, h->nr, s_r);
+    if (h->nr)
+        k_add_damping<TS><<<grid_for(h->nr, 256), 256, 0, s>>>(nzval, h->d_diag_inds.ptr, h->nr, scaling_r, lambda, s_r);
     h->timing.mark(1, s);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
 
-extern "C" int bls_reset_schur_dense_f64(bls_schur *h, double *S, const double *R, const double *scaling_r,
-                                         double lambda, const double *s_r, void *stream) {
+template <class TS, class TR>
+int reset_dense(bls_schur *h, TS *S, const TR *R, const double *scaling_r, double lambda, const double *s_r,
+                hipStream_t s) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (!(std::isfinite(lambda) && lambda >= 0))
         return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
     if (h->nr && (!S || !R || !scaling_r)) return fail(BLS_ERR_ARG, "reset_schur_dense: a required array is NULL");
-    hipStream_t s = (hipStream_t)stream;
     h->timing.mark(0, s);
-    if (h->nr) k_reset_dense<<<std::min<int64_t>(grid_for(h->nr * h->nr, 256), 4096), 256, 0, s>>>(S, R, h->nr, scaling_r, lambda, s_r);
+    if (h->nr)
+        k_reset_dense<TS, TR><<<std::min<int64_t>(grid_for(h->nr * h->nr, 256), 4096), 256, 0, s>>>(S, R, h->nr, scaling_r,
+                                                                                                   lambda, s_r);
     h->timing.mark(1, s);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
+}
+
+template <class TC, class TEI, class TSC>
+int backsub(bls_schur *h, double *x_e, const double *x_r, const TC *C_data, const TEI *E_inv, const double *z,
+            const double *s_r, const double *s_e, hipStream_t s) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (h->neb && (!x_e || !x_r || !C_data || !E_inv || !z)) return fail(BLS_ERR_ARG, "backsubstitute: a required array is NULL");
+    if (h->neb)
+        k_backsub<TC, TEI, TSC><<<grid_for(h->neb, 256), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr,
+                                                                  h->d_boffsets_e.ptr, h->d_eliminated_ptr.ptr,
+                                                                  h->d_entry_retained.ptr, h->d_coffsets.ptr,
+                                                                  h->d_r_dims.ptr, h->d_boffsets_r.ptr, C_data, E_inv,
+                                                                  z, x_r, s_r, s_e, x_e);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+template <class T> int copyto(int64_t n, T *dest, const T *src, const int64_t *map, hipStream_t s) {
+    if (n < 0) return fail(BLS_ERR_ARG, "copyto: negative length");
+    if (n == 0) return BLS_OK;
+    if (!dest || !src || !map) return fail(BLS_ERR_ARG, "copyto: a required array is NULL");
+    k_gather<T><<<std::min<int64_t>(grid_for(n, 256), 8192), 256, 0, s>>>(dest, src, map, n);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+}  // namespace
+
+// ---- Float64 (SchurBlockHessian{Float64})
+extern "C" int bls_reset_schur_sparse_f64(bls_schur *h, double *nzval, const double *R_nzval, const double *scaling_r,
+                                          double lambda, const double *s_r, void *stream) {
+    return reset_sparse<double, double>(h, nzval, R_nzval, scaling_r, lambda, s_r, (hipStream_t)stream);
+}
+
+extern "C" int bls_reset_schur_dense_f64(bls_schur *h, double *S, const double *R, const double *scaling_r,
+                                         double lambda, const double *s_r, void *stream) {
+    return reset_dense<double, double>(h, S, R, scaling_r, lambda, s_r, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduce_schur_sparse_f64(bls_schur *h, double *nzval, double *g_S, const double *b_r,
                                            const double *b_e, const double *scaling_e, double lambda,
                                            const double *E_data, const double *C_data, double *E_inv, double *z,
                                            const double *s_r, const double *s_e, int64_t *failed_block, void *stream) {
-    return reduce_sync(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e, failed_block,
-                       (hipStream_t)stream);
+    return reduce_sync<PrecF64>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                failed_block, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduce_schur_dense_f64(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
                                           const double *scaling_e, double lambda, const double *E_data,
                                           const double *C_data, double *E_inv, double *z, const double *s_r,
                                           const double *s_e, int64_t *failed_block, void *stream) {
-    return reduce_sync(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e, failed_block,
-                       (hipStream_t)stream);
+    return reduce_sync<PrecF64>(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                failed_block, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduce_schur_sparse_f64_async(bls_schur *h, double *nzval, double *g_S, const double *b_r,
@@ -1309,8 +1467,8 @@ extern "C" int bls_reduce_schur_sparse_f64_async(bls_schur *h, double *nzval, do
                                                  const double *E_data, const double *C_data, double *E_inv, double *z,
                                                  const double *s_r, const double *s_e, int64_t *d_failed_block,
                                                  void *stream) {
-    return reduce_async(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
-                        d_failed_block, (hipStream_t)stream);
+    return reduce_async<PrecF64>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                 d_failed_block, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const double *b_r,
@@ -1318,33 +1476,108 @@ extern "C" int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double 
                                                 const double *E_data, const double *C_data, double *E_inv, double *z,
                                                 const double *s_r, const double *s_e, int64_t *d_failed_block,
                                                 void *stream) {
-    return reduce_async(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e, d_failed_block,
-                        (hipStream_t)stream);
+    return reduce_async<PrecF64>(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                 d_failed_block, (hipStream_t)stream);
 }
 
 extern "C" int bls_backsubstitute_schur_f64(bls_schur *h, double *x_e, const double *x_r, const double *C_data,
                                             const double *E_inv, const double *z, const double *s_r,
                                             const double *s_e, void *stream) {
-    if (!h) return fail(BLS_ERR_ARG, "null handle");
-    if (h->neb && (!x_e || !x_r || !C_data || !E_inv || !z)) return fail(BLS_ERR_ARG, "backsubstitute: a required array is NULL");
-    hipStream_t s = (hipStream_t)stream;
-    if (h->neb)
-        k_backsub<<<grid_for(h->neb, 256), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr, h->d_boffsets_e.ptr,
-                                                          h->d_eliminated_ptr.ptr, h->d_entry_retained.ptr, h->d_coffsets.ptr,
-                                                          h->d_r_dims.ptr, h->d_boffsets_r.ptr, C_data, E_inv, z, x_r, s_r,
-                                                          s_e, x_e);
-    BLS_HIP(hipGetLastError());
-    return BLS_OK;
+    return backsub<double, double, double>(h, x_e, x_r, C_data, E_inv, z, s_r, s_e, (hipStream_t)stream);
 }
 
 extern "C" int bls_copyto_csc_f64(int64_t n, double *dest, const double *src, const int64_t *map, void *stream) {
-    if (n < 0) return fail(BLS_ERR_ARG, "bls_copyto_csc_f64: negative length");
-    if (n == 0) return BLS_OK;
-    if (!dest || !src || !map) return fail(BLS_ERR_ARG, "bls_copyto_csc_f64: a required array is NULL");
-    hipStream_t s = (hipStream_t)stream;
-    k_gather<<<std::min<int64_t>(grid_for(n, 256), 8192), 256, 0, s>>>(dest, src, map, n);
-    BLS_HIP(hipGetLastError());
-    return BLS_OK;
+    return copyto<double>(n, dest, src, map, (hipStream_t)stream);
+}
+
+// ---- Float32 (SchurBlockHessian{Float32})
+extern "C" int bls_reset_schur_sparse_f32(bls_schur *h, float *nzval, const float *R_nzval, const double *scaling_r,
+                                          double lambda, const double *s_r, void *stream) {
+    return reset_sparse<float, float>(h, nzval, R_nzval, scaling_r, lambda, s_r, (hipStream_t)stream);
+}
+
+extern "C" int bls_reset_schur_dense_f32(bls_schur *h, float *S, const float *R, const double *scaling_r,
+                                         double lambda, const double *s_r, void *stream) {
+    return reset_dense<float, float>(h, S, R, scaling_r, lambda, s_r, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_sparse_f32(bls_schur *h, float *nzval, double *g_S, const double *b_r,
+                                           const double *b_e, const double *scaling_e, double lambda,
+                                           const float *E_data, const float *C_data, float *E_inv, double *z,
+                                           const double *s_r, const double *s_e, int64_t *failed_block, void *stream) {
+    return reduce_sync<PrecF32>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_dense_f32(bls_schur *h, float *S, double *g_S, const double *b_r, const double *b_e,
+                                          const double *scaling_e, double lambda, const float *E_data,
+                                          const float *C_data, float *E_inv, double *z, const double *s_r,
+                                          const double *s_e, int64_t *failed_block, void *stream) {
+    return reduce_sync<PrecF32>(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_sparse_f32_async(bls_schur *h, float *nzval, double *g_S, const double *b_r,
+                                                 const double *b_e, const double *scaling_e, double lambda,
+                                                 const float *E_data, const float *C_data, float *E_inv, double *z,
+                                                 const double *s_r, const double *s_e, int64_t *d_failed_block,
+                                                 void *stream) {
+    return reduce_async<PrecF32>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                 d_failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_backsubstitute_schur_f32(bls_schur *h, double *x_e, const double *x_r, const float *C_data,
+                                            const float *E_inv, const double *z, const double *s_r,
+                                            const double *s_e, void *stream) {
+    return backsub<float, float, float>(h, x_e, x_r, C_data, E_inv, z, s_r, s_e, (hipStream_t)stream);
+}
+
+extern "C" int bls_copyto_csc_f32(int64_t n, float *dest, const float *src, const int64_t *map, void *stream) {
+    return copyto<float>(n, dest, src, map, (hipStream_t)stream);
+}
+
+// ---- mixed: Float32 blocks in, Float64 factorisation and reduced system
+extern "C" int bls_reset_schur_sparse_mixed(bls_schur *h, double *nzval, const float *R_nzval,
+                                            const double *scaling_r, double lambda, const double *s_r, void *stream) {
+    return reset_sparse<double, float>(h, nzval, R_nzval, scaling_r, lambda, s_r, (hipStream_t)stream);
+}
+
+extern "C" int bls_reset_schur_dense_mixed(bls_schur *h, double *S, const float *R, const double *scaling_r,
+                                           double lambda, const double *s_r, void *stream) {
+    return reset_dense<double, float>(h, S, R, scaling_r, lambda, s_r, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_sparse_mixed(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                             const double *b_e, const double *scaling_e, double lambda,
+                                             const float *E_data, const float *C_data, double *E_inv, double *z,
+                                             const double *s_r, const double *s_e, int64_t *failed_block,
+                                             void *stream) {
+    return reduce_sync<PrecMixed>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                  failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_dense_mixed(bls_schur *h, double *S, double *g_S, const double *b_r,
+                                            const double *b_e, const double *scaling_e, double lambda,
+                                            const float *E_data, const float *C_data, double *E_inv, double *z,
+                                            const double *s_r, const double *s_e, int64_t *failed_block,
+                                            void *stream) {
+    return reduce_sync<PrecMixed>(h, 2, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                  failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_sparse_mixed_async(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                                   const double *b_e, const double *scaling_e, double lambda,
+                                                   const float *E_data, const float *C_data, double *E_inv,
+                                                   double *z, const double *s_r, const double *s_e,
+                                                   int64_t *d_failed_block, void *stream) {
+    return reduce_async<PrecMixed>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
+                                   d_failed_block, (hipStream_t)stream);
+}
+
+extern "C" int bls_backsubstitute_schur_mixed(bls_schur *h, double *x_e, const double *x_r, const float *C_data,
+                                              const double *E_inv, const double *z, const double *s_r,
+                                              const double *s_e, void *stream) {
+    return backsub<float, double, double>(h, x_e, x_r, C_data, E_inv, z, s_r, s_e, (hipStream_t)stream);
 }
 
 extern "C" int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_map) {

@@ -65,6 +65,7 @@ SIGNATURES = {
     "bls_reduced_tmul_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_reduced_residual_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_reduced_diagonal_blocks_f64": (ct.c_int, [VP, VP, VP, VP]),
+    "bls_pcg_relres_f64": (ct.c_int, [VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
     "bls_pcg_features_f32": (ct.c_int, [VP, VP, ct.c_double, VP, VP]),
     "bls_pcg_shift_blocks_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_mlp_dims": (ct.c_int, [ct.POINTER(ct.c_int)] * 4),
@@ -73,6 +74,28 @@ SIGNATURES = {
     "bls_mlp_adam_f32": (ct.c_int, [VP, VP, VP, VP, ct.c_int64, ct.c_float, VP]),
     "bls_schur_kernel_times": (ct.c_int, [VP, ct.POINTER(ct.c_float), ct.c_int]),
 }
+# Float32 (SchurBlockHessian{Float32} / SchurBlockJacobian{Float32}) and mixed variants: the same argument lists
+for _v in ("f32", "mixed"):
+    for _n in ("bls_reset_schur_sparse", "bls_reset_schur_dense", "bls_reduce_schur_sparse", "bls_reduce_schur_dense",
+               "bls_reduce_schur_sparse_%s_async", "bls_backsubstitute_schur"):
+        _src = _n.replace("%s", "f64") if "%s" in _n else _n + "_f64"
+        SIGNATURES[_n % _v if "%s" in _n else _n + "_" + _v] = SIGNATURES[_src]
+for _n in ("bls_copyto_csc", "bls_reduce_qr", "bls_reduced_mul", "bls_reduced_tmul", "bls_reduced_diagonal_blocks"):
+    SIGNATURES[_n + "_f32"] = SIGNATURES[_n + "_f64"]
+SIGNATURES["bls_reduce_qr_f32_async"] = SIGNATURES["bls_reduce_qr_f64_async"]
+
+
+def _variant(s_dtype, e_dtype):
+    """The precision variant of a call from the element types of the reduced system and of the blocks:
+    f64 (both Float64), f32 (both Float32, SchurBlockHessian{Float32}), mixed (Float32 blocks, Float64 S)."""
+    import torch
+    if s_dtype == torch.float64 and e_dtype == torch.float64:
+        return "f64"
+    if s_dtype == torch.float32 and e_dtype == torch.float32:
+        return "f32"
+    if s_dtype == torch.float64 and e_dtype == torch.float32:
+        return "mixed"
+    raise BLSError(f"no device variant for a reduced system in {s_dtype} from blocks in {e_dtype}")
 
 
 def lib():
@@ -167,13 +190,15 @@ class SchurHandle:
 
 
 class SchurReductionWorkspace:
-    """Device counterpart of SchurReductionWorkspace{Float64} (src/schur_reduction.jl:16-27): E_inv, z and the
-    failure flag of the last reduction."""
+    """Device counterpart of SchurReductionWorkspace{T} (src/schur_reduction.jl:16-27): E_inv, z and the failure
+    flag of the last reduction. variant "f64" / "mixed": E_inv in Float64; "f32": in Float32 (z is Float64)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, variant="f64"):
         import torch
         st = handle.st
-        self.E_inv_data = torch.empty(st.eliminated_storage_length(), dtype=torch.float64, device="cuda")
+        self.variant = variant
+        edt = torch.float32 if variant == "f32" else torch.float64
+        self.E_inv_data = torch.empty(st.eliminated_storage_length(), dtype=edt, device="cuda")
         self.z_data = torch.empty(st.ne, dtype=torch.float64, device="cuda")
         self.d_failed = torch.zeros(1, dtype=torch.int64, device="cuda")
         self.failed_block = 0
@@ -184,15 +209,17 @@ def reset_schur_sparse_(nzval, R_nzval, handle, scaling_r, lam, s_r=None, stream
     scaling_r (a device reduction and a synchronization) for a caller that validated it already."""
     if check:
         _check_damping(lam, scaling_r, "scaling_r")
-    _check(lib().bls_reset_schur_sparse_f64(handle.h, _dp(nzval), _dp(R_nzval), _dp(scaling_r), float(lam), _dp(s_r),
-                                            _stream(stream)))
+    v = _variant(nzval.dtype, R_nzval.dtype if R_nzval is not None else nzval.dtype)
+    f = getattr(lib(), "bls_reset_schur_sparse_" + v)
+    _check(f(handle.h, _dp(nzval), _dp(R_nzval), _dp(scaling_r), float(lam), _dp(s_r), _stream(stream)))
 
 
 def reset_schur_dense_(S, R, handle, scaling_r, lam, s_r=None, stream=None, check=True):
     """reset_schur_dense! (src/schur_reduction.jl:389-407); S, R column-major flat (nr*nr)."""
     if check:
         _check_damping(lam, scaling_r, "scaling_r")
-    _check(lib().bls_reset_schur_dense_f64(handle.h, _dp(S), _dp(R), _dp(scaling_r), float(lam), _dp(s_r), _stream(stream)))
+    f = getattr(lib(), "bls_reset_schur_dense_" + _variant(S.dtype, R.dtype))
+    _check(f(handle.h, _dp(S), _dp(R), _dp(scaling_r), float(lam), _dp(s_r), _stream(stream)))
 
 
 def _check_damping(lam, scaling, name):
@@ -227,14 +254,20 @@ def reduce_schur_dense_(S, g_S, b_r, b_e, scaling_e, lam, handle, E_data, C_data
 
 def _reduce(handle, dense, S, g, b_r, b_e, sc, lam, E, C, ws, s_r, s_e, stream, sync):
     L = lib()
+    v = _variant(S.dtype, E.dtype)
+    if C.dtype != E.dtype or ws.variant != v:
+        raise BLSError(f"the blocks ({E.dtype}, {C.dtype}), S ({S.dtype}) and the workspace ({ws.variant}) disagree")
+    kind = "dense" if dense else "sparse"
     if sync:
         fb = ct.c_int64(0)
-        f = L.bls_reduce_schur_dense_f64 if dense else L.bls_reduce_schur_sparse_f64
+        f = getattr(L, f"bls_reduce_schur_{kind}_{v}")
         rc = _check(f(handle.h, _dp(S), _dp(g), _dp(b_r), _dp(b_e), _dp(sc), float(lam), _dp(E), _dp(C),
                       _dp(ws.E_inv_data), _dp(ws.z_data), _dp(s_r), _dp(s_e), ct.byref(fb), _stream(stream)))
         ws.failed_block = int(fb.value)
         return rc == BLS_OK
-    f = L.bls_reduce_schur_dense_f64_async if dense else L.bls_reduce_schur_sparse_f64_async
+    if dense and v != "f64":
+        raise BLSError("the asynchronous dense reduction is Float64 only")
+    f = getattr(L, f"bls_reduce_schur_{kind}_{v}_async")
     _check(f(handle.h, _dp(S), _dp(g), _dp(b_r), _dp(b_e), _dp(sc), float(lam), _dp(E), _dp(C), _dp(ws.E_inv_data),
              _dp(ws.z_data), _dp(s_r), _dp(s_e), _dp(ws.d_failed), _stream(stream)))
     return None
@@ -242,8 +275,9 @@ def _reduce(handle, dense, S, g, b_r, b_e, sc, lam, E, C, ws, s_r, s_e, stream, 
 
 def backsubstitute_(x_e, x_r, handle, C_data, ws, s_r=None, s_e=None, stream=None):
     """backsubstitute! (src/schur_reduction.jl:519-576)."""
-    _check(lib().bls_backsubstitute_schur_f64(handle.h, _dp(x_e), _dp(x_r), _dp(C_data), _dp(ws.E_inv_data),
-                                              _dp(ws.z_data), _dp(s_r), _dp(s_e), _stream(stream)))
+    f = getattr(lib(), "bls_backsubstitute_schur_" + ws.variant)
+    _check(f(handle.h, _dp(x_e), _dp(x_r), _dp(C_data), _dp(ws.E_inv_data), _dp(ws.z_data), _dp(s_r), _dp(s_e),
+             _stream(stream)))
 
 
 def copyto_(dest, src, scatter_map, stream=None):
@@ -254,7 +288,10 @@ def copyto_(dest, src, scatter_map, stream=None):
         raise ValueError(f"Destination nzval length ({dest.numel()}) does not match scatter map length ({scatter_map.numel()})")
     if scatter_map.dtype != torch.int64:
         raise BLSError("scatter map must be int64")
-    _check(lib().bls_copyto_csc_f64(dest.numel(), _dp(dest), _dp(src), _dp(scatter_map), _stream(stream)))
+    if dest.dtype != src.dtype or dest.dtype not in (torch.float64, torch.float32):
+        raise BLSError("copyto!: dest and src must both be Float64 or both Float32")
+    f = lib().bls_copyto_csc_f64 if dest.dtype == torch.float64 else lib().bls_copyto_csc_f32
+    _check(f(dest.numel(), _dp(dest), _dp(src), _dp(scatter_map), _stream(stream)))
     return dest
 
 
@@ -297,14 +334,14 @@ class QRHandle:
 
 
 class QRReductionWorkspace:
-    """Device counterpart of QRReductionWorkspace{Float64} (src/qr_reduction.jl:11-23): the factored tables, the
-    Float64 factored residual and the failure flag."""
+    """Device counterpart of QRReductionWorkspace{T} (src/qr_reduction.jl:11-23): the factored tables (T, Float64
+    or Float32), the Float64 factored residual and the failure flag."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, dtype=None):
         import torch
         jst = handle.jst
         self.structure = jst
-        self.factored = torch.empty(jst.raw_length(), dtype=torch.float64, device="cuda")
+        self.factored = torch.empty(jst.raw_length(), dtype=dtype or torch.float64, device="cuda")
         self.fres = torch.empty(jst.residual_length(), dtype=torch.float64, device="cuda")
         self.d_failed = torch.zeros(1, dtype=torch.int64, device="cuda")
         self.failed_block = 0
@@ -321,13 +358,16 @@ def reduce_qr_(ws, handle, J_data, J_r, scaling_e, lam, s=None, stream=None, syn
     if check:
         _check_damping(lam, scaling_e, "scaling_e")
     L = lib()
+    if J_data.dtype != ws.factored.dtype:
+        raise BLSError(f"the tables ({J_data.dtype}) and the workspace ({ws.factored.dtype}) disagree")
+    sfx = _fsfx(J_data)
     if sync:
         fb = ct.c_int64(0)
-        rc = _check(L.bls_reduce_qr_f64(handle.h, _dp(ws.factored), _dp(ws.fres), _dp(J_data), _dp(J_r), _dp(scaling_e),
+        rc = _check(getattr(L, "bls_reduce_qr_" + sfx)(handle.h, _dp(ws.factored), _dp(ws.fres), _dp(J_data), _dp(J_r), _dp(scaling_e),
                                         float(lam), _dp(s), ct.byref(fb), _stream(stream)))
         ws.failed_block = int(fb.value)
         return rc == BLS_OK
-    _check(L.bls_reduce_qr_f64_async(handle.h, _dp(ws.factored), _dp(ws.fres), _dp(J_data), _dp(J_r), _dp(scaling_e),
+    _check(getattr(L, f"bls_reduce_qr_{sfx}_async")(handle.h, _dp(ws.factored), _dp(ws.fres), _dp(J_data), _dp(J_r), _dp(scaling_e),
                                      float(lam), _dp(s), _dp(ws.d_failed), _stream(stream)))
     return None
 
@@ -358,6 +398,12 @@ class PCGHandle:
         _check(lib().bls_pcg_values_size(self.h, ct.byref(nv), ct.byref(nf)))
         return int(nf.value)
 
+    def relres(self, S_nzval, g, x, stream=None):
+        """||S x - g|| / ||g|| per problem (FP64, on the device), as a numpy array."""
+        out = (ct.c_double * self.nprob)()
+        _check(lib().bls_pcg_relres_f64(self.h, _dp(S_nzval), _dp(g), _dp(x), out, _stream(stream)))
+        return np.array(out[:])
+
     def solve(self, S_nzval, g, x, tol=1e-8, maxit=10000, precond_blocks=None, stream=None):
         """x = S^-1 g per problem to ||S x - g|| <= tol ||g||. Returns (all_converged, iters[nprob], relres[nprob])."""
         iters = np.zeros(self.nprob, dtype=np.int64)
@@ -369,14 +415,23 @@ class PCGHandle:
 
 
 # ---- products with the reduced Jacobian J~ (src/qr_reduced.jl:161-243, :484-512) ------------------------------
+def _fsfx(t):
+    import torch
+    if t.dtype == torch.float64:
+        return "f64"
+    if t.dtype == torch.float32:
+        return "f32"
+    raise BLSError(f"tables must be Float64 or Float32, got {t.dtype}")
+
+
 def reduced_mul_(y, handle, factored, v, stream=None):
     """reduced_mul!: y = J~ v."""
-    _check(lib().bls_reduced_mul_f64(handle.h, _dp(y), _dp(factored), _dp(v), _stream(stream)))
+    _check(getattr(lib(), "bls_reduced_mul_" + _fsfx(factored))(handle.h, _dp(y), _dp(factored), _dp(v), _stream(stream)))
 
 
 def reduced_tmul_(g, handle, factored, w, stream=None):
     """reduced_tmul!: g = J~' w."""
-    _check(lib().bls_reduced_tmul_f64(handle.h, _dp(g), _dp(factored), _dp(w), _stream(stream)))
+    _check(getattr(lib(), "bls_reduced_tmul_" + _fsfx(factored))(handle.h, _dp(g), _dp(factored), _dp(w), _stream(stream)))
 
 
 def reduced_residual_(rt, handle, fres, stream=None):
@@ -386,4 +441,6 @@ def reduced_residual_(rt, handle, fres, stream=None):
 
 def reduced_diagonal_blocks_(D, handle, factored, stream=None):
     """reduced_diagonal_blocks!: the diagonal blocks of J~'J~ (full, column-major, at doffsets)."""
-    _check(lib().bls_reduced_diagonal_blocks_f64(handle.h, _dp(D), _dp(factored), _stream(stream)))
+    if D.dtype != factored.dtype:
+        raise BLSError("D and the factored tables must have one element type")
+    _check(getattr(lib(), "bls_reduced_diagonal_blocks_" + _fsfx(factored))(handle.h, _dp(D), _dp(factored), _stream(stream)))

@@ -113,6 +113,57 @@ int bls_backsubstitute_schur_f64(bls_schur *h, double *x_e, const double *x_r, c
  * by bls_scatter_map_create). */
 int bls_copyto_csc_f64(int64_t n, double *dest, const double *src, const int64_t *scatter_map_device_i64,
                        void *stream);
+/* ---- Float32 and mixed variants of the Schur path.
+ * _f32: SchurBlockHessian{Float32} (src/schur_hessian.jl:31-77 with T = Float32; the element-type rules of
+ * src/schur_linear_systems.jl:13-21): E, C, R, S and E_inv in Float32; b, z, g_S, the damping and the scales in
+ * Float64. The landmark blocks are inverted in Float32 (inv_sym_block with T = Float32, src/kernels.jl:14-83), so a
+ * block is rejected exactly when the reference's Float32 path rejects it; the phase-2 products are formed and
+ * accumulated in Float64 registers and S is rounded to Float32 once.
+ * _mixed (no reference counterpart; for callers holding Float32 blocks who want the Float64 reduced system): E, C
+ * and R in Float32; the landmark factorisation, E_inv, S and g_S in Float64. S is then the Schur complement of the
+ * stored Float32 blocks to Float64 rounding. Arguments are those of the _f64 entry points. */
+int bls_reset_schur_sparse_f32(bls_schur *h, float *nzval, const float *R_nzval, const double *scaling_r,
+                               double lambda, const double *s_r, void *stream);
+int bls_reset_schur_dense_f32(bls_schur *h, float *S, const float *R, const double *scaling_r, double lambda,
+                              const double *s_r, void *stream);
+int bls_reduce_schur_sparse_f32(bls_schur *h, float *nzval, double *g_S, const double *b_r, const double *b_e,
+                                const double *scaling_e, double lambda, const float *E_data, const float *C_data,
+                                float *E_inv, double *z, const double *s_r, const double *s_e,
+                                int64_t *failed_block, void *stream);
+int bls_reduce_schur_dense_f32(bls_schur *h, float *S, double *g_S, const double *b_r, const double *b_e,
+                               const double *scaling_e, double lambda, const float *E_data, const float *C_data,
+                               float *E_inv, double *z, const double *s_r, const double *s_e,
+                               int64_t *failed_block, void *stream);
+int bls_reduce_schur_sparse_f32_async(bls_schur *h, float *nzval, double *g_S, const double *b_r,
+                                      const double *b_e, const double *scaling_e, double lambda,
+                                      const float *E_data, const float *C_data, float *E_inv, double *z,
+                                      const double *s_r, const double *s_e, int64_t *d_failed_block, void *stream);
+int bls_backsubstitute_schur_f32(bls_schur *h, double *x_e, const double *x_r, const float *C_data,
+                                 const float *E_inv, const double *z, const double *s_r, const double *s_e,
+                                 void *stream);
+int bls_copyto_csc_f32(int64_t n, float *dest, const float *src, const int64_t *scatter_map_device_i64,
+                       void *stream);
+int bls_reset_schur_sparse_mixed(bls_schur *h, double *nzval, const float *R_nzval, const double *scaling_r,
+                                 double lambda, const double *s_r, void *stream);
+int bls_reset_schur_dense_mixed(bls_schur *h, double *S, const float *R, const double *scaling_r, double lambda,
+                                const double *s_r, void *stream);
+int bls_reduce_schur_sparse_mixed(bls_schur *h, double *nzval, double *g_S, const double *b_r, const double *b_e,
+                                  const double *scaling_e, double lambda, const float *E_data, const float *C_data,
+                                  double *E_inv, double *z, const double *s_r, const double *s_e,
+                                  int64_t *failed_block, void *stream);
+int bls_reduce_schur_dense_mixed(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
+                                 const double *scaling_e, double lambda, const float *E_data, const float *C_data,
+                                 double *E_inv, double *z, const double *s_r, const double *s_e,
+                                 int64_t *failed_block, void *stream);
+int bls_reduce_schur_sparse_mixed_async(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                        const double *b_e, const double *scaling_e, double lambda,
+                                        const float *E_data, const float *C_data, double *E_inv, double *z,
+                                        const double *s_r, const double *s_e, int64_t *d_failed_block,
+                                        void *stream);
+int bls_backsubstitute_schur_mixed(bls_schur *h, double *x_e, const double *x_r, const float *C_data,
+                                   const double *E_inv, const double *z, const double *s_r, const double *s_e,
+                                   void *stream);
+
 /* A device copy of a scatter map (1-based Int64, host) for repeated copyto!: returns a device pointer. */
 int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_map);
 
@@ -143,8 +194,18 @@ int bls_reduce_qr_f64(bls_qr *h, double *factored, double *fres, const double *r
 int bls_reduce_qr_f64_async(bls_qr *h, double *factored, double *fres, const double *raw, const double *r,
                             const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
                             void *stream);
+/* reduce_qr! of a SchurBlockJacobian{Float32} (src/qr_linear_systems.jl:2-14: tables and their elimination in T,
+ * the residual in Float64): raw and factored tables in Float32, fres in Float64. The reflectors are formed and
+ * applied in Float64 registers from the Float32 tables and rounded to Float32 once; the singularity test uses
+ * eps(Float32) as the reference's does. */
+int bls_reduce_qr_f32(bls_qr *h, float *factored, double *fres, const float *raw, const double *r,
+                      const double *scaling_e, double lambda, const double *s, int64_t *failed_block, void *stream);
+int bls_reduce_qr_f32_async(bls_qr *h, float *factored, double *fres, const float *raw, const double *r,
+                            const double *scaling_e, double lambda, const double *s, int64_t *d_failed_block,
+                            void *stream);
 /* The table entries and reduced-Jacobian structure of the registered SchurJacobianStructure
- * (src/schur_jacobian.jl:30-160), for the products below and for reduce_qr's conditioning scale: table_top (ntables), reduced_offsets (ntables + 1), table_ptr (ntables + 1), and per table
+ * (src/schur_jacobian.jl:30-160), for the products below and for reduce_qr's conditioning scale: table_top
+ * (ntables), reduced_offsets (ntables + 1), table_ptr (ntables + 1), and per table
  * entry entry_retained, entry_col, entry_table (nentries), retained_ptr (nrblocks + 1), retained_entry (nentries),
  * r_dims / boffsets_r of its SchurStructure, doffsets (nrblocks + 1). All 1-based as stored. */
 int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int64_t *reduced_offsets, const int64_t *table_ptr,
@@ -153,13 +214,16 @@ int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int64_t *reduc
                        const int64_t *r_dims, const int64_t *boffsets_r, const int64_t *doffsets);
 /* reduced_mul! (src/qr_reduced.jl:161-193): y = J~ v, J~ the rows of the factored tables below U. */
 int bls_reduced_mul_f64(bls_qr *h, double *y, const double *factored, const double *v, void *stream);
+int bls_reduced_mul_f32(bls_qr *h, double *y, const float *factored, const double *v, void *stream);
 /* reduced_tmul! (src/qr_reduced.jl:195-223): g = J~' w (deterministic: fixed-order partial sums). */
 int bls_reduced_tmul_f64(bls_qr *h, double *g, const double *factored, const double *w, void *stream);
+int bls_reduced_tmul_f32(bls_qr *h, double *g, const float *factored, const double *w, void *stream);
 /* reduced_residual! (src/qr_reduced.jl:227-243): r~, the factored residual rows below U. */
 int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *fres, void *stream);
 /* reduced_diagonal_blocks! (src/qr_reduced.jl:484-512): the diagonal blocks of J~'J~, full d x d column-major at
  * doffsets (the block-Jacobi blocks of the QR path). */
 int bls_reduced_diagonal_blocks_f64(bls_qr *h, double *D, const double *factored, void *stream);
+int bls_reduced_diagonal_blocks_f32(bls_qr *h, float *D, const float *factored, void *stream);
 /* Kernel time of the last synchronous reduce_qr (HIP events, ms), when enabled. */
 int bls_qr_set_timing(bls_qr *h, int enable);
 int bls_qr_kernel_time(bls_qr *h, float *ms);
@@ -186,6 +250,10 @@ int bls_pcg_values_size(bls_pcg *h, int64_t *nvals, int64_t *nfactor);
  * definite) and relres[p] (host arrays of nprob). BLS_OK when every problem converged, else BLS_FAILED. */
 int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol, int64_t maxit,
                       const double *precond_blocks, int64_t *iters, double *relres, void *stream);
+/* The true residual ||S x - g|| / ||g|| of each problem for a given x (device arrays; relres: host, nprob), e.g.
+ * of one system's solution against another system of the same pattern. Synchronizes `stream`. */
+int bls_pcg_relres_f64(bls_pcg *h, const double *S_nzval, const double *g, const double *x, double *relres,
+                       void *stream);
 
 /* ---- learned correction of the block-Jacobi preconditioner (no reference counterpart) ------------------ */
 

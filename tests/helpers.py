@@ -136,3 +136,38 @@ def reference_schur_problem():
     """The hand-built structure of test/schur.jl:12-66: r_dims [6,3,6,9], e_dims 8 x 3 + 2 x 1, couplings."""
     couplings = [[1, 2], [1, 3], [2, 3, 4], [1, 4], [1, 2, 3], [2, 4], [3, 4], [1, 2, 3, 4], [1, 3], [2, 4]]
     return [6, 3, 6, 9], [3] * 8 + [1, 1], couplings
+
+
+def dense_magnitude_blocks(p):
+    """Per upper block of a dense problem, the magnitude its entries are summed from: |R + damping| plus, over the
+    landmarks, |C_1|' |(E + damping)^-1| |C_2| (entrywise). Rounding in any summation order is bounded by a small
+    multiple of eps times this (the standard bound |fl(sum) - sum| <= gamma_n sum |terms|), which is what a
+    relative error per block cannot promise where the terms cancel."""
+    st = p.structure
+    nr = st.nr
+    M = np.abs(p.R) + p.lam * np.diag(np.asarray(p.scaling_r))
+    for e in range(st.neblocks):
+        de = st.e_dims[e]
+        eo = st.eoffsets[e] - 1
+        bo = st.boffsets_e[e] - 1
+        E = p.E_data[eo:eo + de * de].reshape(de, de).T + p.lam * np.diag(p.scaling_e[bo:bo + de])
+        cols, idx = [], []
+        for c in range(st.eliminated_ptr[e] - 1, st.eliminated_ptr[e + 1] - 1):
+            r = st.entry_retained[c] - 1
+            dr = st.r_dims[r]
+            co = st.coffsets[c] - 1
+            cols.append(p.C_data[co:co + de * dr].reshape(dr, de).T)
+            idx.extend(range(st.boffsets_r[r] - 1, st.boffsets_r[r] - 1 + dr))
+        Ca = np.abs(np.hstack(cols))
+        M[np.ix_(idx, idx)] += Ca.T @ np.abs(np.linalg.inv(E)) @ Ca
+    return dense_upper_blocks(st, M.T.reshape(-1))
+
+
+def max_block_rel_to(A, B, Mag):
+    """max over blocks of ||A_b - B_b||_F / ||Mag_b||_F."""
+    worst = 0.0
+    for k, b in B.items():
+        nb = np.linalg.norm(Mag[k])
+        d = np.linalg.norm(A[k] - b)
+        worst = max(worst, d / nb if nb > 0 else d)
+    return worst

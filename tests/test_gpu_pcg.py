@@ -49,6 +49,12 @@ def test_pcg_single_problem(blsamd):
     args = oracle.SchurArgs(p.structure, p.ws_index, p.pattern)
     xo, ito, relo = oracle.pcg(p.structure, args, S.cpu().numpy(), g.cpu().numpy(), tol=1e-8)
     assert abs(ito - iters[0]) <= max(2, 0.1 * ito)
+    # relres: the true residual of a given x, against the host's dense product
+    xr = torch.as_tensor(np.random.default_rng(1).standard_normal(p.structure.nr)).cuda()
+    gh = g.cpu().numpy()
+    want = np.linalg.norm(Sd @ xr.cpu().numpy() - gh) / np.linalg.norm(gh)
+    assert abs(pcg.relres(S, g, xr)[0] - want) <= 1e-12 * want
+    assert pcg.relres(S, g, x)[0] <= 1.0001e-8
     # deterministic
     x2 = torch.zeros_like(g)
     ok2, iters2, rel2 = pcg.solve(S, g, x2, tol=1e-8)

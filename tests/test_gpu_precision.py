@@ -1,0 +1,355 @@
+"""GPU parity of the Float32 and mixed variants (SchurBlockHessian{Float32}, SchurBlockJacobian{Float32}; the mixed
+Float32-blocks / Float64-system reduction) through the C ABI.
+
+Bounds, all on the same Float32-rounded inputs:
+- f32 against the reference's own Float32 error. With err(X) = the largest relative Frobenius error over the blocks
+  of S (tables of F) against the oracle's Float64 result on those inputs promoted to Float64, the device's Float32
+  result must satisfy err(device) <= 2 err(oracle f32) + 4 eps(Float32): at least about as accurate as the
+  reference's Float32 path. Decisions (the lowest failing block) must equal the oracle's Float32 decisions; the
+  Float64 quantities (g_S, z) agree with the oracle's Float32 run within 1e-10 relative.
+- mixed against the oracle's Float64 reduction of the promoted inputs: 1e-12 relative Frobenius per block.
+- resets and copyto! in Float32: bit for bit.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import dense_magnitude_blocks, dense_upper_blocks, max_block_rel, max_block_rel_to, sparse_upper_blocks
+
+pytestmark = pytest.mark.gpu
+EPS32 = float(np.finfo(np.float32).eps)
+# Float32's acceptance test (4504 eps(Float32) = 5.4e-4, src/kernels.jl:3) rejects some two-view landmark blocks of
+# the sequential-SLAM generator at its lambda = 1e-4, as the reference's Float32 path would; the f32 cases damp
+# with lambda = 1e-2, where every block passes.
+LAM32 = 1e-2
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def _rounded(p):
+    """The problem with E, C and R rounded to Float32 and promoted back (the Float64 view of Float32 storage)."""
+    q = copy.copy(p)
+    q.E_data = p.E_data.astype(np.float32).astype(np.float64)
+    q.C_data = p.C_data.astype(np.float32).astype(np.float64)
+    if p.dense:
+        q.R = p.R.astype(np.float32).astype(np.float64)
+    else:
+        q.R_data = p.R_data.astype(np.float32).astype(np.float64)
+    return q
+
+
+def run_variant(blsamd, p, variant, lam=None, s_r=None, s_e=None):
+    """reset + reduce with the f32 or mixed entry points; returns host copies."""
+    torch = _torch()
+    dev = blsamd.device
+    st = p.structure
+    lam = p.lam if lam is None else lam
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=getattr(p, "r_to_s", None), dense=p.dense)
+    ws = dev.SchurReductionWorkspace(h, variant)
+    sdt = torch.float32 if variant == "f32" else torch.float64
+    T64 = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    T32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    E, C = T32(p.E_data), T32(p.C_data)
+    g = torch.full((st.nr,), float("nan"), dtype=torch.float64, device="cuda")
+    sr, se = T64(s_r), T64(s_e)
+    if p.dense:
+        S = torch.empty(st.nr * st.nr, dtype=sdt, device="cuda")
+        dev.reset_schur_dense_(S, T32(p.R.T.reshape(-1)), h, T64(p.scaling_r), lam, s_r=sr)
+        ok = dev.reduce_schur_dense_(S, g, T64(p.b_r), T64(p.b_e), T64(p.scaling_e), lam, h, E, C, ws, s_r=sr, s_e=se)
+    else:
+        S = torch.empty(p.pattern.nnz, dtype=sdt, device="cuda")
+        dev.reset_schur_sparse_(S, T32(p.R_data), h, T64(p.scaling_r), lam, s_r=sr)
+        ok = dev.reduce_schur_sparse_(S, g, T64(p.b_r), T64(p.b_e), T64(p.scaling_e), lam, h, E, C, ws, s_r=sr, s_e=se)
+    torch.cuda.synchronize()
+    return dict(S=S.cpu().numpy(), g=g.cpu().numpy(), E_inv=ws.E_inv_data.cpu().numpy(), z=ws.z_data.cpu().numpy(),
+                ok=ok, failed_block=ws.failed_block, h=h, ws=ws, C=C)
+
+
+def _oracle(p, dtype, lam=None, s_r=None, s_e=None):
+    if s_r is None and s_e is None:
+        return oracle.schur_reduce_problem(p, dtype=dtype, lam=lam)
+    st = p.structure
+    lam = p.lam if lam is None else lam
+    args = oracle.SchurArgs(st, p.ws_index, p.pattern)
+    E, C = p.E_data.astype(dtype), p.C_data.astype(dtype)
+    E_inv, z, W, g = np.zeros(len(E), dtype), np.zeros(st.ne), np.zeros(len(C), dtype), np.zeros(st.nr)
+    if p.dense:
+        S = np.zeros(st.nr * st.nr, dtype)
+        oracle.reset_schur_dense(S, np.asfortranarray(p.R).T.reshape(-1).astype(dtype), p.scaling_r, lam, s_r)
+    else:
+        S = np.zeros(p.pattern.nnz, dtype)
+        oracle.reset_schur_sparse(S, p.R_data.astype(dtype), p.r_to_s, p.pattern, p.scaling_r, lam, s_r)
+    ok, fb = oracle.reduce_schur(not p.dense, args, S, g, p.b_r, p.b_e, p.scaling_e, lam, E, C, E_inv, z, W, s_r, s_e)
+    return dict(S=S, g=g, E_inv=E_inv, z=z, ok=ok, failed_block=fb, args=args)
+
+
+def _blocks(p, S):
+    st = p.structure
+    S = np.asarray(S, dtype=np.float64)
+    return dense_upper_blocks(st, S) if p.dense else sparse_upper_blocks(st, p.pattern, S)
+
+
+def _check_f32(p, dev32, s_r=None, s_e=None, lam=LAM32):
+    q = _rounded(p)
+    o64 = _oracle(q, np.float64, lam=lam, s_r=s_r, s_e=s_e)
+    o32 = _oracle(p, np.float32, lam=lam, s_r=s_r, s_e=s_e)
+    assert dev32["ok"] and o32["ok"] and o64["ok"]
+    ref = _blocks(p, o64["S"])
+    e_orc = max_block_rel(_blocks(p, o32["S"]), ref)
+    e_dev = max_block_rel(_blocks(p, dev32["S"]), ref)
+    print(f"f32 block error: device {e_dev:.3g}, oracle Float32 {e_orc:.3g}")
+    assert e_dev <= 2 * e_orc + 4 * EPS32
+    for k in ("g", "z"):
+        assert np.abs(dev32[k] - o32[k]).max() <= 1e-10 * max(1.0, np.abs(o32[k]).max()), k
+    assert np.abs(dev32["E_inv"] - o32["E_inv"]).max() <= 4 * EPS32 * np.abs(o32["E_inv"]).max()
+
+
+def _check_mixed(p, devm, s_r=None, s_e=None, magnitude=False):
+    q = _rounded(p)
+    o64 = _oracle(q, np.float64, s_r=s_r, s_e=s_e)
+    assert devm["ok"] and o64["ok"]
+    if magnitude:   # relative to the summands' magnitude (see test_gpu_schur.test_config3_shape_dense)
+        assert max_block_rel_to(_blocks(p, devm["S"]), _blocks(p, o64["S"]), dense_magnitude_blocks(q)) <= 1e-12
+    else:
+        assert max_block_rel(_blocks(p, devm["S"]), _blocks(p, o64["S"])) <= 1e-12
+    for k in ("g", "z", "E_inv"):
+        assert np.abs(devm[k] - o64[k]).max() <= 1e-12 * max(1.0, np.abs(o64[k]).max()), k
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_f32_within_reference_float32_error(blsamd, dense):
+    p = blsamd.generators.config2(scale=0.01 if not dense else 0.003, dense=dense)
+    _check_f32(p, run_variant(blsamd, p, "f32", lam=LAM32))
+
+
+def test_f32_config1_and_split_columns(blsamd):
+    p = blsamd.generators.config1()
+    _check_f32(p, run_variant(blsamd, p, "f32", lam=LAM32))
+    p = blsamd.generators.config2(scale=0.01, ncam=10)   # > 384 landmarks per camera: split items
+    _check_f32(p, run_variant(blsamd, p, "f32", lam=LAM32))
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_mixed_equals_float64_of_float32_blocks(blsamd, dense):
+    p = blsamd.generators.config2(scale=0.01 if not dense else 0.003, dense=dense)
+    _check_mixed(p, run_variant(blsamd, p, "mixed"))
+
+
+def test_mixed_config3_shape(blsamd):
+    """Mixed on the dense configuration's shape (256 cameras, k ~ U{2..64}, per-camera scales 10^U(-2, 2))."""
+    p = blsamd.generators.config3(scale=0.003)
+    _check_mixed(p, run_variant(blsamd, p, "mixed"), magnitude=True)
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_f32_and_mixed_conditioning_scales(blsamd, dense):
+    p = blsamd.generators.config2(scale=0.006 if not dense else 0.003, dense=dense)
+    rng = np.random.default_rng(4)
+    s_r = 10.0 ** rng.uniform(-1, 1, p.structure.nr)
+    s_e = 10.0 ** rng.uniform(-1, 1, p.structure.ne)
+    _check_f32(p, run_variant(blsamd, p, "f32", lam=LAM32, s_r=s_r, s_e=s_e), s_r, s_e)
+    _check_mixed(p, run_variant(blsamd, p, "mixed", s_r=s_r, s_e=s_e), s_r, s_e)
+
+
+def test_f32_rejects_where_the_reference_float32_path_does(blsamd):
+    """Landmark blocks with c33 = 1e-5 H11 H22 pass Float64's test (tol 4504 eps(Float64)) and fail Float32's
+    (tol 4504 eps(Float32) = 5.4e-4; src/kernels.jl:3,38-67): the f32 variant fails at the lowest such block, as
+    the oracle's Float32 path does, while the mixed variant (Float64 factorisation) succeeds."""
+    p = blsamd.generators.config2(scale=0.002)
+    st = p.structure
+    E = p.E_data.copy()
+    for e in (9, 4, 15):
+        eo = st.eoffsets[e - 1] - 1
+        H = np.array([[1.0, 1.0, 0.0], [1.0, 1.0 + 1e-5, 0.0], [0.0, 0.0, 1.0]])
+        E[eo:eo + 9] = H.T.reshape(-1)
+    p.E_data = E
+    o32 = _oracle(p, np.float32, lam=0.0)
+    o64 = _oracle(_rounded(p), np.float64, lam=0.0)
+    assert not o32["ok"] and o32["failed_block"] == 4 and o64["ok"]
+    d32 = run_variant(blsamd, p, "f32", lam=0.0)
+    assert not d32["ok"] and d32["failed_block"] == 4
+    assert np.isnan(d32["g"]).all()
+    dm = run_variant(blsamd, p, "mixed", lam=0.0)
+    assert dm["ok"]
+    assert max_block_rel(_blocks(p, dm["S"]), _blocks(p, o64["S"])) <= 1e-9
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_f32_reset_bitwise(blsamd, dense):
+    """reset_schur_*! in Float32 (damping formed in Float64, narrowed once) equals the oracle's bit for bit."""
+    torch = _torch()
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.005 if not dense else 0.002, dense=dense)
+    st = p.structure
+    s_r = 10.0 ** np.random.default_rng(2).uniform(-1, 1, st.nr)
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=getattr(p, "r_to_s", None), dense=dense)
+    T64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    for sr in (None, s_r):
+        if dense:
+            R = np.asfortranarray(p.R).T.reshape(-1).astype(np.float32)
+            S = torch.empty(st.nr * st.nr, dtype=torch.float32, device="cuda")
+            dev.reset_schur_dense_(S, torch.as_tensor(R).cuda(), h, T64(p.scaling_r), 0.7,
+                                   s_r=None if sr is None else T64(sr))
+            So = np.zeros(st.nr * st.nr, np.float32)
+            oracle.reset_schur_dense(So, R, p.scaling_r, 0.7, sr)
+        else:
+            R = p.R_data.astype(np.float32)
+            S = torch.empty(p.pattern.nnz, dtype=torch.float32, device="cuda")
+            dev.reset_schur_sparse_(S, torch.as_tensor(R).cuda(), h, T64(p.scaling_r), 0.7,
+                                    s_r=None if sr is None else T64(sr))
+            So = np.zeros(p.pattern.nnz, np.float32)
+            oracle.reset_schur_sparse(So, R, p.r_to_s, p.pattern, p.scaling_r, 0.7, sr)
+        assert np.array_equal(S.cpu().numpy(), So)
+
+
+def test_f32_copyto_and_backsubstitute(blsamd):
+    torch = _torch()
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.005)
+    st = p.structure
+    d32 = run_variant(blsamd, p, "f32", lam=LAM32)
+    o32 = _oracle(p, np.float32, lam=LAM32)
+    colptr, rowval, smap = blsamd.structure.reduced_symmetric_scatter(st, p.pattern)
+    dest = torch.empty(len(smap), dtype=torch.float32, device="cuda")
+    dev.copyto_(dest, torch.as_tensor(d32["S"]).cuda(), dev.scatter_map_to_device(smap))
+    assert np.array_equal(dest.cpu().numpy(), d32["S"][smap - 1])
+    x_r = np.random.default_rng(0).standard_normal(st.nr)
+    xe = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    dev.backsubstitute_(xe, torch.as_tensor(x_r).cuda(), d32["h"], d32["C"], d32["ws"])
+    xo = np.zeros(st.ne)
+    oracle.backsubstitute_schur(o32["args"], xo, x_r, p.C_data.astype(np.float32), d32["E_inv"], d32["z"])
+    assert np.abs(xe.cpu().numpy() - xo).max() <= 1e-12 * np.abs(xo).max()
+    dm = run_variant(blsamd, p, "mixed", lam=LAM32)
+    xe = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    dev.backsubstitute_(xe, torch.as_tensor(x_r).cuda(), dm["h"], dm["C"], dm["ws"])
+    xo = np.zeros(st.ne)
+    oracle.backsubstitute_schur(o32["args"], xo, x_r, _rounded(p).C_data, dm["E_inv"], dm["z"])
+    assert np.abs(xe.cpu().numpy() - xo).max() <= 1e-12 * np.abs(xo).max()
+
+
+def test_variant_mismatch_rejected(blsamd):
+    torch = _torch()
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.001)
+    st = p.structure
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=p.r_to_s)
+    ws = dev.SchurReductionWorkspace(h, "f64")
+    f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    f64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    S = torch.zeros(p.pattern.nnz, dtype=torch.float32, device="cuda")
+    g = torch.zeros(st.nr, dtype=torch.float64, device="cuda")
+    with pytest.raises(dev.BLSError):   # Float32 S with a Float64 workspace
+        dev.reduce_schur_sparse_(S, g, f64(p.b_r), f64(p.b_e), f64(p.scaling_e), p.lam, h, f32(p.E_data),
+                                 f32(p.C_data), ws)
+
+
+# ---- QR path in Float32 --------------------------------------------------------------------------------------
+def _qr_tables(blsamd, p, dtype, lam, s=None):
+    torch = _torch()
+    dev = blsamd.device
+    h = dev.QRHandle(p.jst)
+    ws = dev.QRReductionWorkspace(h, dtype=dtype)
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a, dtype=dt)).cuda()
+    ok = dev.reduce_qr_(ws, h, T(p.data, npdt), T(p.r, np.float64), T(p.scaling_e, np.float64), lam,
+                        s=None if s is None else T(s, np.float64))
+    torch.cuda.synchronize()
+    return ok, ws.failed_block, ws.factored, ws.fres, h
+
+
+def _qr_oracle(p, dtype, lam, data=None):
+    jst, st = p.jst, p.structure
+    from blsload import load
+    sargs = oracle.SchurArgs(st, load().structure.SchurReductionWorkspaceIndex(st, 1))
+    jargs = oracle.JacobianArgs(jst)
+    F = np.zeros(jst.raw_length(), dtype)
+    Fr = np.zeros(jst.residual_length())
+    raw = (p.data if data is None else data).astype(dtype)
+    ok, fb = oracle.reduce_qr(sargs, jargs, F, Fr, raw, p.r, p.scaling_e, lam)
+    return ok, fb, F, Fr, sargs, jargs
+
+
+def _table_err(jst, A, B):
+    e = 0.0
+    for t in range(jst.ntables):
+        a, b = jst.raw_offsets[t] - 1, jst.raw_offsets[t + 1] - 1
+        n = np.linalg.norm(B[a:b])
+        if n > 0:
+            e = max(e, np.linalg.norm(A[a:b].astype(np.float64) - B[a:b]) / n)
+    return e
+
+
+def test_qr_f32_within_reference_float32_error(blsamd):
+    torch = _torch()
+    p = blsamd.generators.config4(scale=0.0005, ncam=40, kmax=30)
+    jst = p.jst
+    ok, fb, F, Fr, h = _qr_tables(blsamd, p, torch.float32, 1e-4)
+    rounded = p.data.astype(np.float32).astype(np.float64)
+    ok64, _, F64, Fr64, sargs, jargs = _qr_oracle(p, np.float64, 1e-4, data=rounded)
+    ok32, _, F32, Fr32, _, _ = _qr_oracle(p, np.float32, 1e-4)
+    assert ok and ok64 and ok32
+    Fd, Frd = F.cpu().numpy(), Fr.cpu().numpy()
+    e_dev, e_orc = _table_err(jst, Fd, F64), _table_err(jst, F32, F64)
+    print(f"QR f32 table error: device {e_dev:.3g}, oracle Float32 {e_orc:.3g}")
+    assert e_dev <= 2 * e_orc + 4 * EPS32
+    er_dev = np.abs(Frd - Fr64).max() / np.abs(Fr64).max()
+    er_orc = np.abs(Fr32 - Fr64).max() / np.abs(Fr64).max()
+    assert er_dev <= 2 * er_orc + 4 * EPS32
+    # the reduced products on the Float32 tables: Float64 accumulation, as the oracle's (src/qr_reduced.jl:149-223)
+    st = p.structure
+    dev = blsamd.device
+    rng = np.random.default_rng(1)
+    v, w = rng.standard_normal(st.nr), rng.standard_normal(jst.reduced_nrows())
+    y = torch.empty(jst.reduced_nrows(), dtype=torch.float64, device="cuda")
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    D = torch.empty(int((st.r_dims ** 2).sum()), dtype=torch.float32, device="cuda")
+    dev.reduced_mul_(y, h, F, torch.as_tensor(v).cuda())
+    dev.reduced_tmul_(g, h, F, torch.as_tensor(w).cuda())
+    dev.reduced_diagonal_blocks_(D, h, F)
+    torch.cuda.synchronize()
+    yo, go, Do = np.zeros(len(w)), np.zeros(st.nr), np.zeros(len(D), np.float32)
+    oracle.reduced_mul(sargs, jargs, yo, Fd, v)
+    oracle.reduced_tmul(sargs, jargs, go, Fd, w)
+    oracle.reduced_diagonal_blocks(sargs, jargs, Do, Fd)
+    assert np.abs(y.cpu().numpy() - yo).max() <= 1e-12 * np.abs(yo).max()
+    assert np.abs(g.cpu().numpy() - go).max() <= 1e-12 * np.abs(go).max()
+    assert np.abs(D.cpu().numpy() - Do).max() <= 16 * EPS32 * np.abs(Do).max()
+
+
+def test_qr_f32_rejects_where_the_reference_float32_path_does(blsamd):
+    """Tables whose second eliminated column is the first plus 3e-7 relative noise: |beta| ~ 3e-7 ||col|| is below
+    M eps(Float32) ||col|| and far above M eps(Float64) ||col|| (src/qr_reduction.jl:88-99). The f32 variant fails
+    at the lowest such table, as the oracle's Float32 path does; the Float64 reduction of the same Float32 tables
+    succeeds."""
+    torch = _torch()
+    from test_oracle_qr import _small_jacobian
+    p = _small_jacobian(blsamd, 6)
+    jst, st = p.jst, p.structure
+    rng = np.random.default_rng(3)
+    data = p.data.copy()
+    # the two tallest eliminated tables (M eps(Float32) well above the Float32 rounding noise of the test)
+    es = sorted((np.argsort(-jst.table_nrows[:st.neblocks], kind="stable")[:2] + 1).tolist())
+    for e in es:
+        M, C, de = jst.table_nrows[e - 1], jst.table_ncols[e - 1] - 1, st.e_dims[e - 1]
+        top = jst.table_top[e - 1]
+        off = jst.raw_offsets[e - 1] - 1
+        T = data[off:off + M * C].reshape(C, M)
+        m = M - top
+        T[C - de, :m] = rng.uniform(1.0, 2.0, m)
+        T[C - de + 1, :m] = T[C - de, :m] * (1.0 + 3e-7 * rng.standard_normal(m))
+    p.data = data
+    ok32, fb32, _, _, _, _ = _qr_oracle(p, np.float32, 0.0)
+    ok64, _, _, _, _, _ = _qr_oracle(p, np.float64, 0.0, data=data.astype(np.float32).astype(np.float64))
+    assert not ok32 and fb32 == es[0] and ok64
+    ok, fb, _, _, _ = _qr_tables(blsamd, p, torch.float32, 0.0)
+    assert not ok and fb == es[0]
+    q = copy.copy(p)
+    q.data = data.astype(np.float32).astype(np.float64)
+    ok, fb, _, _, _ = _qr_tables(blsamd, q, torch.float64, 0.0)
+    assert ok

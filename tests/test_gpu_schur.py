@@ -7,8 +7,8 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import (dense_upper_blocks, max_block_rel, problem_from_dense, reference_schur_problem,
-                     schur_test_problem, sparse_upper_blocks, standard_problem_dims)
+from helpers import (dense_magnitude_blocks, dense_upper_blocks, max_block_rel, max_block_rel_to, problem_from_dense,
+                     reference_schur_problem, schur_test_problem, sparse_upper_blocks, standard_problem_dims)
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -78,6 +78,23 @@ def test_config2_small_sparse(blsamd):
 def test_config2_small_dense(blsamd):
     p = blsamd.generators.config2(scale=0.002, dense=True)
     compare(p, run_gpu(blsamd, p), oracle_run(p))
+
+
+def test_config3_shape_dense(blsamd):
+    """The dense configuration's shape at a reduced landmark count: 256 cameras, k ~ U{2..64} random cameras per
+    landmark, Jacobian blocks scaled per camera by 10^U(-2, 2). Blocks joining a 1e2-scaled camera with a
+    1e-2-scaled one cancel their summands to ~1e-9 of their size: two exact-arithmetic-equivalent orders (the
+    oracle and a numpy LU reduction) differ there by 3e-9 relative per block, so S is compared relative to the
+    magnitude it is summed from, |R| + sum |C_1|'|E^-1||C_2| (1e-12); g, E_inv and z as everywhere."""
+    p = blsamd.generators.config3(scale=0.003)
+    assert p.dense and p.structure.nrblocks == 256
+    gpu, orc = run_gpu(blsamd, p), oracle_run(p)
+    st = p.structure
+    assert gpu["ok"] and orc["ok"]
+    mag = dense_magnitude_blocks(p)
+    assert max_block_rel_to(dense_upper_blocks(st, gpu["S"]), dense_upper_blocks(st, orc["S"]), mag) <= TOL
+    for k in ("g", "E_inv", "z"):
+        assert np.abs(gpu[k] - orc[k]).max() <= TOL * max(1.0, np.abs(orc[k]).max()), k
 
 
 @pytest.mark.parametrize("dense", [True, False])
