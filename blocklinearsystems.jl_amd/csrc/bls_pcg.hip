@@ -117,6 +117,59 @@ __global__ __launch_bounds__(256) void k_bsr_spmv_w(PcgArgs a, const double *xin
     }
 }
 
+// The same product when every block is D x D (the camera systems): lane (b, r) of a wavefront takes row r of the
+// b-th block of the pass (64 / D blocks per pass), so a pass reads a contiguous run of the block row's values, row
+// segments of D doubles as 16-byte loads; the lanes of one row r are then summed in fixed lane order through LDS.
+template <int D>
+__global__ __launch_bounds__(256) void k_bsr_spmv_u(PcgArgs a, const double *xin, double *yout, double *pq_part,
+                                                     int only_active) {
+    constexpr int NB = 64 / D;          // blocks per pass
+    __shared__ double red[4][64];
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + w;
+    if (i >= a.nrb) return;
+    if (only_active && !a.st[a.prob_of_block[i]].active) return;
+    const int bsub = lane / D, r = lane - bsub * D;
+    const int k0 = a.bsr_rowptr[i], k1 = a.bsr_rowptr[i + 1];
+    const double *V = a.vals + a.bsr_voff[k0];   // the row's blocks are contiguous, D*D values each
+    double acc = 0.0;
+    if (bsub < NB) {
+        for (int k = k0 + bsub; k < k1; k += NB) {
+            const double *B = V + (int64_t)(k - k0) * (D * D) + r * D;
+            const double *xj = xin + a.bo[a.bsr_col[k]];
+            if constexpr (D % 2 == 0) {
+#pragma unroll
+                for (int c = 0; c < D; c += 2) {
+                    const double2 bv = *reinterpret_cast<const double2 *>(B + c);
+                    const double2 xv = *reinterpret_cast<const double2 *>(xj + c);
+                    acc += bv.x * xv.x;
+                    acc += bv.y * xv.y;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < D; c++) acc += B[c] * xj[c];
+            }
+        }
+    }
+    red[w][lane] = acc;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double y = 0.0, pq = 0.0;
+    const int64_t b = a.bo[i];
+    if (lane < D) {
+#pragma unroll
+        for (int q = 0; q < NB; q++) y += red[w][q * D + lane];
+        yout[b + lane] = y;
+        pq = xin[b + lane] * y;
+    }
+    if (pq_part) {
+        pq = wave_sum64(pq);
+        if (lane == 0) pq_part[i] = pq;
+    }
+}
+
 // z_b = (L L')^-1 r_b for one block (solve_diagonal_blocks!, src/diagonal_blocks.jl:33-61)
 __device__ inline void block_solve(const double *L, int d, const double *rb, double *zb) {
     double y[MAXD];
@@ -410,6 +463,7 @@ struct bls_pcg {
     DevArray<int64_t> d_grp_b0, d_grp_b1;
     int64_t ngroups = 0;
     int maxd = 1;
+    int uniform_d = 0;   // the block dimension when every block has it, else 0
     DevArray<ProbState> d_st;
     DevArray<int> d_nactive;
     hipStream_t stream = nullptr;
@@ -450,6 +504,17 @@ struct bls_pcg {
         return a;
     }
 };
+
+// S x for the handle's blocks: the row-segment kernel when every block has one dimension 6 or 9, else the generic one
+static void launch_spmv(const bls_pcg *h, const PcgArgs &a, const double *xin, double *yout, double *pq,
+                        int only_active, hipStream_t s) {
+    const unsigned grid = (unsigned)((h->nrb + 3) / 4);
+    if (h->uniform_d == 6) k_bsr_spmv_u<6><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    else if (h->uniform_d == 9) k_bsr_spmv_u<9><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    else if (h->maxd <= 6) k_bsr_spmv_w<6><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    else if (h->maxd <= 9) k_bsr_spmv_w<9><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    else k_bsr_spmv_w<16><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+}
 
 extern "C" int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dims, const int64_t *boffsets_r,
                               int64_t nnz, const int64_t *S_colptr, int64_t nnz_blocks, const int64_t *block_colptr,
@@ -528,6 +593,9 @@ extern "C" int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dim
         }
     h->ngroups = (int64_t)grp_prob.size();
     for (int64_t r = 0; r < nrblocks; r++) h->maxd = std::max(h->maxd, (int)r_dims[r]);
+    h->uniform_d = nrblocks ? (int)r_dims[0] : 0;
+    for (int64_t r = 0; r < nrblocks; r++)
+        if (r_dims[r] != h->uniform_d) h->uniform_d = 0;
     h->nvals = nv;
     h->nbsr = (int64_t)col.size();
     h->nL = nl;
@@ -596,10 +664,7 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
     PcgArgs a = h->args(tol, maxit);
     const int nprob = (int)h->nprob;
     auto spmv = [&](const double *xin, double *yout, double *pq, int only_active) {
-        const unsigned grid = (unsigned)((h->nrb + 3) / 4);
-        if (h->maxd <= 6) k_bsr_spmv_w<6><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
-        else if (h->maxd <= 9) k_bsr_spmv_w<9><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
-        else k_bsr_spmv_w<16><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+        launch_spmv(h, a, xin, yout, pq, only_active, s);
     };
     const unsigned ngroups = (unsigned)h->ngroups;
     k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
@@ -664,10 +729,9 @@ extern "C" int bls_pcg_relres_f64(bls_pcg *h, const double *S_nzval, const doubl
     hipStream_t s = (hipStream_t)stream;
     PcgArgs a = h->args(1.0, 0);
     k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
-    const unsigned grid = (unsigned)((h->nrb + 3) / 4);
-    if (h->maxd <= 6) k_bsr_spmv_w<6><<<grid, 256, 0, s>>>(a, x, h->d_q.ptr, nullptr, 0);
-    else if (h->maxd <= 9) k_bsr_spmv_w<9><<<grid, 256, 0, s>>>(a, x, h->d_q.ptr, nullptr, 0);
-    else k_bsr_spmv_w<16><<<grid, 256, 0, s>>>(a, x, h->d_q.ptr, nullptr, 0);
+    // the row-segment product reads x with 16-byte loads: copy the caller's x into an aligned handle buffer
+    k_copy_f64<<<std::min<int64_t>(grid_for(h->nr, 256), 4096), 256, 0, s>>>(h->d_z.ptr, x, h->nr);
+    launch_spmv(h, a, h->d_z.ptr, h->d_q.ptr, nullptr, 0, s);
     k_pcg_relres<<<(unsigned)h->nprob, UPD_THREADS, 0, s>>>(a, g, h->d_pq.ptr);
     BLS_HIP(hipGetLastError());
     BLS_HIP(hipMemcpyAsync(relres, h->d_pq.ptr, sizeof(double) * h->nprob, hipMemcpyDeviceToHost, s));

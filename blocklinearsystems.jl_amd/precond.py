@@ -132,8 +132,9 @@ def target_logit(mu_best):
 
 def solve(pcg, S, g, x, lam, model=None, tol=1e-8, maxit=10000):
     """PCG with the learned correction (model given) or plain block-Jacobi; a learned solve that fails falls back to
-    plain block-Jacobi. Returns (ok, iters, relres, used_learned)."""
-    if model is not None:
+    plain block-Jacobi. Returns (ok, iters, relres, used_learned). A model with output scale 0 shifts no block
+    (mu = 0), so its solve is the plain one and neither features nor the network are evaluated."""
+    if model is not None and model.alpha != 0.0:
         mu = model.mu(features(pcg, S, lam))
         M = shift_blocks(pcg, S, mu)
         x.zero_()
