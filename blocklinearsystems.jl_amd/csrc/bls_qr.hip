@@ -294,6 +294,38 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const T *F,
     }
 }
 
+// backsubstitute! of the QR path (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per eliminated table, one
+// lane per table: q the first d_e rows of the factored residual, B the first d_e rows of the retained columns, U the
+// triangle of the eliminated columns; accumulated in Float64 whatever the table type.
+template <class T>
+__global__ void k_qr_backsub(int64_t neb, const int32_t *nrows, const int32_t *ncols, const int32_t *edims,
+                             const int64_t *raw_off, const int64_t *res_off, const int64_t *boff_e,
+                             const int32_t *table_ptr, const int32_t *entry_retained, const int32_t *entry_col,
+                             const int32_t *r_dims, const int64_t *boff_r, const T *F, const double *Fr,
+                             const double *x_r, double *x_e) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= neb) return;
+    const int de = edims[e], M = nrows[e], ce = ncols[e] - de;
+    const int64_t off = raw_off[e];
+    double t[QR_MAXDE];
+    for (int i = 0; i < de; i++) t[i] = Fr[res_off[e] + i];
+    for (int u = table_ptr[e]; u < table_ptr[e + 1]; u++) {
+        const int r = entry_retained[u];
+        const double *xr = x_r + boff_r[r];
+        const T *cb = F + off + (int64_t)entry_col[u] * M;
+        for (int j = 0; j < r_dims[r]; j++) {
+            const double xj = xr[j];
+            for (int i = 0; i < de; i++) t[i] -= (double)cb[(int64_t)j * M + i] * xj;
+        }
+    }
+    double *xe = x_e + boff_e[e];
+    for (int i = de - 1; i >= 0; i--) {
+        double acc = t[i];
+        for (int j = i + 1; j < de; j++) acc -= (double)F[off + (int64_t)(ce + j) * M + i] * xe[j];
+        xe[i] = acc / (double)F[off + (int64_t)(ce + i) * M + i];
+    }
+}
+
 __global__ void k_qr_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 __global__ void k_qr_finalize(const unsigned long long *flag, int64_t *out) {
     const unsigned long long f = *flag;
@@ -537,6 +569,32 @@ int reduce_qr_sync(bls_qr *h, T *factored, double *fres, const T *raw, const dou
     return fb ? BLS_FAILED : BLS_OK;
 }
 }  // namespace
+
+namespace {
+template <class T> int qr_backsub(bls_qr *h, double *x_e, const double *x_r, const T *F, const double *Fr,
+                                  hipStream_t s) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (h->neb && (!x_e || !x_r || !F || !Fr)) return fail(BLS_ERR_ARG, "backsubstitute: a required array is NULL");
+    if (h->neb)
+        k_qr_backsub<T><<<grid_for(h->neb, 128), 128, 0, s>>>(h->neb, h->d_nrows.ptr, h->d_ncols.ptr, h->d_edims.ptr,
+                                                             h->d_raw_off.ptr, h->d_res_off.ptr, h->d_boff_e.ptr,
+                                                             h->d_table_ptr.ptr, h->d_entry_retained.ptr,
+                                                             h->d_entry_col.ptr, h->d_r_dims.ptr, h->d_boff_r.ptr, F,
+                                                             Fr, x_r, x_e);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_qr_backsubstitute_f64(bls_qr *h, double *x_e, const double *x_r, const double *factored,
+                                         const double *fres, void *stream) {
+    return qr_backsub<double>(h, x_e, x_r, factored, fres, (hipStream_t)stream);
+}
+extern "C" int bls_qr_backsubstitute_f32(bls_qr *h, double *x_e, const double *x_r, const float *factored,
+                                         const double *fres, void *stream) {
+    return qr_backsub<float>(h, x_e, x_r, factored, fres, (hipStream_t)stream);
+}
 
 extern "C" int bls_reduced_mul_f64(bls_qr *h, double *y, const double *F, const double *v, void *stream) {
     return reduced_mul<double>(h, y, F, v, (hipStream_t)stream);

@@ -66,6 +66,14 @@ SIGNATURES = {
     "bls_reduced_residual_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_reduced_diagonal_blocks_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_pcg_relres_f64": (ct.c_int, [VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
+    "bls_qr_backsubstitute_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_qr_backsubstitute_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_diag_create": (ct.c_int, [ct.c_int64, P64, ct.POINTER(VP)]),
+    "bls_diag_destroy": (None, [VP]),
+    "bls_factor_diagonal_blocks_f64": (ct.c_int, [VP, VP, P64, VP]),
+    "bls_factor_diagonal_blocks_f32": (ct.c_int, [VP, VP, P64, VP]),
+    "bls_solve_diagonal_blocks_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_solve_diagonal_blocks_f32": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_pcg_features_f32": (ct.c_int, [VP, VP, ct.c_double, VP, VP]),
     "bls_pcg_shift_blocks_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_mlp_dims": (ct.c_int, [ct.POINTER(ct.c_int)] * 4),
@@ -444,3 +452,41 @@ def reduced_diagonal_blocks_(D, handle, factored, stream=None):
     if D.dtype != factored.dtype:
         raise BLSError("D and the factored tables must have one element type")
     _check(getattr(lib(), "bls_reduced_diagonal_blocks_" + _fsfx(factored))(handle.h, _dp(D), _dp(factored), _stream(stream)))
+
+
+def qr_backsubstitute_(x_e, x_r, handle, ws, stream=None):
+    """backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r)."""
+    _check(getattr(lib(), "bls_qr_backsubstitute_" + _fsfx(ws.factored))(handle.h, _dp(x_e), _dp(x_r),
+                                                                         _dp(ws.factored), _dp(ws.fres),
+                                                                         _stream(stream)))
+
+
+class DiagonalBlocks:
+    """The block sizes of a flat array of square blocks on the device (the C ABI's bls_diag), for
+    factor_diagonal_blocks! / solve_diagonal_blocks! (src/diagonal_blocks.jl:6-61)."""
+
+    def __init__(self, dims):
+        keep = []
+        h = VP()
+        dims = np.ascontiguousarray(dims, dtype=np.int64)
+        _check(lib().bls_diag_create(len(dims), _p64(dims, keep), ct.byref(h)))
+        self.h = h
+        self.dims = dims
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_diag_destroy(self.h)
+            self.h = None
+
+
+def factor_diagonal_blocks_(D, blocks, stream=None):
+    """factor_diagonal_blocks!(D, dims): True, or False with the lowest failing block as second value."""
+    fb = ct.c_int64(0)
+    rc = _check(getattr(lib(), "bls_factor_diagonal_blocks_" + _fsfx(D))(blocks.h, _dp(D), ct.byref(fb), _stream(stream)))
+    return rc == BLS_OK, int(fb.value)
+
+
+def solve_diagonal_blocks_(y, D, blocks, x, stream=None):
+    """solve_diagonal_blocks!(y, D, dims, x): y = (L L')^-1 x per block (Float64 y, x)."""
+    _check(getattr(lib(), "bls_solve_diagonal_blocks_" + _fsfx(D))(blocks.h, _dp(y), _dp(D), _dp(x), _stream(stream)))
+    return y

@@ -212,6 +212,13 @@ int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int64_t *reduc
                        int64_t nentries, const int64_t *entry_retained, const int64_t *entry_col,
                        const int64_t *entry_table, const int64_t *retained_ptr, const int64_t *retained_entry,
                        const int64_t *r_dims, const int64_t *boffsets_r, const int64_t *doffsets);
+/* backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per
+ * eliminated table, from the factored tables and residual of the last reduce_qr; accumulated in Float64. Needs
+ * bls_qr_set_reduced. */
+int bls_qr_backsubstitute_f64(bls_qr *h, double *x_e, const double *x_r, const double *factored, const double *fres,
+                              void *stream);
+int bls_qr_backsubstitute_f32(bls_qr *h, double *x_e, const double *x_r, const float *factored, const double *fres,
+                              void *stream);
 /* reduced_mul! (src/qr_reduced.jl:161-193): y = J~ v, J~ the rows of the factored tables below U. */
 int bls_reduced_mul_f64(bls_qr *h, double *y, const double *factored, const double *v, void *stream);
 int bls_reduced_mul_f32(bls_qr *h, double *y, const float *factored, const double *v, void *stream);
@@ -254,6 +261,24 @@ int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double
  * of one system's solution against another system of the same pattern. Synchronizes `stream`. */
 int bls_pcg_relres_f64(bls_pcg *h, const double *S_nzval, const double *g, const double *x, double *relres,
                        void *stream);
+
+/* ---- block-diagonal kernels (src/diagonal_blocks.jl:6-61) ---------------------------------------------- */
+
+typedef struct bls_diag bls_diag;   /* the block sizes of a flat array of square blocks */
+
+/* Register `dims` (nblocks sizes, 1..16): block b is column-major at the sum of the d^2 before it. */
+int bls_diag_create(int64_t nblocks, const int64_t *dims, bls_diag **out);
+void bls_diag_destroy(bls_diag *h);
+/* factor_diagonal_blocks!(D, dims) (src/diagonal_blocks.jl:6-30): the Cholesky factor of every block in its lower
+ * triangle, in the element type. BLS_FAILED when a block is not positive definite, with the lowest such block
+ * (1-based) in *failed_block; the reference stops at that block, the device factors the others too, and D is then
+ * not to be used, as the reference's caller treats it. Synchronizes `stream`. */
+int bls_factor_diagonal_blocks_f64(bls_diag *h, double *D, int64_t *failed_block, void *stream);
+int bls_factor_diagonal_blocks_f32(bls_diag *h, float *D, int64_t *failed_block, void *stream);
+/* solve_diagonal_blocks!(y, D, dims, x) (src/diagonal_blocks.jl:33-61): y = (L L')^-1 x per block, in Float64;
+ * y may be x. */
+int bls_solve_diagonal_blocks_f64(bls_diag *h, double *y, const double *D, const double *x, void *stream);
+int bls_solve_diagonal_blocks_f32(bls_diag *h, double *y, const float *D, const double *x, void *stream);
 
 /* ---- learned correction of the block-Jacobi preconditioner (no reference counterpart) ------------------ */
 

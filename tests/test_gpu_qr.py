@@ -185,3 +185,28 @@ def test_qr_conditioning_scale(blsamd):
         oko, fbo = oracle.reduce_qr(sargs, oracle.JacobianArgs(jst), Fo, Fro, p.data, p.r, p.scaling_e, 0.25, cs=s)
         assert ok and oko
         compare_tables(p, ws.factored.cpu().numpy(), ws.fres.cpu().numpy(), Fo, Fro)
+
+
+@pytest.mark.parametrize("table_type", ["f64", "f32"])
+def test_qr_backsubstitute_matches_oracle(blsamd, table_type):
+    """backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:24-110) on the device from the
+    device-factored tables, against the oracle on the same tables (1e-12 relative; Float64 accumulation for both
+    table types)."""
+    import torch
+    dev = blsamd.device
+    p = _small_jacobian(blsamd, 12, nland=90, ncam=9, kmax=6)
+    jst, st = p.jst, p.structure
+    dt = torch.float64 if table_type == "f64" else torch.float32
+    h = dev.QRHandle(jst)
+    ws = dev.QRReductionWorkspace(h, dtype=dt)
+    T = lambda a, d=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(d).cuda()
+    assert dev.reduce_qr_(ws, h, T(p.data, dt), T(p.r), T(p.scaling_e), 0.2)
+    x_r = np.random.default_rng(3).standard_normal(st.nr)
+    x_e = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    dev.qr_backsubstitute_(x_e, T(x_r), h, ws)
+    torch.cuda.synchronize()
+    F, Fr = ws.factored.cpu().numpy(), ws.fres.cpu().numpy()
+    sargs = oracle.SchurArgs(st, blsamd.structure.SchurReductionWorkspaceIndex(st, 1))
+    xo = np.zeros(st.ne)
+    oracle.qr_backsubstitute(sargs, oracle.JacobianArgs(jst), xo, x_r, F, Fr)
+    assert np.abs(x_e.cpu().numpy() - xo).max() <= TOL * np.abs(xo).max()
