@@ -1,12 +1,11 @@
 // QR elimination on gfx950: reduce_qr! (src/qr_reduction.jl:162-187) behind the C ABI of include/bls_amd.h.
 //
-// One wavefront per table T_e = [J_r | J_e] (M = m_e + d_e rows, column-major; residual in its own Float64
-// vector). Lane i holds row i of the d_e eliminated columns; the wave computes the d_e Householder reflectors
-// of qr_table! (src/qr_reduction.jl:50-131) cooperatively -- norms by wave reduction, the singularity test
-// |beta| > M eps ||col|| -- and leaves them in LDS. Each lane then streams whole retained columns through
-// registers (at most 64 rows), applies H_1 ... H_de to each in the reference's sequence, and writes it back; one
-// lane does the same for the residual, in Float64. Tables are bucketed by row count (16 / 32 / 64) so that a
-// column fits the registers of its bucket. The raw tables are never modified (retry semantics of the reference).
+// Each table T_e = [J_r | J_e] (M = m_e + d_e rows, column-major; residual in its own Float64 vector) is worked
+// by a group of lanes, one lane per row: the group computes the d_e Householder reflectors of qr_table!
+// (src/qr_reduction.jl:50-131) -- norms and dot products by group reductions, the singularity test
+// |beta| > M eps ||col|| -- and applies them to every retained column and the residual, each read and written as
+// one coalesced run of M values. Tables are bucketed by row count (16 / 32 / 64 lanes per group, 4 / 2 / 1 groups
+// per wavefront). The raw tables are never modified (retry semantics of the reference).
 #include <algorithm>
 #include <cmath>
 
@@ -16,7 +15,6 @@ using namespace bls;
 
 namespace {
 
-constexpr int QR_WAVES = 4;   // tables per workgroup (one wavefront each)
 constexpr int QR_MAXDE = 8;
 
 // T: element type of the raw and factored tables (SchurBlockJacobian{T}); the residual is Float64 and the
@@ -56,131 +54,200 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-template <int MB, class T>
-__global__ __launch_bounds__(64 * QR_WAVES) void k_qr_tables(QRArgs<T> a) {
-    __shared__ double sV[QR_WAVES][QR_MAXDE][64];
-    __shared__ double sTau[QR_WAVES][QR_MAXDE];
+// A 64-bit DPP move within a row of 16 lanes (two 32-bit moves); every lane of the row is active where it is used.
+template <int CTRL> __device__ inline double dpp_mov(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes of a row, returned to all of them: partners at xor 1 and xor 2 (quad_perm), then the other
+// half of each 8 (row_half_mirror) and the other 8 (row_mirror). Each step adds the same two partials in every
+// lane, so all 16 lanes hold the same bits.
+__device__ inline double row_sum16(double v) {
+    v += dpp_mov<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_mov<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_mov<0x141>(v);   // row_half_mirror
+    v += dpp_mov<0x140>(v);   // row_mirror
+    return v;
+}
+
+// One table per row of 16 lanes, four per wavefront: lane l of a table's row holds rows l, l + 16, ... (R of them;
+// bucket R takes tables of at most 16 R rows), so a column is R coalesced runs of 16 values. The lanes first form the
+// d_e reflectors from the eliminated columns (norms and dot products by row sums, the singularity test
+// |beta| > M eps(T) ||col||), keeping v_k and tau_k in registers; they then stream the retained columns and the
+// residual NC at a time, applying H_1 ... H_de to each in the reference's sequence. DEM bounds d_e (3 or 8).
+template <int R, int DEM, class T>
+__global__ __launch_bounds__(256) void k_qr_tables(QRArgs<T> a) {
+    constexpr int NC = R == 4 ? 2 : 4;   // columns a lane holds at once
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int64_t ti = (int64_t)blockIdx.x * QR_WAVES + w;
-    if (ti >= a.ntab) return;
+    const int l = lane & 15;
+    const int base = lane & ~15;
+    const int64_t first = ((int64_t)blockIdx.x * 4 + w) * 4;
+    if (first >= a.ntab) return;   // the whole wavefront is past the last table
+    const int64_t ti = first + (lane >> 4);
+    if (ti >= a.ntab) return;      // this row of lanes has no table
     const int t = a.tables[ti];
     const int M = a.nrows[t];
     const int C = a.ncols[t];
     const int de = a.e_dims[t];
     const int64_t off = a.raw_off[t];
     const int64_t rq = a.res_off[t];
-    double bad = 0.0;
     if (de == 0) {   // a free table: copied (copy_table!, its result unused by reduce_qr!), columns scaled
-        if (!a.cs) {
-            for (int64_t i = lane; i < (int64_t)M * C; i += 64) a.F[off + i] = a.Rw[off + i];
-        } else {
-            for (int j = 0; j < C; j++) {
-                const double f = column_scale(a, t, j);
-                for (int i = lane; i < M; i += 64) a.F[off + (int64_t)j * M + i] = (T)((double)a.Rw[off + (int64_t)j * M + i] * f);
+        for (int j = 0; j < C; j++) {
+            const double f = a.cs ? column_scale(a, t, j) : 1.0;
+            for (int i = l; i < M; i += 16) {
+                const T x = a.Rw[off + (int64_t)j * M + i];
+                a.F[off + (int64_t)j * M + i] = a.cs ? (T)((double)x * f) : x;
             }
         }
-        for (int i = lane; i < M; i += 64) a.Fr[rq + i] = a.Rr[rq + i];
+        for (int i = l; i < M; i += 16) a.Fr[rq + i] = a.Rr[rq + i];
         return;
     }
     const int me = M - de, ce = C - de;
     const int64_t so = a.boff_e[t];
-    // panel: lane i holds row i of the eliminated columns (damping rows zero, sqrt(lambda sc) on their diagonal)
-    double P[QR_MAXDE];
+    double bad = 0.0;
+    // panel: rows l + 16 r of the eliminated columns (damping rows zero, sqrt(lambda sc) s_e on their diagonal)
+    double P[DEM][R];
 #pragma unroll
-    for (int k = 0; k < QR_MAXDE; k++) {
-        P[k] = 0.0;
-        if (k < de) {
-            const double f = a.cs ? a.cs[a.nr + so + k] : 1.0;
-            if (lane < me) {
-                const double x = (double)a.Rw[off + (int64_t)(ce + k) * M + lane];
-                P[k] = (double)(T)(x * f);
-                bad += x - x;
-            } else if (lane == me + k) {
-                P[k] = (double)(T)(sqrt(a.lambda * a.scaling_e[so + k]) * f);
+    for (int k = 0; k < DEM; k++) {
+        const double f = (k < de && a.cs) ? a.cs[a.nr + so + k] : 1.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = r * 16 + l;
+            double v = 0.0;
+            if (k < de) {
+                if (row < me) {
+                    const double x = (double)a.Rw[off + (int64_t)(ce + k) * M + row];
+                    v = (double)(T)(x * f);
+                    bad += x - x;
+                } else if (row == me + k) {
+                    v = (double)(T)(sqrt(a.lambda * a.scaling_e[so + k]) * f);
+                }
             }
+            P[k][r] = v;
         }
     }
     const double tol = (double)M * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07);
+    double V[DEM][R], tau[DEM];
     bool singular = false;
 #pragma unroll
-    for (int k = 0; k < QR_MAXDE; k++) {
+    for (int k = 0; k < DEM; k++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) V[k][r] = 0.0;
+        tau[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < DEM; k++) {
         if (k >= de) break;
-        const double x = lane < M ? P[k] : 0.0;
-        const double s_top = wave_sum(lane < k ? x * x : 0.0);
-        const double s_sub = wave_sum(lane >= k ? x * x : 0.0);
+        double top = 0.0, sub = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = r * 16 + l;
+            const double x = P[k][r];
+            if (row < k) top += x * x;
+            else sub += x * x;
+        }
+        const double s_top = row_sum16(top), s_sub = row_sum16(sub);
         const double nrm = sqrt(s_sub);
         const double full = sqrt(s_top + s_sub);
-        const double akk = __shfl(x, k, 64);
+        const double akk = __shfl(P[k][0], base + k, 64);   // row k (< 16) sits in lane k, slot 0
         const double beta = akk >= 0.0 ? -nrm : nrm;
         if (!(isfinite(beta) && fabs(beta) > tol * full)) { singular = true; break; }
         const double scale = 1.0 / (akk - beta);
-        const double tau = (beta - akk) / beta;
-        const double v = lane > k ? x * scale : (lane == k ? 1.0 : 0.0);
-        sV[w][k][lane] = v;
-        if (lane == 0) sTau[w][k] = tau;
-        P[k] = lane == k ? beta : (lane < k ? x : 0.0);
+        const double tk = (beta - akk) / beta;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = r * 16 + l;
+            const double x = P[k][r];
+            V[k][r] = row > k ? x * scale : (row == k ? 1.0 : 0.0);
+            P[k][r] = row == k ? beta : (row < k ? x : 0.0);
+        }
+        tau[k] = tk;
         // apply H_k to the eliminated columns not yet triangularised
 #pragma unroll
-        for (int j = 0; j < QR_MAXDE; j++) {
+        for (int j = 0; j < DEM; j++) {
             if (j > k && j < de) {
-                const double y = lane < M ? P[j] : 0.0;
-                double s = wave_sum(lane > k ? v * y : 0.0) + __shfl(y, k, 64);
-                s *= tau;
-                if (lane == k) P[j] = y - s;
-                else if (lane > k) P[j] = y - s * v;
+                double part = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (r * 16 + l > k) part += V[k][r] * P[j][r];
+                const double yk = __shfl(P[j][0], base + k, 64);
+                const double sj = (row_sum16(part) + yk) * tk;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int row = r * 16 + l;
+                    if (row == k) P[j][r] -= sj;
+                    else if (row > k) P[j][r] -= sj * V[k][r];
+                }
             }
         }
     }
-    const bool any_bad_panel = __any(isnan(bad));
-    if (singular || any_bad_panel) {
-        if (lane == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
+    const unsigned long long gmask = 0xFFFFull << base;
+    if (singular || (__ballot(isnan(bad)) & gmask)) {
+        if (l == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
         return;
     }
     // eliminated columns of F: [U; 0]
     for (int k = 0; k < de; k++)
-        if (lane < M) a.F[off + (int64_t)(ce + k) * M + lane] = (T)P[k];
-    // sV / sTau are private to this wave: a wavefront-scope fence and wave barrier order its LDS writes before
-    // the reads below (no workgroup barrier: the other waves of the block may have finished already)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // retained columns and the residual: lane-per-column through registers
-    for (int j = lane; j <= ce; j += 64) {
-        const bool res = j == ce;
-        const double f = (a.cs && !res) ? column_scale(a, t, j) : 1.0;
-        double col[MB];
 #pragma unroll
-        for (int i = 0; i < MB; i++) {
-            double x = 0.0;
-            if (i < me) x = res ? a.Rr[rq + i] : (double)a.Rw[off + (int64_t)j * M + i];
-            bad += x - x;
-            col[i] = res ? x : (double)(T)(x * f);
+        for (int r = 0; r < R; r++) {
+            const int row = r * 16 + l;
+            if (row < M) a.F[off + (int64_t)(ce + k) * M + row] = (T)P[k][r];
+        }
+    // retained columns and the residual (column ce), NC at a time
+    for (int j0 = 0; j0 <= ce; j0 += NC) {
+        double x[NC][R];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int j = j0 + c;
+            const double f = (a.cs && j < ce) ? column_scale(a, t, j) : 1.0;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int row = r * 16 + l;
+                double xv = 0.0;
+                if (j < ce) {
+                    if (row < me) xv = (double)a.Rw[off + (int64_t)j * M + row];
+                    bad += xv - xv;
+                    if (a.cs) xv = (double)(T)(xv * f);
+                } else if (j == ce) {
+                    if (row < me) xv = a.Rr[rq + row];
+                    bad += xv - xv;
+                }
+                x[c][r] = xv;
+            }
         }
 #pragma unroll
-        for (int k = 0; k < QR_MAXDE; k++) {
+        for (int k = 0; k < DEM; k++) {
             if (k >= de) break;
-            const double tau = sTau[w][k];
-            double s = 0.0;
+            double sc[NC];
 #pragma unroll
-            for (int i = 0; i < MB; i++)
-                if (i >= k && i < M) s += (i == k ? 1.0 : sV[w][k][i]) * col[i];
-            s *= tau;
+            for (int c = 0; c < NC; c++) {
+                double p = 0.0;
 #pragma unroll
-            for (int i = 0; i < MB; i++)
-                if (i >= k && i < M) col[i] -= s * (i == k ? 1.0 : sV[w][k][i]);
+                for (int r = 0; r < R; r++) p += V[k][r] * x[c][r];
+                sc[c] = row_sum16(p);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int r = 0; r < R; r++) x[c][r] -= (tau[k] * sc[c]) * V[k][r];
         }
-        if (res) {
 #pragma unroll
-            for (int i = 0; i < MB; i++)
-                if (i < M) a.Fr[rq + i] = col[i];
-        } else {
+        for (int c = 0; c < NC; c++) {
+            const int j = j0 + c;
 #pragma unroll
-            for (int i = 0; i < MB; i++)
-                if (i < M) a.F[off + (int64_t)j * M + i] = (T)col[i];
+            for (int r = 0; r < R; r++) {
+                const int row = r * 16 + l;
+                if (row < M) {
+                    if (j < ce) a.F[off + (int64_t)j * M + row] = (T)x[c][r];
+                    else if (j == ce) a.Fr[rq + row] = x[c][r];
+                }
+            }
         }
     }
-    if (__any(isnan(bad)) && lane == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
+    if ((__ballot(isnan(bad)) & gmask) && l == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -336,6 +403,7 @@ __global__ void k_qr_finalize(const unsigned long long *flag, int64_t *out) {
 
 struct bls_qr {
     int64_t nr = 0, nrb = 0, neb = 0, ntables = 0, raw_len = 0, res_len = 0;
+    int max_de = 0;   // largest eliminated block dimension
     std::vector<int32_t> bucket_tables[3];
     DevArray<int32_t> d_bucket[3];
     DevArray<int32_t> d_nrows, d_ncols, d_edims;
@@ -415,10 +483,17 @@ int reduce_qr_async(bls_qr *h, T *F, double *Fr, const T *Rw, const double *Rr, 
         a.tables = h->d_bucket[b].ptr;
         a.ntab = (int64_t)h->bucket_tables[b].size();
         if (!a.ntab) continue;
-        const unsigned grid = (unsigned)((a.ntab + QR_WAVES - 1) / QR_WAVES);
-        if (b == 0) k_qr_tables<16, T><<<grid, 64 * QR_WAVES, 0, s>>>(a);
-        else if (b == 1) k_qr_tables<32, T><<<grid, 64 * QR_WAVES, 0, s>>>(a);
-        else k_qr_tables<64, T><<<grid, 64 * QR_WAVES, 0, s>>>(a);
+        // four wavefronts per workgroup, four tables per wavefront
+        const unsigned grid = (unsigned)((a.ntab + 15) / 16);
+        if (h->max_de <= 3) {
+            if (b == 0) k_qr_tables<1, 3, T><<<grid, 256, 0, s>>>(a);
+            else if (b == 1) k_qr_tables<2, 3, T><<<grid, 256, 0, s>>>(a);
+            else k_qr_tables<4, 3, T><<<grid, 256, 0, s>>>(a);
+        } else {
+            if (b == 0) k_qr_tables<1, QR_MAXDE, T><<<grid, 256, 0, s>>>(a);
+            else if (b == 1) k_qr_tables<2, QR_MAXDE, T><<<grid, 256, 0, s>>>(a);
+            else k_qr_tables<4, QR_MAXDE, T><<<grid, 256, 0, s>>>(a);
+        }
     }
     h->timing.mark(1, s);
     k_qr_finalize<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
@@ -447,6 +522,7 @@ extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, con
         if (t < neblocks) {
             edims[t] = (int32_t)e_dims[t];
             boff[t] = boffsets_e[t] - 1;
+            h->max_de = std::max<int>(h->max_de, (int)e_dims[t]);
             if (e_dims[t] < 1 || e_dims[t] > QR_MAXDE) {
                 delete h;
                 return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension outside 1..8 on the device QR path");
