@@ -536,6 +536,11 @@ extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, con
         const int b = t >= neblocks ? 0 : (table_nrows[t] <= 16 ? 0 : table_nrows[t] <= 32 ? 1 : 2);
         h->bucket_tables[b].push_back((int32_t)t);
     }
+    // within a bucket, tables with the most columns first: the four tables of a wavefront run in lockstep, so
+    // neighbours of similar width waste little, and the dispatcher fills the tail with the narrow ones
+    for (int b = 0; b < 3; b++)
+        std::stable_sort(h->bucket_tables[b].begin(), h->bucket_tables[b].end(),
+                         [&](int32_t x, int32_t y) { return ncols[x] > ncols[y]; });
     h->raw_len = raw_offsets[ntables] - 1;
     h->res_len = residual_offsets[ntables] - 1;
     int rc;
