@@ -1,11 +1,12 @@
 // QR elimination on gfx950: reduce_qr! (src/qr_reduction.jl:162-187) behind the C ABI of include/bls_amd.h.
 //
 // Each table T_e = [J_r | J_e] (M = m_e + d_e rows, column-major; residual in its own Float64 vector) is worked
-// by a group of lanes, one lane per row: the group computes the d_e Householder reflectors of qr_table!
-// (src/qr_reduction.jl:50-131) -- norms and dot products by group reductions, the singularity test
-// |beta| > M eps ||col|| -- and applies them to every retained column and the residual, each read and written as
-// one coalesced run of M values. Tables are bucketed by row count (16 / 32 / 64 lanes per group, 4 / 2 / 1 groups
-// per wavefront). The raw tables are never modified (retry semantics of the reference).
+// by a row of 16 lanes, lane l holding rows l, l + 16, ...: the lanes compute the d_e Householder reflectors of
+// qr_table! (src/qr_reduction.jl:50-131) -- norms and dot products by in-lane partial sums and DPP row sums, the
+// singularity test |beta| > M eps ||col|| -- and apply them to every retained column and the residual, each read
+// and written as coalesced runs of 16 values. Tables are bucketed by row count (at most 16, 32 or 64 rows: 1, 2 or
+// 4 rows per lane, four tables per wavefront) and sorted widest first within a bucket. The raw tables are never
+// modified (retry semantics of the reference).
 #include <algorithm>
 #include <cmath>
 
