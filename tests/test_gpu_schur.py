@@ -317,3 +317,55 @@ def test_conditioning_scales(blsamd, dense):
     dev.backsubstitute_(xe, T(x_r), h, C, ws, s_r=dsr, s_e=dse)
     torch.cuda.synchronize()
     assert np.abs(xe.cpu().numpy() - xe_o).max() <= TOL * max(1.0, np.abs(xe_o).max())
+
+
+@pytest.mark.parametrize("variant", ["f64", "mixed"])
+def test_graph_captured_step_replays(blsamd, variant):
+    """reset_schur_sparse! + reduce_schur_sparse! (asynchronous status) + copyto! captured once in a HIP graph and
+    replayed three times, with eager work on the same buffers between the replays: every replay equals the eager
+    step bit for bit (the path records no memset node: flags and S are cleared by kernels)."""
+    torch = _torch()
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.02)
+    st = p.structure
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=p.r_to_s)
+    ws = dev.SchurReductionWorkspace(h, variant)
+    bt = torch.float32 if variant == "mixed" else torch.float64
+    T = lambda a, d=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(d).cuda()
+    E, C, Rd = T(p.E_data, bt), T(p.C_data, bt), T(p.R_data, bt)
+    b_r, b_e, sr, se = T(p.b_r), T(p.b_e), T(p.scaling_r), T(p.scaling_e)
+    colptr, rowval, smap = blsamd.structure.reduced_symmetric_scatter(st, p.pattern)
+    dmap = dev.scatter_map_to_device(smap)
+    S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    csc = torch.empty(len(smap), dtype=torch.float64, device="cuda")
+
+    def step():
+        dev.reset_schur_sparse_(S, Rd, h, sr, p.lam, check=False)
+        dev.reduce_schur_sparse_(S, g, b_r, b_e, se, p.lam, h, E, C, ws, sync=False, check=False)
+        dev.copyto_(csc, S, dmap)
+
+    step()
+    torch.cuda.synchronize()
+    ref = [x.clone() for x in (S, g, csc)]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for rep in range(3):
+        S.fill_(float("nan"))
+        g.fill_(float("nan"))
+        csc.fill_(0.0)
+        ws.d_failed.fill_(-1)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert int(ws.d_failed.item()) == 0, rep
+        for a, b in zip((S, g, csc), ref):
+            assert torch.equal(a, b), rep
+        step()   # eager work between replays
+        torch.cuda.synchronize()
