@@ -271,8 +271,8 @@ def qr_leg(torch, dev, b, scale, reps=10):
     log(f"QR leg: {ms:.3f} ms, {st.neblocks / (ms * 1e-3):.4g} landmarks/s, {gbs:.0f} GB/s "
         f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM)")
     # SchurBlockJacobian{Float32}: Float32 tables, Float64 residual
-    F64 = ws.factored
-    del raw
+    F64 = ws.factored.clone()
+    raw64 = raw
     ws32 = dev.QRReductionWorkspace(h, dtype=torch.float32)
     raw32 = torch.as_tensor(np.ascontiguousarray(p.data, dtype=np.float32)).cuda()
     assert dev.reduce_qr_(ws32, h, raw32, r, sc, lam)
@@ -290,7 +290,40 @@ def qr_leg(torch, dev, b, scale, reps=10):
         f"||F32 - F64|| / ||F64|| = {rel32:.3g}")
     f32 = {"ms": ms32, "landmarks_per_s": st.neblocks / (ms32 * 1e-3), "achieved_GBs": gbs32,
            "frac": gbs32 / HBM_PEAK_GBS, "rel_diff_vs_f64": rel32}
-    return {"landmark_scale": scale, "f32": f32, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
+    # the reduced system on the device (reduced_normal_sparse!, plus lambda on the retained diagonal) and block-Jacobi
+    # PCG to 1e-8: time to 1e-8 from the Jacobian, with Float64 tables and with Float32 tables
+    pat = b.structure.build_schur_pattern(st)
+    h.set_sparse(pat)
+    pcg = dev.PCGHandle(st, pat)
+    diag = torch.as_tensor(pat.diag_inds - 1).cuda()
+    solve = {}
+    for name, wsx, raw_x, qms in (("f64", ws, raw64, ms), ("f32", ws32, raw32, ms32)):
+        S = torch.empty(pat.nnz, dtype=wsx.factored.dtype, device="cuda")
+        gS = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+        x = torch.zeros(st.nr, dtype=torch.float64, device="cuda")
+        t_n, t_p = [], []
+        for rep_ in range(4):
+            assert dev.reduce_qr_(wsx, h, raw_x, r, sc, lam, check=False)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dev.reduced_normal_sparse_(S, gS, h, wsx)
+            S64 = S if S.dtype == torch.float64 else S.double()
+            S64[diag] += lam
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            x.zero_()
+            ok, iters, rel = pcg.solve(S64, gS, x, tol=1e-8, maxit=5000)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if rep_:
+                t_n.append(t1 - t0)
+                t_p.append(t2 - t1)
+        nm, pm = float(np.median(t_n)) * 1e3, float(np.median(t_p)) * 1e3
+        solve[name] = {"reduced_normal_ms": nm, "pcg_ms": pm, "pcg_iters": int(iters.max()), "relres": float(rel.max()),
+                       "converged": bool(ok), "time_to_1e-8_ms": qms + nm + pm if ok else None}
+        log(f"QR leg, {name} tables: reduced_normal_sparse! + lambda I {nm:.3f} ms, PCG {pm:.3f} ms ({int(iters.max())} "
+            f"iterations), time to 1e-8 from the Jacobian {qms + nm + pm:.3f} ms")
+    return {"landmark_scale": scale, "f32": f32, "solve": solve, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
             "cameras": int(st.nrblocks), "raw_GB": jst.raw_length() * 8 / 1e9, "ms": ms,
             "landmarks_per_s": st.neblocks / (ms * 1e-3),
             "roofline": {"bound": "hbm", "kernel": "k_qr_tables", "achieved": gbs, "peak": HBM_PEAK_GBS,

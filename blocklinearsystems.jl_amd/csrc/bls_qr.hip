@@ -362,6 +362,120 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const T *F,
     }
 }
 
+// reduced_normal_dense! / reduced_normal_sparse! (src/qr_reduced.jl:373-480): the upper triangle of S = J~'J~ and
+// g_S = J~'r~, one workgroup per block column r2. The column is zeroed (every scalar column of it, as the reference
+// does), then the tables holding r2 are taken in the reference's order, one at a time: the workgroup spreads the
+// (u1, row, col) dot products of the table's entries u1 <= u2 over its threads and adds each to its S entry, with a
+// barrier between tables, so every S entry receives its contributions in the reference's table order.
+constexpr int RN_MAXE = 64;   // entries of one table handled per pass
+struct NormalArgs {
+    int dense;
+    int64_t nr;
+    const int64_t *S_colptr;                          // sparse: 0-based start of every scalar column
+    const int32_t *block_colptr, *block_rowval;       // sparse block pattern (0-based)
+    const int64_t *block_row_offsets, *col_heights;
+};
+
+template <class T>
+__global__ __launch_bounds__(RED_THREADS) void k_red_normal(RedArgs a, NormalArgs na, const T *__restrict__ F,
+                                                            const double *__restrict__ Fr, T *__restrict__ S,
+                                                            double *__restrict__ g) {
+    __shared__ int s_pre[RN_MAXE + 1];   // prefix of dr1 * dr2 over the pass's entries u1
+    __shared__ int s_dr1[RN_MAXE], s_col1[RN_MAXE];
+    __shared__ int64_t s_base[RN_MAXE];  // S index of (row 0, col 0) of block (r1, r2), or the dense offset
+    const int r2 = blockIdx.x;
+    const int dr2 = a.r_dims[r2];
+    const int64_t ro2 = a.boff_r[r2];
+    // zero the column and g
+    if (na.dense) {
+        for (int64_t q = threadIdx.x; q < (int64_t)dr2 * na.nr; q += RED_THREADS) S[ro2 * na.nr + q] = (T)0;
+    } else {
+        const int64_t lo = na.S_colptr[ro2], hi = na.S_colptr[ro2 + dr2];
+        for (int64_t q = lo + threadIdx.x; q < hi; q += RED_THREADS) S[q] = (T)0;
+    }
+    if (threadIdx.x < dr2) g[ro2 + threadIdx.x] = 0.0;
+    __syncthreads();
+    for (int k = a.retained_ptr[r2]; k < a.retained_ptr[r2 + 1]; k++) {
+        const int u2 = a.retained_entry[k];
+        const int t = a.entry_table[u2];
+        const int M = a.nrows[t], top = a.top[t], n = M - top;
+        const T *Ft = F + a.raw_off[t] + top;
+        const T *c2 = Ft + (int64_t)a.entry_col[u2] * M;
+        // g_S[r2] += F[:, c2]' r~ (Float64)
+        if (threadIdx.x < dr2) {
+            const double *rt = Fr + a.res_off[t] + top;
+            const T *cc = c2 + (int64_t)threadIdx.x * M;
+            double acc = 0.0;
+            for (int i = 0; i < n; i++) acc += (double)cc[i] * rt[i];
+            g[ro2 + threadIdx.x] += acc;
+        }
+        for (int u0 = a.table_ptr[t]; u0 <= u2; u0 += RN_MAXE) {
+            const int ne = min(RN_MAXE, u2 - u0 + 1);
+            if ((int)threadIdx.x < ne) {
+                const int u1 = u0 + threadIdx.x;
+                const int r1 = a.entry_retained[u1];
+                s_dr1[threadIdx.x] = a.r_dims[r1];
+                s_col1[threadIdx.x] = a.entry_col[u1];
+                if (na.dense) {
+                    s_base[threadIdx.x] = ro2 * na.nr + a.boff_r[r1];
+                } else {   // the block (r1, r2) of the pattern (present: checked when the pattern was registered)
+                    int lo = na.block_colptr[r2], hi = na.block_colptr[r2 + 1] - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (na.block_rowval[mid] < r1) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    s_base[threadIdx.x] = na.S_colptr[ro2] + na.block_row_offsets[lo];
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int acc = 0;
+                for (int e = 0; e < ne; e++) {
+                    s_pre[e] = acc;
+                    acc += s_dr1[e] * dr2;
+                }
+                s_pre[ne] = acc;
+            }
+            __syncthreads();
+            const int64_t cstride = na.dense ? na.nr : na.col_heights[r2];
+            for (int q = threadIdx.x; q < s_pre[ne]; q += RED_THREADS) {
+                int lo = 0, hi = ne - 1;   // the entry e with s_pre[e] <= q < s_pre[e + 1]
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= q) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int e = lo;
+                const int dr1 = s_dr1[e];
+                const int w = q - s_pre[e];
+                const int col = w / dr1, row = w - col * dr1;
+                if (u0 + e == u2 && row > col) continue;   // diagonal block: upper triangle only
+                const T *x = Ft + (int64_t)(s_col1[e] + row) * M;
+                const T *y = c2 + (int64_t)col * M;
+                const int64_t di = s_base[e] + (int64_t)col * cstride + row;
+                const double d0 = (double)S[di];
+                // eight products in flight per step, summed in index order
+                double acc = 0.0;
+                int i = 0;
+                for (; i + 8 <= n; i += 8) {
+                    double xv[8], yv[8];
+#pragma unroll
+                    for (int q8 = 0; q8 < 8; q8++) {
+                        xv[q8] = (double)x[i + q8];
+                        yv[q8] = (double)y[i + q8];
+                    }
+#pragma unroll
+                    for (int q8 = 0; q8 < 8; q8++) acc += xv[q8] * yv[q8];
+                }
+                for (; i < n; i++) acc += (double)x[i] * (double)y[i];
+                S[di] = (T)(d0 + acc);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // backsubstitute! of the QR path (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per eliminated table, one
 // lane per table: q the first d_e rows of the factored residual, B the first d_e rows of the retained columns, U the
 // triangle of the eliminated columns; accumulated in Float64 whatever the table type.
@@ -418,6 +532,11 @@ struct bls_qr {
     DevArray<int32_t> d_top, d_table_ptr, d_entry_retained, d_entry_col, d_entry_table, d_retained_ptr,
         d_retained_entry, d_r_dims;
     DevArray<int64_t> d_red_off, d_boff_r, d_doff;
+    // the sparse reduced-system layout (bls_qr_set_sparse)
+    bool has_sparse = false;
+    int64_t S_nnz = 0;
+    DevArray<int64_t> d_S_colptr, d_block_row_offsets, d_col_heights;
+    DevArray<int32_t> d_block_colptr, d_block_rowval;
     RedArgs red() const {
         RedArgs a{};
         a.ntables = ntables;
@@ -668,6 +787,80 @@ template <class T> int qr_backsub(bls_qr *h, double *x_e, const double *x_r, con
     return BLS_OK;
 }
 }  // namespace
+
+extern "C" int bls_qr_set_sparse(bls_qr *h, int64_t nnz, const int64_t *S_colptr, int64_t nnz_blocks,
+                                 const int64_t *block_colptr, const int64_t *block_rowval,
+                                 const int64_t *block_row_offsets, const int64_t *col_heights,
+                                 const int64_t *table_ptr, const int64_t *entry_retained) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (nnz < 0 || nnz_blocks < 0 || nnz_blocks > INT32_MAX || !S_colptr || !block_colptr || !col_heights ||
+        (nnz_blocks && (!block_rowval || !block_row_offsets)) || !table_ptr)
+        return fail(BLS_ERR_ARG, "bls_qr_set_sparse: bad arguments");
+    const int64_t nrb = h->nrb, nt = h->ntables;
+    std::vector<int32_t> bcp(nrb + 1), brv(nnz_blocks);
+    for (int64_t r = 0; r <= nrb; r++) bcp[r] = (int32_t)(block_colptr[r] - 1);
+    for (int64_t q = 0; q < nnz_blocks; q++) brv[q] = (int32_t)(block_rowval[q] - 1);
+    // every pair of entries u1 <= u2 of a table needs the block (r1, r2) in the pattern (the reference throws
+    // throw_missing_block otherwise)
+    for (int64_t t = 0; t < nt; t++)
+        for (int64_t u2 = table_ptr[t] - 1; u2 < table_ptr[t + 1] - 1; u2++) {
+            const int32_t r2 = (int32_t)(entry_retained[u2] - 1);
+            for (int64_t u1 = table_ptr[t] - 1; u1 <= u2; u1++) {
+                const int32_t r1 = (int32_t)(entry_retained[u1] - 1);
+                if (r2 < 0 || r2 >= nrb || !std::binary_search(brv.begin() + bcp[r2], brv.begin() + bcp[r2 + 1], r1))
+                    return fail(BLS_ERR_ARG, "The sparse pattern has no block (" + std::to_string(r1 + 1) + ", " +
+                                                 std::to_string(r2 + 1) + ")");
+            }
+        }
+    std::vector<int64_t> Scp(S_colptr, S_colptr + h->nr + 1), bro(block_row_offsets, block_row_offsets + nnz_blocks),
+        ch(col_heights, col_heights + nrb);
+    for (auto &x : Scp) x -= 1;
+    if ((rc = h->d_S_colptr.upload(Scp)) || (rc = h->d_block_colptr.upload(bcp)) || (rc = h->d_block_rowval.upload(brv)) ||
+        (rc = h->d_block_row_offsets.upload(bro)) || (rc = h->d_col_heights.upload(ch)))
+        return rc;
+    h->S_nnz = nnz;
+    h->has_sparse = true;
+    return BLS_OK;
+}
+
+namespace {
+template <class T> int reduced_normal(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
+                                      hipStream_t s) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!dense && !h->has_sparse) return fail(BLS_ERR_STATE, "no sparse layout (bls_qr_set_sparse)");
+    if (h->nrb && (!S || !g || !F || !Fr)) return fail(BLS_ERR_ARG, "reduced_normal: a required array is NULL");
+    NormalArgs na{};
+    na.dense = dense;
+    na.nr = h->nr;
+    na.S_colptr = h->d_S_colptr.ptr;
+    na.block_colptr = h->d_block_colptr.ptr;
+    na.block_rowval = h->d_block_rowval.ptr;
+    na.block_row_offsets = h->d_block_row_offsets.ptr;
+    na.col_heights = h->d_col_heights.ptr;
+    if (h->nrb) k_red_normal<T><<<(unsigned)h->nrb, RED_THREADS, 0, s>>>(h->red(), na, F, Fr, S, g);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_reduced_normal_sparse_f64(bls_qr *h, double *S_nzval, double *g_S, const double *factored,
+                                             const double *fres, void *stream) {
+    return reduced_normal<double>(h, 0, S_nzval, g_S, factored, fres, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_normal_sparse_f32(bls_qr *h, float *S_nzval, double *g_S, const float *factored,
+                                             const double *fres, void *stream) {
+    return reduced_normal<float>(h, 0, S_nzval, g_S, factored, fres, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_normal_dense_f64(bls_qr *h, double *S, double *g_S, const double *factored,
+                                            const double *fres, void *stream) {
+    return reduced_normal<double>(h, 1, S, g_S, factored, fres, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_normal_dense_f32(bls_qr *h, float *S, double *g_S, const float *factored,
+                                            const double *fres, void *stream) {
+    return reduced_normal<float>(h, 1, S, g_S, factored, fres, (hipStream_t)stream);
+}
 
 extern "C" int bls_qr_backsubstitute_f64(bls_qr *h, double *x_e, const double *x_r, const double *factored,
                                          const double *fres, void *stream) {

@@ -66,6 +66,11 @@ SIGNATURES = {
     "bls_reduced_residual_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_reduced_diagonal_blocks_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_pcg_relres_f64": (ct.c_int, [VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
+    "bls_qr_set_sparse": (ct.c_int, [VP, ct.c_int64, P64, ct.c_int64, P64, P64, P64, P64, P64, P64]),
+    "bls_reduced_normal_sparse_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_reduced_normal_sparse_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_reduced_normal_dense_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_reduced_normal_dense_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_qr_backsubstitute_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_qr_backsubstitute_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_diag_create": (ct.c_int, [ct.c_int64, P64, ct.POINTER(VP)]),
@@ -327,6 +332,16 @@ class QRHandle:
                                         _p64(jst.retained_entry, keep), _p64(st.r_dims, keep),
                                         _p64(st.boffsets_r, keep), _p64(jst.doffsets, keep)))
 
+    def set_sparse(self, pattern):
+        """Register the sparse reduced-system layout (build_schur_pattern's result) for reduced_normal_sparse!."""
+        keep = []
+        jst = self.jst
+        _check(lib().bls_qr_set_sparse(self.h, pattern.nnz, _p64(pattern.S_colptr, keep), len(pattern.block_rowval),
+                                       _p64(pattern.block_colptr, keep), _p64(pattern.block_rowval, keep),
+                                       _p64(pattern.block_row_offsets, keep), _p64(pattern.col_heights, keep),
+                                       _p64(jst.table_ptr, keep), _p64(jst.entry_retained, keep)))
+        self.pattern = pattern
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.bls_qr_destroy(self.h)
@@ -490,3 +505,19 @@ def solve_diagonal_blocks_(y, D, blocks, x, stream=None):
     """solve_diagonal_blocks!(y, D, dims, x): y = (L L')^-1 x per block (Float64 y, x)."""
     _check(getattr(lib(), "bls_solve_diagonal_blocks_" + _fsfx(D))(blocks.h, _dp(y), _dp(D), _dp(x), _stream(stream)))
     return y
+
+
+def reduced_normal_sparse_(S_nzval, g_S, handle, ws, stream=None):
+    """reduced_normal_sparse!(S, g_S, ws, pattern) (src/qr_reduced.jl:437-480), the pattern of handle.set_sparse."""
+    _check(getattr(lib(), "bls_reduced_normal_sparse_" + _fsfx(ws.factored))(handle.h, _dp(S_nzval), _dp(g_S),
+                                                                             _dp(ws.factored), _dp(ws.fres),
+                                                                             _stream(stream)))
+    return True
+
+
+def reduced_normal_dense_(S, g_S, handle, ws, stream=None):
+    """reduced_normal_dense!(S, g_S, ws) (src/qr_reduced.jl:409-435); S column-major flat (nr*nr)."""
+    _check(getattr(lib(), "bls_reduced_normal_dense_" + _fsfx(ws.factored))(handle.h, _dp(S), _dp(g_S),
+                                                                            _dp(ws.factored), _dp(ws.fres),
+                                                                            _stream(stream)))
+    return True

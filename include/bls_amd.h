@@ -212,6 +212,24 @@ int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int64_t *reduc
                        int64_t nentries, const int64_t *entry_retained, const int64_t *entry_col,
                        const int64_t *entry_table, const int64_t *retained_ptr, const int64_t *retained_entry,
                        const int64_t *r_dims, const int64_t *boffsets_r, const int64_t *doffsets);
+/* The sparse reduced-system layout for reduced_normal_sparse!: S.colptr (nr + 1) and the block pattern of
+ * build_schur_pattern (src/schur_structure.jl:241-323: block_colptr, block_rowval, block_row_offsets, col_heights),
+ * with the structure's table_ptr / entry_retained. Every pair of entries of a table must have its block in the
+ * pattern (BLS_ERR_ARG otherwise: the reference's throw_missing_block). Needs bls_qr_set_reduced. */
+int bls_qr_set_sparse(bls_qr *h, int64_t nnz, const int64_t *S_colptr, int64_t nnz_blocks, const int64_t *block_colptr,
+                      const int64_t *block_rowval, const int64_t *block_row_offsets, const int64_t *col_heights,
+                      const int64_t *table_ptr, const int64_t *entry_retained);
+/* reduced_normal_sparse!(S, g_S, ws, pattern) / reduced_normal_dense!(S, g_S, ws) (src/qr_reduced.jl:373-480):
+ * every scalar column of S is zeroed, then its upper triangle is J~'J~ and g_S = J~'r~ (Float64), each S entry summed
+ * over the tables in the reference's order (deterministic). Dense S is nr x nr column-major. */
+int bls_reduced_normal_sparse_f64(bls_qr *h, double *S_nzval, double *g_S, const double *factored, const double *fres,
+                                  void *stream);
+int bls_reduced_normal_sparse_f32(bls_qr *h, float *S_nzval, double *g_S, const float *factored, const double *fres,
+                                  void *stream);
+int bls_reduced_normal_dense_f64(bls_qr *h, double *S, double *g_S, const double *factored, const double *fres,
+                                 void *stream);
+int bls_reduced_normal_dense_f32(bls_qr *h, float *S, double *g_S, const float *factored, const double *fres,
+                                 void *stream);
 /* backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per
  * eliminated table, from the factored tables and residual of the last reduce_qr; accumulated in Float64. Needs
  * bls_qr_set_reduced. */

@@ -210,3 +210,55 @@ def test_qr_backsubstitute_matches_oracle(blsamd, table_type):
     xo = np.zeros(st.ne)
     oracle.qr_backsubstitute(sargs, oracle.JacobianArgs(jst), xo, x_r, F, Fr)
     assert np.abs(x_e.cpu().numpy() - xo).max() <= TOL * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("table_type", ["f64", "f32"])
+def test_reduced_normal_matches_oracle(blsamd, table_type):
+    """reduced_normal_dense! / reduced_normal_sparse! (src/qr_reduced.jl:373-480) on the device from the
+    device-factored tables (eliminated and free ones), against the oracle on the same tables. Float64: 1e-12
+    relative Frobenius per block. Float32 (S in Float32, sums in Float64 before the per-table rounding): within
+    64 eps(Float32) of the magnitude J~'J~ of |F| per block; g_S (Float64 accumulation) within 1e-12."""
+    import torch
+    from helpers import dense_upper_blocks, sparse_upper_blocks, max_block_rel, max_block_rel_to
+    from test_oracle_qr import _qr_pattern
+    dev = blsamd.device
+    for p in (_small_jacobian(blsamd, 14, nland=80, ncam=9, kmax=6), _jac_with_free_rows(blsamd)):
+        jst, st = p.jst, p.structure
+        dt = torch.float64 if table_type == "f64" else torch.float32
+        npdt = np.float64 if table_type == "f64" else np.float32
+        h = dev.QRHandle(jst)
+        pat = _qr_pattern(blsamd, p)
+        h.set_sparse(pat)
+        ws = dev.QRReductionWorkspace(h, dtype=dt)
+        T = lambda a, d=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(d).cuda()
+        assert dev.reduce_qr_(ws, h, T(p.data, dt), T(p.r), T(p.scaling_e), 0.2)
+        Sd = torch.full((st.nr * st.nr,), float("nan"), dtype=dt, device="cuda")
+        Ss = torch.full((pat.nnz,), float("nan"), dtype=dt, device="cuda")
+        gd = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+        gs = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+        dev.reduced_normal_dense_(Sd, gd, h, ws)
+        dev.reduced_normal_sparse_(Ss, gs, h, ws)
+        torch.cuda.synchronize()
+        F, Fr = ws.factored.cpu().numpy(), ws.fres.cpu().numpy()
+        sargs = oracle.SchurArgs(st, blsamd.structure.SchurReductionWorkspaceIndex(st, 1), pat)
+        jargs = oracle.JacobianArgs(jst)
+        So, go = np.zeros(st.nr * st.nr, npdt), np.zeros(st.nr)
+        oracle.reduced_normal(False, sargs, jargs, So, go, F, Fr)
+        Sdh = Sd.cpu().numpy()
+        # the dense variant zeroes whole scalar columns, lower triangle included
+        L = np.tril(Sdh.reshape(st.nr, st.nr).T, -1)
+        assert not np.isnan(Sdh).any() and np.all(L == 0)
+        A, B = dense_upper_blocks(st, Sdh.astype(np.float64)), dense_upper_blocks(st, So.astype(np.float64))
+        As = sparse_upper_blocks(st, pat, Ss.cpu().numpy().astype(np.float64))
+        if table_type == "f64":
+            assert max_block_rel(A, B) <= TOL
+            assert max_block_rel({k: As[k] for k in As}, {k: B[k] for k in As}) <= TOL
+        else:
+            Mo = np.zeros(st.nr * st.nr)
+            oracle.reduced_normal(False, sargs, jargs, Mo, np.zeros(st.nr), np.abs(F).astype(np.float64), np.abs(Fr))
+            Mag = dense_upper_blocks(st, Mo)
+            eps = 64 * float(np.finfo(np.float32).eps)
+            assert max_block_rel_to(A, B, Mag) <= eps
+            assert max_block_rel_to({k: As[k] for k in As}, {k: B[k] for k in As}, Mag) <= eps
+        for gg in (gd, gs):
+            assert np.abs(gg.cpu().numpy() - go).max() <= TOL * np.abs(go).max()

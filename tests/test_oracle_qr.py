@@ -224,3 +224,36 @@ def test_qr_conditioning_scale_equals_prescaled_columns(blsamd):
 def _jac_with_free_rows_cpu(blsamd):
     from test_gpu_qr import _jac_with_free_rows
     return _jac_with_free_rows(blsamd)
+
+
+def _qr_pattern(blsamd, p):
+    """build_schur_pattern with the free tables' block pairs as extra blocks (free_block_pairs,
+    src/qr_reduced.jl:7-18)."""
+    jst, st = p.jst, p.structure
+    extra = []
+    for t in range(st.neblocks, jst.ntables):
+        u = list(range(jst.table_ptr[t] - 1, jst.table_ptr[t + 1] - 1))
+        for b in range(len(u)):
+            for a in range(b):
+                extra.append((jst.entry_retained[u[a]], jst.entry_retained[u[b]]))
+    return blsamd.structure.build_schur_pattern(st, extra_blocks=extra)
+
+
+def test_reduced_normal_sparse_equals_dense(blsamd):
+    """reduced_normal_sparse! writes the same upper blocks as reduced_normal_dense! (same loops, sparse addressing),
+    bit for bit, including free tables."""
+    from helpers import dense_upper_blocks, sparse_upper_blocks
+    for p in (_small_jacobian(blsamd, 3), _jac_with_free_rows_cpu(blsamd)):
+        ok, fb, F, Fr, _, jargs = _qr(p, 0.1)
+        assert ok
+        st = p.structure
+        pat = _qr_pattern(blsamd, p)
+        sargs = oracle.SchurArgs(st, blsamd.structure.SchurReductionWorkspaceIndex(st, 1), pat)
+        Sd, gd = np.zeros(st.nr * st.nr), np.zeros(st.nr)
+        oracle.reduced_normal(False, sargs, jargs, Sd, gd, F, Fr)
+        Ss, gs = np.zeros(pat.nnz), np.zeros(st.nr)
+        oracle.reduced_normal(True, sargs, jargs, Ss, gs, F, Fr)
+        A, B = dense_upper_blocks(st, Sd), sparse_upper_blocks(st, pat, Ss)
+        for k, blk in B.items():
+            assert np.array_equal(A[k], blk), k
+        assert np.array_equal(gd, gs)
