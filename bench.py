@@ -97,6 +97,12 @@ def main():
     # host-side check_damping once, before the timed loop (it does not change between steps)
     dev._check_damping(lam, sr, "scaling_r")
     dev._check_damping(lam, se, "scaling_e")
+    # the autotuner picks the phase-2 work-list shape on this instance (landmarks per part, rows per run)
+    t0 = time.time()
+    cands = [(pt, ln) for pt in (192, 256, 384, 512, 768) for ln in (64, 96, 128, 192)]
+    tuned = h.autotune(cands, S, g, b_r, b_e, se, lam, E, C, ws)
+    log(f"autotune: part {tuned[0]}, lanes {tuned[1]} ({tuned[2]:.4f} ms reduce; default 384 / 96) "
+        f"over {len(cands)} shapes ({time.time() - t0:.1f}s)")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -169,6 +175,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded PCG64 generator)",
         "eager_ms_per_step": eager_ms, "step": "reset + reduce + copyto! captured in a HIP graph, replayed",
+        "tuning": {"part": tuned[0], "lanes": tuned[1], "candidates": len(cands)},
         "config": {"workload": "reduce_schur_sparse! + copyto!, sequential SLAM, 1000 cams (d6) x 200000 landmarks (d3), "
                                "FP64", "landmarks": int(st.neblocks), "cameras": int(st.nrblocks),
                    "couplings": int(st.ncoup), "S_nnz": int(nnz), "csc_nnz": int(len(smap))},

@@ -62,7 +62,7 @@ template <class P> constexpr bool all_f64() {
 
 constexpr int P2_THREADS = 256;
 constexpr int P2_MAXE = 64;        // entries (landmarks) per staged batch
-constexpr int P2_BASE_LANES = 96;  // (slot, row) lanes of one item before its heavy slots are split into chunks
+constexpr int P2_BASE_LANES = 96;  // default: (slot, row) lanes of one item before its heavy slots get lane groups
 constexpr int MAXDR = 16;          // largest retained block handled
 
 struct P2Item {
@@ -277,7 +277,7 @@ constexpr int P2_STAGE = 2048;                 // doubles per staging buffer (16
 constexpr int P2_Q = P2_STAGE / P2_THREADS;    // staged doubles per lane per batch (mixed dimensions)
 constexpr int P2_G = 4;                        // landmarks whose C_c1 columns a lane loads together
 constexpr int P2_WORDS = P2_STAGE / 64;
-constexpr int P2_PART = 384;                   // landmarks per workgroup before an item is split into parts
+constexpr int P2_PART = 384;                   // default: landmarks per workgroup before an item is split
 
 // Loaded doubles per lane per batch. One landmark dimension: four lanes per landmark, lane 0 loading E_inv and z
 // (DE^2 + DE), lanes 1-3 every third column of C_c2 (DE each, up to DRM columns); mixed dimensions: P2_Q.
@@ -927,6 +927,10 @@ struct bls_schur {
     DevArray<int64_t> d_failed_out;
     int64_t nitems = 0, nentries = 0;
     Timing timing;
+    // the shape of the phase-2 work list (bls_schur_set_tuning / bls_schur_autotune_f64) and the registered sparse
+    // layout it is rebuilt from (0-based block pattern, 1-based S_colptr, col_heights)
+    int64_t p2_part = P2_PART, p2_lanes = P2_BASE_LANES;
+    std::vector<int64_t> lay_bcp, lay_brv, lay_bro, lay_Scp, lay_ch;
 };
 
 namespace {
@@ -975,7 +979,7 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
         while (k0 < col.size()) {
             size_t k1 = k0;
             int rows = 0;
-            while (k1 < col.size() && k1 - k0 < 64 && rows + h->r_dims[col[k1]] <= P2_BASE_LANES)
+            while (k1 < col.size() && k1 - k0 < 64 && (k1 == k0 || rows + h->r_dims[col[k1]] <= h->p2_lanes))
                 rows += (int)h->r_dims[col[k1++]];
             P2Item it{};
             it.base = base;
@@ -1025,7 +1029,7 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
             }
             const int64_t e_end = (int64_t)entries.size();
             const int64_t ne = e_end - e_begin;
-            const int np = ne > P2_PART ? (int)((ne + P2_PART - 1) / P2_PART) : 1;
+            const int np = ne > h->p2_part ? (int)((ne + h->p2_part - 1) / h->p2_part) : 1;
             it.nparts = np;
             if (np > 1) {
                 P2Merge mg{};
@@ -1256,6 +1260,31 @@ std::vector<int32_t> narrow32(const std::vector<int64_t> &v) { return std::vecto
 
 }  // namespace
 
+namespace {
+// The phase-2 work list of the registered layout, with the handle's shape parameters.
+int build_sparse_phase2(bls_schur *h) {
+    const std::vector<int64_t> &bcp = h->lay_bcp, &brv = h->lay_brv, &bro = h->lay_bro, &Scp = h->lay_Scp,
+                               &ch = h->lay_ch;
+    return build_phase2(
+        h, [&](int64_t r2, std::vector<int64_t> &col) { col.assign(brv.begin() + bcp[r2], brv.begin() + bcp[r2 + 1]); },
+        [&](int64_t r2, size_t k, int64_t) { return bro[bcp[r2] + (int64_t)k]; },
+        [&](int64_t r2, int64_t &base, int64_t &cs) {
+            base = Scp[h->boffsets_r[r2]] - 1;
+            cs = ch[r2];
+        });
+}
+
+int build_dense_phase2(bls_schur *h) {
+    return build_phase2(
+        h, [&](int64_t r2, std::vector<int64_t> &col) { for (int64_t r = 0; r <= r2; r++) col.push_back(r); },
+        [&](int64_t, size_t, int64_t r1) { return h->boffsets_r[r1]; },
+        [&](int64_t r2, int64_t &base, int64_t &cs) {
+            base = h->boffsets_r[r2] * h->nr;
+            cs = h->nr;
+        });
+}
+}  // namespace
+
 extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, int64_t ncoup, const int64_t *r_dims,
                                 const int64_t *e_dims, const int64_t *boffsets_r, const int64_t *boffsets_e,
                                 const int64_t *eoffsets, const int64_t *eliminated_ptr, const int64_t *entry_retained,
@@ -1323,13 +1352,12 @@ extern "C" int bls_schur_set_sparse(bls_schur *h, int64_t nnz, const int64_t *S_
     for (int64_t r = 0; r < h->nrb; r++)
         for (int64_t k = bcp[r]; k < bcp[r + 1]; k++)
             if (brv[k] < 0 || brv[k] > r) return fail(BLS_ERR_ARG, "block pattern is not upper triangular");
-    int rc = build_phase2(
-        h, [&](int64_t r2, std::vector<int64_t> &col) { col.assign(brv.begin() + bcp[r2], brv.begin() + bcp[r2 + 1]); },
-        [&](int64_t r2, size_t k, int64_t) { return block_row_offsets[bcp[r2] + (int64_t)k]; },
-        [&](int64_t r2, int64_t &base, int64_t &cs) {
-            base = Scp[h->boffsets_r[r2]] - 1;
-            cs = col_heights[r2];
-        });
+    h->lay_bcp = bcp;
+    h->lay_brv = brv;
+    h->lay_bro.assign(block_row_offsets, block_row_offsets + nnz_blocks);
+    h->lay_Scp = Scp;
+    h->lay_ch.assign(col_heights, col_heights + h->nrb);
+    int rc = build_sparse_phase2(h);
     if (rc) return rc;
     std::vector<int64_t> rts(r_to_s_map, r_to_s_map + nR), di(diag_inds, diag_inds + h->nr);
     // scalar row of every stored entry (1-based), for the conditioning scale of reset_schur_sparse!: a scalar column
@@ -1357,13 +1385,7 @@ extern "C" int bls_schur_set_sparse(bls_schur *h, int64_t nnz, const int64_t *S_
 
 extern "C" int bls_schur_set_dense(bls_schur *h) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
-    int rc = build_phase2(
-        h, [&](int64_t r2, std::vector<int64_t> &col) { for (int64_t r = 0; r <= r2; r++) col.push_back(r); },
-        [&](int64_t, size_t, int64_t r1) { return h->boffsets_r[r1]; },
-        [&](int64_t r2, int64_t &base, int64_t &cs) {
-            base = h->boffsets_r[r2] * h->nr;
-            cs = h->nr;
-        });
+    int rc = build_dense_phase2(h);
     if (rc) return rc;
     h->mode = 2;
     return BLS_OK;
@@ -1601,4 +1623,65 @@ extern "C" int bls_schur_kernel_times(bls_schur *h, float *times, int n) {
     if (h->timing.enabled) (void)hipEventElapsedTime(&h->timing.ms[0], h->timing.ev[0], h->timing.ev[1]);
     for (int i = 0; i < n && i < 4; i++) times[i] = h->timing.ms[i];
     return BLS_OK;
+}
+
+// ---- the shape of the phase-2 work list ------------------------------------------------------------------------
+extern "C" int bls_schur_set_tuning(bls_schur *h, int64_t part, int64_t lanes) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (part < 16 || part > (1 << 20) || lanes < 16 || lanes > P2_THREADS)
+        return fail(BLS_ERR_ARG, "tuning: part must be 16..2^20 landmarks and lanes 16..256");
+    h->p2_part = part;
+    h->p2_lanes = lanes;
+    if (h->mode == 1) return build_sparse_phase2(h);
+    if (h->mode == 2) return build_dense_phase2(h);
+    return BLS_OK;
+}
+
+extern "C" int bls_schur_get_tuning(bls_schur *h, int64_t *part, int64_t *lanes) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (part) *part = h->p2_part;
+    if (lanes) *lanes = h->p2_lanes;
+    return BLS_OK;
+}
+
+// Time one reduction (after one untimed) per candidate shape on the caller's own data, keep the fastest. Every
+// shape computes the same sums in a fixed order of its own, so any of them is deterministic.
+extern "C" int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t *parts, const int64_t *lanes,
+                                      double *S, double *g_S, const double *b_r, const double *b_e,
+                                      const double *scaling_e, double lambda, const double *E_data,
+                                      const double *C_data, double *E_inv, double *z, const double *s_r,
+                                      const double *s_e, float *best_ms, void *stream) {
+    if (!h || ncand < 1 || !parts || !lanes) return fail(BLS_ERR_ARG, "bls_schur_autotune_f64: bad arguments");
+    if (h->mode != 1 && h->mode != 2) return fail(BLS_ERR_STATE, "handle has no reduced-system layout");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    BLS_HIP(hipEventCreate(&e0));
+    BLS_HIP(hipEventCreate(&e1));
+    float best = -1.0f;
+    int64_t bp = h->p2_part, bl = h->p2_lanes;
+    int rc = BLS_OK;
+    for (int64_t c = 0; c < ncand && rc == BLS_OK; c++) {
+        if ((rc = bls_schur_set_tuning(h, parts[c], lanes[c]))) break;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && rc == BLS_OK; rep++) {
+            if (rep == 1 && hipEventRecord(e0, st) != hipSuccess) rc = fail(BLS_ERR_HIP, "hipEventRecord");
+            if (rc == BLS_OK)
+                rc = reduce_async<PrecF64>(h, h->mode, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z,
+                                           s_r, s_e, h->d_failed_out.ptr, st);
+            if (rc == BLS_OK && rep == 1 &&
+                (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(BLS_ERR_HIP, "autotune timing");
+        }
+        if (rc == BLS_OK && (best < 0 || ms < best)) {
+            best = ms;
+            bp = parts[c];
+            bl = lanes[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (best_ms) *best_ms = best;
+    return bls_schur_set_tuning(h, bp, bl);
 }

@@ -49,6 +49,10 @@ SIGNATURES = {
     "bls_copyto_csc_f64": (ct.c_int, [ct.c_int64, VP, VP, VP, VP]),
     "bls_scatter_map_create": (ct.c_int, [ct.c_int64, P64, ct.POINTER(VP)]),
     "bls_schur_set_timing": (ct.c_int, [VP, ct.c_int]),
+    "bls_schur_set_tuning": (ct.c_int, [VP, ct.c_int64, ct.c_int64]),
+    "bls_schur_get_tuning": (ct.c_int, [VP, P64, P64]),
+    "bls_schur_autotune_f64": (ct.c_int, [VP, ct.c_int64, P64, P64] + [VP] * 5 + [ct.c_double] + [VP] * 6
+                               + [ct.POINTER(ct.c_float), VP]),
     "bls_pcg_create": (ct.c_int, [ct.c_int64, ct.c_int64, P64, P64, ct.c_int64, P64, ct.c_int64, P64, P64, P64, P64,
                                   ct.c_int64, P64, ct.POINTER(VP)]),
     "bls_pcg_destroy": (None, [VP]),
@@ -195,6 +199,28 @@ class SchurHandle:
 
     def set_timing(self, on=True):
         _check(lib().bls_schur_set_timing(self.h, 1 if on else 0))
+
+    def set_tuning(self, part, lanes):
+        """The phase-2 work-list shape (landmarks per part, rows per run of block rows); rebuilds the work list."""
+        _check(lib().bls_schur_set_tuning(self.h, int(part), int(lanes)))
+
+    def tuning(self):
+        a, b = ct.c_int64(0), ct.c_int64(0)
+        _check(lib().bls_schur_get_tuning(self.h, ct.byref(a), ct.byref(b)))
+        return int(a.value), int(b.value)
+
+    def autotune(self, candidates, S, g_S, b_r, b_e, scaling_e, lam, E_data, C_data, ws, s_r=None, s_e=None,
+                 stream=None):
+        """Time one Float64 reduction per (part, lanes) candidate and keep the fastest; returns (part, lanes, ms)."""
+        keep = []
+        parts = np.array([c[0] for c in candidates], dtype=np.int64)
+        lanes = np.array([c[1] for c in candidates], dtype=np.int64)
+        best = ct.c_float(0)
+        _check(lib().bls_schur_autotune_f64(self.h, len(parts), _p64(parts, keep), _p64(lanes, keep), _dp(S), _dp(g_S),
+                                            _dp(b_r), _dp(b_e), _dp(scaling_e), float(lam), _dp(E_data), _dp(C_data),
+                                            _dp(ws.E_inv_data), _dp(ws.z_data), _dp(s_r), _dp(s_e), ct.byref(best),
+                                            _stream(stream)))
+        return self.tuning() + (float(best.value),)
 
     def kernel_times(self):
         arr = (ct.c_float * 4)()

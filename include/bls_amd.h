@@ -167,6 +167,22 @@ int bls_backsubstitute_schur_mixed(bls_schur *h, double *x_e, const double *x_r,
 /* A device copy of a scatter map (1-based Int64, host) for repeated copyto!: returns a device pointer. */
 int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_map);
 
+/* The shape of the phase-2 work list (no reference counterpart; the reference's col_chunks are sized by its own
+ * work target, src/schur_reduction.jl:30-49): `part` landmarks per workgroup before a run of block rows of one column
+ * is split into parts summed in part order (default 384), `lanes` scalar rows of one run before the next block row
+ * starts another (default 96). Rebuilds the work list. Every shape sums in a fixed order of its own: deterministic;
+ * shapes differ from each other by rounding only. */
+int bls_schur_set_tuning(bls_schur *h, int64_t part, int64_t lanes);
+int bls_schur_get_tuning(bls_schur *h, int64_t *part, int64_t *lanes);
+/* The autotuner: times one Float64 reduction (the arguments of bls_reduce_schur_{sparse,dense}_f64, on the handle's
+ * layout) for each of the ncand (parts[i], lanes[i]) shapes, after one untimed run each, and keeps the fastest
+ * (*best_ms: its time). S is reduced repeatedly without a reset, so S, g_S, E_inv and z are scratch for the timing:
+ * reset and reduce again after tuning. Synchronizes `stream`. */
+int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t *parts, const int64_t *lanes, double *S,
+                           double *g_S, const double *b_r, const double *b_e, const double *scaling_e, double lambda,
+                           const double *E_data, const double *C_data, double *E_inv, double *z, const double *s_r,
+                           const double *s_e, float *best_ms, void *stream);
+
 /* Optional per-kernel timing of the last call on a handle (HIP events on the launch stream), in ms:
  * times[0] reset, [1] phase 1, [2] phase 2 (k_phase2), [3] the merge of split block columns. Enabled by
  * bls_schur_set_timing(h, 1). */

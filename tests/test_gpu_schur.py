@@ -369,3 +369,34 @@ def test_graph_captured_step_replays(blsamd, variant):
             assert torch.equal(a, b), rep
         step()   # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_autotune_shapes_agree(blsamd):
+    """The phase-2 work-list shape changes the grouping of the sums, not their value: every tuned shape reproduces the
+    oracle within the Float64 bound, each shape is deterministic, and the autotuner keeps one of its candidates."""
+    torch = _torch()
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.01, ncam=10)   # > 384 landmarks per camera: split items
+    st = p.structure
+    orc = oracle_run(p)
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=p.r_to_s)
+    ws = dev.SchurReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    E, C, b_r, b_e, sr, se, Rd = (T(x) for x in (p.E_data, p.C_data, p.b_r, p.b_e, p.scaling_r, p.scaling_e, p.R_data))
+    S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    cands = [(384, 96), (128, 48), (1024, 256), (64, 16)]
+    for part, lanes in cands:
+        h.set_tuning(part, lanes)
+        outs = []
+        for _ in range(2):
+            dev.reset_schur_sparse_(S, Rd, h, sr, p.lam)
+            assert dev.reduce_schur_sparse_(S, g, b_r, b_e, se, p.lam, h, E, C, ws)
+            torch.cuda.synchronize()
+            outs.append((S.cpu().numpy(), g.cpu().numpy()))
+        assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+        compare(p, dict(S=outs[0][0], g=outs[0][1], E_inv=ws.E_inv_data.cpu().numpy(), z=ws.z_data.cpu().numpy(),
+                        ok=True), orc)
+    dev.reset_schur_sparse_(S, Rd, h, sr, p.lam)
+    part, lanes, ms = h.autotune(cands, S, g, b_r, b_e, se, p.lam, E, C, ws)
+    assert (part, lanes) in cands and ms > 0
