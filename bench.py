@@ -99,7 +99,7 @@ def main():
     dev._check_damping(lam, se, "scaling_e")
     # the autotuner picks the phase-2 work-list shape on this instance (landmarks per part, rows per run)
     t0 = time.time()
-    cands = [(pt, ln) for pt in (192, 256, 384, 512, 768) for ln in (64, 96, 128, 192)]
+    cands = [(pt, ln) for pt in (384, 768, 1152, 1536, 2304) for ln in (32, 48, 64, 96)]
     tuned = h.autotune(cands, S, g, b_r, b_e, se, lam, E, C, ws)
     log(f"autotune: part {tuned[0]}, lanes {tuned[1]} ({tuned[2]:.4f} ms reduce; default 384 / 96) "
         f"over {len(cands)} shapes ({time.time() - t0:.1f}s)")
@@ -140,12 +140,13 @@ def main():
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    ms = ev0.elapsed_time(ev1) / args.steps
+    graph_ms = ev0.elapsed_time(ev1) / args.steps
     assert int(ws.d_failed.item()) == 0
-    value = st.neblocks / (ms * 1e-3)
-    log(f"timed (graph replay): {args.steps} steps, {ms:.4f} ms/step (wall {wall * 1e3 / args.steps:.4f} ms/step), "
-        f"{value:.4g} landmarks/s")
+    log(f"graph replay: {args.steps} steps, {graph_ms:.4f} ms/step (wall {wall * 1e3 / args.steps:.4f} ms/step), "
+        f"{st.neblocks / (graph_ms * 1e-3):.4g} landmarks/s")
     del graph
+    ms = eager_ms   # the metric: the library's calls launched as a caller makes them
+    value = st.neblocks / (ms * 1e-3)
 
     # per-kernel durations (HIP events recorded by the library on the launch stream)
     h.set_timing(True)
@@ -174,7 +175,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "landmarks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded PCG64 generator)",
-        "eager_ms_per_step": eager_ms, "step": "reset + reduce + copyto! captured in a HIP graph, replayed",
+        "graph_ms_per_step": graph_ms, "step": "reset + reduce + copyto!, device-resident inputs",
         "tuning": {"part": tuned[0], "lanes": tuned[1], "candidates": len(cands)},
         "config": {"workload": "reduce_schur_sparse! + copyto!, sequential SLAM, 1000 cams (d6) x 200000 landmarks (d3), "
                                "FP64", "landmarks": int(st.neblocks), "cameras": int(st.nrblocks),
@@ -356,6 +357,12 @@ def batched_pcg_leg(torch, dev, b, scale, reps=3):
     x = torch.zeros(st.nr, dtype=torch.float64, device="cuda")
     log(f"batched leg setup: {p.nprob} problems, {st.nrblocks} cameras, {st.neblocks} landmarks, S nnz {p.pattern.nnz} "
         f"({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    dev.reset_schur_sparse_(S, Rd, h, sr, p.lam)
+    cands = [(384, 96), (1536, 96), (2304, 64), (2304, 96)]   # each shape rebuilds the work list of 9.85 M landmarks
+    tuned = h.autotune(cands, S, g, b_r, b_e, se, p.lam, E, C, ws)
+    log(f"batched leg autotune: part {tuned[0]}, lanes {tuned[1]} ({tuned[2]:.3f} ms reduce) over {len(cands)} shapes "
+        f"({time.time() - t0:.1f}s)")
 
     def reduce():
         dev.reset_schur_sparse_(S, Rd, h, sr, p.lam, check=False)
@@ -407,6 +414,7 @@ def batched_pcg_leg(torch, dev, b, scale, reps=3):
             "landmarks_per_s": st.neblocks / (red * 1e-3), "pcg_ms": sol, "time_to_1e-8_ms": red + sol,
             "pcg_iters_max": int(iters.max()), "pcg_iters_mean": float(iters.mean()),
             "relres_max": float(rel.max()), "preconditioner": "block-Jacobi (FP64)", "learned": learned,
+            "tuning": {"part": tuned[0], "lanes": tuned[1]},
             "mixed": mixed}
 
 

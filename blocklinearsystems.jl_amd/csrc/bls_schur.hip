@@ -1644,8 +1644,8 @@ extern "C" int bls_schur_get_tuning(bls_schur *h, int64_t *part, int64_t *lanes)
     return BLS_OK;
 }
 
-// Time one reduction (after one untimed) per candidate shape on the caller's own data, keep the fastest. Every
-// shape computes the same sums in a fixed order of its own, so any of them is deterministic.
+// Time three reductions (after one untimed) per candidate shape on the caller's own data and keep the shape with the
+// fastest median. Every shape computes the same sums in a fixed order of its own, so any of them is deterministic.
 extern "C" int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t *parts, const int64_t *lanes,
                                       double *S, double *g_S, const double *b_r, const double *b_e,
                                       const double *scaling_e, double lambda, const double *E_data,
@@ -1662,17 +1662,19 @@ extern "C" int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t
     int rc = BLS_OK;
     for (int64_t c = 0; c < ncand && rc == BLS_OK; c++) {
         if ((rc = bls_schur_set_tuning(h, parts[c], lanes[c]))) break;
-        float ms = 0.0f;
-        for (int rep = 0; rep < 2 && rc == BLS_OK; rep++) {
-            if (rep == 1 && hipEventRecord(e0, st) != hipSuccess) rc = fail(BLS_ERR_HIP, "hipEventRecord");
+        float t[3] = {0.0f, 0.0f, 0.0f};
+        for (int rep = 0; rep < 4 && rc == BLS_OK; rep++) {
+            if (rep > 0 && hipEventRecord(e0, st) != hipSuccess) rc = fail(BLS_ERR_HIP, "hipEventRecord");
             if (rc == BLS_OK)
                 rc = reduce_async<PrecF64>(h, h->mode, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z,
                                            s_r, s_e, h->d_failed_out.ptr, st);
-            if (rc == BLS_OK && rep == 1 &&
+            if (rc == BLS_OK && rep > 0 &&
                 (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                 hipEventElapsedTime(&t[rep - 1], e0, e1) != hipSuccess))
                 rc = fail(BLS_ERR_HIP, "autotune timing");
         }
+        std::sort(t, t + 3);
+        const float ms = t[1];
         if (rc == BLS_OK && (best < 0 || ms < best)) {
             best = ms;
             bp = parts[c];

@@ -174,9 +174,9 @@ int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_ma
  * shapes differ from each other by rounding only. */
 int bls_schur_set_tuning(bls_schur *h, int64_t part, int64_t lanes);
 int bls_schur_get_tuning(bls_schur *h, int64_t *part, int64_t *lanes);
-/* The autotuner: times one Float64 reduction (the arguments of bls_reduce_schur_{sparse,dense}_f64, on the handle's
- * layout) for each of the ncand (parts[i], lanes[i]) shapes, after one untimed run each, and keeps the fastest
- * (*best_ms: its time). S is reduced repeatedly without a reset, so S, g_S, E_inv and z are scratch for the timing:
+/* The autotuner: times three Float64 reductions (the arguments of bls_reduce_schur_{sparse,dense}_f64, on the
+ * handle's layout) for each of the ncand (parts[i], lanes[i]) shapes, after one untimed run each, and keeps the shape
+ * with the fastest median (*best_ms: that median). S is reduced repeatedly without a reset, so S, g_S, E_inv and z are scratch for the timing:
  * reset and reduce again after tuning. Synchronizes `stream`. */
 int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t *parts, const int64_t *lanes, double *S,
                            double *g_S, const double *b_r, const double *b_e, const double *scaling_e, double lambda,
