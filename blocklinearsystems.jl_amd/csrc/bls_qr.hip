@@ -534,6 +534,142 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_normal_merge(RedArgs a, Nor
     }
 }
 
+// reduced_normal by per-table Gram products. k_red_gram: one workgroup per table forms the blocks
+// X(:, cols(u1))' X(:, cols(u2)) of every pair of its entries u1 <= u2 (X: the table's reduced rows, staged in LDS when
+// they fit) into a contribution buffer, each table read once. k_red_gram_sum: one wavefront per block (r1, r2) of S
+// adds its contributions in ascending table order -- the order in which the reference adds them to each entry -- and
+// writes the whole block (the lower triangle of a diagonal block, and a block without contributions, as zeros).
+constexpr int RG_MAXE = 31;    // entries of one table
+constexpr int RG_MAXW = 512;   // columns spanned by the entries of one table
+
+typedef double rg_v4d __attribute__((ext_vector_type(4)));
+
+// Launch classes by the size of a table's staged block (rows rounded up to 4, columns to 16, one pad column): up to
+// 4096 doubles, up to 8192, up to the device's LDS per workgroup, and the rest, which read their operands from global
+// memory.
+constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 8192, RG_CLASS_CAP2 = 19456;
+
+// X = the reduced rows of table t over the columns of its entries (n x W, column c of the table at c0 + c). Each
+// wavefront forms 16 x 16 tiles (I <= J) of X'X with the FP64 matrix instruction (16x16x4: lane l holds A(l % 16,
+// l / 16), B(l / 16, l % 16), and D(l / 16 + 4 v, l % 16) in accumulator v), from X staged row-major in LDS when it
+// fits, and writes each element (a, b), a <= b, to its pair block (e1, e2): the pairs of the table's entries i1 <= i2
+// in the order i2 (i2 + 1) / 2 + i1, each d1 x d2 row-major, at offset Q[e2] + d2 P[e1] with P the prefix sums of
+// the entry dims and Q[e] = sum_{j < e} d_j P[j + 1]. Lower entries of a diagonal block (row > col) are not written.
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__restrict__ tabs, const int64_t *__restrict__ goff,
+                                                 int stage_cap, const T *__restrict__ F, double *__restrict__ G) {
+    extern __shared__ double XL[];
+    __shared__ int16_t s_ent[RG_MAXW];
+    __shared__ int s_cst[RG_MAXE], s_dim[RG_MAXE], s_P[RG_MAXE + 1], s_Q[RG_MAXE];
+    const int t = tabs[blockIdx.x];
+    const int M = a.nrows[t], top = a.top[t], n = M - top;
+    const int u0 = a.table_ptr[t], ne = a.table_ptr[t + 1] - u0;   // 1..RG_MAXE (host lists only such tables)
+    const int c0 = a.entry_col[u0];
+    const int W = a.entry_col[u0 + ne - 1] + a.r_dims[a.entry_retained[u0 + ne - 1]] - c0;   // <= RG_MAXW
+    const int Wp = (W + 15) & ~15, np4 = (n + 3) & ~3, SW = Wp + 1;
+    const T *Xg = F + a.raw_off[t] + top;   // X(k, c) = Xg[(c0 + c) M + k]
+    const bool fits = (int64_t)np4 * SW <= stage_cap;
+    for (int i = threadIdx.x; i < Wp; i += NT) s_ent[i] = -1;
+    for (int e = threadIdx.x; e < ne; e += NT) {
+        s_cst[e] = a.entry_col[u0 + e] - c0;
+        s_dim[e] = a.r_dims[a.entry_retained[u0 + e]];
+    }
+    if (fits)
+        for (int q = threadIdx.x; q < np4 * Wp; q += NT) {
+            const int c = q / np4, k = q - c * np4;
+            XL[k * SW + c] = (k < n && c < W) ? (double)Xg[(int64_t)(c0 + c) * M + k] : 0.0;
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += NT)
+        for (int j = 0; j < s_dim[e]; j++) s_ent[s_cst[e] + j] = (int16_t)e;
+    if (threadIdx.x == 0) {
+        int p = 0, q = 0;
+        for (int e = 0; e < ne; e++) {
+            s_P[e] = p;
+            s_Q[e] = q;
+            p += s_dim[e];
+            q += s_dim[e] * p;
+        }
+        s_P[ne] = p;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int ntl = Wp >> 4, ntt = ntl * (ntl + 1) / 2;
+    double *Gt = G + goff[t];
+    for (int tau = wave; tau < ntt; tau += NT / 64) {
+        int J = 0;
+        while ((J + 1) * (J + 2) / 2 <= tau) J++;
+        const int I = tau - J * (J + 1) / 2;
+        rg_v4d acc = {0.0, 0.0, 0.0, 0.0};
+        if (fits) {
+            const double *pa = XL + lk * SW + 16 * I + li, *pb = XL + lk * SW + 16 * J + li;
+            for (int k0 = 0; k0 < np4; k0 += 4)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[k0 * SW], pb[k0 * SW], acc, 0, 0, 0);
+        } else {
+            const bool va = 16 * I + li < W, vb = 16 * J + li < W;
+            const T *xa = Xg + (int64_t)(c0 + 16 * I + li) * M, *xb = Xg + (int64_t)(c0 + 16 * J + li) * M;
+            for (int k0 = 0; k0 < np4; k0 += 4) {
+                const int k = k0 + lk;
+                const double av = (va && k < n) ? (double)xa[k] : 0.0;
+                const double bv = (vb && k < n) ? (double)xb[k] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+            }
+        }
+        const int b = 16 * J + li;
+        const int e2 = b < W ? s_ent[b] : -1;
+        if (e2 >= 0) {
+            const int col = b - s_cst[e2], d2 = s_dim[e2];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int aa = 16 * I + lk + 4 * v;
+                const int e1 = aa <= b ? s_ent[aa] : -1;
+                if (e1 >= 0) Gt[s_Q[e2] + d2 * s_P[e1] + (aa - s_cst[e1]) * d2 + col] = acc[v];
+            }
+        }
+    }
+}
+
+struct RGBlock {
+    int64_t dst;       // S index of entry (0, 0) of the block
+    int64_t cstride;   // distance between its columns in S
+    int32_t d1, d2, diag, pad;
+};
+
+// One wavefront per block of S: entry w = row d2 + col of the block is the sum of its contributions (row-major pair
+// blocks of the Gram buffer) in ascending table order; zero below the diagonal of a diagonal block.
+template <class T>
+__global__ __launch_bounds__(256) void k_red_gram_sum(int64_t nblk, const RGBlock *__restrict__ blocks,
+                                                      const int64_t *__restrict__ ptr, const int64_t *__restrict__ contrib,
+                                                      const double *__restrict__ G, T *__restrict__ S) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= nblk) return;
+    const RGBlock bk = blocks[b];
+    const int sz = bk.d1 * bk.d2;
+    const int64_t cb = ptr[b], ce = ptr[b + 1];
+    for (int w = lane; w < sz; w += 64) {
+        const int row = w / bk.d2, col = w - row * bk.d2;
+        double acc = 0.0;
+        if (!(bk.diag && row > col)) {
+            int64_t c = cb;
+            for (; c + 8 <= ce; c += 8) {
+                double v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[j] = G[contrib[c + j] + w];
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc += v[j];
+            }
+            for (; c < ce; c++) acc += G[contrib[c] + w];
+        }
+        S[bk.dst + (int64_t)col * bk.cstride + row] = (T)acc;
+    }
+}
+
+template <class T> __global__ void k_rg_fill(T *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (T)0;
+}
+
 // backsubstitute! of the QR path (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per eliminated table, one
 // lane per table: q the first d_e rows of the factored residual, B the first d_e rows of the retained columns, U the
 // triangle of the eliminated columns; accumulated in Float64 whatever the table type.
@@ -596,6 +732,19 @@ struct bls_qr {
     DevArray<RNMerge> d_rn_merges[2];
     int64_t rn_nitems[2] = {0, 0}, rn_nmerges[2] = {0, 0};
     DevArray<double> d_rn_partial;
+    // the Gram formulation: per-table offsets into the contribution buffer, and per layout the S blocks with their
+    // contribution lists (ascending table order)
+    bool rg_built[2] = {false, false};
+    DevArray<int64_t> d_rg_goff, d_rg_ptr[2], d_rg_contrib[2];
+    DevArray<RGBlock> d_rg_blocks[2];
+    int64_t rg_nblk[2] = {0, 0}, rg_glen = 0;
+    DevArray<double> d_rg_G, d_rg_rt;
+    DevArray<int32_t> d_rg_tabs;                // tables of the four launch classes, one after another
+    int64_t rg_cls_off[5] = {0, 0, 0, 0, 0};
+    int rg_cap2 = 0;                            // staged doubles of the third class (from the device's LDS)
+    std::vector<int32_t> h_tp, h_er, h_ec;   // table_ptr, entry_retained, entry_col (0-based)
+    std::vector<int32_t> h_nrows, h_top;
+    std::vector<int64_t> h_bcp, h_brv, h_bro, h_ch;   // the sparse pattern (0-based)
     std::vector<int64_t> h_retained_ptr, h_S_colptr;   // host copies for the plans (0-based)
     std::vector<int32_t> h_r_dims;
     std::vector<int64_t> h_boff_r;
@@ -740,6 +889,7 @@ extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, con
     }
     for (int b = 0; b < 3; b++)
         if ((rc = h->d_bucket[b].upload(h->bucket_tables[b]))) { delete h; return rc; }
+    h->h_nrows = nrows;
     *out = h;
     return BLS_OK;
 }
@@ -787,6 +937,10 @@ extern "C" int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int
     h->red_len = ro[nt];
     h->dlen = dl;
     h->h_retained_ptr.assign(rp.begin(), rp.end());
+    h->h_tp = tp;
+    h->h_top = top;
+    h->h_er = er;
+    h->h_ec = ec;
     h->h_r_dims = rd;
     h->h_boff_r = bo;
     h->has_reduced = true;
@@ -891,14 +1045,173 @@ extern "C" int bls_qr_set_sparse(bls_qr *h, int64_t nnz, const int64_t *S_colptr
         return rc;
     h->S_nnz = nnz;
     h->h_S_colptr = Scp;
+    h->h_bcp.assign(bcp.begin(), bcp.end());
+    h->h_brv.assign(brv.begin(), brv.end());
+    h->h_bro = bro;
+    h->h_ch = ch;
     h->rn_built[0] = false;
+    h->rg_built[0] = false;
     h->has_sparse = true;
     return BLS_OK;
 }
 
 namespace {
+// The plan of the Gram formulation for one layout: per-table offsets of the pair blocks, the blocks of S with their
+// write addresses, and each block's contributions in ascending table order.
+int build_gram_plan(bls_qr *h, int dense) {
+    const int m = dense ? 1 : 0;
+    const int64_t nt = h->ntables, nrb = h->nrb;
+    const std::vector<int32_t> &tp = h->h_tp, &er = h->h_er, &ec = h->h_ec, &rd = h->h_r_dims;
+    const std::vector<int64_t> &bo = h->h_boff_r;
+    std::vector<int64_t> goff(nt + 1);
+    int64_t acc = 0;
+    for (int64_t t = 0; t < nt; t++) {
+        goff[t] = acc;
+        const int ne = tp[t + 1] - tp[t];
+        if (ne > RG_MAXE)
+            return fail(BLS_ERR_UNSUPPORTED, "table " + std::to_string(t + 1) + " holds " + std::to_string(ne) +
+                                                 " retained blocks; reduced_normal on the device takes up to 31");
+        if (ne && ec[tp[t] + ne - 1] + rd[er[tp[t] + ne - 1]] - ec[tp[t]] > RG_MAXW)
+            return fail(BLS_ERR_UNSUPPORTED, "table " + std::to_string(t + 1) +
+                                                 ": its entries span more than 512 columns; reduced_normal on the device "
+                                                 "takes up to 512");
+        for (int i = 1; i < ne; i++)   // pair blocks land in the upper triangle; the last entry ends the columns
+            if (er[tp[t] + i] <= er[tp[t] + i - 1] || ec[tp[t] + i] <= ec[tp[t] + i - 1])
+                return fail(BLS_ERR_ARG, "table " + std::to_string(t + 1) +
+                                             ": entries not in strictly ascending retained-block and column order");
+        for (int i2 = 0; i2 < ne; i2++)
+            for (int i1 = 0; i1 <= i2; i1++) acc += (int64_t)rd[er[tp[t] + i1]] * rd[er[tp[t] + i2]];
+    }
+    goff[nt] = acc;
+    // launch classes by staged size; the third holds what fits the device's LDS per workgroup
+    int dev = 0, lds = 0;
+    BLS_HIP(hipGetDevice(&dev));
+    BLS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    h->rg_cap2 = (int)std::min<int64_t>(RG_CLASS_CAP2, std::max<int64_t>(0, (lds - 2048) / 8));
+    const int64_t caps[3] = {RG_CLASS_CAP0, RG_CLASS_CAP1, h->rg_cap2};
+    std::vector<int32_t> cls[4];
+    for (int64_t t = 0; t < nt; t++) {
+        const int ne = tp[t + 1] - tp[t];
+        if (!ne) continue;
+        const int64_t W = ec[tp[t] + ne - 1] + rd[er[tp[t] + ne - 1]] - ec[tp[t]];
+        const int64_t n = h->h_nrows[t] - h->h_top[t];
+        const int64_t st = ((n + 3) & ~3LL) * (((W + 15) & ~15LL) + 1);
+        int c = 3;
+        for (int k = 2; k >= 0; k--)
+            if (st <= caps[k]) c = k;
+        cls[c].push_back((int32_t)t);
+    }
+    std::vector<int32_t> tabs;
+    for (int c = 0; c < 4; c++) {
+        h->rg_cls_off[c] = (int64_t)tabs.size();
+        tabs.insert(tabs.end(), cls[c].begin(), cls[c].end());
+    }
+    h->rg_cls_off[4] = (int64_t)tabs.size();
+    int rc0;
+    if ((rc0 = h->d_rg_tabs.upload(tabs))) return rc0;
+    std::vector<RGBlock> blocks;
+    if (dense) {
+        blocks.reserve((size_t)(nrb * (nrb + 1) / 2));
+        for (int64_t r2 = 0; r2 < nrb; r2++)
+            for (int64_t r1 = 0; r1 <= r2; r1++)
+                blocks.push_back(RGBlock{bo[r1] + bo[r2] * h->nr, h->nr, rd[r1], rd[r2], r1 == r2, 0});
+    } else {
+        for (int64_t r2 = 0; r2 < nrb; r2++)
+            for (int64_t q = h->h_bcp[r2]; q < h->h_bcp[r2 + 1]; q++) {
+                const int64_t r1 = h->h_brv[q];
+                blocks.push_back(RGBlock{h->h_S_colptr[bo[r2]] + h->h_bro[q], h->h_ch[r2], rd[r1], rd[r2], r1 == r2, 0});
+            }
+    }
+    const int64_t nblk = (int64_t)blocks.size();
+    auto block_of = [&](int64_t r1, int64_t r2) -> int64_t {
+        if (dense) return r2 * (r2 + 1) / 2 + r1;
+        const int64_t *b0 = h->h_brv.data() + h->h_bcp[r2], *b1 = h->h_brv.data() + h->h_bcp[r2 + 1];
+        return (int64_t)(std::lower_bound(b0, b1, r1) - h->h_brv.data());   // present: checked by bls_qr_set_sparse
+    };
+    std::vector<int64_t> ptr(nblk + 1, 0);
+    for (int pass = 0; pass < 2; pass++) {
+        std::vector<int64_t> pos;
+        std::vector<int64_t> contrib;
+        if (pass == 1) {
+            for (int64_t b = 0; b < nblk; b++) ptr[b + 1] += ptr[b];
+            pos.assign(ptr.begin(), ptr.end() - 1);
+            contrib.resize((size_t)ptr[nblk]);
+        }
+        for (int64_t t = 0; t < nt; t++) {
+            const int ne = tp[t + 1] - tp[t];
+            int64_t off = goff[t];
+            for (int i2 = 0; i2 < ne; i2++)
+                for (int i1 = 0; i1 <= i2; i1++) {
+                    const int64_t r1 = er[tp[t] + i1], r2 = er[tp[t] + i2];
+                    const int64_t b = block_of(r1, r2);
+                    if (pass == 0) ptr[b + 1]++;
+                    else contrib[(size_t)pos[b]++] = off;
+                    off += (int64_t)rd[r1] * rd[r2];
+                }
+        }
+        if (pass == 1) {
+            int rc;
+            if ((rc = h->d_rg_ptr[m].upload(ptr)) || (rc = h->d_rg_contrib[m].upload(contrib))) return rc;
+        }
+    }
+    int rc;
+    if ((rc = h->d_rg_goff.upload(goff)) || (rc = h->d_rg_blocks[m].upload(blocks))) return rc;
+    if (acc > h->rg_glen) {
+        if ((rc = h->d_rg_G.alloc((size_t)acc))) return rc;
+        h->rg_glen = acc;
+    }
+    if ((int64_t)h->d_rg_rt.n < h->red_len && (rc = h->d_rg_rt.alloc((size_t)h->red_len))) return rc;
+    h->rg_nblk[m] = nblk;
+    h->rg_built[m] = true;
+    return BLS_OK;
+}
+
+template <class T> int reduced_normal_gram(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
+                                           hipStream_t s) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!dense && !h->has_sparse) return fail(BLS_ERR_STATE, "no sparse layout (bls_qr_set_sparse)");
+    if (h->nrb && (!S || !g || !F || !Fr)) return fail(BLS_ERR_ARG, "reduced_normal: a required array is NULL");
+    const int m = dense ? 1 : 0;
+    if (!h->rg_built[m] && (rc = build_gram_plan(h, dense))) return rc;
+    if (dense && h->nr) k_rg_fill<T><<<std::min<int64_t>(grid_for(h->nr * h->nr, 256), 4096), 256, 0, s>>>(S, h->nr * h->nr);
+    {
+        const int caps[4] = {RG_CLASS_CAP0, RG_CLASS_CAP1, h->rg_cap2, 0};
+        for (int c = 0; c < 4; c++) {
+            const int64_t cnt = h->rg_cls_off[c + 1] - h->rg_cls_off[c];
+            if (!cnt) continue;
+            const int32_t *tabs = h->d_rg_tabs.ptr + h->rg_cls_off[c];
+            const size_t lds = (size_t)caps[c] * sizeof(double);
+            if (c == 2) {
+                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                k_red_gram<T, 512><<<(unsigned)cnt, 512, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, caps[c], F,
+                                                                 h->d_rg_G.ptr);
+            } else {
+                k_red_gram<T, 256><<<(unsigned)cnt, 256, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, caps[c], F,
+                                                                 h->d_rg_G.ptr);
+            }
+        }
+    }
+    if (h->rg_nblk[m])
+        k_red_gram_sum<T><<<(unsigned)((h->rg_nblk[m] + 3) / 4), 256, 0, s>>>(h->rg_nblk[m], h->d_rg_blocks[m].ptr,
+                                                                            h->d_rg_ptr[m].ptr, h->d_rg_contrib[m].ptr,
+                                                                            h->d_rg_G.ptr, S);
+    // g_S = J~' r~: the reduced residual rows, then the transposed product (fixed-order sums)
+    if (h->ntables)
+        k_red_residual<<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), Fr, h->d_rg_rt.ptr);
+    if (h->nrb) k_red_tmul<T><<<(unsigned)h->nrb, RED_THREADS, 0, s>>>(h->red(), F, h->d_rg_rt.ptr, g);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
 template <class T> int reduced_normal(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
                                       hipStream_t s) {
+    return reduced_normal_gram<T>(h, dense, S, g, F, Fr, s);
+}
+
+template <class T> int reduced_normal_by_column(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
+                                                hipStream_t s) {
     int rc = reduced_ready(h);
     if (rc) return rc;
     if (!dense && !h->has_sparse) return fail(BLS_ERR_STATE, "no sparse layout (bls_qr_set_sparse)");
