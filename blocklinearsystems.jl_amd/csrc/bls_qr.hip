@@ -300,31 +300,77 @@ __global__ __launch_bounds__(256) void k_red_residual(RedArgs a, const double *F
 // added in order, so the result is deterministic.
 constexpr int RED_THREADS = 256;
 
-// g = J~' w (reduced_tmul!, src/qr_reduced.jl:195-223)
+// g = J~' w (reduced_tmul!, src/qr_reduced.jl:195-223) in two passes. k_red_tmul_part: one wavefront per table,
+// lanes over the columns of its entries, writes the per-entry partials p[eoff[u] + j] = X_t(:, entry_col[u] + j)' w_t
+// (eoff: prefix sums of the entry dims in entry order). k_red_gsum: one wavefront per retained block r adds the
+// partials of the entries coupling r in retained_entry order: lane group q of the 64 / d_r groups takes the q-th
+// contiguous run of that list, and the runs are then added in order, so the result is deterministic.
 template <class T>
-__global__ __launch_bounds__(RED_THREADS) void k_red_tmul(RedArgs a, const T *F, const double *w, double *g) {
-    __shared__ double part[RED_THREADS];
-    const int r = blockIdx.x;
-    const int dr = a.r_dims[r];
-    const int nch = RED_THREADS / dr;
-    const int j = threadIdx.x % dr, ch = threadIdx.x / dr;
+__global__ __launch_bounds__(256) void k_red_tmul_part(RedArgs a, const int64_t *__restrict__ eoff,
+                                                       const T *__restrict__ F, const double *__restrict__ w,
+                                                       double *__restrict__ gp) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= a.ntables) return;
+    const int u0 = a.table_ptr[t], ne = a.table_ptr[t + 1] - u0;
+    const int M = a.nrows[t], top = a.top[t], n = M - top;
+    const T *Xt = F + a.raw_off[t] + top;
+    const double *wt = w + a.red_off[t];
+    const int64_t base = eoff[u0];
+    const int ncol = (int)(eoff[u0 + ne] - base);
+    for (int q = lane; q < ncol; q += 64) {
+        int e = 0;
+        while (e + 1 < ne && eoff[u0 + e + 1] - base <= q) e++;
+        const T *x = Xt + (int64_t)(a.entry_col[u0 + e] + (q - (int)(eoff[u0 + e] - base))) * M;
+        double acc = 0.0;
+        int i = 0;
+        for (; i + 8 <= n; i += 8) {
+            double xv[8], wv[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                xv[j] = (double)x[i + j];
+                wv[j] = wt[i + j];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) acc += xv[j] * wv[j];
+        }
+        for (; i < n; i++) acc += (double)x[i] * wt[i];
+        gp[base + q] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_red_gsum(RedArgs a, const int64_t *__restrict__ eoff,
+                                                  const double *__restrict__ gp, double *__restrict__ g) {
+    __shared__ double part[4][64];
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double acc = 0.0;
-    if (ch < nch) {
-        for (int k = a.retained_ptr[r] + ch; k < a.retained_ptr[r + 1]; k += nch) {
-            const int u = a.retained_entry[k];
-            const int t = a.entry_table[u];
-            const int M = a.nrows[t], top = a.top[t], n = M - top;
-            const T *col = F + a.raw_off[t] + (int64_t)(a.entry_col[u] + j) * M + top;
-            const double *wt = w + a.red_off[t];
-            for (int i = 0; i < n; i++) acc += (double)col[i] * wt[i];
+    int d = 1, ng = 0;
+    if (r < a.nrb) {
+        d = a.r_dims[r];
+        ng = 64 / d;
+        const int k0 = a.retained_ptr[r], L = a.retained_ptr[r + 1] - k0;
+        const int grp = lane / d, j = lane - grp * d;
+        if (grp < ng) {
+            const int chunk = (L + ng - 1) / ng;
+            const int kb = k0 + min(L, grp * chunk), ke = k0 + min(L, (grp + 1) * chunk);
+            int k = kb;
+            for (; k + 8 <= ke; k += 8) {
+                double v[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) v[q] = gp[eoff[a.retained_entry[k + q]] + j];
+#pragma unroll
+                for (int q = 0; q < 8; q++) acc += v[q];
+            }
+            for (; k < ke; k++) acc += gp[eoff[a.retained_entry[k]] + j];
         }
     }
-    part[threadIdx.x] = acc;
+    part[wv][lane] = acc;
     __syncthreads();
-    if (threadIdx.x < dr) {
-        double s = 0.0;
-        for (int c = 0; c < nch; c++) s += part[c * dr + threadIdx.x];
-        g[a.boff_r[r] + threadIdx.x] = s;
+    if (r < a.nrb && lane < d) {
+        double sum = 0.0;
+        for (int q = 0; q < ng; q++) sum += part[wv][q * d + lane];
+        g[a.boff_r[r] + lane] = sum;
     }
 }
 
@@ -557,7 +603,9 @@ constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 8192, RG_CLASS_CAP2 = 19456;
 // the entry dims and Q[e] = sum_{j < e} d_j P[j + 1]. Lower entries of a diagonal block (row > col) are not written.
 template <class T, int NT>
 __global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__restrict__ tabs, const int64_t *__restrict__ goff,
-                                                 int stage_cap, const T *__restrict__ F, double *__restrict__ G) {
+                                                 const int64_t *__restrict__ eoff, int stage_cap, const T *__restrict__ F,
+                                                 const double *__restrict__ Fr, double *__restrict__ G,
+                                                 double *__restrict__ gp) {
     extern __shared__ double XL[];
     __shared__ int16_t s_ent[RG_MAXW];
     __shared__ int s_cst[RG_MAXE], s_dim[RG_MAXE], s_P[RG_MAXE + 1], s_Q[RG_MAXE];
@@ -568,17 +616,30 @@ __global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__res
     const int W = a.entry_col[u0 + ne - 1] + a.r_dims[a.entry_retained[u0 + ne - 1]] - c0;   // <= RG_MAXW
     const int Wp = (W + 15) & ~15, np4 = (n + 3) & ~3, SW = Wp + 1;
     const T *Xg = F + a.raw_off[t] + top;   // X(k, c) = Xg[(c0 + c) M + k]
+    const double *rg = Fr + a.res_off[t] + top;   // r~(k)
     const bool fits = (int64_t)np4 * SW <= stage_cap;
     for (int i = threadIdx.x; i < Wp; i += NT) s_ent[i] = -1;
     for (int e = threadIdx.x; e < ne; e += NT) {
         s_cst[e] = a.entry_col[u0 + e] - c0;
         s_dim[e] = a.r_dims[a.entry_retained[u0 + e]];
     }
-    if (fits)
-        for (int q = threadIdx.x; q < np4 * Wp; q += NT) {
-            const int c = q / np4, k = q - c * np4;
-            XL[k * SW + c] = (k < n && c < W) ? (double)Xg[(int64_t)(c0 + c) * M + k] : 0.0;
+    if (fits) {   // X row-major with stride SW, r~ in the pad column Wp; eight loads in flight per thread
+        const int tot = np4 * Wp;
+        for (int q0 = threadIdx.x; q0 < tot; q0 += 8 * NT) {
+            double v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int q = q0 + j * NT, c = q / np4, k = q - c * np4;
+                v[j] = (q < tot && k < n && c < W) ? (double)Xg[(int64_t)(c0 + c) * M + k] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int q = q0 + j * NT, c = q / np4, k = q - c * np4;
+                if (q < tot) XL[k * SW + c] = v[j];
+            }
         }
+        for (int k = threadIdx.x; k < np4; k += NT) XL[k * SW + Wp] = k < n ? rg[k] : 0.0;
+    }
     __syncthreads();
     for (int e = threadIdx.x; e < ne; e += NT)
         for (int j = 0; j < s_dim[e]; j++) s_ent[s_cst[e] + j] = (int16_t)e;
@@ -593,6 +654,20 @@ __global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__res
         s_P[ne] = p;
     }
     __syncthreads();
+    // per-entry partials of J~' r~ (as k_red_tmul_part writes them), one thread per column
+    const int64_t gbase = eoff[u0];
+    for (int c = threadIdx.x; c < W; c += NT) {
+        const int e = s_ent[c];
+        if (e < 0) continue;
+        double acc = 0.0;
+        if (fits) {
+            for (int k = 0; k < n; k++) acc += XL[k * SW + c] * XL[k * SW + Wp];
+        } else {
+            const T *x = Xg + (int64_t)(c0 + c) * M;
+            for (int k = 0; k < n; k++) acc += (double)x[k] * rg[k];
+        }
+        gp[gbase + s_P[e] + (c - s_cst[e])] = acc;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const int ntl = Wp >> 4, ntt = ntl * (ntl + 1) / 2;
     double *Gt = G + goff[t];
@@ -738,8 +813,10 @@ struct bls_qr {
     DevArray<int64_t> d_rg_goff, d_rg_ptr[2], d_rg_contrib[2];
     DevArray<RGBlock> d_rg_blocks[2];
     int64_t rg_nblk[2] = {0, 0}, rg_glen = 0;
-    DevArray<double> d_rg_G, d_rg_rt;
+    DevArray<double> d_rg_G;
     DevArray<int32_t> d_rg_tabs;                // tables of the four launch classes, one after another
+    DevArray<int64_t> d_eoff;                   // per-entry offsets of the partials of J~' w (prefix of entry dims)
+    DevArray<double> d_gp;                      // the partials
     int64_t rg_cls_off[5] = {0, 0, 0, 0, 0};
     int rg_cap2 = 0;                            // staged doubles of the third class (from the device's LDS)
     std::vector<int32_t> h_tp, h_er, h_ec;   // table_ptr, entry_retained, entry_col (0-based)
@@ -934,6 +1011,10 @@ extern "C" int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int
         (rc = h->d_retained_entry.upload(re)) || (rc = h->d_r_dims.upload(rd)) || (rc = h->d_red_off.upload(ro)) ||
         (rc = h->d_boff_r.upload(bo)) || (rc = h->d_doff.upload(dof)))
         return rc;
+    std::vector<int64_t> eoff(nentries + 1, 0);
+    for (int64_t u = 0; u < nentries; u++) eoff[u + 1] = eoff[u] + rd[er[u]];
+    if ((rc = h->d_eoff.upload(eoff))) return rc;
+    if ((int64_t)h->d_gp.n < eoff[nentries] && (rc = h->d_gp.alloc((size_t)std::max<int64_t>(1, eoff[nentries])))) return rc;
     h->red_len = ro[nt];
     h->dlen = dl;
     h->h_retained_ptr.assign(rp.begin(), rp.end());
@@ -967,7 +1048,9 @@ template <class T> int reduced_tmul(bls_qr *h, double *g, const T *F, const doub
     int rc = reduced_ready(h);
     if (rc) return rc;
     if (!g || !F || !w) return fail(BLS_ERR_ARG, "reduced_tmul: a required array is NULL");
-    if (h->nrb) k_red_tmul<T><<<(unsigned)h->nrb, RED_THREADS, 0, s>>>(h->red(), F, w, g);
+    if (h->ntables)
+        k_red_tmul_part<T><<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), h->d_eoff.ptr, F, w, h->d_gp.ptr);
+    if (h->nrb) k_red_gsum<<<(unsigned)((h->nrb + 3) / 4), 256, 0, s>>>(h->red(), h->d_eoff.ptr, h->d_gp.ptr, g);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
@@ -1160,7 +1243,6 @@ int build_gram_plan(bls_qr *h, int dense) {
         if ((rc = h->d_rg_G.alloc((size_t)acc))) return rc;
         h->rg_glen = acc;
     }
-    if ((int64_t)h->d_rg_rt.n < h->red_len && (rc = h->d_rg_rt.alloc((size_t)h->red_len))) return rc;
     h->rg_nblk[m] = nblk;
     h->rg_built[m] = true;
     return BLS_OK;
@@ -1185,11 +1267,11 @@ template <class T> int reduced_normal_gram(bls_qr *h, int dense, T *S, double *g
             if (c == 2) {
                 BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 512><<<(unsigned)cnt, 512, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, caps[c], F,
-                                                                 h->d_rg_G.ptr);
+                k_red_gram<T, 512><<<(unsigned)cnt, 512, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, h->d_eoff.ptr,
+                                                                 caps[c], F, Fr, h->d_rg_G.ptr, h->d_gp.ptr);
             } else {
-                k_red_gram<T, 256><<<(unsigned)cnt, 256, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, caps[c], F,
-                                                                 h->d_rg_G.ptr);
+                k_red_gram<T, 256><<<(unsigned)cnt, 256, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, h->d_eoff.ptr,
+                                                                 caps[c], F, Fr, h->d_rg_G.ptr, h->d_gp.ptr);
             }
         }
     }
@@ -1197,10 +1279,8 @@ template <class T> int reduced_normal_gram(bls_qr *h, int dense, T *S, double *g
         k_red_gram_sum<T><<<(unsigned)((h->rg_nblk[m] + 3) / 4), 256, 0, s>>>(h->rg_nblk[m], h->d_rg_blocks[m].ptr,
                                                                             h->d_rg_ptr[m].ptr, h->d_rg_contrib[m].ptr,
                                                                             h->d_rg_G.ptr, S);
-    // g_S = J~' r~: the reduced residual rows, then the transposed product (fixed-order sums)
-    if (h->ntables)
-        k_red_residual<<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), Fr, h->d_rg_rt.ptr);
-    if (h->nrb) k_red_tmul<T><<<(unsigned)h->nrb, RED_THREADS, 0, s>>>(h->red(), F, h->d_rg_rt.ptr, g);
+    // g_S = J~' r~ from the per-entry partials the Gram kernel wrote (fixed-order sums)
+    if (h->nrb) k_red_gsum<<<(unsigned)((h->nrb + 3) / 4), 256, 0, s>>>(h->red(), h->d_eoff.ptr, h->d_gp.ptr, g);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
