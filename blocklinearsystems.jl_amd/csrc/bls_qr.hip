@@ -594,6 +594,7 @@ typedef double rg_v4d __attribute__((ext_vector_type(4)));
 // 4096 doubles, up to 8192, up to the device's LDS per workgroup, and the rest, which read their operands from global
 // memory.
 constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 8192, RG_CLASS_CAP2 = 19456;
+static_assert(8 * 512 >= RG_CLASS_CAP0 && 16 * 512 >= RG_CLASS_CAP1 && 19 * 1024 >= RG_CLASS_CAP2, "prefetch covers the classes");
 
 // X = the reduced rows of table t over the columns of its entries (n x W, column c of the table at c0 + c). Each
 // wavefront forms 16 x 16 tiles (I <= J) of X'X with the FP64 matrix instruction (16x16x4: lane l holds A(l % 16,
@@ -601,98 +602,130 @@ constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 8192, RG_CLASS_CAP2 = 19456;
 // fits, and writes each element (a, b), a <= b, to its pair block (e1, e2): the pairs of the table's entries i1 <= i2
 // in the order i2 (i2 + 1) / 2 + i1, each d1 x d2 row-major, at offset Q[e2] + d2 P[e1] with P the prefix sums of
 // the entry dims and Q[e] = sum_{j < e} d_j P[j + 1]. Lower entries of a diagonal block (row > col) are not written.
-template <class T, int NT>
-__global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__restrict__ tabs, const int64_t *__restrict__ goff,
-                                                 const int64_t *__restrict__ eoff, int stage_cap, const T *__restrict__ F,
-                                                 const double *__restrict__ Fr, double *__restrict__ G,
-                                                 double *__restrict__ gp) {
+struct RGGeo {   // where one table's block lies and how it is staged
+    int t, M, top, n, u0, ne, c0, W, Wp, np4, SW;
+    bool fits;
+};
+
+__device__ inline RGGeo rg_geo(const RedArgs &a, int t, int stage_cap) {
+    RGGeo g;
+    g.t = t;
+    g.M = a.nrows[t];
+    g.top = a.top[t];
+    g.n = g.M - g.top;
+    g.u0 = a.table_ptr[t];
+    g.ne = a.table_ptr[t + 1] - g.u0;   // 1..RG_MAXE (the host lists only such tables)
+    g.c0 = a.entry_col[g.u0];
+    g.W = a.entry_col[g.u0 + g.ne - 1] + a.r_dims[a.entry_retained[g.u0 + g.ne - 1]] - g.c0;   // <= RG_MAXW
+    g.Wp = (g.W + 15) & ~15;
+    g.np4 = (g.n + 3) & ~3;
+    g.SW = g.Wp + 1;
+    g.fits = (int64_t)g.np4 * g.SW <= stage_cap;
+    return g;
+}
+
+// Persistent: workgroup b takes tables tabs[b], tabs[b + grid], ...; while it works on one table, the next one's
+// staged block (X row-major with row stride SW = Wp + 1, r~ in column Wp, zeros elsewhere; PF values per thread,
+// PF NT >= stage_cap) is already loading into registers.
+template <class T, int NT, int PF>
+__global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__restrict__ tabs, int64_t cnt,
+                                                 const int64_t *__restrict__ goff, const int64_t *__restrict__ eoff,
+                                                 int stage_cap, const T *__restrict__ F, const double *__restrict__ Fr,
+                                                 double *__restrict__ G, double *__restrict__ gp) {
     extern __shared__ double XL[];
     __shared__ int16_t s_ent[RG_MAXW];
     __shared__ int s_cst[RG_MAXE], s_dim[RG_MAXE], s_P[RG_MAXE + 1], s_Q[RG_MAXE];
-    const int t = tabs[blockIdx.x];
-    const int M = a.nrows[t], top = a.top[t], n = M - top;
-    const int u0 = a.table_ptr[t], ne = a.table_ptr[t + 1] - u0;   // 1..RG_MAXE (host lists only such tables)
-    const int c0 = a.entry_col[u0];
-    const int W = a.entry_col[u0 + ne - 1] + a.r_dims[a.entry_retained[u0 + ne - 1]] - c0;   // <= RG_MAXW
-    const int Wp = (W + 15) & ~15, np4 = (n + 3) & ~3, SW = Wp + 1;
-    const T *Xg = F + a.raw_off[t] + top;   // X(k, c) = Xg[(c0 + c) M + k]
-    const double *rg = Fr + a.res_off[t] + top;   // r~(k)
-    const bool fits = (int64_t)np4 * SW <= stage_cap;
-    for (int i = threadIdx.x; i < Wp; i += NT) s_ent[i] = -1;
-    for (int e = threadIdx.x; e < ne; e += NT) {
-        s_cst[e] = a.entry_col[u0 + e] - c0;
-        s_dim[e] = a.r_dims[a.entry_retained[u0 + e]];
-    }
-    if (fits) {   // X row-major with stride SW, r~ in the pad column Wp; eight loads in flight per thread
-        const int tot = np4 * Wp;
-        for (int q0 = threadIdx.x; q0 < tot; q0 += 8 * NT) {
-            double v[8];
+    double pf[PF];
+    auto prefetch = [&](const RGGeo &g) {
+        if (!g.fits) return;
+        const T *Xg = F + a.raw_off[g.t] + g.top;
+        const double *rg = Fr + a.res_off[g.t] + g.top;
+        // element q = threadIdx.x + j NT is (k, c) = (q % np4, q / np4), walked without divisions
+        const int dc = NT / g.np4, dk = NT - dc * g.np4;
+        int c = threadIdx.x / g.np4, k = threadIdx.x - c * g.np4;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int q = q0 + j * NT, c = q / np4, k = q - c * np4;
-                v[j] = (q < tot && k < n && c < W) ? (double)Xg[(int64_t)(c0 + c) * M + k] : 0.0;
+        for (int j = 0; j < PF; j++) {
+            double v = 0.0;
+            if (c < g.SW && k < g.n) {
+                if (c < g.W) v = (double)Xg[(int64_t)(g.c0 + c) * g.M + k];
+                else if (c == g.Wp) v = rg[k];
             }
+            pf[j] = v;
+            c += dc;
+            k += dk;
+            if (k >= g.np4) {
+                k -= g.np4;
+                c++;
+            }
+        }
+    };
+    RGGeo cur = rg_geo(a, tabs[blockIdx.x], stage_cap);   // the grid is at most cnt
+    prefetch(cur);
+    for (int64_t it = blockIdx.x; it < cnt; it += gridDim.x) {
+        __syncthreads();   // the previous table is done with XL and the maps
+        const int t = cur.t, M = cur.M, top = cur.top, n = cur.n, u0 = cur.u0, ne = cur.ne, c0 = cur.c0, W = cur.W,
+                  Wp = cur.Wp, np4 = cur.np4, SW = cur.SW;
+        const bool fits = cur.fits;
+        if (fits) {
+            const int dc = NT / np4, dk = NT - dc * np4;
+            int c = threadIdx.x / np4, k = threadIdx.x - c * np4;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int q = q0 + j * NT, c = q / np4, k = q - c * np4;
-                if (q < tot) XL[k * SW + c] = v[j];
+            for (int j = 0; j < PF; j++) {
+                if (c < SW) XL[k * SW + c] = pf[j];
+                c += dc;
+                k += dk;
+                if (k >= np4) {
+                    k -= np4;
+                    c++;
+                }
             }
         }
-        for (int k = threadIdx.x; k < np4; k += NT) XL[k * SW + Wp] = k < n ? rg[k] : 0.0;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < ne; e += NT)
-        for (int j = 0; j < s_dim[e]; j++) s_ent[s_cst[e] + j] = (int16_t)e;
-    if (threadIdx.x == 0) {
-        int p = 0, q = 0;
-        for (int e = 0; e < ne; e++) {
-            s_P[e] = p;
-            s_Q[e] = q;
-            p += s_dim[e];
-            q += s_dim[e] * p;
+        for (int i = threadIdx.x; i < Wp; i += NT) s_ent[i] = -1;
+        for (int e = threadIdx.x; e < ne; e += NT) {
+            s_cst[e] = a.entry_col[u0 + e] - c0;
+            s_dim[e] = a.r_dims[a.entry_retained[u0 + e]];
         }
-        s_P[ne] = p;
-    }
-    __syncthreads();
-    // per-entry partials of J~' r~ (as k_red_tmul_part writes them), one thread per column
-    const int64_t gbase = eoff[u0];
-    for (int c = threadIdx.x; c < W; c += NT) {
-        const int e = s_ent[c];
-        if (e < 0) continue;
-        double acc = 0.0;
-        if (fits) {
-            for (int k = 0; k < n; k++) acc += XL[k * SW + c] * XL[k * SW + Wp];
-        } else {
-            const T *x = Xg + (int64_t)(c0 + c) * M;
-            for (int k = 0; k < n; k++) acc += (double)x[k] * rg[k];
-        }
-        gp[gbase + s_P[e] + (c - s_cst[e])] = acc;
-    }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const int ntl = Wp >> 4, ntt = ntl * (ntl + 1) / 2;
-    double *Gt = G + goff[t];
-    for (int tau = wave; tau < ntt; tau += NT / 64) {
-        int J = 0;
-        while ((J + 1) * (J + 2) / 2 <= tau) J++;
-        const int I = tau - J * (J + 1) / 2;
-        rg_v4d acc = {0.0, 0.0, 0.0, 0.0};
-        if (fits) {
-            const double *pa = XL + lk * SW + 16 * I + li, *pb = XL + lk * SW + 16 * J + li;
-            for (int k0 = 0; k0 < np4; k0 += 4)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[k0 * SW], pb[k0 * SW], acc, 0, 0, 0);
-        } else {
-            const bool va = 16 * I + li < W, vb = 16 * J + li < W;
-            const T *xa = Xg + (int64_t)(c0 + 16 * I + li) * M, *xb = Xg + (int64_t)(c0 + 16 * J + li) * M;
-            for (int k0 = 0; k0 < np4; k0 += 4) {
-                const int k = k0 + lk;
-                const double av = (va && k < n) ? (double)xa[k] : 0.0;
-                const double bv = (vb && k < n) ? (double)xb[k] : 0.0;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < ne; e += NT)
+            for (int j = 0; j < s_dim[e]; j++) s_ent[s_cst[e] + j] = (int16_t)e;
+        if (threadIdx.x == 0) {
+            int p = 0, q = 0;
+            for (int e = 0; e < ne; e++) {
+                s_P[e] = p;
+                s_Q[e] = q;
+                p += s_dim[e];
+                q += s_dim[e] * p;
             }
+            s_P[ne] = p;
         }
-        const int b = 16 * J + li;
-        const int e2 = b < W ? s_ent[b] : -1;
-        if (e2 >= 0) {
+        __syncthreads();
+        if (it + gridDim.x < cnt) {
+            cur = rg_geo(a, tabs[it + gridDim.x], stage_cap);
+            prefetch(cur);
+        }
+        const T *Xg = F + a.raw_off[t] + top;          // X(k, c) = Xg[(c0 + c) M + k]
+        const double *rg = Fr + a.res_off[t] + top;    // r~(k)
+        // per-entry partials of J~' r~ (as k_red_tmul_part writes them), one thread per column
+        const int64_t gbase = eoff[u0];
+        for (int c = threadIdx.x; c < W; c += NT) {
+            const int e = s_ent[c];
+            if (e < 0) continue;
+            double acc = 0.0;
+            if (fits) {
+                for (int k = 0; k < n; k++) acc += XL[k * SW + c] * XL[k * SW + Wp];
+            } else {
+                const T *x = Xg + (int64_t)(c0 + c) * M;
+                for (int k = 0; k < n; k++) acc += (double)x[k] * rg[k];
+            }
+            gp[gbase + s_P[e] + (c - s_cst[e])] = acc;
+        }
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+        double *Gt = G + goff[t];
+        // writes tile (I, J) of X'X: element (a, b), a <= b, to its pair block
+        auto store = [&](const rg_v4d &acc, int I, int J) {
+            const int b = 16 * J + li;
+            const int e2 = b < W ? s_ent[b] : -1;
+            if (e2 < 0) return;
             const int col = b - s_cst[e2], d2 = s_dim[e2];
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -700,6 +733,47 @@ __global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__res
                 const int e1 = aa <= b ? s_ent[aa] : -1;
                 if (e1 >= 0) Gt[s_Q[e2] + d2 * s_P[e1] + (aa - s_cst[e1]) * d2 + col] = acc[v];
             }
+        };
+        // work items: column strip J of the upper triangle in pairs of tiles (I0, I0 + 1) that share B
+        const int ntl = Wp >> 4;
+        const int nitems = (ntl / 2 + 1) * (ntl / 2) + ((ntl & 1) ? (ntl + 1) / 2 : 0);
+        for (int item = wave; item < nitems; item += NT / 64) {
+            int J = 0, base = 0;
+            while (base + (J + 2) / 2 <= item) {
+                base += (J + 2) / 2;
+                J++;
+            }
+            const int I0 = 2 * (item - base);
+            const bool two = I0 + 1 <= J;
+            rg_v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+            if (fits) {
+                const double *pb = XL + lk * SW + 16 * J + li, *pa = XL + lk * SW + 16 * I0 + li;
+                if (two) {
+#pragma unroll 4
+                    for (int k0 = 0; k0 < np4; k0 += 4) {
+                        const double bv = pb[k0 * SW], a0 = pa[k0 * SW], a1 = pa[k0 * SW + 16];
+                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc1, 0, 0, 0);
+                    }
+                } else {
+#pragma unroll 4
+                    for (int k0 = 0; k0 < np4; k0 += 4)
+                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[k0 * SW], pb[k0 * SW], acc0, 0, 0, 0);
+                }
+            } else {
+                const bool vb = 16 * J + li < W, va0 = 16 * I0 + li < W, va1 = two && 16 * (I0 + 1) + li < W;
+                const T *xb = Xg + (int64_t)(c0 + 16 * J + li) * M, *xa = Xg + (int64_t)(c0 + 16 * I0 + li) * M;
+                for (int k0 = 0; k0 < np4; k0 += 4) {
+                    const int k = k0 + lk;
+                    const double bv = (vb && k < n) ? (double)xb[k] : 0.0;
+                    const double a0 = (va0 && k < n) ? (double)xa[k] : 0.0;
+                    const double a1 = (va1 && k < n) ? (double)xa[(int64_t)16 * M + k] : 0.0;
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc0, 0, 0, 0);
+                    if (two) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc1, 0, 0, 0);
+                }
+            }
+            store(acc0, I0, J);
+            if (two) store(acc1, I0 + 1, J);
         }
     }
 }
@@ -710,33 +784,42 @@ struct RGBlock {
     int32_t d1, d2, diag, pad;
 };
 
-// One wavefront per block of S: entry w = row d2 + col of the block is the sum of its contributions (row-major pair
-// blocks of the Gram buffer) in ascending table order; zero below the diagonal of a diagonal block.
+// One workgroup per block of S: entry w = row d2 + col of the block is the sum of its contributions (row-major pair
+// blocks of the Gram buffer) in ascending table order, zero below the diagonal of a diagonal block. Thread
+// (w, run) = (tid % sz, tid / sz) adds the run-th of 256 / sz contiguous runs of the block's contribution list, and
+// the runs are then added in order.
 template <class T>
 __global__ __launch_bounds__(256) void k_red_gram_sum(int64_t nblk, const RGBlock *__restrict__ blocks,
                                                       const int64_t *__restrict__ ptr, const int64_t *__restrict__ contrib,
                                                       const double *__restrict__ G, T *__restrict__ S) {
-    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (b >= nblk) return;
+    __shared__ double part[256];
+    const int64_t b = blockIdx.x;
     const RGBlock bk = blocks[b];
-    const int sz = bk.d1 * bk.d2;
-    const int64_t cb = ptr[b], ce = ptr[b + 1];
-    for (int w = lane; w < sz; w += 64) {
-        const int row = w / bk.d2, col = w - row * bk.d2;
-        double acc = 0.0;
-        if (!(bk.diag && row > col)) {
-            int64_t c = cb;
-            for (; c + 8 <= ce; c += 8) {
-                double v[8];
+    const int sz = bk.d1 * bk.d2;   // <= 256
+    const int nrun = 256 / sz;
+    const int w = threadIdx.x % sz, run = threadIdx.x / sz;
+    const int row = w / bk.d2, col = w - row * bk.d2;
+    double acc = 0.0;
+    if (run < nrun && !(bk.diag && row > col)) {
+        const int64_t cb = ptr[b], L = ptr[b + 1] - cb;
+        const int64_t len = (L + nrun - 1) / nrun;
+        const int64_t kb = cb + std::min<int64_t>(L, run * len), ke = cb + std::min<int64_t>(L, (run + 1) * len);
+        int64_t c = kb;
+        for (; c + 8 <= ke; c += 8) {
+            double v[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) v[j] = G[contrib[c + j] + w];
+            for (int j = 0; j < 8; j++) v[j] = G[contrib[c + j] + w];
 #pragma unroll
-                for (int j = 0; j < 8; j++) acc += v[j];
-            }
-            for (; c < ce; c++) acc += G[contrib[c] + w];
+            for (int j = 0; j < 8; j++) acc += v[j];
         }
-        S[bk.dst + (int64_t)col * bk.cstride + row] = (T)acc;
+        for (; c < ke; c++) acc += G[contrib[c] + w];
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < sz) {
+        double sum = 0.0;
+        for (int q = 0; q < nrun; q++) sum += part[q * sz + threadIdx.x];
+        S[bk.dst + (int64_t)col * bk.cstride + row] = (T)sum;
     }
 }
 
@@ -819,6 +902,7 @@ struct bls_qr {
     DevArray<double> d_gp;                      // the partials
     int64_t rg_cls_off[5] = {0, 0, 0, 0, 0};
     int rg_cap2 = 0;                            // staged doubles of the third class (from the device's LDS)
+    int rg_ncu = 0, rg_lds_cu = 0;              // compute units and LDS bytes per compute unit
     std::vector<int32_t> h_tp, h_er, h_ec;   // table_ptr, entry_retained, entry_col (0-based)
     std::vector<int32_t> h_nrows, h_top;
     std::vector<int64_t> h_bcp, h_brv, h_bro, h_ch;   // the sparse pattern (0-based)
@@ -1171,6 +1255,8 @@ int build_gram_plan(bls_qr *h, int dense) {
     BLS_HIP(hipGetDevice(&dev));
     BLS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     h->rg_cap2 = (int)std::min<int64_t>(RG_CLASS_CAP2, std::max<int64_t>(0, (lds - 2048) / 8));
+    BLS_HIP(hipDeviceGetAttribute(&h->rg_ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    BLS_HIP(hipDeviceGetAttribute(&h->rg_lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     const int64_t caps[3] = {RG_CLASS_CAP0, RG_CLASS_CAP1, h->rg_cap2};
     std::vector<int32_t> cls[4];
     for (int64_t t = 0; t < nt; t++) {
@@ -1264,19 +1350,30 @@ template <class T> int reduced_normal_gram(bls_qr *h, int dense, T *S, double *g
             if (!cnt) continue;
             const int32_t *tabs = h->d_rg_tabs.ptr + h->rg_cls_off[c];
             const size_t lds = (size_t)caps[c] * sizeof(double);
-            if (c == 2) {
-                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512>,
+            const int nt = c == 2 ? 1024 : 512;
+            // persistent grid: as many workgroups as the compute units hold at once (LDS and 32 waves per unit)
+            const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(h->rg_lds_cu / (int64_t)(lds + 2048), 32 / (nt / 64)));
+            const unsigned grid = (unsigned)std::min<int64_t>(cnt, per_cu * std::max(1, h->rg_ncu));
+            const RedArgs ra = h->red();
+            const int64_t *go = h->d_rg_goff.ptr, *eo = h->d_eoff.ptr;
+            double *Gp = h->d_rg_G.ptr, *gpp = h->d_gp.ptr;
+            if (c == 0 || c == 3) {
+                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512, 8>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 512><<<(unsigned)cnt, 512, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, h->d_eoff.ptr,
-                                                                 caps[c], F, Fr, h->d_rg_G.ptr, h->d_gp.ptr);
+                k_red_gram<T, 512, 8><<<grid, 512, lds, s>>>(ra, tabs, cnt, go, eo, caps[c], F, Fr, Gp, gpp);
+            } else if (c == 1) {
+                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512, 16>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                k_red_gram<T, 512, 16><<<grid, 512, lds, s>>>(ra, tabs, cnt, go, eo, caps[c], F, Fr, Gp, gpp);
             } else {
-                k_red_gram<T, 256><<<(unsigned)cnt, 256, lds, s>>>(h->red(), tabs, h->d_rg_goff.ptr, h->d_eoff.ptr,
-                                                                 caps[c], F, Fr, h->d_rg_G.ptr, h->d_gp.ptr);
+                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 1024, 19>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                k_red_gram<T, 1024, 19><<<grid, 1024, lds, s>>>(ra, tabs, cnt, go, eo, caps[c], F, Fr, Gp, gpp);
             }
         }
     }
     if (h->rg_nblk[m])
-        k_red_gram_sum<T><<<(unsigned)((h->rg_nblk[m] + 3) / 4), 256, 0, s>>>(h->rg_nblk[m], h->d_rg_blocks[m].ptr,
+        k_red_gram_sum<T><<<(unsigned)h->rg_nblk[m], 256, 0, s>>>(h->rg_nblk[m], h->d_rg_blocks[m].ptr,
                                                                             h->d_rg_ptr[m].ptr, h->d_rg_contrib[m].ptr,
                                                                             h->d_rg_G.ptr, S);
     // g_S = J~' r~ from the per-entry partials the Gram kernel wrote (fixed-order sums)
