@@ -594,7 +594,7 @@ typedef double rg_v4d __attribute__((ext_vector_type(4)));
 // 4096 doubles, up to 8192, up to the device's LDS per workgroup, and the rest, which read their operands from global
 // memory.
 constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 8192, RG_CLASS_CAP2 = 19456;
-static_assert(8 * 512 >= RG_CLASS_CAP0 && 16 * 512 >= RG_CLASS_CAP1 && 19 * 1024 >= RG_CLASS_CAP2, "prefetch covers the classes");
+static_assert(16 * 256 >= RG_CLASS_CAP0 && 16 * 512 >= RG_CLASS_CAP1 && 19 * 1024 >= RG_CLASS_CAP2, "prefetch covers the classes");
 
 // X = the reduced rows of table t over the columns of its entries (n x W, column c of the table at c0 + c). Each
 // wavefront forms 16 x 16 tiles (I <= J) of X'X with the FP64 matrix instruction (16x16x4: lane l holds A(l % 16,
@@ -602,70 +602,82 @@ static_assert(8 * 512 >= RG_CLASS_CAP0 && 16 * 512 >= RG_CLASS_CAP1 && 19 * 1024
 // fits, and writes each element (a, b), a <= b, to its pair block (e1, e2): the pairs of the table's entries i1 <= i2
 // in the order i2 (i2 + 1) / 2 + i1, each d1 x d2 row-major, at offset Q[e2] + d2 P[e1] with P the prefix sums of
 // the entry dims and Q[e] = sum_{j < e} d_j P[j + 1]. Lower entries of a diagonal block (row > col) are not written.
-struct RGGeo {   // where one table's block lies and how it is staged
-    int t, M, top, n, u0, ne, c0, W, Wp, np4, SW;
-    bool fits;
+// One table of the Gram pass, in launch order (built on the host): where its block and residual lie, where its pair
+// blocks and per-entry partials go, its reduced rows n and the W columns its entries span from column c0.
+struct RGTab {
+    int64_t raw_off, res_off, goff, eoff;
+    int32_t M, top, n, u0, ne, c0, W, pad;
 };
 
-__device__ inline RGGeo rg_geo(const RedArgs &a, int t, int stage_cap) {
-    RGGeo g;
-    g.t = t;
-    g.M = a.nrows[t];
-    g.top = a.top[t];
-    g.n = g.M - g.top;
-    g.u0 = a.table_ptr[t];
-    g.ne = a.table_ptr[t + 1] - g.u0;   // 1..RG_MAXE (the host lists only such tables)
-    g.c0 = a.entry_col[g.u0];
-    g.W = a.entry_col[g.u0 + g.ne - 1] + a.r_dims[a.entry_retained[g.u0 + g.ne - 1]] - g.c0;   // <= RG_MAXW
-    g.Wp = (g.W + 15) & ~15;
-    g.np4 = (g.n + 3) & ~3;
-    g.SW = g.Wp + 1;
-    g.fits = (int64_t)g.np4 * g.SW <= stage_cap;
+__device__ inline RGTab rg_load(const RGTab *p) {   // field by field, so the record stays in registers
+    RGTab g;
+    g.raw_off = p->raw_off;
+    g.res_off = p->res_off;
+    g.goff = p->goff;
+    g.eoff = p->eoff;
+    g.M = p->M;
+    g.top = p->top;
+    g.n = p->n;
+    g.u0 = p->u0;
+    g.ne = p->ne;
+    g.c0 = p->c0;
+    g.W = p->W;
+    g.pad = 0;
     return g;
 }
 
 // Persistent: workgroup b takes tables tabs[b], tabs[b + grid], ...; while it works on one table, the next one's
 // staged block (X row-major with row stride SW = Wp + 1, r~ in column Wp, zeros elsewhere; PF values per thread,
-// PF NT >= stage_cap) is already loading into registers.
+// PF NT >= stage_cap) and entry words (ecd[u] = (entry column - c0) << 16 | entry dim) are loading into registers, and
+// the record of the one after it is loading too.
 template <class T, int NT, int PF>
-__global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__restrict__ tabs, int64_t cnt,
-                                                 const int64_t *__restrict__ goff, const int64_t *__restrict__ eoff,
-                                                 int stage_cap, const T *__restrict__ F, const double *__restrict__ Fr,
-                                                 double *__restrict__ G, double *__restrict__ gp) {
+__global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs, int64_t cnt,
+                                                 const int32_t *__restrict__ ecd, int stage_cap, const T *__restrict__ F,
+                                                 const double *__restrict__ Fr, double *__restrict__ G,
+                                                 double *__restrict__ gp) {
     extern __shared__ double XL[];
     __shared__ int16_t s_ent[RG_MAXW];
     __shared__ int s_cst[RG_MAXE], s_dim[RG_MAXE], s_P[RG_MAXE + 1], s_Q[RG_MAXE];
     double pf[PF];
-    auto prefetch = [&](const RGGeo &g) {
-        if (!g.fits) return;
-        const T *Xg = F + a.raw_off[g.t] + g.top;
-        const double *rg = Fr + a.res_off[g.t] + g.top;
+    int my_ecd = 0;
+    auto prefetch = [&](const RGTab g) {
+        my_ecd = (int)threadIdx.x < g.ne ? ecd[g.u0 + threadIdx.x] : 0;
+        const int Wp = (g.W + 15) & ~15, np4 = (g.n + 3) & ~3, SW = Wp + 1;
+        if ((int64_t)np4 * SW > stage_cap) return;
+        const T *Xg = F + g.raw_off + g.top;
+        const double *rg = Fr + g.res_off + g.top;
         // element q = threadIdx.x + j NT is (k, c) = (q % np4, q / np4), walked without divisions
-        const int dc = NT / g.np4, dk = NT - dc * g.np4;
-        int c = threadIdx.x / g.np4, k = threadIdx.x - c * g.np4;
+        const int dc = NT / np4, dk = NT - dc * np4;
+        int c = threadIdx.x / np4, k = threadIdx.x - c * np4;
 #pragma unroll
         for (int j = 0; j < PF; j++) {
             double v = 0.0;
-            if (c < g.SW && k < g.n) {
+            if (c < SW && k < g.n) {
                 if (c < g.W) v = (double)Xg[(int64_t)(g.c0 + c) * g.M + k];
-                else if (c == g.Wp) v = rg[k];
+                else if (c == Wp) v = rg[k];
             }
             pf[j] = v;
             c += dc;
             k += dk;
-            if (k >= g.np4) {
-                k -= g.np4;
+            if (k >= np4) {
+                k -= np4;
                 c++;
             }
         }
     };
-    RGGeo cur = rg_geo(a, tabs[blockIdx.x], stage_cap);   // the grid is at most cnt
+    const int64_t grid = gridDim.x;
+    RGTab cur = rg_load(tabs + blockIdx.x);   // the grid is at most cnt
+    RGTab nxt = rg_load(tabs + (blockIdx.x + grid < cnt ? blockIdx.x + grid : blockIdx.x));
     prefetch(cur);
-    for (int64_t it = blockIdx.x; it < cnt; it += gridDim.x) {
+    for (int64_t it = blockIdx.x; it < cnt; it += grid) {
         __syncthreads();   // the previous table is done with XL and the maps
-        const int t = cur.t, M = cur.M, top = cur.top, n = cur.n, u0 = cur.u0, ne = cur.ne, c0 = cur.c0, W = cur.W,
-                  Wp = cur.Wp, np4 = cur.np4, SW = cur.SW;
-        const bool fits = cur.fits;
+        const int M = cur.M, n = cur.n, ne = cur.ne, c0 = cur.c0, W = cur.W;
+        const int Wp = (W + 15) & ~15, np4 = (n + 3) & ~3, SW = Wp + 1;
+        const bool fits = (int64_t)np4 * SW <= stage_cap;
+        const T *Xg = F + cur.raw_off + cur.top;        // X(k, c) = Xg[(c0 + c) M + k]
+        const double *rg = Fr + cur.res_off + cur.top;  // r~(k)
+        double *Gt = G + cur.goff;
+        double *gpt = gp + cur.eoff;
         if (fits) {
             const int dc = NT / np4, dk = NT - dc * np4;
             int c = threadIdx.x / np4, k = threadIdx.x - c * np4;
@@ -681,46 +693,53 @@ __global__ __launch_bounds__(NT) void k_red_gram(RedArgs a, const int32_t *__res
             }
         }
         for (int i = threadIdx.x; i < Wp; i += NT) s_ent[i] = -1;
-        for (int e = threadIdx.x; e < ne; e += NT) {
-            s_cst[e] = a.entry_col[u0 + e] - c0;
-            s_dim[e] = a.r_dims[a.entry_retained[u0 + e]];
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < ne; e += NT)
-            for (int j = 0; j < s_dim[e]; j++) s_ent[s_cst[e] + j] = (int16_t)e;
-        if (threadIdx.x == 0) {
-            int p = 0, q = 0;
-            for (int e = 0; e < ne; e++) {
-                s_P[e] = p;
-                s_Q[e] = q;
-                p += s_dim[e];
-                q += s_dim[e] * p;
+        const int e_cst = my_ecd >> 16, e_dim = my_ecd & 0xffff;   // entry threadIdx.x (< ne)
+        if (threadIdx.x < 64) {   // prefix sums of the entry dims: P (columns) and Q (pair-block offsets)
+            const int e = threadIdx.x;
+            const int d = e < ne ? e_dim : 0;
+            int p = d;
+            for (int o = 1; o < 32; o <<= 1) {
+                const int v = __shfl_up(p, o);
+                if (e >= o) p += v;
             }
-            s_P[ne] = p;
+            const int q0 = d * p;   // d_e P[e + 1]
+            int q = q0;
+            for (int o = 1; o < 32; o <<= 1) {
+                const int v = __shfl_up(q, o);
+                if (e >= o) q += v;
+            }
+            if (e < ne) {
+                s_cst[e] = e_cst;
+                s_dim[e] = d;
+                s_P[e] = p - d;
+                s_Q[e] = q - q0;
+            }
+            if (e == ne - 1) s_P[ne] = p;
         }
         __syncthreads();
-        if (it + gridDim.x < cnt) {
-            cur = rg_geo(a, tabs[it + gridDim.x], stage_cap);
+        if ((int)threadIdx.x < ne)
+            for (int j = 0; j < e_dim; j++) s_ent[e_cst + j] = (int16_t)threadIdx.x;
+        __syncthreads();
+        if (it + grid < cnt) {   // the next table's loads go out before this one's work
+            cur = nxt;
             prefetch(cur);
+            if (it + 2 * grid < cnt) nxt = rg_load(tabs + it + 2 * grid);
         }
-        const T *Xg = F + a.raw_off[t] + top;          // X(k, c) = Xg[(c0 + c) M + k]
-        const double *rg = Fr + a.res_off[t] + top;    // r~(k)
         // per-entry partials of J~' r~ (as k_red_tmul_part writes them), one thread per column
-        const int64_t gbase = eoff[u0];
         for (int c = threadIdx.x; c < W; c += NT) {
             const int e = s_ent[c];
             if (e < 0) continue;
             double acc = 0.0;
             if (fits) {
+#pragma unroll 8
                 for (int k = 0; k < n; k++) acc += XL[k * SW + c] * XL[k * SW + Wp];
             } else {
                 const T *x = Xg + (int64_t)(c0 + c) * M;
                 for (int k = 0; k < n; k++) acc += (double)x[k] * rg[k];
             }
-            gp[gbase + s_P[e] + (c - s_cst[e])] = acc;
+            gpt[s_P[e] + (c - s_cst[e])] = acc;
         }
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-        double *Gt = G + goff[t];
         // writes tile (I, J) of X'X: element (a, b), a <= b, to its pair block
         auto store = [&](const rg_v4d &acc, int I, int J) {
             const int b = 16 * J + li;
@@ -890,14 +909,15 @@ struct bls_qr {
     DevArray<RNMerge> d_rn_merges[2];
     int64_t rn_nitems[2] = {0, 0}, rn_nmerges[2] = {0, 0};
     DevArray<double> d_rn_partial;
-    // the Gram formulation: per-table offsets into the contribution buffer, and per layout the S blocks with their
-    // contribution lists (ascending table order)
+    // the Gram formulation: per-table records in launch order (with offsets into the contribution buffer), and per
+    // layout the S blocks with their contribution lists (ascending table order)
     bool rg_built[2] = {false, false};
-    DevArray<int64_t> d_rg_goff, d_rg_ptr[2], d_rg_contrib[2];
+    DevArray<int64_t> d_rg_ptr[2], d_rg_contrib[2];
     DevArray<RGBlock> d_rg_blocks[2];
     int64_t rg_nblk[2] = {0, 0}, rg_glen = 0;
     DevArray<double> d_rg_G;
-    DevArray<int32_t> d_rg_tabs;                // tables of the four launch classes, one after another
+    DevArray<RGTab> d_rg_tabs;                  // tables of the four launch classes, one after another
+    DevArray<int32_t> d_rg_ecd;                 // per entry: (column - first column of its table) << 16 | dim
     DevArray<int64_t> d_eoff;                   // per-entry offsets of the partials of J~' w (prefix of entry dims)
     DevArray<double> d_gp;                      // the partials
     int64_t rg_cls_off[5] = {0, 0, 0, 0, 0};
@@ -905,6 +925,7 @@ struct bls_qr {
     int rg_ncu = 0, rg_lds_cu = 0;              // compute units and LDS bytes per compute unit
     std::vector<int32_t> h_tp, h_er, h_ec;   // table_ptr, entry_retained, entry_col (0-based)
     std::vector<int32_t> h_nrows, h_top;
+    std::vector<int64_t> h_raw_off, h_res_off;   // 0-based
     std::vector<int64_t> h_bcp, h_brv, h_bro, h_ch;   // the sparse pattern (0-based)
     std::vector<int64_t> h_retained_ptr, h_S_colptr;   // host copies for the plans (0-based)
     std::vector<int32_t> h_r_dims;
@@ -1041,9 +1062,11 @@ extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, con
     h->raw_len = raw_offsets[ntables] - 1;
     h->res_len = residual_offsets[ntables] - 1;
     int rc;
+    h->h_raw_off = sub1(raw_offsets, ntables + 1);
+    h->h_res_off = sub1(residual_offsets, ntables + 1);
     if ((rc = h->d_nrows.upload(nrows)) || (rc = h->d_ncols.upload(ncols)) || (rc = h->d_edims.upload(edims)) ||
-        (rc = h->d_raw_off.upload(sub1(raw_offsets, ntables + 1))) ||
-        (rc = h->d_res_off.upload(sub1(residual_offsets, ntables + 1))) || (rc = h->d_boff_e.upload(boff)) ||
+        (rc = h->d_raw_off.upload(h->h_raw_off)) || (rc = h->d_res_off.upload(h->h_res_off)) ||
+        (rc = h->d_boff_e.upload(boff)) ||
         (rc = h->d_flag.alloc(1)) || (rc = h->d_failed_out.alloc(1))) {
         delete h;
         return rc;
@@ -1270,14 +1293,35 @@ int build_gram_plan(bls_qr *h, int dense) {
             if (st <= caps[k]) c = k;
         cls[c].push_back((int32_t)t);
     }
-    std::vector<int32_t> tabs;
+    std::vector<RGTab> tabs;
+    std::vector<int64_t> eoff(er.size() + 1, 0);
+    for (size_t u = 0; u < er.size(); u++) eoff[u + 1] = eoff[u] + rd[er[u]];
     for (int c = 0; c < 4; c++) {
         h->rg_cls_off[c] = (int64_t)tabs.size();
-        tabs.insert(tabs.end(), cls[c].begin(), cls[c].end());
+        for (int32_t t : cls[c]) {
+            const int u0 = tp[t], ne = tp[t + 1] - u0;
+            RGTab g;
+            g.raw_off = h->h_raw_off[t];
+            g.res_off = h->h_res_off[t];
+            g.goff = goff[t];
+            g.eoff = eoff[u0];
+            g.M = h->h_nrows[t];
+            g.top = h->h_top[t];
+            g.n = g.M - g.top;
+            g.u0 = u0;
+            g.ne = ne;
+            g.c0 = ec[u0];
+            g.W = ec[u0 + ne - 1] + rd[er[u0 + ne - 1]] - ec[u0];
+            g.pad = 0;
+            tabs.push_back(g);
+        }
     }
     h->rg_cls_off[4] = (int64_t)tabs.size();
+    std::vector<int32_t> ecd(er.size());
+    for (int64_t t = 0; t < nt; t++)
+        for (int u = tp[t]; u < tp[t + 1]; u++) ecd[u] = ((ec[u] - ec[tp[t]]) << 16) | rd[er[u]];
     int rc0;
-    if ((rc0 = h->d_rg_tabs.upload(tabs))) return rc0;
+    if ((rc0 = h->d_rg_tabs.upload(tabs)) || (rc0 = h->d_rg_ecd.upload(ecd))) return rc0;
     std::vector<RGBlock> blocks;
     if (dense) {
         blocks.reserve((size_t)(nrb * (nrb + 1) / 2));
@@ -1324,7 +1368,7 @@ int build_gram_plan(bls_qr *h, int dense) {
         }
     }
     int rc;
-    if ((rc = h->d_rg_goff.upload(goff)) || (rc = h->d_rg_blocks[m].upload(blocks))) return rc;
+    if ((rc = h->d_rg_blocks[m].upload(blocks))) return rc;
     if (acc > h->rg_glen) {
         if ((rc = h->d_rg_G.alloc((size_t)acc))) return rc;
         h->rg_glen = acc;
@@ -1348,27 +1392,26 @@ template <class T> int reduced_normal_gram(bls_qr *h, int dense, T *S, double *g
         for (int c = 0; c < 4; c++) {
             const int64_t cnt = h->rg_cls_off[c + 1] - h->rg_cls_off[c];
             if (!cnt) continue;
-            const int32_t *tabs = h->d_rg_tabs.ptr + h->rg_cls_off[c];
+            const RGTab *tabs = h->d_rg_tabs.ptr + h->rg_cls_off[c];
             const size_t lds = (size_t)caps[c] * sizeof(double);
-            const int nt = c == 2 ? 1024 : 512;
+            const int nt = c == 2 ? 1024 : c == 1 ? 512 : 256;
             // persistent grid: as many workgroups as the compute units hold at once (LDS and 32 waves per unit)
             const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(h->rg_lds_cu / (int64_t)(lds + 2048), 32 / (nt / 64)));
             const unsigned grid = (unsigned)std::min<int64_t>(cnt, per_cu * std::max(1, h->rg_ncu));
-            const RedArgs ra = h->red();
-            const int64_t *go = h->d_rg_goff.ptr, *eo = h->d_eoff.ptr;
+            const int32_t *ecd = h->d_rg_ecd.ptr;
             double *Gp = h->d_rg_G.ptr, *gpp = h->d_gp.ptr;
             if (c == 0 || c == 3) {
-                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512, 8>,
+                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 256, 16>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 512, 8><<<grid, 512, lds, s>>>(ra, tabs, cnt, go, eo, caps[c], F, Fr, Gp, gpp);
+                k_red_gram<T, 256, 16><<<grid, 256, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
             } else if (c == 1) {
                 BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512, 16>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 512, 16><<<grid, 512, lds, s>>>(ra, tabs, cnt, go, eo, caps[c], F, Fr, Gp, gpp);
+                k_red_gram<T, 512, 16><<<grid, 512, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
             } else {
                 BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 1024, 19>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 1024, 19><<<grid, 1024, lds, s>>>(ra, tabs, cnt, go, eo, caps[c], F, Fr, Gp, gpp);
+                k_red_gram<T, 1024, 19><<<grid, 1024, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
             }
         }
     }
