@@ -409,182 +409,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const T *F,
 }
 
 // reduced_normal_dense! / reduced_normal_sparse! (src/qr_reduced.jl:373-480): the upper triangle of S = J~'J~ and
-// g_S = J~'r~. A work item is a block column r2 and a contiguous run of the tables holding r2, in the reference's
-// order; the workgroup takes those tables one at a time, spreads the (u1, row, col) dot products of each table's
-// entries u1 <= u2 over its threads, and adds each to its entry, with a barrier between tables. A column whose tables
-// form one item is written in place (every scalar column of it zeroed first, as the reference does); a column split
-// into parts sums into one partial copy of the column per part, and k_red_normal_merge adds the parts in order.
-// Every entry is therefore summed over the tables in a fixed order: deterministic, no atomics.
-constexpr int RN_MAXE = 64;      // entries of one table handled per pass
-constexpr int RN_PATTERN = 4096; // pattern blocks of a column kept in LDS for the row-offset search
-struct NormalArgs {
-    int dense;
-    int64_t nr;
-    const int64_t *S_colptr;                          // sparse: 0-based start of every scalar column
-    const int32_t *block_colptr, *block_rowval;       // sparse block pattern (0-based)
-    const int64_t *block_row_offsets, *col_heights;
-};
-struct RNItem {
-    int32_t r2, k_lo, k_hi, nparts;
-    int64_t pbase;   // parts: this part's copy of the column ([S column | g]) in the partial buffer
-};
-
-template <class T>
-__global__ __launch_bounds__(RED_THREADS) void k_red_normal(RedArgs a, NormalArgs na, const RNItem *__restrict__ items,
-                                                            const T *__restrict__ F, const double *__restrict__ Fr,
-                                                            T *__restrict__ S, double *__restrict__ g,
-                                                            double *__restrict__ partial) {
-    __shared__ int s_pre[RN_MAXE + 1];   // prefix of dr1 * dr2 over the pass's entries u1
-    __shared__ int s_dr1[RN_MAXE], s_col1[RN_MAXE];
-    __shared__ int64_t s_base[RN_MAXE];  // offset of (row 0, col 0) of block (r1, r2) within the column
-    __shared__ int s_rows[RN_PATTERN];   // the column's pattern block rows (sparse)
-    const RNItem it = items[blockIdx.x];
-    const int r2 = it.r2;
-    const int dr2 = a.r_dims[r2];
-    const int64_t ro2 = a.boff_r[r2];
-    // the column: dense rows 0..nr-1 of columns ro2.., or the sparse scalar columns of block column r2
-    const int64_t cbase = na.dense ? ro2 * na.nr : na.S_colptr[ro2];
-    const int64_t csize = na.dense ? (int64_t)dr2 * na.nr : na.S_colptr[ro2 + dr2] - cbase;
-    const bool direct = it.nparts == 1;
-    double *P = partial + it.pbase;   // parts: [column | g]
-    if (direct) {
-        for (int64_t q = threadIdx.x; q < csize; q += RED_THREADS) S[cbase + q] = (T)0;
-        if (threadIdx.x < dr2) g[ro2 + threadIdx.x] = 0.0;
-    } else {
-        for (int64_t q = threadIdx.x; q < csize + dr2; q += RED_THREADS) P[q] = 0.0;
-    }
-    int b_lo = 0, nb = 0;
-    if (!na.dense) {
-        b_lo = na.block_colptr[r2];
-        nb = na.block_colptr[r2 + 1] - b_lo;
-        if (nb <= RN_PATTERN)
-            for (int q = threadIdx.x; q < nb; q += RED_THREADS) s_rows[q] = na.block_rowval[b_lo + q];
-    }
-    __syncthreads();
-    const int64_t cstride = na.dense ? na.nr : na.col_heights[r2];
-    for (int k = it.k_lo; k < it.k_hi; k++) {
-        const int u2 = a.retained_entry[k];
-        const int t = a.entry_table[u2];
-        const int M = a.nrows[t], top = a.top[t], n = M - top;
-        const T *Ft = F + a.raw_off[t] + top;
-        const T *c2 = Ft + (int64_t)a.entry_col[u2] * M;
-        // g_S[r2] += F[:, c2]' r~ (Float64)
-        if (threadIdx.x < dr2) {
-            const double *rt = Fr + a.res_off[t] + top;
-            const T *cc = c2 + (int64_t)threadIdx.x * M;
-            double acc = 0.0;
-            int i = 0;
-            for (; i + 8 <= n; i += 8) {
-                double xv[8], rv[8];
-#pragma unroll
-                for (int q8 = 0; q8 < 8; q8++) {
-                    xv[q8] = (double)cc[i + q8];
-                    rv[q8] = rt[i + q8];
-                }
-#pragma unroll
-                for (int q8 = 0; q8 < 8; q8++) acc += xv[q8] * rv[q8];
-            }
-            for (; i < n; i++) acc += (double)cc[i] * rt[i];
-            if (direct) g[ro2 + threadIdx.x] += acc;
-            else P[csize + threadIdx.x] += acc;
-        }
-        for (int u0 = a.table_ptr[t]; u0 <= u2; u0 += RN_MAXE) {
-            const int ne = min(RN_MAXE, u2 - u0 + 1);
-            if ((int)threadIdx.x < ne) {
-                const int u1 = u0 + threadIdx.x;
-                const int r1 = a.entry_retained[u1];
-                s_dr1[threadIdx.x] = a.r_dims[r1];
-                s_col1[threadIdx.x] = a.entry_col[u1];
-                if (na.dense) {
-                    s_base[threadIdx.x] = a.boff_r[r1];
-                } else {   // the block (r1, r2) of the pattern (present: checked when the pattern was registered)
-                    int lo = 0, hi = nb - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        const int rv = nb <= RN_PATTERN ? s_rows[mid] : na.block_rowval[b_lo + mid];
-                        if (rv < r1) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    s_base[threadIdx.x] = na.block_row_offsets[b_lo + lo];
-                }
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                int acc = 0;
-                for (int e = 0; e < ne; e++) {
-                    s_pre[e] = acc;
-                    acc += s_dr1[e] * dr2;
-                }
-                s_pre[ne] = acc;
-            }
-            __syncthreads();
-            for (int q = threadIdx.x; q < s_pre[ne]; q += RED_THREADS) {
-                int lo = 0, hi = ne - 1;   // the entry e with s_pre[e] <= q < s_pre[e + 1]
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_pre[mid] <= q) lo = mid;
-                    else hi = mid - 1;
-                }
-                const int e = lo;
-                const int dr1 = s_dr1[e];
-                const int w = q - s_pre[e];
-                const int col = w / dr1, row = w - col * dr1;
-                if (u0 + e == u2 && row > col) continue;   // diagonal block: upper triangle only
-                const T *x = Ft + (int64_t)(s_col1[e] + row) * M;
-                const T *y = c2 + (int64_t)col * M;
-                const int64_t di = s_base[e] + (int64_t)col * cstride + row;   // within the column
-                const double d0 = direct ? (double)S[cbase + di] : P[di];
-                // eight products in flight per step, summed in index order
-                double acc = 0.0;
-                int i = 0;
-                for (; i + 8 <= n; i += 8) {
-                    double xv[8], yv[8];
-#pragma unroll
-                    for (int q8 = 0; q8 < 8; q8++) {
-                        xv[q8] = (double)x[i + q8];
-                        yv[q8] = (double)y[i + q8];
-                    }
-#pragma unroll
-                    for (int q8 = 0; q8 < 8; q8++) acc += xv[q8] * yv[q8];
-                }
-                for (; i < n; i++) acc += (double)x[i] * (double)y[i];
-                if (direct) S[cbase + di] = (T)(d0 + acc);
-                else P[di] = d0 + acc;
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// The parts of a split column, added in part order into S (narrowed to T once) and g: one workgroup per column.
-struct RNMerge {
-    int32_t r2, nparts;
-    int64_t pbase;   // the first part's copy; parts follow at a stride of (column size + d_r2)
-};
-
-template <class T>
-__global__ __launch_bounds__(RED_THREADS) void k_red_normal_merge(RedArgs a, NormalArgs na, const RNMerge *merges,
-                                                                  const double *partial, T *S, double *g) {
-    const RNMerge mg = merges[blockIdx.x];
-    const int dr2 = a.r_dims[mg.r2];
-    const int64_t ro2 = a.boff_r[mg.r2];
-    const int64_t cbase = na.dense ? ro2 * na.nr : na.S_colptr[ro2];
-    const int64_t csize = na.dense ? (int64_t)dr2 * na.nr : na.S_colptr[ro2 + dr2] - cbase;
-    const int64_t stride = csize + dr2;
-    for (int64_t q = threadIdx.x; q < stride; q += RED_THREADS) {
-        const double *p = partial + mg.pbase + q;
-        double acc = p[0];
-        for (int k = 1; k < mg.nparts; k++) acc += p[k * stride];
-        if (q < csize) S[cbase + q] = (T)acc;
-        else g[ro2 + (q - csize)] = acc;
-    }
-}
-
-// reduced_normal by per-table Gram products. k_red_gram: one workgroup per table forms the blocks
-// X(:, cols(u1))' X(:, cols(u2)) of every pair of its entries u1 <= u2 (X: the table's reduced rows, staged in LDS when
-// they fit) into a contribution buffer, each table read once. k_red_gram_sum: one wavefront per block (r1, r2) of S
-// adds its contributions in ascending table order -- the order in which the reference adds them to each entry -- and
-// writes the whole block (the lower triangle of a diagonal block, and a block without contributions, as zeros).
+// g_S = J~'r~, by per-table Gram products. k_red_gram forms, for every table, the blocks X(:, cols(u1))' X(:, cols(u2))
+// of each pair of its entries u1 <= u2 (X: the table's reduced rows) on the FP64 matrix instruction, each table read
+// once, into a contribution buffer, and the per-entry partials X(:, cols(u))' r~. k_red_gram_sum adds the
+// contributions of each block (r1, r2) of S in ascending table order -- the order in which the reference adds them to
+// each entry -- and writes the whole block (the lower triangle of a diagonal block, and a block without
+// contributions, as zeros); k_red_gsum adds the partials per retained block. Deterministic, no atomics.
 constexpr int RG_MAXE = 31;    // entries of one table
 constexpr int RG_MAXW = 512;   // columns spanned by the entries of one table
 
@@ -596,12 +426,6 @@ typedef double rg_v4d __attribute__((ext_vector_type(4)));
 constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 8192, RG_CLASS_CAP2 = 19456;
 static_assert(16 * 256 >= RG_CLASS_CAP0 && 16 * 512 >= RG_CLASS_CAP1 && 19 * 1024 >= RG_CLASS_CAP2, "prefetch covers the classes");
 
-// X = the reduced rows of table t over the columns of its entries (n x W, column c of the table at c0 + c). Each
-// wavefront forms 16 x 16 tiles (I <= J) of X'X with the FP64 matrix instruction (16x16x4: lane l holds A(l % 16,
-// l / 16), B(l / 16, l % 16), and D(l / 16 + 4 v, l % 16) in accumulator v), from X staged row-major in LDS when it
-// fits, and writes each element (a, b), a <= b, to its pair block (e1, e2): the pairs of the table's entries i1 <= i2
-// in the order i2 (i2 + 1) / 2 + i1, each d1 x d2 row-major, at offset Q[e2] + d2 P[e1] with P the prefix sums of
-// the entry dims and Q[e] = sum_{j < e} d_j P[j + 1]. Lower entries of a diagonal block (row > col) are not written.
 // One table of the Gram pass, in launch order (built on the host): where its block and residual lie, where its pair
 // blocks and per-entry partials go, its reduced rows n and the W columns its entries span from column c0.
 struct RGTab {
@@ -626,6 +450,12 @@ __device__ inline RGTab rg_load(const RGTab *p) {   // field by field, so the re
     return g;
 }
 
+// X = the reduced rows of table t over the columns of its entries (n x W, column c of the table at c0 + c). Each
+// wavefront forms 16 x 16 tiles (I <= J) of X'X with the FP64 matrix instruction (16x16x4: lane l holds A(l % 16,
+// l / 16), B(l / 16, l % 16), and D(l / 16 + 4 v, l % 16) in accumulator v), from X staged row-major in LDS when it
+// fits, and writes each element (a, b), a <= b, to its pair block (e1, e2): the pairs of the table's entries i1 <= i2
+// in the order i2 (i2 + 1) / 2 + i1, each d1 x d2 row-major, at offset Q[e2] + d2 P[e1] with P the prefix sums of
+// the entry dims and Q[e] = sum_{j < e} d_j P[j + 1]. Lower entries of a diagonal block (row > col) are not written.
 // Persistent: workgroup b takes tables tabs[b], tabs[b + grid], ...; while it works on one table, the next one's
 // staged block (X row-major with row stride SW = Wp + 1, r~ in column Wp, zeros elsewhere; PF values per thread,
 // PF NT >= stage_cap) and entry words (ecd[u] = (entry column - c0) << 16 | entry dim) are loading into registers, and
@@ -903,12 +733,6 @@ struct bls_qr {
     DevArray<int32_t> d_top, d_table_ptr, d_entry_retained, d_entry_col, d_entry_table, d_retained_ptr,
         d_retained_entry, d_r_dims;
     DevArray<int64_t> d_red_off, d_boff_r, d_doff;
-    // work plans of reduced_normal (per layout: 0 sparse, 1 dense), built at the first call
-    bool rn_built[2] = {false, false};
-    DevArray<RNItem> d_rn_items[2];
-    DevArray<RNMerge> d_rn_merges[2];
-    int64_t rn_nitems[2] = {0, 0}, rn_nmerges[2] = {0, 0};
-    DevArray<double> d_rn_partial;
     // the Gram formulation: per-table records in launch order (with offsets into the contribution buffer), and per
     // layout the S blocks with their contribution lists (ascending table order)
     bool rg_built[2] = {false, false};
@@ -1239,7 +1063,6 @@ extern "C" int bls_qr_set_sparse(bls_qr *h, int64_t nnz, const int64_t *S_colptr
     h->h_brv.assign(brv.begin(), brv.end());
     h->h_bro = bro;
     h->h_ch = ch;
-    h->rn_built[0] = false;
     h->rg_built[0] = false;
     h->has_sparse = true;
     return BLS_OK;
@@ -1378,8 +1201,8 @@ int build_gram_plan(bls_qr *h, int dense) {
     return BLS_OK;
 }
 
-template <class T> int reduced_normal_gram(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
-                                           hipStream_t s) {
+template <class T> int reduced_normal(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
+                                      hipStream_t s) {
     int rc = reduced_ready(h);
     if (rc) return rc;
     if (!dense && !h->has_sparse) return fail(BLS_ERR_STATE, "no sparse layout (bls_qr_set_sparse)");
@@ -1425,78 +1248,6 @@ template <class T> int reduced_normal_gram(bls_qr *h, int dense, T *S, double *g
     return BLS_OK;
 }
 
-template <class T> int reduced_normal(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
-                                      hipStream_t s) {
-    return reduced_normal_gram<T>(h, dense, S, g, F, Fr, s);
-}
-
-template <class T> int reduced_normal_by_column(bls_qr *h, int dense, T *S, double *g, const T *F, const double *Fr,
-                                                hipStream_t s) {
-    int rc = reduced_ready(h);
-    if (rc) return rc;
-    if (!dense && !h->has_sparse) return fail(BLS_ERR_STATE, "no sparse layout (bls_qr_set_sparse)");
-    if (h->nrb && (!S || !g || !F || !Fr)) return fail(BLS_ERR_ARG, "reduced_normal: a required array is NULL");
-    NormalArgs na{};
-    na.dense = dense;
-    na.nr = h->nr;
-    na.S_colptr = h->d_S_colptr.ptr;
-    na.block_colptr = h->d_block_colptr.ptr;
-    na.block_rowval = h->d_block_rowval.ptr;
-    na.block_row_offsets = h->d_block_row_offsets.ptr;
-    na.col_heights = h->d_col_heights.ptr;
-    const int m = dense ? 1 : 0;
-    if (!h->rn_built[m]) {
-        // parts of about RN_TABLES tables, as many as the partial buffer (at most RN_PARTIAL doubles over all
-        // split columns) allows
-        constexpr int64_t RN_TABLES = 48, RN_PARTIAL = (int64_t)1 << 28;
-        std::vector<int64_t> csz(h->nrb);
-        int64_t want = 0;
-        for (int64_t r = 0; r < h->nrb; r++) {
-            const int dr = h->h_r_dims[r];
-            const int64_t ro = h->h_boff_r[r];
-            csz[r] = (dense ? (int64_t)dr * h->nr : h->h_S_colptr[ro + dr] - h->h_S_colptr[ro]) + dr;
-            const int64_t nt = h->h_retained_ptr[r + 1] - h->h_retained_ptr[r];
-            const int64_t p = std::max<int64_t>(1, (nt + RN_TABLES - 1) / RN_TABLES);
-            if (p > 1) want += p * csz[r];
-        }
-        const double shrink = want > RN_PARTIAL ? (double)RN_PARTIAL / (double)want : 1.0;
-        std::vector<RNItem> items;
-        std::vector<RNMerge> merges;
-        int64_t pb = 0;
-        for (int64_t r = 0; r < h->nrb; r++) {
-            const int64_t k0 = h->h_retained_ptr[r], nt = h->h_retained_ptr[r + 1] - k0;
-            int64_t p = std::max<int64_t>(1, (nt + RN_TABLES - 1) / RN_TABLES);
-            if (p > 1) p = std::max<int64_t>(1, (int64_t)(p * shrink));
-            if (p > 1) merges.push_back(RNMerge{(int32_t)r, (int32_t)p, pb});
-            for (int64_t q = 0; q < p; q++) {
-                RNItem it{};
-                it.r2 = (int32_t)r;
-                it.k_lo = (int32_t)(k0 + q * nt / p);
-                it.k_hi = (int32_t)(k0 + (q + 1) * nt / p);
-                it.nparts = (int32_t)p;
-                it.pbase = p > 1 ? pb + q * csz[r] : 0;
-                items.push_back(it);
-            }
-            if (p > 1) pb += p * csz[r];
-        }
-        // the longest items first
-        std::stable_sort(items.begin(), items.end(),
-                         [](const RNItem &x, const RNItem &y) { return x.k_hi - x.k_lo > y.k_hi - y.k_lo; });
-        if ((rc = h->d_rn_items[m].upload(items)) || (rc = h->d_rn_merges[m].upload(merges))) return rc;
-        if (pb > (int64_t)h->d_rn_partial.n && (rc = h->d_rn_partial.alloc(pb))) return rc;
-        h->rn_nitems[m] = (int64_t)items.size();
-        h->rn_nmerges[m] = (int64_t)merges.size();
-        h->rn_built[m] = true;
-    }
-    if (h->rn_nitems[m])
-        k_red_normal<T><<<(unsigned)h->rn_nitems[m], RED_THREADS, 0, s>>>(h->red(), na, h->d_rn_items[m].ptr, F, Fr, S,
-                                                                         g, h->d_rn_partial.ptr);
-    if (h->rn_nmerges[m])
-        k_red_normal_merge<T><<<(unsigned)h->rn_nmerges[m], RED_THREADS, 0, s>>>(h->red(), na, h->d_rn_merges[m].ptr,
-                                                                                h->d_rn_partial.ptr, S, g);
-    BLS_HIP(hipGetLastError());
-    return BLS_OK;
-}
 }  // namespace
 
 extern "C" int bls_reduced_normal_sparse_f64(bls_qr *h, double *S_nzval, double *g_S, const double *factored,
