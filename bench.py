@@ -331,6 +331,38 @@ def qr_leg(torch, dev, b, scale, reps=10):
                        "converged": bool(ok), "time_to_1e-8_ms": qms + nm + pm if ok else None}
         log(f"QR leg, {name} tables: reduced_normal_sparse! + lambda I {nm:.3f} ms, PCG {pm:.3f} ms ({int(iters.max())} "
             f"iterations), time to 1e-8 from the Jacobian {qms + nm + pm:.3f} ms")
+    # products with the reduced Jacobian on the Float64 tables: each reads the retained columns of every table's
+    # reduced rows once (8 B per value) plus its vectors
+    assert dev.reduce_qr_(ws, h, raw64, r, sc, lam, check=False)
+    nred = int(jst.reduced_nrows())
+    ncols = np.bincount(np.asarray(jst.entry_table) - 1, weights=np.asarray(st.r_dims)[np.asarray(jst.entry_retained) - 1],
+                        minlength=jst.ntables)
+    xbytes = float(np.sum((jst.table_nrows - jst.table_top) * ncols) * 8)
+    rng = np.random.default_rng(5)
+    vv = T(rng.standard_normal(st.nr))
+    ww = T(rng.standard_normal(nred))
+    yv = torch.empty(nred, dtype=torch.float64, device="cuda")
+    gv = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    Dv = torch.empty(int((np.asarray(st.r_dims) ** 2).sum()), dtype=torch.float64, device="cuda")
+    prods = {}
+    for pname, fn, vec_bytes in (("reduced_mul", lambda: dev.reduced_mul_(yv, h, ws.factored, vv), 8 * nred),
+                                 ("reduced_tmul", lambda: dev.reduced_tmul_(gv, h, ws.factored, ww), 8 * nred),
+                                 ("reduced_diagonal_blocks", lambda: dev.reduced_diagonal_blocks_(Dv, h, ws.factored),
+                                  8 * Dv.numel())):
+        fn()
+        times = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        pms = float(np.median(times))
+        gb = (xbytes + vec_bytes) / (pms * 1e-3) / 1e9
+        prods[pname] = {"ms": pms, "achieved_GBs": gb, "frac": gb / HBM_PEAK_GBS}
+        log(f"QR leg, {pname}: {pms:.3f} ms, {gb:.0f} GB/s of {(xbytes + vec_bytes) / 1e9:.2f} GB")
+    solve["products_f64"] = prods
     return {"landmark_scale": scale, "f32": f32, "solve": solve, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
             "cameras": int(st.nrblocks), "raw_GB": jst.raw_length() * 8 / 1e9, "ms": ms,
             "landmarks_per_s": st.neblocks / (ms * 1e-3),

@@ -295,11 +295,6 @@ __global__ __launch_bounds__(256) void k_red_residual(RedArgs a, const double *F
     for (int i = lane; i < n; i += 64) rt[a.red_off[t] + i] = Fr[a.res_off[t] + top + i];
 }
 
-// One workgroup per retained block r: `npair` quantities, each a sum over the tables coupling r. Thread t takes
-// quantity t % npair and every nch-th coupling from t / npair (nch = 256 / npair); the nch partial sums are then
-// added in order, so the result is deterministic.
-constexpr int RED_THREADS = 256;
-
 // g = J~' w (reduced_tmul!, src/qr_reduced.jl:195-223) in two passes. k_red_tmul_part: one wavefront per table,
 // lanes over the columns of its entries, writes the per-entry partials p[eoff[u] + j] = X_t(:, entry_col[u] + j)' w_t
 // (eoff: prefix sums of the entry dims in entry order). k_red_gsum: one wavefront per retained block r adds the
@@ -374,37 +369,140 @@ __global__ __launch_bounds__(256) void k_red_gsum(RedArgs a, const int64_t *__re
     }
 }
 
-// diagonal blocks of J~'J~ (reduced_diagonal_blocks!, src/qr_reduced.jl:484-512), full d x d column-major at doff
+// diagonal blocks of J~'J~ (reduced_diagonal_blocks!, src/qr_reduced.jl:484-512) in two passes. k_red_diag_part: one
+// wavefront per table stages each entry's d columns of the reduced rows in LDS, 64 rows at a time (row stride 65, so
+// the lanes of different columns hit different banks), and lane p forms upper pair p = b (b + 1) / 2 + a (a <= b)
+// of X_u' X_u into dp[dpo[u] + p] (dpo: prefix sums of d (d + 1) / 2 in entry order). k_red_dsum: one workgroup per
+// retained block r adds the partials of the entries coupling r in retained_entry order -- thread (p, run) takes the
+// run-th of 256 / npair contiguous runs of that list, and the runs are added in order -- and writes the full block,
+// both triangles, column-major at doff[r].
+__device__ inline void wave_sync() {   // LDS written by one lane of the wavefront, read by another
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class T>
-__global__ __launch_bounds__(RED_THREADS) void k_red_diag(RedArgs a, const T *F, T *D) {
-    __shared__ double part[RED_THREADS];
-    const int r = blockIdx.x;
-    const int dr = a.r_dims[r];
-    const int npair = dr * (dr + 1) / 2;
-    const int nch = RED_THREADS / npair;
-    const int pr = threadIdx.x % npair, ch = threadIdx.x / npair;
-    int col = 0;
-    while ((col + 1) * (col + 2) / 2 <= pr) col++;
-    const int row = pr - col * (col + 1) / 2;     // row <= col
-    double acc = 0.0;
-    if (ch < nch) {
-        for (int k = a.retained_ptr[r] + ch; k < a.retained_ptr[r + 1]; k += nch) {
-            const int u = a.retained_entry[k];
-            const int t = a.entry_table[u];
-            const int M = a.nrows[t], top = a.top[t], n = M - top;
-            const T *cb = F + a.raw_off[t] + (int64_t)a.entry_col[u] * M + top;
-            const T *x = cb + (int64_t)row * M, *y = cb + (int64_t)col * M;
-            for (int i = 0; i < n; i++) acc += (double)x[i] * (double)y[i];
+__global__ __launch_bounds__(256) void k_red_diag_part(RedArgs a, const int64_t *__restrict__ dpo,
+                                                       const T *__restrict__ F, double *__restrict__ dp) {
+    __shared__ double xs[4][16 * 65];
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= a.ntables) return;   // wavefront-level synchronisation only
+    double *xw = xs[threadIdx.x >> 6];
+    const int M = a.nrows[t], top = a.top[t], n = M - top;
+    const T *Xt = F + a.raw_off[t] + top;
+    const int ub = a.table_ptr[t], ue = a.table_ptr[t + 1];
+    // pair p -> (a, b) of one entry's d x d block
+    auto pairs = [&](int d, int *pa, int *pb) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int p = lane + 64 * k;
+            int b = 0;
+            while (b + 1 < d && (b + 1) * (b + 2) / 2 <= p) b++;
+            pb[k] = b;
+            pa[k] = min(p - b * (b + 1) / 2, b);
         }
+    };
+    if (n <= 64) {   // one row chunk: the next entry's columns load into registers while this one's dots run
+        double nx[16];
+        int d = 0, c0 = 0;
+        if (ub < ue) {
+            d = a.r_dims[a.entry_retained[ub]];
+            c0 = a.entry_col[ub];
+#pragma unroll
+            for (int j = 0; j < 16; j++) nx[j] = (j < d && lane < n) ? (double)Xt[(int64_t)(c0 + j) * M + lane] : 0.0;
+        }
+        for (int u = ub; u < ue; u++) {
+            const int dc = d;
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (j < dc) xw[j * 65 + lane] = nx[j];
+            wave_sync();
+            if (u + 1 < ue) {
+                d = a.r_dims[a.entry_retained[u + 1]];
+                c0 = a.entry_col[u + 1];
+#pragma unroll
+                for (int j = 0; j < 16; j++) nx[j] = (j < d && lane < n) ? (double)Xt[(int64_t)(c0 + j) * M + lane] : 0.0;
+            }
+            const int np = dc * (dc + 1) / 2;   // <= 136: three pairs per lane at most
+            int pa[3], pb[3];
+            pairs(dc, pa, pb);
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (lane + 64 * k < np) {
+                    const double *x = xw + pa[k] * 65, *y = xw + pb[k] * 65;
+                    double s = 0.0;
+#pragma unroll 8
+                    for (int q = 0; q < n; q++) s += x[q] * y[q];
+                    dp[dpo[u] + lane + 64 * k] = s;
+                }
+            wave_sync();
+        }
+        return;
+    }
+    for (int u = ub; u < ue; u++) {
+        const int d = a.r_dims[a.entry_retained[u]], c0 = a.entry_col[u];
+        const int np = d * (d + 1) / 2;   // <= 136: three pairs per lane at most
+        int pa[3], pb[3];
+        pairs(d, pa, pb);
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int i0 = 0; i0 < n; i0 += 64) {
+            const int i = i0 + lane;
+            for (int j = 0; j < d; j++) xw[j * 65 + lane] = i < n ? (double)Xt[(int64_t)(c0 + j) * M + i] : 0.0;
+            wave_sync();
+            const int m = min(64, n - i0);
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (lane + 64 * k < np) {
+                    const double *x = xw + pa[k] * 65, *y = xw + pb[k] * 65;
+                    double s = 0.0;
+#pragma unroll 8
+                    for (int q = 0; q < m; q++) s += x[q] * y[q];
+                    acc[k] += s;
+                }
+            wave_sync();
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (lane + 64 * k < np) dp[dpo[u] + lane + 64 * k] = acc[k];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_red_dsum(RedArgs a, const int64_t *__restrict__ dpo, const double *__restrict__ dp,
+                                                  T *__restrict__ D) {
+    __shared__ double part[256];
+    const int r = blockIdx.x;
+    const int d = a.r_dims[r], np = d * (d + 1) / 2;
+    const int nrun = 256 / np;
+    const int p = threadIdx.x % np, run = threadIdx.x / np;
+    double acc = 0.0;
+    if (run < nrun) {
+        const int k0 = a.retained_ptr[r], L = a.retained_ptr[r + 1] - k0;
+        const int len = (L + nrun - 1) / nrun;
+        const int kb = k0 + min(L, run * len), ke = k0 + min(L, (run + 1) * len);
+        int k = kb;
+        for (; k + 8 <= ke; k += 8) {
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = dp[dpo[a.retained_entry[k + q]] + p];
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc += v[q];
+        }
+        for (; k < ke; k++) acc += dp[dpo[a.retained_entry[k]] + p];
     }
     part[threadIdx.x] = acc;
     __syncthreads();
-    if (threadIdx.x < npair) {
-        double s = 0.0;
-        for (int c = 0; c < nch; c++) s += part[c * npair + threadIdx.x];
+    if ((int)threadIdx.x < np) {
+        double sum = 0.0;
+        for (int q = 0; q < nrun; q++) sum += part[q * np + threadIdx.x];
+        int b = 0;
+        while ((b + 1) * (b + 2) / 2 <= (int)threadIdx.x) b++;
+        const int aa = threadIdx.x - b * (b + 1) / 2;
         T *Db = D + a.doff[r];
-        Db[col * dr + row] = (T)s;
-        Db[row * dr + col] = (T)s;
+        Db[b * d + aa] = (T)sum;
+        Db[aa * d + b] = (T)sum;
     }
 }
 
@@ -729,7 +827,7 @@ struct bls_qr {
     Timing timing;
     // reduced-Jacobian structure (bls_qr_set_reduced)
     bool has_reduced = false;
-    int64_t red_len = 0, dlen = 0;
+    int64_t red_len = 0, dlen = 0, dp_len = 0;
     DevArray<int32_t> d_top, d_table_ptr, d_entry_retained, d_entry_col, d_entry_table, d_retained_ptr,
         d_retained_entry, d_r_dims;
     DevArray<int64_t> d_red_off, d_boff_r, d_doff;
@@ -744,6 +842,8 @@ struct bls_qr {
     DevArray<int32_t> d_rg_ecd;                 // per entry: (column - first column of its table) << 16 | dim
     DevArray<int64_t> d_eoff;                   // per-entry offsets of the partials of J~' w (prefix of entry dims)
     DevArray<double> d_gp;                      // the partials
+    DevArray<int64_t> d_dpo;                    // per-entry offsets of the diagonal-block partials (d (d + 1) / 2 each)
+    DevArray<double> d_dp;                      // the diagonal-block partials (allocated at the first call)
     int64_t rg_cls_off[5] = {0, 0, 0, 0, 0};
     int rg_cap2 = 0;                            // staged doubles of the third class (from the device's LDS)
     int rg_ncu = 0, rg_lds_cu = 0;              // compute units and LDS bytes per compute unit
@@ -946,6 +1046,10 @@ extern "C" int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int
     for (int64_t u = 0; u < nentries; u++) eoff[u + 1] = eoff[u] + rd[er[u]];
     if ((rc = h->d_eoff.upload(eoff))) return rc;
     if ((int64_t)h->d_gp.n < eoff[nentries] && (rc = h->d_gp.alloc((size_t)std::max<int64_t>(1, eoff[nentries])))) return rc;
+    std::vector<int64_t> dpo(nentries + 1, 0);
+    for (int64_t u = 0; u < nentries; u++) dpo[u + 1] = dpo[u] + (int64_t)rd[er[u]] * (rd[er[u]] + 1) / 2;
+    if ((rc = h->d_dpo.upload(dpo))) return rc;
+    h->dp_len = dpo[nentries];
     h->red_len = ro[nt];
     h->dlen = dl;
     h->h_retained_ptr.assign(rp.begin(), rp.end());
@@ -990,7 +1094,10 @@ template <class T> int reduced_diag(bls_qr *h, T *D, const T *F, hipStream_t s) 
     int rc = reduced_ready(h);
     if (rc) return rc;
     if (!D || !F) return fail(BLS_ERR_ARG, "reduced_diagonal_blocks: a required array is NULL");
-    if (h->nrb) k_red_diag<T><<<(unsigned)h->nrb, RED_THREADS, 0, s>>>(h->red(), F, D);
+    if ((int64_t)h->d_dp.n < h->dp_len && (rc = h->d_dp.alloc((size_t)std::max<int64_t>(1, h->dp_len)))) return rc;
+    if (h->ntables)
+        k_red_diag_part<T><<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), h->d_dpo.ptr, F, h->d_dp.ptr);
+    if (h->nrb) k_red_dsum<T><<<(unsigned)h->nrb, 256, 0, s>>>(h->red(), h->d_dpo.ptr, h->d_dp.ptr, D);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
