@@ -276,6 +276,7 @@ __global__ void k_finalize_flag(const unsigned long long *flag, int64_t *out) {
 constexpr int P2_STAGE = 2048;                 // doubles per staging buffer (16 KB)
 constexpr int P2_Q = P2_STAGE / P2_THREADS;    // staged doubles per lane per batch (mixed dimensions)
 constexpr int P2_G = 4;                        // landmarks whose C_c1 columns a lane loads together
+constexpr int P2_G4 = 2;                       // the same at four workgroups per CU (d_e 3, d_r <= 6)
 constexpr int P2_WORDS = P2_STAGE / 64;
 constexpr int P2_PART = 384;                   // default: landmarks per workgroup before an item is split
 
@@ -539,18 +540,27 @@ __device__ inline int nth_set_bit(unsigned long long x, int r) {
 
 __device__ inline unsigned long long bits_below(int p) { return p >= 64 ? ~0ull : ((1ull << p) - 1ull); }
 
-// at least 3 waves per SIMD (3 workgroups per CU) where the row registers allow it
+// Staging capacity: one landmark dimension needs at most P2_MAXE records of (DE + 1) DRM doubles, so d_e 3 with
+// d_r up to 6 stages a full batch in 1536 doubles and four workgroups fit a CU's LDS.
+template <int DE, int DRM>
+constexpr int p2_stage_cap() {
+    return DE > 0 && P2_MAXE * (DE + 1) * DRM < P2_STAGE ? P2_MAXE * (DE + 1) * DRM : P2_STAGE;
+}
+
+// 4 waves per SIMD (4 workgroups per CU) for d_e 3, d_r up to 6; at least 3 elsewhere where the row registers allow
 template <int DE, int DRM, class P>
-__global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args<P> a) {
+__global__ __launch_bounds__(P2_THREADS, (DE > 0 && DRM <= 6) ? 4 : (DRM <= 9 ? 3 : 2)) void k_phase2(P2Args<P> a) {
     using TC = typename P::TC;
     using TS = typename P::TS;
     constexpr int MERGE = P2_THREADS * (DRM + 1);
     // the update reads DE*DRM doubles of C_c2 (and a column at a row up to MAXDR) whatever d_r2 is: pad each
     // staging buffer by the largest such read
-    constexpr int STAGE = P2_STAGE + 8 * (DRM > MAXDR ? DRM : MAXDR);
+    constexpr int SCAP = p2_stage_cap<DE, DRM>();
+    constexpr int STAGE = SCAP + 8 * (DRM > MAXDR ? DRM : MAXDR);
     constexpr int LDSN = 2 * STAGE > MERGE ? 2 * STAGE : MERGE;
     constexpr int DEC = DE ? DE : 8;   // register width of a C_c1 column
     constexpr int QN = P2Width<DE, DRM>::value;
+    constexpr int GL = (DE > 0 && DRM <= 6) ? P2_G4 : P2_G;   // landmarks per C_c1 load group
     __shared__ double lds[LDSN];
     __shared__ P2Meta meta[3];
     __shared__ int s_slot_dr[64];
@@ -576,7 +586,7 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args<
     const bool diag = it.has_diag && my_slot == it.nslots - 1;
     const int dr2 = it.dr2;
     const int size = p2_record<DE>(DE, dr2);   // staged record (one landmark dimension)
-    const int nbmax = DE > 0 ? (P2_STAGE / size < P2_MAXE ? P2_STAGE / size : P2_MAXE) : P2_MAXE;
+    const int nbmax = DE > 0 ? (SCAP / size < P2_MAXE ? SCAP / size : P2_MAXE) : P2_MAXE;
     double acc[DRM];
     double gacc = 0.0;
     int64_t rowaddr = 0;
@@ -629,10 +639,10 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args<
             }
         }
         while (bits) {
-            int jj[P2_G];
-            const TC *cp[P2_G];
+            int jj[GL];
+            const TC *cp[GL];
 #pragma unroll
-            for (int gi = 0; gi < P2_G; gi++) {
+            for (int gi = 0; gi < GL; gi++) {
                 const bool hv = bits != 0;
                 const int j = hv ? __builtin_ctzll(bits) : 0;
                 bits &= bits - 1ull;
@@ -652,9 +662,9 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args<
                 jj[gi] = hv ? j : -1;
                 cp[gi] = hv ? a.C + m.src_c1[j] + pre + my_row * de : a.C;
             }
-            double cv[P2_G][DEC];
+            double cv[GL][DEC];
 #pragma unroll
-            for (int gi = 0; gi < P2_G; gi++) {
+            for (int gi = 0; gi < GL; gi++) {
                 if constexpr (DE > 0) {
 #pragma unroll
                     for (int k = 0; k < DE; k++) cv[gi][k] = (double)cp[gi][k];
@@ -665,7 +675,7 @@ __global__ __launch_bounds__(P2_THREADS, DRM <= 9 ? 3 : 2) void k_phase2(P2Args<
                 }
             }
 #pragma unroll
-            for (int gi = 0; gi < P2_G; gi++) {
+            for (int gi = 0; gi < GL; gi++) {
                 const int j = jj[gi];
                 if (j < 0) continue;
                 if constexpr (DE > 0) {
