@@ -12,11 +12,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _jac_with_free_rows(blsamd, seed=5, nland=60, ncam=8):
-    """A description with observations (camera, landmark), camera-camera rows and camera priors (free groups)."""
+def _jac_with_free_rows(blsamd, seed=5, nland=60, ncam=8, tall_pairs=()):
+    """A description with observations (camera, landmark), camera-camera rows and camera priors (free groups).
+    tall_pairs: (a, b, n) adds n relative-pose row blocks (3 rows each) between cameras a and b, one free table of
+    3 n rows (the cameras then have 9 parameters each)."""
     rng = np.random.default_rng(seed)
     S = blsamd.structure
     cam_dims = np.where(rng.random(ncam) < 0.5, 6, 9)
+    if tall_pairs:
+        cam_dims[:] = 9
     blocksizes = np.concatenate([cam_dims, np.full(nland, 3)])
     eliminated = np.concatenate([np.zeros(ncam, bool), np.ones(nland, bool)])
     rowblocks, rowsz = [], []
@@ -28,6 +32,9 @@ def _jac_with_free_rows(blsamd, seed=5, nland=60, ncam=8):
     for _ in range(5):
         a, b = rng.choice(ncam, size=2, replace=False)
         rowblocks.append([int(a) + 1, int(b) + 1]); rowsz.append(3)       # relative-pose rows
+    for a, b, n in tall_pairs:
+        for _ in range(n):
+            rowblocks.append([a + 1, b + 1]); rowsz.append(3)
     jst = S.SchurJacobianStructure.from_description(blocksizes, rowblocks, rowsz, eliminated)
     data = rng.standard_normal(jst.raw_length())
     # damping rows of eliminated tables are zero in the raw storage
@@ -123,13 +130,19 @@ def test_qr_bitwise_reproducible(blsamd):
     assert np.array_equal(Fa, Fb) and np.array_equal(Fra, Frb)
 
 
-@pytest.mark.parametrize("free", [False, True])
+@pytest.mark.parametrize("free", ["none", "free", "tall"])
 def test_reduced_products_match_oracle(blsamd, free):
     """reduced_mul!, reduced_tmul!, reduced_residual!, reduced_diagonal_blocks! (src/qr_reduced.jl:161-243,
-    :484-512) on the device, from the device-factored tables, against the oracle on the same factored tables."""
+    :484-512) on the device, from the device-factored tables, against the oracle on the same factored tables.
+    "tall" adds free tables of 210, 300 and 1200 rows (more than one 64-row chunk of the diagonal-block pass)."""
     import torch
     dev = blsamd.device
-    p = _jac_with_free_rows(blsamd) if free else _small_jacobian(blsamd, 7, nland=90, ncam=9, kmax=6)
+    if free == "tall":
+        p = _jac_with_free_rows(blsamd, seed=9, tall_pairs=((0, 1, 70), (2, 3, 100), (4, 5, 400)))
+    elif free == "free":
+        p = _jac_with_free_rows(blsamd)
+    else:
+        p = _small_jacobian(blsamd, 7, nland=90, ncam=9, kmax=6)
     ok, fb, Fg, Frg = run_gpu(blsamd, p, 0.2)
     assert ok
     jst, st = p.jst, p.structure
@@ -212,6 +225,24 @@ def test_qr_backsubstitute_matches_oracle(blsamd, table_type):
     assert np.abs(x_e.cpu().numpy() - xo).max() <= TOL * np.abs(xo).max()
 
 
+def _gram_classes(jst):
+    """Launch classes of reduced_normal's Gram pass that the tables of jst fall in: staged size (reduced rows rounded
+    up to 4) x (entry columns rounded up to 16, plus one) against 4096, 8192 and 19456 doubles."""
+    tp = np.asarray(jst.table_ptr) - 1
+    ec = np.asarray(jst.entry_col) - 1
+    rd = np.asarray(jst.schur.r_dims)[np.asarray(jst.entry_retained) - 1]
+    out = set()
+    for t in range(jst.ntables):
+        u0, u1 = tp[t], tp[t + 1]
+        if u1 == u0:
+            continue
+        W = ec[u1 - 1] + rd[u1 - 1] - ec[u0]
+        n = jst.table_nrows[t] - jst.table_top[t]
+        st = ((n + 3) // 4 * 4) * ((W + 15) // 16 * 16 + 1)
+        out.add(next((k for k, cap in enumerate((4096, 8192, 19456)) if st <= cap), 3))
+    return out
+
+
 @pytest.mark.parametrize("table_type", ["f64", "f32"])
 def test_reduced_normal_matches_oracle(blsamd, table_type):
     """reduced_normal_dense! / reduced_normal_sparse! (src/qr_reduced.jl:373-480) on the device from the
@@ -223,8 +254,12 @@ def test_reduced_normal_matches_oracle(blsamd, table_type):
     from test_oracle_qr import _qr_pattern
     dev = blsamd.device
     # the third problem puts about 250 tables on every camera: its columns are split into parts and merged
+    # the fourth problem adds camera-pair tables of 210, 300 and 1200 rows: with the landmark tables they fill all
+    # four launch classes of the Gram pass (staged block up to 4096, 8192, 19456 doubles, and global memory)
+    tall = _jac_with_free_rows(blsamd, seed=9, tall_pairs=((0, 1, 70), (2, 3, 100), (4, 5, 400)))
+    assert _gram_classes(tall.jst) == {0, 1, 2, 3}
     for p in (_small_jacobian(blsamd, 14, nland=80, ncam=9, kmax=6), _jac_with_free_rows(blsamd),
-              _small_jacobian(blsamd, 15, nland=500, ncam=6, kmax=4)):
+              _small_jacobian(blsamd, 15, nland=500, ncam=6, kmax=4), tall):
         jst, st = p.jst, p.structure
         dt = torch.float64 if table_type == "f64" else torch.float32
         npdt = np.float64 if table_type == "f64" else np.float32
@@ -240,7 +275,11 @@ def test_reduced_normal_matches_oracle(blsamd, table_type):
         gs = torch.empty(st.nr, dtype=torch.float64, device="cuda")
         dev.reduced_normal_dense_(Sd, gd, h, ws)
         dev.reduced_normal_sparse_(Ss, gs, h, ws)
+        Ss2 = torch.full_like(Ss, float("nan"))
+        gs2 = torch.empty_like(gs)
+        dev.reduced_normal_sparse_(Ss2, gs2, h, ws)   # a second call: bitwise the same (fixed-order sums)
         torch.cuda.synchronize()
+        assert torch.equal(Ss, Ss2) and torch.equal(gs, gs2)
         F, Fr = ws.factored.cpu().numpy(), ws.fres.cpu().numpy()
         sargs = oracle.SchurArgs(st, blsamd.structure.SchurReductionWorkspaceIndex(st, 1), pat)
         jargs = oracle.JacobianArgs(jst)
