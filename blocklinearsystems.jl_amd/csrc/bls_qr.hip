@@ -564,8 +564,10 @@ __global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs,
                                                  const double *__restrict__ Fr, double *__restrict__ G,
                                                  double *__restrict__ gp) {
     extern __shared__ double XL[];
-    __shared__ int16_t s_ent[RG_MAXW];
-    __shared__ int s_cst[RG_MAXE], s_dim[RG_MAXE], s_P[RG_MAXE + 1], s_Q[RG_MAXE];
+    // per staged column c of the current table (-1 where no entry holds it): s_colA[c] = P[e] + j, its column in
+    // the entries' packed order (entry e, column j of it), and s_colB[c] = (Q[e] + j) << 5 | d_e, so that element
+    // (a, b) of X'X goes to pair-block offset (s_colB[b] >> 5) + (s_colB[b] & 31) s_colA[a]
+    __shared__ int s_colA[RG_MAXW], s_colB[RG_MAXW];
     double pf[PF];
     int my_ecd = 0;
     auto prefetch = [&](const RGTab g) {
@@ -620,8 +622,12 @@ __global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs,
                 }
             }
         }
-        for (int i = threadIdx.x; i < Wp; i += NT) s_ent[i] = -1;
+        for (int i = threadIdx.x; i < Wp; i += NT) {
+            s_colA[i] = -1;
+            s_colB[i] = -1;
+        }
         const int e_cst = my_ecd >> 16, e_dim = my_ecd & 0xffff;   // entry threadIdx.x (< ne)
+        int e_P = 0, e_Q = 0;
         if (threadIdx.x < 64) {   // prefix sums of the entry dims: P (columns) and Q (pair-block offsets)
             const int e = threadIdx.x;
             const int d = e < ne ? e_dim : 0;
@@ -636,61 +642,66 @@ __global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs,
                 const int v = __shfl_up(q, o);
                 if (e >= o) q += v;
             }
-            if (e < ne) {
-                s_cst[e] = e_cst;
-                s_dim[e] = d;
-                s_P[e] = p - d;
-                s_Q[e] = q - q0;
-            }
-            if (e == ne - 1) s_P[ne] = p;
+            e_P = p - d;
+            e_Q = q - q0;
         }
         __syncthreads();
         if ((int)threadIdx.x < ne)
-            for (int j = 0; j < e_dim; j++) s_ent[e_cst + j] = (int16_t)threadIdx.x;
+            for (int j = 0; j < e_dim; j++) {
+                s_colA[e_cst + j] = e_P + j;
+                s_colB[e_cst + j] = ((e_Q + j) << 5) | e_dim;
+            }
         __syncthreads();
         if (it + grid < cnt) {   // the next table's loads go out before this one's work
             cur = nxt;
             prefetch(cur);
             if (it + 2 * grid < cnt) nxt = rg_load(tabs + it + 2 * grid);
         }
-        // per-entry partials of J~' r~ (as k_red_tmul_part writes them), one thread per column
-        for (int c = threadIdx.x; c < W; c += NT) {
-            const int e = s_ent[c];
-            if (e < 0) continue;
-            double acc = 0.0;
-            if (fits) {
-#pragma unroll 8
-                for (int k = 0; k < n; k++) acc += XL[k * SW + c] * XL[k * SW + Wp];
-            } else {
-                const T *x = Xg + (int64_t)(c0 + c) * M;
-                for (int k = 0; k < n; k++) acc += (double)x[k] * rg[k];
-            }
-            gpt[s_P[e] + (c - s_cst[e])] = acc;
-        }
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
         // writes tile (I, J) of X'X: element (a, b), a <= b, to its pair block
         auto store = [&](const rg_v4d &acc, int I, int J) {
             const int b = 16 * J + li;
-            const int e2 = b < W ? s_ent[b] : -1;
-            if (e2 < 0) return;
-            const int col = b - s_cst[e2], d2 = s_dim[e2];
+            const int cb = b < W ? s_colB[b] : -1;
+            if (cb < 0) return;
+            const int baseB = cb >> 5, d2 = cb & 31;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int aa = 16 * I + lk + 4 * v;
-                const int e1 = aa <= b ? s_ent[aa] : -1;
-                if (e1 >= 0) Gt[s_Q[e2] + d2 * s_P[e1] + (aa - s_cst[e1]) * d2 + col] = acc[v];
+                const int pa = aa <= b ? s_colA[aa] : -1;
+                if (pa >= 0) Gt[baseB + d2 * pa] = acc[v];
             }
         };
-        // work items: column strip J of the upper triangle in pairs of tiles (I0, I0 + 1) that share B
+        // work items, dealt to the wavefronts in turn: column strip J of the upper triangle in pairs of tiles
+        // (I0, I0 + 1) that share B, then the per-entry partials of J~' r~ (as k_red_tmul_part writes them), 64
+        // columns per item, one lane per column -- last, so that they go to the wavefronts with one tile item fewer
+        const int ngp = (W + 63) >> 6;
         const int ntl = Wp >> 4;
-        const int nitems = (ntl / 2 + 1) * (ntl / 2) + ((ntl & 1) ? (ntl + 1) / 2 : 0);
+        const int ntitems = (ntl / 2 + 1) * (ntl / 2) + ((ntl & 1) ? (ntl + 1) / 2 : 0);
+        const int nitems = ntitems + ngp;
         for (int item = wave; item < nitems; item += NT / 64) {
+            if (item >= ntitems) {
+                const int c = 64 * (item - ntitems) + lane;
+                const int pc = c < W ? s_colA[c] : -1;
+                if (pc >= 0) {
+                    double acc = 0.0;
+                    if (fits) {
+#pragma unroll 8
+                        for (int k = 0; k < n; k++) acc += XL[k * SW + c] * XL[k * SW + Wp];
+                    } else {
+                        const T *x = Xg + (int64_t)(c0 + c) * M;
+                        for (int k = 0; k < n; k++) acc += (double)x[k] * rg[k];
+                    }
+                    gpt[pc] = acc;
+                }
+                continue;
+            }
+            const int titem = item;
             int J = 0, base = 0;
-            while (base + (J + 2) / 2 <= item) {
+            while (base + (J + 2) / 2 <= titem) {
                 base += (J + 2) / 2;
                 J++;
             }
-            const int I0 = 2 * (item - base);
+            const int I0 = 2 * (titem - base);
             const bool two = I0 + 1 <= J;
             rg_v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
             if (fits) {
