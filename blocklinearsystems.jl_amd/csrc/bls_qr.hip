@@ -763,10 +763,20 @@ __global__ __launch_bounds__(256) void k_red_gram_sum(int64_t nblk, const RGBloc
         const int64_t len = (L + nrun - 1) / nrun;
         const int64_t kb = cb + std::min<int64_t>(L, run * len), ke = cb + std::min<int64_t>(L, (run + 1) * len);
         int64_t c = kb;
+        // eight contributions per round; the next round's indices load while this round's values are in flight
+        int64_t idx[8];
+        if (c + 8 <= ke) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) idx[j] = contrib[c + j];
+        }
         for (; c + 8 <= ke; c += 8) {
             double v[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = G[contrib[c + j] + w];
+            for (int j = 0; j < 8; j++) v[j] = G[idx[j] + w];
+            if (c + 16 <= ke) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) idx[j] = contrib[c + 8 + j];
+            }
 #pragma unroll
             for (int j = 0; j < 8; j++) acc += v[j];
         }
