@@ -513,8 +513,10 @@ __global__ __launch_bounds__(256) void k_red_dsum(RedArgs a, const int64_t *__re
 // contributions of each block (r1, r2) of S in ascending table order -- the order in which the reference adds them to
 // each entry -- and writes the whole block (the lower triangle of a diagonal block, and a block without
 // contributions, as zeros); k_red_gsum adds the partials per retained block. Deterministic, no atomics.
-constexpr int RG_MAXE = 31;    // entries of one table
-constexpr int RG_MAXW = 512;   // columns spanned by the entries of one table
+constexpr int RG_MAXE = 64;    // entries of one table (one wavefront scans their dims)
+constexpr int RG_MAXW = 1024;  // columns spanned by the entries of one table
+constexpr int RG_STATIC_LDS = 8192;   // bytes of LDS reserved for the per-column words of the Gram pass
+static_assert(RG_MAXW * (sizeof(int16_t) + sizeof(int)) <= RG_STATIC_LDS, "per-column words fit the reserve");
 
 typedef double rg_v4d __attribute__((ext_vector_type(4)));
 
@@ -567,7 +569,8 @@ __global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs,
     // per staged column c of the current table (-1 where no entry holds it): s_colA[c] = P[e] + j, its column in
     // the entries' packed order (entry e, column j of it), and s_colB[c] = (Q[e] + j) << 5 | d_e, so that element
     // (a, b) of X'X goes to pair-block offset (s_colB[b] >> 5) + (s_colB[b] & 31) s_colA[a]
-    __shared__ int s_colA[RG_MAXW], s_colB[RG_MAXW];
+    __shared__ int16_t s_colA[RG_MAXW];
+    __shared__ int s_colB[RG_MAXW];
     double pf[PF];
     int my_ecd = 0;
     auto prefetch = [&](const RGTab g) {
@@ -632,13 +635,13 @@ __global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs,
             const int e = threadIdx.x;
             const int d = e < ne ? e_dim : 0;
             int p = d;
-            for (int o = 1; o < 32; o <<= 1) {
+            for (int o = 1; o < 64; o <<= 1) {
                 const int v = __shfl_up(p, o);
                 if (e >= o) p += v;
             }
             const int q0 = d * p;   // d_e P[e + 1]
             int q = q0;
-            for (int o = 1; o < 32; o <<= 1) {
+            for (int o = 1; o < 64; o <<= 1) {
                 const int v = __shfl_up(q, o);
                 if (e >= o) q += v;
             }
@@ -648,7 +651,7 @@ __global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs,
         __syncthreads();
         if ((int)threadIdx.x < ne)
             for (int j = 0; j < e_dim; j++) {
-                s_colA[e_cst + j] = e_P + j;
+                s_colA[e_cst + j] = (int16_t)(e_P + j);
                 s_colB[e_cst + j] = ((e_Q + j) << 5) | e_dim;
             }
         __syncthreads();
@@ -1211,11 +1214,11 @@ int build_gram_plan(bls_qr *h, int dense) {
         const int ne = tp[t + 1] - tp[t];
         if (ne > RG_MAXE)
             return fail(BLS_ERR_UNSUPPORTED, "table " + std::to_string(t + 1) + " holds " + std::to_string(ne) +
-                                                 " retained blocks; reduced_normal on the device takes up to 31");
+                                                 " retained blocks; reduced_normal on the device takes up to 64");
         if (ne && ec[tp[t] + ne - 1] + rd[er[tp[t] + ne - 1]] - ec[tp[t]] > RG_MAXW)
             return fail(BLS_ERR_UNSUPPORTED, "table " + std::to_string(t + 1) +
-                                                 ": its entries span more than 512 columns; reduced_normal on the device "
-                                                 "takes up to 512");
+                                                 ": its entries span more than 1024 columns; reduced_normal on the device "
+                                                 "takes up to 1024");
         for (int i = 1; i < ne; i++)   // pair blocks land in the upper triangle; the last entry ends the columns
             if (er[tp[t] + i] <= er[tp[t] + i - 1] || ec[tp[t] + i] <= ec[tp[t] + i - 1])
                 return fail(BLS_ERR_ARG, "table " + std::to_string(t + 1) +
@@ -1228,7 +1231,7 @@ int build_gram_plan(bls_qr *h, int dense) {
     int dev = 0, lds = 0;
     BLS_HIP(hipGetDevice(&dev));
     BLS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    h->rg_cap2 = (int)std::min<int64_t>(RG_CLASS_CAP2, std::max<int64_t>(0, (lds - 2048) / 8));
+    h->rg_cap2 = (int)std::min<int64_t>(RG_CLASS_CAP2, std::max<int64_t>(0, (lds - RG_STATIC_LDS) / 8));
     BLS_HIP(hipDeviceGetAttribute(&h->rg_ncu, hipDeviceAttributeMultiprocessorCount, dev));
     BLS_HIP(hipDeviceGetAttribute(&h->rg_lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     const int64_t caps[3] = {RG_CLASS_CAP0, RG_CLASS_CAP1, h->rg_cap2};
@@ -1347,7 +1350,8 @@ template <class T> int reduced_normal(bls_qr *h, int dense, T *S, double *g, con
             const size_t lds = (size_t)caps[c] * sizeof(double);
             const int nt = c == 2 ? 1024 : c == 1 ? 512 : 256;
             // persistent grid: as many workgroups as the compute units hold at once (LDS and 32 waves per unit)
-            const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(h->rg_lds_cu / (int64_t)(lds + 2048), 32 / (nt / 64)));
+            const int64_t per_cu =
+                std::max<int64_t>(1, std::min<int64_t>(h->rg_lds_cu / (int64_t)(lds + RG_STATIC_LDS), 32 / (nt / 64)));
             const unsigned grid = (unsigned)std::min<int64_t>(cnt, per_cu * std::max(1, h->rg_ncu));
             const int32_t *ecd = h->d_rg_ecd.ptr;
             double *Gp = h->d_rg_G.ptr, *gpp = h->d_gp.ptr;

@@ -238,7 +238,7 @@ int bls_qr_set_sparse(bls_qr *h, int64_t nnz, const int64_t *S_colptr, int64_t n
 /* reduced_normal_sparse!(S, g_S, ws, pattern) / reduced_normal_dense!(S, g_S, ws) (src/qr_reduced.jl:373-480):
  * every scalar column of S is zeroed, then its upper triangle is J~'J~ and g_S = J~'r~ (Float64), each S entry summed
  * over the tables in the reference's order (deterministic). Dense S is nr x nr column-major. A table may hold up to
- * 31 retained blocks spanning up to 512 columns (BLS_ERR_UNSUPPORTED otherwise), in strictly ascending order
+ * 64 retained blocks spanning up to 1024 columns (BLS_ERR_UNSUPPORTED otherwise), in strictly ascending order
  * (BLS_ERR_ARG otherwise, as the reference's structure builds them). The first call per layout builds a plan and
  * allocates a Float64 contribution buffer of sum over tables of sum over entry pairs of d1 d2 values. */
 int bls_reduced_normal_sparse_f64(bls_qr *h, double *S_nzval, double *g_S, const double *factored, const double *fres,

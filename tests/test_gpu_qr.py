@@ -12,10 +12,11 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _jac_with_free_rows(blsamd, seed=5, nland=60, ncam=8, tall_pairs=()):
+def _jac_with_free_rows(blsamd, seed=5, nland=60, ncam=8, tall_pairs=(), wide=0):
     """A description with observations (camera, landmark), camera-camera rows and camera priors (free groups).
     tall_pairs: (a, b, n) adds n relative-pose row blocks (3 rows each) between cameras a and b, one free table of
-    3 n rows (the cameras then have 9 parameters each)."""
+    3 n rows (the cameras then have 9 parameters each). wide: two row blocks of 3 rows over cameras 0 .. wide - 1,
+    one free table with that many retained blocks."""
     rng = np.random.default_rng(seed)
     S = blsamd.structure
     cam_dims = np.where(rng.random(ncam) < 0.5, 6, 9)
@@ -35,6 +36,8 @@ def _jac_with_free_rows(blsamd, seed=5, nland=60, ncam=8, tall_pairs=()):
     for a, b, n in tall_pairs:
         for _ in range(n):
             rowblocks.append([a + 1, b + 1]); rowsz.append(3)
+    for _ in range(2 if wide else 0):
+        rowblocks.append(list(range(1, wide + 1))); rowsz.append(3)
     jst = S.SchurJacobianStructure.from_description(blocksizes, rowblocks, rowsz, eliminated)
     data = rng.standard_normal(jst.raw_length())
     # damping rows of eliminated tables are zero in the raw storage
@@ -225,6 +228,24 @@ def test_qr_backsubstitute_matches_oracle(blsamd, table_type):
     assert np.abs(x_e.cpu().numpy() - xo).max() <= TOL * np.abs(xo).max()
 
 
+def test_reduced_normal_rejects_too_wide_tables(blsamd):
+    """A free table over 70 cameras is past reduced_normal's 64 retained blocks per table: the call raises a clear
+    error from the host-side plan, before any launch."""
+    import torch
+    from test_oracle_qr import _qr_pattern
+    dev = blsamd.device
+    p = _jac_with_free_rows(blsamd, seed=17, nland=40, ncam=72, wide=70)
+    h = dev.QRHandle(p.jst)
+    h.set_sparse(_qr_pattern(blsamd, p))
+    ws = dev.QRReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    assert dev.reduce_qr_(ws, h, T(p.data), T(p.r), T(p.scaling_e), 0.2)
+    S = torch.empty(_qr_pattern(blsamd, p).nnz, dtype=torch.float64, device="cuda")
+    g = torch.empty(p.structure.nr, dtype=torch.float64, device="cuda")
+    with pytest.raises(blsamd.device.BLSError, match="64"):
+        dev.reduced_normal_sparse_(S, g, h, ws)
+
+
 def _gram_classes(jst):
     """Launch classes of reduced_normal's Gram pass that the tables of jst fall in: staged size (reduced rows rounded
     up to 4) x (entry columns rounded up to 16, plus one) against 4096, 8192 and 19456 doubles."""
@@ -258,8 +279,10 @@ def test_reduced_normal_matches_oracle(blsamd, table_type):
     # four launch classes of the Gram pass (staged block up to 4096, 8192, 19456 doubles, and global memory)
     tall = _jac_with_free_rows(blsamd, seed=9, tall_pairs=((0, 1, 70), (2, 3, 100), (4, 5, 400)))
     assert _gram_classes(tall.jst) == {0, 1, 2, 3}
+    # the fifth has a free table over 40 cameras (40 retained blocks, about 300 columns)
+    wide = _jac_with_free_rows(blsamd, seed=13, nland=60, ncam=45, wide=40)
     for p in (_small_jacobian(blsamd, 14, nland=80, ncam=9, kmax=6), _jac_with_free_rows(blsamd),
-              _small_jacobian(blsamd, 15, nland=500, ncam=6, kmax=4), tall):
+              _small_jacobian(blsamd, 15, nland=500, ncam=6, kmax=4), tall, wide):
         jst, st = p.jst, p.structure
         dt = torch.float64 if table_type == "f64" else torch.float32
         npdt = np.float64 if table_type == "f64" else np.float32
