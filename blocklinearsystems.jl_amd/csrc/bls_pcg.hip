@@ -8,8 +8,9 @@
 // Layout: S arrives in the upper-block CSC of build_schur_pattern; at each solve it is gathered once (the copyto!
 // kernel) into a symmetric block-row layout (BSR: every block (i, j) of the full matrix, row-major, blocks of a
 // block row contiguous). An iteration is three kernels, each skipping converged problems:
-//   k_bsr_spmv_w  q = S p, one wavefront per block row (lanes over its blocks, a fixed-order wave reduction), and
-//                 the block row's partial p.q;
+//   k_bsr_spmv_u / k_bsr_spmv_v  q = S p, one wavefront per block row (lane (b, r) takes row r of every
+//                 (64 / d_i)-th block from the b-th: contiguous runs of the row's values; fixed-order sums), and the
+//                 block row's partial p.q (u: one block dim throughout, v: dims that differ);
 //   k_pcg_step1   per group of up to 256 blocks of one problem: alpha = rz / (sum of the problem's p.q partials,
 //                 in a fixed order), x += alpha p, r -= alpha q, z = M^-1 r per block, partials r.z and r.r;
 //   k_pcg_step2   beta = rz_new / rz from the same fixed-order sums, p = z + beta p; the problem's first group
@@ -69,51 +70,48 @@ __device__ inline double wave_sum64(double v) {
     return v;
 }
 
-// yout = S xin for the block rows of active problems (all rows when only_active = 0); D >= every block dimension.
-// One wavefront per block row: lane l takes blocks l, l + 64, ... of the row. With pq_part, the block row's
-// partial sum of p . yout (p = xin) is stored too.
+// Block dims that differ between block rows (d_i, d_j <= D): one wavefront per block row i; lane (b, r) =
+// (lane / d_i, lane % d_i) takes row r of every (64 / d_i)-th block from the b-th, reading that row's d_j contiguous
+// values, so a wavefront streams contiguous runs of the row's blocks; the per-lane sums are added in a fixed order.
 template <int D>
-__global__ __launch_bounds__(256) void k_bsr_spmv_w(PcgArgs a, const double *xin, double *yout, double *pq_part,
+__global__ __launch_bounds__(256) void k_bsr_spmv_v(PcgArgs a, const double *xin, double *yout, double *pq_part,
                                                      int only_active) {
-    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ double red[4][64];
+    const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + w;
     if (i >= a.nrb) return;
     if (only_active && !a.st[a.prob_of_block[i]].active) return;
     const int di = a.dims[i];
-    double y[D];
+    const int nb = 64 / di;   // blocks per pass
+    const int bsub = lane / di, r = lane - bsub * di;
+    const int k0 = a.bsr_rowptr[i], k1 = a.bsr_rowptr[i + 1];
+    double acc = 0.0;
+    if (bsub < nb) {
+        for (int k = k0 + bsub; k < k1; k += nb) {
+            const int j = a.bsr_col[k];
+            const int dj = a.dims[j];
+            const double *B = a.vals + a.bsr_voff[k] + (int64_t)r * dj;   // row r of the d_i x d_j block
+            const double *xj = xin + a.bo[j];
 #pragma unroll
-    for (int r = 0; r < D; r++) y[r] = 0.0;
-    for (int k = a.bsr_rowptr[i] + lane; k < a.bsr_rowptr[i + 1]; k += 64) {
-        const int j = a.bsr_col[k];
-        const int dj = a.dims[j];
-        const double *B = a.vals + a.bsr_voff[k];
-        const double *xj = xin + a.bo[j];
-        double xv[D];
-#pragma unroll
-        for (int c = 0; c < D; c++) xv[c] = c < dj ? xj[c] : 0.0;
-#pragma unroll
-        for (int r = 0; r < D; r++) {
-            if (r < di) {
-                double t = 0.0;
-#pragma unroll
-                for (int c = 0; c < D; c++)
-                    if (c < dj) t += B[r * dj + c] * xv[c];
-                y[r] += t;
-            }
+            for (int c = 0; c < D; c++)
+                if (c < dj) acc += B[c] * xj[c];
         }
     }
-#pragma unroll
-    for (int r = 0; r < D; r++) y[r] = wave_sum64(y[r]);
-    if (lane == 0) {
-        const int64_t b = a.bo[i];
-        double pq = 0.0;
-#pragma unroll
-        for (int r = 0; r < D; r++)
-            if (r < di) {
-                yout[b + r] = y[r];
-                pq += xin[b + r] * y[r];
-            }
-        if (pq_part) pq_part[i] = pq;
+    red[w][lane] = acc;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double y = 0.0, pq = 0.0;
+    const int64_t b = a.bo[i];
+    if (lane < di) {
+        for (int q = 0; q < nb; q++) y += red[w][q * di + lane];
+        yout[b + lane] = y;
+        pq = xin[b + lane] * y;
+    }
+    if (pq_part) {
+        pq = wave_sum64(pq);
+        if (lane == 0) pq_part[i] = pq;
     }
 }
 
@@ -511,9 +509,9 @@ static void launch_spmv(const bls_pcg *h, const PcgArgs &a, const double *xin, d
     const unsigned grid = (unsigned)((h->nrb + 3) / 4);
     if (h->uniform_d == 6) k_bsr_spmv_u<6><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
     else if (h->uniform_d == 9) k_bsr_spmv_u<9><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
-    else if (h->maxd <= 6) k_bsr_spmv_w<6><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
-    else if (h->maxd <= 9) k_bsr_spmv_w<9><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
-    else k_bsr_spmv_w<16><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    else if (h->maxd <= 6) k_bsr_spmv_v<6><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    else if (h->maxd <= 9) k_bsr_spmv_v<9><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
+    else k_bsr_spmv_v<16><<<grid, 256, 0, s>>>(a, xin, yout, pq, only_active);
 }
 
 extern "C" int bls_pcg_create(int64_t nr, int64_t nrblocks, const int64_t *r_dims, const int64_t *boffsets_r,

@@ -112,3 +112,45 @@ def test_pcg_batched_problems(blsamd):
         assert okq and itq[0] == iters[i]
         a, b = st.boffsets_r[pbp[i] - 1] - 1, st.boffsets_r[pbp[i + 1] - 2] - 1 + st.r_dims[pbp[i + 1] - 2]
         assert np.allclose(x.cpu().numpy()[a:b], xq.cpu().numpy(), rtol=1e-9, atol=1e-12 * np.abs(xq.cpu().numpy()).max())
+
+
+def test_pcg_mixed_block_dims(blsamd):
+    """PCG on a reduced system whose camera blocks are 6 and 9 wide (the product with block dims that differ between
+    block rows): S = J~'J~ + lambda I from the QR path on the device; true residual, a dense solve, relres against
+    the host's product, and determinism."""
+    import torch
+    from test_oracle_qr import _small_jacobian, _qr_pattern
+    dev = blsamd.device
+    p = _small_jacobian(blsamd, 21, nland=300, ncam=12, kmax=6)
+    st = p.structure
+    assert len(set(int(d) for d in st.r_dims)) == 2
+    h = dev.QRHandle(p.jst)
+    pat = _qr_pattern(blsamd, p)
+    h.set_sparse(pat)
+    ws = dev.QRReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    assert dev.reduce_qr_(ws, h, T(p.data), T(p.r), T(p.scaling_e), 0.1)
+    S = torch.empty(pat.nnz, dtype=torch.float64, device="cuda")
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    dev.reduced_normal_sparse_(S, g, h, ws)
+    S[torch.as_tensor(pat.diag_inds - 1).cuda()] += 0.1
+    pcg = dev.PCGHandle(st, pat)
+    x = torch.zeros_like(g)
+    ok, iters, rel = pcg.solve(S, g, x, tol=1e-8, maxit=2000)
+    assert ok and rel[0] <= 1e-8
+    M = np.zeros((st.nr, st.nr))
+    Sh = S.cpu().numpy()
+    for j in range(st.nr):
+        for k in range(pat.S_colptr[j] - 1, pat.S_colptr[j + 1] - 1):
+            M[pat.S_rowval[k] - 1, j] = Sh[k]
+    Sd = np.triu(M) + np.triu(M, 1).T
+    gh, xg = g.cpu().numpy(), x.cpu().numpy()
+    xd = np.linalg.solve(Sd, gh)
+    assert np.linalg.norm(Sd @ xg - gh) <= 1.0001e-8 * np.linalg.norm(gh)
+    assert np.linalg.norm(xg - xd) <= 1e-5 * np.linalg.norm(xd)
+    xr = torch.as_tensor(np.random.default_rng(3).standard_normal(st.nr)).cuda()
+    want = np.linalg.norm(Sd @ xr.cpu().numpy() - gh) / np.linalg.norm(gh)
+    assert abs(pcg.relres(S, g, xr)[0] - want) <= 1e-12 * want
+    x2 = torch.zeros_like(g)
+    pcg.solve(S, g, x2, tol=1e-8, maxit=2000)
+    assert np.array_equal(x2.cpu().numpy(), xg)
