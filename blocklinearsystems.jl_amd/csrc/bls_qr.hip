@@ -560,8 +560,9 @@ __device__ inline RGTab rg_load(const RGTab *p) {   // field by field, so the re
 // staged block (X row-major with row stride SW = Wp + 1, r~ in column Wp, zeros elsewhere; PF values per thread,
 // PF NT >= stage_cap) and entry words (ecd[u] = (entry column - c0) << 16 | entry dim) are loading into registers, and
 // the record of the one after it is loading too.
+// at least 4 waves per SIMD: 16 wavefronts per CU in every class (4, 2 and 1 workgroups of 4, 8 and 16)
 template <class T, int NT, int PF>
-__global__ __launch_bounds__(NT) void k_red_gram(const RGTab *__restrict__ tabs, int64_t cnt,
+__global__ __launch_bounds__(NT, 4) void k_red_gram(const RGTab *__restrict__ tabs, int64_t cnt,
                                                  const int32_t *__restrict__ ecd, int stage_cap, const T *__restrict__ F,
                                                  const double *__restrict__ Fr, double *__restrict__ G,
                                                  double *__restrict__ gp) {
