@@ -515,16 +515,16 @@ __global__ __launch_bounds__(256) void k_red_dsum(RedArgs a, const int64_t *__re
 // contributions, as zeros); k_red_gsum adds the partials per retained block. Deterministic, no atomics.
 constexpr int RG_MAXE = 64;    // entries of one table (one wavefront scans their dims)
 constexpr int RG_MAXW = 1024;  // columns spanned by the entries of one table
-constexpr int RG_STATIC_LDS = 8192;   // bytes of LDS reserved for the per-column words of the Gram pass
-static_assert(RG_MAXW * (sizeof(int16_t) + sizeof(int)) <= RG_STATIC_LDS, "per-column words fit the reserve");
+constexpr int RG_STATIC_LDS = 6144;   // bytes of static LDS of the Gram pass (its per-column words)
+static_assert(RG_MAXW * (sizeof(int16_t) + sizeof(int)) == RG_STATIC_LDS, "static LDS of the Gram pass");
 
 typedef double rg_v4d __attribute__((ext_vector_type(4)));
 
 // Launch classes by the size of a table's staged block (rows rounded up to 4, columns to 16, one pad column): up to
-// 4096 doubles, up to 8192, up to the device's LDS per workgroup, and the rest, which read their operands from global
-// memory.
-constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 8192, RG_CLASS_CAP2 = 19456;
-static_assert(16 * 256 >= RG_CLASS_CAP0 && 16 * 512 >= RG_CLASS_CAP1 && 19 * 1024 >= RG_CLASS_CAP2, "prefetch covers the classes");
+// 4096 doubles (four workgroups per CU), up to 9472 (two: 2 x (9472 x 8 + 6144) bytes fit 160 KB), up to the
+// device's LDS per workgroup (one), and the rest, which read their operands from global memory.
+constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 9472, RG_CLASS_CAP2 = 19456;
+static_assert(16 * 256 >= RG_CLASS_CAP0 && 19 * 512 >= RG_CLASS_CAP1 && 19 * 1024 >= RG_CLASS_CAP2, "prefetch covers the classes");
 
 // One table of the Gram pass, in launch order (built on the host): where its block and residual lie, where its pair
 // blocks and per-entry partials go, its reduced rows n and the W columns its entries span from column c0.
@@ -1361,9 +1361,9 @@ template <class T> int reduced_normal(bls_qr *h, int dense, T *S, double *g, con
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 k_red_gram<T, 256, 16><<<grid, 256, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
             } else if (c == 1) {
-                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512, 16>,
+                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512, 19>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 512, 16><<<grid, 512, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
+                k_red_gram<T, 512, 19><<<grid, 512, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
             } else {
                 BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 1024, 19>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -248,7 +248,7 @@ def test_reduced_normal_rejects_too_wide_tables(blsamd):
 
 def _gram_classes(jst):
     """Launch classes of reduced_normal's Gram pass that the tables of jst fall in: staged size (reduced rows rounded
-    up to 4) x (entry columns rounded up to 16, plus one) against 4096, 8192 and 19456 doubles."""
+    up to 4) x (entry columns rounded up to 16, plus one) against 4096, 9472 and 19456 doubles."""
     tp = np.asarray(jst.table_ptr) - 1
     ec = np.asarray(jst.entry_col) - 1
     rd = np.asarray(jst.schur.r_dims)[np.asarray(jst.entry_retained) - 1]
@@ -260,7 +260,7 @@ def _gram_classes(jst):
         W = ec[u1 - 1] + rd[u1 - 1] - ec[u0]
         n = jst.table_nrows[t] - jst.table_top[t]
         st = ((n + 3) // 4 * 4) * ((W + 15) // 16 * 16 + 1)
-        out.add(next((k for k, cap in enumerate((4096, 8192, 19456)) if st <= cap), 3))
+        out.add(next((k for k, cap in enumerate((4096, 9472, 19456)) if st <= cap), 3))
     return out
 
 
@@ -276,7 +276,7 @@ def test_reduced_normal_matches_oracle(blsamd, table_type):
     dev = blsamd.device
     # the third problem puts about 250 tables on every camera: its columns are split into parts and merged
     # the fourth problem adds camera-pair tables of 210, 300 and 1200 rows: with the landmark tables they fill all
-    # four launch classes of the Gram pass (staged block up to 4096, 8192, 19456 doubles, and global memory)
+    # four launch classes of the Gram pass (staged block up to 4096, 9472, 19456 doubles, and global memory)
     tall = _jac_with_free_rows(blsamd, seed=9, tall_pairs=((0, 1, 70), (2, 3, 100), (4, 5, 400)))
     assert _gram_classes(tall.jst) == {0, 1, 2, 3}
     # the fifth has a free table over 40 cameras (40 retained blocks, about 300 columns)
