@@ -1,0 +1,290 @@
+// Assembly of a Schur Hessian from row blocks on gfx950: the batched form of addblock!
+// (src/schur_assembly.jl:140-246) behind the C ABI of include/bls_amd.h.
+//
+// A row block k (observation rows touching some layout blocks, in the column order of its Jacobian J_k) contributes
+// J_k' J_k to every pair of its blocks and J_k' r_k to each block. Each row block is a table of the Gram pass
+// (bls_gram.h): one read of J_k forms every pair block on the FP64 matrix instruction into a contribution buffer, and
+// every stored block of E_data, C_data and R -- and each block of b_r and b_e -- is then the sum of its
+// contributions in ascending row-block order, the order in which addblock! calls made in row-block order add them.
+// No atomics; bitwise reproducible. A pair's contribution is read in the orientation its destination stores
+// (add_retained_diagonal!, add_retained_pair!, add_eliminated!, add_coupling!, src/schur_assembly.jl:189-246).
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "bls_internal.h"
+
+using namespace bls;
+
+namespace {
+#include "bls_gram.h"
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------
+// The handle: the plan of one fill pattern
+// ---------------------------------------------------------------------------------------------------------
+struct bls_asm {
+    int64_t nr = 0, nrb = 0, neb = 0, ncoup = 0, nrowblocks = 0;
+    int64_t E_len = 0, C_len = 0, R_len = 0;
+    bool dense = false;
+    RGPlan rg;
+    DevArray<RGTab> d_tabs;
+    DevArray<int32_t> d_ecd;
+    int64_t glen = 0, gplen = 0;
+    DevArray<double> d_G, d_gp;
+    // destinations: E, C, R, b_r, b_e, each a block list with its contribution lists
+    static constexpr int ND = 5;
+    DevArray<RGBlock> d_blocks[ND];
+    DevArray<int64_t> d_ptr[ND], d_contrib[ND];
+    int64_t nblk[ND] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+struct Dest {   // one destination block while planning
+    RGBlock blk;
+    std::vector<int64_t> contrib;
+};
+
+std::string pair_name(int64_t i, int64_t j) {
+    return "Blocks " + std::to_string(i) + " and " + std::to_string(j);
+}
+
+}  // namespace
+
+extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t *r_dims, const int64_t *e_dims,
+                              const int64_t *boffsets_r, const int64_t *boffsets_e, const int64_t *eoffsets,
+                              const int64_t *eliminated_ptr, const int64_t *entry_retained, const int64_t *coffsets,
+                              const int64_t *r_colptr, const int64_t *r_rowval, const int64_t *r_offsets,
+                              int64_t nrowblocks, const int64_t *rb_ptr, const int64_t *rb_blocks,
+                              const int64_t *rb_rows, bls_asm **out) {
+    if (!out || nrblocks < 0 || neblocks < 0 || nrowblocks < 0 || (nrblocks && (!r_dims || !boffsets_r)) ||
+        (neblocks && (!e_dims || !boffsets_e || !eoffsets || !eliminated_ptr || !coffsets)) || !rb_ptr ||
+        (nrowblocks && (!rb_rows || !rb_blocks)) || (r_colptr && (!r_rowval || !r_offsets)))
+        return fail(BLS_ERR_ARG, "bls_asm_create: bad arguments");
+    *out = nullptr;
+    const int64_t nrb = nrblocks, neb = neblocks, nb = nrb + neb;
+    const int64_t ncoup = neb ? eliminated_ptr[neb] - 1 : 0;
+    if (ncoup && !entry_retained) return fail(BLS_ERR_ARG, "bls_asm_create: bad arguments");
+    std::vector<int64_t> dim(nb), boff(nb);
+    int64_t nr = 0;
+    for (int64_t r = 0; r < nrb; r++) {
+        dim[r] = r_dims[r];
+        boff[r] = boffsets_r[r] - 1;
+        nr = std::max(nr, boff[r] + dim[r]);
+        if (dim[r] < 1 || dim[r] > 16) return fail(BLS_ERR_UNSUPPORTED, "retained block dimension outside 1..16");
+    }
+    for (int64_t e = 0; e < neb; e++) {
+        dim[nrb + e] = e_dims[e];
+        boff[nrb + e] = boffsets_e[e] - 1;
+        if (dim[nrb + e] < 1 || dim[nrb + e] > 16) return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension outside 1..16");
+    }
+    auto h = new bls_asm();
+    std::unique_ptr<bls_asm> guard(h);
+    h->nr = nr;
+    h->nrb = nrb;
+    h->neb = neb;
+    h->ncoup = ncoup;
+    h->nrowblocks = nrowblocks;
+    h->dense = r_colptr == nullptr;
+
+    // destination blocks: E (full), couplings, R (dense upper or sparse lower; full diagonal blocks), b_r, b_e
+    std::vector<Dest> dE(neb), dC(ncoup), dR, dbr(nrb), dbe(neb);
+    for (int64_t e = 0; e < neb; e++) {
+        const int d = (int)dim[nrb + e];
+        dE[e].blk = RGBlock{eoffsets[e] - 1, d, d, d, 1, RG_FULL};
+        h->E_len = std::max(h->E_len, eoffsets[e] - 1 + (int64_t)d * d);
+        dbe[e].blk = RGBlock{boff[nrb + e], 1, 1, d, 0, 0};
+        for (int64_t c = eliminated_ptr[e] - 1; c < eliminated_ptr[e + 1] - 1; c++) {
+            const int64_t r = entry_retained[c] - 1;
+            if (r < 0 || r >= nrb) return fail(BLS_ERR_ARG, "bls_asm_create: entry_retained out of range");
+            dC[c].blk = RGBlock{coffsets[c] - 1, d, d, (int32_t)dim[r], 0, 0};   // d_e x d_r column-major
+            h->C_len = std::max(h->C_len, coffsets[c] - 1 + (int64_t)d * dim[r]);
+        }
+    }
+    for (int64_t r = 0; r < nrb; r++) dbr[r].blk = RGBlock{boff[r], 1, 1, (int32_t)dim[r], 0, 0};
+    // R: index of the destination of retained pair (p, q)
+    std::vector<int64_t> rcol;   // sparse: block column start in dR
+    if (h->dense) {
+        dR.resize((size_t)(nrb * (nrb + 1) / 2));
+        for (int64_t q = 0; q < nrb; q++)
+            for (int64_t p = 0; p <= q; p++)   // upper block (p, q): element (p row, q col) at (bo_q + col) nr + bo_p + row
+                dR[q * (q + 1) / 2 + p].blk =
+                    RGBlock{boff[p] + boff[q] * nr, nr, (int32_t)dim[p], (int32_t)dim[q], p == q, p == q ? RG_FULL : 0};
+        h->R_len = nr * nr;
+    } else {
+        rcol.assign(nrb + 1, 0);
+        for (int64_t q = 0; q < nrb; q++) {
+            rcol[q] = (int64_t)dR.size();
+            for (int64_t k = r_colptr[q] - 1; k < r_colptr[q + 1] - 1; k++) {
+                const int64_t p = r_rowval[k] - 1;   // lower block (p, q), p >= q: d_p x d_q column-major
+                if (p < q || p >= nrb) return fail(BLS_ERR_ARG, "bls_asm_create: R block row outside q..nrblocks");
+                if (k > r_colptr[q] - 1 && p <= r_rowval[k - 1] - 1)
+                    return fail(BLS_ERR_ARG, "bls_asm_create: R block rows not ascending");
+                Dest d;
+                d.blk = RGBlock{r_offsets[k] - 1, dim[p], (int32_t)dim[p], (int32_t)dim[q], p == q, p == q ? RG_FULL : 0};
+                h->R_len = std::max(h->R_len, r_offsets[k] - 1 + dim[p] * dim[q]);
+                dR.push_back(std::move(d));
+            }
+        }
+        rcol[nrb] = (int64_t)dR.size();
+    }
+    auto coupling = [&](int64_t e, int64_t r) -> int64_t {   // find_coupling (src/schur_assembly.jl:94-98)
+        const int64_t *lo = entry_retained + eliminated_ptr[e] - 1, *hi = entry_retained + eliminated_ptr[e + 1] - 1;
+        const int64_t *it = std::lower_bound(lo, hi, r + 1);
+        return (it != hi && *it == r + 1) ? (int64_t)(it - entry_retained) : -1;
+    };
+    auto rblock = [&](int64_t p, int64_t q) -> int64_t {   // stored R block of retained pair p != q or p == q
+        if (h->dense) {
+            const int64_t a = std::min(p, q), b = std::max(p, q);
+            return b * (b + 1) / 2 + a;
+        }
+        const int64_t a = std::max(p, q), b = std::min(p, q);   // lower block (a, b)
+        const int64_t *lo = r_rowval + r_colptr[b] - 1, *hi = r_rowval + r_colptr[b + 1] - 1;
+        const int64_t *it = std::lower_bound(lo, hi, a + 1);
+        return (it != hi && *it == a + 1) ? rcol[b] + (int64_t)(it - lo) : -1;
+    };
+
+    // row blocks: records of the Gram pass, entry words, and each pair's destination
+    std::vector<RGTab> recs;
+    std::vector<int32_t> ecd;
+    int64_t joff = 0, roff = 0, goff = 0, eoff = 0;
+    for (int64_t k = 0; k < nrowblocks; k++) {
+        const int64_t u0 = rb_ptr[k] - 1, u1 = rb_ptr[k + 1] - 1, ne = u1 - u0;
+        const int64_t m = rb_rows[k];
+        if (ne < 0 || m < 0) return fail(BLS_ERR_ARG, "bls_asm_create: row block " + std::to_string(k + 1) + " is malformed");
+        if (ne > RG_MAXE)
+            return fail(BLS_ERR_UNSUPPORTED, "row block " + std::to_string(k + 1) + " touches " + std::to_string(ne) +
+                                                 " blocks; the device assembly takes up to 64");
+        std::vector<int64_t> blk(ne), col(ne);
+        int64_t W = 0;
+        for (int64_t i = 0; i < ne; i++) {
+            blk[i] = rb_blocks[u0 + i] - 1;
+            if (blk[i] < 0 || blk[i] >= nb)
+                return fail(BLS_ERR_ARG, "row block " + std::to_string(k + 1) + ": block " +
+                                             std::to_string(rb_blocks[u0 + i]) + " out of range");
+            for (int64_t j = 0; j < i; j++)
+                if (blk[j] == blk[i])
+                    return fail(BLS_ERR_ARG, "row block " + std::to_string(k + 1) + " lists block " +
+                                                 std::to_string(blk[i] + 1) + " twice");
+            col[i] = W;
+            W += dim[blk[i]];
+        }
+        if (W > RG_MAXW)
+            return fail(BLS_ERR_UNSUPPORTED, "row block " + std::to_string(k + 1) +
+                                                 " spans more than 1024 columns; the device assembly takes up to 1024");
+        if (ne && m) {
+            RGTab g;
+            g.raw_off = joff;
+            g.res_off = roff;
+            g.goff = goff;
+            g.eoff = eoff;
+            g.M = (int32_t)m;
+            g.top = 0;
+            g.n = (int32_t)m;
+            g.u0 = (int32_t)ecd.size();
+            g.ne = (int32_t)ne;
+            g.c0 = 0;
+            g.W = (int32_t)W;
+            g.pad = 0;
+            recs.push_back(g);
+            for (int64_t i = 0; i < ne; i++) ecd.push_back((int32_t)((col[i] << 16) | dim[blk[i]]));
+            // pairs i1 <= i2 in column order, d1 x d2 row-major at goff + (running offset): the Gram pass's order
+            int64_t off = goff;
+            for (int64_t i2 = 0; i2 < ne; i2++)
+                for (int64_t i1 = 0; i1 <= i2; i1++) {
+                    const int64_t p = blk[i1], q = blk[i2];
+                    const int64_t sz = dim[p] * dim[q];
+                    if (i1 == i2) {
+                        if (p >= nrb) dE[p - nrb].contrib.push_back(off);
+                        else {
+                            const int64_t b = rblock(p, p);
+                            if (b < 0)
+                                return fail(BLS_ERR_ARG, pair_name(p + 1, p + 1) + ": R stores no diagonal block");
+                            dR[b].contrib.push_back(off);
+                        }
+                    } else if (p >= nrb && q >= nrb) {
+                        return fail(BLS_ERR_ARG, pair_name(p + 1, q + 1) + " are both eliminated: the system has no storage for the pair");
+                    } else if (p >= nrb || q >= nrb) {   // coupling (e, r): stored d_e x d_r, natural orientation (e, r)
+                        const int64_t e = (p >= nrb ? p : q) - nrb, r = p >= nrb ? q : p;
+                        const int64_t c = coupling(e, r);
+                        if (c < 0)
+                            return fail(BLS_ERR_ARG, pair_name(p + 1, q + 1) + " share no row block: the system has no storage for the pair");
+                        dC[c].contrib.push_back(p >= nrb ? off : ~off);
+                    } else {   // retained pair: dense stores (min, max) upper, sparse (max, min) lower
+                        const int64_t b = rblock(p, q);
+                        if (b < 0)
+                            return fail(BLS_ERR_ARG, pair_name(p + 1, q + 1) + " share no row block: the system has no storage for the pair");
+                        const bool natural = h->dense ? p < q : p > q;
+                        dR[b].contrib.push_back(natural ? off : ~off);
+                    }
+                    off += sz;
+                }
+            for (int64_t i = 0; i < ne; i++) {
+                const int64_t p = blk[i];
+                (p >= nrb ? dbe[p - nrb] : dbr[p]).contrib.push_back(eoff + col[i]);
+            }
+            goff = off;
+            eoff += W;
+        }
+        joff += m * W;
+        roff += m;
+    }
+    std::vector<RGTab> tabs;
+    int rc;
+    if ((rc = rg_classify(recs, tabs, h->rg))) return rc;
+    if ((rc = h->d_tabs.upload(tabs)) || (rc = h->d_ecd.upload(ecd))) return rc;
+    h->rg.tabs = h->d_tabs.ptr;
+    h->rg.ecd = h->d_ecd.ptr;
+    if ((rc = h->d_G.alloc((size_t)std::max<int64_t>(1, goff))) || (rc = h->d_gp.alloc((size_t)std::max<int64_t>(1, eoff))))
+        return rc;
+    h->glen = goff;
+    h->gplen = eoff;
+    std::vector<Dest> *lists[bls_asm::ND] = {&dE, &dC, &dR, &dbr, &dbe};
+    for (int l = 0; l < bls_asm::ND; l++) {
+        std::vector<RGBlock> blocks;
+        std::vector<int64_t> ptr(1, 0), contrib;
+        for (const Dest &d : *lists[l]) {
+            blocks.push_back(d.blk);
+            contrib.insert(contrib.end(), d.contrib.begin(), d.contrib.end());
+            ptr.push_back((int64_t)contrib.size());
+        }
+        if ((rc = h->d_blocks[l].upload(blocks)) || (rc = h->d_ptr[l].upload(ptr)) || (rc = h->d_contrib[l].upload(contrib)))
+            return rc;
+        h->nblk[l] = (int64_t)blocks.size();
+    }
+    *out = guard.release();
+    return BLS_OK;
+}
+
+extern "C" void bls_asm_destroy(bls_asm *a) { delete a; }
+
+extern "C" int bls_asm_lengths(const bls_asm *a, int64_t *lengths) {
+    if (!a || !lengths) return fail(BLS_ERR_ARG, "bls_asm_lengths: bad arguments");
+    lengths[0] = a->E_len;
+    lengths[1] = a->C_len;
+    lengths[2] = a->R_len;
+    lengths[3] = a->nr;
+    return BLS_OK;
+}
+
+extern "C" int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r, double *E_data, double *C_data,
+                                   double *R, double *b_r, double *b_e, void *stream) {
+    if (!a) return fail(BLS_ERR_ARG, "null handle");
+    if ((a->glen && (!J || !r)) || (a->neb && (!E_data || !b_e)) || (a->ncoup && !C_data) || (a->nrb && (!R || !b_r)))
+        return fail(BLS_ERR_ARG, "bls_asm_hessian_f64: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (a->dense && a->R_len)
+        k_rg_fill<double><<<(unsigned)std::min<int64_t>((a->R_len + 255) / 256, 4096), 256, 0, s>>>(R, a->R_len);
+    if ((rc = rg_launch<double>(a->rg, J, r, a->d_G.ptr, a->d_gp.ptr, s))) return rc;
+    double *dst[bls_asm::ND] = {E_data, C_data, R, b_r, b_e};
+    for (int l = 0; l < bls_asm::ND; l++) {
+        if (!a->nblk[l]) continue;
+        const double *src = l < 3 ? a->d_G.ptr : a->d_gp.ptr;
+        k_red_gram_sum<double><<<(unsigned)a->nblk[l], 256, 0, s>>>(a->nblk[l], a->d_blocks[l].ptr, a->d_ptr[l].ptr,
+                                                                     a->d_contrib[l].ptr, src, dst[l]);
+    }
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}

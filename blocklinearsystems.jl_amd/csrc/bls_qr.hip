@@ -513,292 +513,7 @@ __global__ __launch_bounds__(256) void k_red_dsum(RedArgs a, const int64_t *__re
 // contributions of each block (r1, r2) of S in ascending table order -- the order in which the reference adds them to
 // each entry -- and writes the whole block (the lower triangle of a diagonal block, and a block without
 // contributions, as zeros); k_red_gsum adds the partials per retained block. Deterministic, no atomics.
-constexpr int RG_MAXE = 64;    // entries of one table (one wavefront scans their dims)
-constexpr int RG_MAXW = 1024;  // columns spanned by the entries of one table
-constexpr int RG_STATIC_LDS = 6144;   // bytes of static LDS of the Gram pass (its per-column words)
-static_assert(RG_MAXW * (sizeof(int16_t) + sizeof(int)) == RG_STATIC_LDS, "static LDS of the Gram pass");
-
-typedef double rg_v4d __attribute__((ext_vector_type(4)));
-
-// Launch classes by the size of a table's staged block (rows rounded up to 4, columns to 16, one pad column): up to
-// 4096 doubles (four workgroups per CU), up to 9472 (two: 2 x (9472 x 8 + 6144) bytes fit 160 KB), up to the
-// device's LDS per workgroup (one), and the rest, which read their operands from global memory.
-constexpr int RG_CLASS_CAP0 = 4096, RG_CLASS_CAP1 = 9472, RG_CLASS_CAP2 = 19456;
-static_assert(16 * 256 >= RG_CLASS_CAP0 && 19 * 512 >= RG_CLASS_CAP1 && 19 * 1024 >= RG_CLASS_CAP2, "prefetch covers the classes");
-
-// One table of the Gram pass, in launch order (built on the host): where its block and residual lie, where its pair
-// blocks and per-entry partials go, its reduced rows n and the W columns its entries span from column c0.
-struct RGTab {
-    int64_t raw_off, res_off, goff, eoff;
-    int32_t M, top, n, u0, ne, c0, W, pad;
-};
-
-__device__ inline RGTab rg_load(const RGTab *p) {   // field by field, so the record stays in registers
-    RGTab g;
-    g.raw_off = p->raw_off;
-    g.res_off = p->res_off;
-    g.goff = p->goff;
-    g.eoff = p->eoff;
-    g.M = p->M;
-    g.top = p->top;
-    g.n = p->n;
-    g.u0 = p->u0;
-    g.ne = p->ne;
-    g.c0 = p->c0;
-    g.W = p->W;
-    g.pad = 0;
-    return g;
-}
-
-// X = the reduced rows of table t over the columns of its entries (n x W, column c of the table at c0 + c). Each
-// wavefront forms 16 x 16 tiles (I <= J) of X'X with the FP64 matrix instruction (16x16x4: lane l holds A(l % 16,
-// l / 16), B(l / 16, l % 16), and D(l / 16 + 4 v, l % 16) in accumulator v), from X staged row-major in LDS when it
-// fits, and writes each element (a, b), a <= b, to its pair block (e1, e2): the pairs of the table's entries i1 <= i2
-// in the order i2 (i2 + 1) / 2 + i1, each d1 x d2 row-major, at offset Q[e2] + d2 P[e1] with P the prefix sums of
-// the entry dims and Q[e] = sum_{j < e} d_j P[j + 1]. Lower entries of a diagonal block (row > col) are not written.
-// Persistent: workgroup b takes tables tabs[b], tabs[b + grid], ...; while it works on one table, the next one's
-// staged block (X row-major with row stride SW = Wp + 1, r~ in column Wp, zeros elsewhere; PF values per thread,
-// PF NT >= stage_cap) and entry words (ecd[u] = (entry column - c0) << 16 | entry dim) are loading into registers, and
-// the record of the one after it is loading too.
-// at least 4 waves per SIMD: 16 wavefronts per CU in every class (4, 2 and 1 workgroups of 4, 8 and 16)
-template <class T, int NT, int PF>
-__global__ __launch_bounds__(NT, 4) void k_red_gram(const RGTab *__restrict__ tabs, int64_t cnt,
-                                                 const int32_t *__restrict__ ecd, int stage_cap, const T *__restrict__ F,
-                                                 const double *__restrict__ Fr, double *__restrict__ G,
-                                                 double *__restrict__ gp) {
-    extern __shared__ double XL[];
-    // per staged column c of the current table (-1 where no entry holds it): s_colA[c] = P[e] + j, its column in
-    // the entries' packed order (entry e, column j of it), and s_colB[c] = (Q[e] + j) << 5 | d_e, so that element
-    // (a, b) of X'X goes to pair-block offset (s_colB[b] >> 5) + (s_colB[b] & 31) s_colA[a]
-    __shared__ int16_t s_colA[RG_MAXW];
-    __shared__ int s_colB[RG_MAXW];
-    double pf[PF];
-    int my_ecd = 0;
-    auto prefetch = [&](const RGTab g) {
-        my_ecd = (int)threadIdx.x < g.ne ? ecd[g.u0 + threadIdx.x] : 0;
-        const int Wp = (g.W + 15) & ~15, np4 = (g.n + 3) & ~3, SW = Wp + 1;
-        if ((int64_t)np4 * SW > stage_cap) return;
-        const T *Xg = F + g.raw_off + g.top;
-        const double *rg = Fr + g.res_off + g.top;
-        // element q = threadIdx.x + j NT is (k, c) = (q % np4, q / np4), walked without divisions
-        const int dc = NT / np4, dk = NT - dc * np4;
-        int c = threadIdx.x / np4, k = threadIdx.x - c * np4;
-#pragma unroll
-        for (int j = 0; j < PF; j++) {
-            double v = 0.0;
-            if (c < SW && k < g.n) {
-                if (c < g.W) v = (double)Xg[(int64_t)(g.c0 + c) * g.M + k];
-                else if (c == Wp) v = rg[k];
-            }
-            pf[j] = v;
-            c += dc;
-            k += dk;
-            if (k >= np4) {
-                k -= np4;
-                c++;
-            }
-        }
-    };
-    const int64_t grid = gridDim.x;
-    RGTab cur = rg_load(tabs + blockIdx.x);   // the grid is at most cnt
-    RGTab nxt = rg_load(tabs + (blockIdx.x + grid < cnt ? blockIdx.x + grid : blockIdx.x));
-    prefetch(cur);
-    for (int64_t it = blockIdx.x; it < cnt; it += grid) {
-        __syncthreads();   // the previous table is done with XL and the maps
-        const int M = cur.M, n = cur.n, ne = cur.ne, c0 = cur.c0, W = cur.W;
-        const int Wp = (W + 15) & ~15, np4 = (n + 3) & ~3, SW = Wp + 1;
-        const bool fits = (int64_t)np4 * SW <= stage_cap;
-        const T *Xg = F + cur.raw_off + cur.top;        // X(k, c) = Xg[(c0 + c) M + k]
-        const double *rg = Fr + cur.res_off + cur.top;  // r~(k)
-        double *Gt = G + cur.goff;
-        double *gpt = gp + cur.eoff;
-        if (fits) {
-            const int dc = NT / np4, dk = NT - dc * np4;
-            int c = threadIdx.x / np4, k = threadIdx.x - c * np4;
-#pragma unroll
-            for (int j = 0; j < PF; j++) {
-                if (c < SW) XL[k * SW + c] = pf[j];
-                c += dc;
-                k += dk;
-                if (k >= np4) {
-                    k -= np4;
-                    c++;
-                }
-            }
-        }
-        for (int i = threadIdx.x; i < Wp; i += NT) {
-            s_colA[i] = -1;
-            s_colB[i] = -1;
-        }
-        const int e_cst = my_ecd >> 16, e_dim = my_ecd & 0xffff;   // entry threadIdx.x (< ne)
-        int e_P = 0, e_Q = 0;
-        if (threadIdx.x < 64) {   // prefix sums of the entry dims: P (columns) and Q (pair-block offsets)
-            const int e = threadIdx.x;
-            const int d = e < ne ? e_dim : 0;
-            int p = d;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(p, o);
-                if (e >= o) p += v;
-            }
-            const int q0 = d * p;   // d_e P[e + 1]
-            int q = q0;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(q, o);
-                if (e >= o) q += v;
-            }
-            e_P = p - d;
-            e_Q = q - q0;
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < ne)
-            for (int j = 0; j < e_dim; j++) {
-                s_colA[e_cst + j] = (int16_t)(e_P + j);
-                s_colB[e_cst + j] = ((e_Q + j) << 5) | e_dim;
-            }
-        __syncthreads();
-        if (it + grid < cnt) {   // the next table's loads go out before this one's work
-            cur = nxt;
-            prefetch(cur);
-            if (it + 2 * grid < cnt) nxt = rg_load(tabs + it + 2 * grid);
-        }
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-        // writes tile (I, J) of X'X: element (a, b), a <= b, to its pair block
-        auto store = [&](const rg_v4d &acc, int I, int J) {
-            const int b = 16 * J + li;
-            const int cb = b < W ? s_colB[b] : -1;
-            if (cb < 0) return;
-            const int baseB = cb >> 5, d2 = cb & 31;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int aa = 16 * I + lk + 4 * v;
-                const int pa = aa <= b ? s_colA[aa] : -1;
-                if (pa >= 0) Gt[baseB + d2 * pa] = acc[v];
-            }
-        };
-        // work items, dealt to the wavefronts in turn: column strip J of the upper triangle in pairs of tiles
-        // (I0, I0 + 1) that share B, then the per-entry partials of J~' r~ (as k_red_tmul_part writes them), 64
-        // columns per item, one lane per column -- last, so that they go to the wavefronts with one tile item fewer
-        const int ngp = (W + 63) >> 6;
-        const int ntl = Wp >> 4;
-        const int ntitems = (ntl / 2 + 1) * (ntl / 2) + ((ntl & 1) ? (ntl + 1) / 2 : 0);
-        const int nitems = ntitems + ngp;
-        for (int item = wave; item < nitems; item += NT / 64) {
-            if (item >= ntitems) {
-                const int c = 64 * (item - ntitems) + lane;
-                const int pc = c < W ? s_colA[c] : -1;
-                if (pc >= 0) {
-                    double acc = 0.0;
-                    if (fits) {
-#pragma unroll 8
-                        for (int k = 0; k < n; k++) acc += XL[k * SW + c] * XL[k * SW + Wp];
-                    } else {
-                        const T *x = Xg + (int64_t)(c0 + c) * M;
-                        for (int k = 0; k < n; k++) acc += (double)x[k] * rg[k];
-                    }
-                    gpt[pc] = acc;
-                }
-                continue;
-            }
-            const int titem = item;
-            int J = 0, base = 0;
-            while (base + (J + 2) / 2 <= titem) {
-                base += (J + 2) / 2;
-                J++;
-            }
-            const int I0 = 2 * (titem - base);
-            const bool two = I0 + 1 <= J;
-            rg_v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-            if (fits) {
-                const double *pb = XL + lk * SW + 16 * J + li, *pa = XL + lk * SW + 16 * I0 + li;
-                if (two) {
-#pragma unroll 4
-                    for (int k0 = 0; k0 < np4; k0 += 4) {
-                        const double bv = pb[k0 * SW], a0 = pa[k0 * SW], a1 = pa[k0 * SW + 16];
-                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc1, 0, 0, 0);
-                    }
-                } else {
-#pragma unroll 4
-                    for (int k0 = 0; k0 < np4; k0 += 4)
-                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[k0 * SW], pb[k0 * SW], acc0, 0, 0, 0);
-                }
-            } else {
-                const bool vb = 16 * J + li < W, va0 = 16 * I0 + li < W, va1 = two && 16 * (I0 + 1) + li < W;
-                const T *xb = Xg + (int64_t)(c0 + 16 * J + li) * M, *xa = Xg + (int64_t)(c0 + 16 * I0 + li) * M;
-                for (int k0 = 0; k0 < np4; k0 += 4) {
-                    const int k = k0 + lk;
-                    const double bv = (vb && k < n) ? (double)xb[k] : 0.0;
-                    const double a0 = (va0 && k < n) ? (double)xa[k] : 0.0;
-                    const double a1 = (va1 && k < n) ? (double)xa[(int64_t)16 * M + k] : 0.0;
-                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc0, 0, 0, 0);
-                    if (two) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc1, 0, 0, 0);
-                }
-            }
-            store(acc0, I0, J);
-            if (two) store(acc1, I0 + 1, J);
-        }
-    }
-}
-
-struct RGBlock {
-    int64_t dst;       // S index of entry (0, 0) of the block
-    int64_t cstride;   // distance between its columns in S
-    int32_t d1, d2, diag, pad;
-};
-
-// One workgroup per block of S: entry w = row d2 + col of the block is the sum of its contributions (row-major pair
-// blocks of the Gram buffer) in ascending table order, zero below the diagonal of a diagonal block. Thread
-// (w, run) = (tid % sz, tid / sz) adds the run-th of 256 / sz contiguous runs of the block's contribution list, and
-// the runs are then added in order.
-template <class T>
-__global__ __launch_bounds__(256) void k_red_gram_sum(int64_t nblk, const RGBlock *__restrict__ blocks,
-                                                      const int64_t *__restrict__ ptr, const int64_t *__restrict__ contrib,
-                                                      const double *__restrict__ G, T *__restrict__ S) {
-    __shared__ double part[256];
-    const int64_t b = blockIdx.x;
-    const RGBlock bk = blocks[b];
-    const int sz = bk.d1 * bk.d2;   // <= 256
-    const int nrun = 256 / sz;
-    const int w = threadIdx.x % sz, run = threadIdx.x / sz;
-    const int row = w / bk.d2, col = w - row * bk.d2;
-    double acc = 0.0;
-    if (run < nrun && !(bk.diag && row > col)) {
-        const int64_t cb = ptr[b], L = ptr[b + 1] - cb;
-        const int64_t len = (L + nrun - 1) / nrun;
-        const int64_t kb = cb + std::min<int64_t>(L, run * len), ke = cb + std::min<int64_t>(L, (run + 1) * len);
-        int64_t c = kb;
-        // eight contributions per round; the next round's indices load while this round's values are in flight
-        int64_t idx[8];
-        if (c + 8 <= ke) {
-#pragma unroll
-            for (int j = 0; j < 8; j++) idx[j] = contrib[c + j];
-        }
-        for (; c + 8 <= ke; c += 8) {
-            double v[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = G[idx[j] + w];
-            if (c + 16 <= ke) {
-#pragma unroll
-                for (int j = 0; j < 8; j++) idx[j] = contrib[c + 8 + j];
-            }
-#pragma unroll
-            for (int j = 0; j < 8; j++) acc += v[j];
-        }
-        for (; c < ke; c++) acc += G[contrib[c] + w];
-    }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x < sz) {
-        double sum = 0.0;
-        for (int q = 0; q < nrun; q++) sum += part[q * sz + threadIdx.x];
-        S[bk.dst + (int64_t)col * bk.cstride + row] = (T)sum;
-    }
-}
-
-template <class T> __global__ void k_rg_fill(T *p, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = (T)0;
-}
+#include "bls_gram.h"
 
 // backsubstitute! of the QR path (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per eliminated table, one
 // lane per table: q the first d_e rows of the factored residual, B the first d_e rows of the retained columns, U the
@@ -869,9 +584,7 @@ struct bls_qr {
     DevArray<double> d_gp;                      // the partials
     DevArray<int64_t> d_dpo;                    // per-entry offsets of the diagonal-block partials (d (d + 1) / 2 each)
     DevArray<double> d_dp;                      // the diagonal-block partials (allocated at the first call)
-    int64_t rg_cls_off[5] = {0, 0, 0, 0, 0};
-    int rg_cap2 = 0;                            // staged doubles of the third class (from the device's LDS)
-    int rg_ncu = 0, rg_lds_cu = 0;              // compute units and LDS bytes per compute unit
+    RGPlan rg;                                  // the Gram pass over those records
     std::vector<int32_t> h_tp, h_er, h_ec;   // table_ptr, entry_retained, entry_col (0-based)
     std::vector<int32_t> h_nrows, h_top;
     std::vector<int64_t> h_raw_off, h_res_off;   // 0-based
@@ -1228,55 +941,36 @@ int build_gram_plan(bls_qr *h, int dense) {
             for (int i1 = 0; i1 <= i2; i1++) acc += (int64_t)rd[er[tp[t] + i1]] * rd[er[tp[t] + i2]];
     }
     goff[nt] = acc;
-    // launch classes by staged size; the third holds what fits the device's LDS per workgroup
-    int dev = 0, lds = 0;
-    BLS_HIP(hipGetDevice(&dev));
-    BLS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    h->rg_cap2 = (int)std::min<int64_t>(RG_CLASS_CAP2, std::max<int64_t>(0, (lds - RG_STATIC_LDS) / 8));
-    BLS_HIP(hipDeviceGetAttribute(&h->rg_ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    BLS_HIP(hipDeviceGetAttribute(&h->rg_lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    const int64_t caps[3] = {RG_CLASS_CAP0, RG_CLASS_CAP1, h->rg_cap2};
-    std::vector<int32_t> cls[4];
-    for (int64_t t = 0; t < nt; t++) {
-        const int ne = tp[t + 1] - tp[t];
-        if (!ne) continue;
-        const int64_t W = ec[tp[t] + ne - 1] + rd[er[tp[t] + ne - 1]] - ec[tp[t]];
-        const int64_t n = h->h_nrows[t] - h->h_top[t];
-        const int64_t st = ((n + 3) & ~3LL) * (((W + 15) & ~15LL) + 1);
-        int c = 3;
-        for (int k = 2; k >= 0; k--)
-            if (st <= caps[k]) c = k;
-        cls[c].push_back((int32_t)t);
-    }
-    std::vector<RGTab> tabs;
+    // one record per table with entries, ordered into the launch classes
     std::vector<int64_t> eoff(er.size() + 1, 0);
     for (size_t u = 0; u < er.size(); u++) eoff[u + 1] = eoff[u] + rd[er[u]];
-    for (int c = 0; c < 4; c++) {
-        h->rg_cls_off[c] = (int64_t)tabs.size();
-        for (int32_t t : cls[c]) {
-            const int u0 = tp[t], ne = tp[t + 1] - u0;
-            RGTab g;
-            g.raw_off = h->h_raw_off[t];
-            g.res_off = h->h_res_off[t];
-            g.goff = goff[t];
-            g.eoff = eoff[u0];
-            g.M = h->h_nrows[t];
-            g.top = h->h_top[t];
-            g.n = g.M - g.top;
-            g.u0 = u0;
-            g.ne = ne;
-            g.c0 = ec[u0];
-            g.W = ec[u0 + ne - 1] + rd[er[u0 + ne - 1]] - ec[u0];
-            g.pad = 0;
-            tabs.push_back(g);
-        }
+    std::vector<RGTab> recs, tabs;
+    for (int64_t t = 0; t < nt; t++) {
+        const int u0 = tp[t], ne = tp[t + 1] - u0;
+        if (!ne) continue;
+        RGTab g;
+        g.raw_off = h->h_raw_off[t];
+        g.res_off = h->h_res_off[t];
+        g.goff = goff[t];
+        g.eoff = eoff[u0];
+        g.M = h->h_nrows[t];
+        g.top = h->h_top[t];
+        g.n = g.M - g.top;
+        g.u0 = u0;
+        g.ne = ne;
+        g.c0 = ec[u0];
+        g.W = ec[u0 + ne - 1] + rd[er[u0 + ne - 1]] - ec[u0];
+        g.pad = 0;
+        recs.push_back(g);
     }
-    h->rg_cls_off[4] = (int64_t)tabs.size();
+    int rc0;
+    if ((rc0 = rg_classify(recs, tabs, h->rg))) return rc0;
     std::vector<int32_t> ecd(er.size());
     for (int64_t t = 0; t < nt; t++)
         for (int u = tp[t]; u < tp[t + 1]; u++) ecd[u] = ((ec[u] - ec[tp[t]]) << 16) | rd[er[u]];
-    int rc0;
     if ((rc0 = h->d_rg_tabs.upload(tabs)) || (rc0 = h->d_rg_ecd.upload(ecd))) return rc0;
+    h->rg.tabs = h->d_rg_tabs.ptr;
+    h->rg.ecd = h->d_rg_ecd.ptr;
     std::vector<RGBlock> blocks;
     if (dense) {
         blocks.reserve((size_t)(nrb * (nrb + 1) / 2));
@@ -1342,35 +1036,7 @@ template <class T> int reduced_normal(bls_qr *h, int dense, T *S, double *g, con
     const int m = dense ? 1 : 0;
     if (!h->rg_built[m] && (rc = build_gram_plan(h, dense))) return rc;
     if (dense && h->nr) k_rg_fill<T><<<std::min<int64_t>(grid_for(h->nr * h->nr, 256), 4096), 256, 0, s>>>(S, h->nr * h->nr);
-    {
-        const int caps[4] = {RG_CLASS_CAP0, RG_CLASS_CAP1, h->rg_cap2, 0};
-        for (int c = 0; c < 4; c++) {
-            const int64_t cnt = h->rg_cls_off[c + 1] - h->rg_cls_off[c];
-            if (!cnt) continue;
-            const RGTab *tabs = h->d_rg_tabs.ptr + h->rg_cls_off[c];
-            const size_t lds = (size_t)caps[c] * sizeof(double);
-            const int nt = c == 2 ? 1024 : c == 1 ? 512 : 256;
-            // persistent grid: as many workgroups as the compute units hold at once (LDS and 32 waves per unit)
-            const int64_t per_cu =
-                std::max<int64_t>(1, std::min<int64_t>(h->rg_lds_cu / (int64_t)(lds + RG_STATIC_LDS), 32 / (nt / 64)));
-            const unsigned grid = (unsigned)std::min<int64_t>(cnt, per_cu * std::max(1, h->rg_ncu));
-            const int32_t *ecd = h->d_rg_ecd.ptr;
-            double *Gp = h->d_rg_G.ptr, *gpp = h->d_gp.ptr;
-            if (c == 0 || c == 3) {
-                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 256, 16>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 256, 16><<<grid, 256, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
-            } else if (c == 1) {
-                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 512, 19>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 512, 19><<<grid, 512, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
-            } else {
-                BLS_HIP(hipFuncSetAttribute((const void *)k_red_gram<T, 1024, 19>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                k_red_gram<T, 1024, 19><<<grid, 1024, lds, s>>>(tabs, cnt, ecd, caps[c], F, Fr, Gp, gpp);
-            }
-        }
-    }
+    if ((rc = rg_launch<T>(h->rg, F, Fr, h->d_rg_G.ptr, h->d_gp.ptr, s))) return rc;
     if (h->rg_nblk[m])
         k_red_gram_sum<T><<<(unsigned)h->rg_nblk[m], 256, 0, s>>>(h->rg_nblk[m], h->d_rg_blocks[m].ptr,
                                                                             h->d_rg_ptr[m].ptr, h->d_rg_contrib[m].ptr,

@@ -78,6 +78,11 @@ SIGNATURES = {
     "bls_qr_backsubstitute_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_qr_backsubstitute_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_diag_create": (ct.c_int, [ct.c_int64, P64, ct.POINTER(VP)]),
+    "bls_asm_create": (ct.c_int, [ct.c_int64, ct.c_int64, P64, P64, P64, P64, P64, P64, P64, P64, P64, P64, P64,
+                                  ct.c_int64, P64, P64, P64, ct.POINTER(VP)]),
+    "bls_asm_destroy": (None, [VP]),
+    "bls_asm_lengths": (ct.c_int, [VP, P64]),
+    "bls_asm_hessian_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_diag_destroy": (None, [VP]),
     "bls_factor_diagonal_blocks_f64": (ct.c_int, [VP, VP, P64, VP]),
     "bls_factor_diagonal_blocks_f32": (ct.c_int, [VP, VP, P64, VP]),
@@ -518,6 +523,46 @@ class DiagonalBlocks:
         if getattr(self, "h", None) and _lib is not None:
             _lib.bls_diag_destroy(self.h)
             self.h = None
+
+
+class AssemblyHandle:
+    """The fill pattern of a Schur Hessian's addblock! calls for a batch of row blocks (the C ABI's bls_asm,
+    src/schur_assembly.jl:140-246). st: a SchurStructure; rowblocks: per row block, its 1-based layout block numbers
+    in its Jacobian's column order; rb_rows: its row counts; R: dense (R_layout None) or the stored lower blocks of a
+    SparseBlockLinearSystem as (colptr, rowval, offsets), 1-based."""
+
+    def __init__(self, st, rowblocks, rb_rows, R_layout=None):
+        keep = []
+        h = VP()
+        i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
+        rb_ptr = i64(np.concatenate([[1], 1 + np.cumsum([len(b) for b in rowblocks])]))
+        rb_blocks = i64(np.concatenate([np.asarray(b, dtype=np.int64) for b in rowblocks]) if rowblocks else [])
+        rb_rows = i64(rb_rows)
+        rc_, rr_, ro_ = (None, None, None) if R_layout is None else (i64(x) for x in R_layout)
+        p = lambda a: _p64(a, keep) if a is not None else None
+        _check(lib().bls_asm_create(st.nrblocks, st.neblocks, p(i64(st.r_dims)), p(i64(st.e_dims)),
+                                    p(i64(st.boffsets_r)), p(i64(st.boffsets_e)), p(i64(st.eoffsets)),
+                                    p(i64(st.eliminated_ptr)), p(i64(st.entry_retained)), p(i64(st.coffsets)),
+                                    p(rc_), p(rr_), p(ro_), len(rowblocks), p(rb_ptr), p(rb_blocks), p(rb_rows),
+                                    ct.byref(h)))
+        self.h = h
+        lens = np.zeros(4, dtype=np.int64)
+        _check(lib().bls_asm_lengths(h, p(lens)))
+        self.E_len, self.C_len, self.R_len, self.nr = (int(x) for x in lens)
+        self.dense = R_layout is None
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_asm_destroy(self.h)
+            self.h = None
+
+
+def assemble_hessian_(E_data, C_data, R, b_r, b_e, handle, J, r, stream=None):
+    """H = sum_k J_k' J_k into E_data, C_data, R and b = sum_k J_k' r_k into b_r, b_e (Float64 device tensors):
+    the addblock! calls of the handle's pattern on a zero Hessian, in row-block order."""
+    _check(lib().bls_asm_hessian_f64(handle.h, _dp(J), _dp(r), _dp(E_data), _dp(C_data), _dp(R), _dp(b_r), _dp(b_e),
+                                     _stream(stream)))
+    return True
 
 
 def factor_diagonal_blocks_(D, blocks, stream=None):

@@ -341,6 +341,37 @@ int bls_mlp_grad_f32(const float *params, const float *fmean, const float *fstd,
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected for step >= 1) on the parameters. */
 int bls_mlp_adam_f32(float *params, const float *grad, float *m, float *v, int64_t step, float lr, void *stream);
 
+/* ---- assembly of a Schur Hessian from row blocks (src/schur_assembly.jl:140-246) --------------------------- */
+
+typedef struct bls_asm bls_asm;   /* the plan of one fill pattern */
+
+/* Register the fill pattern that addblock!(H, i, j, J_k'J_k, rows_i, cols_j) (src/schur_assembly.jl:140-159) would
+ * follow for a batch of row blocks, every pair of blocks of each row block, row block by row block. Structure: the
+ * SchurStructure fields as for bls_schur_create. R: r_colptr == NULL for a dense nr x nr column-major Matrix (upper
+ * triangle and full diagonal blocks written, as add_retained_diagonal! / add_retained_pair! do, :189-223), else a
+ * SparseBlockLinearSystem's stored lower blocks (src/SparseBlockLinearSystem.jl:11-64): r_colptr (nrblocks + 1),
+ * r_rowval (the block rows >= each block column, ascending), r_offsets (start of each block in R.data, column-major
+ * d_row x d_col), all 1-based. Row blocks: rb_ptr (nrowblocks + 1) into rb_blocks, the layout block numbers
+ * (1-based: retained 1..nrblocks, eliminated nrblocks + e, as addblock! numbers them) in the column order of the
+ * row block's Jacobian, and rb_rows its row count. A pair the system has no storage for is BLS_ERR_ARG with the
+ * reference's reason (check_addblock, :161-181); a row block over more than 64 blocks or 1024 columns is
+ * BLS_ERR_UNSUPPORTED. */
+int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t *r_dims, const int64_t *e_dims,
+                   const int64_t *boffsets_r, const int64_t *boffsets_e, const int64_t *eoffsets,
+                   const int64_t *eliminated_ptr, const int64_t *entry_retained, const int64_t *coffsets,
+                   const int64_t *r_colptr, const int64_t *r_rowval, const int64_t *r_offsets, int64_t nrowblocks,
+                   const int64_t *rb_ptr, const int64_t *rb_blocks, const int64_t *rb_rows, bls_asm **out);
+void bls_asm_destroy(bls_asm *a);
+/* Lengths the arrays of bls_asm_hessian_f64 need: E_data, C_data, R, and nr. */
+int bls_asm_lengths(const bls_asm *a, int64_t *lengths /* 4 */);
+/* H = sum_k J_k' J_k into E_data, C_data and R, and b = sum_k J_k' r_k into b_r and b_e: what a SchurBlockHessian
+ * that starts at zero holds after the addblock! calls of the registered pattern, made in row-block order (each
+ * stored entry summed over the row blocks in that order; bitwise reproducible). Every stored block is written
+ * (zero where no row block touches it; a dense R is zeroed first). J: the row blocks one after another, each
+ * rb_rows x (sum of its block dims) column-major; r: their residual rows one after another. */
+int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r, double *E_data, double *C_data, double *R,
+                        double *b_r, double *b_e, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -171,3 +171,54 @@ def max_block_rel_to(A, B, Mag):
         d = np.linalg.norm(A[k] - b)
         worst = max(worst, d / nb if nb > 0 else d)
     return worst
+
+
+def assembly_problem(blsamd, seed, ncam=6, nland=40, kmax=4, npairs=4, sparse=False, tall=()):
+    """A description with observations (2 rows over [landmark; camera], the landmark listed first or second at
+    random), camera priors and camera-pair rows; its SchurStructure, the row blocks as layout block numbers in each
+    Jacobian's column order, random J and r, and (sparse) the stored lower blocks of R: every retained pair that
+    shares a row block plus the diagonal, as a block CSC with 1-based data offsets. tall: row counts of extra
+    row blocks over cameras 1 and 2 (which then have 9 parameters each)."""
+    rng = np.random.default_rng(seed)
+    cam = np.where(rng.random(ncam) < 0.5, 6, 9)
+    if tall:
+        cam[:2] = 9
+    blocksizes = np.concatenate([cam, np.full(nland, 3)])
+    eliminated = np.concatenate([np.zeros(ncam, bool), np.ones(nland, bool)])
+    rows, sizes = [], []
+    for l in range(nland):
+        for c in rng.choice(ncam, size=int(rng.integers(2, kmax + 1)), replace=False):
+            rb = [ncam + l + 1, int(c) + 1]
+            rows.append(rb if rng.random() < 0.5 else rb[::-1]); sizes.append(2)
+    for c in range(ncam):
+        rows.append([c + 1]); sizes.append(int(cam[c]))
+    for _ in range(npairs):
+        a, b = rng.choice(ncam, size=2, replace=False)
+        rows.append([int(a) + 1, int(b) + 1]); sizes.append(3)
+    for m in tall:
+        rows.append([2, 1]); sizes.append(int(m))
+    st = blsamd.structure.SchurStructure.from_description(blocksizes, [sorted(r) for r in rows], eliminated)
+    layout = np.argsort(np.asarray(st.blockorder()) - 1) + 1     # invperm: layout number of each caller block
+    rowblocks = [[int(layout[b - 1]) for b in r] for r in rows]
+    dims = list(st.r_dims) + list(st.e_dims)
+    W = [sum(int(dims[b - 1]) for b in r) for r in rowblocks]
+    J = rng.standard_normal(int(sum(m * w for m, w in zip(sizes, W))))
+    r = rng.standard_normal(int(sum(sizes)))
+    R_layout = None
+    if sparse:
+        nrb = st.nrblocks
+        pairs = {(q, q) for q in range(1, nrb + 1)}
+        for rb in rowblocks:
+            ret = [b for b in rb if b <= nrb]
+            for a in ret:
+                for b in ret:
+                    if a >= b:
+                        pairs.add((a, b))
+        colptr, rowval, offs, off = [1], [], [], 1
+        rd = st.r_dims
+        for q in range(1, nrb + 1):
+            for p_ in sorted(p for (p, qq) in pairs if qq == q):
+                rowval.append(p_); offs.append(off); off += int(rd[p_ - 1] * rd[q - 1])
+            colptr.append(len(rowval) + 1)
+        R_layout = (np.array(colptr), np.array(rowval), np.array(offs))
+    return st, rowblocks, np.array(sizes), J, r, R_layout
