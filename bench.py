@@ -168,6 +168,7 @@ def main():
     log(f"kernels: reset {kt[0]:.4f} ms, phase1 {kt[1]:.4f} ms ({p1_bytes / (kt[1] * 1e-3) / 1e9:.0f} GB/s), "
         f"phase2 {kt[2]:.4f} ms ({p2_gbs:.0f} GB/s), merge {kt[3]:.4f} ms")
     mixed = mixed_leg(torch, dev, p, h, S, csc, dmap, args.steps, args.warmup)
+    assembly = assembly_leg(torch, dev, st)
     pcie = pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, max(3, args.steps // 4))
     log(f"PCIe-inclusive (host E, C, b, R in; S, g, CSC out): {pcie['ms_per_step']:.3f} ms/step, "
         f"{pcie['value']:.4g} landmarks/s")
@@ -186,6 +187,7 @@ def main():
                      "step_achieved": whole_bytes / (ms * 1e-3) / 1e9},
         "pcie_inclusive": pcie,
         "mixed": mixed,
+        "assembly": assembly,
     }
     del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
     torch.cuda.empty_cache()
@@ -197,6 +199,54 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)
     print(json.dumps(result), flush=True)
+
+
+def assembly_leg(torch, dev, st, reps=10):
+    """The SLAM workload's Schur Hessian assembled on the device from its row blocks (batched addblock!,
+    src/schur_assembly.jl:140-246): one 2-row observation block per coupling ([landmark; camera], J 2 x 9) and a
+    6-row prior per camera, R = the camera blocks; J, r ~ N(0, 1). HBM figure: J and r read once, E_data, C_data,
+    R and b written once."""
+    t0 = time.time()
+    nrb, neb = int(st.nrblocks), int(st.neblocks)
+    ep = np.asarray(st.eliminated_ptr) - 1
+    er = np.asarray(st.entry_retained, dtype=np.int64)
+    rd = np.asarray(st.r_dims, dtype=np.int64)
+    ed = np.asarray(st.e_dims, dtype=np.int64)
+    nobs = len(er)
+    e_of = np.repeat(np.arange(neb, dtype=np.int64), np.diff(ep))
+    rb_ptr = np.concatenate([1 + 2 * np.arange(nobs + 1, dtype=np.int64), 2 * nobs + 2 + np.arange(nrb, dtype=np.int64)])
+    rb_blocks = np.concatenate([np.stack([nrb + e_of + 1, er], axis=1).reshape(-1), np.arange(1, nrb + 1, dtype=np.int64)])
+    rb_rows = np.concatenate([np.full(nobs, 2, dtype=np.int64), rd])
+    roff = np.concatenate([[1], 1 + np.cumsum(rd * rd)])[:-1]
+    R_layout = (np.arange(1, nrb + 2), np.arange(1, nrb + 1), roff)
+    h = dev.AssemblyHandle(st, (rb_ptr, rb_blocks), rb_rows, R_layout)
+    nJ = int(np.sum(2 * (ed[e_of] + rd[er - 1])) + np.sum(rd * rd))
+    nres = int(2 * nobs + np.sum(rd))
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    J = torch.randn(nJ, dtype=torch.float64, device="cuda", generator=gen)
+    r = torch.randn(nres, dtype=torch.float64, device="cuda", generator=gen)
+    E = torch.empty(h.E_len, dtype=torch.float64, device="cuda")
+    C = torch.empty(h.C_len, dtype=torch.float64, device="cuda")
+    R = torch.empty(h.R_len, dtype=torch.float64, device="cuda")
+    br = torch.empty(h.nr, dtype=torch.float64, device="cuda")
+    be = torch.empty(int(ed.sum()), dtype=torch.float64, device="cuda")
+    setup_s = time.time() - t0
+    dev.assemble_hessian_(E, C, R, br, be, h, J, r)
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dev.assemble_hessian_(E, C, R, br, be, h, J, r)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    nbytes = 8.0 * (nJ + nres + h.E_len + h.C_len + h.R_len + h.nr + int(ed.sum()))
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"assembly (batched addblock!): {nobs} observation row blocks + {nrb} priors: {ms:.3f} ms, "
+        f"{gbs:.0f} GB/s of {nbytes / 1e9:.3f} GB (plan {setup_s:.1f} s)")
+    return {"row_blocks": int(nobs + nrb), "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "bytes": nbytes, "landmarks_per_s": neb / (ms * 1e-3)}
 
 
 def mixed_leg(torch, dev, p, h, S64, csc, dmap, steps, warmup):

@@ -18,6 +18,89 @@ using namespace bls;
 
 namespace {
 #include "bls_gram.h"
+
+constexpr int TINY_W = 16, TINY_M = 8;   // row blocks of at most 16 columns and 8 rows take the tiny path
+
+// Tiny row blocks (observations: 2 rows over a 3- and a 6-column block): 16 lanes per row block, lane c holding
+// column c of J_k. Column a is passed round by shuffle and lane b (>= a) forms element (a, b) of J_k'J_k, written
+// where the Gram pass writes it: pair block (e1, e2) of the row block's entries at Q[e2] + d2 P[e1], row-major, so
+// (a, b) -> goff + Q[e2] + (b - P[e2]) + d2 a (the columns of a row block are its entries' in order, from 0); and
+// lane c forms (J_k' r_k)[c] at eoff + c.
+template <class T>
+__global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs, int64_t cnt,
+                                                  const int32_t *__restrict__ ecd, const T *__restrict__ F,
+                                                  const double *__restrict__ Fr, double *__restrict__ G,
+                                                  double *__restrict__ gp) {
+    const int64_t t = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int l = threadIdx.x & 15, base = (threadIdx.x & 63) & ~15;
+    const bool valid = t < cnt;
+    const RGTab g = rg_load(tabs + (valid ? t : 0));
+    const int m = valid ? g.n : 0, W = valid ? g.W : 0;
+    double x[TINY_M];
+#pragma unroll
+    for (int i = 0; i < TINY_M; i++) x[i] = (l < W && i < m) ? (double)F[g.raw_off + (int64_t)l * m + i] : 0.0;
+    // the entry holding column l: its offset Q[e] + (l - P[e]) in the row block's pair blocks, and its dim
+    int qb = 0, db = 1;
+    if (l < W) {
+        int q = 0;
+        for (int e = 0; e < g.ne; e++) {
+            const int w = ecd[g.u0 + e], cst = w >> 16, d = w & 0xffff;
+            if (l >= cst && l < cst + d) {
+                qb = q + (l - cst);
+                db = d;
+            }
+            q += d * (cst + d);   // Q[e + 1] = Q[e] + d_e P[e + 1]
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < TINY_M; i++)
+            if (i < m) acc += x[i] * Fr[g.res_off + i];
+        gp[g.eoff + l] = acc;
+    }
+    for (int a = 0; a < TINY_W; a++) {
+        double xa[TINY_M];
+#pragma unroll
+        for (int i = 0; i < TINY_M; i++) xa[i] = __shfl(x[i], base + a, 64);
+        if (a < W && a <= l && l < W) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < TINY_M; i++) acc += xa[i] * x[i];
+            G[g.goff + qb + (int64_t)db * a] = acc;
+        }
+    }
+}
+
+// Destination blocks with few contributions (at most LIGHT_MAX): one wavefront per block, four per workgroup,
+// lanes over its entries, each summing the block's contributions in order (as k_red_gram_sum, without runs).
+constexpr int LIGHT_MAX = 32;
+template <class T>
+__global__ __launch_bounds__(256) void k_red_gram_sum_light(int64_t nblk, const RGBlock *__restrict__ blocks,
+                                                            const int64_t *__restrict__ ptr,
+                                                            const int64_t *__restrict__ contrib,
+                                                            const double *__restrict__ G, T *__restrict__ S) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= nblk) return;
+    const RGBlock bk = blocks[b];
+    const int sz = bk.d1 * bk.d2;
+    const int64_t cb = ptr[b], ce = ptr[b + 1];
+    const bool tr = bk.flags & RG_TRANSPOSED, full = bk.flags & RG_FULL;
+    for (int w = lane; w < sz; w += 64) {
+        const int row = w / bk.d2, col = w - row * bk.d2;
+        const bool lower = bk.diag && row > col;
+        double acc = 0.0;
+        if (!lower) {
+            const int wt = col * bk.d1 + row;
+            for (int64_t c = cb; c < ce; c++) {
+                const int64_t ci = contrib[c];
+                acc += G[ci >= 0 ? ci + w : ~ci + wt];
+            }
+        }
+        const int64_t at = tr ? bk.dst + (int64_t)row * bk.cstride + col : bk.dst + (int64_t)col * bk.cstride + row;
+        if (!(lower && full)) S[at] = (T)acc;
+        if (full && bk.diag && row < col) S[bk.dst + (int64_t)row * bk.cstride + col] = (T)acc;
+    }
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------------------
@@ -28,7 +111,8 @@ struct bls_asm {
     int64_t E_len = 0, C_len = 0, R_len = 0;
     bool dense = false;
     RGPlan rg;
-    DevArray<RGTab> d_tabs;
+    DevArray<RGTab> d_tabs, d_tiny;
+    int64_t ntiny = 0;
     DevArray<int32_t> d_ecd;
     int64_t glen = 0, gplen = 0;
     DevArray<double> d_G, d_gp;
@@ -37,6 +121,7 @@ struct bls_asm {
     DevArray<RGBlock> d_blocks[ND];
     DevArray<int64_t> d_ptr[ND], d_contrib[ND];
     int64_t nblk[ND] = {0, 0, 0, 0, 0};
+    bool light[ND] = {false, false, false, false, false};
 };
 
 namespace {
@@ -146,7 +231,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
     };
 
     // row blocks: records of the Gram pass, entry words, and each pair's destination
-    std::vector<RGTab> recs;
+    std::vector<RGTab> recs, tiny;
     std::vector<int32_t> ecd;
     int64_t joff = 0, roff = 0, goff = 0, eoff = 0;
     for (int64_t k = 0; k < nrowblocks; k++) {
@@ -187,7 +272,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
             g.c0 = 0;
             g.W = (int32_t)W;
             g.pad = 0;
-            recs.push_back(g);
+            (W <= TINY_W && m <= TINY_M ? tiny : recs).push_back(g);
             for (int64_t i = 0; i < ne; i++) ecd.push_back((int32_t)((col[i] << 16) | dim[blk[i]]));
             // pairs i1 <= i2 in column order, d1 x d2 row-major at goff + (running offset): the Gram pass's order
             int64_t off = goff;
@@ -233,7 +318,8 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
     std::vector<RGTab> tabs;
     int rc;
     if ((rc = rg_classify(recs, tabs, h->rg))) return rc;
-    if ((rc = h->d_tabs.upload(tabs)) || (rc = h->d_ecd.upload(ecd))) return rc;
+    if ((rc = h->d_tabs.upload(tabs)) || (rc = h->d_tiny.upload(tiny)) || (rc = h->d_ecd.upload(ecd))) return rc;
+    h->ntiny = (int64_t)tiny.size();
     h->rg.tabs = h->d_tabs.ptr;
     h->rg.ecd = h->d_ecd.ptr;
     if ((rc = h->d_G.alloc((size_t)std::max<int64_t>(1, goff))) || (rc = h->d_gp.alloc((size_t)std::max<int64_t>(1, eoff))))
@@ -244,11 +330,14 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
     for (int l = 0; l < bls_asm::ND; l++) {
         std::vector<RGBlock> blocks;
         std::vector<int64_t> ptr(1, 0), contrib;
+        size_t most = 0;
         for (const Dest &d : *lists[l]) {
             blocks.push_back(d.blk);
             contrib.insert(contrib.end(), d.contrib.begin(), d.contrib.end());
             ptr.push_back((int64_t)contrib.size());
+            most = std::max(most, d.contrib.size());
         }
+        h->light[l] = most <= (size_t)LIGHT_MAX;
         if ((rc = h->d_blocks[l].upload(blocks)) || (rc = h->d_ptr[l].upload(ptr)) || (rc = h->d_contrib[l].upload(contrib)))
             return rc;
         h->nblk[l] = (int64_t)blocks.size();
@@ -278,12 +367,19 @@ extern "C" int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r,
     if (a->dense && a->R_len)
         k_rg_fill<double><<<(unsigned)std::min<int64_t>((a->R_len + 255) / 256, 4096), 256, 0, s>>>(R, a->R_len);
     if ((rc = rg_launch<double>(a->rg, J, r, a->d_G.ptr, a->d_gp.ptr, s))) return rc;
+    if (a->ntiny)
+        k_asm_tiny<double><<<(unsigned)((a->ntiny + 15) / 16), 256, 0, s>>>(a->d_tiny.ptr, a->ntiny, a->d_ecd.ptr, J, r,
+                                                                          a->d_G.ptr, a->d_gp.ptr);
     double *dst[bls_asm::ND] = {E_data, C_data, R, b_r, b_e};
     for (int l = 0; l < bls_asm::ND; l++) {
         if (!a->nblk[l]) continue;
         const double *src = l < 3 ? a->d_G.ptr : a->d_gp.ptr;
-        k_red_gram_sum<double><<<(unsigned)a->nblk[l], 256, 0, s>>>(a->nblk[l], a->d_blocks[l].ptr, a->d_ptr[l].ptr,
-                                                                     a->d_contrib[l].ptr, src, dst[l]);
+        if (a->light[l])
+            k_red_gram_sum_light<double><<<(unsigned)((a->nblk[l] + 3) / 4), 256, 0, s>>>(
+                a->nblk[l], a->d_blocks[l].ptr, a->d_ptr[l].ptr, a->d_contrib[l].ptr, src, dst[l]);
+        else
+            k_red_gram_sum<double><<<(unsigned)a->nblk[l], 256, 0, s>>>(a->nblk[l], a->d_blocks[l].ptr, a->d_ptr[l].ptr,
+                                                                         a->d_contrib[l].ptr, src, dst[l]);
     }
     BLS_HIP(hipGetLastError());
     return BLS_OK;

@@ -528,15 +528,20 @@ class DiagonalBlocks:
 class AssemblyHandle:
     """The fill pattern of a Schur Hessian's addblock! calls for a batch of row blocks (the C ABI's bls_asm,
     src/schur_assembly.jl:140-246). st: a SchurStructure; rowblocks: per row block, its 1-based layout block numbers
-    in its Jacobian's column order; rb_rows: its row counts; R: dense (R_layout None) or the stored lower blocks of a
+    in its Jacobian's column order (or the pair (rb_ptr, rb_blocks) of that list in 1-based CSR); rb_rows: its row
+    counts; R: dense (R_layout None) or the stored lower blocks of a
     SparseBlockLinearSystem as (colptr, rowval, offsets), 1-based."""
 
     def __init__(self, st, rowblocks, rb_rows, R_layout=None):
         keep = []
         h = VP()
         i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
-        rb_ptr = i64(np.concatenate([[1], 1 + np.cumsum([len(b) for b in rowblocks])]))
-        rb_blocks = i64(np.concatenate([np.asarray(b, dtype=np.int64) for b in rowblocks]) if rowblocks else [])
+        if isinstance(rowblocks, tuple):   # (rb_ptr, rb_blocks): CSR, 1-based, as the C ABI takes them
+            rb_ptr, rb_blocks = i64(rowblocks[0]), i64(rowblocks[1])
+            rowblocks = range(len(rb_ptr) - 1)
+        else:
+            rb_ptr = i64(np.concatenate([[1], 1 + np.cumsum([len(b) for b in rowblocks])]))
+            rb_blocks = i64(np.concatenate([np.asarray(b, dtype=np.int64) for b in rowblocks]) if len(rowblocks) else [])
         rb_rows = i64(rb_rows)
         rc_, rr_, ro_ = (None, None, None) if R_layout is None else (i64(x) for x in R_layout)
         p = lambda a: _p64(a, keep) if a is not None else None
