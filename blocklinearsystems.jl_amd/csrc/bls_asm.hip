@@ -57,35 +57,44 @@ __global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs
             if (i < m) acc += x[i] * Fr[g.res_off + i];
         gp[g.eoff + l] = acc;
     }
-    for (int a = 0; a < TINY_W; a++) {
+    // the wavefront's widest and tallest row block bound the shuffles (uniform across the wavefront)
+    int wmax = W, mmax = m;
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+        wmax = max(wmax, __shfl_xor(wmax, o, 64));
+        mmax = max(mmax, __shfl_xor(mmax, o, 64));
+    }
+    for (int a = 0; a < wmax; a++) {
         double xa[TINY_M];
 #pragma unroll
-        for (int i = 0; i < TINY_M; i++) xa[i] = __shfl(x[i], base + a, 64);
+        for (int i = 0; i < TINY_M; i++) xa[i] = i < mmax ? __shfl(x[i], base + a, 64) : 0.0;
         if (a < W && a <= l && l < W) {
             double acc = 0.0;
 #pragma unroll
-            for (int i = 0; i < TINY_M; i++) acc += xa[i] * x[i];
+            for (int i = 0; i < TINY_M; i++)
+                if (i < mmax) acc += xa[i] * x[i];
             G[g.goff + qb + (int64_t)db * a] = acc;
         }
     }
 }
 
-// Destination blocks with few contributions (at most LIGHT_MAX): one wavefront per block, four per workgroup,
-// lanes over its entries, each summing the block's contributions in order (as k_red_gram_sum, without runs).
+// Destination blocks with few contributions (at most LIGHT_MAX): `slot` lanes per block (a power of two >= the
+// largest block of the launch, at most 64), 64 / slot blocks per wavefront; lane w of a block sums entry w (and
+// w + slot, ...) over the block's contributions in order (as k_red_gram_sum, without runs).
 constexpr int LIGHT_MAX = 32;
 template <class T>
-__global__ __launch_bounds__(256) void k_red_gram_sum_light(int64_t nblk, const RGBlock *__restrict__ blocks,
+__global__ __launch_bounds__(256) void k_red_gram_sum_light(int64_t nblk, int slot, const RGBlock *__restrict__ blocks,
                                                             const int64_t *__restrict__ ptr,
                                                             const int64_t *__restrict__ contrib,
                                                             const double *__restrict__ G, T *__restrict__ S) {
-    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (b >= nblk) return;
+    const int64_t b = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / slot) + lane / slot;
+    if (b >= nblk) return;   // no synchronisation below
     const RGBlock bk = blocks[b];
     const int sz = bk.d1 * bk.d2;
     const int64_t cb = ptr[b], ce = ptr[b + 1];
     const bool tr = bk.flags & RG_TRANSPOSED, full = bk.flags & RG_FULL;
-    for (int w = lane; w < sz; w += 64) {
+    for (int w = lane % slot; w < sz; w += slot) {
         const int row = w / bk.d2, col = w - row * bk.d2;
         const bool lower = bk.diag && row > col;
         double acc = 0.0;
@@ -121,7 +130,8 @@ struct bls_asm {
     DevArray<RGBlock> d_blocks[ND];
     DevArray<int64_t> d_ptr[ND], d_contrib[ND];
     int64_t nblk[ND] = {0, 0, 0, 0, 0};
-    bool light[ND] = {false, false, false, false, false};
+    int64_t nlight[ND] = {0, 0, 0, 0, 0};   // blocks [0, nlight) have at most LIGHT_MAX contributions
+    int slot[ND] = {1, 1, 1, 1, 1};
 };
 
 namespace {
@@ -328,16 +338,24 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
     h->gplen = eoff;
     std::vector<Dest> *lists[bls_asm::ND] = {&dE, &dC, &dR, &dbr, &dbe};
     for (int l = 0; l < bls_asm::ND; l++) {
+        // light blocks first, then the rest; each keeps its contributions in row-block order
         std::vector<RGBlock> blocks;
         std::vector<int64_t> ptr(1, 0), contrib;
-        size_t most = 0;
-        for (const Dest &d : *lists[l]) {
-            blocks.push_back(d.blk);
-            contrib.insert(contrib.end(), d.contrib.begin(), d.contrib.end());
-            ptr.push_back((int64_t)contrib.size());
-            most = std::max(most, d.contrib.size());
-        }
-        h->light[l] = most <= (size_t)LIGHT_MAX;
+        int maxsz = 1;
+        for (int pass = 0; pass < 2; pass++)
+            for (const Dest &d : *lists[l]) {
+                if ((d.contrib.size() <= (size_t)LIGHT_MAX) != (pass == 0)) continue;
+                blocks.push_back(d.blk);
+                contrib.insert(contrib.end(), d.contrib.begin(), d.contrib.end());
+                ptr.push_back((int64_t)contrib.size());
+                if (pass == 0) maxsz = std::max(maxsz, d.blk.d1 * d.blk.d2);
+            }
+        int64_t nl = 0;
+        for (const Dest &d : *lists[l]) nl += d.contrib.size() <= (size_t)LIGHT_MAX;
+        h->nlight[l] = nl;
+        int slot = 1;
+        while (slot < maxsz && slot < 64) slot <<= 1;
+        h->slot[l] = slot;
         if ((rc = h->d_blocks[l].upload(blocks)) || (rc = h->d_ptr[l].upload(ptr)) || (rc = h->d_contrib[l].upload(contrib)))
             return rc;
         h->nblk[l] = (int64_t)blocks.size();
@@ -374,12 +392,15 @@ extern "C" int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r,
     for (int l = 0; l < bls_asm::ND; l++) {
         if (!a->nblk[l]) continue;
         const double *src = l < 3 ? a->d_G.ptr : a->d_gp.ptr;
-        if (a->light[l])
-            k_red_gram_sum_light<double><<<(unsigned)((a->nblk[l] + 3) / 4), 256, 0, s>>>(
-                a->nblk[l], a->d_blocks[l].ptr, a->d_ptr[l].ptr, a->d_contrib[l].ptr, src, dst[l]);
-        else
-            k_red_gram_sum<double><<<(unsigned)a->nblk[l], 256, 0, s>>>(a->nblk[l], a->d_blocks[l].ptr, a->d_ptr[l].ptr,
-                                                                         a->d_contrib[l].ptr, src, dst[l]);
+        const int64_t nl = a->nlight[l], nh = a->nblk[l] - nl;
+        if (nl) {
+            const int64_t per_wg = 4 * (64 / a->slot[l]);
+            k_red_gram_sum_light<double><<<(unsigned)((nl + per_wg - 1) / per_wg), 256, 0, s>>>(
+                nl, a->slot[l], a->d_blocks[l].ptr, a->d_ptr[l].ptr, a->d_contrib[l].ptr, src, dst[l]);
+        }
+        if (nh)
+            k_red_gram_sum<double><<<(unsigned)nh, 256, 0, s>>>(nh, a->d_blocks[l].ptr + nl, a->d_ptr[l].ptr + nl,
+                                                                 a->d_contrib[l].ptr, src, dst[l]);
     }
     BLS_HIP(hipGetLastError());
     return BLS_OK;
