@@ -2,10 +2,12 @@
 // (src/schur_assembly.jl:140-246) behind the C ABI of include/bls_amd.h.
 //
 // A row block k (observation rows touching some layout blocks, in the column order of its Jacobian J_k) contributes
-// J_k' J_k to every pair of its blocks and J_k' r_k to each block. Each row block is a table of the Gram pass
-// (bls_gram.h): one read of J_k forms every pair block on the FP64 matrix instruction into a contribution buffer, and
-// every stored block of E_data, C_data and R -- and each block of b_r and b_e -- is then the sum of its
-// contributions in ascending row-block order, the order in which addblock! calls made in row-block order add them.
+// J_k' J_k to every pair of its blocks and J_k' r_k to each block. One read of J_k forms every pair block into a
+// contribution buffer -- by 16 lanes and shuffles for a row block of at most 16 columns and 8 rows, else as a table
+// of the Gram pass (bls_gram.h) on the FP64 matrix instruction -- and every stored block of E_data, C_data and R, and
+// each block of b_r and b_e, is then the sum of its contributions in ascending row-block order, the order in which
+// addblock! calls made in row-block order add them (blocks with few contributions several per wavefront, the others
+// by runs over a workgroup).
 // No atomics; bitwise reproducible. A pair's contribution is read in the orientation its destination stores
 // (add_retained_diagonal!, add_retained_pair!, add_eliminated!, add_coupling!, src/schur_assembly.jl:189-246).
 #include <algorithm>
