@@ -245,8 +245,27 @@ def assembly_leg(torch, dev, st, reps=10):
     gbs = nbytes / (ms * 1e-3) / 1e9
     log(f"assembly (batched addblock!): {nobs} observation row blocks + {nrb} priors: {ms:.3f} ms, "
         f"{gbs:.0f} GB/s of {nbytes / 1e9:.3f} GB (plan {setup_s:.1f} s)")
+    # dot(x, H, y) over the storage just filled: E_data, C_data, R read once, x and y read once
+    n = h.nr + int(ed.sum())
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=gen)
+    y = torch.randn(n, dtype=torch.float64, device="cuda", generator=gen)
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    dev.hessian_dot(x, h, y, E, C, R)
+    dtimes = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dev.lib().bls_asm_dot_f64(h.h, dev._dp(x), dev._dp(y), dev._dp(E), dev._dp(C), dev._dp(R), dev._dp(out),
+                                  dev._stream(None))
+        e1.record()
+        torch.cuda.synchronize()
+        dtimes.append(e0.elapsed_time(e1))
+    dms = float(np.median(dtimes))
+    dbytes = 8.0 * (h.E_len + h.C_len + h.R_len + 2 * n)
+    log(f"dot(x, H, y) on that storage: {dms:.4f} ms, {dbytes / (dms * 1e-3) / 1e9:.0f} GB/s")
     return {"row_blocks": int(nobs + nrb), "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
-            "bytes": nbytes, "landmarks_per_s": neb / (ms * 1e-3)}
+            "bytes": nbytes, "landmarks_per_s": neb / (ms * 1e-3),
+            "dot": {"ms": dms, "achieved_GBs": dbytes / (dms * 1e-3) / 1e9, "bytes": dbytes}}
 
 
 def mixed_leg(torch, dev, p, h, S64, csc, dmap, steps, warmup):

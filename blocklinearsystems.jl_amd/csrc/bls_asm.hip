@@ -112,6 +112,84 @@ __global__ __launch_bounds__(256) void k_red_gram_sum_light(int64_t nblk, int sl
         if (full && bk.diag && row < col) S[bk.dst + (int64_t)row * bk.cstride + col] = (T)acc;
     }
 }
+
+// dot(x, H, y) (src/linalg.jl:133-260), x and y laid out [x_r; x_e]. Partials in a fixed order, then one sum.
+struct HDotArgs {
+    int64_t nr, neb, nrblk;
+    const int64_t *eoff, *boff_e, *cptr, *coff, *boff_r;   // 0-based
+    const int32_t *de, *c_r, *rd;
+    const RGBlock *rblk;                                    // the stored blocks of R (dst, column stride, dims)
+    const int32_t *rp, *rq;                                 // their block row and column
+    int dense;
+};
+
+// one lane per eliminated block: x_e' E y_e + sum over its couplings and rows ir of x_r[ir] (c_ir . y_e) +
+// y_r[ir] (c_ir . x_e), c_ir column ir of C_er (schur_bilinear_eliminated, src/linalg.jl:208-260)
+__global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__restrict__ x, const double *__restrict__ y,
+                                                const double *__restrict__ E, const double *__restrict__ C,
+                                                double *__restrict__ part) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.neb) return;
+    const int d = a.de[e];
+    const double *xe = x + a.nr + a.boff_e[e], *ye = y + a.nr + a.boff_e[e];
+    const double *Ee = E + a.eoff[e];
+    double acc = 0.0;
+    for (int col = 0; col < d; col++) {
+        const double yc = ye[col];
+        for (int row = 0; row < d; row++) acc += Ee[col * d + row] * xe[row] * yc;
+    }
+    for (int64_t c = a.cptr[e]; c < a.cptr[e + 1]; c++) {
+        const int r = a.c_r[c], dr = a.rd[r];
+        const double *xr = x + a.boff_r[r], *yr = y + a.boff_r[r], *Cc = C + a.coff[c];
+        for (int ir = 0; ir < dr; ir++) {
+            double cx = 0.0, cy = 0.0;
+            for (int i = 0; i < d; i++) {
+                const double cv = Cc[ir * d + i];
+                cx += cv * xe[i];
+                cy += cv * ye[i];
+            }
+            acc += xr[ir] * cy + yr[ir] * cx;
+        }
+    }
+    part[e] = acc;
+}
+
+// one lane per stored block of R (src/linalg.jl:153-190): a dense R is read in its upper triangle (each
+// off-diagonal entry counted both ways, a diagonal block by its upper triangle); a sparse R's blocks whole, an
+// off-diagonal block both ways
+__global__ __launch_bounds__(256) void k_hdot_r(HDotArgs a, const double *__restrict__ x, const double *__restrict__ y,
+                                                const double *__restrict__ R, double *__restrict__ part) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.nrblk) return;
+    const RGBlock bk = a.rblk[b];
+    const int p = a.rp[b], q = a.rq[b];
+    const double *xp = x + a.boff_r[p], *yp = y + a.boff_r[p], *xq = x + a.boff_r[q], *yq = y + a.boff_r[q];
+    const double *B = R + bk.dst;
+    double acc = 0.0;
+    for (int j = 0; j < bk.d2; j++)
+        for (int i = 0; i < bk.d1; i++) {
+            const double v = B[(int64_t)j * bk.cstride + i];
+            if (p != q) acc += v * (xp[i] * yq[j] + yp[i] * xq[j]);
+            else if (!a.dense) acc += v * xp[i] * yp[j];
+            else if (i < j) acc += v * (xp[i] * yp[j] + xp[j] * yp[i]);
+            else if (i == j) acc += v * xp[i] * yp[i];
+        }
+    part[a.neb + b] = acc;
+}
+
+// the partials added in one fixed order: thread t takes t, t + 1024, ...; then a fixed tree
+__global__ __launch_bounds__(1024) void k_hdot_sum(int64_t n, const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double red[1024];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 512; o >= 1; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------------------
@@ -134,6 +212,12 @@ struct bls_asm {
     int64_t nblk[ND] = {0, 0, 0, 0, 0};
     int64_t nlight[ND] = {0, 0, 0, 0, 0};   // blocks [0, nlight) have at most LIGHT_MAX contributions
     int slot[ND] = {1, 1, 1, 1, 1};
+    // dot(x, H, y): the storage layout
+    DevArray<int64_t> d_eoff, d_boff_e, d_cptr, d_coff, d_boff_r;
+    DevArray<int32_t> d_de, d_c_r, d_rd, d_rp, d_rq;
+    DevArray<RGBlock> d_rblk;
+    DevArray<double> d_part;
+    int64_t nrblk = 0;
 };
 
 namespace {
@@ -338,6 +422,48 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         return rc;
     h->glen = goff;
     h->gplen = eoff;
+    {   // the storage layout for dot(x, H, y)
+        std::vector<int64_t> eo(neb), be(neb), cp(neb + 1, 0), co(ncoup), br(nrb);
+        std::vector<int32_t> dd(neb), cr(ncoup), rdv(nrb), rp, rq;
+        std::vector<RGBlock> rb;
+        for (int64_t e = 0; e < neb; e++) {
+            eo[e] = eoffsets[e] - 1;
+            be[e] = boff[nrb + e];
+            dd[e] = (int32_t)dim[nrb + e];
+            cp[e + 1] = eliminated_ptr[e + 1] - 1;
+        }
+        for (int64_t c = 0; c < ncoup; c++) {
+            co[c] = coffsets[c] - 1;
+            cr[c] = (int32_t)(entry_retained[c] - 1);
+        }
+        for (int64_t r = 0; r < nrb; r++) {
+            br[r] = boff[r];
+            rdv[r] = (int32_t)dim[r];
+        }
+        if (h->dense) {
+            for (int64_t q = 0; q < nrb; q++)
+                for (int64_t p = 0; p <= q; p++) {
+                    rb.push_back(dR[q * (q + 1) / 2 + p].blk);
+                    rp.push_back((int32_t)p);
+                    rq.push_back((int32_t)q);
+                }
+        } else {
+            for (int64_t q = 0; q < nrb; q++)
+                for (int64_t k = r_colptr[q] - 1; k < r_colptr[q + 1] - 1; k++) {
+                    rb.push_back(dR[rcol[q] + (k - (r_colptr[q] - 1))].blk);
+                    rp.push_back((int32_t)(r_rowval[k] - 1));
+                    rq.push_back((int32_t)q);
+                }
+        }
+        int rc1;
+        if ((rc1 = h->d_eoff.upload(eo)) || (rc1 = h->d_boff_e.upload(be)) || (rc1 = h->d_cptr.upload(cp)) ||
+            (rc1 = h->d_coff.upload(co)) || (rc1 = h->d_boff_r.upload(br)) || (rc1 = h->d_de.upload(dd)) ||
+            (rc1 = h->d_c_r.upload(cr)) || (rc1 = h->d_rd.upload(rdv)) || (rc1 = h->d_rp.upload(rp)) ||
+            (rc1 = h->d_rq.upload(rq)) || (rc1 = h->d_rblk.upload(rb)) ||
+            (rc1 = h->d_part.alloc((size_t)std::max<int64_t>(1, neb + (int64_t)rb.size()))))
+            return rc1;
+        h->nrblk = (int64_t)rb.size();
+    }
     std::vector<Dest> *lists[bls_asm::ND] = {&dE, &dC, &dR, &dbr, &dbe};
     for (int l = 0; l < bls_asm::ND; l++) {
         // light blocks first, then the rest; each keeps its contributions in row-block order
@@ -404,6 +530,35 @@ extern "C" int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r,
             k_red_gram_sum<double><<<(unsigned)nh, 256, 0, s>>>(nh, a->d_blocks[l].ptr + nl, a->d_ptr[l].ptr + nl,
                                                                  a->d_contrib[l].ptr, src, dst[l]);
     }
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
+                               const double *R, double *result, void *stream) {
+    if (!a) return fail(BLS_ERR_ARG, "null handle");
+    if (!x || !y || !result || (a->neb && !E_data) || (a->ncoup && !C_data) || (a->nrblk && !R))
+        return fail(BLS_ERR_ARG, "bls_asm_dot_f64: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    HDotArgs d{};
+    d.nr = a->nr;
+    d.neb = a->neb;
+    d.nrblk = a->nrblk;
+    d.eoff = a->d_eoff.ptr;
+    d.boff_e = a->d_boff_e.ptr;
+    d.cptr = a->d_cptr.ptr;
+    d.coff = a->d_coff.ptr;
+    d.boff_r = a->d_boff_r.ptr;
+    d.de = a->d_de.ptr;
+    d.c_r = a->d_c_r.ptr;
+    d.rd = a->d_rd.ptr;
+    d.rblk = a->d_rblk.ptr;
+    d.rp = a->d_rp.ptr;
+    d.rq = a->d_rq.ptr;
+    d.dense = a->dense;
+    if (a->neb) k_hdot_e<<<(unsigned)((a->neb + 255) / 256), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
+    if (a->nrblk) k_hdot_r<<<(unsigned)((a->nrblk + 255) / 256), 256, 0, s>>>(d, x, y, R, a->d_part.ptr);
+    k_hdot_sum<<<1, 1024, 0, s>>>(a->neb + a->nrblk, a->d_part.ptr, result);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }

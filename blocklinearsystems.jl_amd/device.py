@@ -83,6 +83,7 @@ SIGNATURES = {
     "bls_asm_destroy": (None, [VP]),
     "bls_asm_lengths": (ct.c_int, [VP, P64]),
     "bls_asm_hessian_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_asm_dot_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_diag_destroy": (None, [VP]),
     "bls_factor_diagonal_blocks_f64": (ct.c_int, [VP, VP, P64, VP]),
     "bls_factor_diagonal_blocks_f32": (ct.c_int, [VP, VP, P64, VP]),
@@ -568,6 +569,15 @@ def assemble_hessian_(E_data, C_data, R, b_r, b_e, handle, J, r, stream=None):
     _check(lib().bls_asm_hessian_f64(handle.h, _dp(J), _dp(r), _dp(E_data), _dp(C_data), _dp(R), _dp(b_r), _dp(b_e),
                                      _stream(stream)))
     return True
+
+
+def hessian_dot(x, H_handle, y, E_data, C_data, R, stream=None):
+    """dot(x, H, y) (src/linalg.jl:133-260) over the storage of the handle's layout: a Python float."""
+    import torch
+    out = torch.empty(1, dtype=torch.float64, device=x.device)
+    _check(lib().bls_asm_dot_f64(H_handle.h, _dp(x), _dp(y), _dp(E_data), _dp(C_data), _dp(R), _dp(out),
+                                 _stream(stream)))
+    return float(out.item())
 
 
 def factor_diagonal_blocks_(D, blocks, stream=None):

@@ -371,6 +371,13 @@ int bls_asm_lengths(const bls_asm *a, int64_t *lengths /* 4 */);
  * rb_rows x (sum of its block dims) column-major; r: their residual rows one after another. */
 int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r, double *E_data, double *C_data, double *R,
                         double *b_r, double *b_e, void *stream);
+/* dot(x, H, y) (src/linalg.jl:133-260; schur_quadform when x == y): x' H y over the storage of the registered
+ * layout (a handle registered with nrowblocks = 0 serves for this alone), x and y laid out [x_r; x_e], length
+ * nr + sum of e_dims. A dense R is read in its upper triangle, a sparse R by its stored blocks, each off-diagonal
+ * block both ways. The terms are added in a fixed order (bitwise reproducible); the sum is written to the device
+ * double *result when the stream reaches it. */
+int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
+                    const double *R, double *result, void *stream);
 
 #ifdef __cplusplus
 }

@@ -5,9 +5,11 @@ assembly (bls_asm_hessian_f64). A row block k with Jacobian J_k over blocks (i_1
 residual r_k calls addblock!(H, i_a, i_b, B, rows_a, rows_b) with B = J_k' J_k for every pair a <= b in that order,
 and adds J_k' r_k to b; H starts at zero. Layout block numbers are 1-based: retained 1..nrb, eliminated nrb + e.
 
+Also dot(x, H, y) over that storage (schur_bilinear, src/linalg.jl:145-260), term by term.
+
 Parity pinning: the reference docstring's own example (src/schur_assembly.jl:126-137: a point and a camera, B =
 ones(9, 9), sums (9, 18, 36)), and H assembled here against the dense J'J of the stacked row blocks, read back
-through the storage layouts (tests/test_assembly.py).
+through the storage layouts; bilinear against x' (J'J) y (tests/test_assembly.py).
 """
 import numpy as np
 
@@ -142,3 +144,47 @@ def assemble(H, rowblocks, rb_rows, J, r):
         joff += m * W
         roff += m
     return H
+
+
+def bilinear(H, x, y):
+    """schur_bilinear (src/linalg.jl:145-260): x' H y from the stored triangle of a dense R (:153-161) or the stored
+    blocks of a sparse R (:162-190), then x_e' E y_e and the coupling terms per eliminated block (:208-260)."""
+    st = H.st
+    nr = H.nr
+    xr, xe, yr, ye = x[:nr], x[nr:], y[:nr], y[nr:]
+    total = 0.0
+    if H.dense:
+        for j in range(nr):
+            for i in range(j):
+                total += H.R[i, j] * xr[i] * yr[j] + H.R[i, j] * xr[j] * yr[i]
+            total += H.R[j, j] * xr[j] * yr[j]
+    else:
+        rd, bo = st.r_dims, np.asarray(st.boffsets_r) - 1
+        for q in range(st.nrblocks):
+            for k in range(H.r_colptr[q] - 1, H.r_colptr[q + 1] - 1):
+                p = H.r_rowval[k] - 1
+                off, drow, dcol = H.r_offsets[k] - 1, int(rd[p]), int(rd[q])
+                for jc in range(dcol):
+                    bx = by = 0.0
+                    for ir in range(drow):
+                        bv = H.R[off + jc * drow + ir]
+                        bx += bv * xr[bo[p] + ir]
+                        by += bv * yr[bo[p] + ir]
+                    total += bx * yr[bo[q] + jc]
+                    if p != q:
+                        total += by * xr[bo[q] + jc]
+    for e in range(st.neblocks):
+        d = int(st.e_dims[e])
+        eo, be = st.eoffsets[e] - 1, st.boffsets_e[e] - 1
+        for col in range(d):
+            for row in range(d):
+                total += H.E[eo + col * d + row] * xe[be + row] * ye[be + col]
+        for c in range(st.eliminated_ptr[e] - 1, st.eliminated_ptr[e + 1] - 1):
+            r = st.entry_retained[c] - 1
+            dr, ro, co = int(st.r_dims[r]), st.boffsets_r[r] - 1, st.coffsets[c] - 1
+            for ir in range(dr):
+                cx = sum(H.C[co + ir * d + i] * xe[be + i] for i in range(d))
+                cy = sum(H.C[co + ir * d + i] * ye[be + i] for i in range(d))
+                total += xr[ro + ir] * cy + yr[ro + ir] * cx
+    return total
+

@@ -89,3 +89,18 @@ def test_oracle_rejects_pairs_without_storage(blsamd):
         A.addblock(H, p[0], p[1], B, list(range(3)), list(range(3, 6)))
     with pytest.raises(ValueError, match="share no row block"):
         A.addblock(H, p[0], p[3], B, list(range(3)), list(range(6, 12)))
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_bilinear_matches_dense(blsamd, sparse):
+    """schur_bilinear's restatement (src/linalg.jl:145-260) on the assembled storage equals x' (J'J) y, and x' H x
+    for x == y (schur_quadform's value)."""
+    st, rowblocks, rb_rows, J, r, R_layout = assembly_problem(blsamd, 4, sparse=sparse)
+    H = A.assemble(A.HessianStorage(st, *(R_layout or ())), rowblocks, rb_rows, J, r)
+    Jd, _ = _dense_J(st, rowblocks, rb_rows, J, r)
+    M = Jd.T @ Jd
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(M.shape[0]), rng.standard_normal(M.shape[0])
+    for a, b in ((x, y), (x, x)):
+        want = a @ M @ b
+        assert abs(A.bilinear(H, a, b) - want) <= 1e-12 * np.abs(M).sum() * max(1.0, np.abs(a).max() * np.abs(b).max())
