@@ -123,35 +123,44 @@ struct HDotArgs {
     int dense;
 };
 
-// one lane per eliminated block: x_e' E y_e + sum over its couplings and rows ir of x_r[ir] (c_ir . y_e) +
-// y_r[ir] (c_ir . x_e), c_ir column ir of C_er (schur_bilinear_eliminated, src/linalg.jl:208-260)
+// one wavefront per eliminated block e, four per workgroup: x_e' E y_e + sum over its couplings c and rows ir of
+// x_r[ir] (c_ir . y_e) + y_r[ir] (c_ir . x_e), c_ir column ir of C_er (schur_bilinear_eliminated,
+// src/linalg.jl:208-260), as one sum of per-element terms: lane q takes element q (and q + 64, ...) of E_e and of
+// the couplings' blocks, which lie one after another in C_data, so the wavefront reads them as one contiguous run;
+// the lanes' sums are then added in a fixed tree
 __global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__restrict__ x, const double *__restrict__ y,
                                                 const double *__restrict__ E, const double *__restrict__ C,
                                                 double *__restrict__ part) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.neb) return;
+    const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (e >= a.neb) return;   // the whole wavefront: no block-level synchronisation below
     const int d = a.de[e];
     const double *xe = x + a.nr + a.boff_e[e], *ye = y + a.nr + a.boff_e[e];
     const double *Ee = E + a.eoff[e];
     double acc = 0.0;
-    for (int col = 0; col < d; col++) {
-        const double yc = ye[col];
-        for (int row = 0; row < d; row++) acc += Ee[col * d + row] * xe[row] * yc;
+    for (int w = lane; w < d * d; w += 64) {
+        const int col = w / d, row = w - col * d;
+        acc += Ee[w] * xe[row] * ye[col];
     }
-    for (int64_t c = a.cptr[e]; c < a.cptr[e + 1]; c++) {
-        const int r = a.c_r[c], dr = a.rd[r];
-        const double *xr = x + a.boff_r[r], *yr = y + a.boff_r[r], *Cc = C + a.coff[c];
-        for (int ir = 0; ir < dr; ir++) {
-            double cx = 0.0, cy = 0.0;
-            for (int i = 0; i < d; i++) {
-                const double cv = Cc[ir * d + i];
-                cx += cv * xe[i];
-                cy += cv * ye[i];
+    const int64_t c0 = a.cptr[e], c1 = a.cptr[e + 1];
+    if (c1 > c0) {
+        const double *span = C + a.coff[c0];
+        int64_t c = c0, start = 0;   // the coupling holding element q, and where its block starts in the span
+        int len = d * a.rd[a.c_r[c0]];
+        for (int64_t q = lane;; q += 64) {
+            while (c < c1 && q >= start + len) {
+                start += len;
+                if (++c < c1) len = d * a.rd[a.c_r[c]];
             }
-            acc += xr[ir] * cy + yr[ir] * cx;
+            if (c >= c1) break;
+            const int w = (int)(q - start), ir = w / d, i = w - ir * d;
+            const int64_t ro = a.boff_r[a.c_r[c]];
+            acc += span[q] * (x[ro + ir] * ye[i] + y[ro + ir] * xe[i]);
         }
     }
-    part[e] = acc;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) part[e] = acc;
 }
 
 // one lane per stored block of R (src/linalg.jl:153-190): a dense R is read in its upper triangle (each
@@ -177,18 +186,22 @@ __global__ __launch_bounds__(256) void k_hdot_r(HDotArgs a, const double *__rest
     part[a.neb + b] = acc;
 }
 
-// the partials added in one fixed order: thread t takes t, t + 1024, ...; then a fixed tree
-__global__ __launch_bounds__(1024) void k_hdot_sum(int64_t n, const double *__restrict__ part, double *__restrict__ out) {
-    __shared__ double red[1024];
+// the partials added in one fixed order, in two levels: workgroup g sums chunk g (thread t takes elements t,
+// t + 256, ... of it, then a fixed tree) into out[g]; the same kernel with one workgroup then sums the chunk sums
+constexpr int HDOT_CHUNKS = 1024;
+__global__ __launch_bounds__(256) void k_hdot_sum(int64_t n, int64_t chunk, const double *__restrict__ part,
+                                                  double *__restrict__ out) {
+    __shared__ double red[256];
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = std::min<int64_t>(n, lo + chunk);
     double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 1024) acc += part[i];
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) acc += part[i];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int o = 512; o >= 1; o >>= 1) {
+    for (int o = 128; o >= 1; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = red[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 }  // namespace
 
@@ -281,6 +294,9 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
             if (r < 0 || r >= nrb) return fail(BLS_ERR_ARG, "bls_asm_create: entry_retained out of range");
             dC[c].blk = RGBlock{coffsets[c] - 1, d, d, (int32_t)dim[r], 0, 0};   // d_e x d_r column-major
             h->C_len = std::max(h->C_len, coffsets[c] - 1 + (int64_t)d * dim[r]);
+            if (c + 1 < eliminated_ptr[e + 1] - 1 && coffsets[c + 1] != coffsets[c] + (int64_t)d * dim[r])
+                return fail(BLS_ERR_ARG, "bls_asm_create: the coupling blocks of eliminated block " + std::to_string(e + 1) +
+                                             " are not one after another in C_data (coffsets)");
         }
     }
     for (int64_t r = 0; r < nrb; r++) dbr[r].blk = RGBlock{boff[r], 1, 1, (int32_t)dim[r], 0, 0};
@@ -460,7 +476,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
             (rc1 = h->d_coff.upload(co)) || (rc1 = h->d_boff_r.upload(br)) || (rc1 = h->d_de.upload(dd)) ||
             (rc1 = h->d_c_r.upload(cr)) || (rc1 = h->d_rd.upload(rdv)) || (rc1 = h->d_rp.upload(rp)) ||
             (rc1 = h->d_rq.upload(rq)) || (rc1 = h->d_rblk.upload(rb)) ||
-            (rc1 = h->d_part.alloc((size_t)std::max<int64_t>(1, neb + (int64_t)rb.size()))))
+            (rc1 = h->d_part.alloc((size_t)(neb + (int64_t)rb.size() + HDOT_CHUNKS))))
             return rc1;
         h->nrblk = (int64_t)rb.size();
     }
@@ -556,9 +572,18 @@ extern "C" int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, con
     d.rp = a->d_rp.ptr;
     d.rq = a->d_rq.ptr;
     d.dense = a->dense;
-    if (a->neb) k_hdot_e<<<(unsigned)((a->neb + 255) / 256), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
+    if (a->neb) k_hdot_e<<<(unsigned)((a->neb + 3) / 4), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
     if (a->nrblk) k_hdot_r<<<(unsigned)((a->nrblk + 255) / 256), 256, 0, s>>>(d, x, y, R, a->d_part.ptr);
-    k_hdot_sum<<<1, 1024, 0, s>>>(a->neb + a->nrblk, a->d_part.ptr, result);
+    const int64_t n = a->neb + a->nrblk;
+    const int64_t chunk = std::max<int64_t>(256, (n + HDOT_CHUNKS - 1) / HDOT_CHUNKS);
+    const int64_t nchunk = (n + chunk - 1) / chunk;
+    double *sums = a->d_part.ptr + n;   // HDOT_CHUNKS slots after the partials
+    if (nchunk > 1) {
+        k_hdot_sum<<<(unsigned)nchunk, 256, 0, s>>>(n, chunk, a->d_part.ptr, sums);
+        k_hdot_sum<<<1, 256, 0, s>>>(nchunk, nchunk, sums, result);
+    } else {
+        k_hdot_sum<<<1, 256, 0, s>>>(n, std::max<int64_t>(n, 1), a->d_part.ptr, result);
+    }
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
