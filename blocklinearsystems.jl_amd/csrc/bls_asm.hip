@@ -20,6 +20,7 @@ using namespace bls;
 
 namespace {
 #include "bls_gram.h"
+#include "bls_fsum.h"
 
 constexpr int TINY_W = 16, TINY_M = 8;   // row blocks of at most 16 columns and 8 rows take the tiny path
 
@@ -115,12 +116,13 @@ __global__ __launch_bounds__(256) void k_red_gram_sum_light(int64_t nblk, int sl
 
 // dot(x, H, y) (src/linalg.jl:133-260), x and y laid out [x_r; x_e]. Partials in a fixed order, then one sum.
 struct HDotArgs {
-    int64_t nr, neb, nrblk;
+    int64_t nr, nrb, neb, nrblk;
     const int64_t *eoff, *boff_e, *cptr, *coff, *boff_r;   // 0-based
     const int32_t *de, *c_r, *rd;
     const RGBlock *rblk;                                    // the stored blocks of R (dst, column stride, dims)
     const int32_t *rp, *rq;                                 // their block row and column
     int dense;
+    const int64_t *rdg;                                     // per retained block: its diagonal block in rblk, or -1
 };
 
 // one wavefront per eliminated block e, four per workgroup: x_e' E y_e + sum over its couplings c and rows ir of
@@ -186,22 +188,28 @@ __global__ __launch_bounds__(256) void k_hdot_r(HDotArgs a, const double *__rest
     part[a.neb + b] = acc;
 }
 
-// the partials added in one fixed order, in two levels: workgroup g sums chunk g (thread t takes elements t,
-// t + 256, ... of it, then a fixed tree) into out[g]; the same kernel with one workgroup then sums the chunk sums
-constexpr int HDOT_CHUNKS = 1024;
-__global__ __launch_bounds__(256) void k_hdot_sum(int64_t n, int64_t chunk, const double *__restrict__ part,
-                                                  double *__restrict__ out) {
-    __shared__ double red[256];
-    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = std::min<int64_t>(n, lo + chunk);
-    double acc = 0.0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) acc += part[i];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o >= 1; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+// getdiag! (src/schur_linear_systems.jl:239-270): d = [diag(R); diag(E_1); ...], one thread per block, retained
+// blocks first; a retained block whose diagonal block a sparse R does not store has a zero diagonal
+__global__ __launch_bounds__(256) void k_hdiag(HDotArgs a, const double *__restrict__ E, const double *__restrict__ R,
+                                               double *__restrict__ d) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < a.nrb) {
+        const int n = a.rd[b];
+        const int64_t k = a.rdg[b];
+        double *o = d + a.boff_r[b];
+        if (k >= 0) {
+            const RGBlock bk = a.rblk[k];
+            for (int i = 0; i < n; i++) o[i] = R[bk.dst + (int64_t)i * bk.cstride + i];
+        } else {
+            for (int i = 0; i < n; i++) o[i] = 0.0;
+        }
+    } else if (b < a.nrb + a.neb) {
+        const int64_t e = b - a.nrb;
+        const int n = a.de[e];
+        const double *Ee = E + a.eoff[e];
+        double *o = d + a.nr + a.boff_e[e];
+        for (int i = 0; i < n; i++) o[i] = Ee[i * (n + 1)];
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 }  // namespace
 
@@ -229,6 +237,7 @@ struct bls_asm {
     DevArray<int64_t> d_eoff, d_boff_e, d_cptr, d_coff, d_boff_r;
     DevArray<int32_t> d_de, d_c_r, d_rd, d_rp, d_rq;
     DevArray<RGBlock> d_rblk;
+    DevArray<int64_t> d_rdg;   // per retained block: its diagonal block in d_rblk, or -1 (getdiag!)
     DevArray<double> d_part;
     int64_t nrblk = 0;
 };
@@ -441,6 +450,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
     {   // the storage layout for dot(x, H, y)
         std::vector<int64_t> eo(neb), be(neb), cp(neb + 1, 0), co(ncoup), br(nrb);
         std::vector<int32_t> dd(neb), cr(ncoup), rdv(nrb), rp, rq;
+        std::vector<int64_t> rdg(nrb, -1);
         std::vector<RGBlock> rb;
         for (int64_t e = 0; e < neb; e++) {
             eo[e] = eoffsets[e] - 1;
@@ -459,6 +469,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         if (h->dense) {
             for (int64_t q = 0; q < nrb; q++)
                 for (int64_t p = 0; p <= q; p++) {
+                    if (p == q) rdg[q] = (int64_t)rb.size();
                     rb.push_back(dR[q * (q + 1) / 2 + p].blk);
                     rp.push_back((int32_t)p);
                     rq.push_back((int32_t)q);
@@ -466,6 +477,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         } else {
             for (int64_t q = 0; q < nrb; q++)
                 for (int64_t k = r_colptr[q] - 1; k < r_colptr[q + 1] - 1; k++) {
+                    if (r_rowval[k] - 1 == q) rdg[q] = (int64_t)rb.size();
                     rb.push_back(dR[rcol[q] + (k - (r_colptr[q] - 1))].blk);
                     rp.push_back((int32_t)(r_rowval[k] - 1));
                     rq.push_back((int32_t)q);
@@ -475,8 +487,8 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         if ((rc1 = h->d_eoff.upload(eo)) || (rc1 = h->d_boff_e.upload(be)) || (rc1 = h->d_cptr.upload(cp)) ||
             (rc1 = h->d_coff.upload(co)) || (rc1 = h->d_boff_r.upload(br)) || (rc1 = h->d_de.upload(dd)) ||
             (rc1 = h->d_c_r.upload(cr)) || (rc1 = h->d_rd.upload(rdv)) || (rc1 = h->d_rp.upload(rp)) ||
-            (rc1 = h->d_rq.upload(rq)) || (rc1 = h->d_rblk.upload(rb)) ||
-            (rc1 = h->d_part.alloc((size_t)(neb + (int64_t)rb.size() + HDOT_CHUNKS))))
+            (rc1 = h->d_rq.upload(rq)) || (rc1 = h->d_rblk.upload(rb)) || (rc1 = h->d_rdg.upload(rdg)) ||
+            (rc1 = h->d_part.alloc((size_t)(neb + (int64_t)rb.size() + FSUM_CHUNKS))))
             return rc1;
         h->nrblk = (int64_t)rb.size();
     }
@@ -550,14 +562,11 @@ extern "C" int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r,
     return BLS_OK;
 }
 
-extern "C" int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
-                               const double *R, double *result, void *stream) {
-    if (!a) return fail(BLS_ERR_ARG, "null handle");
-    if (!x || !y || !result || (a->neb && !E_data) || (a->ncoup && !C_data) || (a->nrblk && !R))
-        return fail(BLS_ERR_ARG, "bls_asm_dot_f64: a required array is NULL");
-    hipStream_t s = (hipStream_t)stream;
+namespace {
+HDotArgs hdot_args(const bls_asm *a) {
     HDotArgs d{};
     d.nr = a->nr;
+    d.nrb = a->nrb;
     d.neb = a->neb;
     d.nrblk = a->nrblk;
     d.eoff = a->d_eoff.ptr;
@@ -572,18 +581,30 @@ extern "C" int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, con
     d.rp = a->d_rp.ptr;
     d.rq = a->d_rq.ptr;
     d.dense = a->dense;
+    d.rdg = a->d_rdg.ptr;
+    return d;
+}
+}  // namespace
+
+extern "C" int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
+                               const double *R, double *result, void *stream) {
+    if (!a) return fail(BLS_ERR_ARG, "null handle");
+    if (!x || !y || !result || (a->neb && !E_data) || (a->ncoup && !C_data) || (a->nrblk && !R))
+        return fail(BLS_ERR_ARG, "bls_asm_dot_f64: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const HDotArgs d = hdot_args(a);
     if (a->neb) k_hdot_e<<<(unsigned)((a->neb + 3) / 4), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
     if (a->nrblk) k_hdot_r<<<(unsigned)((a->nrblk + 255) / 256), 256, 0, s>>>(d, x, y, R, a->d_part.ptr);
-    const int64_t n = a->neb + a->nrblk;
-    const int64_t chunk = std::max<int64_t>(256, (n + HDOT_CHUNKS - 1) / HDOT_CHUNKS);
-    const int64_t nchunk = (n + chunk - 1) / chunk;
-    double *sums = a->d_part.ptr + n;   // HDOT_CHUNKS slots after the partials
-    if (nchunk > 1) {
-        k_hdot_sum<<<(unsigned)nchunk, 256, 0, s>>>(n, chunk, a->d_part.ptr, sums);
-        k_hdot_sum<<<1, 256, 0, s>>>(nchunk, nchunk, sums, result);
-    } else {
-        k_hdot_sum<<<1, 256, 0, s>>>(n, std::max<int64_t>(n, 1), a->d_part.ptr, result);
-    }
+    fsum_launch(a->neb + a->nrblk, a->d_part.ptr, a->d_part.ptr + a->neb + a->nrblk, result, s);   // FSUM_CHUNKS slots after the partials
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_asm_diag_f64(bls_asm *a, const double *E_data, const double *R, double *d, void *stream) {
+    if (!a) return fail(BLS_ERR_ARG, "null handle");
+    if (!d || (a->neb && !E_data) || (a->nrblk && !R)) return fail(BLS_ERR_ARG, "bls_asm_diag_f64: a required array is NULL");
+    const int64_t nb = a->nrb + a->neb;
+    if (nb) k_hdiag<<<grid_for(nb, 256), 256, 0, (hipStream_t)stream>>>(hdot_args(a), E_data, R, d);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }

@@ -514,6 +514,87 @@ __global__ __launch_bounds__(256) void k_red_dsum(RedArgs a, const int64_t *__re
 // each entry -- and writes the whole block (the lower triangle of a diagonal block, and a block without
 // contributions, as zeros); k_red_gsum adds the partials per retained block. Deterministic, no atomics.
 #include "bls_gram.h"
+#include "bls_fsum.h"
+
+// Quantities of the full problem from the raw (or the factored) tables, all M rows of every column, accumulated in
+// Float64 whatever the table type (src/qr_reduced.jl:574-645). Table t's raw columns are its entries' columns (entry
+// u: r_dims[entry_retained[u]] columns from entry_col[u]) and, for an eliminated table, its d_e own columns last.
+struct RawArgs {
+    int64_t nr, neb;
+    const int32_t *edims, *ncols;   // per table: d_e (0 for a free table), raw columns
+    const int64_t *boff_e;          // per eliminated block, 0-based, into the eliminated part
+};
+
+// jtr! (MODE 0: column . residual, g = J'r) and jtj_diag! (MODE 1: column . column, diag(J'J)),
+// src/qr_reduced.jl:574-596: one wavefront per table, lanes over its columns, each a run down the M rows in row
+// order. The entries' columns write the per-entry partials gp[eoff[u] + j] (added per retained block by k_red_gsum,
+// in retained_entry order, as sum_retained_columns! adds them); an eliminated table's own columns write
+// out[nr + boffsets_e[t] + k] directly.
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_raw_colq(RedArgs a, RawArgs b, const int64_t *__restrict__ eoff,
+                                                  const T *__restrict__ F, const double *__restrict__ Fr,
+                                                  double *__restrict__ gp, double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= a.ntables) return;
+    const int u0 = a.table_ptr[t], ne = a.table_ptr[t + 1] - u0, M = a.nrows[t];
+    const int de = b.edims[t];
+    const T *Xt = F + a.raw_off[t];
+    const double *rt = Fr + a.res_off[t];
+    const int64_t base = eoff[u0];
+    const int ncol = (int)(eoff[u0 + ne] - base);
+    for (int q = lane; q < ncol + de; q += 64) {
+        int col;
+        if (q < ncol) {
+            int e = 0;
+            while (e + 1 < ne && eoff[u0 + e + 1] - base <= q) e++;
+            col = a.entry_col[u0 + e] + (q - (int)(eoff[u0 + e] - base));
+        } else {
+            col = b.ncols[t] - de + (q - ncol);
+        }
+        const T *x = Xt + (int64_t)col * M;
+        double acc = 0.0;
+        for (int i = 0; i < M; i++) {
+            const double v = (double)x[i];
+            acc += v * (MODE == 1 ? v : rt[i]);
+        }
+        if (q < ncol) gp[base + q] = acc;
+        else out[b.nr + b.boff_e[t] + (q - ncol)] = acc;
+    }
+}
+
+// ||J v||^2 (jtj_quadform, src/qr_reduced.jl:597-645): one wavefront per table, lane i its rows i, i + 64, ...; row
+// i's y_i = sum over the entries in order, then the eliminated block, of J(i, cols) v(cols); the table's sum of y_i^2
+// (the lanes' sums added in a fixed tree) into part[t], then a fixed-order sum over the tables
+template <class T>
+__global__ __launch_bounds__(256) void k_raw_quad(RedArgs a, RawArgs b, const T *__restrict__ F,
+                                                  const double *__restrict__ v, double *__restrict__ part) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= a.ntables) return;
+    const int u0 = a.table_ptr[t], u1 = a.table_ptr[t + 1], M = a.nrows[t];
+    const int de = b.edims[t];
+    const T *Xt = F + a.raw_off[t];
+    double acc = 0.0;
+    for (int i = lane; i < M; i += 64) {
+        double yi = 0.0;
+        for (int u = u0; u < u1; u++) {
+            const int r = a.entry_retained[u];
+            const T *x = Xt + (int64_t)a.entry_col[u] * M + i;
+            const double *vr = v + a.boff_r[r];
+            for (int j = 0; j < a.r_dims[r]; j++) yi += (double)x[(int64_t)j * M] * vr[j];
+        }
+        if (de) {
+            const T *x = Xt + (int64_t)(b.ncols[t] - de) * M + i;
+            const double *ve = v + b.nr + b.boff_e[t];
+            for (int k = 0; k < de; k++) yi += (double)x[(int64_t)k * M] * ve[k];
+        }
+        acc += yi * yi;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) part[t] = acc;
+}
 
 // backsubstitute! of the QR path (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per eliminated table, one
 // lane per table: q the first d_e rows of the factored residual, B the first d_e rows of the retained columns, U the
@@ -584,6 +665,7 @@ struct bls_qr {
     DevArray<double> d_gp;                      // the partials
     DevArray<int64_t> d_dpo;                    // per-entry offsets of the diagonal-block partials (d (d + 1) / 2 each)
     DevArray<double> d_dp;                      // the diagonal-block partials (allocated at the first call)
+    DevArray<double> d_qpart;                   // ||J v||^2: per-table partials, then FSUM_CHUNKS chunk sums
     RGPlan rg;                                  // the Gram pass over those records
     std::vector<int32_t> h_tp, h_er, h_ec;   // table_ptr, entry_retained, entry_col (0-based)
     std::vector<int32_t> h_nrows, h_top;
@@ -787,6 +869,7 @@ extern "C" int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int
     std::vector<int64_t> dpo(nentries + 1, 0);
     for (int64_t u = 0; u < nentries; u++) dpo[u + 1] = dpo[u] + (int64_t)rd[er[u]] * (rd[er[u]] + 1) / 2;
     if ((rc = h->d_dpo.upload(dpo))) return rc;
+    if ((rc = h->d_qpart.alloc((size_t)(nt + FSUM_CHUNKS)))) return rc;
     h->dp_len = dpo[nentries];
     h->red_len = ro[nt];
     h->dlen = dl;
@@ -813,6 +896,40 @@ template <class T> int reduced_mul(bls_qr *h, double *y, const T *F, const doubl
     if (rc) return rc;
     if (!y || !F || !v) return fail(BLS_ERR_ARG, "reduced_mul: a required array is NULL");
     if (h->ntables) k_red_mul<T><<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), F, v, y);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+RawArgs raw_args(const bls_qr *h) {
+    RawArgs b{};
+    b.nr = h->nr;
+    b.neb = h->neb;
+    b.edims = h->d_edims.ptr;
+    b.ncols = h->d_ncols.ptr;
+    b.boff_e = h->d_boff_e.ptr;
+    return b;
+}
+
+// jtr! (MODE 0) and jtj_diag! (MODE 1) into out = [out_r; out_e] (length nr + sum(e_dims))
+template <class T, int MODE> int raw_colq(bls_qr *h, double *out, const T *F, const double *r, hipStream_t s) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!out || (h->raw_len && !F) || (MODE == 0 && h->res_len && !r))
+        return fail(BLS_ERR_ARG, MODE ? "jtj_diag: a required array is NULL" : "jtr: a required array is NULL");
+    if (h->ntables)
+        k_raw_colq<T, MODE><<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), raw_args(h), h->d_eoff.ptr, F, r,
+                                                                            h->d_gp.ptr, out);
+    if (h->nrb) k_red_gsum<<<(unsigned)((h->nrb + 3) / 4), 256, 0, s>>>(h->red(), h->d_eoff.ptr, h->d_gp.ptr, out);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+template <class T> int raw_quad(bls_qr *h, const T *F, const double *v, double *result, hipStream_t s) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!result || !v || (h->raw_len && !F)) return fail(BLS_ERR_ARG, "jtj_quadform: a required array is NULL");
+    if (h->ntables) k_raw_quad<T><<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), raw_args(h), F, v, h->d_qpart.ptr);
+    fsum_launch(h->ntables, h->d_qpart.ptr, h->d_qpart.ptr + h->ntables, result, s);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
@@ -1086,6 +1203,25 @@ extern "C" int bls_reduced_tmul_f64(bls_qr *h, double *g, const double *F, const
 }
 extern "C" int bls_reduced_tmul_f32(bls_qr *h, double *g, const float *F, const double *w, void *stream) {
     return reduced_tmul<float>(h, g, F, w, (hipStream_t)stream);
+}
+
+extern "C" int bls_jtr_f64(bls_qr *h, double *g, const double *F, const double *r, void *stream) {
+    return raw_colq<double, 0>(h, g, F, r, (hipStream_t)stream);
+}
+extern "C" int bls_jtr_f32(bls_qr *h, double *g, const float *F, const double *r, void *stream) {
+    return raw_colq<float, 0>(h, g, F, r, (hipStream_t)stream);
+}
+extern "C" int bls_jtj_diag_f64(bls_qr *h, double *d, const double *F, void *stream) {
+    return raw_colq<double, 1>(h, d, F, nullptr, (hipStream_t)stream);
+}
+extern "C" int bls_jtj_diag_f32(bls_qr *h, double *d, const float *F, void *stream) {
+    return raw_colq<float, 1>(h, d, F, nullptr, (hipStream_t)stream);
+}
+extern "C" int bls_jtj_quadform_f64(bls_qr *h, const double *F, const double *v, double *result, void *stream) {
+    return raw_quad<double>(h, F, v, result, (hipStream_t)stream);
+}
+extern "C" int bls_jtj_quadform_f32(bls_qr *h, const float *F, const double *v, double *result, void *stream) {
+    return raw_quad<float>(h, F, v, result, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *Fr, void *stream) {

@@ -84,6 +84,16 @@ SIGNATURES = {
     "bls_asm_lengths": (ct.c_int, [VP, P64]),
     "bls_asm_hessian_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_asm_dot_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_asm_diag_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_jtr_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_jtj_diag_f64": (ct.c_int, [VP, VP, VP, VP]),
+    "bls_jtj_quadform_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_cond_create": (ct.c_int, [ct.c_int64, ct.POINTER(VP)]),
+    "bls_cond_destroy": (None, [VP]),
+    "bls_cond_is_stale": (ct.c_int, [VP]),
+    "bls_cond_markstale": (ct.c_int, [VP]),
+    "bls_prepare_scale_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, ct.c_int, VP]),
+    "bls_cond_nrescale": (ct.c_int, [VP, P64, VP]),
     "bls_diag_destroy": (None, [VP]),
     "bls_factor_diagonal_blocks_f64": (ct.c_int, [VP, VP, P64, VP]),
     "bls_factor_diagonal_blocks_f32": (ct.c_int, [VP, VP, P64, VP]),
@@ -103,7 +113,8 @@ for _v in ("f32", "mixed"):
                "bls_reduce_schur_sparse_%s_async", "bls_backsubstitute_schur"):
         _src = _n.replace("%s", "f64") if "%s" in _n else _n + "_f64"
         SIGNATURES[_n % _v if "%s" in _n else _n + "_" + _v] = SIGNATURES[_src]
-for _n in ("bls_copyto_csc", "bls_reduce_qr", "bls_reduced_mul", "bls_reduced_tmul", "bls_reduced_diagonal_blocks"):
+for _n in ("bls_copyto_csc", "bls_reduce_qr", "bls_reduced_mul", "bls_reduced_tmul", "bls_reduced_diagonal_blocks",
+           "bls_jtr", "bls_jtj_diag", "bls_jtj_quadform"):
     SIGNATURES[_n + "_f32"] = SIGNATURES[_n + "_f64"]
 SIGNATURES["bls_reduce_qr_f32_async"] = SIGNATURES["bls_reduce_qr_f64_async"]
 
@@ -501,6 +512,28 @@ def reduced_diagonal_blocks_(D, handle, factored, stream=None):
     _check(getattr(lib(), "bls_reduced_diagonal_blocks_" + _fsfx(factored))(handle.h, _dp(D), _dp(factored), _stream(stream)))
 
 
+# ---- quantities of the full problem from the raw (or factored) tables (src/qr_reduced.jl:574-645) -------------
+def jtr_(g, handle, tables, r, stream=None):
+    """jtr!(g, J): g = J'r, [g_r; g_e] (Float64), from the tables (J.data or ws.factored) and their Float64 residual."""
+    _check(getattr(lib(), "bls_jtr_" + _fsfx(tables))(handle.h, _dp(g), _dp(tables), _dp(r), _stream(stream)))
+    return g
+
+
+def jtj_diag_(d, handle, tables, stream=None):
+    """jtj_diag!(d, J): d = diag(J'J), the squared column norms, [d_r; d_e] (Float64)."""
+    _check(getattr(lib(), "bls_jtj_diag_" + _fsfx(tables))(handle.h, _dp(d), _dp(tables), _stream(stream)))
+    return d
+
+
+def jtj_quadform(handle, tables, v, stream=None):
+    """jtj_quadform(J, v): ||J v||^2, a Python float."""
+    import torch
+    out = torch.empty(1, dtype=torch.float64, device=v.device)
+    _check(getattr(lib(), "bls_jtj_quadform_" + _fsfx(tables))(handle.h, _dp(tables), _dp(v), _dp(out),
+                                                               _stream(stream)))
+    return float(out.item())
+
+
 def qr_backsubstitute_(x_e, x_r, handle, ws, stream=None):
     """backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r)."""
     _check(getattr(lib(), "bls_qr_backsubstitute_" + _fsfx(ws.factored))(handle.h, _dp(x_e), _dp(x_r),
@@ -578,6 +611,60 @@ def hessian_dot(x, H_handle, y, E_data, C_data, R, stream=None):
     _check(lib().bls_asm_dot_f64(H_handle.h, _dp(x), _dp(y), _dp(E_data), _dp(C_data), _dp(R), _dp(out),
                                  _stream(stream)))
     return float(out.item())
+
+
+def hessian_diag_(d, H_handle, E_data, R, stream=None):
+    """getdiag!(d, ls) of a Schur system (src/schur_linear_systems.jl:239-270): the diagonal of H as filled."""
+    _check(lib().bls_asm_diag_f64(H_handle.h, _dp(E_data), _dp(R), _dp(d), _stream(stream)))
+    return d
+
+
+# ---- the conditioning scale (src/conditioning.jl:24-87) ---------------------------------------------------------
+class EliminationConditioning:
+    """EliminationConditioning(n, on) (src/conditioning.jl:47-55) on the device: s (ones) and d (zeros), n Float64
+    each, with the stale flag and the recomputation count held by the C ABI's bls_cond. `on` is the internal switch:
+    an unconditioned solve passes no scale (UnitScale) to the kernels."""
+
+    def __init__(self, n, on=True, device="cuda"):
+        import torch
+        h = VP()
+        _check(lib().bls_cond_create(n, ct.byref(h)))
+        self.h = h
+        self.n = n
+        self.on = on
+        self.s = torch.ones(n, dtype=torch.float64, device=device)
+        self.d = torch.zeros(n, dtype=torch.float64, device=device)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_cond_destroy(self.h)
+            self.h = None
+
+    @property
+    def stale(self):
+        return bool(_check(lib().bls_cond_is_stale(self.h)))
+
+    def markstale(self):
+        """zero! / markstale!: the next solve computes the scale from the diagonal afresh."""
+        _check(lib().bls_cond_markstale(self.h))
+
+    def nrescale(self, stream=None):
+        out = np.zeros(1, dtype=np.int64)
+        _check(lib().bls_cond_nrescale(self.h, out.ctypes.data_as(P64), _stream(stream)))
+        return int(out[0])
+
+
+def prepare_scale_(c, diag_, lam, sc, elem_dtype=None, stream=None):
+    """prepare_scale!(c, diag!, lambda, sc, T) (src/conditioning.jl:60-87): the scale for a solve at the damping
+    lambda Diagonal(sc) (sc a device Float64 vector, or None for ones); diag_(d) writes the undamped diagonal into
+    c.d, and is called only when the scale is stale. elem_dtype: the system's element type (torch.float32 raises the
+    scale's floor to floatmin(Float32)). Returns c.s."""
+    import torch
+    if c.stale:
+        diag_(c.d)
+    f32 = 1 if elem_dtype == torch.float32 else 0
+    _check(lib().bls_prepare_scale_f64(c.h, _dp(c.s), _dp(c.d), _dp(sc), float(lam), f32, _stream(stream)))
+    return c.s
 
 
 def factor_diagonal_blocks_(D, blocks, stream=None):

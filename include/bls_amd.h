@@ -264,6 +264,19 @@ int bls_reduced_tmul_f64(bls_qr *h, double *g, const double *factored, const dou
 int bls_reduced_tmul_f32(bls_qr *h, double *g, const float *factored, const double *w, void *stream);
 /* reduced_residual! (src/qr_reduced.jl:227-243): r~, the factored residual rows below U. */
 int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *fres, void *stream);
+/* Quantities of the full problem from the tables F (the raw tables J.data, or the factored ws.factored) over all
+ * rows, accumulated in Float64 (src/qr_reduced.jl:574-645); needs bls_qr_set_reduced. Vectors are [v_r; v_e] of
+ * length nr + sum of e_dims.
+ * jtr! (:574-583): g = J'r, r the Float64 residual at residual_offsets (J.r, or ws.fres).
+ * jtj_diag! (:585-595): d = diag(J'J), the squared column norms (colnorms2!, src/qr_linear_systems.jl:58-64).
+ * jtj_quadform (:597-645): ||J v||^2 into the device double *result, the tables' partials added in a fixed order.
+ * A retained column's value adds its tables' in retained_entry order, in runs (deterministic). */
+int bls_jtr_f64(bls_qr *h, double *g, const double *F, const double *r, void *stream);
+int bls_jtr_f32(bls_qr *h, double *g, const float *F, const double *r, void *stream);
+int bls_jtj_diag_f64(bls_qr *h, double *d, const double *F, void *stream);
+int bls_jtj_diag_f32(bls_qr *h, double *d, const float *F, void *stream);
+int bls_jtj_quadform_f64(bls_qr *h, const double *F, const double *v, double *result, void *stream);
+int bls_jtj_quadform_f32(bls_qr *h, const float *F, const double *v, double *result, void *stream);
 /* reduced_diagonal_blocks! (src/qr_reduced.jl:484-512): the diagonal blocks of J~'J~, full d x d column-major at
  * doffsets (the block-Jacobi blocks of the QR path). */
 int bls_reduced_diagonal_blocks_f64(bls_qr *h, double *D, const double *factored, void *stream);
@@ -378,6 +391,30 @@ int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r, double *E_
  * double *result when the stream reaches it. */
 int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
                     const double *R, double *result, void *stream);
+/* getdiag! (src/schur_linear_systems.jl:239-270): d = [diag(R); diag(E_1); ...], the undamped diagonal of H as
+ * filled, length nr + sum of e_dims; a retained block whose diagonal block a sparse R does not store reads zeros. */
+int bls_asm_diag_f64(bls_asm *a, const double *E_data, const double *R, double *d, void *stream);
+
+/* ---- The conditioning scale (src/conditioning.jl:24-87) ----------------------------------------------------
+ * bls_cond holds EliminationConditioning's state (:47-55) except its vectors: whether the next prepare computes the
+ * scale afresh (stale; set at creation and by bls_cond_markstale, as zero! and markstale! set it) and the number of
+ * recomputations at a later damping (nrescale). The caller owns s and d (n device doubles each). */
+typedef struct bls_cond bls_cond;
+int bls_cond_create(int64_t n, bls_cond **out);
+void bls_cond_destroy(bls_cond *c);
+/* 1 if the next bls_prepare_scale_f64 computes the scale from d: the caller's diag! (bls_asm_diag_f64 on the
+ * Hessian side, bls_jtj_diag_* on the Jacobian side) writes d first; 0 if it reuses the scale; < 0 on error. */
+int bls_cond_is_stale(const bls_cond *c);
+int bls_cond_markstale(bls_cond *c);
+/* prepare_scale!(c, diag!, lambda, sc, T) (:60-87) with d already written when stale: s_j = conditioning_scale(d_j +
+ * max(lambda, 0) sc_j) (:29-35; sc NULL = ones) for every j when stale; otherwise the same for every j if, for some
+ * variable, |d_j + lambda sc_j| s_j^2 lies outside [2^-16, 2^16] and its scale would change (nrescale += 1), else s is
+ * left as it is -- decided on the device. elem_f32: the system's element type is Float32 (the scale's floor is then
+ * floatmin(Float32)). Clears stale. */
+int bls_prepare_scale_f64(bls_cond *c, double *s, const double *d, const double *sc, double lambda, int elem_f32,
+                          void *stream);
+/* nrescale, after synchronizing the stream. */
+int bls_cond_nrescale(bls_cond *c, int64_t *out, void *stream);
 
 #ifdef __cplusplus
 }

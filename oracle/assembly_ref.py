@@ -5,11 +5,13 @@ assembly (bls_asm_hessian_f64). A row block k with Jacobian J_k over blocks (i_1
 residual r_k calls addblock!(H, i_a, i_b, B, rows_a, rows_b) with B = J_k' J_k for every pair a <= b in that order,
 and adds J_k' r_k to b; H starts at zero. Layout block numbers are 1-based: retained 1..nrb, eliminated nrb + e.
 
-Also dot(x, H, y) over that storage (schur_bilinear, src/linalg.jl:145-260), term by term.
+Also dot(x, H, y) over that storage (schur_bilinear, src/linalg.jl:145-260), term by term, and its diagonal
+(getdiag!, src/schur_linear_systems.jl:239-270).
 
 Parity pinning: the reference docstring's own example (src/schur_assembly.jl:126-137: a point and a camera, B =
 ones(9, 9), sums (9, 18, 36)), and H assembled here against the dense J'J of the stacked row blocks, read back
-through the storage layouts; bilinear against x' (J'J) y (tests/test_assembly.py).
+through the storage layouts; bilinear against x' (J'J) y; getdiag against diag(J'J), and zeros for a retained block
+whose diagonal block a sparse R does not store (tests/test_assembly.py, tests/test_conditioning.py).
 """
 import numpy as np
 
@@ -188,3 +190,27 @@ def bilinear(H, x, y):
                 total += xr[ro + ir] * cy + yr[ro + ir] * cx
     return total
 
+
+
+def getdiag(H):
+    """getdiag! (src/schur_linear_systems.jl:239-270): [diag(R); diag(E_1); ...]; diag_retained! reads the diagonal
+    block of a sparse R, zeros where it is not stored."""
+    st = H.st
+    d = np.zeros(H.nr + int(np.sum(st.e_dims)))
+    if H.dense:
+        d[:H.nr] = np.diag(H.R)
+    else:
+        for b in range(st.nrblocks):
+            db, bo = int(st.r_dims[b]), st.boffsets_r[b] - 1
+            try:
+                off = H.r_block_offset(b + 1, b + 1)
+            except ValueError:
+                continue
+            for i in range(db):
+                d[bo + i] = H.R[off + i * (db + 1)]
+    for e in range(st.neblocks):
+        eo, de = st.eoffsets[e] - 1, int(st.e_dims[e])
+        o = H.nr + st.boffsets_e[e] - 1
+        for i in range(de):
+            d[o + i] = H.E[eo + i * (de + 1)]
+    return d

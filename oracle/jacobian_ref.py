@@ -1,0 +1,88 @@
+"""Quantities of the full problem from the QR tables (TEST INFRASTRUCTURE ONLY): the checker of the device's
+bls_jtr_*, bls_jtj_diag_* and bls_jtj_quadform_*.
+
+Loop-by-loop restatement of src/qr_reduced.jl:529-645 (sum_retained_columns!, foreach_eliminated_column, jtr!,
+jtj_diag!, jtj_quadform) over a SchurJacobianStructure's tables (raw or factored, column-major, M rows each) and
+their Float64 residual, accumulated in Float64.
+
+Parity pinning (tests/test_conditioning.py): against the dense J and r of the same tables, J'r, diag(J'J) and
+||J v||^2 by numpy.
+"""
+import numpy as np
+
+
+def _eliminated_columns(jst, t):
+    """(1-based first raw column of table t's eliminated block, its dim): the last d_e of its table_ncols - 1 raw
+    columns ((C - de - 2 + k) M from the table's start, k = 1..d_e, :562-567); free tables have none."""
+    st = jst.schur
+    if t >= st.neblocks:
+        return 0, 0
+    de = int(st.e_dims[t])
+    return jst.table_ncols[t] - de, de
+
+
+def _columns(jst, D, R, colfun, out):
+    st = jst.schur
+    for r in range(st.nrblocks):                               # sum_retained_columns! (:529-551)
+        ro, dr = st.boffsets_r[r] - 1, int(st.r_dims[r])
+        out[ro:ro + dr] = 0.0
+        for k in range(jst.retained_ptr[r] - 1, jst.retained_ptr[r + 1] - 1):
+            u = jst.retained_entry[k] - 1
+            t = jst.entry_table[u] - 1
+            M = jst.table_nrows[t]
+            cb = jst.raw_offsets[t] - 1 + (jst.entry_col[u] - 1) * M
+            rb = jst.residual_offsets[t] - 1
+            for j in range(dr):
+                out[ro + j] += colfun(D, cb + j * M, R, rb, M)
+    for e in range(st.neblocks):                               # foreach_eliminated_column (:553-570)
+        M = jst.table_nrows[e]
+        c1, de = _eliminated_columns(jst, e)
+        eo = st.nr + st.boffsets_e[e] - 1
+        for k in range(de):
+            out[eo + k] = colfun(D, jst.raw_offsets[e] - 1 + (c1 - 1 + k) * M, R, jst.residual_offsets[e] - 1, M)
+    return out
+
+
+def _dot_run64(a, ia, b, ib, n):
+    acc = 0.0
+    for i in range(n):
+        acc += float(a[ia + i]) * float(b[ib + i])
+    return acc
+
+
+def jtr(jst, D, R):
+    """jtr! (:574-583): J'r."""
+    n = jst.schur.nr + int(np.sum(jst.schur.e_dims))
+    return _columns(jst, D, R, lambda D_, cb, R_, rb, M: _dot_run64(D_, cb, R_, rb, M), np.zeros(n))
+
+
+def jtj_diag(jst, D):
+    """jtj_diag! (:585-595): diag(J'J)."""
+    n = jst.schur.nr + int(np.sum(jst.schur.e_dims))
+    return _columns(jst, D, None, lambda D_, cb, R_, rb, M: _dot_run64(D_, cb, D_, cb, M), np.zeros(n))
+
+
+def jtj_quadform(jst, D, v):
+    """jtj_quadform (:597-645): ||J v||^2, per table and row."""
+    st = jst.schur
+    total = 0.0
+    for t in range(jst.ntables):
+        M = jst.table_nrows[t]
+        off = jst.raw_offsets[t] - 1
+        c1, de = _eliminated_columns(jst, t)
+        eo = st.nr + st.boffsets_e[t] - 1 if de else 0
+        part = 0.0
+        for i in range(M):
+            yi = 0.0
+            for u in range(jst.table_ptr[t] - 1, jst.table_ptr[t + 1] - 1):
+                r = jst.entry_retained[u] - 1
+                ro = st.boffsets_r[r] - 1
+                cb = off + (jst.entry_col[u] - 1) * M + i
+                for j in range(int(st.r_dims[r])):
+                    yi += float(D[cb + j * M]) * v[ro + j]
+            cb = off + (c1 - 1) * M + i
+            for k in range(de):
+                yi += float(D[cb + k * M]) * v[eo + k]
+            part += yi * yi
+        total += part
+    return total
