@@ -201,6 +201,20 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def time_ms(torch, fn, reps=5):
+    """Median HIP-event time of fn() on the current stream, after one untimed call."""
+    fn()
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    return float(np.median(times))
+
+
 def assembly_leg(torch, dev, st, reps=10):
     """The SLAM workload's Schur Hessian assembled on the device from its row blocks (batched addblock!,
     src/schur_assembly.jl:140-246): one 2-row observation block per coupling ([landmark; camera], J 2 x 9) and a
@@ -263,9 +277,32 @@ def assembly_leg(torch, dev, st, reps=10):
     dms = float(np.median(dtimes))
     dbytes = 8.0 * (h.E_len + h.C_len + h.R_len + 2 * n)
     log(f"dot(x, H, y) on that storage: {dms:.4f} ms, {dbytes / (dms * 1e-3) / 1e9:.0f} GB/s")
+    # the conditioning scale of a fresh fill: getdiag! (the diagonals of R and E_data) then prepare_scale! (d and sc
+    # read, s written); and at a later damping, the drift test (d, sc, s read) with no recomputation
+    d = torch.empty(n, dtype=torch.float64, device="cuda")
+    sc = torch.ones(n, dtype=torch.float64, device="cuda")
+    c = dev.EliminationConditioning(n)
+    diag_ = lambda dd: dev.hessian_diag_(dd, h, E, R)
+
+    def fresh():
+        c.markstale()
+        dev.prepare_scale_(c, diag_, 0.0, sc)
+    fms = time_ms(torch, fresh, reps)
+    lam = 1e-3 * float(c.d.max())
+    dev.prepare_scale_(c, diag_, lam, sc)
+    later_ms = time_ms(torch, lambda: dev.prepare_scale_(c, diag_, lam, sc), reps)
+    diag_ms = time_ms(torch, lambda: diag_(d), reps)
+    assert c.nrescale() == 0
+    fbytes = 8.0 * (h.nr + int(ed.sum()) + n + 3 * n)   # the diagonal entries read, d written; d, sc read, s written
+    log(f"conditioning scale: getdiag! {diag_ms:.4f} ms; fresh fill (getdiag! + scale) {fms:.4f} ms, "
+        f"{fbytes / (fms * 1e-3) / 1e9:.0f} GB/s; later damping (drift test) {later_ms:.4f} ms, "
+        f"{24.0 * n / (later_ms * 1e-3) / 1e9:.0f} GB/s")
     return {"row_blocks": int(nobs + nrb), "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
             "bytes": nbytes, "landmarks_per_s": neb / (ms * 1e-3),
-            "dot": {"ms": dms, "achieved_GBs": dbytes / (dms * 1e-3) / 1e9, "bytes": dbytes}}
+            "dot": {"ms": dms, "achieved_GBs": dbytes / (dms * 1e-3) / 1e9, "bytes": dbytes},
+            "conditioning": {"variables": n, "getdiag_ms": diag_ms, "fresh_fill_ms": fms, "later_damping_ms": later_ms,
+                             "fresh_fill_GBs": fbytes / (fms * 1e-3) / 1e9,
+                             "later_damping_GBs": 24.0 * n / (later_ms * 1e-3) / 1e9}}
 
 
 def mixed_leg(torch, dev, p, h, S64, csc, dmap, steps, warmup):
@@ -432,6 +469,21 @@ def qr_leg(torch, dev, b, scale, reps=10):
         prods[pname] = {"ms": pms, "achieved_GBs": gb, "frac": gb / HBM_PEAK_GBS}
         log(f"QR leg, {pname}: {pms:.3f} ms, {gb:.0f} GB/s of {(xbytes + vec_bytes) / 1e9:.2f} GB")
     solve["products_f64"] = prods
+    # quantities of the full problem from the raw Float64 tables: each reads every table once (8 B per value) plus
+    # its vectors
+    nv = st.nr + st.ne
+    rawbytes = float(jst.raw_length() * 8)
+    gq = torch.empty(nv, dtype=torch.float64, device="cuda")
+    vq = T(rng.standard_normal(nv))
+    full = {}
+    for pname, fn, extra in (("jtr", lambda: dev.jtr_(gq, h, raw64, r), 8 * (jst.residual_length() + nv)),
+                             ("jtj_diag", lambda: dev.jtj_diag_(gq, h, raw64), 8 * nv),
+                             ("jtj_quadform", lambda: dev.jtj_quadform(h, raw64, vq), 8 * nv)):
+        pms = time_ms(torch, fn)
+        gb = (rawbytes + extra) / (pms * 1e-3) / 1e9
+        full[pname] = {"ms": pms, "achieved_GBs": gb, "frac": gb / HBM_PEAK_GBS}
+        log(f"QR leg, {pname} (raw tables): {pms:.3f} ms, {gb:.0f} GB/s of {(rawbytes + extra) / 1e9:.2f} GB")
+    solve["full_problem_f64"] = full
     return {"landmark_scale": scale, "f32": f32, "solve": solve, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
             "cameras": int(st.nrblocks), "raw_GB": jst.raw_length() * 8 / 1e9, "ms": ms,
             "landmarks_per_s": st.neblocks / (ms * 1e-3),

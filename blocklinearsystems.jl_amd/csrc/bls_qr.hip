@@ -554,7 +554,18 @@ __global__ __launch_bounds__(256) void k_raw_colq(RedArgs a, RawArgs b, const in
         }
         const T *x = Xt + (int64_t)col * M;
         double acc = 0.0;
-        for (int i = 0; i < M; i++) {
+        int i = 0;
+        for (; i + 8 <= M; i += 8) {   // eight loads in flight, added in row order
+            double xv[8], wv[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                xv[j] = (double)x[i + j];
+                wv[j] = MODE == 1 ? xv[j] : rt[i + j];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) acc += xv[j] * wv[j];
+        }
+        for (; i < M; i++) {
             const double v = (double)x[i];
             acc += v * (MODE == 1 ? v : rt[i]);
         }
