@@ -10,6 +10,9 @@
 // by runs over a workgroup).
 // No atomics; bitwise reproducible. A pair's contribution is read in the orientation its destination stores
 // (add_retained_diagonal!, add_retained_pair!, add_eliminated!, add_coupling!, src/schur_assembly.jl:189-246).
+//
+// The handle's storage layout also serves the queries that read a filled Hessian: dot(x, H, y) (src/linalg.jl:133-260,
+// per-block partials and a fixed-order sum) and getdiag! (src/schur_linear_systems.jl:239-270).
 #include <algorithm>
 #include <cmath>
 #include <memory>
