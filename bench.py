@@ -451,7 +451,13 @@ def qr_leg(torch, dev, b, scale, reps=10):
     gv = torch.empty(st.nr, dtype=torch.float64, device="cuda")
     Dv = torch.empty(int((np.asarray(st.r_dims) ** 2).sum()), dtype=torch.float64, device="cuda")
     prods = {}
+    jcp, jrv = dev.build_reduced_jacobian_pattern(h)
+    dev.check_reduced_jacobian_pattern(h, nred, st.nr, jcp, jrv)
+    jnz = torch.empty(len(jrv), dtype=torch.float64, device="cuda")
+    jrt = torch.empty(nred, dtype=torch.float64, device="cuda")
     for pname, fn, vec_bytes in (("reduced_mul", lambda: dev.reduced_mul_(yv, h, ws.factored, vv), 8 * nred),
+                                 ("reduced_jacobian_sparse", lambda: dev.reduced_jacobian_(jnz, jrt, h, ws, sparse=True),
+                                  xbytes + 16 * nred),
                                  ("reduced_tmul", lambda: dev.reduced_tmul_(gv, h, ws.factored, ww), 8 * nred),
                                  ("reduced_diagonal_blocks", lambda: dev.reduced_diagonal_blocks_(Dv, h, ws.factored),
                                   8 * Dv.numel())):

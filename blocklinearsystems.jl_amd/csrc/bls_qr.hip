@@ -607,6 +607,39 @@ __global__ __launch_bounds__(256) void k_raw_quad(RedArgs a, RawArgs b, const T 
     if (lane == 0) part[t] = acc;
 }
 
+// reduced_jacobian! (src/qr_reduced.jl:247-270, :334-370): J~ written out, sparse into the nzval of
+// build_reduced_jacobian_pattern or dense (reduced_nrows x nr, column-major, zeroed first). One wavefront per table;
+// lane q takes value q of the table's retained columns of reduced rows (column q / n, row q % n, n = M - top), so
+// consecutive lanes read consecutive rows of a column and write consecutive entries of a run. Sparse: every column of
+// retained block r holds h_r values, the runs of its tables in retained_entry order; entry u's run starts at jdst[u]
+// in the first column of r and at jdst[u] + j h_r in column j. Plain copies: bitwise.
+template <class T, bool SPARSE>
+__global__ __launch_bounds__(256) void k_red_jacobian(RedArgs a, const int64_t *__restrict__ jdst,
+                                                      const int64_t *__restrict__ jh, int64_t m,
+                                                      const T *__restrict__ F, T *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= a.ntables) return;
+    const int M = a.nrows[t], top = a.top[t], n = M - top;
+    if (n <= 0) return;
+    const T *Xt = F + a.raw_off[t] + top;
+    for (int u = a.table_ptr[t]; u < a.table_ptr[t + 1]; u++) {
+        const int r = a.entry_retained[u], d = a.r_dims[r];
+        const T *x = Xt + (int64_t)a.entry_col[u] * M;
+        for (int q = lane; q < d * n; q += 64) {
+            const int j = q / n, i = q - j * n;
+            const T v = x[(int64_t)j * M + i];
+            if (SPARSE) out[jdst[u] + (int64_t)j * jh[r] + i] = v;
+            else out[(int64_t)(a.boff_r[r] + j) * m + a.red_off[t] + i] = v;
+        }
+    }
+}
+
+template <class T> __global__ void k_fill_zero(int64_t n, T *p) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (T)0;
+}
+
 // backsubstitute! of the QR path (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per eliminated table, one
 // lane per table: q the first d_e rows of the factored residual, B the first d_e rows of the retained columns, U the
 // triangle of the eliminated columns; accumulated in Float64 whatever the table type.
@@ -677,6 +710,12 @@ struct bls_qr {
     DevArray<int64_t> d_dpo;                    // per-entry offsets of the diagonal-block partials (d (d + 1) / 2 each)
     DevArray<double> d_dp;                      // the diagonal-block partials (allocated at the first call)
     DevArray<double> d_qpart;                   // ||J v||^2: per-table partials, then FSUM_CHUNKS chunk sums
+    // build_reduced_jacobian_pattern's layout: per entry its run's start (0-based, first column of its block), per
+    // retained block its column height; host copies for building and checking the pattern
+    DevArray<int64_t> d_jdst, d_jh;
+    int64_t jac_nnz = 0;
+    std::vector<int64_t> h_jh, h_red_off;
+    std::vector<int32_t> h_re, h_et;
     RGPlan rg;                                  // the Gram pass over those records
     std::vector<int32_t> h_tp, h_er, h_ec;   // table_ptr, entry_retained, entry_col (0-based)
     std::vector<int32_t> h_nrows, h_top;
@@ -881,6 +920,27 @@ extern "C" int bls_qr_set_reduced(bls_qr *h, const int64_t *table_top, const int
     for (int64_t u = 0; u < nentries; u++) dpo[u + 1] = dpo[u] + (int64_t)rd[er[u]] * (rd[er[u]] + 1) / 2;
     if ((rc = h->d_dpo.upload(dpo))) return rc;
     if ((rc = h->d_qpart.alloc((size_t)(nt + FSUM_CHUNKS)))) return rc;
+    {   // build_reduced_jacobian_pattern (src/qr_reduced.jl:289-310): columns in retained-block order, each column of
+        // block r holding the reduced rows of r's tables in retained_entry order
+        std::vector<int64_t> jh(nrb), jdst(nentries);
+        int64_t cp = 0;
+        for (int64_t r = 0; r < nrb; r++) {
+            int64_t pos = 0;
+            for (int k = rp[r]; k < rp[r + 1]; k++) {
+                const int32_t u = re[k], t = et[u];
+                jdst[u] = cp + pos;
+                pos += h->h_nrows[t] - top[t];
+            }
+            jh[r] = pos;
+            cp += pos * rd[r];
+        }
+        if ((rc = h->d_jdst.upload(jdst)) || (rc = h->d_jh.upload(jh))) return rc;
+        h->jac_nnz = cp;
+        h->h_jh = jh;
+        h->h_red_off = ro;
+        h->h_re = re;
+        h->h_et = et;
+    }
     h->dp_len = dpo[nentries];
     h->red_len = ro[nt];
     h->dlen = dl;
@@ -1233,6 +1293,99 @@ extern "C" int bls_jtj_quadform_f64(bls_qr *h, const double *F, const double *v,
 }
 extern "C" int bls_jtj_quadform_f32(bls_qr *h, const float *F, const double *v, double *result, void *stream) {
     return raw_quad<float>(h, F, v, result, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduced_jacobian_nnz(bls_qr *h, int64_t *nnz) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!nnz) return fail(BLS_ERR_ARG, "bls_reduced_jacobian_nnz: nnz is NULL");
+    *nnz = h->jac_nnz;
+    return BLS_OK;
+}
+
+extern "C" int bls_build_reduced_jacobian_pattern(bls_qr *h, int64_t *colptr, int64_t *rowval) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if (!colptr || (h->jac_nnz && !rowval)) return fail(BLS_ERR_ARG, "bls_build_reduced_jacobian_pattern: bad arguments");
+    int64_t p = 0, c = 0;
+    colptr[0] = 1;
+    for (int64_t r = 0; r < h->nrb; r++)
+        for (int j = 0; j < h->h_r_dims[r]; j++, c++) {
+            for (int64_t k = h->h_retained_ptr[r]; k < h->h_retained_ptr[r + 1]; k++) {
+                const int32_t t = h->h_et[h->h_re[k]];
+                for (int64_t i = 0; i < h->h_nrows[t] - h->h_top[t]; i++) rowval[p++] = h->h_red_off[t] + i + 1;
+            }
+            colptr[c + 1] = p + 1;
+        }
+    return BLS_OK;
+}
+
+extern "C" int bls_check_reduced_jacobian_pattern(bls_qr *h, int64_t m, int64_t n, const int64_t *colptr,
+                                                  const int64_t *rowval) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    const int64_t mm = h->red_len;
+    if (m != mm || n != h->nr)
+        return fail(BLS_ERR_ARG, "DimensionMismatch: Jt has size (" + std::to_string(m) + ", " + std::to_string(n) +
+                                     "), expected (" + std::to_string(mm) + ", " + std::to_string(h->nr) + ")");
+    if (!colptr || (colptr[n] > 1 && !rowval)) return fail(BLS_ERR_ARG, "bls_check_reduced_jacobian_pattern: bad arguments");
+    for (int64_t r = 0; r < h->nrb; r++)
+        for (int j = 0; j < h->h_r_dims[r]; j++) {
+            const int64_t c = h->h_boff_r[r] + j;   // 0-based column
+            if (colptr[c + 1] - colptr[c] != h->h_jh[r])
+                return fail(BLS_ERR_ARG, "ArgumentError: Jt does not have the pattern of build_reduced_jacobian_pattern: "
+                                         "column " + std::to_string(c + 1) + " stores " +
+                                         std::to_string(colptr[c + 1] - colptr[c]) + " entries, expected " +
+                                         std::to_string(h->h_jh[r]));
+            int64_t p = colptr[c] - 1;
+            for (int64_t k = h->h_retained_ptr[r]; k < h->h_retained_ptr[r + 1]; k++) {
+                const int32_t t = h->h_et[h->h_re[k]];
+                const int64_t nt = h->h_nrows[t] - h->h_top[t];
+                if (nt && rowval[p] != h->h_red_off[t] + 1)
+                    return fail(BLS_ERR_ARG, "ArgumentError: Jt does not have the pattern of "
+                                             "build_reduced_jacobian_pattern: column " + std::to_string(c + 1) +
+                                             " stores row " + std::to_string(rowval[p]) + ", expected " +
+                                             std::to_string(h->h_red_off[t] + 1));
+                p += nt;
+            }
+        }
+    return BLS_OK;
+}
+
+namespace {
+template <class T, bool SPARSE>
+int reduced_jacobian(bls_qr *h, T *Jt, double *rt, const T *F, const double *Fr, hipStream_t s) {
+    int rc = reduced_ready(h);
+    if (rc) return rc;
+    if ((!Jt && (SPARSE ? h->jac_nnz : h->red_len * h->nr)) || (h->red_len && (!rt || !Fr)) || (h->raw_len && !F))
+        return fail(BLS_ERR_ARG, "reduced_jacobian: a required array is NULL");
+    if (!SPARSE && h->red_len * h->nr)
+        k_fill_zero<T><<<(unsigned)std::min<int64_t>(4096, grid_for(h->red_len * h->nr, 256)), 256, 0, s>>>(h->red_len * h->nr, Jt);
+    if (h->ntables) {
+        k_red_jacobian<T, SPARSE><<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), h->d_jdst.ptr, h->d_jh.ptr,
+                                                                                  h->red_len, F, Jt);
+        k_red_residual<<<(unsigned)((h->ntables + 3) / 4), 256, 0, s>>>(h->red(), Fr, rt);
+    }
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_reduced_jacobian_sparse_f64(bls_qr *h, double *nzval, double *rt, const double *F, const double *Fr,
+                                               void *stream) {
+    return reduced_jacobian<double, true>(h, nzval, rt, F, Fr, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_jacobian_sparse_f32(bls_qr *h, float *nzval, double *rt, const float *F, const double *Fr,
+                                               void *stream) {
+    return reduced_jacobian<float, true>(h, nzval, rt, F, Fr, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_jacobian_dense_f64(bls_qr *h, double *Jt, double *rt, const double *F, const double *Fr,
+                                              void *stream) {
+    return reduced_jacobian<double, false>(h, Jt, rt, F, Fr, (hipStream_t)stream);
+}
+extern "C" int bls_reduced_jacobian_dense_f32(bls_qr *h, float *Jt, double *rt, const float *F, const double *Fr,
+                                              void *stream) {
+    return reduced_jacobian<float, false>(h, Jt, rt, F, Fr, (hipStream_t)stream);
 }
 
 extern "C" int bls_reduced_residual_f64(bls_qr *h, double *rt, const double *Fr, void *stream) {

@@ -88,6 +88,11 @@ SIGNATURES = {
     "bls_jtr_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_jtj_diag_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_jtj_quadform_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_reduced_jacobian_nnz": (ct.c_int, [VP, P64]),
+    "bls_build_reduced_jacobian_pattern": (ct.c_int, [VP, P64, P64]),
+    "bls_check_reduced_jacobian_pattern": (ct.c_int, [VP, ct.c_int64, ct.c_int64, P64, P64]),
+    "bls_reduced_jacobian_sparse_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_reduced_jacobian_dense_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_cond_create": (ct.c_int, [ct.c_int64, ct.POINTER(VP)]),
     "bls_cond_destroy": (None, [VP]),
     "bls_cond_is_stale": (ct.c_int, [VP]),
@@ -114,7 +119,7 @@ for _v in ("f32", "mixed"):
         _src = _n.replace("%s", "f64") if "%s" in _n else _n + "_f64"
         SIGNATURES[_n % _v if "%s" in _n else _n + "_" + _v] = SIGNATURES[_src]
 for _n in ("bls_copyto_csc", "bls_reduce_qr", "bls_reduced_mul", "bls_reduced_tmul", "bls_reduced_diagonal_blocks",
-           "bls_jtr", "bls_jtj_diag", "bls_jtj_quadform"):
+           "bls_jtr", "bls_jtj_diag", "bls_jtj_quadform", "bls_reduced_jacobian_sparse", "bls_reduced_jacobian_dense"):
     SIGNATURES[_n + "_f32"] = SIGNATURES[_n + "_f64"]
 SIGNATURES["bls_reduce_qr_f32_async"] = SIGNATURES["bls_reduce_qr_f64_async"]
 
@@ -532,6 +537,35 @@ def jtj_quadform(handle, tables, v, stream=None):
     _check(getattr(lib(), "bls_jtj_quadform_" + _fsfx(tables))(handle.h, _dp(tables), _dp(v), _dp(out),
                                                                _stream(stream)))
     return float(out.item())
+
+
+def build_reduced_jacobian_pattern(handle):
+    """build_reduced_jacobian_pattern(jst) (src/qr_reduced.jl:289-310): (colptr, rowval), 1-based numpy Int64, of J~'s
+    CSC (reduced_nrows x nr)."""
+    cnt = np.zeros(1, dtype=np.int64)
+    _check(lib().bls_reduced_jacobian_nnz(handle.h, cnt.ctypes.data_as(P64)))
+    nnz = int(cnt[0])
+    colptr = np.zeros(handle.jst.schur.nr + 1, dtype=np.int64)
+    rowval = np.zeros(max(nnz, 1), dtype=np.int64)
+    _check(lib().bls_build_reduced_jacobian_pattern(handle.h, colptr.ctypes.data_as(P64), rowval.ctypes.data_as(P64)))
+    return colptr, rowval[:nnz]
+
+
+def check_reduced_jacobian_pattern(handle, m, n, colptr, rowval):
+    """check_reduced_jacobian_pattern (src/qr_reduced.jl:312-332): raises BLSError with the reference's reason."""
+    keep = []
+    _check(lib().bls_check_reduced_jacobian_pattern(handle.h, m, n, _p64(colptr, keep), _p64(rowval, keep)))
+
+
+def reduced_jacobian_(Jt, rt, handle, ws, sparse, stream=None):
+    """reduced_jacobian!(Jt, rt, ws) (src/qr_reduced.jl:247-270, :334-370): J~ into Jt (sparse: the nzval of
+    build_reduced_jacobian_pattern, checked by the caller; dense: reduced_nrows x nr column-major, flat) and
+    rt = r~ (Float64)."""
+    if Jt.dtype != ws.factored.dtype:
+        raise BLSError("Jt and the factored tables must have one element type")
+    name = "bls_reduced_jacobian_%s_%s" % ("sparse" if sparse else "dense", _fsfx(ws.factored))
+    _check(getattr(lib(), name)(handle.h, _dp(Jt), _dp(rt), _dp(ws.factored), _dp(ws.fres), _stream(stream)))
+    return Jt
 
 
 def qr_backsubstitute_(x_e, x_r, handle, ws, stream=None):

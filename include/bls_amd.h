@@ -277,6 +277,23 @@ int bls_jtj_diag_f64(bls_qr *h, double *d, const double *F, void *stream);
 int bls_jtj_diag_f32(bls_qr *h, double *d, const float *F, void *stream);
 int bls_jtj_quadform_f64(bls_qr *h, const double *F, const double *v, double *result, void *stream);
 int bls_jtj_quadform_f32(bls_qr *h, const float *F, const double *v, double *result, void *stream);
+/* build_reduced_jacobian_pattern (src/qr_reduced.jl:289-310) and reduced_jacobian! (:247-270, :334-370): J~ (the
+ * reduced rows of every table's retained columns, reduced_nrows x nr) written out for a sparse or dense solver.
+ * bls_reduced_jacobian_nnz: the pattern's stored entries. bls_build_reduced_jacobian_pattern: its host colptr (nr + 1)
+ * and rowval (nnz), 1-based: each column of retained block r holds the reduced rows of r's tables, tables in
+ * retained_entry order. bls_check_reduced_jacobian_pattern (check_reduced_jacobian_pattern, :312-332): a host CSC of
+ * size m x n against that pattern (size, column heights, the first row of every run), with the reference's errors;
+ * the caller checks a matrix once before filling it. The fills write J~ from the factored tables F (sparse: into the
+ * nzval of that pattern; dense: a column-major m x nr array, zeroed first) and rt = r~ (Float64) from fres. */
+int bls_reduced_jacobian_nnz(bls_qr *h, int64_t *nnz);
+int bls_build_reduced_jacobian_pattern(bls_qr *h, int64_t *colptr, int64_t *rowval);
+int bls_check_reduced_jacobian_pattern(bls_qr *h, int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval);
+int bls_reduced_jacobian_sparse_f64(bls_qr *h, double *nzval, double *rt, const double *F, const double *fres,
+                                    void *stream);
+int bls_reduced_jacobian_sparse_f32(bls_qr *h, float *nzval, double *rt, const float *F, const double *fres,
+                                    void *stream);
+int bls_reduced_jacobian_dense_f64(bls_qr *h, double *Jt, double *rt, const double *F, const double *fres, void *stream);
+int bls_reduced_jacobian_dense_f32(bls_qr *h, float *Jt, double *rt, const float *F, const double *fres, void *stream);
 /* reduced_diagonal_blocks! (src/qr_reduced.jl:484-512): the diagonal blocks of J~'J~, full d x d column-major at
  * doffsets (the block-Jacobi blocks of the QR path). */
 int bls_reduced_diagonal_blocks_f64(bls_qr *h, double *D, const double *factored, void *stream);

@@ -1,12 +1,17 @@
-"""Quantities of the full problem from the QR tables (TEST INFRASTRUCTURE ONLY): the checker of the device's
-bls_jtr_*, bls_jtj_diag_* and bls_jtj_quadform_*.
+"""Quantities of the full problem from the QR tables, and J~ written out (TEST INFRASTRUCTURE ONLY): the checker of
+the device's bls_jtr_*, bls_jtj_diag_*, bls_jtj_quadform_*, bls_build_reduced_jacobian_pattern and
+bls_reduced_jacobian_*.
 
 Loop-by-loop restatement of src/qr_reduced.jl:529-645 (sum_retained_columns!, foreach_eliminated_column, jtr!,
 jtj_diag!, jtj_quadform) over a SchurJacobianStructure's tables (raw or factored, column-major, M rows each) and
 their Float64 residual, accumulated in Float64.
 
+build_reduced_jacobian_pattern and reduced_jacobian (src/qr_reduced.jl:247-370) restated the same way.
+
 Parity pinning (tests/test_conditioning.py): against the dense J and r of the same tables, J'r, diag(J'J) and
-||J v||^2 by numpy.
+||J v||^2 by numpy; J~ against the C oracle's reduced_mul / reduced_tmul on the same factored tables (J~ v and J~'w
+for random v, w), its pattern against the CSC invariants (ascending rows, nnz = the sum of d_r (M - top) over the
+entries) and its dense form against the sparse one, entry for entry (test/schur_qr.jl:180-270).
 """
 import numpy as np
 
@@ -86,3 +91,55 @@ def jtj_quadform(jst, D, v):
             part += yi * yi
         total += part
     return total
+
+
+def build_reduced_jacobian_pattern(jst):
+    """src/qr_reduced.jl:289-310: (colptr, rowval), 1-based; each column of retained block r holds the reduced rows
+    of every table coupled to r, tables in retained_entry order."""
+    st = jst.schur
+    colptr = [1]
+    rowval = []
+    for r in range(st.nrblocks):
+        for _ in range(int(st.r_dims[r])):
+            for k in range(jst.retained_ptr[r] - 1, jst.retained_ptr[r + 1] - 1):
+                t = jst.entry_table[jst.retained_entry[k] - 1] - 1
+                rowval.extend(jst.reduced_offsets[t] + i for i in range(jst.table_nrows[t] - jst.table_top[t]))
+            colptr.append(len(rowval) + 1)
+    return np.array(colptr, dtype=np.int64), np.array(rowval, dtype=np.int64)
+
+
+def reduced_jacobian_sparse(jst, F, colptr):
+    """src/qr_reduced.jl:334-370: nzval of J~ in the pattern, from the factored tables F."""
+    st = jst.schur
+    nz = np.zeros(colptr[-1] - 1, dtype=F.dtype)
+    for r in range(st.nrblocks):
+        ro, dr = st.boffsets_r[r], int(st.r_dims[r])
+        p = 0
+        for k in range(jst.retained_ptr[r] - 1, jst.retained_ptr[r + 1] - 1):
+            u = jst.retained_entry[k] - 1
+            t = jst.entry_table[u] - 1
+            M, top = jst.table_nrows[t], jst.table_top[t]
+            n = M - top
+            cb = jst.raw_offsets[t] - 1 + (jst.entry_col[u] - 1) * M + top
+            for j in range(dr):
+                q = colptr[ro - 1 + j] - 1 + p
+                nz[q:q + n] = F[cb + j * M:cb + j * M + n]
+            p += n
+    return nz
+
+
+def reduced_jacobian_dense(jst, F):
+    """src/qr_reduced.jl:247-270: J~ as a dense reduced_nrows x nr array."""
+    st = jst.schur
+    m = int(jst.reduced_offsets[-1] - 1)
+    Jt = np.zeros((m, st.nr), dtype=F.dtype)
+    for t in range(jst.ntables):
+        M, top = jst.table_nrows[t], jst.table_top[t]
+        off, yo = jst.raw_offsets[t] - 1, jst.reduced_offsets[t] - 1
+        for u in range(jst.table_ptr[t] - 1, jst.table_ptr[t + 1] - 1):
+            r = jst.entry_retained[u] - 1
+            ro = st.boffsets_r[r] - 1
+            cb = off + (jst.entry_col[u] - 1) * M + top
+            for j in range(int(st.r_dims[r])):
+                Jt[yo:yo + M - top, ro + j] = F[cb + j * M:cb + j * M + M - top]
+    return Jt

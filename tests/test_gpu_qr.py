@@ -326,3 +326,50 @@ def test_reduced_normal_matches_oracle(blsamd, table_type):
             assert max_block_rel_to({k: As[k] for k in As}, {k: B[k] for k in As}, Mag) <= eps
         for gg in (gd, gs):
             assert np.abs(gg.cpu().numpy() - go).max() <= TOL * np.abs(go).max()
+
+
+@pytest.mark.parametrize("case", ["f64", "f32", "free"])
+def test_reduced_jacobian_matches_oracle(blsamd, case):
+    """build_reduced_jacobian_pattern, check_reduced_jacobian_pattern and reduced_jacobian! (sparse and dense,
+    src/qr_reduced.jl:247-370) through the C ABI: the pattern equal to the oracle's, every stored entry written (NaN
+    before), values bitwise equal to the oracle's copies of the device-factored tables, rt == reduced_residual!, and
+    the reference's rejections (test/schur_qr.jl:257-269)."""
+    import torch
+    import jacobian_ref as JR
+    dev = blsamd.device
+    p = _jac_with_free_rows(blsamd, seed=12) if case == "free" else _small_jacobian(blsamd, 6, nland=70, ncam=8)
+    jst, st = p.jst, p.structure
+    dt = torch.float32 if case == "f32" else torch.float64
+    h = dev.QRHandle(jst)
+    ws = dev.QRReductionWorkspace(h, dtype=dt)
+    T = lambda a, d=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(d).cuda()
+    assert dev.reduce_qr_(ws, h, T(p.data, dt), T(p.r), T(p.scaling_e), 0.3)
+    m = int(jst.reduced_nrows())
+    colptr, rowval = dev.build_reduced_jacobian_pattern(h)
+    co, ro = JR.build_reduced_jacobian_pattern(jst)
+    assert np.array_equal(colptr, co) and np.array_equal(rowval, ro)
+    dev.check_reduced_jacobian_pattern(h, m, st.nr, colptr, rowval)
+    nz = torch.full((len(rowval),), float("nan"), dtype=dt, device="cuda")
+    rt = torch.full((m,), float("nan"), dtype=torch.float64, device="cuda")
+    dev.reduced_jacobian_(nz, rt, h, ws, sparse=True)
+    Jd = torch.full((m * st.nr,), float("nan"), dtype=dt, device="cuda")
+    rt2 = torch.full((m,), float("nan"), dtype=torch.float64, device="cuda")
+    dev.reduced_jacobian_(Jd, rt2, h, ws, sparse=False)
+    F = ws.factored.cpu().numpy()
+    rto = torch.empty(m, dtype=torch.float64, device="cuda")
+    dev.reduced_residual_(rto, h, ws.fres)
+    torch.cuda.synchronize()
+    assert np.array_equal(nz.cpu().numpy(), JR.reduced_jacobian_sparse(jst, F, co))
+    assert np.array_equal(Jd.cpu().numpy().reshape(st.nr, m).T, JR.reduced_jacobian_dense(jst, F))
+    assert np.array_equal(rt.cpu().numpy(), rto.cpu().numpy()) and np.array_equal(rt2.cpu().numpy(), rto.cpu().numpy())
+    # a matrix without the pattern is rejected
+    with pytest.raises(dev.BLSError, match="DimensionMismatch"):
+        dev.check_reduced_jacobian_pattern(h, m, st.nr + 1, np.concatenate([colptr, colptr[-1:]]), rowval)
+    full = np.arange(st.nr + 1, dtype=np.int64) * m + 1                         # every column full: wrong heights
+    with pytest.raises(dev.BLSError, match="column 1 stores"):
+        dev.check_reduced_jacobian_pattern(h, m, st.nr, full, np.tile(np.arange(1, m + 1), st.nr))
+    bad = rowval.copy()
+    c = int(np.nonzero(np.diff(colptr))[0][0])
+    bad[colptr[c] - 1] = 1 if bad[colptr[c] - 1] == m else m                    # a run that starts at the wrong row
+    with pytest.raises(dev.BLSError, match="stores row"):
+        dev.check_reduced_jacobian_pattern(h, m, st.nr, colptr, bad)

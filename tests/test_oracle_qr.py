@@ -257,3 +257,36 @@ def test_reduced_normal_sparse_equals_dense(blsamd):
         for k, blk in B.items():
             assert np.array_equal(A[k], blk), k
         assert np.array_equal(gd, gs)
+
+
+@pytest.mark.parametrize("free", [False, True])
+def test_reduced_jacobian_pattern_and_values(blsamd, free):
+    """build_reduced_jacobian_pattern / reduced_jacobian! (src/qr_reduced.jl:247-370), restated in
+    oracle/jacobian_ref.py: the CSC invariants (strictly ascending rows per column, nnz = sum over entries of
+    d_r (M - top)), J~ v and J~'w equal to the C oracle's reduced_mul / reduced_tmul on the same factored tables, and
+    the dense form equal to the sparse one entry for entry (test/schur_qr.jl:180-270)."""
+    import scipy.sparse as sp
+    import jacobian_ref as JR
+    p = _jac_with_free_rows_cpu(blsamd) if free else _small_jacobian(blsamd, 4, nland=50, ncam=7)
+    ok, _, F, Fr, sargs, jargs = _qr(p, 0.3)
+    assert ok
+    jst, st = p.jst, p.structure
+    m = int(jst.reduced_nrows())
+    colptr, rowval = JR.build_reduced_jacobian_pattern(jst)
+    assert len(colptr) == st.nr + 1 and colptr[-1] - 1 == len(rowval)
+    assert len(rowval) == sum(int(st.r_dims[jst.entry_retained[u] - 1]) *
+                              (jst.table_nrows[jst.entry_table[u] - 1] - jst.table_top[jst.entry_table[u] - 1])
+                              for u in range(len(jst.entry_retained)))
+    for c in range(st.nr):
+        rows = rowval[colptr[c] - 1:colptr[c + 1] - 1]
+        assert np.all(np.diff(rows) > 0) and (len(rows) == 0 or (rows[0] >= 1 and rows[-1] <= m))
+    nz = JR.reduced_jacobian_sparse(jst, F, colptr)
+    Jt = sp.csc_matrix((nz, rowval - 1, colptr - 1), shape=(m, st.nr))
+    rng = np.random.default_rng(8)
+    v, w = rng.standard_normal(st.nr), rng.standard_normal(m)
+    y, g = np.zeros(m), np.zeros(st.nr)
+    oracle.reduced_mul(sargs, jargs, y, F, v)
+    oracle.reduced_tmul(sargs, jargs, g, F, w)
+    assert np.abs(Jt @ v - y).max() <= 1e-12 * np.abs(y).max()
+    assert np.abs(Jt.T @ w - g).max() <= 1e-12 * np.abs(g).max()
+    assert np.array_equal(JR.reduced_jacobian_dense(jst, F), Jt.toarray())
