@@ -93,6 +93,12 @@ SIGNATURES = {
     "bls_check_reduced_jacobian_pattern": (ct.c_int, [VP, ct.c_int64, ct.c_int64, P64, P64]),
     "bls_reduced_jacobian_sparse_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_reduced_jacobian_dense_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_chol_create": (ct.c_int, [ct.c_int64, ct.POINTER(VP)]),
+    "bls_chol_destroy": (None, [VP]),
+    "bls_chol_factor_f64": (ct.c_int, [VP, VP, P64, VP]),
+    "bls_chol_factor_f64_async": (ct.c_int, [VP, VP, VP]),
+    "bls_chol_solve_f64": (ct.c_int, [VP, VP, VP, VP]),
+    "bls_chol_factor_copy_f64": (ct.c_int, [VP, VP, VP]),
     "bls_cond_create": (ct.c_int, [ct.c_int64, ct.POINTER(VP)]),
     "bls_cond_destroy": (None, [VP]),
     "bls_cond_is_stale": (ct.c_int, [VP]),
@@ -699,6 +705,36 @@ def prepare_scale_(c, diag_, lam, sc, elem_dtype=None, stream=None):
     f32 = 1 if elem_dtype == torch.float32 else 0
     _check(lib().bls_prepare_scale_f64(c.h, _dp(c.s), _dp(c.d), _dp(sc), float(lam), f32, _stream(stream)))
     return c.s
+
+
+class DenseCholesky:
+    """The :cholesky branch of solve_reduced_dense! (src/schur_linear_systems.jl:312-319) for a dense reduced system
+    of order n (the C ABI's bls_chol): factor(S) -> (True, 0) or (False, first failing column, 1-based);
+    solve(g, x) -> x = S^-1 g with the last factor."""
+
+    def __init__(self, n):
+        h = VP()
+        _check(lib().bls_chol_create(n, ct.byref(h)))
+        self.h = h
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_chol_destroy(self.h)
+            self.h = None
+
+    def factor(self, S, stream=None):
+        fc = ct.c_int64(0)
+        rc = _check(lib().bls_chol_factor_f64(self.h, _dp(S), ct.byref(fc), _stream(stream)))
+        return rc == BLS_OK, int(fc.value)
+
+    def solve(self, g, x, stream=None):
+        _check(lib().bls_chol_solve_f64(self.h, _dp(g), _dp(x), _stream(stream)))
+        return x
+
+    def factor_copy(self, U, stream=None):
+        _check(lib().bls_chol_factor_copy_f64(self.h, _dp(U), _stream(stream)))
+        return U
 
 
 def factor_diagonal_blocks_(D, blocks, stream=None):

@@ -412,6 +412,23 @@ int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *
  * filled, length nr + sum of e_dims; a retained block whose diagonal block a sparse R does not store reads zeros. */
 int bls_asm_diag_f64(bls_asm *a, const double *E_data, const double *R, double *d, void *stream);
 
+/* ---- The dense reduced solve (src/schur_linear_systems.jl:312-319) ----------------------------------------
+ * The :cholesky branch of solve_reduced_dense!: S = U'U for the dense reduced system S (n x n column-major, its
+ * upper triangle as reduce_schur_dense! writes it; the lower triangle is not read), factored in a copy held by the
+ * handle (S is left as built), then x = S^-1 g. The factorization is DampedLinearSolve's (LAPACK potrf): a pivot that
+ * is not positive (or NaN) stops it, BLS_FAILED with the 1-based column in *failed_col (NotPositiveDefinite: the
+ * caller falls back to an indefinite factorization). */
+typedef struct bls_chol bls_chol;
+int bls_chol_create(int64_t n, bls_chol **out);
+void bls_chol_destroy(bls_chol *h);
+int bls_chol_factor_f64(bls_chol *h, const double *S, int64_t *failed_col, void *stream);
+/* The same, queued without waiting: a failure is then seen by bls_chol_solve_f64 (which leaves x as g) and by the
+ * next bls_chol_factor_f64. */
+int bls_chol_factor_f64_async(bls_chol *h, const double *S, void *stream);
+int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void *stream);
+/* The factor U (upper triangle of an n x n column-major array; the lower triangle is unspecified), for tests. */
+int bls_chol_factor_copy_f64(bls_chol *h, double *U, void *stream);
+
 /* ---- The conditioning scale (src/conditioning.jl:24-87) ----------------------------------------------------
  * bls_cond holds EliminationConditioning's state (:47-55) except its vectors: whether the next prepare computes the
  * scale afresh (stale; set at creation and by bls_cond_markstale, as zero! and markstale! set it) and the number of

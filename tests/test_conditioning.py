@@ -118,3 +118,38 @@ def test_table_quantities_match_dense_J(blsamd, kind):
     assert np.abs(g - gr).max() <= 1e-12 * np.abs(gr).max()
     assert np.abs(d - dr).max() <= 1e-12 * dr.max()
     assert abs(q - qr) <= 1e-12 * qr
+
+
+def _spd(rng, n, cond=1e3):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (Q * np.geomspace(1.0, cond, n)) @ Q.T
+
+
+@pytest.mark.parametrize("n", [1, 5, 40])
+def test_dense_cholesky_oracle(n):
+    """U'U == S, U == numpy's (LAPACK potrf) Cholesky factor transposed, and the solve == numpy.linalg.solve."""
+    import dense_ref as D
+    rng = np.random.default_rng(n)
+    S = _spd(rng, n)
+    U, info = D.cholesky_upper(S)
+    assert info == 0
+    assert np.abs(U.T @ U - S).max() <= 1e-13 * np.abs(S).max()
+    assert np.abs(U - np.linalg.cholesky(S).T).max() <= 1e-12 * np.abs(U).max()
+    g = rng.standard_normal(n)
+    x = D.cholesky_solve(U, g)
+    assert np.abs(x - np.linalg.solve(S, g)).max() <= 1e-10 * np.abs(x).max()
+
+
+def test_dense_cholesky_oracle_failing_column():
+    """The first leading minor that is not positive definite: a negative pivot at column 4, an exactly singular
+    2 x 2 minor ([[1, 1], [1, 1]]), a NaN at column 3, and numpy agreeing that the matrix is not SPD."""
+    import dense_ref as D
+    S = np.diag([1.0, 2.0, 3.0, -1.0, 5.0])
+    assert D.cholesky_upper(S)[1] == 4
+    S = np.array([[1.0, 1.0, 0.0], [1.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    assert D.cholesky_upper(S)[1] == 2
+    S = np.eye(4)
+    S[2, 2] = np.nan
+    assert D.cholesky_upper(S)[1] == 3
+    with pytest.raises(np.linalg.LinAlgError):
+        np.linalg.cholesky(np.diag([1.0, 2.0, 3.0, -1.0, 5.0]))
