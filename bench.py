@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 METRIC = "landmarks eliminated/s + time-to-1e-8 PCG residual, FP64 plain vs mixed/learned"
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TF = 78.6
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_phase2_slam_traffic.md")
 
 
@@ -191,6 +192,8 @@ def main():
     }
     del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
     torch.cuda.empty_cache()
+    result["dense_cholesky"] = dense_leg(torch, dev, int(st.nr))
+    torch.cuda.empty_cache()
     if args.qr_scale > 0:
         result["reduce_qr"] = qr_leg(torch, dev, b, args.qr_scale)
         torch.cuda.empty_cache()
@@ -303,6 +306,30 @@ def assembly_leg(torch, dev, st, reps=10):
             "conditioning": {"variables": n, "getdiag_ms": diag_ms, "fresh_fill_ms": fms, "later_damping_ms": later_ms,
                              "fresh_fill_GBs": fbytes / (fms * 1e-3) / 1e9,
                              "later_damping_GBs": 24.0 * n / (later_ms * 1e-3) / 1e9}}
+
+
+def dense_leg(torch, dev, n, reps=5):
+    """The dense reduced solve at the SLAM system's order (n = nr): Cholesky of a copy of S (the :cholesky branch of
+    solve_reduced_dense!) and the two triangular solves, on a synthetic SPD S = A A'/n + I (A ~ N(0, 1)). FLOP figure:
+    n^3 / 3 for the factorization (its trailing updates run on the FP64 matrix instruction)."""
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
+    S = (A @ A.T / n + torch.eye(n, dtype=torch.float64, device="cuda")).contiguous()
+    del A
+    g = torch.randn(n, dtype=torch.float64, device="cuda", generator=gen)
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    ch = dev.DenseCholesky(n)
+    ok, _ = ch.factor(S)
+    assert ok
+    st = dev._stream(None)
+    fms = time_ms(torch, lambda: dev.lib().bls_chol_factor_f64_async(ch.h, dev._dp(S), st), reps)
+    sms = time_ms(torch, lambda: ch.solve(g, x), reps)
+    rel = float(torch.linalg.norm(S @ x - g) / torch.linalg.norm(g))
+    tf = n ** 3 / 3.0 / (fms * 1e-3) / 1e12
+    log(f"dense reduced solve, n = {n}: Cholesky {fms:.3f} ms ({tf:.1f} TFLOP/s, {100 * tf / FP64_MFMA_PEAK_TF:.1f}% of "
+        f"the FP64 matrix peak), solve {sms:.3f} ms, ||S x - g|| / ||g|| = {rel:.1e}")
+    return {"n": n, "factor_ms": fms, "solve_ms": sms, "factor_TFLOPs": tf, "frac_fp64_mfma": tf / FP64_MFMA_PEAK_TF,
+            "relres": rel}
 
 
 def mixed_leg(torch, dev, p, h, S64, csc, dmap, steps, warmup):

@@ -11,8 +11,8 @@
 //   k_chol_syrk   the trailing upper triangle S(i, j) -= U(k, i)' U(k, j), one workgroup per 64 x 64 tile (tiles
 //                 below the diagonal skipped), the two row-panel slices staged in LDS and multiplied on the FP64
 //                 matrix instruction (16 x 16 x 4), each wavefront one 32 x 32 quadrant.
-// The solve runs U'y = g then U x = y, NB rows at a time: one workgroup solves the diagonal block, then every other
-// thread updates the rest of the vector with the block's panel. Every sum is taken in a fixed order (no atomics):
+// The solve runs U'y = g then U x = y, NB rows at a time: one wavefront solves the diagonal block, then the rest of
+// the vector is updated with the block's panel (a wavefront per entry going forward, a thread per entry going back). Every sum is taken in a fixed order (no atomics):
 // bitwise reproducible. A failure found by k_chol_diag is read by every later kernel, which then does nothing.
 #include <algorithm>
 #include <cmath>
@@ -26,77 +26,125 @@ constexpr int NB = 64;
 constexpr int SLD = 72;   // LDS row stride of the staged panels (doubles): two-way bank use, the minimum for FP64
 typedef double cv4d __attribute__((ext_vector_type(4)));
 
-// U_kk from the block's rows k0 .. k0 + kb of A (upper triangle, column-major, leading dimension n): for each row j
-// in turn, the pivot's square root, the row divided by it, and the trailing upper triangle of the block updated.
-// A pivot that is not positive (or NaN) records column k0 + j + 1 in *info and stops.
-__global__ __launch_bounds__(256) void k_chol_diag(int64_t n, int64_t k0, int kb, double *__restrict__ A,
-                                                   int64_t *__restrict__ info) {
-    __shared__ double T[NB][NB + 1];
-    __shared__ int failed;
-    if (*info) return;
-    for (int w = threadIdx.x; w < kb * kb; w += blockDim.x) {
-        const int c = w / kb, r = w - c * kb;
-        if (r <= c) T[r][c] = A[(k0 + r) + (k0 + c) * n];
+// lane i's v, to every lane (i the same for the whole wavefront)
+__device__ inline double bcast(double v, int i) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
+    return __hiloint2double(hi, lo);
+}
+
+// The 64 x 64 diagonal block's upper triangle (rows and columns past kb: zero) into LDS, the global loads of each
+// thread all issued before its LDS stores (16 per thread)
+__device__ inline void stage_diag_block(int64_t n, int64_t k0, int kb, const double *__restrict__ A,
+                                        double (*T)[NB + 1], double pad) {
+    double v[NB * NB / 256];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+        const int w = threadIdx.x + 256 * q, c = w >> 6, r = w & 63;
+        v[q] = (r < kb && c < kb && r <= c) ? A[(k0 + r) + (k0 + c) * n] : (r == c ? pad : 0.0);
     }
-    if (threadIdx.x == 0) failed = 0;
-    __syncthreads();
-    for (int j = 0; j < kb; j++) {
-        if (threadIdx.x == 0) {
-            const double piv = T[j][j];
-            if (!(piv > 0.0)) {
-                failed = 1;
-                *info = k0 + j + 1;
-            } else {
-                T[j][j] = sqrt(piv);
-            }
-        }
-        __syncthreads();
-        if (failed) return;
-        const double djj = T[j][j];
-        for (int c = j + 1 + (int)threadIdx.x; c < kb; c += blockDim.x) T[j][c] = T[j][c] / djj;
-        __syncthreads();
-        const int m = kb - j - 1;   // the trailing m x m upper triangle, as pairs (r <= c)
-        for (int w = threadIdx.x; w < m * m; w += blockDim.x) {
-            const int c = j + 1 + w / m, r = j + 1 + w % m;
-            if (r <= c) T[r][c] -= T[j][r] * T[j][c];
-        }
-        __syncthreads();
-    }
-    for (int w = threadIdx.x; w < kb * kb; w += blockDim.x) {
-        const int c = w / kb, r = w - c * kb;
-        if (r <= c) A[(k0 + r) + (k0 + c) * n] = T[r][c];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+        const int w = threadIdx.x + 256 * q;
+        T[w & 63][w >> 6] = v[q];
     }
 }
 
-// U(k, j) = U_kk^-T A(k, j) for the columns j >= k0 + kb, TRSM_COLS columns per workgroup, one wavefront per
-// column at a time: lane i holds row i; at step i, y_i = v_i / U_ii is broadcast and every lane t > i subtracts
+// U_kk from the block's rows k0 .. k0 + kb of A (upper triangle, column-major, leading dimension n), row by row of U
+// with one barrier per row. Thread (ty, tx) of a 16 x 16 grid keeps the 4 x 4 entries (rows 4 ty + a, columns
+// 4 tx + b) in registers, so wavefront w holds rows 16 w .. 16 w + 15 and drops out once the pivot row has passed
+// them. Only the pivot row is shared: at row j every thread reads row j of T (published by its owners before the
+// barrier) and forms u_jj = sqrt(T[j][j]) and its reciprocal itself; the owners of row j write U's row j (scaled by
+// the reciprocal, as LAPACK's dpotf2 scales it) to Uo, every entry of the trailing triangle takes
+// t -= (T[j][r] / u_jj) (T[j][c] / u_jj), and the owners of row j + 1 publish it. A pivot that is not positive (or
+// NaN) records column k0 + j + 1 in *info and stops.
+__global__ __launch_bounds__(256) void k_chol_diag(int64_t n, int64_t k0, int kb, double *__restrict__ A,
+                                                   int64_t *__restrict__ info) {
+    __shared__ double T[NB][NB + 1], Uo[NB][NB + 1];
+    if (*info) return;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, wave = threadIdx.x >> 6;
+    const int r0 = 4 * ty, c0 = 4 * tx;
+    stage_diag_block(n, k0, kb, A, T, 0.0);
+    __syncthreads();
+    double t[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) t[a][b] = T[r0 + a][c0 + b];
+    for (int j = 0; j < kb; j++) {
+        const double piv = T[j][j];
+        if (!(piv > 0.0)) {   // every thread sees the same pivot
+            if (threadIdx.x == 0) *info = k0 + j + 1;
+            return;
+        }
+        const double ujj = sqrt(piv), rinv = 1.0 / ujj;
+        if (ty == (j >> 2)) {   // U's row j
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int c = c0 + b;
+                if (c >= j && c < kb) Uo[j][c] = c == j ? ujj : T[j][c] * rinv;
+            }
+        }
+        if (16 * wave + 15 > j) {   // this wavefront still holds rows below the pivot
+            double ur[4], uc[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) ur[a] = T[j][r0 + a] * rinv;
+#pragma unroll
+            for (int b = 0; b < 4; b++) uc[b] = T[j][c0 + b] * rinv;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    t[a][b] = (r0 + a > j && c0 + b >= r0 + a) ? t[a][b] - ur[a] * uc[b] : t[a][b];
+            const int nx = j + 1;   // publish row j + 1 (final now) for the next step
+            if (nx < kb && ty == (nx >> 2)) {
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+                    if (a == (nx & 3))
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            if (c0 + b >= nx && c0 + b < kb) T[nx][c0 + b] = t[a][b];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+        const int w = threadIdx.x + 256 * q, c = w >> 6, r = w & 63;
+        if (r < kb && c < kb && r <= c) A[(k0 + r) + (k0 + c) * n] = Uo[r][c];
+    }
+}
+
+// U(k, j) = U_kk^-T A(k, j) for the columns j >= k0 + kb, TRSM_COLS columns per workgroup, four per wavefront
+// side by side: lane i holds row i; at step i, y_i = v_i (1 / U_ii) is broadcast and every lane t > i subtracts
 // U(i, t) y_i -- forward substitution, each v_t reduced in row order. Rows past kb are padded (unit diagonal, zero
 // coupling).
-constexpr int TRSM_COLS = 64;
+constexpr int TRSM_COLS = 16;
 __global__ __launch_bounds__(256) void k_chol_trsm(int64_t n, int64_t k0, int kb, double *__restrict__ A,
                                                    const int64_t *__restrict__ info) {
     __shared__ double U[NB][NB + 1];
+    __shared__ double dinv[NB];
     if (*info) return;
-    for (int w = threadIdx.x; w < NB * NB; w += blockDim.x) {
-        const int c = w / NB, r = w - c * NB;
-        double v = (r == c) ? 1.0 : 0.0;
-        if (r < kb && c < kb && r <= c) v = A[(k0 + r) + (k0 + c) * n];
-        U[r][c] = v;
-    }
+    stage_diag_block(n, k0, kb, A, U, 1.0);
+    __syncthreads();
+    if (threadIdx.x < NB) dinv[threadIdx.x] = 1.0 / U[threadIdx.x][threadIdx.x];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int q = wave; q < TRSM_COLS; q += 4) {
-        const int64_t col = k0 + kb + (int64_t)blockIdx.x * TRSM_COLS + q;
-        if (col >= n) break;   // the same for the whole wavefront
-        double *a = A + k0 + col * n;
-        double v = lane < kb ? a[lane] : 0.0;
-        for (int i = 0; i < kb; i++) {
-            const double yi = __shfl(v, i, 64) / U[i][i];
-            if (lane == i) v = yi;
-            else if (lane > i) v -= U[i][lane] * yi;
+    const int64_t cb = k0 + kb + (int64_t)blockIdx.x * TRSM_COLS + 4 * wave;   // four columns per wavefront
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = (cb + u < n && lane < kb) ? A[k0 + lane + (cb + u) * n] : 0.0;
+    for (int i = 0; i < kb; i++) {   // the four substitutions interleaved
+        const double di = dinv[i], uil = U[i][lane];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double yi = bcast(v[u], i) * di;
+            v[u] = lane == i ? yi : (lane > i ? v[u] - uil * yi : v[u]);
         }
-        if (lane < kb) a[lane] = v;
     }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+        if (cb + u < n && lane < kb) A[k0 + lane + (cb + u) * n] = v[u];
 }
 
 // A(i, j) -= U(k, i)' U(k, j) over the trailing upper triangle, one workgroup per 64 x 64 tile (ti <= tj) of the
@@ -116,11 +164,21 @@ __global__ __launch_bounds__(256) void k_chol_syrk(int64_t n, int64_t k0, int kb
     }
     const int64_t ti = rem;
     const int64_t r0 = t0 + NB * ti, c0 = t0 + NB * tj;
-    for (int w = threadIdx.x; w < NB * NB; w += blockDim.x) {
-        const int c = w / NB, p = w - c * NB;         // p fastest: the panel rows of one column are contiguous
-        const bool ok = p < kb;
-        Pi[p][c] = (ok && r0 + c < n) ? A[(k0 + p) + (r0 + c) * n] : 0.0;
-        Pj[p][c] = (ok && c0 + c < n) ? A[(k0 + p) + (c0 + c) * n] : 0.0;
+    {   // p fastest: the panel rows of one column are contiguous; all loads issued before the LDS stores
+        double vi[NB * NB / 256], vj[NB * NB / 256];
+#pragma unroll
+        for (int q = 0; q < NB * NB / 256; q++) {
+            const int w = threadIdx.x + 256 * q, c = w >> 6, p = w & 63;
+            const bool ok = p < kb;
+            vi[q] = (ok && r0 + c < n) ? A[(k0 + p) + (r0 + c) * n] : 0.0;
+            vj[q] = (ok && c0 + c < n) ? A[(k0 + p) + (c0 + c) * n] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < NB * NB / 256; q++) {
+            const int w = threadIdx.x + 256 * q;
+            Pi[w & 63][w >> 6] = vi[q];
+            Pj[w & 63][w >> 6] = vj[q];
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
@@ -155,50 +213,72 @@ __global__ __launch_bounds__(256) void k_chol_syrk(int64_t n, int64_t k0, int kb
             }
 }
 
+// The diagonal block's rows into LDS by one wavefront, lane t reading row t of every column (coalesced), in groups
+// of 16 loads issued before their stores
+__device__ inline void stage_rows(int64_t n, int64_t k0, int kb, const double *__restrict__ A, double (*U)[NB + 1]) {
+    const int t = threadIdx.x;
+    for (int c0 = 0; c0 < NB; c0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = c0 + q;
+            v[q] = (t < kb && c < kb && t <= c) ? A[(k0 + t) + (k0 + c) * n] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) U[t][c0 + q] = v[q];
+    }
+}
+
 // U'y = g, block k: solve the diagonal block (rows k0 .. k0 + kb of x, already updated by the earlier blocks)
 __global__ __launch_bounds__(NB) void k_chol_fwd_diag(int64_t n, int64_t k0, int kb, const double *__restrict__ A,
                                                       double *__restrict__ x, const int64_t *__restrict__ info) {
-    __shared__ double y[NB];
+    __shared__ double U[NB][NB + 1];
+    __shared__ double dinv[NB];
     if (*info) return;
     const int t = threadIdx.x;
-    if (t < kb) y[t] = x[k0 + t];
+    stage_rows(n, k0, kb, A, U);
+    double v = t < kb ? x[k0 + t] : 0.0;
     __syncthreads();
-    for (int i = 0; i < kb; i++) {
-        if (t == i) y[i] = y[i] / A[(k0 + i) + (k0 + i) * n];
-        __syncthreads();
-        if (t > i && t < kb) y[t] -= A[(k0 + i) + (k0 + t) * n] * y[i];
-        __syncthreads();
+    dinv[t] = t < kb ? 1.0 / U[t][t] : 1.0;
+    __syncthreads();
+    for (int i = 0; i < kb; i++) {   // lane t holds row t: y_i broadcast, then every t > i updated
+        const double yi = bcast(v, i) * dinv[i];
+        v = t == i ? yi : (t > i ? v - U[i][t] * yi : v);
     }
-    if (t < kb) x[k0 + t] = y[t];
+    if (t < kb) x[k0 + t] = v;
 }
 
-// x_j -= U(k, j)' y_k for the rows j >= k0 + kb, one thread per row, the block's values added in row order
+// x_j -= U(k, j)' y_k for the rows j >= k0 + kb, one wavefront per row j: lane p takes U(k0 + p, j) y_p (the
+// column's kb values, contiguous), added in a fixed tree
 __global__ __launch_bounds__(256) void k_chol_fwd_update(int64_t n, int64_t k0, int kb, const double *__restrict__ A,
                                                          double *__restrict__ x, const int64_t *__restrict__ info) {
     if (*info) return;
-    const int64_t j = k0 + kb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const double *u = A + k0 + j * n;
-    double s = 0.0;
-    for (int p = 0; p < kb; p++) s += u[p] * x[k0 + p];
-    x[j] -= s;
+    const int64_t j = k0 + kb + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int p = threadIdx.x & 63;
+    if (j >= n) return;   // the whole wavefront
+    double s = p < kb ? A[k0 + p + j * n] * x[k0 + p] : 0.0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (p == 0) x[j] -= s;
 }
 
 // U x = y, block k (from the last): solve the diagonal block, rows k0 .. k0 + kb
 __global__ __launch_bounds__(NB) void k_chol_bwd_diag(int64_t n, int64_t k0, int kb, const double *__restrict__ A,
                                                       double *__restrict__ x, const int64_t *__restrict__ info) {
-    __shared__ double y[NB];
+    __shared__ double U[NB][NB + 1];
+    __shared__ double dinv[NB];
     if (*info) return;
     const int t = threadIdx.x;
-    if (t < kb) y[t] = x[k0 + t];
+    stage_rows(n, k0, kb, A, U);
+    double v = t < kb ? x[k0 + t] : 0.0;
     __syncthreads();
-    for (int i = kb - 1; i >= 0; i--) {
-        if (t == i) y[i] = y[i] / A[(k0 + i) + (k0 + i) * n];
-        __syncthreads();
-        if (t < i) y[t] -= A[(k0 + t) + (k0 + i) * n] * y[i];
-        __syncthreads();
+    dinv[t] = t < kb ? 1.0 / U[t][t] : 1.0;
+    __syncthreads();
+    for (int i = kb - 1; i >= 0; i--) {   // lane t holds row t: x_i broadcast, then every t < i updated
+        const double xi = bcast(v, i) * dinv[i];
+        v = t == i ? xi : (t < i ? v - U[t][i] * xi : v);
     }
-    if (t < kb) x[k0 + t] = y[t];
+    if (t < kb) x[k0 + t] = v;
 }
 
 // y_i -= U(i, k) x_k for the rows i < k0, one thread per row (consecutive rows: coalesced down each column)
@@ -290,7 +370,7 @@ extern "C" int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void 
     for (int64_t k0 = 0; k0 < n; k0 += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k0);
         k_chol_fwd_diag<<<1, NB, 0, s>>>(n, k0, kb, A, x, info);
-        if (n - k0 - kb > 0) k_chol_fwd_update<<<grid_for(n - k0 - kb, 256), 256, 0, s>>>(n, k0, kb, A, x, info);
+        if (n - k0 - kb > 0) k_chol_fwd_update<<<grid_for(n - k0 - kb, 4), 256, 0, s>>>(n, k0, kb, A, x, info);
     }
     for (int64_t k0 = ((n - 1) / NB) * NB; n && k0 >= 0; k0 -= NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k0);

@@ -2,7 +2,8 @@
 
 The reference's dense reduced solve (solve_reduced_dense!, src/schur_linear_systems.jl:312-319) factors a copy of S
 with Cholesky through DampedLinearSolve 0.1.1, a dependency absent here, whose Cholesky is LAPACK's potrf (upper):
-S = U'U, U upper triangular with a positive diagonal, computed row by row of U; a leading minor that is not positive
+S = U'U, U upper triangular with a positive diagonal, computed row by row of U (dpotf2: each row scaled by the
+reciprocal of its pivot); a leading minor that is not positive
 definite stops it (info = its order). Restated here unblocked, in numpy Float64.
 
 Parity pinning (tests/test_conditioning.py): U'U == S and U equal to numpy.linalg.cholesky's (LAPACK) transposed on
@@ -22,7 +23,7 @@ def cholesky_upper(S):
         if not piv > 0.0:
             return U, j + 1
         U[j, j] = np.sqrt(piv)
-        U[j, j + 1:] /= U[j, j]
+        U[j, j + 1:] *= 1.0 / U[j, j]          # dpotf2 scales the row by the reciprocal
         for r in range(j + 1, n):
             U[r, r:] -= U[j, r] * U[j, r:]
     return U, 0
