@@ -5,7 +5,7 @@
 // potrf); a matrix that is not positive definite returns BLS_FAILED with the first column whose pivot is not
 // positive (NotPositiveDefinite), the caller's cue to fall back to an indefinite factorization.
 //
-// Blocked right-looking factorization, panels of NB = 64 rows of U:
+// Blocked right-looking factorization with a look-ahead of one panel, panels of NB = 64 rows of U:
 //   k_chol_diag   one workgroup factors the diagonal block in LDS (unblocked, row by row of U);
 //   k_chol_trsm   the row panel U(k, j) = U_kk^-T S(k, j), one wavefront per column (a lane per row), U_kk in LDS;
 //   k_chol_syrk   the trailing upper triangle S(i, j) -= U(k, i)' U(k, j), one workgroup per 64 x 64 tile (tiles
@@ -24,6 +24,7 @@ using namespace bls;
 namespace {
 constexpr int NB = 64;
 constexpr int SLD = 72;   // LDS row stride of the staged panels (doubles): two-way bank use, the minimum for FP64
+constexpr int SYRK_ALL = 0, SYRK_FIRST = 1, SYRK_REST = 2;   // which tiles of the trailing update
 typedef double cv4d __attribute__((ext_vector_type(4)));
 
 // lane i's v, to every lane (i the same for the whole wavefront)
@@ -148,21 +149,37 @@ __global__ __launch_bounds__(256) void k_chol_trsm(int64_t n, int64_t k0, int kb
 }
 
 // A(i, j) -= U(k, i)' U(k, j) over the trailing upper triangle, one workgroup per 64 x 64 tile (ti <= tj) of the
-// trailing columns, the tiles listed column by column. The row-panel slices of both tiles are staged in LDS as
+// trailing columns, the tiles listed column by column: all of them, the first tile row only (the next panel's rows),
+// or the others. The row-panel slices of both tiles are staged in LDS as
 // P[p][c] (p the panel row); wavefront w takes quadrant (w & 1, w >> 1), 2 x 2 tiles of 16 x 16, k in steps of 4:
 // lane l supplies A(l % 16, l / 16) = P[k + l / 16][i] and B(l / 16, l % 16) = P[k + l / 16][j], and accumulator v
 // holds D(l / 16 + 4 v, l % 16). The accumulators start from A(i, j) and take the negated products.
 __global__ __launch_bounds__(256) void k_chol_syrk(int64_t n, int64_t k0, int kb, double *__restrict__ A,
-                                                   const int64_t *__restrict__ info) {
+                                                   const int64_t *__restrict__ info, int rows) {
     __shared__ double Pi[NB][SLD], Pj[NB][SLD];
     if (*info) return;
     const int64_t t0 = k0 + kb;                       // first trailing row / column
-    int64_t tj = 0, rem = blockIdx.x;
-    while (rem > tj) {                                // tile (ti, tj): column tj holds tj + 1 tiles
-        rem -= tj + 1;
-        tj++;
+    int64_t ti, tj;
+    if (rows == SYRK_FIRST) {                         // the first tile row: tile (0, blockIdx)
+        ti = 0;
+        tj = blockIdx.x;
+    } else if (rows == SYRK_REST) {                   // the others: column tj >= 1 holds tiles 1 .. tj
+        int64_t rem = blockIdx.x;
+        tj = 1;
+        while (rem >= tj) {
+            rem -= tj;
+            tj++;
+        }
+        ti = rem + 1;
+    } else {                                          // all: column tj holds tiles 0 .. tj
+        int64_t rem = blockIdx.x;
+        tj = 0;
+        while (rem > tj) {
+            rem -= tj + 1;
+            tj++;
+        }
+        ti = rem;
     }
-    const int64_t ti = rem;
     const int64_t r0 = t0 + NB * ti, c0 = t0 + NB * tj;
     {   // p fastest: the panel rows of one column are contiguous; all loads issued before the LDS stores
         double vi[NB * NB / 256], vj[NB * NB / 256];
@@ -305,6 +322,14 @@ struct bls_chol {
     bool factored = false;
     DevArray<double> U;       // the factor (upper triangle of an n x n column-major array)
     DevArray<int64_t> info;   // 0, or the first column (1-based) whose pivot is not positive
+    // the look-ahead: the bulk of each trailing update runs on a second stream (default priority)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
+    ~bls_chol() {
+        if (ev_panel) (void)hipEventDestroy(ev_panel);
+        if (ev_bulk) (void)hipEventDestroy(ev_bulk);
+        if (s2) (void)hipStreamDestroy(s2);
+    }
 };
 
 extern "C" int bls_chol_create(int64_t n, bls_chol **out) {
@@ -317,6 +342,13 @@ extern "C" int bls_chol_create(int64_t n, bls_chol **out) {
     if ((rc = h->U.alloc((size_t)std::max<int64_t>(1, n * n))) || (rc = h->info.alloc(1))) {
         delete h;
         return rc;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&h->s2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_bulk, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(BLS_ERR_HIP, std::string("bls_chol_create: ") + hipGetErrorString(e));
     }
     *out = h;
     return BLS_OK;
@@ -333,16 +365,30 @@ extern "C" int bls_chol_factor_f64_async(bls_chol *h, const double *S, void *str
     int64_t *info = h->info.ptr;
     k_chol_zero<<<1, 1, 0, s>>>(info);
     if (n) k_chol_copy<<<(unsigned)std::min<int64_t>(8192, grid_for(n * n, 256)), 256, 0, s>>>(n * n, S, A);
+    // Panel k on the caller's stream: its row solve, then the first tile row of its trailing update (the rows of
+    // panel k + 1), then panel k + 1's diagonal block. The other tile rows of the update go to the second stream once
+    // the row solve is done, and run beside panel k + 1's diagonal block and row solve; panel k + 1's own update waits
+    // for them (they write the rows of panel k + 2 it reads and updates).
+    bool bulk_pending = false;
+    if (n) k_chol_diag<<<1, 256, 0, s>>>(n, 0, (int)std::min<int64_t>(NB, n), A, info);
     for (int64_t k0 = 0; k0 < n; k0 += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k0);
-        k_chol_diag<<<1, 256, 0, s>>>(n, k0, kb, A, info);
         const int64_t rest = n - k0 - kb;
-        if (rest > 0) {
-            k_chol_trsm<<<grid_for(rest, TRSM_COLS), 256, 0, s>>>(n, k0, kb, A, info);
-            const int64_t nt = (rest + NB - 1) / NB;
-            k_chol_syrk<<<(unsigned)(nt * (nt + 1) / 2), 256, 0, s>>>(n, k0, kb, A, info);
+        if (rest <= 0) break;
+        k_chol_trsm<<<grid_for(rest, TRSM_COLS), 256, 0, s>>>(n, k0, kb, A, info);
+        if (bulk_pending) BLS_HIP(hipStreamWaitEvent(s, h->ev_bulk, 0));   // panel k - 1's bulk update
+        const int64_t nt = (rest + NB - 1) / NB;
+        bulk_pending = nt > 1;
+        if (bulk_pending) {
+            BLS_HIP(hipEventRecord(h->ev_panel, s));
+            BLS_HIP(hipStreamWaitEvent(h->s2, h->ev_panel, 0));
+            k_chol_syrk<<<(unsigned)(nt * (nt - 1) / 2), 256, 0, h->s2>>>(n, k0, kb, A, info, SYRK_REST);
+            BLS_HIP(hipEventRecord(h->ev_bulk, h->s2));
         }
+        k_chol_syrk<<<(unsigned)nt, 256, 0, s>>>(n, k0, kb, A, info, SYRK_FIRST);
+        k_chol_diag<<<1, 256, 0, s>>>(n, k0 + kb, (int)std::min<int64_t>(NB, rest), A, info);
     }
+    if (bulk_pending) BLS_HIP(hipStreamWaitEvent(s, h->ev_bulk, 0));
     BLS_HIP(hipGetLastError());
     h->factored = true;
     return BLS_OK;
