@@ -305,7 +305,15 @@ __global__ __launch_bounds__(256) void k_chol_bwd_update(int64_t n, int64_t k0, 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k0) return;
     double s = 0.0;
-    for (int c = 0; c < kb; c++) s += A[i + (k0 + c) * n] * x[k0 + c];
+    int c = 0;
+    for (; c + 16 <= kb; c += 16) {   // sixteen loads in flight, added in column order
+        double u[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) u[q] = A[i + (k0 + c + q) * n];
+#pragma unroll
+        for (int q = 0; q < 16; q++) s += u[q] * x[k0 + c + q];
+    }
+    for (; c < kb; c++) s += A[i + (k0 + c) * n] * x[k0 + c];
     x[i] -= s;
 }
 
