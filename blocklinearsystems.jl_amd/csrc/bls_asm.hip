@@ -133,23 +133,24 @@ struct HDotArgs {
 // src/linalg.jl:208-260), as one sum of per-element terms: lane q takes element q (and q + 64, ...) of E_e and of
 // the couplings' blocks, which lie one after another in C_data, so the wavefront reads them as one contiguous run;
 // the lanes' sums are then added in a fixed tree
+template <class T>
 __global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ E, const double *__restrict__ C,
+                                                const T *__restrict__ E, const T *__restrict__ C,
                                                 double *__restrict__ part) {
     const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (e >= a.neb) return;   // the whole wavefront: no block-level synchronisation below
     const int d = a.de[e];
     const double *xe = x + a.nr + a.boff_e[e], *ye = y + a.nr + a.boff_e[e];
-    const double *Ee = E + a.eoff[e];
+    const T *Ee = E + a.eoff[e];
     double acc = 0.0;
     for (int w = lane; w < d * d; w += 64) {
         const int col = w / d, row = w - col * d;
-        acc += Ee[w] * xe[row] * ye[col];
+        acc += (double)Ee[w] * xe[row] * ye[col];
     }
     const int64_t c0 = a.cptr[e], c1 = a.cptr[e + 1];
     if (c1 > c0) {
-        const double *span = C + a.coff[c0];
+        const T *span = C + a.coff[c0];
         int64_t c = c0, start = 0;   // the coupling holding element q, and where its block starts in the span
         int len = d * a.rd[a.c_r[c0]];
         for (int64_t q = lane;; q += 64) {
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__rest
             if (c >= c1) break;
             const int w = (int)(q - start), ir = w / d, i = w - ir * d;
             const int64_t ro = a.boff_r[a.c_r[c]];
-            acc += span[q] * (x[ro + ir] * ye[i] + y[ro + ir] * xe[i]);
+            acc += (double)span[q] * (x[ro + ir] * ye[i] + y[ro + ir] * xe[i]);
         }
     }
 #pragma unroll
@@ -171,18 +172,19 @@ __global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__rest
 // one lane per stored block of R (src/linalg.jl:153-190): a dense R is read in its upper triangle (each
 // off-diagonal entry counted both ways, a diagonal block by its upper triangle); a sparse R's blocks whole, an
 // off-diagonal block both ways
+template <class T>
 __global__ __launch_bounds__(256) void k_hdot_r(HDotArgs a, const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ R, double *__restrict__ part) {
+                                                const T *__restrict__ R, double *__restrict__ part) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= a.nrblk) return;
     const RGBlock bk = a.rblk[b];
     const int p = a.rp[b], q = a.rq[b];
     const double *xp = x + a.boff_r[p], *yp = y + a.boff_r[p], *xq = x + a.boff_r[q], *yq = y + a.boff_r[q];
-    const double *B = R + bk.dst;
+    const T *B = R + bk.dst;
     double acc = 0.0;
     for (int j = 0; j < bk.d2; j++)
         for (int i = 0; i < bk.d1; i++) {
-            const double v = B[(int64_t)j * bk.cstride + i];
+            const double v = (double)B[(int64_t)j * bk.cstride + i];
             if (p != q) acc += v * (xp[i] * yq[j] + yp[i] * xq[j]);
             else if (!a.dense) acc += v * xp[i] * yp[j];
             else if (i < j) acc += v * (xp[i] * yp[j] + xp[j] * yp[i]);
@@ -193,7 +195,8 @@ __global__ __launch_bounds__(256) void k_hdot_r(HDotArgs a, const double *__rest
 
 // getdiag! (src/schur_linear_systems.jl:239-270): d = [diag(R); diag(E_1); ...], one thread per block, retained
 // blocks first; a retained block whose diagonal block a sparse R does not store has a zero diagonal
-__global__ __launch_bounds__(256) void k_hdiag(HDotArgs a, const double *__restrict__ E, const double *__restrict__ R,
+template <class T>
+__global__ __launch_bounds__(256) void k_hdiag(HDotArgs a, const T *__restrict__ E, const T *__restrict__ R,
                                                double *__restrict__ d) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b < a.nrb) {
@@ -202,16 +205,16 @@ __global__ __launch_bounds__(256) void k_hdiag(HDotArgs a, const double *__restr
         double *o = d + a.boff_r[b];
         if (k >= 0) {
             const RGBlock bk = a.rblk[k];
-            for (int i = 0; i < n; i++) o[i] = R[bk.dst + (int64_t)i * bk.cstride + i];
+            for (int i = 0; i < n; i++) o[i] = (double)R[bk.dst + (int64_t)i * bk.cstride + i];
         } else {
             for (int i = 0; i < n; i++) o[i] = 0.0;
         }
     } else if (b < a.nrb + a.neb) {
         const int64_t e = b - a.nrb;
         const int n = a.de[e];
-        const double *Ee = E + a.eoff[e];
+        const T *Ee = E + a.eoff[e];
         double *o = d + a.nr + a.boff_e[e];
-        for (int i = 0; i < n; i++) o[i] = Ee[i * (n + 1)];
+        for (int i = 0; i < n; i++) o[i] = (double)Ee[i * (n + 1)];
     }
 }
 }  // namespace
@@ -589,25 +592,42 @@ HDotArgs hdot_args(const bls_asm *a) {
 }
 }  // namespace
 
-extern "C" int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
-                               const double *R, double *result, void *stream) {
+namespace {
+template <class T>
+int hessian_dot(bls_asm *a, const double *x, const double *y, const T *E_data, const T *C_data, const T *R,
+                double *result, hipStream_t s) {
     if (!a) return fail(BLS_ERR_ARG, "null handle");
     if (!x || !y || !result || (a->neb && !E_data) || (a->ncoup && !C_data) || (a->nrblk && !R))
-        return fail(BLS_ERR_ARG, "bls_asm_dot_f64: a required array is NULL");
-    hipStream_t s = (hipStream_t)stream;
+        return fail(BLS_ERR_ARG, "dot(x, H, y): a required array is NULL");
     const HDotArgs d = hdot_args(a);
-    if (a->neb) k_hdot_e<<<(unsigned)((a->neb + 3) / 4), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
-    if (a->nrblk) k_hdot_r<<<(unsigned)((a->nrblk + 255) / 256), 256, 0, s>>>(d, x, y, R, a->d_part.ptr);
+    if (a->neb) k_hdot_e<T><<<(unsigned)((a->neb + 3) / 4), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
+    if (a->nrblk) k_hdot_r<T><<<(unsigned)((a->nrblk + 255) / 256), 256, 0, s>>>(d, x, y, R, a->d_part.ptr);
     fsum_launch(a->neb + a->nrblk, a->d_part.ptr, a->d_part.ptr + a->neb + a->nrblk, result, s);   // FSUM_CHUNKS slots after the partials
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
 
-extern "C" int bls_asm_diag_f64(bls_asm *a, const double *E_data, const double *R, double *d, void *stream) {
+template <class T> int hessian_diag(bls_asm *a, const T *E_data, const T *R, double *d, hipStream_t s) {
     if (!a) return fail(BLS_ERR_ARG, "null handle");
-    if (!d || (a->neb && !E_data) || (a->nrblk && !R)) return fail(BLS_ERR_ARG, "bls_asm_diag_f64: a required array is NULL");
+    if (!d || (a->neb && !E_data) || (a->nrblk && !R)) return fail(BLS_ERR_ARG, "getdiag!: a required array is NULL");
     const int64_t nb = a->nrb + a->neb;
-    if (nb) k_hdiag<<<grid_for(nb, 256), 256, 0, (hipStream_t)stream>>>(hdot_args(a), E_data, R, d);
+    if (nb) k_hdiag<T><<<grid_for(nb, 256), 256, 0, s>>>(hdot_args(a), E_data, R, d);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
+                               const double *R, double *result, void *stream) {
+    return hessian_dot<double>(a, x, y, E_data, C_data, R, result, (hipStream_t)stream);
+}
+extern "C" int bls_asm_dot_f32(bls_asm *a, const double *x, const double *y, const float *E_data, const float *C_data,
+                               const float *R, double *result, void *stream) {
+    return hessian_dot<float>(a, x, y, E_data, C_data, R, result, (hipStream_t)stream);
+}
+extern "C" int bls_asm_diag_f64(bls_asm *a, const double *E_data, const double *R, double *d, void *stream) {
+    return hessian_diag<double>(a, E_data, R, d, (hipStream_t)stream);
+}
+extern "C" int bls_asm_diag_f32(bls_asm *a, const float *E_data, const float *R, double *d, void *stream) {
+    return hessian_diag<float>(a, E_data, R, d, (hipStream_t)stream);
 }

@@ -85,6 +85,8 @@ SIGNATURES = {
     "bls_asm_hessian_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_asm_dot_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_asm_diag_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_asm_dot_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_asm_diag_f32": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_jtr_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_jtj_diag_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_jtj_quadform_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
@@ -648,14 +650,14 @@ def hessian_dot(x, H_handle, y, E_data, C_data, R, stream=None):
     """dot(x, H, y) (src/linalg.jl:133-260) over the storage of the handle's layout: a Python float."""
     import torch
     out = torch.empty(1, dtype=torch.float64, device=x.device)
-    _check(lib().bls_asm_dot_f64(H_handle.h, _dp(x), _dp(y), _dp(E_data), _dp(C_data), _dp(R), _dp(out),
-                                 _stream(stream)))
+    _check(getattr(lib(), "bls_asm_dot_" + _fsfx(E_data))(H_handle.h, _dp(x), _dp(y), _dp(E_data), _dp(C_data), _dp(R),
+                                                          _dp(out), _stream(stream)))
     return float(out.item())
 
 
 def hessian_diag_(d, H_handle, E_data, R, stream=None):
     """getdiag!(d, ls) of a Schur system (src/schur_linear_systems.jl:239-270): the diagonal of H as filled."""
-    _check(lib().bls_asm_diag_f64(H_handle.h, _dp(E_data), _dp(R), _dp(d), _stream(stream)))
+    _check(getattr(lib(), "bls_asm_diag_" + _fsfx(E_data))(H_handle.h, _dp(E_data), _dp(R), _dp(d), _stream(stream)))
     return d
 
 

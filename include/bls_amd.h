@@ -408,9 +408,15 @@ int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r, double *E_
  * double *result when the stream reaches it. */
 int bls_asm_dot_f64(bls_asm *a, const double *x, const double *y, const double *E_data, const double *C_data,
                     const double *R, double *result, void *stream);
+/* The same over a SchurBlockHessian{Float32}'s storage, each stored value widened and the terms added in Float64
+ * (the promoted type of Float32 blocks and Float64 vectors, src/linalg.jl:12). */
+int bls_asm_dot_f32(bls_asm *a, const double *x, const double *y, const float *E_data, const float *C_data,
+                    const float *R, double *result, void *stream);
 /* getdiag! (src/schur_linear_systems.jl:239-270): d = [diag(R); diag(E_1); ...], the undamped diagonal of H as
  * filled, length nr + sum of e_dims; a retained block whose diagonal block a sparse R does not store reads zeros. */
 int bls_asm_diag_f64(bls_asm *a, const double *E_data, const double *R, double *d, void *stream);
+/* The same from a SchurBlockHessian{Float32}'s storage into the Float64 d (EliminationConditioning.d). */
+int bls_asm_diag_f32(bls_asm *a, const float *E_data, const float *R, double *d, void *stream);
 
 /* ---- The dense reduced solve (src/schur_linear_systems.jl:312-319) ----------------------------------------
  * The :cholesky branch of solve_reduced_dense!: S = U'U for the dense reduced system S (n x n column-major, its

@@ -133,3 +133,32 @@ def test_table_quantities_match_oracle(blsamd, tables):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), name
     assert abs(q - qo) <= 1e-12 * qo
     assert dev.jtj_quadform(h, F, _T(v)) == q          # fixed order: bitwise reproducible
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_hessian_dot_and_diag_float32(blsamd, sparse):
+    """dot(x, H, y) and getdiag! over a SchurBlockHessian{Float32}'s storage: the oracle on the same Float32 values
+    (widened): getdiag! bitwise, dot within 1e-12 of the sum of the terms' magnitudes (both sum in Float64)."""
+    import torch
+    dev = blsamd.device
+    st, rowblocks, rb_rows, J, r, R_layout = assembly_problem(blsamd, 9, nland=50, sparse=sparse)
+    h = dev.AssemblyHandle(st, rowblocks, rb_rows, R_layout)
+    Ho = A.assemble(A.HessianStorage(st, *(R_layout or ())), rowblocks, rb_rows, J, r)
+    E32 = Ho.E.astype(np.float32)
+    C32 = Ho.C.astype(np.float32)
+    R32 = (Ho.R.reshape(-1, order="F") if R_layout is None else Ho.R).astype(np.float32)
+    H = A.HessianStorage(st, *(R_layout or ()))
+    H.E, H.C = E32.astype(np.float64), C32.astype(np.float64)
+    H.R = R32.astype(np.float64).reshape(h.nr, h.nr, order="F") if R_layout is None else R32.astype(np.float64)
+    n = h.nr + int(np.sum(st.e_dims))
+    rng = np.random.default_rng(10)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    Ed, Cd, Rd = _T(E32, np.float32), _T(C32, np.float32), _T(R32, np.float32)
+    ref = A.bilinear(H, x, y)
+    Habs = A.HessianStorage(st, *(R_layout or ()))
+    Habs.E, Habs.C, Habs.R = np.abs(H.E), np.abs(H.C), np.abs(H.R)
+    scale = A.bilinear(Habs, np.abs(x), np.abs(y))          # the sum of the terms' magnitudes
+    assert abs(dev.hessian_dot(_T(x), h, _T(y), Ed, Cd, Rd) - ref) <= 1e-12 * scale
+    d = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+    dev.hessian_diag_(d, h, Ed, Rd)
+    assert np.array_equal(d.cpu().numpy(), A.getdiag(H))
