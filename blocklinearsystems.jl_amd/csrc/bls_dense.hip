@@ -317,9 +317,10 @@ __global__ __launch_bounds__(256) void k_chol_bwd_update(int64_t n, int64_t k0, 
     x[i] -= s;
 }
 
-__global__ void k_chol_copy(int64_t len, const double *__restrict__ src, double *__restrict__ dst) {
+template <class T>
+__global__ void k_chol_copy(int64_t len, const T *__restrict__ src, double *__restrict__ dst) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+        dst[i] = (double)src[i];
 }
 
 __global__ void k_chol_zero(int64_t *p) { p[0] = 0; }
@@ -364,15 +365,26 @@ extern "C" int bls_chol_create(int64_t n, bls_chol **out) {
 
 extern "C" void bls_chol_destroy(bls_chol *h) { delete h; }
 
+namespace {
+template <class T> int chol_factor(bls_chol *h, const T *S, hipStream_t s);
+}
+
 extern "C" int bls_chol_factor_f64_async(bls_chol *h, const double *S, void *stream) {
+    return chol_factor<double>(h, S, (hipStream_t)stream);
+}
+extern "C" int bls_chol_factor_f32_async(bls_chol *h, const float *S, void *stream) {
+    return chol_factor<float>(h, S, (hipStream_t)stream);
+}
+
+namespace {
+template <class T> int chol_factor(bls_chol *h, const T *S, hipStream_t s) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
-    if (h->n && !S) return fail(BLS_ERR_ARG, "bls_chol_factor_f64: S is NULL");
-    hipStream_t s = (hipStream_t)stream;
+    if (h->n && !S) return fail(BLS_ERR_ARG, "bls_chol_factor: S is NULL");
     const int64_t n = h->n;
     double *A = h->U.ptr;
     int64_t *info = h->info.ptr;
     k_chol_zero<<<1, 1, 0, s>>>(info);
-    if (n) k_chol_copy<<<(unsigned)std::min<int64_t>(8192, grid_for(n * n, 256)), 256, 0, s>>>(n * n, S, A);
+    if (n) k_chol_copy<T><<<(unsigned)std::min<int64_t>(8192, grid_for(n * n, 256)), 256, 0, s>>>(n * n, S, A);
     // Panel k on the caller's stream: its row solve, then the first tile row of its trailing update (the rows of
     // panel k + 1), then panel k + 1's diagonal block. The other tile rows of the update go to the second stream once
     // the row solve is done, and run beside panel k + 1's diagonal block and row solve; panel k + 1's own update waits
@@ -402,15 +414,27 @@ extern "C" int bls_chol_factor_f64_async(bls_chol *h, const double *S, void *str
     return BLS_OK;
 }
 
+int chol_wait(bls_chol *h, int64_t *failed_col, void *stream);
+}  // namespace
+
 extern "C" int bls_chol_factor_f64(bls_chol *h, const double *S, int64_t *failed_col, void *stream) {
     int rc = bls_chol_factor_f64_async(h, S, stream);
-    if (rc) return rc;
+    return rc ? rc : chol_wait(h, failed_col, stream);
+}
+extern "C" int bls_chol_factor_f32(bls_chol *h, const float *S, int64_t *failed_col, void *stream) {
+    int rc = bls_chol_factor_f32_async(h, S, stream);
+    return rc ? rc : chol_wait(h, failed_col, stream);
+}
+
+namespace {
+int chol_wait(bls_chol *h, int64_t *failed_col, void *stream) {
     int64_t v = 0;
     BLS_HIP(hipMemcpyAsync(&v, h->info.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
     BLS_HIP(hipStreamSynchronize((hipStream_t)stream));
     if (failed_col) *failed_col = v;
     return v ? BLS_FAILED : BLS_OK;
 }
+}  // namespace
 
 extern "C" int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void *stream) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
@@ -420,7 +444,7 @@ extern "C" int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void 
     const int64_t n = h->n;
     const double *A = h->U.ptr;
     const int64_t *info = h->info.ptr;
-    if (n) k_chol_copy<<<(unsigned)std::min<int64_t>(1024, grid_for(n, 256)), 256, 0, s>>>(n, g, x);
+    if (n) k_chol_copy<double><<<(unsigned)std::min<int64_t>(1024, grid_for(n, 256)), 256, 0, s>>>(n, g, x);
     for (int64_t k0 = 0; k0 < n; k0 += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k0);
         k_chol_fwd_diag<<<1, NB, 0, s>>>(n, k0, kb, A, x, info);
@@ -439,7 +463,7 @@ extern "C" int bls_chol_factor_copy_f64(bls_chol *h, double *U, void *stream) {
     if (!h || !U) return fail(BLS_ERR_ARG, "bls_chol_factor_copy_f64: bad arguments");
     if (!h->factored) return fail(BLS_ERR_STATE, "bls_chol_factor_copy_f64: no factorization");
     if (h->n)
-        k_chol_copy<<<(unsigned)std::min<int64_t>(8192, grid_for(h->n * h->n, 256)), 256, 0, (hipStream_t)stream>>>(
+        k_chol_copy<double><<<(unsigned)std::min<int64_t>(8192, grid_for(h->n * h->n, 256)), 256, 0, (hipStream_t)stream>>>(
             h->n * h->n, h->U.ptr, U);
     BLS_HIP(hipGetLastError());
     return BLS_OK;

@@ -99,6 +99,8 @@ SIGNATURES = {
     "bls_chol_destroy": (None, [VP]),
     "bls_chol_factor_f64": (ct.c_int, [VP, VP, P64, VP]),
     "bls_chol_factor_f64_async": (ct.c_int, [VP, VP, VP]),
+    "bls_chol_factor_f32": (ct.c_int, [VP, VP, P64, VP]),
+    "bls_chol_factor_f32_async": (ct.c_int, [VP, VP, VP]),
     "bls_chol_solve_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_chol_factor_copy_f64": (ct.c_int, [VP, VP, VP]),
     "bls_cond_create": (ct.c_int, [ct.c_int64, ct.POINTER(VP)]),
@@ -727,7 +729,7 @@ class DenseCholesky:
 
     def factor(self, S, stream=None):
         fc = ct.c_int64(0)
-        rc = _check(lib().bls_chol_factor_f64(self.h, _dp(S), ct.byref(fc), _stream(stream)))
+        rc = _check(getattr(lib(), "bls_chol_factor_" + _fsfx(S))(self.h, _dp(S), ct.byref(fc), _stream(stream)))
         return rc == BLS_OK, int(fc.value)
 
     def solve(self, g, x, stream=None):

@@ -431,6 +431,10 @@ int bls_chol_factor_f64(bls_chol *h, const double *S, int64_t *failed_col, void 
 /* The same, queued without waiting: a failure is then seen by bls_chol_solve_f64 (which leaves x as g) and by the
  * next bls_chol_factor_f64. */
 int bls_chol_factor_f64_async(bls_chol *h, const double *S, void *stream);
+/* A Float32 S (a SchurBlockHessian{Float32}'s reduced system) is widened into the handle's Float64 copy and factored
+ * in Float64. */
+int bls_chol_factor_f32(bls_chol *h, const float *S, int64_t *failed_col, void *stream);
+int bls_chol_factor_f32_async(bls_chol *h, const float *S, void *stream);
 int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void *stream);
 /* The factor U (upper triangle of an n x n column-major array; the lower triangle is unspecified), for tests. */
 int bls_chol_factor_copy_f64(bls_chol *h, double *U, void *stream);

@@ -69,3 +69,21 @@ def test_dense_cholesky_failing_column(blsamd):
     assert ch.factor(_dev_S(S)) == (False, 70) and D.cholesky_upper(S)[1] == 70
     S = _spd(rng, n)
     assert ch.factor(_dev_S(S)) == (True, 0)
+
+
+def test_dense_cholesky_float32_input(blsamd):
+    """A Float32 S is widened and factored in Float64: the same factor and solution, bit for bit, as the Float64 path
+    on the widened values."""
+    import torch
+    dev = blsamd.device
+    rng = np.random.default_rng(7)
+    n = 150
+    S32 = _spd(rng, n).astype(np.float32)
+    g = torch.as_tensor(rng.standard_normal(n)).cuda()
+    a, b = dev.DenseCholesky(n), dev.DenseCholesky(n)
+    Sd = _dev_S(S32.astype(np.float64))
+    assert a.factor(Sd.to(torch.float32)) == (True, 0) and b.factor(Sd) == (True, 0)
+    xa, xb = torch.empty(n, dtype=torch.float64, device="cuda"), torch.empty(n, dtype=torch.float64, device="cuda")
+    a.solve(g, xa)
+    b.solve(g, xb)
+    assert torch.equal(xa, xb)
