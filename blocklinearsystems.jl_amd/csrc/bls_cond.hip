@@ -70,6 +70,30 @@ __global__ __launch_bounds__(COND_WG) void k_cond_any(int nwg, int32_t *__restri
 }
 
 __global__ void k_cond_zero(int64_t *p) { p[0] = 0; }
+__global__ void k_cond_flag_clear(int32_t *p) { p[0] = 0; }
+
+// unscale! (src/conditioning.jl:90-95): x_j *= s_j; flags[g] = 1 if workgroup g's entries of the result hold a value
+// that is not finite (all(isfinite, x), src/schur_linear_systems.jl:333-334)
+__global__ __launch_bounds__(COND_WG) void k_unscale(int64_t n, double *__restrict__ x, const double *__restrict__ s,
+                                                     int32_t *__restrict__ flags) {
+    int bad = 0;
+    for (int64_t j = (int64_t)blockIdx.x * COND_WG + threadIdx.x; j < n; j += (int64_t)gridDim.x * COND_WG) {
+        const double v = s ? x[j] * s[j] : x[j];
+        if (s) x[j] = v;
+        bad |= !isfinite(v);
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) flags[blockIdx.x] = bad;
+}
+
+// out[0] = any of flags[0 .. nwg)
+__global__ __launch_bounds__(COND_WG) void k_flags_any(int nwg, const int32_t *__restrict__ flags,
+                                                       int32_t *__restrict__ out) {
+    int any = 0;
+    for (int g = threadIdx.x; g < nwg; g += COND_WG) any |= flags[g];
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) out[0] = any;
+}
 }  // namespace
 
 struct bls_cond {
@@ -140,5 +164,15 @@ extern "C" int bls_cond_nrescale(bls_cond *c, int64_t *out, void *stream) {
     if (!c || !out) return fail(BLS_ERR_ARG, "bls_cond_nrescale: bad arguments");
     BLS_HIP(hipStreamSynchronize((hipStream_t)stream));
     BLS_HIP(hipMemcpy(out, c->nrescale.ptr, sizeof(int64_t), hipMemcpyDeviceToHost));
+    return BLS_OK;
+}
+
+extern "C" int bls_unscale_f64(bls_cond *c, double *x, const double *s, int32_t *nonfinite, void *stream) {
+    if (!c || !nonfinite || (c->n && !x)) return fail(BLS_ERR_ARG, "bls_unscale_f64: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    if (c->n) k_unscale<<<(unsigned)c->nwg, COND_WG, 0, st>>>(c->n, x, s, c->flags.ptr);
+    else k_cond_flag_clear<<<1, 1, 0, st>>>(c->flags.ptr);
+    k_flags_any<<<1, COND_WG, 0, st>>>(c->n ? c->nwg : 1, c->flags.ptr, nonfinite);
+    BLS_HIP(hipGetLastError());
     return BLS_OK;
 }

@@ -109,6 +109,7 @@ SIGNATURES = {
     "bls_cond_markstale": (ct.c_int, [VP]),
     "bls_prepare_scale_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, ct.c_int, VP]),
     "bls_cond_nrescale": (ct.c_int, [VP, P64, VP]),
+    "bls_unscale_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
     "bls_diag_destroy": (None, [VP]),
     "bls_factor_diagonal_blocks_f64": (ct.c_int, [VP, VP, P64, VP]),
     "bls_factor_diagonal_blocks_f32": (ct.c_int, [VP, VP, P64, VP]),
@@ -739,6 +740,13 @@ class DenseCholesky:
     def factor_copy(self, U, stream=None):
         _check(lib().bls_chol_factor_copy_f64(self.h, _dp(U), _stream(stream)))
         return U
+
+
+def unscale_(c, x, s, nonfinite, stream=None):
+    """unscale!(x, s) (src/conditioning.jl:90-95) over c's n variables (s None: UnitScale), and the device int32
+    nonfinite[0] = 1 if x then holds a value that is not finite."""
+    _check(lib().bls_unscale_f64(c.h, _dp(x), _dp(s), _dp(nonfinite), _stream(stream)))
+    return x
 
 
 def factor_diagonal_blocks_(D, blocks, stream=None):

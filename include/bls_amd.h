@@ -459,6 +459,10 @@ int bls_prepare_scale_f64(bls_cond *c, double *s, const double *d, const double 
                           void *stream);
 /* nrescale, after synchronizing the stream. */
 int bls_cond_nrescale(bls_cond *c, int64_t *out, void *stream);
+/* unscale!(x, s) (:90-95) for the n variables of the handle: x = S y in place (s NULL = UnitScale: x untouched), and
+ * the device int *nonfinite = 1 if x then holds a value that is not finite (the reference returns
+ * NotPositiveDefinite then, src/schur_linear_systems.jl:333-334), else 0. */
+int bls_unscale_f64(bls_cond *c, double *x, const double *s, int32_t *nonfinite, void *stream);
 
 #ifdef __cplusplus
 }
