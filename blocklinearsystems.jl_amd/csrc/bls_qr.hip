@@ -371,8 +371,9 @@ __global__ __launch_bounds__(256) void k_red_gsum(RedArgs a, const int64_t *__re
 
 // diagonal blocks of J~'J~ (reduced_diagonal_blocks!, src/qr_reduced.jl:484-512) in two passes. k_red_diag_part: one
 // wavefront per table stages each entry's d columns of the reduced rows in LDS, 64 rows at a time (row stride 65, so
-// the lanes of different columns hit different banks), and lane p forms upper pair p = b (b + 1) / 2 + a (a <= b)
-// of X_u' X_u into dp[dpo[u] + p] (dpo: prefix sums of d (d + 1) / 2 in entry order). k_red_dsum: one workgroup per
+// the lanes of different columns hit different banks), and forms the upper pairs p = b (b + 1) / 2 + a (a <= b) of
+// X_u' X_u into dp[dpo[u] + p] (dpo: prefix sums of d (d + 1) / 2 in entry order): for a table of at most 64 reduced
+// rows on the FP64 matrix instruction (one 16 x 16 tile per entry), else lane p forms pair p by a dot product. k_red_dsum: one workgroup per
 // retained block r adds the partials of the entries coupling r in retained_entry order -- thread (p, run) takes the
 // run-th of 256 / npair contiguous runs of that list, and the runs are added in order -- and writes the full block,
 // both triangles, column-major at doff[r].
@@ -381,6 +382,8 @@ __device__ inline void wave_sync() {   // LDS written by one lane of the wavefro
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+typedef double qd4 __attribute__((ext_vector_type(4)));
 
 template <class T>
 __global__ __launch_bounds__(256) void k_red_diag_part(RedArgs a, const int64_t *__restrict__ dpo,
@@ -425,18 +428,20 @@ __global__ __launch_bounds__(256) void k_red_diag_part(RedArgs a, const int64_t 
 #pragma unroll
                 for (int j = 0; j < 16; j++) nx[j] = (j < d && lane < n) ? (double)Xt[(int64_t)(c0 + j) * M + lane] : 0.0;
             }
-            const int np = dc * (dc + 1) / 2;   // <= 136: three pairs per lane at most
-            int pa[3], pb[3];
-            pairs(dc, pa, pb);
+            // X_u'X_u on the FP64 matrix instruction, one 16 x 16 tile, k over the rows in steps of 4: lane
+            // (li, lk) = (lane % 16, lane / 16) supplies A(li, k) = B(k, li) = X(k + lk, li) (columns past d and rows
+            // past n read as zero), and accumulator v holds D(lk + 4 v, li)
+            const int li = lane & 15, lk = lane >> 4;
+            qd4 acc = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < n; k += 4) {
+                const double xv = li < dc ? xw[li * 65 + k + lk] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, xv, acc, 0, 0, 0);
+            }
 #pragma unroll
-            for (int k = 0; k < 3; k++)
-                if (lane + 64 * k < np) {
-                    const double *x = xw + pa[k] * 65, *y = xw + pb[k] * 65;
-                    double s = 0.0;
-#pragma unroll 8
-                    for (int q = 0; q < n; q++) s += x[q] * y[q];
-                    dp[dpo[u] + lane + 64 * k] = s;
-                }
+            for (int v = 0; v < 4; v++) {
+                const int ra = lk + 4 * v;
+                if (ra <= li && li < dc) dp[dpo[u] + li * (li + 1) / 2 + ra] = acc[v];
+            }
             wave_sync();
         }
         return;
