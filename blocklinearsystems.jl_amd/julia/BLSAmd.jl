@@ -5,6 +5,8 @@
 # signatures (src/schur_reduction.jl:389-507, src/qr_reduction.jl:162-187, src/SparseBlockLinearSystem.jl:356-362);
 # the device is selected by the workspace argument, a DeviceSchurWorkspace / DeviceQRWorkspace wrapping the
 # handle of the C ABI (include/bls_amd.h). Host arrays are staged through device buffers owned by the workspace.
+# Further down: dot(x, H, y), getdiag! and prepare_scale! over a device copy of a filled Hessian, and the Cholesky
+# branch of the dense reduced solve.
 # Written against the C ABI; not executed in this repository (no Julia toolchain where it was built).
 module BLSAmd
 
@@ -135,6 +137,128 @@ function device_copyto!(dest::Vector{Float64}, src::Vector{Float64}, scatter_map
     download!(dest, d)
     ccall((:bls_free, libblsamd), Cint, (Ptr{Cvoid},), dm[])
     return dest
+end
+
+# ---- The storage of a filled Hessian: dot(x, H, y), getdiag! and the conditioning scale -----------------------
+
+# The stored lower blocks of a SparseBlockLinearSystem R as the block CSC the C ABI takes: for block column q, the
+# rows p >= q with a stored block (p, q), ascending, and the blocks' 1-based data offsets. R.indicestransposed lists,
+# in its column i (a block row), the stored blocks (i, j) and their offsets (src/SparseBlockLinearSystem.jl:52-61).
+function lower_block_csc(R::BLS.SparseBlockLinearSystem)
+    It = R.indicestransposed
+    nb = size(It, 2)
+    cols = [Tuple{Int64, Int64}[] for _ in 1:nb]
+    for i in 1:nb, k in nzrange(It, i)
+        push!(cols[rowvals(It)[k]], (Int64(i), Int64(nonzeros(It)[k])))
+    end
+    colptr, rowval, offs = Int64[1], Int64[], Int64[]
+    for q in 1:nb
+        for (p, o) in sort(cols[q])
+            push!(rowval, p); push!(offs, o)
+        end
+        push!(colptr, length(rowval) + 1)
+    end
+    return colptr, rowval, offs
+end
+
+"""
+    DeviceHessianLayout(H::SchurBlockHessian{Float64})
+
+The storage layout of `H` on the device (a bls_asm handle registered with no row blocks) and device copies of its
+E_data, C_data and R, for the queries below. A sparse R passes its stored lower blocks as a block CSC.
+"""
+mutable struct DeviceHessianLayout
+    handle::Ptr{Cvoid}
+    E::DevBuf{Float64}
+    C::DevBuf{Float64}
+    R::DevBuf{Float64}
+end
+function DeviceHessianLayout(H::BLS.SchurBlockHessian{Float64})
+    st = H.structure
+    dense = H.R isa Matrix
+    rc_, rr_, ro_ = dense ? (C_NULL, C_NULL, C_NULL) : lower_block_csc(H.R)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    none = Int64[1]
+    check(ccall((:bls_asm_create, libblsamd), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Ptr{Cvoid}}),
+                st.nrblocks, st.neblocks, collect(Int64, st.r_dims), collect(Int64, st.e_dims), st.boffsets_r,
+                st.boffsets_e, st.eoffsets, st.eliminated_ptr, st.entry_retained, st.coffsets, rc_, rr_, ro_,
+                0, none, C_NULL, C_NULL, h))
+    Rv = dense ? vec(H.R) : H.R.data
+    l = DeviceHessianLayout(h[], DevBuf{Float64}(length(H.E_data)), DevBuf{Float64}(length(H.C_data)),
+                            DevBuf{Float64}(length(Rv)))
+    upload!(l.E, H.E_data); upload!(l.C, H.C_data); upload!(l.R, Rv)
+    finalizer(x -> ccall((:bls_asm_destroy, libblsamd), Cvoid, (Ptr{Cvoid},), x.handle), l)
+    return l
+end
+
+# dot(x, H, y) (src/linalg.jl:133-260) on the device copy of H
+function device_dot(x::Vector{Float64}, l::DeviceHessianLayout, y::Vector{Float64})
+    dx = DevBuf{Float64}(length(x)); upload!(dx, x)
+    dy = DevBuf{Float64}(length(y)); upload!(dy, y)
+    out = DevBuf{Float64}(1)
+    check(ccall((:bls_asm_dot_f64, libblsamd), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                l.handle, dx.ptr, dy.ptr, l.E.ptr, l.C.ptr, l.R.ptr, out.ptr, C_NULL))
+    r = zeros(1); download!(r, out)
+    return r[1]
+end
+
+"""
+    DeviceConditioning(n)
+
+EliminationConditioning (src/conditioning.jl:47-55) on the device: the stale flag and nrescale in a bls_cond handle,
+s and d in device buffers. `device_prepare_scale!` is prepare_scale! (:60-87) with getdiag! of the layout as diag!.
+"""
+mutable struct DeviceConditioning
+    handle::Ptr{Cvoid}
+    s::DevBuf{Float64}
+    d::DevBuf{Float64}
+end
+function DeviceConditioning(n::Integer)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:bls_cond_create, libblsamd), Cint, (Int64, Ptr{Ptr{Cvoid}}), n, h))
+    c = DeviceConditioning(h[], DevBuf{Float64}(n), DevBuf{Float64}(n))
+    finalizer(x -> ccall((:bls_cond_destroy, libblsamd), Cvoid, (Ptr{Cvoid},), x.handle), c)
+    return c
+end
+function device_prepare_scale!(c::DeviceConditioning, l::DeviceHessianLayout, lambda::Float64, sc::DevBuf{Float64},
+                               ::Type{T}) where T
+    if check(ccall((:bls_cond_is_stale, libblsamd), Cint, (Ptr{Cvoid},), c.handle)) == 1
+        check(ccall((:bls_asm_diag_f64, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                    l.handle, l.E.ptr, l.R.ptr, c.d.ptr, C_NULL))
+    end
+    check(ccall((:bls_prepare_scale_f64, libblsamd), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Cint, Ptr{Cvoid}),
+                c.handle, c.s.ptr, c.d.ptr, sc.ptr, lambda, T === Float32 ? Cint(1) : Cint(0), C_NULL))
+    return c.s                       # device pointer: pass as s_r / s_e (offset by nr) to the reduction
+end
+
+# ---- The dense reduced solve: the :cholesky branch of solve_reduced_dense! (src/schur_linear_systems.jl:312-319) --
+
+# x_r = S \ g_S for the dense reduced S (upper triangle, as reduce_schur_dense! writes it): Success, or
+# NotPositiveDefinite with S left as built (the reference then tries :ldl, on the host)
+function device_cholesky_solve!(x_r::Vector{Float64}, S::Matrix{Float64}, g_S::Vector{Float64})
+    n = size(S, 1)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:bls_chol_create, libblsamd), Cint, (Int64, Ptr{Ptr{Cvoid}}), n, h))
+    try
+        dS = DevBuf{Float64}(n * n); upload!(dS, vec(S))
+        fc = Ref{Int64}(0)
+        rc = check(ccall((:bls_chol_factor_f64, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Ptr{Cvoid}),
+                         h[], dS.ptr, fc, C_NULL))
+        rc == BLS_FAILED && return :NotPositiveDefinite
+        dg = DevBuf{Float64}(n); upload!(dg, g_S)
+        dx = DevBuf{Float64}(n)
+        check(ccall((:bls_chol_solve_f64, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                    h[], dg.ptr, dx.ptr, C_NULL))
+        download!(x_r, dx)
+        return :Success
+    finally
+        ccall((:bls_chol_destroy, libblsamd), Cvoid, (Ptr{Cvoid},), h[])
+    end
 end
 
 end # module BLSAmd
