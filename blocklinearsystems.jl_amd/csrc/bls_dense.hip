@@ -6,7 +6,7 @@
 // positive (NotPositiveDefinite), the caller's cue to fall back to an indefinite factorization.
 //
 // Blocked right-looking factorization with a look-ahead of one panel, panels of NB = 64 rows of U:
-//   k_chol_diag   one workgroup factors the diagonal block in LDS (unblocked, row by row of U);
+//   k_chol_diag   one wavefront factors the diagonal block, a column per lane in registers (unblocked, row by row of U);
 //   k_chol_trsm   the row panel U(k, j) = U_kk^-T S(k, j), one wavefront per column (a lane per row), U_kk in LDS;
 //   k_chol_syrk   the trailing upper triangle S(i, j) -= U(k, i)' U(k, j), one workgroup per 64 x 64 tile (tiles
 //                 below the diagonal skipped), the two row-panel slices staged in LDS and multiplied on the FP64
@@ -51,69 +51,51 @@ __device__ inline void stage_diag_block(int64_t n, int64_t k0, int kb, const dou
     }
 }
 
-// U_kk from the block's rows k0 .. k0 + kb of A (upper triangle, column-major, leading dimension n), row by row of U
-// with one barrier per row. Thread (ty, tx) of a 16 x 16 grid keeps the 4 x 4 entries (rows 4 ty + a, columns
-// 4 tx + b) in registers, so wavefront w holds rows 16 w .. 16 w + 15 and drops out once the pivot row has passed
-// them. Only the pivot row is shared: at row j every thread reads row j of T (published by its owners before the
-// barrier) and forms u_jj = sqrt(T[j][j]) and its reciprocal itself; the owners of row j write U's row j (scaled by
-// the reciprocal, as LAPACK's dpotf2 scales it) to Uo, every entry of the trailing triangle takes
-// t -= (T[j][r] / u_jj) (T[j][c] / u_jj), and the owners of row j + 1 publish it. A pivot that is not positive (or
-// NaN) records column k0 + j + 1 in *info and stops.
-__global__ __launch_bounds__(256) void k_chol_diag(int64_t n, int64_t k0, int kb, double *__restrict__ A,
-                                                   int64_t *__restrict__ info) {
-    __shared__ double T[NB][NB + 1], Uo[NB][NB + 1];
+// U_kk from the block's rows k0 .. k0 + kb of A (upper triangle, column-major, leading dimension n) on one wavefront:
+// lane c keeps column c of the block in registers (staged through LDS with coalesced loads). At row j every lane reads
+// the pivot u = T(j, j) from lane j, forms sqrt and its reciprocal, scales its entry of row j (as LAPACK's dpotf2 scales
+// the row), and updates T(r, c) -= u_jr u_jc for every r > j with u_jr read from lane r; entries below the diagonal
+// are updated too but never read or stored. A pivot that is not positive (or NaN) records column k0 + j + 1 in *info
+// and stops.
+__global__ __launch_bounds__(64) void k_chol_diag(int64_t n, int64_t k0, int kb, double *__restrict__ A,
+                                                  int64_t *__restrict__ info) {
+    __shared__ double T[NB][NB + 1];
     if (*info) return;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, wave = threadIdx.x >> 6;
-    const int r0 = 4 * ty, c0 = 4 * tx;
-    stage_diag_block(n, k0, kb, A, T, 0.0);
+    const int lane = threadIdx.x;
+    for (int c = 0; c < NB; c += 16) {   // lane r reads row r of 16 columns (coalesced), loads before stores
+        double v[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            v[q] = (lane < kb && c + q < kb && lane <= c + q) ? A[(k0 + lane) + (k0 + c + q) * n] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) T[lane][c + q] = v[q];
+    }
     __syncthreads();
-    double t[4][4];
+    double col[NB];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int r = 0; r < NB; r++) col[r] = T[r][lane];
+    bool ok = true;
 #pragma unroll
-        for (int b = 0; b < 4; b++) t[a][b] = T[r0 + a][c0 + b];
-    for (int j = 0; j < kb; j++) {
-        const double piv = T[j][j];
-        if (!(piv > 0.0)) {   // every thread sees the same pivot
-            if (threadIdx.x == 0) *info = k0 + j + 1;
-            return;
-        }
-        const double ujj = sqrt(piv), rinv = 1.0 / ujj;
-        if (ty == (j >> 2)) {   // U's row j
+    for (int j = 0; j < NB; j++) {
+        if (j < kb && ok) {   // uniform: kb and the pivots are the same on every lane
+            const double piv = bcast(col[j], j);
+            if (!(piv > 0.0)) {
+                if (lane == 0) *info = k0 + j + 1;
+                ok = false;
+            } else {
+                const double ujj = sqrt(piv), rinv = 1.0 / ujj;
+                col[j] = lane == j ? ujj : col[j] * rinv;   // row j of U (lanes c > j); lanes c < j hold unused values
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int c = c0 + b;
-                if (c >= j && c < kb) Uo[j][c] = c == j ? ujj : T[j][c] * rinv;
+                for (int r = j + 1; r < NB; r++) col[r] -= bcast(col[j], r) * col[j];
             }
         }
-        if (16 * wave + 15 > j) {   // this wavefront still holds rows below the pivot
-            double ur[4], uc[4];
-#pragma unroll
-            for (int a = 0; a < 4; a++) ur[a] = T[j][r0 + a] * rinv;
-#pragma unroll
-            for (int b = 0; b < 4; b++) uc[b] = T[j][c0 + b] * rinv;
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-                    t[a][b] = (r0 + a > j && c0 + b >= r0 + a) ? t[a][b] - ur[a] * uc[b] : t[a][b];
-            const int nx = j + 1;   // publish row j + 1 (final now) for the next step
-            if (nx < kb && ty == (nx >> 2)) {
-#pragma unroll
-                for (int a = 0; a < 4; a++)
-                    if (a == (nx & 3))
-#pragma unroll
-                        for (int b = 0; b < 4; b++)
-                            if (c0 + b >= nx && c0 + b < kb) T[nx][c0 + b] = t[a][b];
-            }
-        }
-        __syncthreads();
     }
+    if (!ok) return;
 #pragma unroll
-    for (int q = 0; q < NB * NB / 256; q++) {
-        const int w = threadIdx.x + 256 * q, c = w >> 6, r = w & 63;
-        if (r < kb && c < kb && r <= c) A[(k0 + r) + (k0 + c) * n] = Uo[r][c];
-    }
+    for (int r = 0; r < NB; r++) T[r][lane] = col[r];
+    __syncthreads();
+    for (int c = 0; c < kb; c++)
+        if (lane <= c && lane < kb) A[(k0 + lane) + (k0 + c) * n] = T[lane][c];
 }
 
 // U(k, j) = U_kk^-T A(k, j) for the columns j >= k0 + kb, TRSM_COLS columns per workgroup, four per wavefront
@@ -390,7 +372,7 @@ template <class T> int chol_factor(bls_chol *h, const T *S, hipStream_t s) {
     // the row solve is done, and run beside panel k + 1's diagonal block and row solve; panel k + 1's own update waits
     // for them (they write the rows of panel k + 2 it reads and updates).
     bool bulk_pending = false;
-    if (n) k_chol_diag<<<1, 256, 0, s>>>(n, 0, (int)std::min<int64_t>(NB, n), A, info);
+    if (n) k_chol_diag<<<1, 64, 0, s>>>(n, 0, (int)std::min<int64_t>(NB, n), A, info);
     for (int64_t k0 = 0; k0 < n; k0 += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k0);
         const int64_t rest = n - k0 - kb;
@@ -406,7 +388,7 @@ template <class T> int chol_factor(bls_chol *h, const T *S, hipStream_t s) {
             BLS_HIP(hipEventRecord(h->ev_bulk, h->s2));
         }
         k_chol_syrk<<<(unsigned)nt, 256, 0, s>>>(n, k0, kb, A, info, SYRK_FIRST);
-        k_chol_diag<<<1, 256, 0, s>>>(n, k0 + kb, (int)std::min<int64_t>(NB, rest), A, info);
+        k_chol_diag<<<1, 64, 0, s>>>(n, k0 + kb, (int)std::min<int64_t>(NB, rest), A, info);
     }
     if (bulk_pending) BLS_HIP(hipStreamWaitEvent(s, h->ev_bulk, 0));
     BLS_HIP(hipGetLastError());
