@@ -126,6 +126,8 @@ struct HDotArgs {
     const int32_t *rp, *rq;                                 // their block row and column
     int dense;
     const int64_t *rdg;                                     // per retained block: its diagonal block in rblk, or -1
+    const int64_t *cro;                                     // per coupling: its retained block's offset in x_r
+    const int32_t *clen;                                    // per coupling: d_e d_r, its block's length in C_data
 };
 
 // one wavefront per eliminated block e, four per workgroup: x_e' E y_e + sum over its couplings c and rows ir of
@@ -152,15 +154,15 @@ __global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__rest
     if (c1 > c0) {
         const T *span = C + a.coff[c0];
         int64_t c = c0, start = 0;   // the coupling holding element q, and where its block starts in the span
-        int len = d * a.rd[a.c_r[c0]];
+        int len = a.clen[c0];
         for (int64_t q = lane;; q += 64) {
             while (c < c1 && q >= start + len) {
                 start += len;
-                if (++c < c1) len = d * a.rd[a.c_r[c]];
+                if (++c < c1) len = a.clen[c];
             }
             if (c >= c1) break;
             const int w = (int)(q - start), ir = w / d, i = w - ir * d;
-            const int64_t ro = a.boff_r[a.c_r[c]];
+            const int64_t ro = a.cro[c];
             acc += (double)span[q] * (x[ro + ir] * ye[i] + y[ro + ir] * xe[i]);
         }
     }
@@ -244,6 +246,8 @@ struct bls_asm {
     DevArray<int32_t> d_de, d_c_r, d_rd, d_rp, d_rq;
     DevArray<RGBlock> d_rblk;
     DevArray<int64_t> d_rdg;   // per retained block: its diagonal block in d_rblk, or -1 (getdiag!)
+    DevArray<int64_t> d_cro;   // per coupling: its retained block's offset in x_r (dot)
+    DevArray<int32_t> d_clen;  // per coupling: d_e d_r (dot)
     DevArray<double> d_part;
     int64_t nrblk = 0;
 };
@@ -464,10 +468,15 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
             dd[e] = (int32_t)dim[nrb + e];
             cp[e + 1] = eliminated_ptr[e + 1] - 1;
         }
-        for (int64_t c = 0; c < ncoup; c++) {
-            co[c] = coffsets[c] - 1;
-            cr[c] = (int32_t)(entry_retained[c] - 1);
-        }
+        std::vector<int64_t> cro(ncoup);
+        std::vector<int32_t> clen(ncoup);
+        for (int64_t e = 0; e < neb; e++)
+            for (int64_t c = eliminated_ptr[e] - 1; c < eliminated_ptr[e + 1] - 1; c++) {
+                co[c] = coffsets[c] - 1;
+                cr[c] = (int32_t)(entry_retained[c] - 1);
+                cro[c] = boff[cr[c]];
+                clen[c] = (int32_t)(dim[nrb + e] * dim[cr[c]]);
+            }
         for (int64_t r = 0; r < nrb; r++) {
             br[r] = boff[r];
             rdv[r] = (int32_t)dim[r];
@@ -494,6 +503,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
             (rc1 = h->d_coff.upload(co)) || (rc1 = h->d_boff_r.upload(br)) || (rc1 = h->d_de.upload(dd)) ||
             (rc1 = h->d_c_r.upload(cr)) || (rc1 = h->d_rd.upload(rdv)) || (rc1 = h->d_rp.upload(rp)) ||
             (rc1 = h->d_rq.upload(rq)) || (rc1 = h->d_rblk.upload(rb)) || (rc1 = h->d_rdg.upload(rdg)) ||
+            (rc1 = h->d_cro.upload(cro)) || (rc1 = h->d_clen.upload(clen)) ||
             (rc1 = h->d_part.alloc((size_t)(neb + (int64_t)rb.size() + FSUM_CHUNKS))))
             return rc1;
         h->nrblk = (int64_t)rb.size();
@@ -588,6 +598,8 @@ HDotArgs hdot_args(const bls_asm *a) {
     d.rq = a->d_rq.ptr;
     d.dense = a->dense;
     d.rdg = a->d_rdg.ptr;
+    d.cro = a->d_cro.ptr;
+    d.clen = a->d_clen.ptr;
     return d;
 }
 }  // namespace
