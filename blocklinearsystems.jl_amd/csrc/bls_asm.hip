@@ -130,45 +130,47 @@ struct HDotArgs {
     const int32_t *clen;                                    // per coupling: d_e d_r, its block's length in C_data
 };
 
-// one wavefront per eliminated block e, four per workgroup: x_e' E y_e + sum over its couplings c and rows ir of
-// x_r[ir] (c_ir . y_e) + y_r[ir] (c_ir . x_e), c_ir column ir of C_er (schur_bilinear_eliminated,
-// src/linalg.jl:208-260), as one sum of per-element terms: lane q takes element q (and q + 64, ...) of E_e and of
-// the couplings' blocks, which lie one after another in C_data, so the wavefront reads them as one contiguous run;
-// the lanes' sums are then added in a fixed tree
+// HD_LANES lanes per eliminated block (several per wavefront, 256 / HD_LANES per workgroup): x_e' E y_e + sum over its couplings c and
+// rows ir of x_r[ir] (c_ir . y_e) + y_r[ir] (c_ir . x_e), c_ir column ir of C_er (schur_bilinear_eliminated,
+// src/linalg.jl:208-260), as one sum of per-element terms: lane q takes element q (and q + HD_LANES, ...) of E_e and of the
+// couplings' blocks, which lie one after another in C_data, so the lanes read them as one contiguous run; the lanes'
+// sums are then added in a fixed tree within the group
+constexpr int HD_LANES = 16;
 template <class T>
 __global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__restrict__ x, const double *__restrict__ y,
                                                 const T *__restrict__ E, const T *__restrict__ C,
                                                 double *__restrict__ part) {
-    const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (e >= a.neb) return;   // the whole wavefront: no block-level synchronisation below
-    const int d = a.de[e];
-    const double *xe = x + a.nr + a.boff_e[e], *ye = y + a.nr + a.boff_e[e];
-    const T *Ee = E + a.eoff[e];
+    const int64_t e = (int64_t)blockIdx.x * (256 / HD_LANES) + threadIdx.x / HD_LANES;
+    const int lane = threadIdx.x % HD_LANES;
     double acc = 0.0;
-    for (int w = lane; w < d * d; w += 64) {
-        const int col = w / d, row = w - col * d;
-        acc += (double)Ee[w] * xe[row] * ye[col];
-    }
-    const int64_t c0 = a.cptr[e], c1 = a.cptr[e + 1];
-    if (c1 > c0) {
-        const T *span = C + a.coff[c0];
-        int64_t c = c0, start = 0;   // the coupling holding element q, and where its block starts in the span
-        int len = a.clen[c0];
-        for (int64_t q = lane;; q += 64) {
-            while (c < c1 && q >= start + len) {
-                start += len;
-                if (++c < c1) len = a.clen[c];
+    if (e < a.neb) {
+        const int d = a.de[e];
+        const double *xe = x + a.nr + a.boff_e[e], *ye = y + a.nr + a.boff_e[e];
+        const T *Ee = E + a.eoff[e];
+        for (int w = lane; w < d * d; w += HD_LANES) {
+            const int col = w / d, row = w - col * d;
+            acc += (double)Ee[w] * xe[row] * ye[col];
+        }
+        const int64_t c0 = a.cptr[e], c1 = a.cptr[e + 1];
+        if (c1 > c0) {
+            const T *span = C + a.coff[c0];
+            int64_t c = c0, start = 0;   // the coupling holding element q, and where its block starts in the span
+            int len = a.clen[c0];
+            for (int64_t q = lane;; q += HD_LANES) {
+                while (c < c1 && q >= start + len) {
+                    start += len;
+                    if (++c < c1) len = a.clen[c];
+                }
+                if (c >= c1) break;
+                const int w = (int)(q - start), ir = w / d, i = w - ir * d;
+                const int64_t ro = a.cro[c];
+                acc += (double)span[q] * (x[ro + ir] * ye[i] + y[ro + ir] * xe[i]);
             }
-            if (c >= c1) break;
-            const int w = (int)(q - start), ir = w / d, i = w - ir * d;
-            const int64_t ro = a.cro[c];
-            acc += (double)span[q] * (x[ro + ir] * ye[i] + y[ro + ir] * xe[i]);
         }
     }
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) part[e] = acc;
+    for (int o = HD_LANES / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);   // within the group
+    if (lane == 0 && e < a.neb) part[e] = acc;
 }
 
 // one lane per stored block of R (src/linalg.jl:153-190): a dense R is read in its upper triangle (each
@@ -612,7 +614,8 @@ int hessian_dot(bls_asm *a, const double *x, const double *y, const T *E_data, c
     if (!x || !y || !result || (a->neb && !E_data) || (a->ncoup && !C_data) || (a->nrblk && !R))
         return fail(BLS_ERR_ARG, "dot(x, H, y): a required array is NULL");
     const HDotArgs d = hdot_args(a);
-    if (a->neb) k_hdot_e<T><<<(unsigned)((a->neb + 3) / 4), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
+    constexpr int per_wg = 256 / HD_LANES;
+    if (a->neb) k_hdot_e<T><<<(unsigned)((a->neb + per_wg - 1) / per_wg), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
     if (a->nrblk) k_hdot_r<T><<<(unsigned)((a->nrblk + 255) / 256), 256, 0, s>>>(d, x, y, R, a->d_part.ptr);
     fsum_launch(a->neb + a->nrblk, a->d_part.ptr, a->d_part.ptr + a->neb + a->nrblk, result, s);   // FSUM_CHUNKS slots after the partials
     BLS_HIP(hipGetLastError());
