@@ -84,8 +84,8 @@ __global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs
     }
 }
 
-// Destination blocks with few contributions (at most LIGHT_MAX): `slot` lanes per block (a power of two >= the
-// largest block of the launch, at most 64), 64 / slot blocks per wavefront; lane w of a block sums entry w (and
+// Destination blocks with few contributions (at most LIGHT_MAX): `slot` lanes per block (the largest block of the
+// launch, at most 64), 64 / slot whole blocks per wavefront (the lanes left over stay idle); lane w of a block sums entry w (and
 // w + slot, ...) over the block's contributions in order (as k_red_gram_sum, without runs).
 constexpr int LIGHT_MAX = 32;
 template <class T>
@@ -93,9 +93,10 @@ __global__ __launch_bounds__(256) void k_red_gram_sum_light(int64_t nblk, int sl
                                                             const int64_t *__restrict__ ptr,
                                                             const int64_t *__restrict__ contrib,
                                                             const double *__restrict__ G, T *__restrict__ S) {
-    const int lane = threadIdx.x & 63;
-    const int64_t b = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / slot) + lane / slot;
-    if (b >= nblk) return;   // no synchronisation below
+    const int lane = threadIdx.x & 63, per = 64 / slot;
+    if (lane >= per * slot) return;   // lanes past the wavefront's last whole slot; no synchronisation below
+    const int64_t b = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * per + lane / slot;
+    if (b >= nblk) return;
     const RGBlock bk = blocks[b];
     const int sz = bk.d1 * bk.d2;
     const int64_t cb = ptr[b], ce = ptr[b + 1];
@@ -527,9 +528,7 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         int64_t nl = 0;
         for (const Dest &d : *lists[l]) nl += d.contrib.size() <= (size_t)LIGHT_MAX;
         h->nlight[l] = nl;
-        int slot = 1;
-        while (slot < maxsz && slot < 64) slot <<= 1;
-        h->slot[l] = slot;
+        h->slot[l] = std::min(maxsz, 64);
         if ((rc = h->d_blocks[l].upload(blocks)) || (rc = h->d_ptr[l].upload(ptr)) || (rc = h->d_contrib[l].upload(contrib)))
             return rc;
         h->nblk[l] = (int64_t)blocks.size();
