@@ -27,19 +27,20 @@ namespace {
 
 constexpr int TINY_W = 16, TINY_M = 8;   // row blocks of at most 16 columns and 8 rows take the tiny path
 
-// Tiny row blocks (observations: 2 rows over a 3- and a 6-column block): 16 lanes per row block, lane c holding
-// column c of J_k. Column a is passed round by shuffle and lane b (>= a) forms element (a, b) of J_k'J_k, written
+// Tiny row blocks (observations: 2 rows over a 3- and a 6-column block): `slot` lanes per row block (the widest tiny
+// row block, at most 16; 64 / slot row blocks per wavefront, the lanes left over idle), lane c holding column c of J_k. Column a is passed round by shuffle and lane b (>= a) forms element (a, b) of J_k'J_k, written
 // where the Gram pass writes it: pair block (e1, e2) of the row block's entries at Q[e2] + d2 P[e1], row-major, so
 // (a, b) -> goff + Q[e2] + (b - P[e2]) + d2 a (the columns of a row block are its entries' in order, from 0); and
 // lane c forms (J_k' r_k)[c] at eoff + c.
 template <class T>
-__global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs, int64_t cnt,
+__global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs, int64_t cnt, int slot,
                                                   const int32_t *__restrict__ ecd, const T *__restrict__ F,
                                                   const double *__restrict__ Fr, double *__restrict__ G,
                                                   double *__restrict__ gp) {
-    const int64_t t = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    const int l = threadIdx.x & 15, base = (threadIdx.x & 63) & ~15;
-    const bool valid = t < cnt;
+    const int wl = threadIdx.x & 63, per = 64 / slot, grp = wl / slot;
+    const int l = wl - grp * slot, base = grp * slot;   // idle lanes (grp == per) join the shuffles, write nothing
+    const int64_t t = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * per + grp;
+    const bool valid = grp < per && t < cnt;
     const RGTab g = rg_load(tabs + (valid ? t : 0));
     const int m = valid ? g.n : 0, W = valid ? g.W : 0;
     double x[TINY_M];
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs
     // the wavefront's widest and tallest row block bound the shuffles (uniform across the wavefront)
     int wmax = W, mmax = m;
 #pragma unroll
-    for (int o = 16; o < 64; o <<= 1) {
+    for (int o = 1; o < 64; o <<= 1) {
         wmax = max(wmax, __shfl_xor(wmax, o, 64));
         mmax = max(mmax, __shfl_xor(mmax, o, 64));
     }
@@ -234,6 +235,7 @@ struct bls_asm {
     RGPlan rg;
     DevArray<RGTab> d_tabs, d_tiny;
     int64_t ntiny = 0;
+    int tiny_slot = 16;   // lanes per tiny row block: the widest of them
     DevArray<int32_t> d_ecd;
     int64_t glen = 0, gplen = 0;
     DevArray<double> d_G, d_gp;
@@ -454,6 +456,8 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
     if ((rc = rg_classify(recs, tabs, h->rg))) return rc;
     if ((rc = h->d_tabs.upload(tabs)) || (rc = h->d_tiny.upload(tiny)) || (rc = h->d_ecd.upload(ecd))) return rc;
     h->ntiny = (int64_t)tiny.size();
+    h->tiny_slot = 1;
+    for (const RGTab &g : tiny) h->tiny_slot = std::max(h->tiny_slot, (int)g.W);
     h->rg.tabs = h->d_tabs.ptr;
     h->rg.ecd = h->d_ecd.ptr;
     if ((rc = h->d_G.alloc((size_t)std::max<int64_t>(1, goff))) || (rc = h->d_gp.alloc((size_t)std::max<int64_t>(1, eoff))))
@@ -558,9 +562,11 @@ extern "C" int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r,
     if (a->dense && a->R_len)
         k_rg_fill<double><<<(unsigned)std::min<int64_t>((a->R_len + 255) / 256, 4096), 256, 0, s>>>(R, a->R_len);
     if ((rc = rg_launch<double>(a->rg, J, r, a->d_G.ptr, a->d_gp.ptr, s))) return rc;
-    if (a->ntiny)
-        k_asm_tiny<double><<<(unsigned)((a->ntiny + 15) / 16), 256, 0, s>>>(a->d_tiny.ptr, a->ntiny, a->d_ecd.ptr, J, r,
-                                                                          a->d_G.ptr, a->d_gp.ptr);
+    if (a->ntiny) {
+        const int64_t per_wg = 4 * (64 / a->tiny_slot);
+        k_asm_tiny<double><<<(unsigned)((a->ntiny + per_wg - 1) / per_wg), 256, 0, s>>>(
+            a->d_tiny.ptr, a->ntiny, a->tiny_slot, a->d_ecd.ptr, J, r, a->d_G.ptr, a->d_gp.ptr);
+    }
     double *dst[bls_asm::ND] = {E_data, C_data, R, b_r, b_e};
     for (int l = 0; l < bls_asm::ND; l++) {
         if (!a->nblk[l]) continue;
