@@ -168,6 +168,14 @@ def main():
     p2_gbs = p2_bytes / (kt[2] * 1e-3) / 1e9
     log(f"kernels: reset {kt[0]:.4f} ms, phase1 {kt[1]:.4f} ms ({p1_bytes / (kt[1] * 1e-3) / 1e9:.0f} GB/s), "
         f"phase2 {kt[2]:.4f} ms ({p2_gbs:.0f} GB/s), merge {kt[3]:.4f} ms")
+    # backsubstitute! (x_e = z - E_inv sum_c C_c x_r) from the reduction just run: C_data, E_inv and z read once, x_r
+    # gathered, x_e written
+    xr = torch.randn(st.nr, dtype=torch.float64, device="cuda")
+    xe = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    bms = time_ms(torch, lambda: dev.backsubstitute_(xe, xr, h, C, ws), 10)
+    bbytes = 8.0 * (p.C_data.size + st.eliminated_storage_length() + 2 * st.ne + st.nr)
+    log(f"backsubstitute!: {bms:.4f} ms, {bbytes / (bms * 1e-3) / 1e9:.0f} GB/s of {bbytes / 1e9:.3f} GB")
+    backsub = {"ms": bms, "achieved_GBs": bbytes / (bms * 1e-3) / 1e9, "bytes": bbytes}
     mixed = mixed_leg(torch, dev, p, h, S, csc, dmap, args.steps, args.warmup)
     assembly = assembly_leg(torch, dev, st)
     pcie = pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, max(3, args.steps // 4))
@@ -189,6 +197,7 @@ def main():
         "pcie_inclusive": pcie,
         "mixed": mixed,
         "assembly": assembly,
+        "backsubstitute": backsub,
     }
     del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
     torch.cuda.empty_cache()
@@ -502,6 +511,15 @@ def qr_leg(torch, dev, b, scale, reps=10):
         prods[pname] = {"ms": pms, "achieved_GBs": gb, "frac": gb / HBM_PEAK_GBS}
         log(f"QR leg, {pname}: {pms:.3f} ms, {gb:.0f} GB/s of {(xbytes + vec_bytes) / 1e9:.2f} GB")
     solve["products_f64"] = prods
+    # the QR backsubstitute! (x_e = U^-1 (q - B x_r) per eliminated table): the first d_e rows of every column of
+    # each eliminated table, and q, read once; x_r gathered, x_e written
+    xr = T(rng.standard_normal(st.nr))
+    xe = torch.empty(st.ne, dtype=torch.float64, device="cuda")
+    ed_ = np.asarray(st.e_dims, dtype=np.int64)
+    qbytes = 8.0 * float(np.sum(ed_ * (np.asarray(jst.table_ncols[:st.neblocks]) - 1) + ed_) + st.nr + st.ne)
+    qbms = time_ms(torch, lambda: dev.qr_backsubstitute_(xe, xr, h, ws))
+    log(f"QR leg, backsubstitute!: {qbms:.3f} ms, {qbytes / (qbms * 1e-3) / 1e9:.0f} GB/s of {qbytes / 1e9:.3f} GB")
+    solve["backsubstitute_f64"] = {"ms": qbms, "achieved_GBs": qbytes / (qbms * 1e-3) / 1e9, "bytes": qbytes}
     # quantities of the full problem from the raw Float64 tables: each reads every table once (8 B per value) plus
     # its vectors
     nv = st.nr + st.ne
