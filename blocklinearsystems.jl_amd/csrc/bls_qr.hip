@@ -648,27 +648,44 @@ template <class T> __global__ void k_fill_zero(int64_t n, T *p) {
 // backsubstitute! of the QR path (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r) per eliminated table, one
 // lane per table: q the first d_e rows of the factored residual, B the first d_e rows of the retained columns, U the
 // triangle of the eliminated columns; accumulated in Float64 whatever the table type.
+constexpr int QB_LANES = 16;   // lanes per table in k_qr_backsub
 template <class T>
-__global__ void k_qr_backsub(int64_t neb, const int32_t *nrows, const int32_t *ncols, const int32_t *edims,
-                             const int64_t *raw_off, const int64_t *res_off, const int64_t *boff_e,
-                             const int32_t *table_ptr, const int32_t *entry_retained, const int32_t *entry_col,
-                             const int32_t *r_dims, const int64_t *boff_r, const T *F, const double *Fr,
-                             const double *x_r, double *x_e) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= neb) return;
-    const int de = edims[e], M = nrows[e], ce = ncols[e] - de;
-    const int64_t off = raw_off[e];
+__global__ __launch_bounds__(256) void k_qr_backsub(int64_t neb, const int32_t *nrows, const int32_t *ncols,
+                                                    const int32_t *edims, const int64_t *raw_off,
+                                                    const int64_t *res_off, const int64_t *boff_e,
+                                                    const int32_t *table_ptr, const int32_t *entry_retained,
+                                                    const int32_t *entry_col, const int32_t *r_dims,
+                                                    const int64_t *boff_r, const T *F, const double *Fr,
+                                                    const double *x_r, double *x_e) {
+    // QB_LANES lanes per table: lane q adds B x_r over the entries q, q + QB_LANES, ... of the table (their columns'
+    // first d_e rows); the lanes' partials are added in a fixed tree, and lane 0 solves U x_e = q - B x_r
+    const int64_t e = (int64_t)blockIdx.x * (256 / QB_LANES) + threadIdx.x / QB_LANES;
+    const int lane = threadIdx.x % QB_LANES;
+    const bool valid = e < neb;
+    const int de = valid ? edims[e] : 0, M = valid ? nrows[e] : 0;
+    const int64_t off = valid ? raw_off[e] : 0;
     double t[QR_MAXDE];
-    for (int i = 0; i < de; i++) t[i] = Fr[res_off[e] + i];
-    for (int u = table_ptr[e]; u < table_ptr[e + 1]; u++) {
+#pragma unroll
+    for (int i = 0; i < QR_MAXDE; i++) t[i] = 0.0;
+    const int u0 = valid ? table_ptr[e] : 0, u1 = valid ? table_ptr[e + 1] : 0;
+    for (int u = u0 + lane; u < u1; u += QB_LANES) {
         const int r = entry_retained[u];
         const double *xr = x_r + boff_r[r];
         const T *cb = F + off + (int64_t)entry_col[u] * M;
         for (int j = 0; j < r_dims[r]; j++) {
             const double xj = xr[j];
-            for (int i = 0; i < de; i++) t[i] -= (double)cb[(int64_t)j * M + i] * xj;
+#pragma unroll
+            for (int i = 0; i < QR_MAXDE; i++)
+                if (i < de) t[i] += (double)cb[(int64_t)j * M + i] * xj;
         }
     }
+#pragma unroll
+    for (int i = 0; i < QR_MAXDE; i++)
+#pragma unroll
+        for (int o = QB_LANES / 2; o >= 1; o >>= 1) t[i] += __shfl_xor(t[i], o, 64);
+    if (!valid || lane != 0) return;
+    const int ce = ncols[e] - de;
+    for (int i = 0; i < de; i++) t[i] = Fr[res_off[e] + i] - t[i];
     double *xe = x_e + boff_e[e];
     for (int i = de - 1; i >= 0; i--) {
         double acc = t[i];
@@ -1054,7 +1071,7 @@ template <class T> int qr_backsub(bls_qr *h, double *x_e, const double *x_r, con
     if (rc) return rc;
     if (h->neb && (!x_e || !x_r || !F || !Fr)) return fail(BLS_ERR_ARG, "backsubstitute: a required array is NULL");
     if (h->neb)
-        k_qr_backsub<T><<<grid_for(h->neb, 128), 128, 0, s>>>(h->neb, h->d_nrows.ptr, h->d_ncols.ptr, h->d_edims.ptr,
+        k_qr_backsub<T><<<grid_for(h->neb, 256 / QB_LANES), 256, 0, s>>>(h->neb, h->d_nrows.ptr, h->d_ncols.ptr, h->d_edims.ptr,
                                                              h->d_raw_off.ptr, h->d_res_off.ptr, h->d_boff_e.ptr,
                                                              h->d_table_ptr.ptr, h->d_entry_retained.ptr,
                                                              h->d_entry_col.ptr, h->d_r_dims.ptr, h->d_boff_r.ptr, F,
