@@ -846,17 +846,22 @@ __global__ void k_gather(T *dest, const T *src, const int64_t *map, int64_t n) {
 }
 
 // backsubstitute! (src/schur_reduction.jl:541-576): one lane per eliminated block
+constexpr int BS_LANES = 16;   // lanes per eliminated block in k_backsub
 template <class TC, class TEI, class TSC>
 __global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eoffsets, const int64_t *boffsets_e,
                           const int64_t *eliminated_ptr, const int32_t *entry_retained, const int64_t *coffsets,
                           const int32_t *r_dims, const int64_t *boffsets_r, const TC *C, const TEI *E_inv,
                           const double *z, const double *x_r, const double *s_r, const double *s_e, double *x_e) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= neb) return;
-    const int de = e_dims[e];
-    const int64_t eo = eoffsets[e], bo = boffsets_e[e];
+    // BS_LANES lanes per eliminated block: lane q adds the couplings q, q + BS_LANES, ... of the block; the lanes'
+    // partials are added in a fixed tree, and the lanes i < d_e apply E_inv
+    const int64_t e = (int64_t)blockIdx.x * (256 / BS_LANES) + threadIdx.x / BS_LANES;
+    const int lane = threadIdx.x % BS_LANES;
+    const bool valid = e < neb;
+    const int de = valid ? e_dims[e] : 0;
+    const int64_t eo = valid ? eoffsets[e] : 0, bo = valid ? boffsets_e[e] : 0;
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t c = eliminated_ptr[e]; c < eliminated_ptr[e + 1]; c++) {
+    const int64_t cb0 = valid ? eliminated_ptr[e] : 0, cb1 = valid ? eliminated_ptr[e + 1] : 0;
+    for (int64_t c = cb0 + lane; c < cb1; c += BS_LANES) {
         const int r = entry_retained[c];
         const int dr = r_dims[r];
         const int64_t ro = boffsets_r[r], co = coffsets[c];
@@ -871,11 +876,16 @@ __global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eof
             }
         }
     }
-    for (int i = 0; i < de; i++) {
-        double acc = 0.0;
-        for (int j = 0; j < de; j++) acc += (double)E_inv[eo + j * de + i] * t[j];
-        x_e[bo + i] = z[bo + i] - acc;
-    }
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int o = BS_LANES / 2; o >= 1; o >>= 1) t[i] += __shfl_xor(t[i], o, 64);
+    if (!valid || lane >= de) return;
+    double acc = 0.0;   // lane i: row i of E_inv t
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if (j < de) acc += (double)E_inv[eo + j * de + lane] * t[j];
+    x_e[bo + lane] = z[bo + lane] - acc;
 }
 
 // Conditioning scales (src/conditioning.jl:99-141): Cs = the value the reference reads for every coupling entry,
@@ -1447,7 +1457,7 @@ int backsub(bls_schur *h, double *x_e, const double *x_r, const TC *C_data, cons
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (h->neb && (!x_e || !x_r || !C_data || !E_inv || !z)) return fail(BLS_ERR_ARG, "backsubstitute: a required array is NULL");
     if (h->neb)
-        k_backsub<TC, TEI, TSC><<<grid_for(h->neb, 256), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr,
+        k_backsub<TC, TEI, TSC><<<grid_for(h->neb, 256 / BS_LANES), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr,
                                                                   h->d_boffsets_e.ptr, h->d_eliminated_ptr.ptr,
                                                                   h->d_entry_retained.ptr, h->d_coffsets.ptr,
                                                                   h->d_r_dims.ptr, h->d_boffsets_r.ptr, C_data, E_inv,
