@@ -55,7 +55,13 @@ def main():
                     help="landmark scale of the reduce_qr! leg (500 cameras d 6/9, 500000 landmarks x scale); 0 skips")
     ap.add_argument("--config5-scale", type=float, default=1.0,
                     help="landmark scale of the batched 64-problem leg (cameras at full size); 0 skips it")
+    ap.add_argument("--tuning", default="",
+                    help="PART,LANES: use this phase-2 work-list shape instead of autotuning (profiling runs)")
+    ap.add_argument("--slam-only", action="store_true",
+                    help="time only the sequential-SLAM step (no other legs, no CPU baseline): for profiler runs")
     args = ap.parse_args()
+    if args.slam_only:
+        args.no_cpu, args.qr_scale, args.config5_scale = True, 0.0, 0.0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no multi-GPU path; "
               "see the sharding design in DESIGN.md", file=sys.stderr)
@@ -101,7 +107,12 @@ def main():
     # the autotuner picks the phase-2 work-list shape on this instance (landmarks per part, rows per run)
     t0 = time.time()
     cands = [(pt, ln) for pt in (384, 768, 1152, 1536, 2304) for ln in (32, 48, 64, 96)]
-    tuned = h.autotune(cands, S, g, b_r, b_e, se, lam, E, C, ws)
+    if args.tuning:
+        cands = [tuple(int(v) for v in args.tuning.split(","))]
+        h.set_tuning(*cands[0])
+        tuned = cands[0] + (float("nan"),)
+    else:
+        tuned = h.autotune(cands, S, g, b_r, b_e, se, lam, E, C, ws)
     log(f"autotune: part {tuned[0]}, lanes {tuned[1]} ({tuned[2]:.4f} ms reduce; default 384 / 96) "
         f"over {len(cands)} shapes ({time.time() - t0:.1f}s)")
     for _ in range(args.warmup):
@@ -176,11 +187,13 @@ def main():
     bbytes = 8.0 * (p.C_data.size + st.eliminated_storage_length() + 2 * st.ne + st.nr)
     log(f"backsubstitute!: {bms:.4f} ms, {bbytes / (bms * 1e-3) / 1e9:.0f} GB/s of {bbytes / 1e9:.3f} GB")
     backsub = {"ms": bms, "achieved_GBs": bbytes / (bms * 1e-3) / 1e9, "bytes": bbytes}
-    mixed = mixed_leg(torch, dev, p, h, S, csc, dmap, args.steps, args.warmup)
-    assembly = assembly_leg(torch, dev, st)
-    pcie = pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, max(3, args.steps // 4))
-    log(f"PCIe-inclusive (host E, C, b, R in; S, g, CSC out): {pcie['ms_per_step']:.3f} ms/step, "
-        f"{pcie['value']:.4g} landmarks/s")
+    mixed = assembly = pcie = None
+    if not args.slam_only:
+        mixed = mixed_leg(torch, dev, p, h, S, csc, dmap, args.steps, args.warmup)
+        assembly = assembly_leg(torch, dev, st)
+        pcie = pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, max(3, args.steps // 4))
+        log(f"PCIe-inclusive (host E, C, b, R in; S, g, CSC out): {pcie['ms_per_step']:.3f} ms/step, "
+            f"{pcie['value']:.4g} landmarks/s")
     result = {
         "metric": METRIC, "value": value, "unit": "landmarks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -199,18 +212,68 @@ def main():
         "assembly": assembly,
         "backsubstitute": backsub,
     }
+    # the step's outputs, kept for the full-size parity check against the oracle
+    S_dev, g_dev = S.cpu().numpy(), g.cpu().numpy()
     del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
     torch.cuda.empty_cache()
-    result["dense_cholesky"] = dense_leg(torch, dev, int(st.nr))
+    if not args.slam_only:
+        result["dense_cholesky"] = dense_leg(torch, dev, int(st.nr))
     torch.cuda.empty_cache()
     if args.qr_scale > 0:
         result["reduce_qr"] = qr_leg(torch, dev, b, args.qr_scale)
         torch.cuda.empty_cache()
     if args.config5_scale > 0:
-        result["batched_pcg"] = batched_pcg_leg(torch, dev, b, args.config5_scale)
+        result["batched_pcg"] = batched_pcg_leg(torch, dev, b, args.config5_scale, check_parity=not args.no_cpu)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(p, args.cpu_threads)
+        result["cpu_baseline"], S_ref, g_ref = cpu_baseline(p, args.cpu_threads)
+        result["parity"] = parity_record(st, p.pattern, S_dev, g_dev, S_ref, g_ref)
+        log(f"parity at full size against the oracle: S max block rel {result['parity']['max_block_rel']:.3g}, "
+            f"g max block rel {result['parity']['g_max_block_rel']:.3g}")
     print(json.dumps(result), flush=True)
+    checks = [result.get("parity"), (result.get("batched_pcg") or {}).get("parity")]
+    if any(c is not None and not c["ok"] for c in checks):
+        sys.exit("parity check failed: the device's S or g is more than 1e-12 (relative Frobenius per block) from "
+                 "the oracle's")
+
+
+PARITY_TOL = 1e-12
+
+
+def sparse_block_rel(st, pattern, A, B):
+    """Largest ||A_b - B_b||_F / ||B_b||_F over the upper blocks b of a sparse S (only the upper triangle of a
+    diagonal block is compared; absolute where B_b = 0)."""
+    nr, nrb = st.nr, st.nrblocks
+    colptr = np.asarray(pattern.S_colptr) - 1
+    rows = np.asarray(pattern.S_rowval) - 1
+    cols = np.repeat(np.arange(nr), np.diff(colptr))
+    bo = np.asarray(st.boffsets_r) - 1
+    brow = np.searchsorted(bo, rows, side="right") - 1
+    bcol = np.searchsorted(bo, cols, side="right") - 1
+    keep = ~((brow == bcol) & (rows > cols))
+    key = (brow * nrb + bcol)[keep]
+    _, blk = np.unique(key, return_inverse=True)
+    d2 = np.bincount(blk, weights=(A[keep] - B[keep]) ** 2)
+    n2 = np.bincount(blk, weights=B[keep] ** 2)
+    rel = np.where(n2 > 0, np.sqrt(d2) / np.sqrt(np.where(n2 > 0, n2, 1.0)), np.sqrt(d2))
+    return float(rel.max())
+
+
+def vector_block_rel(dims, a, b):
+    """Largest relative 2-norm error over the blocks (of sizes `dims`) of a vector."""
+    blk = np.repeat(np.arange(len(dims)), dims)
+    d2 = np.bincount(blk, weights=(a - b) ** 2, minlength=len(dims))
+    n2 = np.bincount(blk, weights=b ** 2, minlength=len(dims))
+    rel = np.where(n2 > 0, np.sqrt(d2) / np.sqrt(np.where(n2 > 0, n2, 1.0)), np.sqrt(d2))
+    return float(rel.max())
+
+
+def parity_record(st, pattern, S_dev, g_dev, S_ref, g_ref):
+    s_rel = sparse_block_rel(st, pattern, S_dev, S_ref)
+    g_rel = vector_block_rel(np.asarray(st.r_dims), g_dev, g_ref)
+    return {"max_block_rel": s_rel, "g_max_block_rel": g_rel,
+            "g_rel": float(np.linalg.norm(g_dev - g_ref) / np.linalg.norm(g_ref)), "tol": PARITY_TOL,
+            "ok": bool(s_rel <= PARITY_TOL and g_rel <= PARITY_TOL),
+            "against": "the oracle's Float64 reset + reduce of the same instance"}
 
 
 def time_ms(torch, fn, reps=5):
@@ -542,7 +605,7 @@ def qr_leg(torch, dev, b, scale, reps=10):
                          "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
 
 
-def batched_pcg_leg(torch, dev, b, scale, reps=3):
+def batched_pcg_leg(torch, dev, b, scale, reps=3, check_parity=False):
     """64 independent problems (camera counts U{50..2000}, landmark counts U{10k..300k} x scale, SLAM tracks,
     lambda 1e-4) as one block-diagonal structure: one reduction (one launch per kernel), then block-Jacobi PCG on all
     64 reduced systems at once to ||S x - g|| <= 1e-8 ||g|| per problem (FP64 true residual). Reports the
@@ -613,13 +676,30 @@ def batched_pcg_leg(torch, dev, b, scale, reps=3):
         log(f"batched leg, learned correction (output scale {model.alpha}): PCG {lms:.3f} ms (iterations max "
             f"{int(itl.max())}, mean {itl.mean():.1f}), time to 1e-8 {red + lms:.3f} ms")
     mixed = batched_mixed(torch, dev, b, p, h, pcg, S, g, reps)
+    parity = None
+    if check_parity:
+        # the union's reduction at full size against the oracle's (the device's S and g of the last timed reduce)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        t0 = time.time()
+        threads = cpu_share()[0]
+        args = oracle.SchurArgs(st, b_ws(p, threads), p.pattern)
+        S_ref, g_ref = np.zeros(p.pattern.nnz), np.zeros(st.nr)
+        E_inv, z, W = np.zeros(len(p.E_data)), np.zeros(st.ne), np.zeros(len(p.C_data))
+        oracle.reset_schur_sparse(S_ref, p.R_data, p.r_to_s, p.pattern, p.scaling_r, p.lam)
+        ok, _ = oracle.reduce_schur(True, args, S_ref, g_ref, p.b_r, p.b_e, p.scaling_e, p.lam, p.E_data, p.C_data,
+                                    E_inv, z, W, nthreads=threads)
+        assert ok
+        parity = parity_record(st, p.pattern, S.cpu().numpy(), g.cpu().numpy(), S_ref, g_ref)
+        log(f"batched leg parity against the oracle: S max block rel {parity['max_block_rel']:.3g}, g max block rel "
+            f"{parity['g_max_block_rel']:.3g} ({time.time() - t0:.1f}s)")
     return {"problems": int(p.nprob), "landmark_scale": scale, "cameras": int(st.nrblocks),
             "landmarks": int(st.neblocks), "S_nnz": int(p.pattern.nnz), "reduce_ms": red,
             "landmarks_per_s": st.neblocks / (red * 1e-3), "pcg_ms": sol, "time_to_1e-8_ms": red + sol,
             "pcg_iters_max": int(iters.max()), "pcg_iters_mean": float(iters.mean()),
             "relres_max": float(rel.max()), "preconditioner": "block-Jacobi (FP64)", "learned": learned,
             "tuning": {"part": tuned[0], "lanes": tuned[1]},
-            "mixed": mixed}
+            "mixed": mixed, "parity": parity}
 
 
 def batched_mixed(torch, dev, b, p, h, pcg, S64, g64, reps):
@@ -722,16 +802,40 @@ def pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, steps):
             "host_bytes_per_step": moved}
 
 
-def cpu_baseline(p, threads):
-    """The oracle (C restatement of the reference's threaded reduction) on the host cores: reset + reduce of the
-    same instance, median of repeated runs bounded to ~15 s."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def cpu_share():
+    """Cores this process may use: its affinity set, bounded by a cgroup CPU quota when one is set, and by the 16
+    cores that a one-GPU job is given on the GPU hosts. Returns (threads, description)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    share = min(aff, quota) if quota else aff
+    threads = min(share, 16)
+    return threads, {"nproc": os.cpu_count(), "affinity": aff, "cgroup_quota_cores": quota, "threads_all": threads}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_oracle(oracle, p, args, threads, budget_s, max_runs=7):
+    """Median wall time of reset + reduce by the oracle with `threads` OpenMP threads; also returns S and g."""
     st = p.structure
-    threads = threads or min(16, os.cpu_count() or 1)
-    # chunked like the reference's threaded path (nchunks = threads), results identical to serial
-    ws = b_ws(p, threads)
-    args = oracle.SchurArgs(st, ws, p.pattern)
     S = np.zeros(p.pattern.nnz)
     g = np.zeros(st.nr)
     E_inv = np.zeros(len(p.E_data))
@@ -739,17 +843,52 @@ def cpu_baseline(p, threads):
     W = np.zeros(len(p.C_data))
     times = []
     t_start = time.time()
-    while len(times) < 7 and (time.time() - t_start < 15 or len(times) < 2):
+    while len(times) < max_runs and (time.time() - t_start < budget_s or len(times) < 2):
         t0 = time.perf_counter()
         oracle.reset_schur_sparse(S, p.R_data, p.r_to_s, p.pattern, p.scaling_r, p.lam)
         ok, _ = oracle.reduce_schur(True, args, S, g, p.b_r, p.b_e, p.scaling_e, p.lam, p.E_data, p.C_data, E_inv, z, W,
                                     nthreads=threads)
         times.append(time.perf_counter() - t0)
         assert ok
-    t = float(np.median(times))
-    return {"value": st.neblocks / t, "unit": "landmarks/s", "cores": threads, "kind": "port",
-            "sample": f"full workload (reset + reduce of {st.neblocks} landmarks), median of {len(times)} runs, "
-                      f"C oracle -O2 with OpenMP over the reference's chunks"}
+    return float(np.median(times)), len(times), S, g
+
+
+def cpu_baseline(p, threads):
+    """The oracle (C restatement of the reference's threaded reduction, src/threading.jl:24-39 and
+    ext/PolyesterExt.jl:12-23: phase 1 over nchunks chunks per run, phase 2 over the work-balanced column chunks)
+    on the host cores: reset + reduce of the same instance, built -O3 -march=native on this host, timed with every
+    core of the process's share and with one thread. Returns the baseline record and the oracle's S and g (all
+    thread counts give the same bits)."""
+    import shutil
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    st = p.structure
+    share, cores = cpu_share()
+    threads = threads or share
+    cores["threads_all"] = threads
+    tmp = tempfile.mkdtemp(prefix="blsorc_")
+    build = oracle.use_native(tmp)
+    shutil.rmtree(tmp, ignore_errors=True)   # the library stays mapped
+    if build is None:
+        build = "oracle/liborc.so (gcc -O2; the -O3 -march=native build failed on this host)"
+    # chunked like the reference's threaded path (nchunks = threads), results identical to serial
+    ws = b_ws(p, threads)
+    args = oracle.SchurArgs(st, ws, p.pattern)
+    t_all, n_all, S, g = time_oracle(oracle, p, args, threads, 15.0)
+    ws1 = b_ws(p, 1)
+    args1 = oracle.SchurArgs(st, ws1, p.pattern)
+    t_one, n_one, S1, g1 = time_oracle(oracle, p, args1, 1, 20.0, max_runs=5)
+    assert np.array_equal(S, S1) and np.array_equal(g, g1)
+    log(f"CPU baseline ({cpu_model()}, {threads} threads of nproc {cores['nproc']}): {st.neblocks / t_all:.4g} "
+        f"landmarks/s; 1 thread {st.neblocks / t_one:.4g} landmarks/s")
+    rec = {"value": st.neblocks / t_all, "unit": "landmarks/s", "cores": threads, "kind": "port",
+           "value_all": st.neblocks / t_all, "value_1thread": st.neblocks / t_one, "ms_all": t_all * 1e3,
+           "ms_1thread": t_one * 1e3, "cpu_model": cpu_model(), **cores, "build": build,
+           "sample": f"full workload (reset + reduce of {st.neblocks} landmarks), median of {n_all} runs at "
+                     f"{threads} threads and of {n_one} runs at 1 thread; the C oracle with OpenMP over the "
+                     f"reference's chunks"}
+    return rec, S, g
 
 
 def b_ws(p, threads):

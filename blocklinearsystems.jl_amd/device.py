@@ -345,8 +345,31 @@ def _reduce(handle, dense, S, g, b_r, b_e, sc, lam, E, C, ws, s_r, s_e, stream, 
     return None
 
 
+def _require(name, t, dtype, n, optional=False):
+    """Check, before a launch, that a vector argument has the element type and length the kernels index it with
+    (a wrong one would make them read or write past the buffer); raises BLSError otherwise."""
+    if t is None:
+        if optional:
+            return
+        raise BLSError(f"{name} is required")
+    if t.dtype != dtype:
+        raise BLSError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.numel() != n:
+        raise BLSError(f"{name} has length {t.numel()}, expected {n}")
+
+
+def _check_backsub(st, variant, x_e, x_r, C_data, s_r, s_e):
+    import torch
+    _require("x_e", x_e, torch.float64, st.ne)
+    _require("x_r", x_r, torch.float64, st.nr)
+    _require("C_data", C_data, torch.float64 if variant == "f64" else torch.float32, st.coupling_storage_length())
+    _require("s_r", s_r, torch.float64, st.nr, optional=True)
+    _require("s_e", s_e, torch.float64, st.ne, optional=True)
+
+
 def backsubstitute_(x_e, x_r, handle, C_data, ws, s_r=None, s_e=None, stream=None):
     """backsubstitute! (src/schur_reduction.jl:519-576)."""
+    _check_backsub(handle.st, ws.variant, x_e, x_r, C_data, s_r, s_e)
     f = getattr(lib(), "bls_backsubstitute_schur_" + ws.variant)
     _check(f(handle.h, _dp(x_e), _dp(x_r), _dp(C_data), _dp(ws.E_inv_data), _dp(ws.z_data), _dp(s_r), _dp(s_e),
              _stream(stream)))
@@ -579,8 +602,19 @@ def reduced_jacobian_(Jt, rt, handle, ws, sparse, stream=None):
     return Jt
 
 
+def _check_qr_backsub(jst, ws, x_e, x_r):
+    import torch
+    if ws.structure is not jst:
+        raise BLSError("the workspace was built for another structure")
+    _require("x_e", x_e, torch.float64, jst.schur.ne)
+    _require("x_r", x_r, torch.float64, jst.schur.nr)
+    if ws.factored.dtype not in (torch.float64, torch.float32) or ws.factored.numel() != jst.raw_length():
+        raise BLSError("the workspace's factored tables do not match the structure")
+
+
 def qr_backsubstitute_(x_e, x_r, handle, ws, stream=None):
     """backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:24-110): x_e = U^-1 (q - B x_r)."""
+    _check_qr_backsub(handle.jst, ws, x_e, x_r)
     _check(getattr(lib(), "bls_qr_backsubstitute_" + _fsfx(ws.factored))(handle.h, _dp(x_e), _dp(x_r),
                                                                          _dp(ws.factored), _dp(ws.fres),
                                                                          _stream(stream)))

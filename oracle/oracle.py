@@ -45,22 +45,42 @@ def build():
 
 
 _lib = None
+NATIVE_FLAGS = ["-O3", "-march=native", "-fPIC", "-std=c11", "-ffp-contract=off", "-fopenmp"]
+
+
+def use_native(outdir):
+    """Compile the same oracle source with -O3 -march=native (still -ffp-contract=off, so every result is the same
+    as the -O2 build's) into `outdir` on the host that will run it, and make it the library the functions below
+    call. Used only by the CPU-baseline timing; returns the compiler command, or None when the compiler fails (the
+    -O2 build then stays in use)."""
+    so =os.path.join(outdir, "liborc_native.so")
+    cmd = [os.environ.get("CC", "gcc")] + NATIVE_FLAGS + ["-shared", "-o", so, os.path.join(HERE, "bls_oracle.c"), "-lm"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True)
+    except (OSError, subprocess.CalledProcessError):
+        return None
+    _bind(ct.CDLL(so))
+    return " ".join(cmd[:len(NATIVE_FLAGS) + 1]) + " bls_oracle.c"
 
 
 def lib():
-    global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        _lib = ct.CDLL(LIB)
-        for s, ty in (("f64", PD), ("f32", PF)):
-            getattr(_lib, f"orc_reduce_schur_{s}").restype = ct.c_int
-            getattr(_lib, f"orc_reduce_qr_{s}").restype = ct.c_int
-            getattr(_lib, f"orc_inv_sym_block_{s}").restype = ct.c_int
-            getattr(_lib, f"orc_inv_sym_block_chol_{s}").restype = ct.c_int
-            getattr(_lib, f"orc_factor_diagonal_blocks_{s}").restype = ct.c_int
-        _lib.orc_pcg_f64.restype = ct.c_int64
+        _bind(ct.CDLL(LIB))
     return _lib
+
+
+def _bind(handle):
+    global _lib
+    for s in ("f64", "f32"):
+        getattr(handle, f"orc_reduce_schur_{s}").restype = ct.c_int
+        getattr(handle, f"orc_reduce_qr_{s}").restype = ct.c_int
+        getattr(handle, f"orc_inv_sym_block_{s}").restype = ct.c_int
+        getattr(handle, f"orc_inv_sym_block_chol_{s}").restype = ct.c_int
+        getattr(handle, f"orc_factor_diagonal_blocks_{s}").restype = ct.c_int
+    handle.orc_pcg_f64.restype = ct.c_int64
+    _lib = handle
 
 
 def _i64(a):

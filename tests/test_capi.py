@@ -45,3 +45,45 @@ def test_no_gpu_fallback_in_product(blsamd):
             pass
     finally:
         dev.LIBPATH, dev._lib = saved_path, saved_lib
+
+
+def test_backsubstitute_arguments_rejected_before_launch(blsamd):
+    """backsubstitute! on either path checks the element type and length of every vector it indexes, on the host,
+    before any launch: a Float32 x_e, a short x_r or C_data of the wrong type would otherwise send the kernels past
+    the end of a buffer."""
+    import pytest
+    import torch
+    dev = blsamd.device
+    p = blsamd.generators.config2(scale=0.0005)
+    st = p.structure
+    f64 = lambda n: torch.zeros(n, dtype=torch.float64)
+    f32 = lambda n: torch.zeros(n, dtype=torch.float32)
+    nc = st.coupling_storage_length()
+    dev._check_backsub(st, "f64", f64(st.ne), f64(st.nr), f64(nc), None, f64(st.ne))
+    dev._check_backsub(st, "mixed", f64(st.ne), f64(st.nr), f32(nc), f64(st.nr), None)
+    bad = [("f64", f32(st.ne), f64(st.nr), f64(nc), None, None),
+           ("f64", f64(st.ne), f64(st.nr - 1), f64(nc), None, None),
+           ("f64", f64(st.ne), f64(st.nr), f32(nc), None, None),
+           ("f32", f64(st.ne), f64(st.nr), f64(nc), None, None),
+           ("f64", f64(st.ne), f64(st.nr), f64(nc - 3), None, None),
+           ("f64", f64(st.ne), f64(st.nr), f64(nc), f64(st.nr + 1), None),
+           ("f64", f64(st.ne), f64(st.nr), f64(nc), None, f32(st.ne)),
+           ("f64", None, f64(st.nr), f64(nc), None, None)]
+    for args in bad:
+        with pytest.raises(dev.BLSError):
+            dev._check_backsub(st, *args)
+    q = blsamd.generators.config4(scale=0.0001, ncam=12)
+    jst = q.jst
+
+    class WS:
+        structure = jst
+        factored = f64(jst.raw_length())
+    dev._check_qr_backsub(jst, WS, f64(jst.schur.ne), f64(jst.schur.nr))
+    for xe, xr in ((f32(jst.schur.ne), f64(jst.schur.nr)), (f64(jst.schur.ne), f64(jst.schur.nr + 6))):
+        with pytest.raises(dev.BLSError):
+            dev._check_qr_backsub(jst, WS, xe, xr)
+
+    class WS2(WS):
+        factored = f64(jst.raw_length() - 1)
+    with pytest.raises(dev.BLSError):
+        dev._check_qr_backsub(jst, WS2, f64(jst.schur.ne), f64(jst.schur.nr))

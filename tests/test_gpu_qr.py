@@ -204,13 +204,17 @@ def test_qr_conditioning_scale(blsamd):
 
 
 @pytest.mark.parametrize("table_type", ["f64", "f32"])
-def test_qr_backsubstitute_matches_oracle(blsamd, table_type):
+@pytest.mark.parametrize("ncam,kmax", [(9, 6), (40, 30)])
+def test_qr_backsubstitute_matches_oracle(blsamd, table_type, ncam, kmax):
     """backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:24-110) on the device from the
     device-factored tables, against the oracle on the same tables (1e-12 relative; Float64 accumulation for both
-    table types)."""
+    table types). kmax 30: tables of up to 30 retained entries (63 rows), so the sixteen lanes of a table take
+    several entries each."""
     import torch
     dev = blsamd.device
-    p = _small_jacobian(blsamd, 12, nland=90, ncam=9, kmax=6)
+    p = _small_jacobian(blsamd, 12, nland=90, ncam=ncam, kmax=kmax)
+    if kmax > 16:
+        assert np.diff(p.jst.table_ptr).max() > 16
     jst, st = p.jst, p.structure
     dt = torch.float64 if table_type == "f64" else torch.float32
     h = dev.QRHandle(jst)

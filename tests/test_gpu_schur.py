@@ -400,3 +400,31 @@ def test_autotune_shapes_agree(blsamd):
     dev.reset_schur_sparse_(S, Rd, h, sr, p.lam)
     part, lanes, ms = h.autotune(cands, S, g, b_r, b_e, se, p.lam, E, C, ws)
     assert (part, lanes) in cands and ms > 0
+
+
+@pytest.mark.parametrize("shape", [None, (1536, 96), (2304, 64)])
+def test_config5_union_matches_oracle(blsamd, shape):
+    """The batched configuration's union structure (problems side by side, one reduction for all of them) on 8
+    problems of uneven size, at the default work-list shape and at the shapes the bench's tuner picks for the
+    union and for the sequential-SLAM workload. The dense columns of the small problems (about 500-900 landmarks
+    per camera) are split into parts and merged, so the split path is covered too. S within 1e-12 relative
+    Frobenius per block of the oracle's; g, E_inv and z as everywhere."""
+    p = blsamd.generators.config5_union(ncams=[50, 80, 120, 60, 200, 90, 150, 70],
+                                        nlands=[24000, 8000, 20000, 6000, 15000, 9000, 11000, 7000])
+    st = p.structure
+    per_col = np.bincount(np.asarray(st.entry_retained) - 1, minlength=st.nrblocks)
+    assert per_col.max() > 2304 and p.nprob == 8   # columns longer than any part: every shape splits some
+    torch = _torch()
+    dev = blsamd.device
+    h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=p.r_to_s)
+    if shape is not None:
+        h.set_tuning(*shape)
+    ws = dev.SchurReductionWorkspace(h)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
+    g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
+    dev.reset_schur_sparse_(S, T(p.R_data), h, T(p.scaling_r), p.lam)
+    ok = dev.reduce_schur_sparse_(S, g, T(p.b_r), T(p.b_e), T(p.scaling_e), p.lam, h, T(p.E_data), T(p.C_data), ws)
+    gpu = dict(S=S.cpu().numpy(), g=g.cpu().numpy(), E_inv=ws.E_inv_data.cpu().numpy(), z=ws.z_data.cpu().numpy(),
+               ok=ok)
+    compare(p, gpu, oracle.schur_reduce_problem(p, nthreads=4))
