@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "bls_internal.h"
@@ -1140,20 +1141,28 @@ void launch_phase1(const Run &r, const typename P::TE *E, const double *b_e, con
                                                             s_e, E_inv, z, flag);
 }
 
+// Device scratch of one reduction: the failure flag (phase 1's CAS-min, read by phase 2) and the row sums of
+// the parts of split items. The handle owns one; bls_reduce_schur_sparse_f64_noalloc uses the library's static
+// array instead.
+struct P2Scratch {
+    unsigned long long *flag;
+    double *partial;
+};
+
 template <class P>
 int phase1(bls_schur *h, const typename P::TE *E, const double *b_e, const double *sc, double lambda,
-           const double *s_e, typename P::TEI *E_inv, double *z, hipStream_t s) {
+           const double *s_e, typename P::TEI *E_inv, double *z, unsigned long long *flag, hipStream_t s) {
     for (const Run &r : h->runs) {
         if (r.count == 0) continue;
         switch (r.de) {
-        case 1: launch_phase1<1, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 2: launch_phase1<2, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 3: launch_phase1<3, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 4: launch_phase1<4, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 5: launch_phase1<5, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 6: launch_phase1<6, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 7: launch_phase1<7, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
-        case 8: launch_phase1<8, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, h->d_flag.ptr, s); break;
+        case 1: launch_phase1<1, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
+        case 2: launch_phase1<2, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
+        case 3: launch_phase1<3, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
+        case 4: launch_phase1<4, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
+        case 5: launch_phase1<5, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
+        case 6: launch_phase1<6, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
+        case 7: launch_phase1<7, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
+        case 8: launch_phase1<8, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
         default: return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension above 8");
         }
     }
@@ -1178,7 +1187,8 @@ template <class P> void launch_phase2(int de, int max_dr, unsigned grid, hipStre
 
 template <class Q>
 void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, const typename Q::TC *C,
-                const double *b_r, const double *s_r, typename Q::TS *S, double *g, hipStream_t s) {
+                const double *b_r, const double *s_r, typename Q::TS *S, double *g, const P2Scratch &ws,
+                hipStream_t s) {
     P2Args<Q> a{};
     a.items = h->d_items.ptr;
     a.nitems = h->nitems;
@@ -1187,7 +1197,7 @@ void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, con
     a.slot_rowbase = h->d_slot_rowbase.ptr;
     a.slot_dr = h->d_slot_dr.ptr;
     a.slot_roff = h->d_slot_roff.ptr;
-    a.partial = h->d_partial.ptr;
+    a.partial = ws.partial;
     a.eoffsets = h->d_eoffsets.ptr;
     a.boffsets_e = h->d_boffsets_e.ptr;
     a.coffsets = h->d_coffsets.ptr;
@@ -1199,7 +1209,7 @@ void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, con
     a.s_r = s_r;
     a.S = S;
     a.g = g;
-    a.flag = h->d_flag.ptr;
+    a.flag = ws.flag;
     a.uniform_offsets = h->uniform_de > 0 && h->uniform_dr_all > 0 && h->runs.size() == 1;
     a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
     if (h->nitems > 0) {
@@ -1207,7 +1217,7 @@ void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, con
         h->timing.mark(3, s);
         if (h->nmerges > 0)
             k_phase2_merge<typename Q::TS><<<(unsigned)h->nmerges, P2_THREADS, 0, s>>>(
-                h->d_merges.ptr, h->d_slot_rowbase.ptr, h->d_slot_dr.ptr, h->d_partial.ptr, S, g, h->d_flag.ptr);
+                h->d_merges.ptr, h->d_slot_rowbase.ptr, h->d_slot_dr.ptr, ws.partial, S, g, ws.flag);
     }
     if (h->nitems == 0) h->timing.mark(3, s);
     h->timing.mark(4, s);
@@ -1217,7 +1227,7 @@ template <class P>
 int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, const double *b_r, const double *b_e,
                  const double *scaling_e, double lambda, const typename P::TE *E, const typename P::TC *C,
                  typename P::TEI *E_inv, double *z, const double *s_r, const double *s_e, int64_t *d_failed,
-                 hipStream_t s) {
+                 hipStream_t s, const P2Scratch *scratch = nullptr) {
     using TC = typename P::TC;
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (h->mode != want_mode)
@@ -1231,9 +1241,10 @@ int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, cons
         int rc = h->d_Cs.alloc(h->clen);
         if (rc) return rc;
     }
+    const P2Scratch ws = scratch ? *scratch : P2Scratch{h->d_flag.ptr, h->d_partial.ptr};
     h->timing.mark(1, s);
-    k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
-    int rc = phase1<P>(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, s);
+    k_fill_u64<<<1, 1, 0, s>>>(ws.flag, (unsigned long long)NO_FAILURE);
+    int rc = phase1<P>(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, ws.flag, s);
     if (rc) return rc;
     if ((s_r || s_e) && h->neb) {
         using Q = typename ScaledPolicy<P>::type;
@@ -1244,12 +1255,12 @@ int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, cons
                                                                   h->d_coffsets.ptr, h->d_r_dims.ptr,
                                                                   h->d_boffsets_r.ptr, C, s_r, s_e, Cs);
         h->timing.mark(2, s);
-        run_phase2<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, s);
+        run_phase2<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, ws, s);
     } else {
         h->timing.mark(2, s);
-        run_phase2<P>(h, E_inv, z, C, b_r, s_r, S, g, s);
+        run_phase2<P>(h, E_inv, z, C, b_r, s_r, S, g, ws, s);
     }
-    k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
+    k_finalize_flag<<<1, 1, 0, s>>>(ws.flag, d_failed);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
@@ -1511,6 +1522,55 @@ extern "C" int bls_reduce_schur_sparse_f64_async(bls_schur *h, double *nzval, do
                                                  void *stream) {
     return reduce_async<PrecF64>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
                                  d_failed_block, (hipStream_t)stream);
+}
+
+// ---- the reduction that allocates nothing ------------------------------------------------------------------
+// Scratch of bls_reduce_schur_sparse_f64_noalloc: word 0 the failure flag, then the row sums of split items.
+// Each GPU holds its own copy for the life of the process; host code uses it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host-side stand-in).
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = int64_t(1) << 21;   // 16 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int scratch_on_current_device(double **out) {
+    int dev = 0;
+    BLS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(BLS_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        BLS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(BLS_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_reduce_schur_sparse_f64_noalloc(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                                   const double *b_e, const double *scaling_e, double lambda,
+                                                   const double *E_data, const double *C_data, double *E_inv,
+                                                   double *z, int64_t *d_failed_block, void *stream) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    const int64_t need = 1 + (int64_t)h->d_partial.n;
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return fail(BLS_ERR_ARG, "reduce_schur (no allocation): the work list's split items need " +
+                                     std::to_string(need) + " doubles of scratch, more than the " +
+                                     std::to_string(NOALLOC_SCRATCH_DOUBLES) +
+                                     " the library holds; use longer parts (bls_schur_set_tuning) or "
+                                     "bls_reduce_schur_sparse_f64_async");
+    double *p = nullptr;
+    int rc = scratch_on_current_device(&p);
+    if (rc) return rc;
+    const P2Scratch ws{reinterpret_cast<unsigned long long *>(p), p + 1};
+    return reduce_async<PrecF64>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, nullptr,
+                                 nullptr, d_failed_block, (hipStream_t)stream, &ws);
 }
 
 extern "C" int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const double *b_r,

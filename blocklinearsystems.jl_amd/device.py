@@ -44,6 +44,7 @@ SIGNATURES = {
     "bls_reduce_schur_sparse_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, P64, VP]),
     "bls_reduce_schur_dense_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, P64, VP]),
     "bls_reduce_schur_sparse_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_reduce_schur_sparse_f64_noalloc": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP]),
     "bls_reduce_schur_dense_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_backsubstitute_schur_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_copyto_csc_f64": (ct.c_int, [ct.c_int64, VP, VP, VP, VP]),
@@ -365,6 +366,21 @@ def _check_backsub(st, variant, x_e, x_r, C_data, s_r, s_e):
     _require("C_data", C_data, torch.float64 if variant == "f64" else torch.float32, st.coupling_storage_length())
     _require("s_r", s_r, torch.float64, st.nr, optional=True)
     _require("s_e", s_e, torch.float64, st.ne, optional=True)
+
+
+def reduce_schur_sparse_noalloc_(nzval, g_S, b_r, b_e, scaling_e, lam, handle, E_data, C_data, ws, stream=None):
+    """The Float64 sparse reduction through bls_reduce_schur_sparse_f64_noalloc: the entry point that allocates no
+    device memory (its scratch is the library's static array; no conditioning scales). The status is left on the
+    device in ws.d_failed, as with sync=False."""
+    import math
+    if not (math.isfinite(lam) and lam >= 0):
+        raise ValueError(f"lambda must be finite and non-negative, got {lam}")
+    if ws.variant != "f64" or E_data.dtype != C_data.dtype or nzval.dtype != E_data.dtype:
+        raise BLSError("the entry point that allocates nothing is Float64 only")
+    _check(lib().bls_reduce_schur_sparse_f64_noalloc(handle.h, _dp(nzval), _dp(g_S), _dp(b_r), _dp(b_e),
+                                                      _dp(scaling_e), float(lam), _dp(E_data), _dp(C_data),
+                                                      _dp(ws.E_inv_data), _dp(ws.z_data), _dp(ws.d_failed),
+                                                      _stream(stream)))
 
 
 def backsubstitute_(x_e, x_r, handle, C_data, ws, s_r=None, s_e=None, stream=None):

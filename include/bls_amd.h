@@ -101,6 +101,17 @@ int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const
                                      const double *scaling_e, double lambda, const double *E_data,
                                      const double *C_data, double *E_inv, double *z, const double *s_r,
                                      const double *s_e, int64_t *d_failed_block, void *stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops): bls_reduce_schur_sparse_f64_async without conditioning scales (s_r = s_e =
+ * UnitScale), whose scratch -- the failure flag and the row sums of split items -- is a static array of the
+ * library (16 MiB per GPU, held until the process ends) instead of the handle's buffers. Makes no allocation.
+ * Returns BLS_ERR_ARG when the handle's work list needs more scratch than that (longer parts, set with
+ * bls_schur_set_tuning, need less). Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (on different streams, or from different host threads) race; serialize them. */
+int bls_reduce_schur_sparse_f64_noalloc(bls_schur *h, double *nzval, double *g_S, const double *b_r,
+                                        const double *b_e, const double *scaling_e, double lambda,
+                                        const double *E_data, const double *C_data, double *E_inv, double *z,
+                                        int64_t *d_failed_block, void *stream);
 
 /* backsubstitute! (src/schur_reduction.jl:519-576): x_e = z_e - E_inv sum_r C_er x_r, from the E_inv and z of
  * the last reduction. */
