@@ -675,7 +675,7 @@ def batched_pcg_leg(torch, dev, b, scale, reps=3, check_parity=False):
                    "output_scale": model.alpha}
         log(f"batched leg, learned correction (output scale {model.alpha}): PCG {lms:.3f} ms (iterations max "
             f"{int(itl.max())}, mean {itl.mean():.1f}), time to 1e-8 {red + lms:.3f} ms")
-    mixed = batched_mixed(torch, dev, b, p, h, pcg, S, g, reps)
+    mixed = batched_mixed(torch, dev, b, p, h, pcg, S, g, reps, f64=dict(E=E, C=C, R=Rd, b_r=b_r, b_e=b_e, sr=sr, se=se))
     parity = None
     if check_parity:
         # the union's reduction at full size against the oracle's (the device's S and g of the last timed reduce)
@@ -702,7 +702,7 @@ def batched_pcg_leg(torch, dev, b, scale, reps=3, check_parity=False):
             "mixed": mixed, "parity": parity}
 
 
-def batched_mixed(torch, dev, b, p, h, pcg, S64, g64, reps):
+def batched_mixed(torch, dev, b, p, h, pcg, S64, g64, reps, f64=None):
     """The 64 problems with their blocks E, C, R stored in Float32: the mixed reduction (Float64 factorisation and
     reduced system of the Float32 blocks), then the same PCG (plain and learned) to 1e-8 on that system. Also the
     residual of its solution against the Float64-stored problem's system, ||S64 x - g64|| / ||g64|| per problem:
@@ -764,7 +764,50 @@ def batched_mixed(torch, dev, b, p, h, pcg, S64, g64, reps):
         out["learned"] = {"pcg_ms": lms, "time_to_1e-8_ms": red + lms, "pcg_iters_max": int(itl.max()),
                           "relres_max": float(rell.max()), "used_learned": bool(used), "output_scale": model.alpha}
         log(f"batched leg, mixed + learned: time to 1e-8 {red + lms:.3f} ms")
+    if f64 is not None:
+        out["refined"] = refined_leg(torch, dev, b, p, h, pcg, S, g, S64, g64, reduce, f64, reps)
     return out
+
+
+def refined_leg(torch, dev, b, p, h, pcg, S_m, g_m, S64, g64, reduce, f64, reps):
+    """The mixed reduction followed by Float64 iterative refinement to ||g - S x|| <= 1e-8 ||g|| against the
+    Float64 blocks' reduced system: the Float64 factorisation of the landmark blocks, R + damping reset into the
+    reduced pattern, then {PCG on the mixed system to 1e-4; x += d; the residual from the Float64 blocks} until
+    met. The whole of it is timed; the formed Float64 system is used only to check the result (untimed)."""
+    R = b.refine
+    st = p.structure
+    SR = torch.empty_like(S64)
+    rd = R.RefinementData(h, f64["E"], f64["C"], f64["b_r"], f64["b_e"], f64["se"], SR, p.lam)
+    x = torch.zeros_like(g64)
+    P = b.precond
+    model = P.MLP.load() if os.path.exists(P.WEIGHTS) else None
+
+    def run(learned):
+        reduce()
+        dev.reset_schur_sparse_(SR, f64["R"], h, f64["sr"], p.lam, check=False)
+        assert R.factor(rd, check=False)
+        blocks = None
+        if learned and model is not None and model.alpha != 0.0:
+            blocks = P.shift_blocks(pcg, S_m, model.mu(P.features(pcg, S_m, p.lam)))
+        return R.refined_solve(pcg, S_m, g_m, x, rd, tol=1e-8, factored=True, precond_blocks=blocks)
+
+    res = {}
+    for name, learned in (("plain", False), ("learned", True)):
+        run(learned)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ok, steps, iters, rel = run(learned)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ms = float(np.median(ts)) * 1e3
+        true_rel = float(pcg.relres(S64, g64, x).max())
+        res[name] = {"time_to_1e-8_ms": ms, "converged": bool(ok), "steps": int(steps),
+                     "pcg_iters_max": int(iters.max()), "relres_vs_f64_system_max": true_rel}
+        log(f"batched leg, mixed + Float64 refinement ({name}): time to 1e-8 of the Float64 system {ms:.3f} ms, "
+            f"{steps} steps, {int(iters.max())} PCG iterations, ||g - S x|| / ||g|| = {true_rel:.2e}")
+    return res
 
 
 def pcie_inclusive(torch, dev, p, h, ws, S, g, csc, dmap, steps):

@@ -441,6 +441,26 @@ __global__ __launch_bounds__(UPD_THREADS) void k_pcg_relres(PcgArgs a, const dou
     if (threadIdx.x == 0) out[pb] = gg > 0 ? sqrt(rr) / sqrt(gg) : sqrt(rr);
 }
 
+// r = t - S x (q holds S x) and, per problem, ||r|| / ||gref|| (gref: the right-hand side the residual is judged
+// against); one workgroup per problem, fixed-order sums.
+__global__ __launch_bounds__(UPD_THREADS) void k_pcg_residual(PcgArgs a, const double *t, const double *gref,
+                                                              double *r, double *out) {
+    __shared__ double red[UPD_THREADS];
+    const int pb = blockIdx.x;
+    const int64_t b0 = a.prob_bptr[pb], b1 = a.prob_bptr[pb + 1];
+    const int64_t s0 = a.bo[b0], s1 = a.bo[b1 - 1] + a.dims[b1 - 1];
+    double gg = 0.0, rr = 0.0;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
+        const double rv = t[s] - a.q[s], gv = gref[s];
+        r[s] = rv;
+        gg += gv * gv;
+        rr += rv * rv;
+    }
+    gg = wg_sum(gg, red);
+    rr = wg_sum(rr, red);
+    if (threadIdx.x == 0) out[pb] = gg > 0 ? sqrt(rr) / sqrt(gg) : sqrt(rr);
+}
+
 __global__ void k_pcg_init_state(ProbState *st, int nprob, int *nactive) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *nactive = 0;
@@ -719,6 +739,33 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
         if (stv[p].failed || !(stv[p].relres <= tol)) all = false;
     }
     return all ? BLS_OK : BLS_FAILED;
+}
+
+__global__ void k_axpy_f64(int64_t n, double alpha, const double *x, double *y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] += alpha * x[i];
+}
+
+extern "C" int bls_axpy_f64(int64_t n, double alpha, const double *x, double *y, void *stream) {
+    if (n < 0 || (n && (!x || !y))) return fail(BLS_ERR_ARG, "bls_axpy_f64: bad arguments");
+    if (n) k_axpy_f64<<<std::min<int64_t>(grid_for(n, 256), 4096), 256, 0, (hipStream_t)stream>>>(n, alpha, x, y);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_pcg_residual_f64(bls_pcg *h, const double *S_nzval, const double *t, const double *x,
+                                    const double *gref, double *r, double *relres, void *stream) {
+    if (!h || !S_nzval || !t || !x || !gref || !r || !relres) return fail(BLS_ERR_ARG, "bls_pcg_residual_f64: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    PcgArgs a = h->args(1.0, 0);
+    k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
+    k_copy_f64<<<std::min<int64_t>(grid_for(h->nr, 256), 4096), 256, 0, s>>>(h->d_z.ptr, x, h->nr);
+    launch_spmv(h, a, h->d_z.ptr, h->d_q.ptr, nullptr, 0, s);
+    k_pcg_residual<<<(unsigned)h->nprob, UPD_THREADS, 0, s>>>(a, t, gref, r, h->d_pq.ptr);
+    BLS_HIP(hipGetLastError());
+    BLS_HIP(hipMemcpyAsync(relres, h->d_pq.ptr, sizeof(double) * h->nprob, hipMemcpyDeviceToHost, s));
+    BLS_HIP(hipStreamSynchronize(s));
+    return BLS_OK;
 }
 
 extern "C" int bls_pcg_relres_f64(bls_pcg *h, const double *S_nzval, const double *g, const double *x,

@@ -911,6 +911,79 @@ __global__ void k_scale_c(int64_t neb, const int32_t *e_dims, const int64_t *bof
     }
 }
 
+// The retained rows' part of the full system's residual: out[r] = (b_r .* s_r)[r] - sum over the couplings c of
+// retained block r (ascending eliminated block) of scaled(C_c)' x_e[e(c)] -- with x_e = backsubstitute!(x_r)
+// this is g_S - (S - R) x_r of the reduction; with x_e = z it is g_S itself. One workgroup per retained block;
+// thread t sums a contiguous share of the block's couplings (given as (coupling, eliminated block) pairs in the
+// retained-major order, one load per coupling before its data), and the shares are added in a fixed tree (no
+// atomics; bitwise reproducible). DE, DR > 0: every block has those dimensions and the offsets are c DE DR, e DE.
+constexpr int CTX_THREADS = 256;
+template <class TC, int DE, int DR>
+__global__ __launch_bounds__(CTX_THREADS) void k_ct_xe(const int64_t *retained_ptr, const int2 *rm_ce,
+                                                       const int32_t *e_dims, const int64_t *boffsets_e,
+                                                       const int64_t *coffsets, const int32_t *r_dims,
+                                                       const int64_t *boffsets_r, const TC *C, const double *x_e,
+                                                       const double *b_r, const double *s_r, const double *s_e,
+                                                       double *out) {
+    constexpr int NR = DR > 0 ? DR : MAXDR;
+    __shared__ double red[CTX_THREADS / 64][NR];
+    const int r = blockIdx.x, t = threadIdx.x;
+    const int dr = DR > 0 ? DR : r_dims[r];
+    const int64_t ro = boffsets_r[r];
+    const int64_t k0 = retained_ptr[r], n = retained_ptr[r + 1] - k0;
+    const int64_t per = (n + CTX_THREADS - 1) / CTX_THREADS;
+    const int64_t lo = k0 + (int64_t)t * per, hi = lo + per < k0 + n ? lo + per : k0 + n;
+    double acc[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) acc[i] = 0.0;
+    for (int64_t k = lo; k < hi; k++) {
+        const int2 ce = rm_ce[k];
+        if constexpr (DE > 0 && DR > 0) {
+            const TC *Cb = C + (int64_t)ce.x * (DE * DR);
+            const double *xp = x_e + (int64_t)ce.y * DE;
+            double cv[DE * DR], xe[DE];
+#pragma unroll
+            for (int q = 0; q < DE * DR; q++) cv[q] = (double)Cb[q];
+#pragma unroll
+            for (int i = 0; i < DE; i++) xe[i] = xp[i] * (s_e ? s_e[(int64_t)ce.y * DE + i] : 1.0);
+#pragma unroll
+            for (int ir = 0; ir < DR; ir++) {
+                double sum = 0.0;
+#pragma unroll
+                for (int i = 0; i < DE; i++) sum += cv[ir * DE + i] * xe[i];
+                acc[ir] += sum;
+            }
+        } else {
+            const int de = e_dims[ce.y];
+            const int64_t bo = boffsets_e[ce.y], co = coffsets[ce.x];
+            double xe[8];
+            for (int i = 0; i < 8; i++) xe[i] = i < de ? x_e[bo + i] * (s_e ? s_e[bo + i] : 1.0) : 0.0;
+#pragma unroll
+            for (int ir = 0; ir < NR; ir++) {
+                if (ir < dr) {
+                    double sum = 0.0;
+                    for (int i = 0; i < de; i++) sum += (double)C[co + (int64_t)ir * de + i] * xe[i];
+                    acc[ir] += sum;
+                }
+            }
+        }
+    }
+    // fixed-order sums: a butterfly within each wavefront, then the wavefronts in order
+#pragma unroll
+    for (int ir = 0; ir < NR; ir++) {
+        double v = acc[ir];
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if ((t & 63) == 0) red[t >> 6][ir] = v;
+    }
+    __syncthreads();
+    if (t < dr) {
+        double v = red[0][t];
+        for (int w = 1; w < CTX_THREADS / 64; w++) v += red[w][t];
+        const double sr = s_r ? s_r[ro + t] : 1.0;
+        out[ro + t] = b_r[ro + t] * sr - sr * v;
+    }
+}
+
 struct Run {
     int64_t first, count, de, eoff0, boff0;
 };
@@ -944,6 +1017,8 @@ struct bls_schur {
     int uniform_dr_all = 0;
     int max_dr = 0;
     DevArray<int64_t> d_slot_rowbase;
+    DevArray<int64_t> d_retained_ptr;      // retained-major index of the couplings (0-based), for k_ct_xe
+    DevArray<int2> d_rm_ce;                // ... its (coupling, eliminated block) pairs
     DevArray<unsigned long long> d_flag;
     DevArray<int64_t> d_failed_out;
     int64_t nitems = 0, nentries = 0;
@@ -1367,6 +1442,20 @@ extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, 
         delete h;
         return rc;
     }
+    {
+        std::vector<int32_t> coup_e((size_t)h->ncoup);
+        for (int64_t e = 0; e < h->neb; e++)
+            for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++) coup_e[c] = (int32_t)e;
+        std::vector<int2> ce((size_t)h->ncoup);
+        for (int64_t k = 0; k < h->ncoup; k++) {
+            const int64_t c = h->retained_entry[k];
+            ce[k] = make_int2((int)c, coup_e[c]);
+        }
+        if ((rc = h->d_retained_ptr.upload(h->retained_ptr)) || (rc = h->d_rm_ce.upload(ce))) {
+            delete h;
+            return rc;
+        }
+    }
     *out = h;
     return BLS_OK;
 }
@@ -1522,6 +1611,58 @@ extern "C" int bls_reduce_schur_sparse_f64_async(bls_schur *h, double *nzval, do
                                                  void *stream) {
     return reduce_async<PrecF64>(h, 1, nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z, s_r, s_e,
                                  d_failed_block, (hipStream_t)stream);
+}
+
+// ---- factor_all_eliminated! and the retained residual (the Float64 iterative refinement of the mixed path) -------
+extern "C" int bls_schur_factor_f64(bls_schur *h, const double *E_data, const double *b_e, const double *scaling_e,
+                                    double lambda, double *E_inv, double *z, const double *s_e, int64_t *failed_block,
+                                    void *stream) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (!(std::isfinite(lambda) && lambda >= 0))
+        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
+    if (h->neb && (!E_data || !b_e || !scaling_e || !E_inv || !z))
+        return fail(BLS_ERR_ARG, "factor_all_eliminated: a required array is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
+    int rc = phase1<PrecF64>(h, E_data, b_e, scaling_e, lambda, s_e, E_inv, z, h->d_flag.ptr, s);
+    if (rc) return rc;
+    k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, h->d_failed_out.ptr);
+    BLS_HIP(hipGetLastError());
+    int64_t fb = 0;
+    BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    BLS_HIP(hipStreamSynchronize(s));
+    if (failed_block) *failed_block = fb;
+    return fb ? BLS_FAILED : BLS_OK;
+}
+
+template <class TC>
+int retained_residual(bls_schur *h, double *out, const double *b_r, const double *x_e, const TC *C,
+                      const double *s_r, const double *s_e, hipStream_t s) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (!out || !b_r || (h->neb && (!x_e || !C))) return fail(BLS_ERR_ARG, "retained residual: a required array is NULL");
+    const bool uni = h->uniform_de == 3 && h->uniform_dr_all == 6 && h->runs.size() == 1;
+    if (h->nrb && uni)
+        k_ct_xe<TC, 3, 6><<<(unsigned)h->nrb, CTX_THREADS, 0, s>>>(h->d_retained_ptr.ptr, h->d_rm_ce.ptr, h->d_e_dims.ptr,
+                                                                   h->d_boffsets_e.ptr, h->d_coffsets.ptr, h->d_r_dims.ptr,
+                                                                   h->d_boffsets_r.ptr, C, x_e, b_r, s_r, s_e, out);
+    else if (h->nrb)
+        k_ct_xe<TC, 0, 0><<<(unsigned)h->nrb, CTX_THREADS, 0, s>>>(h->d_retained_ptr.ptr, h->d_rm_ce.ptr, h->d_e_dims.ptr,
+                                                                   h->d_boffsets_e.ptr, h->d_coffsets.ptr, h->d_r_dims.ptr,
+                                                                   h->d_boffsets_r.ptr, C, x_e, b_r, s_r, s_e, out);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+extern "C" int bls_schur_retained_residual_f64(bls_schur *h, double *out, const double *b_r, const double *x_e,
+                                               const double *C_data, const double *s_r, const double *s_e,
+                                               void *stream) {
+    return retained_residual<double>(h, out, b_r, x_e, C_data, s_r, s_e, (hipStream_t)stream);
+}
+
+extern "C" int bls_schur_retained_residual_f32(bls_schur *h, double *out, const double *b_r, const double *x_e,
+                                               const float *C_data, const double *s_r, const double *s_e,
+                                               void *stream) {
+    return retained_residual<float>(h, out, b_r, x_e, C_data, s_r, s_e, (hipStream_t)stream);
 }
 
 // ---- the reduction that allocates nothing ------------------------------------------------------------------

@@ -71,6 +71,11 @@ SIGNATURES = {
     "bls_reduced_residual_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_reduced_diagonal_blocks_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_pcg_relres_f64": (ct.c_int, [VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
+    "bls_pcg_residual_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
+    "bls_axpy_f64": (ct.c_int, [ct.c_int64, ct.c_double, VP, VP, VP]),
+    "bls_schur_factor_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, VP, VP, VP, ct.POINTER(ct.c_int64), VP]),
+    "bls_schur_retained_residual_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
+    "bls_schur_retained_residual_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_qr_set_sparse": (ct.c_int, [VP, ct.c_int64, P64, ct.c_int64, P64, P64, P64, P64, P64, P64]),
     "bls_reduced_normal_sparse_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_reduced_normal_sparse_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
@@ -368,6 +373,48 @@ def _check_backsub(st, variant, x_e, x_r, C_data, s_r, s_e):
     _require("s_e", s_e, torch.float64, st.ne, optional=True)
 
 
+def axpy_(y, alpha, x, stream=None):
+    """y += alpha x (Float64 device vectors)."""
+    import torch
+    _require("x", x, torch.float64, y.numel())
+    _check(lib().bls_axpy_f64(y.numel(), float(alpha), _dp(x), _dp(y), _stream(stream)))
+    return y
+
+
+def factor_eliminated_(handle, ws, E_data, b_e, scaling_e, lam, s_e=None, stream=None, check=True):
+    """factor_all_eliminated! (src/schur_reduction.jl:439-455): E_inv and z of every eliminated block into the
+    Float64 workspace ws; returns True, or False with ws.failed_block set. check=False skips the host-side
+    check_damping of scaling_e (a device reduction and a synchronization) for a caller that validated it."""
+    import torch
+    if check:
+        _check_damping(lam, scaling_e, "scaling_e")
+    st = handle.st
+    if ws.variant != "f64":
+        raise BLSError("factor_eliminated_ is Float64 only")
+    _require("E_data", E_data, torch.float64, st.eliminated_storage_length())
+    _require("b_e", b_e, torch.float64, st.ne)
+    fb = ct.c_int64(0)
+    rc = _check(lib().bls_schur_factor_f64(handle.h, _dp(E_data), _dp(b_e), _dp(scaling_e), float(lam),
+                                           _dp(ws.E_inv_data), _dp(ws.z_data), _dp(s_e), ct.byref(fb), _stream(stream)))
+    ws.failed_block = int(fb.value)
+    return rc == BLS_OK
+
+
+def retained_residual_(out, handle, b_r, x_e, C_data, s_r=None, s_e=None, stream=None):
+    """out = b_r .* s_r - sum_c scaled(C_c)' x_e per retained block (fixed-order sums): g_S when x_e = z, and
+    g_S - (S_red - R_damped) x_r when x_e = backsubstitute!(x_r)."""
+    import torch
+    st = handle.st
+    _require("out", out, torch.float64, st.nr)
+    _require("b_r", b_r, torch.float64, st.nr)
+    _require("x_e", x_e, torch.float64, st.ne)
+    if C_data.dtype not in (torch.float64, torch.float32) or C_data.numel() != st.coupling_storage_length():
+        raise BLSError("C_data must be Float64 or Float32 of the structure's coupling storage length")
+    f = lib().bls_schur_retained_residual_f64 if C_data.dtype == torch.float64 else lib().bls_schur_retained_residual_f32
+    _check(f(handle.h, _dp(out), _dp(b_r), _dp(x_e), _dp(C_data), _dp(s_r), _dp(s_e), _stream(stream)))
+    return out
+
+
 def reduce_schur_sparse_noalloc_(nzval, g_S, b_r, b_e, scaling_e, lam, handle, E_data, C_data, ws, stream=None):
     """The Float64 sparse reduction through bls_reduce_schur_sparse_f64_noalloc: the entry point that allocates no
     device memory (its scratch is the library's static array; no conditioning scales). The status is left on the
@@ -525,6 +572,13 @@ class PCGHandle:
         _check(lib().bls_pcg_relres_f64(self.h, _dp(S_nzval), _dp(g), _dp(x), out, _stream(stream)))
         return np.array(out[:])
 
+    def residual(self, S_nzval, t, x, gref, r, stream=None):
+        """r = t - S x on the device, and ||r|| / ||gref|| per problem as a numpy array."""
+        out = (ct.c_double * self.nprob)()
+        _check(lib().bls_pcg_residual_f64(self.h, _dp(S_nzval), _dp(t), _dp(x), _dp(gref), _dp(r), out,
+                                          _stream(stream)))
+        return np.array(out[:])
+
     def solve(self, S_nzval, g, x, tol=1e-8, maxit=10000, precond_blocks=None, stream=None):
         """x = S^-1 g per problem to ||S x - g|| <= tol ||g||. Returns (all_converged, iters[nprob], relres[nprob])."""
         iters = np.zeros(self.nprob, dtype=np.int64)
@@ -532,6 +586,7 @@ class PCGHandle:
         rc = _check(lib().bls_pcg_solve_f64(self.h, _dp(S_nzval), _dp(g), _dp(x), float(tol), int(maxit),
                                             _dp(precond_blocks), iters.ctypes.data_as(P64),
                                             rel.ctypes.data_as(ct.POINTER(ct.c_double)), _stream(stream)))
+        self.last_iters = iters
         return rc == BLS_OK, iters, rel
 
 

@@ -101,6 +101,19 @@ int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const
                                      const double *scaling_e, double lambda, const double *E_data,
                                      const double *C_data, double *E_inv, double *z, const double *s_r,
                                      const double *s_e, int64_t *d_failed_block, void *stream);
+/* factor_all_eliminated! (src/schur_reduction.jl:439-455): phase 1 alone -- E_inv and z of every eliminated
+ * block in Float64, the lowest failing block (1-based) in *failed_block and BLS_FAILED if one fails; synchronizes. */
+int bls_schur_factor_f64(bls_schur *h, const double *E_data, const double *b_e, const double *scaling_e, double lambda,
+                         double *E_inv, double *z, const double *s_e, int64_t *failed_block, void *stream);
+/* The retained rows of the full system's residual after a back-substitution: out = b_r .* s_r - sum_c scaled(C_c)'
+ * x_e[e(c)] per retained block, its couplings in ascending eliminated block (fixed-order sums, no atomics). With
+ * x_e = z it is g_S of the reduction; with x_e = backsubstitute!(x_r) it is g_S - (S_red - R_damped) x_r. No
+ * reference counterpart (the Float64 iterative refinement of the mixed reduction); C in Float64 or Float32. */
+int bls_schur_retained_residual_f64(bls_schur *h, double *out, const double *b_r, const double *x_e,
+                                    const double *C_data, const double *s_r, const double *s_e, void *stream);
+int bls_schur_retained_residual_f32(bls_schur *h, double *out, const double *b_r, const double *x_e,
+                                    const float *C_data, const double *s_r, const double *s_e, void *stream);
+
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops): bls_reduce_schur_sparse_f64_async without conditioning scales (s_r = s_e =
  * UnitScale), whose scratch -- the failure flag and the row sums of split items -- is a static array of the
@@ -335,6 +348,14 @@ int bls_pcg_values_size(bls_pcg *h, int64_t *nvals, int64_t *nfactor);
  * definite) and relres[p] (host arrays of nprob). BLS_OK when every problem converged, else BLS_FAILED. */
 int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol, int64_t maxit,
                       const double *precond_blocks, int64_t *iters, double *relres, void *stream);
+/* y += alpha x (device vectors of n doubles): the update of an iterative refinement step. */
+int bls_axpy_f64(int64_t n, double alpha, const double *x, double *y, void *stream);
+/* r = t - S x, and per problem ||r|| / ||gref|| into the host array relres (nprob values); synchronizes. With S the
+ * reset S (R + damping, bls_reset_schur_sparse_f64), t = bls_schur_retained_residual_f64(x_e = backsubstitute!(x))
+ * and gref = g_S, r is the reduced system's residual g_S - S_red x evaluated from the blocks, without S_red. No
+ * reference counterpart (the Float64 iterative refinement of the mixed reduction). */
+int bls_pcg_residual_f64(bls_pcg *h, const double *S_nzval, const double *t, const double *x, const double *gref,
+                         double *r, double *relres, void *stream);
 /* The true residual ||S x - g|| / ||g|| of each problem for a given x (device arrays; relres: host, nprob), e.g.
  * of one system's solution against another system of the same pattern. Synchronizes `stream`. */
 int bls_pcg_relres_f64(bls_pcg *h, const double *S_nzval, const double *g, const double *x, double *relres,
