@@ -53,15 +53,19 @@ def main():
     ap.add_argument("--calibration-copy", action="store_true")
     ap.add_argument("--qr-scale", type=float, default=0.2,
                     help="landmark scale of the reduce_qr! leg (500 cameras d 6/9, 500000 landmarks x scale); 0 skips")
+    ap.add_argument("--config3-scale", type=float, default=1.0,
+                    help="landmark scale of the dense leg (256 cameras, 1000000 landmarks x scale); 0 skips it")
     ap.add_argument("--config5-scale", type=float, default=1.0,
                     help="landmark scale of the batched 64-problem leg (cameras at full size); 0 skips it")
     ap.add_argument("--tuning", default="",
                     help="PART,LANES: use this phase-2 work-list shape instead of autotuning (profiling runs)")
     ap.add_argument("--slam-only", action="store_true",
                     help="time only the sequential-SLAM step (no other legs, no CPU baseline): for profiler runs")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="run only the dense leg (no SLAM line, no CPU baseline): for profiler runs of its kernels")
     args = ap.parse_args()
     if args.slam_only:
-        args.no_cpu, args.qr_scale, args.config5_scale = True, 0.0, 0.0
+        args.no_cpu, args.qr_scale, args.config5_scale, args.config3_scale = True, 0.0, 0.0, 0.0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no multi-GPU path; "
               "see the sharding design in DESIGN.md", file=sys.stderr)
@@ -73,6 +77,10 @@ def main():
     b = blsload.load()
     dev = b.device
     torch.cuda.set_device(0)
+    if args.config3_only:
+        print(json.dumps({"dense_reduction": config3_leg(torch, dev, b, args.config3_scale or 1.0, cpu=False)}),
+              flush=True)
+        return
     if args.calibration_copy:
         src = torch.ones(256 << 17, dtype=torch.float64, device="cuda")   # 256 MiB
         dst = torch.empty_like(src)
@@ -222,6 +230,9 @@ def main():
     if args.qr_scale > 0:
         result["reduce_qr"] = qr_leg(torch, dev, b, args.qr_scale)
         torch.cuda.empty_cache()
+    if args.config3_scale > 0:
+        result["dense_reduction"] = config3_leg(torch, dev, b, args.config3_scale, cpu=not args.no_cpu)
+        torch.cuda.empty_cache()
     if args.config5_scale > 0:
         result["batched_pcg"] = batched_pcg_leg(torch, dev, b, args.config5_scale, check_parity=not args.no_cpu)
     if not args.no_cpu:
@@ -274,6 +285,154 @@ def parity_record(st, pattern, S_dev, g_dev, S_ref, g_ref):
             "g_rel": float(np.linalg.norm(g_dev - g_ref) / np.linalg.norm(g_ref)), "tol": PARITY_TOL,
             "ok": bool(s_rel <= PARITY_TOL and g_rel <= PARITY_TOL),
             "against": "the oracle's Float64 reset + reduce of the same instance"}
+
+
+FP32_MFMA_PEAK_TF = 157.3
+
+
+def config3_leg(torch, dev, b, scale, valu_scale=0.1, cpu=True, reps=3):
+    """The dense configuration: 256 cameras (d 6), 1000000 x scale landmarks (d 3), k ~ U{2..64} random cameras,
+    Jacobian blocks scaled per camera by 10^U(-2, 2), lambda 1e-4, dense reduce_schur_dense!.
+    - The SYRK on the matrix cores (phase 1 in Float64, S -= Y'Y with Y_e = chol(E_inv_e)' C_e): Float64 C and FP64
+      MFMA, and Float32 couplings with FP32 MFMA and Float64 cross-slice accumulation; their dense product's
+      MFMA rate against the peaks, and the useful rate (108 k^2 + 162 k + 68 flop per landmark).
+    - The mixed S against the Float64 one, and the dense solve through the mixed S's Cholesky factor with one
+      Float64 refinement step (its residual from the Float64 blocks), judged against the Float64 system.
+    - The owner-computes VALU kernel (the default dense reduction) at valu_scale of the landmarks (its host work
+      list grows with landmarks x retained blocks), with the MFMA path at the same size, and the CPU oracle there."""
+    t0 = time.time()
+    p = b.generators.config3(scale=scale)
+    st = p.structure
+    log(f"dense leg setup: {st.neblocks} landmarks, {st.nrblocks} cameras, S {st.nr} x {st.nr} ({time.time() - t0:.1f}s)")
+    T = lambda a, d=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(d).cuda()
+    k = np.diff(st.eliminated_ptr).astype(np.float64)
+    useful = float(np.sum(108 * k * k + 162 * k + 68))
+    nr = st.nr
+    h = dev.SchurHandle(st, p.ws_index, dense=True)
+    ws = dev.SchurReductionWorkspace(h)
+    Rv, sr, br, be, se, E = T(p.R.T.reshape(-1)), T(p.scaling_r), T(p.b_r), T(p.b_e), T(p.scaling_e), T(p.E_data)
+    out = {"landmarks": int(st.neblocks), "cameras": int(st.nrblocks), "useful_flop": useful}
+    Ss, gs = {}, {}
+    ntiles = (nr + 127) // 128
+    dense_flop = 2.0 * st.ne * (ntiles * (ntiles + 1) // 2) * 128 * 128
+    for name, dt, peak in (("mfma_f64", torch.float64, FP64_MFMA_PEAK_TF), ("mfma_mixed", torch.float32, FP32_MFMA_PEAK_TF)):
+        C = T(p.C_data, dt)
+        S = torch.empty(nr * nr, dtype=torch.float64, device="cuda")
+        g = torch.empty(nr, dtype=torch.float64, device="cuda")
+        ts = []
+        for rep_ in range(reps + 1):
+            dev.reset_schur_dense_(S, Rv, h, sr, p.lam, check=False)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            assert dev.reduce_schur_dense_mfma_(S, g, br, be, se, p.lam, h, E, C, ws, check=False)
+            ts.append(time.perf_counter() - t1)
+        h.set_timing(True)
+        dev.reset_schur_dense_(S, Rv, h, sr, p.lam, check=False)
+        dev.reduce_schur_dense_mfma_(S, g, br, be, se, p.lam, h, E, C, ws, check=False)
+        kt = h.kernel_times()
+        h.set_timing(False)
+        ms = float(np.median(ts[1:])) * 1e3
+        mfma_tf = dense_flop / (kt[2] * 1e-3) / 1e12
+        out[name] = {"ms": ms, "landmarks_per_s": st.neblocks / (ms * 1e-3), "useful_TFLOPs": useful / (ms * 1e-3) / 1e12,
+                     "kernel_ms": {"ybuild": kt[1], "syrk": kt[2], "slice_sum": kt[3]},
+                     "roofline": {"bound": "mfma", "kernel": "k_syrk", "achieved": mfma_tf, "peak": peak,
+                                  "unit": "TFLOP/s", "frac": mfma_tf / peak,
+                                  "dense_flop": dense_flop, "useful_fraction_of_dense": useful / dense_flop}}
+        log(f"dense leg, {name}: {ms:.2f} ms ({st.neblocks / (ms * 1e-3):.4g} landmarks/s, {useful / (ms * 1e-3) / 1e12:.2f} "
+            f"useful TFLOP/s); SYRK {kt[2]:.2f} ms = {mfma_tf:.1f} TFLOP/s of the dense product ({100 * mfma_tf / peak:.0f}% "
+            f"of {peak}), Y build {kt[1]:.2f} ms")
+        Ss[name], gs[name] = S, g
+        del C
+    S64, g64, Sm, gm = Ss["mfma_f64"], gs["mfma_f64"], Ss["mfma_mixed"], gs["mfma_mixed"]
+    A64 = S64.view(nr, nr).T.triu()
+    Am = Sm.view(nr, nr).T.triu()
+    out["mfma_mixed"]["rel_diff_S_vs_f64"] = float((Am - A64).norm() / A64.norm())
+    out["mfma_mixed"]["rel_diff_g_vs_f64"] = float((gm - g64).norm() / g64.norm())
+    # the dense solve through the mixed S with one Float64 refinement step, against the Float64 system
+    R = b.refine
+    SR = torch.empty_like(S64)
+    dev.reset_schur_dense_(SR, Rv, h, sr, p.lam, check=False)
+    rd = R.RefinementData(h, E, T(p.C_data), br, be, se, SR, p.lam)
+    chol = dev.DenseCholesky(nr)
+    x = torch.empty(nr, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(x)
+
+    def relres_f64(xv):
+        dev.dense_residual_(r, S64, g64, xv)
+        return float(r.norm() / g64.norm())
+
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    assert R.factor(rd, check=False)
+    ok, _ = chol.factor(Sm)
+    assert ok
+    R.refined_dense_solve(chol, gm, x, rd, steps=1)
+    torch.cuda.synchronize()
+    solve_ms = (time.perf_counter() - t1) * 1e3
+    refined = relres_f64(x)
+    R.refined_dense_solve(chol, gm, x, rd, steps=0)
+    unrefined = relres_f64(x)
+    out["mfma_mixed"]["refined_solve"] = {"ms": solve_ms, "relres_vs_f64_system_unrefined": unrefined,
+                                          "relres_vs_f64_system_one_step": refined,
+                                          "time_to_refined_solution_ms": out["mfma_mixed"]["ms"] + solve_ms}
+    log(f"dense leg, mixed: ||S_mixed - S_f64|| / ||S_f64|| = {out['mfma_mixed']['rel_diff_S_vs_f64']:.2e}; the solve "
+        f"through its Cholesky factor: residual against the Float64 system {unrefined:.2e}, {refined:.2e} after one "
+        f"Float64 refinement step ({solve_ms:.2f} ms incl. the Float64 factorisation)")
+    del Ss, gs, S64, Sm, A64, Am, SR, rd, E, h, ws
+    torch.cuda.empty_cache()
+    # the VALU owner-computes kernel, the MFMA path and the CPU oracle at valu_scale of the landmarks
+    q = b.generators.config3(scale=scale * valu_scale)
+    sq = q.structure
+    t0 = time.time()
+    hq = dev.SchurHandle(sq, q.ws_index, dense=True)
+    plan_s = time.time() - t0
+    wq = dev.SchurReductionWorkspace(hq)
+    Eq, Cq, Rq = T(q.E_data), T(q.C_data), T(q.R.T.reshape(-1))
+    brq, beq, srq, seq = T(q.b_r), T(q.b_e), T(q.scaling_r), T(q.scaling_e)
+    Sq = torch.empty(sq.nr * sq.nr, dtype=torch.float64, device="cuda")
+    gq = torch.empty(sq.nr, dtype=torch.float64, device="cuda")
+    kq = np.diff(sq.eliminated_ptr).astype(np.float64)
+    useful_q = float(np.sum(108 * kq * kq + 162 * kq + 68))
+    small = {"landmarks": int(sq.neblocks), "valu_work_list_s": plan_s}
+    for name, fn in (("valu_f64", lambda: dev.reduce_schur_dense_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq, check=False)),
+                     ("mfma_f64", lambda: dev.reduce_schur_dense_mfma_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq, check=False))):
+        ts = []
+        for rep_ in range(reps + 1):
+            dev.reset_schur_dense_(Sq, Rq, hq, srq, q.lam, check=False)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            assert fn()
+            ts.append(time.perf_counter() - t1)
+        ms = float(np.median(ts[1:])) * 1e3
+        small[name] = {"ms": ms, "landmarks_per_s": sq.neblocks / (ms * 1e-3), "useful_TFLOPs": useful_q / (ms * 1e-3) / 1e12}
+        log(f"dense leg at {sq.neblocks} landmarks, {name}: {ms:.3f} ms ({sq.neblocks / (ms * 1e-3):.4g} landmarks/s)")
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = cpu_share()[0]
+        args = oracle.SchurArgs(sq, b_ws(q, threads), None)
+        So, go = np.zeros(sq.nr * sq.nr), np.zeros(sq.nr)
+        E_inv, z, W = np.zeros(len(q.E_data)), np.zeros(sq.ne), np.zeros(len(q.C_data))
+        Rcm = np.ascontiguousarray(q.R.T.reshape(-1))
+        t1 = time.perf_counter()
+        oracle.reset_schur_dense(So, Rcm, q.scaling_r, q.lam)
+        okc, _ = oracle.reduce_schur(False, args, So, go, q.b_r, q.b_e, q.scaling_e, q.lam, q.E_data, q.C_data, E_inv, z,
+                                     W, nthreads=threads)
+        cms = (time.perf_counter() - t1) * 1e3
+        assert okc
+        dev.reset_schur_dense_(Sq, Rq, hq, srq, q.lam, check=False)
+        assert dev.reduce_schur_dense_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq, check=False)
+        A = Sq.cpu().numpy().reshape(sq.nr, sq.nr).T
+        B = So.reshape(sq.nr, sq.nr).T
+        iu = np.triu_indices(sq.nr)
+        small["cpu_baseline"] = {"ms": cms, "landmarks_per_s": sq.neblocks / (cms * 1e-3), "threads": threads,
+                                 "kind": "port", "sample": f"reset + reduce of the {sq.neblocks}-landmark instance"}
+        small["parity_valu_vs_oracle"] = {"rel_S": float(np.linalg.norm(A[iu] - B[iu]) / np.linalg.norm(B[iu])),
+                                          "rel_g": float(np.linalg.norm(gq.cpu().numpy() - go) / np.linalg.norm(go))}
+        log(f"dense leg at {sq.neblocks} landmarks, CPU oracle ({threads} threads): {cms:.1f} ms; device S within "
+            f"{small['parity_valu_vs_oracle']['rel_S']:.2e} of it (norm-wise)")
+    out["reduced_size"] = small
+    return out
 
 
 def time_ms(torch, fn, reps=5):

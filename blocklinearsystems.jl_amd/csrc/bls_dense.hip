@@ -418,6 +418,29 @@ int chol_wait(bls_chol *h, int64_t *failed_col, void *stream) {
 }
 }  // namespace
 
+namespace {
+// r = t - S x for a symmetric S of order n stored by its upper triangle (column-major): one wavefront per row a,
+// S[a, b] read from column max(a, b); the row's products summed by lanes in a fixed butterfly.
+__global__ __launch_bounds__(256) void k_dense_residual(int64_t n, const double *S, const double *t, const double *x,
+                                                        double *r) {
+    const int64_t a = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (a >= n) return;
+    double s = 0.0;
+    for (int64_t b = lane; b < n; b += 64) s += (a <= b ? S[a + b * n] : S[b + a * n]) * x[b];
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) r[a] = t[a] - s;
+}
+}  // namespace
+
+extern "C" int bls_dense_residual_f64(int64_t n, const double *S, const double *t, const double *x, double *r,
+                                      void *stream) {
+    if (n < 0 || (n && (!S || !t || !x || !r))) return fail(BLS_ERR_ARG, "bls_dense_residual_f64: bad arguments");
+    if (n) k_dense_residual<<<(unsigned)((n + 3) / 4), 256, 0, (hipStream_t)stream>>>(n, S, t, x, r);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
 extern "C" int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void *stream) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (!h->factored) return fail(BLS_ERR_STATE, "bls_chol_solve_f64: no factorization (bls_chol_factor_f64)");

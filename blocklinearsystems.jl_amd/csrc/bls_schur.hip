@@ -984,6 +984,158 @@ __global__ __launch_bounds__(CTX_THREADS) void k_ct_xe(const int64_t *retained_p
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------------------
+// reduce_schur_dense! as a SYRK on the matrix cores. With E_inv_e = M M' (M lower, Cholesky of the landmark's
+// inverse), C_e' E_inv_e C_e = Y_e' Y_e for Y_e = M' C_e (d_e rows, the landmark's retained columns), so the
+// phase-2 update of the dense S is S -= Y' Y over the ne x nr matrix Y whose rows are the landmarks' Y_e. Y is
+// built (zeros outside each landmark's retained blocks), the upper 128 x 128 tiles of Y' Y are formed on MFMA
+// in K slices (rows of Y) whose partial tiles accumulate in T registers, and the slices are added in Float64, in
+// slice order, into S (no atomics; bitwise reproducible).
+// ---------------------------------------------------------------------------------------------------------
+typedef float sy_v4f __attribute__((ext_vector_type(4)));
+typedef double sy_v4d __attribute__((ext_vector_type(4)));
+constexpr int SY_T = 128;    // output tile edge
+constexpr int SY_KT = 16;    // rows of Y per LDS stage
+
+template <class T> struct SyMfma;
+template <> struct SyMfma<float> {
+    using acc_t = sy_v4f;
+    static __device__ inline acc_t step(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // v_mfma_f32_16x16x4f32: lane l holds D[4 (l / 16) + v][l % 16]
+    static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <> struct SyMfma<double> {
+    using acc_t = sy_v4d;
+    static __device__ inline acc_t step(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // v_mfma_f64_16x16x4f64: lane l holds D[l / 16 + 4 v][l % 16]
+    static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+// One thread per landmark: M = chol(E_inv_e) (Float64), then Y[bo_e + i][col] = (M' C_e)[i][col] for the columns
+// of each coupled retained block, rounded to TY.
+template <class TC, class TY>
+__global__ void k_ybuild(int64_t neb, const int32_t *e_dims, const int64_t *eoffsets, const int64_t *boffsets_e,
+                         const int64_t *eliminated_ptr, const int32_t *entry_retained, const int64_t *coffsets,
+                         const int32_t *r_dims, const int64_t *boffsets_r, const double *E_inv, const TC *C,
+                         int64_t nr, TY *Y) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= neb) return;
+    const int de = e_dims[e];
+    const double *Ei = E_inv + eoffsets[e];
+    double M[8][8];
+    for (int j = 0; j < de; j++) {
+        for (int i = j; i < de; i++) {
+            double v = Ei[i + j * de];
+            for (int k = 0; k < j; k++) v -= M[i][k] * M[j][k];
+            if (i == j) M[j][j] = v > 0 ? sqrt(v) : 0.0;
+            else M[i][j] = M[j][j] > 0 ? v / M[j][j] : 0.0;
+        }
+    }
+    const int64_t row0 = boffsets_e[e];
+    for (int64_t c = eliminated_ptr[e]; c < eliminated_ptr[e + 1]; c++) {
+        const int r = entry_retained[c];
+        const int dr = r_dims[r];
+        const int64_t ro = boffsets_r[r], co = coffsets[c];
+        for (int ir = 0; ir < dr; ir++) {
+            const TC *col = C + co + (int64_t)ir * de;
+            for (int i = 0; i < de; i++) {
+                double v = 0.0;
+                for (int j = i; j < de; j++) v += M[j][i] * (double)col[j];
+                Y[(row0 + i) * nr + ro + ir] = (TY)v;
+            }
+        }
+    }
+}
+
+// Partial tile (p, t) = sum over rows k of slice p of Y[k, I]' Y[k, J] for the upper tile t = (I, J) of S; four
+// wavefronts, each a 64 x 64 quarter in 4 x 4 MFMA 16 x 16 x 4 tiles (a diagonal tile's lower-left quarter is
+// skipped); Y's rows staged through LDS 16 at a time, the next stage's loads in registers meanwhile.
+template <class T>
+__global__ __launch_bounds__(256) void k_syrk(const T *Y, int64_t nr, int64_t K, int64_t kc, const int2 *tiles,
+                                              int ntiles, T *partial) {
+    using M = SyMfma<T>;
+    __shared__ T As[SY_KT][SY_T + 4], Bs[SY_KT][SY_T + 4];
+    const int t = blockIdx.x, p = blockIdx.y;
+    const int2 tl = tiles[t];
+    const int64_t I0 = (int64_t)tl.x * SY_T, J0 = (int64_t)tl.y * SY_T;
+    const bool diag = tl.x == tl.y;
+    const int64_t k_lo = (int64_t)p * kc, k_hi = k_lo + kc < K ? k_lo + kc : K;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int qi = wave & 1, qj = wave >> 1;
+    const bool active = !(diag && qi > qj);
+    typename M::acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
+    const int lr = tid >> 4, lc = (tid & 15) * 8;
+    T pa[8], pb[8];
+    auto load = [&](int64_t k0) {
+        const int64_t k = k0 + lr;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int64_t ca = I0 + lc + q, cb = J0 + lc + q;
+            pa[q] = (k < k_hi && ca < nr) ? Y[k * nr + ca] : (T)0;
+            pb[q] = (k < k_hi && cb < nr) ? Y[k * nr + cb] : (T)0;
+        }
+    };
+    if (k_lo < k_hi) load(k_lo);
+    for (int64_t k0 = k_lo; k0 < k_hi; k0 += SY_KT) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            As[lr][lc + q] = pa[q];
+            Bs[lr][lc + q] = pb[q];
+        }
+        __syncthreads();
+        if (k0 + SY_KT < k_hi) load(k0 + SY_KT);
+        if (active) {
+#pragma unroll
+            for (int kk = 0; kk < SY_KT / 4; kk++) {
+                T a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    a[i] = As[4 * kk + lk][64 * qi + 16 * i + li];
+                    b[i] = Bs[4 * kk + lk][64 * qj + 16 * i + li];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] = M::step(a[i], b[j], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+    T *P = partial + ((int64_t)p * ntiles + t) * (SY_T * SY_T);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 64 * qi + 16 * i + M::row(lane, v), col = 64 * qj + 16 * j + li;
+                P[row * SY_T + col] = acc[i][j][v];
+            }
+}
+
+// S[a, b] -= sum over slices p (in order, Float64) of partial[p][t][a - I0][b - J0], for a <= b.
+template <class T>
+__global__ void k_syrk_sum(const T *partial, int nslices, int ntiles, const int2 *tiles, int64_t nr, double *S) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)ntiles * SY_T * SY_T) return;
+    const int t = (int)(idx / (SY_T * SY_T));
+    const int el = (int)(idx % (SY_T * SY_T)), row = el / SY_T, col = el % SY_T;
+    const int64_t a = (int64_t)tiles[t].x * SY_T + row, b = (int64_t)tiles[t].y * SY_T + col;
+    if (a >= nr || b >= nr || a > b) return;
+    double s = 0.0;
+    for (int p = 0; p < nslices; p++) s += (double)partial[((int64_t)p * ntiles + t) * (SY_T * SY_T) + el];
+    S[a + b * nr] -= s;
+}
+
 struct Run {
     int64_t first, count, de, eoff0, boff0;
 };
@@ -1019,6 +1171,11 @@ struct bls_schur {
     DevArray<int64_t> d_slot_rowbase;
     DevArray<int64_t> d_retained_ptr;      // retained-major index of the couplings (0-based), for k_ct_xe
     DevArray<int2> d_rm_ce;                // ... its (coupling, eliminated block) pairs
+    // the dense reduction by MFMA SYRK: Y (ne x nr, Float32 or Float64 as bytes), the upper tiles of S, the
+    // split-K partial tiles
+    DevArray<double> d_sy_Y, d_sy_partial;
+    DevArray<int2> d_sy_tiles;
+    int64_t sy_ntiles = 0;
     DevArray<unsigned long long> d_flag;
     DevArray<int64_t> d_failed_out;
     int64_t nitems = 0, nentries = 0;
@@ -1663,6 +1820,97 @@ extern "C" int bls_schur_retained_residual_f32(bls_schur *h, double *out, const 
                                                const float *C_data, const double *s_r, const double *s_e,
                                                void *stream) {
     return retained_residual<float>(h, out, b_r, x_e, C_data, s_r, s_e, (hipStream_t)stream);
+}
+
+
+// ---- reduce_schur_dense! on the matrix cores (the dense configuration's SYRK) ------------------------------------
+namespace {
+// The landmark blocks E, their factorisation (E_inv, z) and S are Float64; the couplings C are TC and Y is TY.
+template <class TC, class TY>
+int reduce_dense_mfma(bls_schur *h, double *S, double *g, const double *b_r, const double *b_e, const double *sc,
+                      double lambda, const double *E, const TC *C, double *E_inv, double *z,
+                      int64_t *failed_block, int64_t nslices_req, hipStream_t s) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (h->mode != 2) return fail(BLS_ERR_STATE, "handle is not dense (bls_schur_set_dense)");
+    if (!(std::isfinite(lambda) && lambda >= 0))
+        return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
+    if (!S || !g || !b_r || !sc || !E || !C || !E_inv || !z || (h->neb && !b_e))
+        return fail(BLS_ERR_ARG, "reduce_schur_dense (MFMA): a required array is NULL");
+    for (int64_t e = 0; e < h->neb; e++)
+        if (h->e_dims[e] > 8) return fail(BLS_ERR_UNSUPPORTED, "MFMA dense reduction: eliminated blocks up to 8");
+    // phase 1 (the reference's landmark factorisation and acceptance), then its verdict
+    k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
+    int rc = phase1<PrecF64>(h, E, b_e, sc, lambda, nullptr, E_inv, z, h->d_flag.ptr, s);
+    if (rc) return rc;
+    k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, h->d_failed_out.ptr);
+    int64_t fb = 0;
+    BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    BLS_HIP(hipStreamSynchronize(s));
+    if (failed_block) *failed_block = fb;
+    if (fb) return BLS_FAILED;
+    const int64_t nr = h->nr, K = h->neb ? h->boffsets_e[h->neb - 1] + h->e_dims[h->neb - 1] : 0;   // rows of Y
+    // the upper tiles of S and the K slices: enough workgroups to fill the device several times over
+    const int nt1 = (int)((nr + SY_T - 1) / SY_T);
+    if (h->sy_ntiles == 0 && nt1 > 0) {
+        std::vector<int2> tiles;
+        for (int j = 0; j < nt1; j++)
+            for (int i = 0; i <= j; i++) tiles.push_back(make_int2(i, j));
+        if ((rc = h->d_sy_tiles.upload(tiles))) return rc;
+        h->sy_ntiles = (int64_t)tiles.size();
+    }
+    const int ntiles = (int)h->sy_ntiles;
+    int64_t nslices = nslices_req > 0 ? nslices_req : std::max<int64_t>(1, (2048 + ntiles - 1) / std::max(ntiles, 1));
+    nslices = std::min<int64_t>(nslices, std::max<int64_t>(1, (K + SY_KT - 1) / SY_KT));
+    int64_t kc = (K + nslices - 1) / nslices;
+    kc = (kc + SY_KT - 1) / SY_KT * SY_KT;
+    nslices = kc > 0 ? (K + kc - 1) / kc : 1;
+    const size_t ybytes = (size_t)K * (size_t)nr * sizeof(TY);
+    const size_t pbytes = (size_t)nslices * ntiles * SY_T * SY_T * sizeof(TY);
+    if (h->d_sy_Y.n * sizeof(double) < ybytes && (rc = h->d_sy_Y.alloc((ybytes + 7) / 8))) return rc;
+    if (h->d_sy_partial.n * sizeof(double) < pbytes && (rc = h->d_sy_partial.alloc((pbytes + 7) / 8))) return rc;
+    TY *Y = reinterpret_cast<TY *>(h->d_sy_Y.ptr);
+    TY *part = reinterpret_cast<TY *>(h->d_sy_partial.ptr);
+    h->timing.mark(1, s);
+    if (K > 0 && nr > 0) {
+        k_fill<TY><<<std::min<int64_t>(grid_for(K * nr, 256), 16384), 256, 0, s>>>(Y, K * nr, (TY)0);
+        k_ybuild<TC, TY><<<grid_for(h->neb, 128), 128, 0, s>>>(
+            h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr, h->d_boffsets_e.ptr, h->d_eliminated_ptr.ptr,
+            h->d_entry_retained.ptr, h->d_coffsets.ptr, h->d_r_dims.ptr, h->d_boffsets_r.ptr, E_inv, C, nr, Y);
+        h->timing.mark(2, s);
+        k_syrk<TY><<<dim3((unsigned)ntiles, (unsigned)nslices), 256, 0, s>>>(Y, nr, K, kc, h->d_sy_tiles.ptr, ntiles, part);
+        h->timing.mark(3, s);
+        k_syrk_sum<TY><<<grid_for((int64_t)ntiles * SY_T * SY_T, 256), 256, 0, s>>>(part, (int)nslices, ntiles,
+                                                                                     h->d_sy_tiles.ptr, nr, S);
+    } else {
+        h->timing.mark(2, s);
+        h->timing.mark(3, s);
+    }
+    h->timing.mark(4, s);
+    // g_S = b_r - sum_c C_c' z_e (the retained residual with x_e = z), Float64
+    rc = retained_residual<TC>(h, g, b_r, z, C, nullptr, nullptr, s);
+    if (rc) return rc;
+    BLS_HIP(hipGetLastError());
+    BLS_HIP(hipStreamSynchronize(s));
+    if (h->timing.enabled)
+        for (int i = 1; i < 4; i++) (void)hipEventElapsedTime(&h->timing.ms[i], h->timing.ev[i], h->timing.ev[i + 1]);
+    return BLS_OK;
+}
+}  // namespace
+
+extern "C" int bls_reduce_schur_dense_mfma_f64(bls_schur *h, double *S, double *g_S, const double *b_r,
+                                               const double *b_e, const double *scaling_e, double lambda,
+                                               const double *E_data, const double *C_data, double *E_inv, double *z,
+                                               int64_t nslices, int64_t *failed_block, void *stream) {
+    return reduce_dense_mfma<double, double>(h, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z,
+                                             failed_block, nslices, (hipStream_t)stream);
+}
+
+extern "C" int bls_reduce_schur_dense_mfma_mixed(bls_schur *h, double *S, double *g_S, const double *b_r,
+                                                 const double *b_e, const double *scaling_e, double lambda,
+                                                 const double *E_data, const float *C_data, double *E_inv, double *z,
+                                                 int64_t nslices, int64_t *failed_block, void *stream) {
+    return reduce_dense_mfma<float, float>(h, S, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data, E_inv, z,
+                                           failed_block, nslices, (hipStream_t)stream);
 }
 
 // ---- the reduction that allocates nothing ------------------------------------------------------------------

@@ -73,6 +73,11 @@ SIGNATURES = {
     "bls_pcg_relres_f64": (ct.c_int, [VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
     "bls_pcg_residual_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
     "bls_axpy_f64": (ct.c_int, [ct.c_int64, ct.c_double, VP, VP, VP]),
+    "bls_dense_residual_f64": (ct.c_int, [ct.c_int64, VP, VP, VP, VP, VP]),
+    "bls_reduce_schur_dense_mfma_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, ct.c_int64,
+                                                   ct.POINTER(ct.c_int64), VP]),
+    "bls_reduce_schur_dense_mfma_mixed": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, ct.c_int64,
+                                                     ct.POINTER(ct.c_int64), VP]),
     "bls_schur_factor_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, VP, VP, VP, ct.POINTER(ct.c_int64), VP]),
     "bls_schur_retained_residual_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
     "bls_schur_retained_residual_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
@@ -328,6 +333,35 @@ def reduce_schur_dense_(S, g_S, b_r, b_e, scaling_e, lam, handle, E_data, C_data
     if check:
         _check_damping(lam, scaling_e, "scaling_e")
     return _reduce(handle, True, S, g_S, b_r, b_e, scaling_e, lam, E_data, C_data, ws, s_r, s_e, stream, sync)
+
+
+def reduce_schur_dense_mfma_(S, g_S, b_r, b_e, scaling_e, lam, handle, E_data, C_data, ws, nslices=0, stream=None,
+                             check=True):
+    """reduce_schur_dense! with phase 2 as a SYRK on the matrix cores (bls_reduce_schur_dense_mfma_*): Float64 E, C
+    and FP64 MFMA (C_data Float64), or Float32 couplings C_data with FP32 MFMA and Float64 cross-slice
+    accumulation (C_data Float32); E and the landmark factorisation are Float64 in both (workspace "f64").
+    Returns True, or False with ws.failed_block set."""
+    import torch
+    if check:
+        _check_damping(lam, scaling_e, "scaling_e")
+    if not handle.dense:
+        raise BLSError("the MFMA reduction needs a dense handle")
+    v = "f64" if C_data.dtype == torch.float64 else "mixed"
+    if ws.variant != "f64":
+        raise BLSError("the MFMA reduction keeps E_inv in Float64: use a Float64 workspace")
+    st = handle.st
+    _require("S", S, torch.float64, st.nr * st.nr)
+    _require("g_S", g_S, torch.float64, st.nr)
+    _require("E_data", E_data, torch.float64, st.eliminated_storage_length())
+    if C_data.dtype not in (torch.float64, torch.float32):
+        raise BLSError("C_data must be Float64 or Float32")
+    _require("C_data", C_data, C_data.dtype, st.coupling_storage_length())
+    fb = ct.c_int64(0)
+    f = getattr(lib(), "bls_reduce_schur_dense_mfma_" + v)
+    rc = _check(f(handle.h, _dp(S), _dp(g_S), _dp(b_r), _dp(b_e), _dp(scaling_e), float(lam), _dp(E_data), _dp(C_data),
+                  _dp(ws.E_inv_data), _dp(ws.z_data), int(nslices), ct.byref(fb), _stream(stream)))
+    ws.failed_block = int(fb.value)
+    return rc == BLS_OK
 
 
 def _reduce(handle, dense, S, g, b_r, b_e, sc, lam, E, C, ws, s_r, s_e, stream, sync):
@@ -845,6 +879,17 @@ class DenseCholesky:
     def factor_copy(self, U, stream=None):
         _check(lib().bls_chol_factor_copy_f64(self.h, _dp(U), _stream(stream)))
         return U
+
+
+def dense_residual_(r, S, t, x, stream=None):
+    """r = t - S x for a dense symmetric S stored by its upper triangle (column-major, n x n)."""
+    import torch
+    n = r.numel()
+    _require("S", S, torch.float64, n * n)
+    _require("t", t, torch.float64, n)
+    _require("x", x, torch.float64, n)
+    _check(lib().bls_dense_residual_f64(n, _dp(S), _dp(t), _dp(x), _dp(r), _stream(stream)))
+    return r
 
 
 def unscale_(c, x, s, nonfinite, stream=None):

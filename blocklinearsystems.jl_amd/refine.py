@@ -71,3 +71,19 @@ def refined_solve(pcg, S_m, g_m, x, rd, tol=1e-8, inner_tol=1e-4, maxsteps=6, ma
             return True, step, iters, rel
         rhs = rd.r
     return False, maxsteps, iters, rel
+
+
+def refined_dense_solve(chol, g_m, x, rd, steps=1):
+    """The dense reduced system of the Float64 blocks solved through the Cholesky factor of the mixed system
+    (chol, factored from S_m): x = S_m^-1 g_m, then `steps` refinement steps x += S_m^-1 (g - S x), the residual
+    from the Float64 blocks (backsubstitute!, the retained residual, and R + damping as the dense reset S in
+    rd.SR). rd.ws must hold the Float64 factorisation (factor(rd)). Returns rd.r, the last residual computed
+    (g - S x before the last update)."""
+    chol.solve(g_m, x)
+    for _ in range(steps):
+        dev.backsubstitute_(rd.x_e, x, rd.h, rd.C, rd.ws)
+        dev.retained_residual_(rd.t, rd.h, rd.b_r, rd.x_e, rd.C)
+        dev.dense_residual_(rd.r, rd.SR, rd.t, x)
+        chol.solve(rd.r, rd.d)
+        dev.axpy_(x, 1.0, rd.d)
+    return rd.r

@@ -101,6 +101,22 @@ int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const
                                      const double *scaling_e, double lambda, const double *E_data,
                                      const double *C_data, double *E_inv, double *z, const double *s_r,
                                      const double *s_e, int64_t *d_failed_block, void *stream);
+/* reduce_schur_dense! (src/schur_reduction.jl:465-479, accumulate_dense_columns! :262-313) with phase 2 as a SYRK on
+ * the matrix cores, for dense reduced systems of many landmarks (no conditioning scales): phase 1 as in
+ * bls_reduce_schur_dense_f64 (E_inv, z, the lowest failing block; S and g_S untouched on failure), then with
+ * E_inv_e = M M' and Y_e = M' C_e, S -= Y' Y over the (sum d_e) x nr matrix Y of all landmarks: upper 128 x 128 tiles
+ * on MFMA 16x16x4 in K slices of Y's rows (nslices; 0 picks one), each slice's tile accumulated in the element type
+ * of Y, the slices added into S in Float64 in slice order; g_S = b_r - sum C' z in Float64. _f64: Float64 E, C, Y
+ * and FP64 MFMA. _mixed: Float64 E (the landmark factorisation stays Float64), Float32 couplings C -- the bulk of
+ * the data --, Y rounded to Float32 and FP32 MFMA with Float32 accumulation per tile and slice (the FP32-MFMA SYRK
+ * with Float64 cross-tile accumulation). Holds Y and
+ * the partial tiles in the handle (allocated at the first call). Synchronizes. */
+int bls_reduce_schur_dense_mfma_f64(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
+                                    const double *scaling_e, double lambda, const double *E_data, const double *C_data,
+                                    double *E_inv, double *z, int64_t nslices, int64_t *failed_block, void *stream);
+int bls_reduce_schur_dense_mfma_mixed(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
+                                      const double *scaling_e, double lambda, const double *E_data, const float *C_data,
+                                      double *E_inv, double *z, int64_t nslices, int64_t *failed_block, void *stream);
 /* factor_all_eliminated! (src/schur_reduction.jl:439-455): phase 1 alone -- E_inv and z of every eliminated
  * block in Float64, the lowest failing block (1-based) in *failed_block and BLS_FAILED if one fails; synchronizes. */
 int bls_schur_factor_f64(bls_schur *h, const double *E_data, const double *b_e, const double *scaling_e, double lambda,
@@ -467,6 +483,9 @@ int bls_chol_factor_f64_async(bls_chol *h, const double *S, void *stream);
  * in Float64. */
 int bls_chol_factor_f32(bls_chol *h, const float *S, int64_t *failed_col, void *stream);
 int bls_chol_factor_f32_async(bls_chol *h, const float *S, void *stream);
+/* r = t - S x for a symmetric S (n x n, upper triangle stored column-major, as reset_schur_dense! writes it): the
+ * product with R + damping in the dense refinement residual. No reference counterpart. */
+int bls_dense_residual_f64(int64_t n, const double *S, const double *t, const double *x, double *r, void *stream);
 int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void *stream);
 /* The factor U (upper triangle of an n x n column-major array; the lower triangle is unspecified), for tests. */
 int bls_chol_factor_copy_f64(bls_chol *h, double *U, void *stream);
