@@ -306,7 +306,7 @@ def reset_schur_dense_(S, R, handle, scaling_r, lam, s_r=None, stream=None, chec
 
 
 def _check_damping(lam, scaling, name):
-    """check_damping (src/utils.jl:353-360), on the host before any work."""
+    """check_damping (src/utils.jl:100-107), on the host before any work."""
     import math
     if not (math.isfinite(lam) and lam >= 0):
         raise ValueError(f"lambda must be finite and non-negative, got {lam}")

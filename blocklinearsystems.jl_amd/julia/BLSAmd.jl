@@ -1,10 +1,15 @@
 # BLSAmd.jl -- ccall bindings of libblsamd.so for BlockLinearSystems.jl.
 #
-# Adds device methods behind the reference's own entry points: reduce_schur_sparse!, reduce_schur_dense!,
-# reset_schur_sparse!, reset_schur_dense!, backsubstitute!, reduce_qr! and copyto!(csc, src, scatter_map) keep their
-# signatures (src/schur_reduction.jl:389-507, src/qr_reduction.jl:162-187, src/SparseBlockLinearSystem.jl:356-362);
-# the device is selected by the workspace argument, a DeviceSchurWorkspace / DeviceQRWorkspace wrapping the
-# handle of the C ABI (include/bls_amd.h). Host arrays are staged through device buffers owned by the workspace.
+# Adds device methods behind the reference's own entry points, with their signatures and argument meaning:
+#   reset_schur_sparse! / reset_schur_dense!   (src/schur_reduction.jl:389-436)
+#   reduce_schur_sparse! / reduce_schur_dense! (src/schur_reduction.jl:465-507), conditioning scales s_r, s_e included
+#   backsubstitute!(x_e, x_r, structure, C_data, ws, s_r, s_e)  (src/schur_reduction.jl:519-576)
+#   reduce_qr!(ws, J, scaling_e, lambda, s)     (src/qr_reduction.jl:162-187)
+#   backsubstitute!(x_e, x_r, ws::QRReductionWorkspace)  (src/qr_reduced.jl:97-110)
+#   Base.copyto!(dest::SparseMatrixCSC, src, scatter_map)  (src/SparseBlockLinearSystem.jl:356-362)
+# The device is selected by the workspace argument (a DeviceSchurWorkspace / DeviceQRWorkspace wrapping the handle of
+# the C ABI, include/bls_amd.h) or, for copyto!, by a DeviceBlockData source. Host arrays are staged through device
+# buffers owned by the workspace; results are copied back into the caller's arrays.
 # Further down: dot(x, H, y), getdiag! and prepare_scale! over a device copy of a filled Hessian, and the Cholesky
 # branch of the dense reduced solve.
 # Written against the C ABI; not executed in this repository (no Julia toolchain where it was built).
@@ -33,6 +38,9 @@ mutable struct DevBuf{T}
         return b
     end
 end
+# a conditioning scale as the C ABI takes it: NULL for UnitScale (the identity), else a device copy
+scaleptr(d, key::Symbol, s::AbstractVector) = s isa BLS.UnitScale ? C_NULL : staged(d, key, collect(Float64, s)).ptr
+
 upload!(d::DevBuf{T}, v::AbstractVector{T}) where T =
     check(ccall((:bls_upload, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}), d.ptr, v, length(v) * sizeof(T), C_NULL))
 download!(v::AbstractVector{T}, d::DevBuf{T}) where T =
@@ -83,16 +91,33 @@ function staged(d, key::Symbol, v::AbstractVector{Float64})
 end
 devbuf(d, key::Symbol, n::Integer) = (b = get!(() -> DevBuf{Float64}(n), d.bufs, key); b.n == n ? b : (d.bufs[key] = DevBuf{Float64}(n)))
 
-# reset_schur_sparse!(nzval, R_nzval, r_to_s_map, diag_inds, scaling_r, lambda, ws) (src/schur_reduction.jl:416-436)
+# reset_schur_sparse!(nzval, R_nzval, r_to_s_map, diag_inds, scaling_r, lambda, ws, s_r, S)
+# (src/schur_reduction.jl:416-436); the map and diag_inds were registered with the workspace (bls_schur_set_sparse)
 function BLS.reset_schur_sparse!(nzval::Vector{Float64}, R_nzval::Vector{Float64}, r_to_s_map::Vector{Int},
-                                 diag_inds::Vector{Int}, scaling_r::AbstractVector, lambda::Real, ws::DeviceSchurWorkspace)
+                                 diag_inds::Vector{Int}, scaling_r::AbstractVector, lambda::Real, ws::DeviceSchurWorkspace,
+                                 s_r::AbstractVector=BLS.UnitScale(length(diag_inds)), S=nothing)
     BLS.check_damping(lambda, scaling_r, :scaling_r)
-    S = devbuf(ws, :S, length(nzval))
+    Sd = devbuf(ws, :S, length(nzval))
     check(ccall((:bls_reset_schur_sparse_f64, libblsamd), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}, Ptr{Cvoid}),
-                ws.handle, S.ptr, staged(ws, :R, R_nzval).ptr, staged(ws, :scaling_r, collect(Float64, scaling_r)).ptr,
-                Float64(lambda), C_NULL, C_NULL))
-    download!(nzval, S)
+                ws.handle, Sd.ptr, staged(ws, :R, R_nzval).ptr, staged(ws, :scaling_r, collect(Float64, scaling_r)).ptr,
+                Float64(lambda), scaleptr(ws, :s_r, s_r), C_NULL))
+    download!(nzval, Sd)
+    ws.host.failed_block[] = 0
+    return nothing
+end
+
+# reset_schur_dense!(S, R, scaling_r, lambda, ws, s_r) (src/schur_reduction.jl:389-407): S = R (scaled) + damping
+function BLS.reset_schur_dense!(S::Matrix{Float64}, R::Matrix{Float64}, scaling_r::AbstractVector, lambda::Real,
+                                ws::DeviceSchurWorkspace, s_r::AbstractVector=BLS.UnitScale(size(S, 1)))
+    BLS.check_damping(lambda, scaling_r, :scaling_r)
+    Sd = devbuf(ws, :S, length(S))
+    check(ccall((:bls_reset_schur_dense_f64, libblsamd), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}, Ptr{Cvoid}),
+                ws.handle, Sd.ptr, staged(ws, :R, vec(R)).ptr, staged(ws, :scaling_r, collect(Float64, scaling_r)).ptr,
+                Float64(lambda), scaleptr(ws, :s_r, s_r), C_NULL))
+    download!(vec(S), Sd)
+    ws.host.failed_block[] = 0
     return nothing
 end
 
@@ -101,42 +126,182 @@ end
 function BLS.reduce_schur_sparse!(S::SparseMatrixCSC{Float64}, g_S::AbstractVector, b_r::AbstractVector,
                                   b_e::AbstractVector, scaling_e::AbstractVector, lambda::Real,
                                   structure::BLS.SchurStructure, E_data::Vector{Float64}, C_data::Vector{Float64},
-                                  block_colptr, block_rowval, block_row_offsets, col_heights, ws::DeviceSchurWorkspace)
+                                  block_colptr, block_rowval, block_row_offsets, col_heights, ws::DeviceSchurWorkspace,
+                                  s_r::AbstractVector=BLS.UnitScale(length(b_r)),
+                                  s_e::AbstractVector=BLS.UnitScale(length(b_e)))
+    ws.dense && throw(ArgumentError("the workspace was registered for a dense reduced system"))
+    return device_reduce!(:bls_reduce_schur_sparse_f64, S.nzval, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data,
+                          ws, s_r, s_e)
+end
+
+# reduce_schur_dense!(S, g_S, b_r, b_e, scaling_e, lambda, structure, E_data, C_data, ws, s_r, s_e)
+# (src/schur_reduction.jl:465-479): S (upper triangle) holds R + damping from reset_schur_dense!
+function BLS.reduce_schur_dense!(S::Matrix{Float64}, g_S::AbstractVector, b_r::AbstractVector, b_e::AbstractVector,
+                                 scaling_e::AbstractVector, lambda::Real, structure::BLS.SchurStructure,
+                                 E_data::Vector{Float64}, C_data::Vector{Float64}, ws::DeviceSchurWorkspace,
+                                 s_r::AbstractVector=BLS.UnitScale(length(b_r)),
+                                 s_e::AbstractVector=BLS.UnitScale(length(b_e)))
+    ws.dense || throw(ArgumentError("the workspace was registered for a sparse reduced system"))
+    return device_reduce!(:bls_reduce_schur_dense_f64, vec(S), g_S, b_r, b_e, scaling_e, lambda, E_data, C_data,
+                          ws, s_r, s_e)
+end
+
+# both reductions: stage the inputs, run, and on success copy S, g_S, E_inv and z back (on failure S and g_S are
+# left as the caller's, as the reference leaves them; E_inv and z stay on the device for backsubstitute!)
+function device_reduce!(sym::Symbol, Sv::AbstractVector{Float64}, g_S, b_r, b_e, scaling_e, lambda, E_data, C_data,
+                        ws::DeviceSchurWorkspace, s_r, s_e)
     BLS.check_damping(lambda, scaling_e, :scaling_e)
-    Sd = staged(ws, :S, S.nzval)           # S holds R + damping (reset_schur_sparse!)
+    Sd = staged(ws, :S, Sv)                # S holds R + damping (the reset)
     g = devbuf(ws, :g, length(g_S))
     Ei = devbuf(ws, :E_inv, length(ws.host.E_inv_data))
     z = devbuf(ws, :z, length(ws.host.z_data))
     fb = Ref{Int64}(0)
-    rc = check(ccall((:bls_reduce_schur_sparse_f64, libblsamd), Cint,
+    rc = check(ccall((sym, libblsamd), Cint,
                      (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid},
                       Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Ptr{Cvoid}),
                      ws.handle, Sd.ptr, g.ptr, staged(ws, :b_r, collect(Float64, b_r)).ptr,
                      staged(ws, :b_e, collect(Float64, b_e)).ptr, staged(ws, :scaling_e, collect(Float64, scaling_e)).ptr,
                      Float64(lambda), staged(ws, :E, E_data).ptr, staged(ws, :C, C_data).ptr, Ei.ptr, z.ptr,
-                     C_NULL, C_NULL, fb, C_NULL))
+                     scaleptr(ws, :s_r, s_r), scaleptr(ws, :s_e, s_e), fb, C_NULL))
     ws.host.failed_block[] = fb[]
     rc == BLS_FAILED && return false
-    download!(S.nzval, Sd)
+    download!(Sv, Sd)
     tmp = Vector{Float64}(undef, length(g_S)); download!(tmp, g); g_S .= tmp
     download!(ws.host.E_inv_data, Ei)
     download!(ws.host.z_data, z)
     return true
 end
 
-# copyto!(dest, src, scatter_map) (src/SparseBlockLinearSystem.jl:356-362) on device copies; for repeated use keep
-# the map on the device (bls_scatter_map_create) and call bls_copyto_csc_f64 directly
-function device_copyto!(dest::Vector{Float64}, src::Vector{Float64}, scatter_map::Vector{Int64})
-    length(dest) == length(scatter_map) || throw(DimensionMismatch("Destination nzval length ($(length(dest))) does not match scatter map length ($(length(scatter_map)))"))
-    dm = Ref{Ptr{Int64}}(C_NULL)
-    check(ccall((:bls_scatter_map_create, libblsamd), Cint, (Int64, Ptr{Int64}, Ptr{Ptr{Int64}}), length(scatter_map), scatter_map, dm))
-    s = DevBuf{Float64}(length(src)); upload!(s, src)
-    d = DevBuf{Float64}(length(dest))
+# backsubstitute!(x_e, x_r, structure, C_data, ws, s_r, s_e) (src/schur_reduction.jl:519-576): x_e = z - E_inv sum
+# C x_r from the E_inv and z of the workspace's last reduction (kept on the device), C_data as staged by it
+function BLS.backsubstitute!(x_e::AbstractVector, x_r::AbstractVector, structure::BLS.SchurStructure,
+                             C_data::Vector{Float64}, ws::DeviceSchurWorkspace,
+                             s_r::AbstractVector=BLS.UnitScale(length(x_r)),
+                             s_e::AbstractVector=BLS.UnitScale(length(x_e)))
+    haskey(ws.bufs, :E_inv) || throw(ArgumentError("backsubstitute! needs a reduction on this workspace first"))
+    xe = devbuf(ws, :x_e, length(x_e))
+    check(ccall((:bls_backsubstitute_schur_f64, libblsamd), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                ws.handle, xe.ptr, staged(ws, :x_r, collect(Float64, x_r)).ptr, staged(ws, :C, C_data).ptr,
+                ws.bufs[:E_inv].ptr, ws.bufs[:z].ptr, scaleptr(ws, :s_r, s_r), scaleptr(ws, :s_e, s_e), C_NULL))
+    tmp = Vector{Float64}(undef, length(x_e)); download!(tmp, xe); x_e .= tmp
+    return nothing
+end
+
+# ---- copyto!(dest::SparseMatrixCSC, src, scatter_map) (src/SparseBlockLinearSystem.jl:356-362) ------------------
+
+"""
+    DeviceBlockData(data::Vector{Float64})
+    DeviceBlockData(ws::DeviceSchurWorkspace)
+
+The `data` of a SparseBlockLinearSystem on the device (a copy of `data`, or the reduced system a
+DeviceSchurWorkspace's last reset/reduction left on the device), as the source of `copyto!`.
+"""
+struct DeviceBlockData
+    buf::DevBuf{Float64}
+end
+DeviceBlockData(data::Vector{Float64}) = (b = DevBuf{Float64}(length(data)); upload!(b, data); DeviceBlockData(b))
+DeviceBlockData(ws::DeviceSchurWorkspace) = DeviceBlockData(ws.bufs[:S])
+
+"""
+    DeviceScatterMap(scatter_map)
+
+A scatter map (the second result of sparse_with_map) uploaded once, for repeated `copyto!`.
+"""
+mutable struct DeviceScatterMap
+    ptr::Ptr{Int64}
+    n::Int
+    function DeviceScatterMap(scatter_map::AbstractVector{<:Integer})
+        dm = Ref{Ptr{Int64}}(C_NULL)
+        m = collect(Int64, scatter_map)
+        check(ccall((:bls_scatter_map_create, libblsamd), Cint, (Int64, Ptr{Int64}, Ptr{Ptr{Int64}}), length(m), m, dm))
+        s = new(dm[], length(m))
+        finalizer(x -> ccall((:bls_free, libblsamd), Cint, (Ptr{Cvoid},), x.ptr), s)
+        return s
+    end
+end
+
+function Base.copyto!(dest::SparseMatrixCSC{Float64}, src::DeviceBlockData, scatter_map::DeviceScatterMap)
+    length(dest.nzval) == scatter_map.n || throw(DimensionMismatch("Destination nzval length ($(length(dest.nzval))) does not match scatter map length ($(scatter_map.n))"))
+    d = DevBuf{Float64}(length(dest.nzval))
     check(ccall((:bls_copyto_csc_f64, libblsamd), Cint, (Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Ptr{Cvoid}),
-                length(dest), d.ptr, s.ptr, dm[], C_NULL))
-    download!(dest, d)
-    ccall((:bls_free, libblsamd), Cint, (Ptr{Cvoid},), dm[])
+                length(dest.nzval), d.ptr, src.buf.ptr, scatter_map.ptr, C_NULL))
+    download!(dest.nzval, d)
     return dest
+end
+
+Base.copyto!(dest::SparseMatrixCSC{Float64}, src::DeviceBlockData, scatter_map::AbstractVector{<:Integer}) =
+    copyto!(dest, src, DeviceScatterMap(scatter_map))
+
+# ---- reduce_qr! and backsubstitute! of the QR path (src/qr_reduction.jl:162-187, src/qr_reduced.jl:97-110) -------
+
+"""
+    DeviceQRWorkspace(ws::QRReductionWorkspace{Float64})
+
+Registers the workspace's SchurJacobianStructure with the device (bls_qr_create + bls_qr_set_reduced) and owns
+device copies of the raw and factored tables. Pass it where the reference takes the QR workspace.
+"""
+mutable struct DeviceQRWorkspace
+    handle::Ptr{Cvoid}
+    host::BLS.QRReductionWorkspace{Float64}
+    bufs::Dict{Symbol, DevBuf{Float64}}
+end
+
+function DeviceQRWorkspace(ws::BLS.QRReductionWorkspace{Float64})
+    jst = ws.structure
+    st = jst.schur
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:bls_qr_create, libblsamd), Cint,
+                (Int64, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Ptr{Cvoid}}),
+                st.nr, st.nrblocks, st.neblocks, collect(Int64, st.e_dims), st.boffsets_e, jst.ntables,
+                jst.table_nrows, jst.table_ncols, jst.raw_offsets, jst.residual_offsets, h))
+    d = DeviceQRWorkspace(h[], ws, Dict{Symbol, DevBuf{Float64}}())
+    finalizer(x -> ccall((:bls_qr_destroy, libblsamd), Cvoid, (Ptr{Cvoid},), x.handle), d)
+    check(ccall((:bls_qr_set_reduced, libblsamd), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+                d.handle, jst.table_top, jst.reduced_offsets, jst.table_ptr, length(jst.entry_retained),
+                jst.entry_retained, jst.entry_col, jst.entry_table, jst.retained_ptr, jst.retained_entry,
+                collect(Int64, st.r_dims), st.boffsets_r, jst.doffsets))
+    return d
+end
+
+# reduce_qr!(ws, J, scaling_e, lambda, s) (src/qr_reduction.jl:162-187): the structure and length checks first, then
+# every table on the device; the factored tables and residual are copied into ws.host (factored, fres)
+function BLS.reduce_qr!(ws::DeviceQRWorkspace, J::BLS.SchurBlockJacobian{Float64}, scaling_e::AbstractVector,
+                        lambda::Real, s::AbstractVector=BLS.UnitScale(0))
+    jst = ws.host.structure
+    J.structure === jst || throw(ArgumentError("The Jacobian and the workspace were built for different structures"))
+    ne = sum(Int, jst.schur.e_dims; init=0)
+    length(scaling_e) == ne || throw(DimensionMismatch("scaling_e has length $(length(scaling_e)), expected $ne"))
+    BLS.check_damping(lambda, scaling_e, :scaling_e)
+    F = devbuf(ws, :factored, length(ws.host.factored))
+    Fr = devbuf(ws, :fres, length(ws.host.fres))
+    fb = Ref{Int64}(0)
+    rc = check(ccall((:bls_reduce_qr_f64, libblsamd), Cint,
+                     (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid},
+                      Ptr{Int64}, Ptr{Cvoid}),
+                     ws.handle, F.ptr, Fr.ptr, staged(ws, :raw, J.data).ptr, staged(ws, :r, J.r).ptr,
+                     staged(ws, :scaling_e, collect(Float64, scaling_e)).ptr, Float64(lambda),
+                     scaleptr(ws, :s, s), fb, C_NULL))
+    ws.host.failed_block[] = fb[]
+    rc == BLS_FAILED && return false
+    download!(ws.host.factored, F)
+    download!(ws.host.fres, Fr)
+    return true
+end
+
+# backsubstitute!(x_e, x_r, ws::QRReductionWorkspace) (src/qr_reduced.jl:97-110): x_e = U^-1 (q - B x_r) per table,
+# from the device's factored tables of the last reduce_qr!
+function BLS.backsubstitute!(x_e::AbstractVector, x_r::AbstractVector, ws::DeviceQRWorkspace)
+    haskey(ws.bufs, :factored) || throw(ArgumentError("backsubstitute! needs reduce_qr! on this workspace first"))
+    xe = devbuf(ws, :x_e, length(x_e))
+    check(ccall((:bls_qr_backsubstitute_f64, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                ws.handle, xe.ptr, staged(ws, :x_r, collect(Float64, x_r)).ptr, ws.bufs[:factored].ptr,
+                ws.bufs[:fres].ptr, C_NULL))
+    tmp = Vector{Float64}(undef, length(x_e)); download!(tmp, xe); x_e .= tmp
+    return nothing
 end
 
 # ---- The storage of a filled Hessian: dot(x, H, y), getdiag! and the conditioning scale -----------------------

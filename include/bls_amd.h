@@ -13,7 +13,7 @@
  *
  * Return codes: BLS_OK (0) success; BLS_FAILED (1) numerical failure, the reference's `false`, with the lowest
  * failing eliminated block (1-based) in *failed_block; negative values are errors, described by bls_last_error().
- * Argument checks the reference makes before any work (check_damping, src/utils.jl:353-360; reduce_qr!'s
+ * Argument checks the reference makes before any work (check_damping, src/utils.jl:100-107; reduce_qr!'s
  * structure and length checks, src/qr_reduction.jl:166-169) are made here on scalars and sizes; checks over
  * device arrays (finite, non-negative scaling vectors) are the caller's, made on its host copies before the call.
  */

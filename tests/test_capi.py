@@ -87,3 +87,19 @@ def test_backsubstitute_arguments_rejected_before_launch(blsamd):
         factored = f64(jst.raw_length() - 1)
     with pytest.raises(dev.BLSError):
         dev._check_qr_backsub(jst, WS2, f64(jst.schur.ne), f64(jst.schur.nr))
+
+
+def test_julia_binding_calls_only_exported_symbols(blsamd):
+    """Every C symbol the Julia binding (julia/BLSAmd.jl) calls is exported by the library and declared in the
+    header, and it defines a method for each of the reference's entry points on the path."""
+    lib = ct.CDLL(blsamd.device.LIBPATH)
+    text = open(os.path.join(ROOT, "blocklinearsystems.jl_amd", "julia", "BLSAmd.jl")).read()
+    called = set(re.findall(r"ccall\(\(:(bls_[a-z0-9_]+)", text)) | set(re.findall(r":(bls_[a-z0-9_]+)", text))
+    declared = set(declared_symbols())
+    assert called and all(hasattr(lib, s) for s in called), sorted(s for s in called if not hasattr(lib, s))
+    assert called <= declared, sorted(called - declared)
+    for method in ("function BLS.reset_schur_sparse!", "function BLS.reset_schur_dense!",
+                   "function BLS.reduce_schur_sparse!", "function BLS.reduce_schur_dense!",
+                   "function BLS.reduce_qr!", "function Base.copyto!(dest::SparseMatrixCSC{Float64}"):
+        assert method in text, method
+    assert text.count("function BLS.backsubstitute!") == 2   # Schur and QR
