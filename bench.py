@@ -51,7 +51,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibration-copy", action="store_true")
-    ap.add_argument("--qr-scale", type=float, default=0.2,
+    ap.add_argument("--qr-scale", type=float, default=1.0,
                     help="landmark scale of the reduce_qr! leg (500 cameras d 6/9, 500000 landmarks x scale); 0 skips")
     ap.add_argument("--config3-scale", type=float, default=1.0,
                     help="landmark scale of the dense leg (256 cameras, 1000000 landmarks x scale); 0 skips it")
@@ -947,8 +947,8 @@ def refined_leg(torch, dev, b, p, h, pcg, S_m, g_m, S64, g64, reduce, f64, reps)
         assert R.factor(rd, check=False)
         blocks = None
         if learned and model is not None and model.alpha != 0.0:
-            blocks = P.shift_blocks(pcg, S_m, model.mu(P.features(pcg, S_m, p.lam)))
-        return R.refined_solve(pcg, S_m, g_m, x, rd, tol=1e-8, factored=True, precond_blocks=blocks)
+            blocks = P.group_blocks(pcg, S_m, model.omega(P.features(pcg, S_m, p.lam)), model.group)
+        return R.refined_solve(pcg, S_m, g_m, x, rd, tol=1e-8, factored=True, group_blocks=blocks)
 
     res = {}
     for name, learned in (("plain", False), ("learned", True)):

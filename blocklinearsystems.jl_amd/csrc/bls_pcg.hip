@@ -33,6 +33,7 @@ constexpr int UPD_THREADS = 512;
 constexpr int GRP_BLOCKS = 256;     // blocks per group of k_pcg_step1 / k_pcg_step2 (one thread each)
 constexpr int ITERS_PER_GRAPH = 8;  // even: the rz parity of every problem is 0 at each replay's start
 constexpr int MAXD = 16;
+constexpr int PC_GMAX = 4;          // blocks per preconditioner group of the learned correction
 
 struct ProbState {
     double rz2[2], gnorm, relres;
@@ -52,6 +53,11 @@ struct PcgArgs {
     const double *vals;                // BSR values
     const double *L;                   // preconditioner factors (lower, column-major d x d per block)
     const int64_t *loff;               // offset of each block's factor
+    // preconditioner groups (the learned correction): gsize[b] = the number of consecutive blocks of the group that
+    // block b starts (1 .. PC_GMAX), 0 for the other blocks of a group; gloff[b] the group's factor offset.
+    // nullptr: every block alone
+    const int32_t *gsize;
+    const int64_t *gloff;
     double *x, *r, *z, *p, *q, *g;
     double *pq_part, *rz_part, *rr_part;   // per block
     const int32_t *grp_prob;           // problem of each group
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(256) void k_bsr_spmv_u(PcgArgs a, const double *xin
 
 // z_b = (L L')^-1 r_b for one block (solve_diagonal_blocks!, src/diagonal_blocks.jl:33-61)
 __device__ inline void block_solve(const double *L, int d, const double *rb, double *zb) {
-    double y[MAXD];
+    double y[PC_GMAX * MAXD];
     for (int i = 0; i < d; i++) y[i] = rb[i];
     for (int j = 0; j < d; j++) {
         const double yj = y[j] / L[j * d + j];
@@ -184,6 +190,16 @@ __device__ inline void block_solve(const double *L, int d, const double *rb, dou
     }
     for (int i = 0; i < d; i++) zb[i] = y[i];
 }
+
+// Dimension of the preconditioner group that block b starts (0 when b is not a group's first block) and its factor.
+__device__ inline int group_dim(const PcgArgs &a, int64_t b) {
+    if (!a.gsize) return a.dims[b];
+    const int gs = a.gsize[b];
+    int d = 0;
+    for (int k = 0; k < gs; k++) d += a.dims[b + k];
+    return d;
+}
+__device__ inline const double *group_L(const PcgArgs &a, int64_t b) { return a.gsize ? a.L + a.gloff[b] : a.L + a.loff[b]; }
 
 // fixed-order workgroup sum (deterministic)
 __device__ inline double wg_sum(double v, double *red) {
@@ -220,8 +236,10 @@ __global__ __launch_bounds__(UPD_THREADS) void k_pcg_restart(PcgArgs a, int firs
     rr = wg_sum(rr, red);
     const bool ok = !ps.failed;
     if (ok)
-        for (int64_t b = b0 + threadIdx.x; b < b1; b += UPD_THREADS)
-            block_solve(a.L + a.loff[b], a.dims[b], a.r + a.bo[b], a.z + a.bo[b]);
+        for (int64_t b = b0 + threadIdx.x; b < b1; b += UPD_THREADS) {
+            const int gd = group_dim(a, b);   // a group's rows follow one another; its first block solves them
+            if (gd) block_solve(group_L(a, b), gd, a.r + a.bo[b], a.z + a.bo[b]);
+        }
     __syncthreads();
     double rz = 0.0;
     for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
@@ -271,20 +289,36 @@ __global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_step1(PcgArgs a, int parity)
     const double pq = problem_sum(a.pq_part, a.prob_bptr[pb], a.prob_bptr[pb + 1], red);
     const double alpha = ps.rz2[parity] / pq;
     const int64_t b = a.grp_b0[blockIdx.x] + threadIdx.x;
-    if (b >= a.grp_b1[blockIdx.x]) return;
-    const int d = a.dims[b];
-    const int64_t o = a.bo[b];
+    const bool in = b < a.grp_b1[blockIdx.x];
+    const int64_t o = in ? a.bo[b] : 0;
     double rr = 0.0;
-    for (int i = 0; i < d; i++) {
-        a.x[o + i] += alpha * a.p[o + i];
-        const double rv = a.r[o + i] - alpha * a.q[o + i];
-        a.r[o + i] = rv;
-        rr += rv * rv;
+    if (in) {
+        const int d = a.dims[b];
+        for (int i = 0; i < d; i++) {
+            a.x[o + i] += alpha * a.p[o + i];
+            const double rv = a.r[o + i] - alpha * a.q[o + i];
+            a.r[o + i] = rv;
+            rr += rv * rv;
+        }
     }
-    block_solve(a.L + a.loff[b], d, a.r + o, a.z + o);
-    double rz = 0.0;
-    for (int i = 0; i < d; i++) rz += a.r[o + i] * a.z[o + i];
-    a.rz_part[b] = rz;
+    // a group's other blocks are updated by the next threads (preconditioner groups never cross an update group of
+    // GRP_BLOCKS: they start at multiples of their size, a divisor of GRP_BLOCKS, from their problem's first block)
+    if (a.gsize) __syncthreads();
+    if (!in) return;
+    const int gd = group_dim(a, b);
+    if (gd) {
+        block_solve(group_L(a, b), gd, a.r + o, a.z + o);
+        // r.z per block, as the plain preconditioner sums it (the group's other blocks' partials by this thread)
+        const int gs = a.gsize ? a.gsize[b] : 1;
+        int64_t oo = o;
+        for (int k = 0; k < gs; k++) {
+            const int dk = a.dims[b + k];
+            double rz = 0.0;
+            for (int i = 0; i < dk; i++) rz += a.r[oo + i] * a.z[oo + i];
+            a.rz_part[b + k] = rz;
+            oo += dk;
+        }
+    }
     a.rr_part[b] = rr;
 }
 
@@ -350,6 +384,71 @@ __global__ void k_pcg_factor(PcgArgs a, int64_t nblocks, const int64_t *diag_vof
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// Preconditioner groups: factor each group's full block (Mblocks at gloff, gd x gd column-major) into Lout at gloff.
+__global__ void k_pcg_factor_groups(PcgArgs a, int64_t nblocks, double *Lout, const double *Mblocks) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const int d = group_dim(a, b);
+    if (!d) return;
+    double *L = Lout + a.gloff[b];
+    for (int q = 0; q < d * d; q++) L[q] = Mblocks[a.gloff[b] + q];
+    for (int j = 0; j < d; j++) {
+        double s = L[j * d + j];
+        for (int k = 0; k < j; k++) s -= L[k * d + j] * L[k * d + j];
+        if (!(s > 0.0) || !isfinite(s)) {
+            a.st[a.prob_of_block[b]].failed = 1;
+            return;
+        }
+        const double ljj = sqrt(s);
+        L[j * d + j] = ljj;
+        for (int i = j + 1; i < d; i++) {
+            double t = L[j * d + i];
+            for (int k = 0; k < j; k++) t -= L[k * d + i] * L[k * d + j];
+            L[j * d + i] = t / ljj;
+        }
+    }
+}
+
+// The group correction of block-Jacobi (the learned correction's family; no reference counterpart): the group of
+// consecutive blocks b .. b + gs - 1 that block b starts gets the block matrix with S's diagonal blocks on its
+// diagonal and w S_ij (w = omega[b] in [0, 1]; zero where the pattern has no block (i, j)) off it: w = 0 is plain
+// block-Jacobi, w = 1 the group's exact principal block submatrix of S. For w in [0, 1] it is a convex combination
+// of the two SPD matrices, so it stays SPD.
+__global__ void k_pcg_group_blocks(PcgArgs a, const double *S, const int64_t *map, const int64_t *diag_voff,
+                                   const float *omega, double *M) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.nrb) return;
+    const int gd = group_dim(a, b);
+    if (!gd) return;
+    const int gs = a.gsize[b];
+    double *out = M + a.gloff[b];
+    for (int q = 0; q < gd * gd; q++) out[q] = 0.0;
+    const double w = fmin(1.0, fmax(0.0, (double)omega[b]));
+    int ri = 0;   // first row of block i in the group
+    for (int i = 0; i < gs; i++) {
+        const int64_t bi = b + i;
+        const int di = a.dims[bi];
+        int cj = 0;
+        for (int j = 0; j < gs; j++) {
+            const int64_t bj = b + j;
+            const int dj = a.dims[bj];
+            if (i == j) {
+                for (int c = 0; c < di; c++)
+                    for (int r = 0; r < di; r++) out[(ri + c) * gd + ri + r] = S[map[diag_voff[bi] + r * di + c] - 1];
+            } else if (w != 0.0) {
+                int64_t voff = -1;
+                for (int k = a.bsr_rowptr[bi]; k < a.bsr_rowptr[bi + 1]; k++)
+                    if (a.bsr_col[k] == bj) voff = a.bsr_voff[k];
+                if (voff >= 0)
+                    for (int r = 0; r < di; r++)
+                        for (int c = 0; c < dj; c++) out[(cj + c) * gd + ri + r] = w * S[map[voff + r * dj + c] - 1];
+            }
+            cj += dj;
+        }
+        ri += di;
+    }
+}
+
 // The learned correction of the block-Jacobi preconditioner (no reference counterpart): per retained block,
 // features for a small MLP, and the corrected blocks M_b = D_b + mu_b m_b I (D_b the diagonal block of S, m_b the
 // mean of its diagonal), read from S through the handle's BSR map.
@@ -490,15 +589,31 @@ struct bls_pcg {
     hipGraphExec_t exec = nullptr;
     double graph_tol = -1;
     int64_t graph_maxit = -1;
+    // preconditioner groups of `group` consecutive blocks (bls_pcg_set_groups) and their iteration graph
+    DevArray<int32_t> d_gsize;
+    DevArray<int64_t> d_gloff;
+    DevArray<double> d_Lp;
+    int64_t nLp = 0;
+    int group = 0;
+    hipGraph_t graph_p = nullptr;
+    hipGraphExec_t exec_p = nullptr;
+    double graph_p_tol = -1;
+    int64_t graph_p_maxit = -1;
     ~bls_pcg() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (exec_p) (void)hipGraphExecDestroy(exec_p);
+        if (graph_p) (void)hipGraphDestroy(graph_p);
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_out) (void)hipEventDestroy(ev_out);
         if (stream) (void)hipStreamDestroy(stream);
     }
-    PcgArgs args(double tol, int64_t maxit) {
+    PcgArgs args(double tol, int64_t maxit, bool groups = false) {
         PcgArgs a{};
+        if (groups) {
+            a.gsize = d_gsize.ptr;
+            a.gloff = d_gloff.ptr;
+        }
         a.nr = nr;
         a.row_block = d_row_block.ptr;
         a.bsr_rowptr = d_rowptr.ptr;
@@ -509,7 +624,7 @@ struct bls_pcg {
         a.prob_of_block = d_prob_of_block.ptr;
         a.prob_bptr = d_prob_bptr.ptr;
         a.vals = d_vals.ptr;
-        a.L = d_L.ptr;
+        a.L = groups ? d_Lp.ptr : d_L.ptr;
         a.loff = d_loff.ptr;
         a.x = d_x.ptr; a.r = d_r.ptr; a.z = d_z.ptr; a.p = d_p.ptr; a.q = d_q.ptr; a.g = d_g.ptr;
         a.pq_part = d_pq.ptr; a.rz_part = d_rzp.ptr; a.rr_part = d_rrp.ptr;
@@ -670,16 +785,46 @@ extern "C" int bls_pcg_values_size(bls_pcg *h, int64_t *nvals, int64_t *nfactor)
     return BLS_OK;
 }
 
-extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol,
-                                 int64_t maxit, const double *precond_blocks, int64_t *iters, double *relres,
-                                 void *stream) {
+// Preconditioner groups of every problem: blocks (b0 + G j, ..., b0 + G j + G - 1) from the problem's first block b0
+// (the last group may be shorter), and factor offsets for the group blocks.
+static int build_groups(bls_pcg *h, int group) {
+    if (group < 1 || group > PC_GMAX || GRP_BLOCKS % group != 0)
+        return fail(BLS_ERR_ARG, "preconditioner groups of 1 to 4 blocks");
+    if (h->group == group) return BLS_OK;
+    std::vector<int32_t> gsize(h->nrb, 0), dims(h->nrb);
+    std::vector<int64_t> gloff(h->nrb, 0);
+    BLS_HIP(hipMemcpy(dims.data(), h->d_dims.ptr, sizeof(int32_t) * h->nrb, hipMemcpyDeviceToHost));
+    int64_t nl = 0;
+    for (int64_t p = 0; p < h->nprob; p++)
+        for (int64_t b = h->prob_bptr[p]; b < h->prob_bptr[p + 1]; b += group) {
+            const int64_t e = std::min<int64_t>(b + group, h->prob_bptr[p + 1]);
+            gsize[b] = (int32_t)(e - b);
+            gloff[b] = nl;
+            int64_t gd = 0;
+            for (int64_t k = b; k < e; k++) gd += dims[k];
+            nl += gd * gd;
+        }
+    int rc;
+    if ((rc = h->d_gsize.upload(gsize)) || (rc = h->d_gloff.upload(gloff)) || (rc = h->d_Lp.alloc((size_t)std::max<int64_t>(1, nl))))
+        return rc;
+    h->nLp = nl;
+    h->group = group;
+    if (h->exec_p) { (void)hipGraphExecDestroy(h->exec_p); h->exec_p = nullptr; }
+    if (h->graph_p) { (void)hipGraphDestroy(h->graph_p); h->graph_p = nullptr; }
+    return BLS_OK;
+}
+
+static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol, int64_t maxit,
+                     const double *precond_blocks, int64_t *iters, double *relres, bool groups, void *stream) {
     if (!h || !S_nzval || !g || !x) return fail(BLS_ERR_ARG, "bls_pcg_solve_f64: bad arguments");
     if (!(tol > 0) || maxit < 0) return fail(BLS_ERR_ARG, "tol must be positive and maxit non-negative");
+    if (groups && (!precond_blocks || h->group == 0))
+        return fail(BLS_ERR_STATE, "the group-corrected solve needs bls_pcg_set_groups and its group blocks");
     hipStream_t cs = (hipStream_t)stream;
     hipStream_t s = h->stream;
     BLS_HIP(hipEventRecord(h->ev_in, cs));
     BLS_HIP(hipStreamWaitEvent(s, h->ev_in, 0));
-    PcgArgs a = h->args(tol, maxit);
+    PcgArgs a = h->args(tol, maxit, groups);
     const int nprob = (int)h->nprob;
     auto spmv = [&](const double *xin, double *yout, double *pq, int only_active) {
         launch_spmv(h, a, xin, yout, pq, only_active, s);
@@ -688,24 +833,31 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
     k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
     k_copy_f64<<<std::min<int64_t>(grid_for(h->nr, 256), 4096), 256, 0, s>>>(h->d_g.ptr, g, h->nr);
     k_pcg_init_state<<<grid_for(nprob, 256), 256, 0, s>>>(h->d_st.ptr, nprob, h->d_nactive.ptr);
-    k_pcg_factor<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_diag_voff.ptr, h->d_L.ptr, precond_blocks);
+    if (groups)
+        k_pcg_factor_groups<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_Lp.ptr, precond_blocks);
+    else
+        k_pcg_factor<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_diag_voff.ptr, h->d_L.ptr, precond_blocks);
     k_pcg_restart<<<nprob, UPD_THREADS, 0, s>>>(a, 1);
     BLS_HIP(hipGetLastError());
-    // the iteration graph (handle-owned buffers only; rebuilt when tol / maxit change)
-    if (!h->exec || h->graph_tol != tol || h->graph_maxit != maxit) {
-        if (h->exec) { (void)hipGraphExecDestroy(h->exec); h->exec = nullptr; }
-        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    // the iteration graph (handle-owned buffers only; rebuilt when tol / maxit change; one per preconditioner kind)
+    hipGraph_t &graph = groups ? h->graph_p : h->graph;
+    hipGraphExec_t &exec = groups ? h->exec_p : h->exec;
+    double &gtol = groups ? h->graph_p_tol : h->graph_tol;
+    int64_t &gmaxit = groups ? h->graph_p_maxit : h->graph_maxit;
+    if (!exec || gtol != tol || gmaxit != maxit) {
+        if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         BLS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int it = 0; it < ITERS_PER_GRAPH; it++) {
             spmv(h->d_p.ptr, h->d_q.ptr, h->d_pq.ptr, 1);
             k_pcg_step1<<<ngroups, GRP_BLOCKS, 0, s>>>(a, it & 1);
             k_pcg_step2<<<ngroups, GRP_BLOCKS, 0, s>>>(a, it & 1);
         }
-        hipError_t e = hipStreamEndCapture(s, &h->graph);
+        hipError_t e = hipStreamEndCapture(s, &graph);
         if (e != hipSuccess) return fail(BLS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        BLS_HIP(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
-        h->graph_tol = tol;
-        h->graph_maxit = maxit;
+        BLS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        gtol = tol;
+        gmaxit = maxit;
     }
     // iterate; then confirm every problem with its true residual, restarting those still above tol
     int nact = 0;
@@ -714,7 +866,7 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
         BLS_HIP(hipStreamSynchronize(s));
         int64_t launched = 0;
         while (nact > 0 && launched <= maxit + ITERS_PER_GRAPH) {
-            BLS_HIP(hipGraphLaunch(h->exec, s));
+            BLS_HIP(hipGraphLaunch(exec, s));
             launched += ITERS_PER_GRAPH;
             BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
             BLS_HIP(hipStreamSynchronize(s));
@@ -739,6 +891,41 @@ extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double
         if (stv[p].failed || !(stv[p].relres <= tol)) all = false;
     }
     return all ? BLS_OK : BLS_FAILED;
+}
+
+extern "C" int bls_pcg_solve_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol,
+                                 int64_t maxit, const double *precond_blocks, int64_t *iters, double *relres,
+                                 void *stream) {
+    return pcg_solve(h, S_nzval, g, x, tol, maxit, precond_blocks, iters, relres, false, stream);
+}
+
+extern "C" int bls_pcg_set_groups(bls_pcg *h, int group) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    return build_groups(h, group);
+}
+
+extern "C" int bls_pcg_solve_groups_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol,
+                                        int64_t maxit, const double *group_blocks, int64_t *iters, double *relres,
+                                        void *stream) {
+    return pcg_solve(h, S_nzval, g, x, tol, maxit, group_blocks, iters, relres, true, stream);
+}
+
+extern "C" int bls_pcg_group_blocks_size(bls_pcg *h, int64_t *n) {
+    if (!h || !n) return fail(BLS_ERR_ARG, "bls_pcg_group_blocks_size: bad arguments");
+    if (h->group == 0) return fail(BLS_ERR_STATE, "no preconditioner groups (bls_pcg_set_groups)");
+    *n = h->nLp;
+    return BLS_OK;
+}
+
+extern "C" int bls_pcg_group_blocks_f64(bls_pcg *h, const double *S_nzval, const float *omega, double *blocks,
+                                        void *stream) {
+    if (!h || !S_nzval || !omega || !blocks) return fail(BLS_ERR_ARG, "bls_pcg_group_blocks_f64: bad arguments");
+    if (h->group == 0) return fail(BLS_ERR_STATE, "no preconditioner groups (bls_pcg_set_groups)");
+    PcgArgs a = h->args(1.0, 0, true);
+    k_pcg_group_blocks<<<grid_for(h->nrb, 256), 256, 0, (hipStream_t)stream>>>(a, S_nzval, h->d_map.ptr,
+                                                                            h->d_diag_voff.ptr, omega, blocks);
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
 }
 
 __global__ void k_axpy_f64(int64_t n, double alpha, const double *x, double *y) {

@@ -72,6 +72,10 @@ SIGNATURES = {
     "bls_reduced_diagonal_blocks_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_pcg_relres_f64": (ct.c_int, [VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
     "bls_pcg_residual_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.POINTER(ct.c_double), VP]),
+    "bls_pcg_set_groups": (ct.c_int, [VP, ct.c_int]),
+    "bls_pcg_group_blocks_size": (ct.c_int, [VP, ct.POINTER(ct.c_int64)]),
+    "bls_pcg_group_blocks_f64": (ct.c_int, [VP, VP, VP, VP, VP]),
+    "bls_pcg_solve_groups_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, ct.c_int64, VP, P64, ct.POINTER(ct.c_double), VP]),
     "bls_axpy_f64": (ct.c_int, [ct.c_int64, ct.c_double, VP, VP, VP]),
     "bls_dense_residual_f64": (ct.c_int, [ct.c_int64, VP, VP, VP, VP, VP]),
     "bls_reduce_schur_dense_mfma_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP, VP, ct.c_int64,
@@ -605,6 +609,28 @@ class PCGHandle:
         out = (ct.c_double * self.nprob)()
         _check(lib().bls_pcg_relres_f64(self.h, _dp(S_nzval), _dp(g), _dp(x), out, _stream(stream)))
         return np.array(out[:])
+
+    def group_blocks(self, S_nzval, omega, group, stream=None):
+        """The group-corrected preconditioner blocks (device Float64) for groups of `group` consecutive retained blocks
+        and coupling weights omega (Float32 per retained block, read at each group's first block)."""
+        import torch
+        _check(lib().bls_pcg_set_groups(self.h, int(group)))
+        n = ct.c_int64(0)
+        _check(lib().bls_pcg_group_blocks_size(self.h, ct.byref(n)))
+        M = torch.empty(int(n.value), dtype=torch.float64, device="cuda")
+        _require("omega", omega, torch.float32, self.st.nrblocks)
+        _check(lib().bls_pcg_group_blocks_f64(self.h, _dp(S_nzval), _dp(omega.contiguous()), _dp(M), _stream(stream)))
+        return M
+
+    def solve_groups(self, S_nzval, g, x, group_blocks, tol=1e-8, maxit=10000, stream=None):
+        """solve() with the group-corrected preconditioner (group_blocks from group_blocks(), same group size)."""
+        iters = np.zeros(self.nprob, dtype=np.int64)
+        rel = np.zeros(self.nprob)
+        rc = _check(lib().bls_pcg_solve_groups_f64(self.h, _dp(S_nzval), _dp(g), _dp(x), float(tol), int(maxit),
+                                                   _dp(group_blocks), iters.ctypes.data_as(P64),
+                                                   rel.ctypes.data_as(ct.POINTER(ct.c_double)), _stream(stream)))
+        self.last_iters = iters
+        return rc == BLS_OK, iters, rel
 
     def residual(self, S_nzval, t, x, gref, r, stream=None):
         """r = t - S x on the device, and ||r|| / ||gref|| per problem as a numpy array."""

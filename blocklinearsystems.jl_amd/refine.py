@@ -48,11 +48,12 @@ def residual(pcg, rd, x, gref):
     return pcg.residual(rd.SR, rd.t, x, gref, rd.r)
 
 
-def refined_solve(pcg, S_m, g_m, x, rd, tol=1e-8, inner_tol=1e-4, maxsteps=6, maxit=10000, precond_blocks=None,
+def refined_solve(pcg, S_m, g_m, x, rd, tol=1e-8, inner_tol=1e-4, maxsteps=6, maxit=10000, group_blocks=None,
                   factored=False):
     """Solve the Float64 blocks' reduced system to ||g - S x|| <= tol ||g|| (judged against g_m, which equals g to
     the rounding of the inputs) using only PCG on the mixed system S_m: x = 0, then repeat {d = S_m^-1 r to
-    inner_tol; x += d; r = g - S x (Float64, from the blocks)} until the residual is met. Returns
+    inner_tol (with the learned group correction when group_blocks is given); x += d; r = g - S x (Float64, from the
+    blocks)} until the residual is met. Returns
     (converged, steps, pcg_iterations_total per problem, relres per problem)."""
     if not factored and not factor(rd):
         raise dev.BLSError(f"Float64 factorisation failed at eliminated block {rd.ws.failed_block}")
@@ -62,7 +63,10 @@ def refined_solve(pcg, S_m, g_m, x, rd, tol=1e-8, inner_tol=1e-4, maxsteps=6, ma
     rel = np.ones(pcg.nprob)
     for step in range(1, maxsteps + 1):
         rd.d.zero_()
-        pcg.solve(S_m, rhs, rd.d, tol=inner_tol, maxit=maxit, precond_blocks=precond_blocks)
+        if group_blocks is None:
+            pcg.solve(S_m, rhs, rd.d, tol=inner_tol, maxit=maxit)
+        else:
+            pcg.solve_groups(S_m, rhs, rd.d, group_blocks, tol=inner_tol, maxit=maxit)
         it = pcg.last_iters
         iters += np.maximum(it, 0)
         dev.axpy_(x, 1.0, rd.d)

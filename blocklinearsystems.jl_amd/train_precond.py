@@ -3,13 +3,13 @@
     python blocklinearsystems.jl_amd/train_precond.py [--seeds 32] [--scale 0.05] [--steps 600] [--out PATH]
 
 Training data: the batched 64-problem generator (config5_union) with seeds 100 .. 100 + seeds - 1, i.e. seeds x 64
-problems, landmark counts scaled by `scale` (at 1/20 block-Jacobi PCG needs hundreds of iterations, so there is
-something to learn). Each union is reduced by the GPU Schur path; for every problem the shift mu in GRID (the same
-for all its cameras) that reaches ||S x - g|| <= 1e-8 ||g|| in the fewest PCG iterations is the target of its
-cameras (mu = 0, plain block-Jacobi, wins ties). The MLP regresses logit(mu / MU_MAX) from each camera's features
-with the library's forward / backward / Adam kernels. Four further seeds are held out: the output scale alpha in
-ALPHAS with the fewest mean PCG iterations on them is kept (alpha = 0 switches the correction off, i.e. plain
-block-Jacobi when the network does not help).
+problems, landmark counts scaled by `scale` (at 1/20 block-Jacobi PCG needs tens to thousands of iterations, so
+there is something to learn). Each union is reduced by the GPU Schur path; for every problem the group coupling
+weight w in GRID (the same for all its groups of precond.GROUP cameras) that reaches ||S x - g|| <= 1e-8 ||g|| in the fewest PCG iterations is
+the target of its cameras (w = 0, plain block-Jacobi, wins ties). The MLP regresses logit(w) from each camera's
+features with the library's forward / backward / Adam kernels. Four further seeds are held out: the output scale
+alpha in ALPHAS with the fewest mean PCG iterations on them is kept (alpha = 0 switches the correction off, i.e.
+plain block-Jacobi when the network does not help).
 """
 import argparse
 import os
@@ -20,8 +20,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
-GRID = (0.0, 0.005, 0.01, 0.02, 0.05, 0.1)
-ALPHAS = (0.0, 0.25, 0.5, 1.0)
+GRID = (0.0, 0.25, 0.5, 0.75, 1.0)
+ALPHAS = (0.0, 0.5, 1.0)
 
 
 def log(*a):
@@ -52,19 +52,19 @@ def dataset(b, torch, seeds, scale, maxit=20000):
         F = P.features(pcg, S, p.lam)
         x = torch.zeros_like(g)
         its = []
-        for mu in GRID:
-            M = P.shift_blocks(pcg, S, torch.full((p.structure.nrblocks,), mu, dtype=torch.float32, device="cuda"))
+        for w in GRID:
+            M = P.group_blocks(pcg, S, torch.full((p.structure.nrblocks,), w, dtype=torch.float32, device="cuda"))
             x.zero_()
-            ok, iters, _ = pcg.solve(S, g, x, tol=1e-8, maxit=maxit, precond_blocks=M)
+            ok, iters, _ = pcg.solve_groups(S, g, x, M, tol=1e-8, maxit=maxit)
             its.append(np.where(iters < 0, maxit, iters))
         its = np.array(its)                               # (grid, problems)
-        best = np.array(GRID)[np.argmin(its, axis=0)]     # first minimum: the smallest shift
+        best = np.array(GRID)[np.argmin(its, axis=0)]     # first minimum: the smallest weight
         prob_of_block = np.repeat(np.arange(p.nprob), np.diff(p.prob_block_ptr))
         Xs.append(F.cpu().numpy())
         ys.append(P.target_logit(best[prob_of_block]))
         table.append(its)
         log(f"seed {seed}: {p.structure.nrblocks} cameras, iterations plain max {its[0].max()} mean {its[0].mean():.1f}, "
-            f"best-shift mean {its.min(0).mean():.1f} ({time.time() - t0:.1f}s)")
+            f"best-weight mean {its.min(0).mean():.1f} ({time.time() - t0:.1f}s)")
     return np.concatenate(Xs).astype(np.float32), np.concatenate(ys).astype(np.float32), table
 
 

@@ -402,6 +402,18 @@ int bls_solve_diagonal_blocks_f32(bls_diag *h, double *y, const float *D, const 
  * m of its diagonal, log10 m, log(1 + off-diagonal blocks in the block row), log10 lambda, log10(m / lambda),
  * log10(off-diagonal Frobenius mass of the block row / ||D||_F). Blocks of dimension other than 6 get zeros. */
 int bls_pcg_features_f32(bls_pcg *h, const double *S_nzval, double lambda, float *features, void *stream);
+/* The group correction of block-Jacobi (the learned correction's family; no reference counterpart).
+ * bls_pcg_set_groups(h, G) groups the retained blocks of each problem in order, G at a time (G = 1..4; the last group
+ * of a problem may be shorter). The preconditioner block of a group is S's principal block submatrix over its
+ * blocks with every off-diagonal block multiplied by w = omega[b] in [0, 1] (b the group's first block): w = 0 is
+ * plain block-Jacobi, w = 1 the group's exact block of S, SPD for every such w. bls_pcg_group_blocks_size gives the
+ * length of the group blocks, bls_pcg_group_blocks_f64 forms them from S (omega: Float32 per retained block), and
+ * bls_pcg_solve_groups_f64 is bls_pcg_solve_f64 with them as the preconditioner. */
+int bls_pcg_set_groups(bls_pcg *h, int group);
+int bls_pcg_group_blocks_size(bls_pcg *h, int64_t *n);
+int bls_pcg_group_blocks_f64(bls_pcg *h, const double *S_nzval, const float *omega, double *blocks, void *stream);
+int bls_pcg_solve_groups_f64(bls_pcg *h, const double *S_nzval, const double *g, double *x, double tol, int64_t maxit,
+                             const double *group_blocks, int64_t *iters, double *relres, void *stream);
 /* The corrected preconditioner blocks M_b = D_b + mu_b m_b I (mu: nrblocks FP32 shifts, >= 0 keeps M_b SPD when D_b
  * is), full d x d column-major (sum d^2 values): the precond_blocks argument of bls_pcg_solve_f64. */
 int bls_pcg_shift_blocks_f64(bls_pcg *h, const double *S_nzval, const float *mu, double *blocks, void *stream);

@@ -78,3 +78,16 @@ def features(Sd, r_dims, boffsets_r, blocks_in_row, lam):
 
 def shift_block(D, mu):
     return D + mu * (np.trace(D) / D.shape[0]) * np.eye(D.shape[0])
+
+
+def group_block(Sd_group, dims, w):
+    """The group correction's preconditioner block: the principal block submatrix of S over a group of consecutive
+    retained blocks (dims), its off-diagonal blocks multiplied by w (the definition the device's
+    bls_pcg_group_blocks_f64 follows; pinned by construction in the tests: w = 0 gives block-Jacobi's blocks, w = 1
+    the group's exact submatrix of S)."""
+    M = w * np.array(Sd_group, dtype=np.float64)
+    o = 0
+    for d in dims:
+        M[o:o + d, o:o + d] = Sd_group[o:o + d, o:o + d]
+        o += d
+    return M

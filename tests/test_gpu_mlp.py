@@ -126,6 +126,40 @@ def test_features_and_shift_blocks_match_definitions(blsamd):
         o += d * d
 
 
+@pytest.mark.parametrize("group", [2, 4])
+def test_group_blocks_match_definition_and_limits(blsamd, group):
+    """The group correction's blocks against their definition on the dense S (groups of `group` consecutive retained
+    blocks per problem, the last one shorter); w = 0 solves exactly as plain block-Jacobi does (same iterations,
+    same x, bit for bit), w = 1 (the groups' exact blocks of S) needs fewer iterations on these sparse problems."""
+    import torch
+    p, S, g, pcg = _small_reduced(blsamd)
+    st = p.structure
+    Sd = _dense(p, S.cpu().numpy())
+    w = torch.as_tensor(np.random.default_rng(1).random(st.nrblocks).astype(np.float32)).cuda()
+    M = pcg.group_blocks(S, w, group).cpu().numpy()
+    o = 0
+    for q in range(p.nprob):
+        b0, b1 = p.prob_block_ptr[q] - 1, p.prob_block_ptr[q + 1] - 1
+        for b in range(b0, b1, group):
+            e = min(b + group, b1)
+            r0, r1 = st.boffsets_r[b] - 1, st.boffsets_r[e - 1] - 1 + st.r_dims[e - 1]
+            ref = mlp_ref.group_block(Sd[r0:r1, r0:r1], st.r_dims[b:e], float(w[b]))
+            n = ref.shape[0]
+            assert np.allclose(M[o:o + n * n].reshape(n, n).T, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+            o += n * n
+    assert o == M.size
+    x0, x1 = torch.zeros_like(g), torch.zeros_like(g)
+    ok0, it0, _ = pcg.solve(S, g, x0, tol=1e-8)
+    zero = torch.zeros(st.nrblocks, dtype=torch.float32, device="cuda")
+    okz, itz, relz = pcg.solve_groups(S, g, x1, pcg.group_blocks(S, zero, group), tol=1e-8)
+    assert ok0 and okz and np.array_equal(it0, itz) and torch.equal(x0, x1)
+    one = torch.ones(st.nrblocks, dtype=torch.float32, device="cuda")
+    ok1, it1, rel1 = pcg.solve_groups(S, g, x1, pcg.group_blocks(S, one, group), tol=1e-8)
+    assert ok1 and (rel1 <= 1e-8).all() and it1.sum() < it0.sum()
+    gn = g.cpu().numpy()
+    assert np.linalg.norm(Sd @ x1.cpu().numpy() - gn) <= 1.0001e-8 * np.linalg.norm(gn) * np.sqrt(pcg.nprob)
+
+
 def test_learned_solve_and_fallback(blsamd):
     import torch
     P = blsamd.precond
@@ -143,9 +177,9 @@ def test_learned_solve_and_fallback(blsamd):
 
 def test_trained_network_against_plain(blsamd):
     """What the committed network gains over plain block-Jacobi on problems it was not trained or calibrated on (the
-    64-problem generator's own seed, landmarks at 1/20 so that block-Jacobi needs hundreds of iterations): max and
-    mean PCG iterations to 1e-8, printed. The network's output scale was chosen on held-out problems so that it does
-    not lose to plain block-Jacobi there; here it must stay within 3 % of plain on the mean."""
+    64-problem generator's own seed, landmarks at 1/20 so that block-Jacobi needs tens to thousands of iterations):
+    max and mean PCG iterations to 1e-8, printed. The network couples the grouped camera blocks with its weights
+    (output scale chosen on held-out problems); it must save at least 10 % of the mean iterations."""
     import os
     import torch
     P = blsamd.precond
@@ -167,10 +201,7 @@ def test_trained_network_against_plain(blsamd):
     ok0, it0, _, _ = P.solve(pcg, S, g, x, p.lam, model=None, maxit=20000)
     ok1, it1, _, used = P.solve(pcg, S, g, x, p.lam, model=model, maxit=20000)
     assert ok0 and ok1
-    if model.alpha == 0.0:   # an output scale of 0 shifts no block: the solve is the plain one
-        assert not used and np.array_equal(it0, it1)
-    else:
-        assert used
     print(f"plain block-Jacobi iterations max {it0.max()} mean {it0.mean():.1f}; learned max {it1.max()} "
           f"mean {it1.mean():.1f}")
-    assert it1.mean() <= 1.03 * it0.mean()
+    assert model.alpha > 0 and used
+    assert it1.mean() <= 0.9 * it0.mean()
