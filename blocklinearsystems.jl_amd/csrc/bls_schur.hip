@@ -911,6 +911,248 @@ __global__ void k_scale_c(int64_t neb, const int32_t *e_dims, const int64_t *bof
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Blocks above the register path: eliminated blocks of d_e > 8 or retained blocks of d_r > MAXDR. The reference
+// takes its run-time-size route for them (src/schur_reduction.jl:164-225 phase 1, :262-354 phase 2, :578-614
+// backsubstitute). A handle holding any such block reduces its whole instance with these kernels ("wide" mode):
+// phase 1 of every large block by one workgroup (Cholesky, pivot-ratio test and inverse through a global
+// workspace), W = E_inv C per coupling as the reference forms W_data, every stored entry of S by one thread
+// walking its block pair's (c1, c2) list in the reference's order, and g_S by one thread per retained row. Every
+// sum is taken in a fixed order; no atomics on values.
+// ---------------------------------------------------------------------------------------------------------
+constexpr int WIDE_MAXDE = 4096;   // largest eliminated block (backsubstitute keeps t in LDS)
+constexpr int WIDE_THREADS = 256;
+constexpr int64_t WIDE_P1_GRID = 1024;   // workgroups per run of large blocks (each owns 2 d_e^2 of workspace)
+
+struct WidePair {      // one stored block (r1, r2), r1 <= r2, of S with its coupling pairs cc[lo, hi)
+    int32_t r1, r2;
+    int64_t base;      // S index of the block's (row 0, col 0)
+    int64_t stride;    // S index step from one scalar column of the block to the next
+    int64_t lo, hi;
+};
+
+#pragma clang fp contract(off)
+// Phase 1 of `count` blocks of dimension de > 8 (the run at first, E at eoff0 + b de^2, b_e at boff0 + b de):
+// LAPACK potrf!('U') / pivot ratio / potri!('U') of src/schur_reduction.jl:164-225, in T for the Float32 variant
+// (Float64 otherwise). U is formed right-looking: entry (i, j) receives its updates in ascending k, as the
+// column-by-column form subtracts them, and is divided by U_ii last.
+template <class P>
+__global__ __launch_bounds__(WIDE_THREADS) void k_phase1_wide(
+    int64_t first, int64_t count, int de, int64_t eoff0, int64_t boff0, const typename P::TE *__restrict__ E,
+    const double *__restrict__ b_e, const double *__restrict__ scaling_e, double lambda,
+    const double *__restrict__ s_e, typename P::TEI *__restrict__ E_inv, double *__restrict__ z,
+    unsigned long long *flag, typename std::conditional<P::F32_FACTOR, float, double>::type *ws) {
+    using F = typename std::conditional<P::F32_FACTOR, float, double>::type;
+    using TEI = typename P::TEI;
+    const double tol = 4504.0 * (P::F32_FACTOR ? 1.1920928955078125e-07 : 2.220446049250313e-16);
+    const int t = threadIdx.x;
+    const int64_t dd = (int64_t)de * de;
+    F *U = ws + (int64_t)blockIdx.x * 2 * dd, *Ui = U + dd;
+    __shared__ int bad;
+    for (int64_t b = blockIdx.x; b < count; b += gridDim.x) {
+        const int64_t eo = eoff0 + b * dd, bo = boff0 + b * de;
+        for (int64_t q = t; q < dd; q += WIDE_THREADS) {
+            const int i = (int)(q % de), j = (int)(q / de);
+            const double si = s_e ? s_e[bo + i] : 1.0, sj = s_e ? s_e[bo + j] : 1.0;
+            F v = s_e ? (F)((double)E[eo + q] * (si * sj)) : (F)E[eo + q];
+            if (i == j) v = (F)((double)v + lambda * scaling_e[bo + i] * (si * si));
+            U[q] = v;
+        }
+        if (t == 0) bad = 0;
+        __syncthreads();
+        for (int k = 0; k < de; k++) {
+            if (t == 0) {
+                const F a = U[(int64_t)k * de + k];
+                if (!(a > 0)) bad = 1;   // potrf's info != 0
+                else U[(int64_t)k * de + k] = sqrt(a);
+            }
+            __syncthreads();
+            if (bad) break;   // read by every thread after the barrier: uniform
+            const F ukk = U[(int64_t)k * de + k];
+            for (int j = k + 1 + t; j < de; j += WIDE_THREADS) U[(int64_t)j * de + k] = U[(int64_t)j * de + k] / ukk;
+            __syncthreads();
+            const int64_t nt = de - k - 1;
+            for (int64_t q = t; q < nt * nt; q += WIDE_THREADS) {
+                const int64_t i = k + 1 + q % nt, j = k + 1 + q / nt;
+                if (i <= j) U[j * de + i] = U[j * de + i] - U[i * de + k] * U[j * de + k];
+            }
+            __syncthreads();
+        }
+        if (!bad) {
+            // every pivot safely positive against its damped diagonal entry, recomputed in Float64
+            for (int i = t; i < de; i += WIDE_THREADS) {
+                const double si = s_e ? s_e[bo + i] : 1.0;
+                const double hii = (double)E[eo + (int64_t)i * (de + 1)] * (si * si) + lambda * scaling_e[bo + i] * (si * si);
+                const double u = (double)U[(int64_t)i * de + i];
+                const double ratio = (u * u) / hii;
+                if (!(isfinite(ratio) && ratio > tol)) bad = 1;   // every writer stores 1
+            }
+        }
+        __syncthreads();
+        if (bad) {
+            if (t == 0) atomicMin(flag, (unsigned long long)(first + b + 1));
+        } else {
+            // potri: Ui = U^-1 a column per thread, then the upper triangle of Ui Ui', mirrored
+            for (int j = t; j < de; j += WIDE_THREADS) {
+                Ui[(int64_t)j * de + j] = (F)1 / U[(int64_t)j * de + j];
+                for (int i = j - 1; i >= 0; i--) {
+                    F s = 0;
+                    for (int k = i + 1; k <= j; k++) s += U[(int64_t)k * de + i] * Ui[(int64_t)j * de + k];
+                    Ui[(int64_t)j * de + i] = -s / U[(int64_t)i * de + i];
+                }
+            }
+            __syncthreads();
+            for (int64_t q = t; q < dd; q += WIDE_THREADS) {
+                const int i = (int)(q % de), j = (int)(q / de);
+                if (i > j) continue;
+                F s = 0;
+                for (int k = j; k < de; k++) s += Ui[(int64_t)k * de + i] * Ui[(int64_t)k * de + j];
+                E_inv[eo + (int64_t)j * de + i] = (TEI)s;
+                E_inv[eo + (int64_t)i * de + j] = (TEI)s;
+            }
+            __syncthreads();
+            // z_e = E_inv (b_e .* s_e), Float64 accumulation
+            for (int i = t; i < de; i += WIDE_THREADS) {
+                double acc = 0.0;
+                for (int j = 0; j < de; j++)
+                    acc += (double)E_inv[eo + (int64_t)j * de + i] * (b_e[bo + j] * (s_e ? s_e[bo + j] : 1.0));
+                z[bo + i] = acc;
+            }
+        }
+        __syncthreads();   // every thread has read `bad` before the next block resets it
+    }
+}
+
+// W = E_inv C of every coupling (the reference's W_data, src/schur_reduction.jl:210-223), in Float64: one
+// wavefront per coupling, lane q the entries q, q + 64, ... of its d_e x d_r block. C is the (scaled) coupling data.
+template <class TEI, class TCS>
+__global__ __launch_bounds__(WIDE_THREADS) void k_wide_w(int64_t ncoup, const int32_t *coup_e, const int32_t *e_dims,
+                                                        const int64_t *eoffsets, const int64_t *coffsets,
+                                                        const int32_t *entry_retained, const int32_t *r_dims,
+                                                        const TEI *E_inv, const TCS *C, double *W) {
+    const int64_t c = (int64_t)blockIdx.x * (WIDE_THREADS / 64) + threadIdx.x / 64;
+    if (c >= ncoup) return;
+    const int lane = threadIdx.x & 63;
+    const int e = coup_e[c];
+    const int de = e_dims[e], dr = r_dims[entry_retained[c]];
+    const int64_t eo = eoffsets[e], co = coffsets[c];
+    for (int64_t q = lane; q < (int64_t)de * dr; q += 64) {
+        const int i = (int)(q % de);
+        const int64_t col = co + (q - i);
+        double acc = 0.0;
+        for (int j = 0; j < de; j++) acc += (double)E_inv[eo + (int64_t)j * de + i] * (double)C[col + j];
+        W[co + q] = acc;
+    }
+}
+
+// S[r1, r2] -= W_c1' C_c2 over the block pair's coupling pairs, in the order the reference's column sweep visits
+// them (src/schur_reduction.jl:262-354): one workgroup per stored block, one thread per entry (upper triangle of
+// a diagonal block). Nothing is written when phase 1 failed.
+template <class TS, class TCS>
+__global__ __launch_bounds__(WIDE_THREADS) void k_phase2_wide(const WidePair *pairs, const int2 *cc,
+                                                             const int32_t *coup_e, const int32_t *e_dims,
+                                                             const int64_t *coffsets, const int32_t *r_dims,
+                                                             const double *W, const TCS *C, TS *S,
+                                                             const unsigned long long *flag) {
+    if (*flag != (unsigned long long)NO_FAILURE) return;
+    const WidePair p = pairs[blockIdx.x];
+    const int dr1 = r_dims[p.r1], dr2 = r_dims[p.r2];
+    for (int64_t q = threadIdx.x; q < (int64_t)dr1 * dr2; q += WIDE_THREADS) {
+        const int row = (int)(q % dr1), col = (int)(q / dr1);
+        if (p.r1 == p.r2 && row > col) continue;
+        const int64_t idx = p.base + (int64_t)col * p.stride + row;
+        double v = (double)S[idx];
+        for (int64_t k = p.lo; k < p.hi; k++) {
+            const int2 pc = cc[k];
+            const int de = e_dims[coup_e[pc.x]];
+            const double *w = W + coffsets[pc.x] + (int64_t)row * de;
+            const TCS *cs = C + coffsets[pc.y] + (int64_t)col * de;
+            double acc = 0.0;
+            for (int i = 0; i < de; i++) acc += w[i] * (double)cs[i];
+            v -= acc;
+        }
+        S[idx] = (TS)v;
+    }
+}
+
+// g_S = b_r .* s_r - C' z over each retained block's couplings in the reference's order: one workgroup per
+// retained block, one thread per row.
+template <class TCS>
+__global__ __launch_bounds__(WIDE_THREADS) void k_g_wide(const int64_t *retained_ptr, const int2 *rm_ce,
+                                                        const int32_t *e_dims, const int64_t *boffsets_e,
+                                                        const int64_t *coffsets, const int32_t *r_dims,
+                                                        const int64_t *boffsets_r, const TCS *C, const double *z,
+                                                        const double *b_r, const double *s_r, double *g,
+                                                        const unsigned long long *flag) {
+    if (*flag != (unsigned long long)NO_FAILURE) return;
+    const int64_t r = blockIdx.x;
+    const int dr = r_dims[r];
+    const int64_t ro = boffsets_r[r];
+    for (int ir = threadIdx.x; ir < dr; ir += WIDE_THREADS) {
+        double v = b_r[ro + ir] * (s_r ? s_r[ro + ir] : 1.0);
+        for (int64_t k = retained_ptr[r]; k < retained_ptr[r + 1]; k++) {
+            const int2 ce = rm_ce[k];
+            const int de = e_dims[ce.y];
+            const int64_t bo = boffsets_e[ce.y];
+            const TCS *cs = C + coffsets[ce.x] + (int64_t)ir * de;
+            double acc = 0.0;
+            for (int i = 0; i < de; i++) acc += (double)cs[i] * z[bo + i];
+            v -= acc;
+        }
+        g[ro + ir] = v;
+    }
+}
+
+// backsubstitute! for any block size: one workgroup per eliminated block, t = sum_c C_c x_r in LDS (Float64), then
+// x_e = z - E_inv t; blocks above 8 subtract term by term as the reference's run-time-size loop does
+// (src/schur_reduction.jl:578-614), smaller ones subtract the product once (its static path).
+template <class TC, class TEI, class TSC>
+__global__ __launch_bounds__(WIDE_THREADS) void k_backsub_wide(int64_t neb, const int32_t *e_dims,
+                                                              const int64_t *eoffsets, const int64_t *boffsets_e,
+                                                              const int64_t *eliminated_ptr,
+                                                              const int32_t *entry_retained, const int64_t *coffsets,
+                                                              const int32_t *r_dims, const int64_t *boffsets_r,
+                                                              const TC *C, const TEI *E_inv, const double *z,
+                                                              const double *x_r, const double *s_r,
+                                                              const double *s_e, double *x_e) {
+    extern __shared__ double tsh[];
+    for (int64_t e = blockIdx.x; e < neb; e += gridDim.x) {
+        const int de = e_dims[e];
+        const int64_t eo = eoffsets[e], bo = boffsets_e[e];
+        for (int i = threadIdx.x; i < de; i += WIDE_THREADS) {
+            const double se = s_e ? s_e[bo + i] : 1.0;
+            double t = 0.0;
+            for (int64_t c = eliminated_ptr[e]; c < eliminated_ptr[e + 1]; c++) {
+                const int r = entry_retained[c];
+                const int dr = r_dims[r];
+                const int64_t ro = boffsets_r[r], co = coffsets[c];
+                for (int ir = 0; ir < dr; ir++) {
+                    const double f = s_r ? s_r[ro + ir] : 1.0;
+                    const double cv = (s_r || s_e) ? (double)(TSC)((double)C[co + (int64_t)ir * de + i] * (se * f))
+                                                   : (double)C[co + (int64_t)ir * de + i];
+                    t += cv * x_r[ro + ir];
+                }
+            }
+            tsh[i] = t;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < de; i += WIDE_THREADS) {
+            double acc;
+            if (de > 8) {
+                acc = z[bo + i];
+                for (int j = 0; j < de; j++) acc -= (double)E_inv[eo + (int64_t)j * de + i] * tsh[j];
+            } else {
+                double m = 0.0;
+                for (int j = 0; j < de; j++) m += (double)E_inv[eo + (int64_t)j * de + i] * tsh[j];
+                acc = z[bo + i] - m;
+            }
+            x_e[bo + i] = acc;
+        }
+        __syncthreads();
+    }
+}
+#pragma clang fp contract(fast)
+
 // The retained rows' part of the full system's residual: out[r] = (b_r .* s_r)[r] - sum over the couplings c of
 // retained block r (ascending eliminated block) of scaled(C_c)' x_e[e(c)] -- with x_e = backsubstitute!(x_r)
 // this is g_S - (S - R) x_r of the reduction; with x_e = z it is g_S itself. One workgroup per retained block;
@@ -1138,6 +1380,8 @@ __global__ void k_syrk_sum(const T *partial, int nslices, int ntiles, const int2
 
 struct Run {
     int64_t first, count, de, eoff0, boff0;
+    int64_t wsoff = -1;   // d_e > 8: offset (doubles) of the run's phase-1 workspace in d_wide_ws
+    int64_t wsgrid = 0;   // ... and its workgroups
 };
 
 }  // namespace
@@ -1184,19 +1428,86 @@ struct bls_schur {
     // layout it is rebuilt from (0-based block pattern, 1-based S_colptr, col_heights)
     int64_t p2_part = P2_PART, p2_lanes = P2_BASE_LANES;
     std::vector<int64_t> lay_bcp, lay_brv, lay_bro, lay_Scp, lay_ch;
+    // blocks above the register path (d_e > 8 or d_r > MAXDR): the whole instance takes the run-time-size kernels
+    bool wide = false;
+    int wide_maxde = 0;
+    DevArray<double> d_wide_ws;      // phase-1 workspace of the large blocks (U and U^-1 per workgroup)
+    DevArray<double> d_W;            // W = E_inv C per coupling
+    DevArray<int32_t> d_coup_e;      // eliminated block of each coupling
+    DevArray<WidePair> d_wpairs;     // stored blocks of S with coupling pairs
+    DevArray<int2> d_wcc;            // their (c1, c2) lists
+    int64_t nwpairs = 0;
 };
 
 namespace {
 
-int check_structure(const bls_schur *h) {
-    for (int64_t e = 0; e < h->neb; e++)
-        if (h->e_dims[e] < 1 || h->e_dims[e] > 8)
+int check_structure(bls_schur *h) {
+    for (int64_t e = 0; e < h->neb; e++) {
+        if (h->e_dims[e] < 1 || h->e_dims[e] > WIDE_MAXDE)
             return fail(BLS_ERR_UNSUPPORTED, "eliminated block " + std::to_string(e + 1) + " has dimension " +
-                                                 std::to_string(h->e_dims[e]) + "; the device path supports 1 to 8");
-    for (int64_t r = 0; r < h->nrb; r++)
-        if (h->r_dims[r] < 1 || h->r_dims[r] > MAXDR)
-            return fail(BLS_ERR_UNSUPPORTED, "retained block " + std::to_string(r + 1) + " has dimension " +
-                                                 std::to_string(h->r_dims[r]) + "; the device path supports 1 to 16");
+                                                 std::to_string(h->e_dims[e]) + "; the device path supports 1 to " +
+                                                 std::to_string(WIDE_MAXDE));
+        if (h->e_dims[e] > 8) h->wide = true;
+        h->wide_maxde = std::max(h->wide_maxde, (int)h->e_dims[e]);
+    }
+    for (int64_t r = 0; r < h->nrb; r++) {
+        if (h->r_dims[r] < 1 || h->r_dims[r] > INT32_MAX / 2)
+            return fail(BLS_ERR_ARG, "retained block " + std::to_string(r + 1) + " has dimension " +
+                                         std::to_string(h->r_dims[r]));
+        if (h->r_dims[r] > MAXDR) h->wide = true;
+    }
+    return BLS_OK;
+}
+
+// Wide mode: the stored blocks of S that coupling pairs reach, each with its (c1, c2) list in the order of the
+// reference's column sweep (src/schur_reduction.jl:300-312): block column r2 ascending, its couplings c2 in
+// retained_entry order, and for each the couplings c1 <= c2 of the same eliminated block. `locate(r1, r2, base,
+// stride)` gives a block's addressing in S, false when the pattern does not store it.
+template <typename Locate>
+int build_wide_pairs(bls_schur *h, Locate locate) {
+    struct Key {
+        int64_t key, seq;
+        int32_t c1, c2;
+    };
+    std::vector<int32_t> coup_e((size_t)h->ncoup);
+    for (int64_t e = 0; e < h->neb; e++)
+        for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++) coup_e[c] = (int32_t)e;
+    std::vector<Key> v;
+    int64_t seq = 0;
+    for (int64_t r2 = 0; r2 < h->nrb; r2++)
+        for (int64_t k = h->retained_ptr[r2]; k < h->retained_ptr[r2 + 1]; k++) {
+            const int64_t c2 = h->retained_entry[k], e = coup_e[c2];
+            for (int64_t c1 = h->eliminated_ptr[e]; c1 <= c2; c1++) {
+                const int64_t r1 = h->entry_retained[c1];
+                if (r1 > r2) return fail(BLS_ERR_ARG, "couplings of an eliminated block are not in ascending retained order");
+                v.push_back(Key{r2 * h->nrb + r1, seq++, (int32_t)c1, (int32_t)c2});
+            }
+        }
+    std::sort(v.begin(), v.end(), [](const Key &a, const Key &b) { return a.key != b.key ? a.key < b.key : a.seq < b.seq; });
+    std::vector<WidePair> pairs;
+    std::vector<int2> cc(v.size());
+    for (size_t i = 0; i < v.size();) {
+        size_t j = i;
+        while (j < v.size() && v[j].key == v[i].key) j++;
+        WidePair p{};
+        p.r2 = (int32_t)(v[i].key / h->nrb);
+        p.r1 = (int32_t)(v[i].key % h->nrb);
+        if (!locate(p.r1, p.r2, p.base, p.stride))
+            return fail(BLS_ERR_ARG, "the Schur pattern has no block (" + std::to_string(p.r1 + 1) + ", " +
+                                         std::to_string(p.r2 + 1) + ")");
+        p.lo = (int64_t)i;
+        p.hi = (int64_t)j;
+        pairs.push_back(p);
+        for (size_t q = i; q < j; q++) cc[q] = make_int2(v[q].c1, v[q].c2);
+        i = j;
+    }
+    int rc;
+    if ((rc = h->d_wpairs.upload(pairs)) || (rc = h->d_wcc.upload(cc)) || (rc = h->d_coup_e.upload(coup_e)) ||
+        (rc = h->d_W.alloc((size_t)h->clen)))
+        return rc;
+    h->nwpairs = (int64_t)pairs.size();
+    h->nitems = 0;
+    h->nmerges = 0;
     return BLS_OK;
 }
 
@@ -1395,7 +1706,14 @@ int phase1(bls_schur *h, const typename P::TE *E, const double *b_e, const doubl
         case 6: launch_phase1<6, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
         case 7: launch_phase1<7, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
         case 8: launch_phase1<8, P>(r, E, b_e, sc, lambda, s_e, E_inv, z, flag, s); break;
-        default: return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension above 8");
+        default: {
+            using F = typename std::conditional<P::F32_FACTOR, float, double>::type;
+            if (r.wsoff < 0) return fail(BLS_ERR_STATE, "no phase-1 workspace for blocks above 8");
+            k_phase1_wide<P><<<(unsigned)r.wsgrid, WIDE_THREADS, 0, s>>>(
+                r.first, r.count, (int)r.de, r.eoff0, r.boff0, E, b_e, sc, lambda, s_e, E_inv, z, flag,
+                reinterpret_cast<F *>(h->d_wide_ws.ptr + r.wsoff));
+            break;
+        }
         }
     }
     BLS_HIP(hipGetLastError());
@@ -1455,6 +1773,28 @@ void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, con
     h->timing.mark(4, s);
 }
 
+// Wide mode's W, S and g_S (every block size): see k_phase2_wide.
+template <class Q>
+void run_phase2_wide(bls_schur *h, const typename Q::TEI *E_inv, const double *z, const typename Q::TC *C,
+                     const double *b_r, const double *s_r, typename Q::TS *S, double *g, const P2Scratch &ws,
+                     hipStream_t s) {
+    using TC = typename Q::TC;
+    if (h->ncoup)
+        k_wide_w<typename Q::TEI, TC><<<grid_for(h->ncoup, WIDE_THREADS / 64), WIDE_THREADS, 0, s>>>(
+            h->ncoup, h->d_coup_e.ptr, h->d_e_dims.ptr, h->d_eoffsets.ptr, h->d_coffsets.ptr, h->d_entry_retained.ptr,
+            h->d_r_dims.ptr, E_inv, C, h->d_W.ptr);
+    if (h->nwpairs)
+        k_phase2_wide<typename Q::TS, TC><<<(unsigned)h->nwpairs, WIDE_THREADS, 0, s>>>(
+            h->d_wpairs.ptr, h->d_wcc.ptr, h->d_coup_e.ptr, h->d_e_dims.ptr, h->d_coffsets.ptr, h->d_r_dims.ptr,
+            h->d_W.ptr, C, S, ws.flag);
+    h->timing.mark(3, s);
+    if (h->nrb)
+        k_g_wide<TC><<<(unsigned)h->nrb, WIDE_THREADS, 0, s>>>(h->d_retained_ptr.ptr, h->d_rm_ce.ptr, h->d_e_dims.ptr,
+                                                              h->d_boffsets_e.ptr, h->d_coffsets.ptr, h->d_r_dims.ptr,
+                                                              h->d_boffsets_r.ptr, C, z, b_r, s_r, g, ws.flag);
+    h->timing.mark(4, s);
+}
+
 template <class P>
 int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, const double *b_r, const double *b_e,
                  const double *scaling_e, double lambda, const typename P::TE *E, const typename P::TC *C,
@@ -1487,10 +1827,12 @@ int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, cons
                                                                   h->d_coffsets.ptr, h->d_r_dims.ptr,
                                                                   h->d_boffsets_r.ptr, C, s_r, s_e, Cs);
         h->timing.mark(2, s);
-        run_phase2<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, ws, s);
+        if (h->wide) run_phase2_wide<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, ws, s);
+        else run_phase2<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, ws, s);
     } else {
         h->timing.mark(2, s);
-        run_phase2<P>(h, E_inv, z, C, b_r, s_r, S, g, ws, s);
+        if (h->wide) run_phase2_wide<P>(h, E_inv, z, C, b_r, s_r, S, g, ws, s);
+        else run_phase2<P>(h, E_inv, z, C, b_r, s_r, S, g, ws, s);
     }
     k_finalize_flag<<<1, 1, 0, s>>>(ws.flag, d_failed);
     BLS_HIP(hipGetLastError());
@@ -1528,6 +1870,16 @@ namespace {
 int build_sparse_phase2(bls_schur *h) {
     const std::vector<int64_t> &bcp = h->lay_bcp, &brv = h->lay_brv, &bro = h->lay_bro, &Scp = h->lay_Scp,
                                &ch = h->lay_ch;
+    if (h->wide)
+        return build_wide_pairs(h, [&](int64_t r1, int64_t r2, int64_t &base, int64_t &stride) {
+            for (int64_t k = bcp[r2]; k < bcp[r2 + 1]; k++)
+                if (brv[k] == r1) {
+                    base = Scp[h->boffsets_r[r2]] - 1 + bro[k];
+                    stride = ch[r2];
+                    return true;
+                }
+            return false;
+        });
     return build_phase2(
         h, [&](int64_t r2, std::vector<int64_t> &col) { col.assign(brv.begin() + bcp[r2], brv.begin() + bcp[r2 + 1]); },
         [&](int64_t r2, size_t k, int64_t) { return bro[bcp[r2] + (int64_t)k]; },
@@ -1538,6 +1890,12 @@ int build_sparse_phase2(bls_schur *h) {
 }
 
 int build_dense_phase2(bls_schur *h) {
+    if (h->wide)
+        return build_wide_pairs(h, [&](int64_t r1, int64_t r2, int64_t &base, int64_t &stride) {
+            base = h->boffsets_r[r2] * h->nr + h->boffsets_r[r1];
+            stride = h->nr;
+            return true;
+        });
     return build_phase2(
         h, [&](int64_t r2, std::vector<int64_t> &col) { for (int64_t r = 0; r <= r2; r++) col.push_back(r); },
         [&](int64_t, size_t, int64_t r1) { return h->boffsets_r[r1]; },
@@ -1587,6 +1945,19 @@ extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, 
         if (h->e_dims[f] != h->uniform_de) h->uniform_de = 0;
     }
     if (h->uniform_de > 3) h->uniform_de = 0;
+    {
+        int64_t wsn = 0;
+        for (Run &r : h->runs)
+            if (r.de > 8) {
+                r.wsgrid = std::min<int64_t>(r.count, WIDE_P1_GRID);
+                r.wsoff = wsn;
+                wsn += 2 * r.de * r.de * r.wsgrid;
+            }
+        if (wsn && (rc = h->d_wide_ws.alloc((size_t)wsn))) {
+            delete h;
+            return rc;
+        }
+    }
     h->uniform_dr_all = nrblocks ? (int)h->r_dims[0] : 0;
     for (int64_t r = 0; r < nrblocks; r++) h->max_dr = std::max(h->max_dr, (int)h->r_dims[r]);
     for (int64_t r = 0; r < nrblocks; r++)
@@ -1713,7 +2084,13 @@ int backsub(bls_schur *h, double *x_e, const double *x_r, const TC *C_data, cons
             const double *s_r, const double *s_e, hipStream_t s) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (h->neb && (!x_e || !x_r || !C_data || !E_inv || !z)) return fail(BLS_ERR_ARG, "backsubstitute: a required array is NULL");
-    if (h->neb)
+    if (h->neb && h->wide)
+        k_backsub_wide<TC, TEI, TSC><<<(unsigned)std::min<int64_t>(h->neb, 8192), WIDE_THREADS,
+                                       (size_t)h->wide_maxde * sizeof(double), s>>>(
+            h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr, h->d_boffsets_e.ptr, h->d_eliminated_ptr.ptr,
+            h->d_entry_retained.ptr, h->d_coffsets.ptr, h->d_r_dims.ptr, h->d_boffsets_r.ptr, C_data, E_inv, z, x_r, s_r,
+            s_e, x_e);
+    else if (h->neb)
         k_backsub<TC, TEI, TSC><<<grid_for(h->neb, 256 / BS_LANES), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr,
                                                                   h->d_boffsets_e.ptr, h->d_eliminated_ptr.ptr,
                                                                   h->d_entry_retained.ptr, h->d_coffsets.ptr,
@@ -1797,6 +2174,8 @@ int retained_residual(bls_schur *h, double *out, const double *b_r, const double
                       const double *s_r, const double *s_e, hipStream_t s) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
     if (!out || !b_r || (h->neb && (!x_e || !C))) return fail(BLS_ERR_ARG, "retained residual: a required array is NULL");
+    if (h->wide)
+        return fail(BLS_ERR_UNSUPPORTED, "retained residual: blocks up to d_e 8 and d_r " + std::to_string(MAXDR));
     const bool uni = h->uniform_de == 3 && h->uniform_dr_all == 6 && h->runs.size() == 1;
     if (h->nrb && uni)
         k_ct_xe<TC, 3, 6><<<(unsigned)h->nrb, CTX_THREADS, 0, s>>>(h->d_retained_ptr.ptr, h->d_rm_ce.ptr, h->d_e_dims.ptr,
@@ -1836,8 +2215,8 @@ int reduce_dense_mfma(bls_schur *h, double *S, double *g, const double *b_r, con
         return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
     if (!S || !g || !b_r || !sc || !E || !C || !E_inv || !z || (h->neb && !b_e))
         return fail(BLS_ERR_ARG, "reduce_schur_dense (MFMA): a required array is NULL");
-    for (int64_t e = 0; e < h->neb; e++)
-        if (h->e_dims[e] > 8) return fail(BLS_ERR_UNSUPPORTED, "MFMA dense reduction: eliminated blocks up to 8");
+    if (h->wide)
+        return fail(BLS_ERR_UNSUPPORTED, "MFMA dense reduction: blocks up to d_e 8 and d_r " + std::to_string(MAXDR));
     // phase 1 (the reference's landmark factorisation and acceptance), then its verdict
     k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
     int rc = phase1<PrecF64>(h, E, b_e, sc, lambda, nullptr, E_inv, z, h->d_flag.ptr, s);

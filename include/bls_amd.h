@@ -52,8 +52,10 @@ int bls_stream_synchronize(void *stream);
 /* Register a SchurStructure (src/schur_structure.jl:50-68) with the retained-block-major index of its
  * SchurReductionWorkspace (src/schur_reduction.jl:51-92). r_dims / e_dims: the block sizes (collect of the
  * BlockRLEs); boffsets_r, boffsets_e, eoffsets, eliminated_ptr, entry_retained, coffsets: the structure's fields;
- * retained_ptr, retained_entry: the workspace's. Eliminated blocks must have d_e <= 8 (the reference's static
- * path, src/kernels.jl:88) and retained blocks d_r <= 16; other sizes return BLS_ERR_UNSUPPORTED. */
+ * retained_ptr, retained_entry: the workspace's. Eliminated blocks of d_e <= 8 (the reference's static path,
+ * src/kernels.jl:88) with retained blocks of d_r <= 16 take the register kernels; an instance holding any larger
+ * block (d_e up to 4096, any d_r) is reduced whole by the run-time-size kernels, as the reference's dynamic path
+ * (src/schur_reduction.jl:164-225, :578-614) treats such blocks. d_e above 4096 returns BLS_ERR_UNSUPPORTED. */
 int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, int64_t ncoup, const int64_t *r_dims,
                      const int64_t *e_dims, const int64_t *boffsets_r, const int64_t *boffsets_e,
                      const int64_t *eoffsets, const int64_t *eliminated_ptr, const int64_t *entry_retained,
@@ -110,7 +112,8 @@ int bls_reduce_schur_dense_f64_async(bls_schur *h, double *S, double *g_S, const
  * and FP64 MFMA. _mixed: Float64 E (the landmark factorisation stays Float64), Float32 couplings C -- the bulk of
  * the data --, Y rounded to Float32 and FP32 MFMA with Float32 accumulation per tile and slice (the FP32-MFMA SYRK
  * with Float64 cross-tile accumulation). Holds Y and
- * the partial tiles in the handle (allocated at the first call). Synchronizes. */
+ * the partial tiles in the handle (allocated at the first call). Synchronizes. Blocks up to d_e 8 and d_r 16
+ * (BLS_ERR_UNSUPPORTED otherwise). */
 int bls_reduce_schur_dense_mfma_f64(bls_schur *h, double *S, double *g_S, const double *b_r, const double *b_e,
                                     const double *scaling_e, double lambda, const double *E_data, const double *C_data,
                                     double *E_inv, double *z, int64_t nslices, int64_t *failed_block, void *stream);
@@ -124,7 +127,8 @@ int bls_schur_factor_f64(bls_schur *h, const double *E_data, const double *b_e, 
 /* The retained rows of the full system's residual after a back-substitution: out = b_r .* s_r - sum_c scaled(C_c)'
  * x_e[e(c)] per retained block, its couplings in ascending eliminated block (fixed-order sums, no atomics). With
  * x_e = z it is g_S of the reduction; with x_e = backsubstitute!(x_r) it is g_S - (S_red - R_damped) x_r. No
- * reference counterpart (the Float64 iterative refinement of the mixed reduction); C in Float64 or Float32. */
+ * reference counterpart (the Float64 iterative refinement of the mixed reduction); C in Float64 or Float32. Blocks
+ * up to d_e 8 and d_r 16 (BLS_ERR_UNSUPPORTED otherwise). */
 int bls_schur_retained_residual_f64(bls_schur *h, double *out, const double *b_r, const double *x_e,
                                     const double *C_data, const double *s_r, const double *s_e, void *stream);
 int bls_schur_retained_residual_f32(bls_schur *h, double *out, const double *b_r, const double *x_e,

@@ -433,9 +433,15 @@ void FN(orc_backsubstitute_schur)(const orc_schur *st, double *x_e, const double
             }
         }
         for (int64_t i = 0; i < de; i++) {
-            double acc = 0.0; /* E_inv * t, then z - that (static path: SMatrix * SVector) */
-            for (int64_t j = 0; j < de; j++) acc += (double)A1(E_inv, eo + j * de + i) * t[j];
-            A1(x_e, e_off + i) = A1(z, e_off + i) - acc;
+            if (de > 8) { /* run-time-size path (src/schur_reduction.jl:578-614): z - each term in turn */
+                double acc = A1(z, e_off + i);
+                for (int64_t j = 0; j < de; j++) acc -= (double)A1(E_inv, eo + j * de + i) * t[j];
+                A1(x_e, e_off + i) = acc;
+            } else {
+                double acc = 0.0; /* E_inv * t, then z - that (static path: SMatrix * SVector) */
+                for (int64_t j = 0; j < de; j++) acc += (double)A1(E_inv, eo + j * de + i) * t[j];
+                A1(x_e, e_off + i) = A1(z, e_off + i) - acc;
+            }
         }
     }
 }

@@ -150,7 +150,7 @@ def test_full_solve_equals_dense_cholesky(blsamd):
         assert np.linalg.norm(x - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
 
 
-@pytest.mark.parametrize("de", [1, 2, 3, 4, 5, 7, 8, 9, 12])
+@pytest.mark.parametrize("de", [1, 2, 3, 4, 5, 7, 8, 9, 12, 300])
 def test_eliminated_sizes_full_solve(blsamd, de):
     """test/schur_problems.jl:13-41: d_e in {1,3,4,5,7,8,9,...}, static and dynamic (d_e > 8) paths."""
     rng = np.random.default_rng(de)
@@ -166,6 +166,25 @@ def test_eliminated_sizes_full_solve(blsamd, de):
     oracle.backsubstitute_schur(res["args"], x_e, x_r, p.C_data, res["E_inv"], res["z"])
     x_ref = np.linalg.solve(H, b)
     assert np.linalg.norm(np.concatenate([x_r, x_e]) - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+
+
+@pytest.mark.parametrize("dense", [True, False])
+def test_wide_retained_and_eliminated_full_solve(blsamd, dense):
+    """test/schur_problems.jl:36-41 (R9): a retained block of 260 and an eliminated block of 300; the oracle's
+    reduction + backsubstitute! equal the dense solve of (H + 0.2 I) x = b."""
+    rng = np.random.default_rng(256)
+    cpl = [[1, 2], [2]]
+    H, b = schur_test_problem(rng, [6, 260], [300, 4], cpl)
+    p = problem_from_dense(blsamd, H, b, [6, 260], [300, 4], cpl, 0.2, dense)
+    res = oracle.schur_reduce_problem(p)
+    assert res["ok"]
+    S = _S_dense(p, res["S"])
+    S = np.triu(S) + np.triu(S, 1).T
+    x_r = np.linalg.solve(S, res["g"])
+    x_e = np.zeros(p.structure.ne)
+    oracle.backsubstitute_schur(res["args"], x_e, x_r, p.C_data, res["E_inv"], res["z"])
+    x_ref = np.linalg.solve(H + 0.2 * np.eye(len(b)), b)
+    assert np.linalg.norm(np.concatenate([x_r, x_e]) - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
 
 
 def test_inv_sym_block_rejects_not_pd():
