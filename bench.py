@@ -55,6 +55,9 @@ def main():
                     help="landmark scale of the reduce_qr! leg (500 cameras d 6/9, 500000 landmarks x scale); 0 skips")
     ap.add_argument("--config3-scale", type=float, default=1.0,
                     help="landmark scale of the dense leg (256 cameras, 1000000 landmarks x scale); 0 skips it")
+    ap.add_argument("--config3-valu-scale", type=float, default=0.1,
+                    help="fraction of the dense leg's landmarks at which the VALU kernel and the CPU oracle run "
+                         "(their host work list and the oracle are slow at the full size)")
     ap.add_argument("--config5-scale", type=float, default=1.0,
                     help="landmark scale of the batched 64-problem leg (cameras at full size); 0 skips it")
     ap.add_argument("--tuning", default="",
@@ -78,7 +81,8 @@ def main():
     dev = b.device
     torch.cuda.set_device(0)
     if args.config3_only:
-        print(json.dumps({"dense_reduction": config3_leg(torch, dev, b, args.config3_scale or 1.0, cpu=False)}),
+        print(json.dumps({"dense_reduction": config3_leg(torch, dev, b, args.config3_scale or 1.0,
+                                                         valu_scale=args.config3_valu_scale, cpu=False)}),
               flush=True)
         return
     if args.calibration_copy:
@@ -231,7 +235,8 @@ def main():
         result["reduce_qr"] = qr_leg(torch, dev, b, args.qr_scale)
         torch.cuda.empty_cache()
     if args.config3_scale > 0:
-        result["dense_reduction"] = config3_leg(torch, dev, b, args.config3_scale, cpu=not args.no_cpu)
+        result["dense_reduction"] = config3_leg(torch, dev, b, args.config3_scale, valu_scale=args.config3_valu_scale,
+                                                cpu=not args.no_cpu)
         torch.cuda.empty_cache()
     if args.config5_scale > 0:
         result["batched_pcg"] = batched_pcg_leg(torch, dev, b, args.config5_scale, check_parity=not args.no_cpu)
