@@ -16,7 +16,8 @@ using namespace bls;
 
 namespace {
 
-constexpr int QR_MAXDE = 8;
+constexpr int QR_MAXDE = 8;        // largest d_e of the register kernels
+constexpr int QRW_MAXDE = 4096;    // largest d_e of the workgroup kernels (backsubstitute's t in LDS)
 
 // T: element type of the raw and factored tables (SchurBlockJacobian{T}); the residual is Float64 and the
 // reflectors are formed and applied in Float64 registers for either T, with the reference's singularity test
@@ -249,6 +250,106 @@ __global__ __launch_bounds__(256) void k_qr_tables(QRArgs<T> a) {
         }
     }
     if ((__ballot(isnan(bad)) & gmask) && l == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
+}
+
+// Tables the register kernel does not take (more than 64 rows, or d_e > QR_MAXDE): one workgroup per table, the
+// table and its residual staged in a Float64 workspace (column-major, M x (C + 1)), the reflectors of qr_table!
+// (src/qr_reduction.jl:50-131) formed by workgroup reductions (a fixed butterfly per wavefront, then the four
+// wavefronts in order) and applied one column per wavefront. Same values, test and singularity rule as k_qr_tables.
+constexpr int QRW_THREADS = 256;
+template <class T>
+__global__ __launch_bounds__(QRW_THREADS) void k_qr_table_wide(QRArgs<T> a, double *ws, int64_t wsz) {
+    __shared__ double red[2][QRW_THREADS / 64];
+    __shared__ int bad_sh;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    double *X = ws + (int64_t)blockIdx.x * wsz;
+    for (int64_t ti = blockIdx.x; ti < a.ntab; ti += gridDim.x) {
+        const int t = a.tables[ti];
+        const int M = a.nrows[t], C = a.ncols[t], de = a.e_dims[t];
+        const int64_t off = a.raw_off[t], rq = a.res_off[t], so = a.boff_e[t];
+        const int me = M - de, ce = C - de;
+        if (tid == 0) bad_sh = 0;
+        __syncthreads();
+        double bad = 0.0;
+        // stage: retained columns (scaled), eliminated columns with the damping rows sqrt(lambda sc) s_e on their
+        // diagonal, the residual; raw rows below me are the damping rows and read as zero
+        for (int j = w; j <= C; j += QRW_THREADS / 64) {
+            double f = 1.0;
+            if (j < ce && a.cs) f = column_scale(a, t, j);
+            else if (j >= ce && j < C && a.cs) f = a.cs[a.nr + so + (j - ce)];
+            double *Y = X + (int64_t)j * M;
+            for (int i = lane; i < M; i += 64) {
+                double v = 0.0;
+                if (j < C) {
+                    if (i < me) {
+                        const double x = (double)a.Rw[off + (int64_t)j * M + i];
+                        bad += x - x;
+                        v = (j >= ce || a.cs) ? (double)(T)(x * f) : x;
+                    } else if (j >= ce && i == me + (j - ce)) {
+                        v = (double)(T)(sqrt(a.lambda * a.scaling_e[so + (j - ce)]) * f);
+                    }
+                } else if (i < me) {
+                    v = a.Rr[rq + i];
+                    bad += v - v;
+                }
+                Y[i] = v;
+            }
+        }
+        if (isnan(bad)) bad_sh = 1;   // every writer stores 1
+        __syncthreads();
+        bool failed = bad_sh != 0;
+        const double tol = (double)M * (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07);
+        for (int k = 0; k < de && !failed; k++) {
+            double *P = X + (int64_t)(ce + k) * M;
+            double pt = 0.0, ps = 0.0;
+            for (int i = tid; i < M; i += QRW_THREADS) {
+                const double x = P[i];
+                if (i < k) pt += x * x;
+                else ps += x * x;
+            }
+            pt = wave_sum(pt);
+            ps = wave_sum(ps);
+            if (lane == 0) { red[0][w] = pt; red[1][w] = ps; }
+            __syncthreads();
+            double s_top = 0.0, s_sub = 0.0;
+            for (int q = 0; q < QRW_THREADS / 64; q++) { s_top += red[0][q]; s_sub += red[1][q]; }
+            const double nrm = sqrt(s_sub), full = sqrt(s_top + s_sub);
+            const double akk = P[k];
+            const double beta = akk >= 0.0 ? -nrm : nrm;
+            __syncthreads();   // every thread has read red and P[k]
+            if (!(isfinite(beta) && fabs(beta) > tol * full)) { failed = true; break; }   // uniform
+            const double scale = 1.0 / (akk - beta), tk = (beta - akk) / beta;
+            for (int i = k + 1 + tid; i < M; i += QRW_THREADS) P[i] *= scale;   // v_k below row k (v_k[k] = 1)
+            if (tid == 0) P[k] = beta;
+            __syncthreads();
+            // H_k on every other column: the later eliminated ones, the retained ones, the residual
+            for (int j = w; j <= C; j += QRW_THREADS / 64) {
+                if (j >= ce && j <= ce + k) continue;
+                double *Y = X + (int64_t)j * M;
+                const double yk = Y[k];
+                double d = 0.0;
+                for (int i = k + 1 + lane; i < M; i += 64) d += P[i] * Y[i];
+                const double sj = (wave_sum(d) + yk) * tk;
+                for (int i = k + 1 + lane; i < M; i += 64) Y[i] -= sj * P[i];
+                if (lane == 0) Y[k] = yk - sj;
+            }
+            __syncthreads();
+        }
+        if (failed) {
+            if (tid == 0) atomicMin(a.flag, (unsigned long long)(t + 1));
+        } else {
+            for (int j = w; j <= C; j += QRW_THREADS / 64) {
+                const double *Y = X + (int64_t)j * M;
+                const int k = j - ce;   // eliminated column k: U above and on the diagonal, zeros below
+                for (int i = lane; i < M; i += 64) {
+                    if (j < ce) a.F[off + (int64_t)j * M + i] = (T)Y[i];
+                    else if (j < C) a.F[off + (int64_t)j * M + i] = i <= k ? (T)Y[i] : (T)0;
+                    else a.Fr[rq + i] = Y[i];
+                }
+            }
+        }
+        __syncthreads();   // the workspace and bad_sh are reused by the next table
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -661,7 +762,7 @@ __global__ __launch_bounds__(256) void k_qr_backsub(int64_t neb, const int32_t *
     // first d_e rows); the lanes' partials are added in a fixed tree, and lane 0 solves U x_e = q - B x_r
     const int64_t e = (int64_t)blockIdx.x * (256 / QB_LANES) + threadIdx.x / QB_LANES;
     const int lane = threadIdx.x % QB_LANES;
-    const bool valid = e < neb;
+    const bool valid = e < neb && edims[e] <= QR_MAXDE;   // larger blocks: k_qr_backsub_wide
     const int de = valid ? edims[e] : 0, M = valid ? nrows[e] : 0;
     const int64_t off = valid ? raw_off[e] : 0;
     double t[QR_MAXDE];
@@ -694,6 +795,45 @@ __global__ __launch_bounds__(256) void k_qr_backsub(int64_t neb, const int32_t *
     }
 }
 
+// The same for the tables of d_e > QR_MAXDE: one workgroup per table (its index in `list`), t = q - B x_r in LDS
+// (thread i its row i over the entries in order), then the triangle by thread 0.
+template <class T>
+__global__ __launch_bounds__(256) void k_qr_backsub_wide(const int32_t *list, int64_t nlist, const int32_t *nrows,
+                                                         const int32_t *ncols, const int32_t *edims,
+                                                         const int64_t *raw_off, const int64_t *res_off,
+                                                         const int64_t *boff_e, const int32_t *table_ptr,
+                                                         const int32_t *entry_retained, const int32_t *entry_col,
+                                                         const int32_t *r_dims, const int64_t *boff_r, const T *F,
+                                                         const double *Fr, const double *x_r, double *x_e) {
+    extern __shared__ double tq[];
+    for (int64_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int e = list[li];
+        const int de = edims[e], M = nrows[e];
+        const int64_t off = raw_off[e];
+        for (int i = threadIdx.x; i < de; i += blockDim.x) {
+            double acc = 0.0;
+            for (int u = table_ptr[e]; u < table_ptr[e + 1]; u++) {
+                const int r = entry_retained[u];
+                const double *xr = x_r + boff_r[r];
+                const T *cb = F + off + (int64_t)entry_col[u] * M;
+                for (int j = 0; j < r_dims[r]; j++) acc += (double)cb[(int64_t)j * M + i] * xr[j];
+            }
+            tq[i] = Fr[res_off[e] + i] - acc;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int ce = ncols[e] - de;
+            double *xe = x_e + boff_e[e];
+            for (int i = de - 1; i >= 0; i--) {
+                double acc = tq[i];
+                for (int j = i + 1; j < de; j++) acc -= (double)F[off + (int64_t)(ce + j) * M + i] * xe[j];
+                xe[i] = acc / (double)F[off + (int64_t)(ce + i) * M + i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_qr_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 __global__ void k_qr_finalize(const unsigned long long *flag, int64_t *out) {
     const unsigned long long f = *flag;
@@ -704,8 +844,15 @@ __global__ void k_qr_finalize(const unsigned long long *flag, int64_t *out) {
 
 struct bls_qr {
     int64_t nr = 0, nrb = 0, neb = 0, ntables = 0, raw_len = 0, res_len = 0;
-    int max_de = 0;   // largest eliminated block dimension
+    int max_de = 0;   // largest eliminated block dimension among the register kernel's tables
     std::vector<int32_t> bucket_tables[3];
+    // eliminated tables of more than 64 rows or d_e > QR_MAXDE: k_qr_table_wide, with a Float64 workspace of
+    // wide_grid x wide_wsz; those of d_e > QR_MAXDE also take k_qr_backsub_wide
+    std::vector<int32_t> wide_tables, bigde_tables;
+    DevArray<int32_t> d_wide, d_bigde;
+    DevArray<double> d_wide_ws;
+    int64_t wide_wsz = 0, wide_grid = 0;
+    int bigde_max = 0;
     DevArray<int32_t> d_bucket[3];
     DevArray<int32_t> d_nrows, d_ncols, d_edims;
     DevArray<int64_t> d_raw_off, d_res_off, d_boff_e;
@@ -829,6 +976,11 @@ int reduce_qr_async(bls_qr *h, T *F, double *Fr, const T *Rw, const double *Rr, 
             else k_qr_tables<4, QR_MAXDE, T><<<grid, 256, 0, s>>>(a);
         }
     }
+    if (!h->wide_tables.empty()) {
+        a.tables = h->d_wide.ptr;
+        a.ntab = (int64_t)h->wide_tables.size();
+        k_qr_table_wide<T><<<(unsigned)h->wide_grid, QRW_THREADS, 0, s>>>(a, h->d_wide_ws.ptr, h->wide_wsz);
+    }
     h->timing.mark(1, s);
     k_qr_finalize<<<1, 1, 0, s>>>(h->d_flag.ptr, d_failed);
     BLS_HIP(hipGetLastError());
@@ -856,16 +1008,21 @@ extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, con
         if (t < neblocks) {
             edims[t] = (int32_t)e_dims[t];
             boff[t] = boffsets_e[t] - 1;
-            h->max_de = std::max<int>(h->max_de, (int)e_dims[t]);
-            if (e_dims[t] < 1 || e_dims[t] > QR_MAXDE) {
+            if (e_dims[t] < 1 || e_dims[t] > QRW_MAXDE) {
                 delete h;
-                return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension outside 1..8 on the device QR path");
+                return fail(BLS_ERR_UNSUPPORTED, "eliminated block dimension outside 1.." + std::to_string(QRW_MAXDE) +
+                                                     " on the device QR path");
             }
-        }
-        if (table_nrows[t] > 64 && t < neblocks) {
-            delete h;
-            return fail(BLS_ERR_UNSUPPORTED, "table " + std::to_string(t + 1) + " has " + std::to_string(table_nrows[t]) +
-                                                 " rows; the device QR path supports up to 64");
+            if (e_dims[t] > QR_MAXDE) {
+                h->bigde_tables.push_back((int32_t)t);
+                h->bigde_max = std::max<int>(h->bigde_max, (int)e_dims[t]);
+            }
+            if (table_nrows[t] > 64 || e_dims[t] > QR_MAXDE) {
+                h->wide_tables.push_back((int32_t)t);
+                h->wide_wsz = std::max<int64_t>(h->wide_wsz, (int64_t)table_nrows[t] * table_ncols[t]);
+                continue;
+            }
+            h->max_de = std::max<int>(h->max_de, (int)e_dims[t]);
         }
         const int b = t >= neblocks ? 0 : (table_nrows[t] <= 16 ? 0 : table_nrows[t] <= 32 ? 1 : 2);
         h->bucket_tables[b].push_back((int32_t)t);
@@ -889,6 +1046,15 @@ extern "C" int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, con
     }
     for (int b = 0; b < 3; b++)
         if ((rc = h->d_bucket[b].upload(h->bucket_tables[b]))) { delete h; return rc; }
+    if (!h->wide_tables.empty()) {   // M x (C + 1) doubles per workgroup (table_ncols counts the residual)
+        h->wide_grid = std::min<int64_t>((int64_t)h->wide_tables.size(),
+                                         std::max<int64_t>(1, std::min<int64_t>(1024, (1ll << 27) / h->wide_wsz)));
+        if ((rc = h->d_wide.upload(h->wide_tables)) || (rc = h->d_bigde.upload(h->bigde_tables)) ||
+            (rc = h->d_wide_ws.alloc((size_t)(h->wide_grid * h->wide_wsz)))) {
+            delete h;
+            return rc;
+        }
+    }
     h->h_nrows = nrows;
     *out = h;
     return BLS_OK;
@@ -1076,6 +1242,12 @@ template <class T> int qr_backsub(bls_qr *h, double *x_e, const double *x_r, con
                                                              h->d_table_ptr.ptr, h->d_entry_retained.ptr,
                                                              h->d_entry_col.ptr, h->d_r_dims.ptr, h->d_boff_r.ptr, F,
                                                              Fr, x_r, x_e);
+    if (!h->bigde_tables.empty())
+        k_qr_backsub_wide<T><<<(unsigned)std::min<size_t>(h->bigde_tables.size(), 4096), 256,
+                               (size_t)h->bigde_max * sizeof(double), s>>>(
+            h->d_bigde.ptr, (int64_t)h->bigde_tables.size(), h->d_nrows.ptr, h->d_ncols.ptr, h->d_edims.ptr,
+            h->d_raw_off.ptr, h->d_res_off.ptr, h->d_boff_e.ptr, h->d_table_ptr.ptr, h->d_entry_retained.ptr,
+            h->d_entry_col.ptr, h->d_r_dims.ptr, h->d_boff_r.ptr, F, Fr, x_r, x_e);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }

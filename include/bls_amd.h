@@ -237,8 +237,9 @@ int bls_schur_kernel_times(bls_schur *h, float *times, int n);
 
 /* Register a SchurJacobianStructure (src/schur_jacobian.jl:30-57): the eliminated blocks' sizes and offsets
  * (e_dims, boffsets_e of its SchurStructure) and the table layout (table_nrows, table_ncols, raw_offsets,
- * residual_offsets; ntables + 1 offsets). Eliminated tables must have at most 64 rows and d_e <= 8 on the
- * device path (BLS_ERR_UNSUPPORTED otherwise). */
+ * residual_offsets; ntables + 1 offsets). Eliminated tables of at most 64 rows and d_e <= 8 take the register
+ * kernel (four tables per wavefront); longer tables and d_e up to 4096 take one workgroup each (d_e above 4096:
+ * BLS_ERR_UNSUPPORTED). */
 int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, const int64_t *e_dims, const int64_t *boffsets_e,
                   int64_t ntables, const int64_t *table_nrows, const int64_t *table_ncols, const int64_t *raw_offsets,
                   const int64_t *residual_offsets, bls_qr **out);
