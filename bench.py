@@ -619,6 +619,53 @@ def mixed_leg(torch, dev, p, h, S64, csc, dmap, steps, warmup):
             "dtype": "E, C, R Float32; E_inv, S, g Float64"}
 
 
+def fill_leg(torch, dev, p, raw, r, reps=10):
+    """The tables filled on the device from the workload's per-observation blocks (block / residual_block of every
+    row block, bls_jfill_f64): the generator's 2 x d camera and 2 x 3 landmark blocks laid out row block after row
+    block, written into fresh tables and compared bit for bit with the tables the generator built. HBM figure: the
+    values and residual rows read once and written once (16 B per value)."""
+    jst, st = p.jst, p.structure
+    fill = dev.JacobianFill(jst)
+    d = st.r_dims[p.cam_of_obs - 1].astype(np.int64)
+    n_k = 2 * d + 6
+    off = np.concatenate([[0], np.cumsum(n_k)[:-1]])
+    vals = np.empty(int(n_k.sum()))
+    for c in range(int(d.max())):
+        m = c < d
+        for i in range(2):
+            vals[off[m] + 2 * c + i] = p.Jc[m, i, c]
+    for c in range(3):
+        for i in range(2):
+            vals[off + 2 * d + 2 * c + i] = p.Jl[:, i, c]
+    assert len(vals) == fill.nvals
+    dv = torch.as_tensor(vals).cuda()
+    dres = torch.as_tensor(np.ascontiguousarray(p.rr.reshape(-1))).cuda()
+    del vals
+    out = torch.zeros_like(raw)
+    rout = torch.zeros_like(r)
+    dev.fill_jacobian_(out, rout, fill, dv, dres)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(out, raw) and torch.equal(rout, r))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(reps):
+        e0.record()
+        dev.fill_jacobian_(out, rout, fill, dv, dres)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    nbytes = 16.0 * (fill.nvals + fill.nres)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"table fill: {ms:.3f} ms for {len(p.cam_of_obs)} row blocks ({fill.nvals} values), {gbs:.0f} GB/s "
+        f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM); equal to the generator's tables: {same}")
+    if not same:
+        raise SystemExit("table fill differs from the generator's tables")
+    del out, rout, dv, dres
+    return {"row_blocks": int(len(p.cam_of_obs)), "values": int(fill.nvals), "ms": ms, "achieved_GBs": gbs,
+            "frac": gbs / HBM_PEAK_GBS, "bytes": nbytes, "bitwise_equal_to_generator_tables": same}
+
+
 def qr_leg(torch, dev, b, scale, reps=10):
     """reduce_qr! on the Jacobian workload: 500 cameras (d 6 or 9, 50/50), 500000 x scale landmarks (d 3), track
     length U{2..30}, 2 x d residual blocks N(0,1), lambda 1e-4, FP64. Time of the table kernel (HIP events) and its
@@ -647,6 +694,7 @@ def qr_leg(torch, dev, b, scale, reps=10):
     gbs = nbytes / (ms * 1e-3) / 1e9
     log(f"QR leg: {ms:.3f} ms, {st.neblocks / (ms * 1e-3):.4g} landmarks/s, {gbs:.0f} GB/s "
         f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM)")
+    fill = fill_leg(torch, dev, p, raw, r)
     # SchurBlockJacobian{Float32}: Float32 tables, Float64 residual
     F64 = ws.factored.clone()
     raw64 = raw
@@ -762,7 +810,8 @@ def qr_leg(torch, dev, b, scale, reps=10):
         full[pname] = {"ms": pms, "achieved_GBs": gb, "frac": gb / HBM_PEAK_GBS}
         log(f"QR leg, {pname} (raw tables): {pms:.3f} ms, {gb:.0f} GB/s of {(rawbytes + extra) / 1e9:.2f} GB")
     solve["full_problem_f64"] = full
-    return {"landmark_scale": scale, "f32": f32, "solve": solve, "tables": int(jst.ntables), "landmarks": int(st.neblocks),
+    return {"landmark_scale": scale, "f32": f32, "fill": fill, "solve": solve, "tables": int(jst.ntables),
+            "landmarks": int(st.neblocks),
             "cameras": int(st.nrblocks), "raw_GB": jst.raw_length() * 8 / 1e9, "ms": ms,
             "landmarks_per_s": st.neblocks / (ms * 1e-3),
             "roofline": {"bound": "hbm", "kernel": "k_qr_tables", "achieved": gbs, "peak": HBM_PEAK_GBS,

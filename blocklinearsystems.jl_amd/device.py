@@ -61,6 +61,11 @@ SIGNATURES = {
     "bls_pcg_solve_f64": (ct.c_int, [VP, VP, VP, VP, ct.c_double, ct.c_int64, VP, P64, ct.POINTER(ct.c_double), VP]),
     "bls_qr_create": (ct.c_int, [ct.c_int64] * 3 + [P64] * 2 + [ct.c_int64] + [P64] * 4 + [ct.POINTER(VP)]),
     "bls_qr_destroy": (None, [VP]),
+    "bls_jfill_create": (ct.c_int, [ct.c_int64, ct.c_int64, P64, P64, ct.c_int64] + [P64] * 8 + [ct.c_int64] + [P64] * 5
+                         + [P64, P64, ct.POINTER(VP)]),
+    "bls_jfill_destroy": (None, [VP]),
+    "bls_jfill_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
+    "bls_jfill_f32": (ct.c_int, [VP, VP, VP, VP, VP, VP]),
     "bls_reduce_qr_f64": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, P64, VP]),
     "bls_reduce_qr_f64_async": (ct.c_int, [VP, VP, VP, VP, VP, VP, ct.c_double, VP, VP, VP]),
     "bls_qr_set_timing": (ct.c_int, [VP, ct.c_int]),
@@ -494,6 +499,50 @@ def copyto_(dest, src, scatter_map, stream=None):
 def scatter_map_to_device(scatter_map):
     import torch
     return torch.as_tensor(np.ascontiguousarray(scatter_map, dtype=np.int64)).cuda()
+
+
+class JacobianFill:
+    """The row blocks of a SchurJacobianStructure built from a description (the C ABI's bls_jfill), for filling its
+    tables on the device: block(J, k, blockentry(jst, k, j)) .= J_kj and residual_block(J, k) .= r_k for every row
+    block k (src/schur_jacobian.jl:330-387). ``nvals`` / ``nres``: the lengths of the value and residual arrays
+    (row block after row block, its blocks in ascending column block, each m x d column-major)."""
+
+    def __init__(self, jst):
+        if jst.rowblocks is None:
+            raise BLSError("the structure has no row blocks (build it with SchurJacobianStructure.from_description)")
+        keep = []
+        st, rb = jst.schur, jst.rowblocks
+        h = VP()
+        nv, nr = ct.c_int64(0), ct.c_int64(0)
+        _check(lib().bls_jfill_create(st.nrblocks, st.neblocks, _p64(st.r_dims, keep), _p64(st.e_dims, keep),
+                                      jst.ntables, _p64(jst.table_nrows, keep), _p64(jst.table_ncols, keep),
+                                      _p64(jst.table_top, keep), _p64(jst.raw_offsets, keep),
+                                      _p64(jst.residual_offsets, keep), _p64(jst.table_ptr, keep),
+                                      _p64(jst.entry_retained, keep), _p64(jst.entry_col, keep), len(rb["table"]),
+                                      _p64(rb["table"], keep), _p64(rb["first"], keep), _p64(rb["nrows"], keep),
+                                      _p64(rb["ptr"], keep), _p64(rb["blocks"], keep), ct.byref(nv), ct.byref(nr),
+                                      ct.byref(h)))
+        self.h, self.jst, self.nvals, self.nres = h, jst, int(nv.value), int(nr.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_jfill_destroy(self.h)
+            self.h = None
+
+
+def fill_jacobian_(raw, r, fill, vals, res, stream=None):
+    """Write every row block's Jacobian blocks ``vals`` and residual rows ``res`` into the raw tables ``raw``
+    (Float64 or Float32, as ``vals``) and the Float64 residual ``r`` (device tensors)."""
+    import torch
+    jst = fill.jst
+    if vals is None or vals.dtype not in (torch.float64, torch.float32):
+        raise BLSError("vals must be a Float64 or Float32 device tensor")
+    _require("vals", vals, vals.dtype, fill.nvals)
+    _require("raw", raw, vals.dtype, jst.raw_length())
+    _require("res", res, torch.float64, fill.nres)
+    _require("r", r, torch.float64, jst.residual_length())
+    f = lib().bls_jfill_f64 if vals.dtype == torch.float64 else lib().bls_jfill_f32
+    _check(f(fill.h, _dp(raw), _dp(r), _dp(vals), _dp(res), _stream(stream)))
 
 
 class QRHandle:

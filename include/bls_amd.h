@@ -34,6 +34,7 @@ extern "C" {
 
 typedef struct bls_schur bls_schur;       /* a registered SchurStructure + its reduced-system layout */
 typedef struct bls_qr bls_qr;             /* a registered SchurJacobianStructure */
+typedef struct bls_jfill bls_jfill;       /* a SchurJacobianStructure's row blocks, for filling its tables */
 
 /* Message of the last error on this host thread ("" if none). */
 const char *bls_last_error(void);
@@ -244,6 +245,28 @@ int bls_qr_create(int64_t nr, int64_t nrblocks, int64_t neblocks, const int64_t 
                   int64_t ntables, const int64_t *table_nrows, const int64_t *table_ncols, const int64_t *raw_offsets,
                   const int64_t *residual_offsets, bls_qr **out);
 void bls_qr_destroy(bls_qr *h);
+/* Filling the tables from per-row-block Jacobian blocks: for every row block k of a structure built from a
+ * description, block(J, k, blockentry(jst, k, j)) .= J_kj for each column block j it touches and
+ * residual_block(J, k) .= r_k (src/schur_jacobian.jl:330-387; the views coupling_block / eliminated_block /
+ * table_residual_block of :264-320). bls_jfill_create registers the row blocks (row_table, row_first, row_nrows,
+ * row_ptr (nrowblocks + 1), row_blocks: the layout column blocks of each row block ascending, retained 1..nrblocks
+ * and eliminated nrblocks + e; all 1-based, as SchurJacobianStructure(blocksizes, rowblocks, rowblocksizes;
+ * eliminated) lays them out, src/schur_description.jl:137-189) against the table layout, and returns in *nvals and
+ * *nres the lengths of the value and residual arrays a fill takes: row block after row block, each row block's
+ * blocks in ascending column block, m_k x d column-major; its residual rows likewise. A row block that touches a
+ * block its table does not hold, or rows outside its table's data rows, is BLS_ERR_ARG with the reference's reason.
+ * bls_jfill_{f64,f32} write those values into the raw tables (element type T) and the Float64 residual, asynchronously
+ * on the stream; the other rows (damping rows, rows of other row blocks) are left as they are. */
+int bls_jfill_create(int64_t nrblocks, int64_t neblocks, const int64_t *r_dims, const int64_t *e_dims, int64_t ntables,
+                     const int64_t *table_nrows, const int64_t *table_ncols, const int64_t *table_top,
+                     const int64_t *raw_offsets, const int64_t *residual_offsets, const int64_t *table_ptr,
+                     const int64_t *entry_retained, const int64_t *entry_col, int64_t nrowblocks,
+                     const int64_t *row_table, const int64_t *row_first, const int64_t *row_nrows,
+                     const int64_t *row_ptr, const int64_t *row_blocks, int64_t *nvals, int64_t *nres,
+                     bls_jfill **out);
+void bls_jfill_destroy(bls_jfill *h);
+int bls_jfill_f64(bls_jfill *h, double *raw, double *r, const double *vals, const double *res, void *stream);
+int bls_jfill_f32(bls_jfill *h, float *raw, double *r, const float *vals, const double *res, void *stream);
 /* reduce_qr!(ws, J, scaling_e, lambda, s) (src/qr_reduction.jl:162-187): factored = Q' [J_r | J_e] per table with
  * the damping rows sqrt(lambda scaling_e) written, fres = Q' r in Float64, free tables copied; raw tables and r
  * are read only. BLS_FAILED with the lowest failing eliminated block for a non-finite table or a numerically

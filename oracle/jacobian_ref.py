@@ -8,6 +8,12 @@ their Float64 residual, accumulated in Float64.
 
 build_reduced_jacobian_pattern and reduced_jacobian (src/qr_reduced.jl:247-370) restated the same way.
 
+fill_tables restates the assignments block(J, k, blockentry(jst, k, j)) .= J_kj and residual_block(J, k) .= r_k
+(src/schur_jacobian.jl:264-387) for every row block k: the checker of the device's bls_jfill_*.
+
+fill_tables is pinned (tests/test_oracle_qr.py) against the tables that the problem generator writes by its own
+vectorised indexing (bitwise), and against the dense J of the row blocks (J'J and J'r of the filled tables).
+
 Parity pinning (tests/test_conditioning.py): against the dense J and r of the same tables, J'r, diag(J'J) and
 ||J v||^2 by numpy; J~ against the C oracle's reduced_mul / reduced_tmul on the same factored tables (J~ v and J~'w
 for random v, w), its pattern against the CSC invariants (ascending rows, nnz = the sum of d_r (M - top) over the
@@ -143,3 +149,33 @@ def reduced_jacobian_dense(jst, F):
             for j in range(int(st.r_dims[r])):
                 Jt[yo:yo + M - top, ro + j] = F[cb + j * M:cb + j * M + M - top]
     return Jt
+
+
+def fill_tables(jst, vals, res, raw, r):
+    """Write row block k's blocks and residual rows into the raw tables and the residual, k = 1, 2, ...: block j
+    of the row block (its layout column blocks ascending) is an m_k x d column-major run of ``vals``; the run of row
+    block k follows that of k - 1. blockentry (:330-337) gives a retained block's entry in the table (its columns
+    from entry_col) or minus the eliminated block (columns table_ncols - d_e, ...); the rows start at row_first."""
+    rb, st = jst.rowblocks, jst.schur
+    src = rsrc = 0
+    for k in range(1, len(rb["table"]) + 1):
+        t, first, m = int(rb["table"][k - 1]), int(rb["first"][k - 1]), int(rb["nrows"][k - 1])
+        M = int(jst.table_nrows[t - 1])
+        base = int(jst.raw_offsets[t - 1]) - 1
+        for j in rb["blocks"][rb["ptr"][k - 1] - 1: rb["ptr"][k] - 1]:
+            entry = jst.blockentry(k, int(j))
+            if entry > 0:
+                col = int(jst.entry_col[entry - 1])
+                d = int(st.r_dims[jst.entry_retained[entry - 1] - 1])
+            else:
+                d = int(st.e_dims[-entry - 1])
+                col = int(jst.table_ncols[-entry - 1]) - d
+            for c in range(d):
+                for i in range(m):
+                    raw[base + (col - 1 + c) * M + first - 1 + i] = vals[src]
+                    src += 1
+        ro = int(jst.residual_offsets[t - 1]) - 1
+        for i in range(m):
+            r[ro + first - 1 + i] = res[rsrc]
+            rsrc += 1
+    return src, rsrc

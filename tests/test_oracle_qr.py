@@ -290,3 +290,64 @@ def test_reduced_jacobian_pattern_and_values(blsamd, free):
     assert np.abs(Jt @ v - y).max() <= 1e-12 * np.abs(y).max()
     assert np.abs(Jt.T @ w - g).max() <= 1e-12 * np.abs(g).max()
     assert np.array_equal(JR.reduced_jacobian_dense(jst, F), Jt.toarray())
+
+
+def _rowblock_values(jst, rng):
+    """Random blocks of every row block (vals: row block after row block, blocks in ascending layout column block,
+    each m x d column-major), residual rows, and the dense J / r they make (rows in row-block order, columns
+    [x_r; x_e])."""
+    st, rb = jst.schur, jst.rowblocks
+    nr, ne = st.nr, st.ne
+    vals, res, Jrows, rrows = [], [], [], []
+    for k in range(len(rb["table"])):
+        m = int(rb["nrows"][k])
+        rows = np.zeros((m, nr + ne))
+        for j in rb["blocks"][rb["ptr"][k] - 1: rb["ptr"][k + 1] - 1]:
+            j = int(j)
+            if j <= st.nrblocks:
+                d, o = int(st.r_dims[j - 1]), int(st.boffsets_r[j - 1]) - 1
+            else:
+                e = j - st.nrblocks
+                d, o = int(st.e_dims[e - 1]), nr + int(st.boffsets_e[e - 1]) - 1
+            B = rng.standard_normal((m, d))
+            vals.append(B.T.reshape(-1))
+            rows[:, o:o + d] = B
+        rr = rng.standard_normal(m)
+        res.append(rr)
+        Jrows.append(rows)
+        rrows.append(rr)
+    return (np.concatenate(vals), np.concatenate(res), np.vstack(Jrows), np.concatenate(rrows))
+
+
+def test_fill_tables_reproduces_generator_tables(blsamd):
+    """fill_tables (block / residual_block assignments, src/schur_jacobian.jl:264-387) from the generator's
+    per-observation blocks writes exactly the tables the generator laid out by its own vectorised indexing."""
+    import jacobian_ref as JR
+    p = blsamd.generators.config4(scale=0.0002, ncam=30, kmax=12)
+    jst, st = p.jst, p.structure
+    vals, res = [], []
+    for k in range(len(p.cam_of_obs)):
+        d = int(st.r_dims[p.cam_of_obs[k] - 1])
+        vals += [p.Jc[k, :, :d].T.reshape(-1), p.Jl[k].T.reshape(-1)]
+        res.append(p.rr[k])
+    raw, r = np.zeros(jst.raw_length()), np.zeros(jst.residual_length())
+    n, nres = JR.fill_tables(jst, np.concatenate(vals), np.concatenate(res), raw, r)
+    assert n == sum(len(v) for v in vals) and nres == 2 * len(p.cam_of_obs)
+    assert np.array_equal(raw, p.data) and np.array_equal(r, p.r)
+
+
+def test_fill_tables_equals_dense_rowblock_jacobian(blsamd):
+    """With free tables (camera priors, camera-pair rows) and mixed camera sizes: the filled tables give the J'J and
+    J'r of the dense J stacked from the row blocks themselves."""
+    import jacobian_ref as JR
+    from test_gpu_qr import _jac_with_free_rows
+    p = _jac_with_free_rows(blsamd, seed=3)
+    jst = p.jst
+    assert jst.ntables > p.structure.neblocks
+    vals, res, J, rr = _rowblock_values(jst, np.random.default_rng(1))
+    p.data, p.r = np.zeros(jst.raw_length()), np.zeros(jst.residual_length())
+    JR.fill_tables(jst, vals, res, p.data, p.r)
+    Jt, rt = _dense_J(p)
+    assert Jt.shape == J.shape
+    assert np.allclose(Jt.T @ Jt, J.T @ J, rtol=1e-13, atol=1e-12)
+    assert np.allclose(Jt.T @ rt, J.T @ rr, rtol=1e-13, atol=1e-12)
