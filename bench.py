@@ -564,8 +564,19 @@ def dense_leg(torch, dev, n, reps=5):
     tf = n ** 3 / 3.0 / (fms * 1e-3) / 1e12
     log(f"dense reduced solve, n = {n}: Cholesky {fms:.3f} ms ({tf:.1f} TFLOP/s, {100 * tf / FP64_MFMA_PEAK_TF:.1f}% of "
         f"the FP64 matrix peak), solve {sms:.3f} ms, ||S x - g|| / ||g|| = {rel:.1e}")
+    # the :ldl fallback (Bunch-Kaufman) on an indefinite S of the same order: S - 1.5 I has negative eigenvalues
+    S.sub_(1.5 * torch.eye(n, dtype=torch.float64, device="cuda"))
+    ldl = dev.DenseLDL(n)
+    t0 = time.time()
+    ok, neg = ldl.factor(S)
+    lfms = (time.time() - t0) * 1e3   # bls_ldl_factor_f64 synchronizes: host wall time of the whole factorization
+    assert ok and neg > 0
+    lsms = time_ms(torch, lambda: ldl.solve(g, x), reps)
+    lrel = float(torch.linalg.norm(S @ x - g) / torch.linalg.norm(g))
+    log(f"dense reduced solve, n = {n}: Bunch-Kaufman (:ldl fallback) {lfms:.1f} ms ({neg} negative eigenvalues), "
+        f"solve {lsms:.3f} ms, ||S x - g|| / ||g|| = {lrel:.1e}")
     return {"n": n, "factor_ms": fms, "solve_ms": sms, "factor_TFLOPs": tf, "frac_fp64_mfma": tf / FP64_MFMA_PEAK_TF,
-            "relres": rel}
+            "relres": rel, "ldl": {"factor_ms": lfms, "solve_ms": lsms, "negative_eigenvalues": neg, "relres": lrel}}
 
 
 def mixed_leg(torch, dev, p, h, S64, csc, dmap, steps, warmup):

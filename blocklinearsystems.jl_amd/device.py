@@ -123,6 +123,11 @@ SIGNATURES = {
     "bls_chol_factor_f32_async": (ct.c_int, [VP, VP, VP]),
     "bls_chol_solve_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_chol_factor_copy_f64": (ct.c_int, [VP, VP, VP]),
+    "bls_ldl_create": (ct.c_int, [ct.c_int64, ct.POINTER(VP)]),
+    "bls_ldl_destroy": (None, [VP]),
+    "bls_ldl_factor_f64": (ct.c_int, [VP, VP, P64, P64, VP]),
+    "bls_ldl_solve_f64": (ct.c_int, [VP, VP, VP, VP]),
+    "bls_ldl_factor_copy_f64": (ct.c_int, [VP, VP, VP, VP]),
     "bls_cond_create": (ct.c_int, [ct.c_int64, ct.POINTER(VP)]),
     "bls_cond_destroy": (None, [VP]),
     "bls_cond_is_stale": (ct.c_int, [VP]),
@@ -954,6 +959,44 @@ class DenseCholesky:
     def factor_copy(self, U, stream=None):
         _check(lib().bls_chol_factor_copy_f64(self.h, _dp(U), _stream(stream)))
         return U
+
+
+class DenseLDL:
+    """The :ldl branch of solve_reduced_dense! (src/schur_linear_systems.jl:312-319): Bunch-Kaufman of a copy of the
+    dense reduced system of order n (the C ABI's bls_ldl). factor(S) -> (True, negative eigenvalues of S) or
+    (False, first zero pivot, 1-based); solve(g, x) -> x = S^-1 g with the last factor."""
+
+    def __init__(self, n):
+        h = VP()
+        _check(lib().bls_ldl_create(n, ct.byref(h)))
+        self.h = h
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bls_ldl_destroy(self.h)
+            self.h = None
+
+    def factor(self, S, stream=None):
+        import torch
+        _require("S", S, torch.float64, self.n * self.n)
+        zc, neg = ct.c_int64(0), ct.c_int64(0)
+        rc = _check(lib().bls_ldl_factor_f64(self.h, _dp(S), ct.byref(zc), ct.byref(neg), _stream(stream)))
+        return (True, int(neg.value)) if rc == BLS_OK else (False, int(zc.value))
+
+    def solve(self, g, x, stream=None):
+        import torch
+        _require("g", g, torch.float64, self.n)
+        _require("x", x, torch.float64, self.n)
+        _check(lib().bls_ldl_solve_f64(self.h, _dp(g), _dp(x), _stream(stream)))
+        return x
+
+    def factor_copy(self, A, ipiv, stream=None):
+        import torch
+        _require("A", A, torch.float64, self.n * self.n)
+        _require("ipiv", ipiv, torch.int64, self.n)
+        _check(lib().bls_ldl_factor_copy_f64(self.h, _dp(A), _dp(ipiv), _stream(stream)))
+        return A, ipiv
 
 
 def dense_residual_(r, S, t, x, stream=None):

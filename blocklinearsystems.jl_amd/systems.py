@@ -5,11 +5,13 @@ entry points, with the reference's names, argument meaning and status values.
 A solve runs, on one stream, with no host round trip until the status is read:
   prepare_scale! (src/conditioning.jl:60-87) from getdiag! of H as filled (conditioned systems only);
   build_reduced!: reset_schur_dense! + reduce_schur_dense! in the scaled variables (:301-307);
-  the :cholesky branch of solve_reduced_dense! (:312-319) on the reduced system;
+  solve_reduced_dense! (:312-319) on the reduced system: Cholesky, falling back to Bunch-Kaufman (:ldl) when it
+  fails ("Indefinite" when S has a negative eigenvalue; :ldl alone when asked for);
   backsubstitute! (src/schur_reduction.jl:519-614), then unscale! and the "x finite" rule (:333-334).
 The reduced system's own conditioning inside DampedLinearSolve is a power-of-two scaling, which commutes with every
-operation of the Cholesky factorization and solve, so it is not repeated here. The Bunch-Kaufman fallback is not on
-the device: a reduced system that Cholesky rejects returns "NotPositiveDefinite".
+operation of the Cholesky factorization and solve, so it is not repeated here (Bunch-Kaufman's pivot choices
+compare magnitudes of the scaled matrix; the factorization it finds is the same up to rounding for a power-of-two
+scale).
 """
 from __future__ import annotations
 
@@ -17,7 +19,7 @@ import numpy as np
 
 from . import device as dev
 
-SUCCESS, NOT_POSITIVE_DEFINITE = "Success", "NotPositiveDefinite"
+SUCCESS, NOT_POSITIVE_DEFINITE, INDEFINITE = "Success", "NotPositiveDefinite", "Indefinite"
 
 
 class DenseSchurLinearSystem:
@@ -38,6 +40,7 @@ class DenseSchurLinearSystem:
         self.ws = dev.SchurReductionWorkspace(self.handle)
         self.layout = dev.AssemblyHandle(st, [], [], None)          # the storage layout, for getdiag!
         self.chol = dev.DenseCholesky(self.nr)
+        self.ldl = dev.DenseLDL(self.nr)
         self.cond = dev.EliminationConditioning(n, on=condition)
         f = lambda k: torch.empty(k, dtype=torch.float64, device="cuda")
         self.S, self.g_S = f(self.nr * self.nr), f(self.nr)
@@ -62,8 +65,11 @@ class DenseSchurLinearSystem:
     def x_e(self):
         return self.x[self.nr:]
 
-    def solve(self, stream=None):
-        """solve!(ls, :cholesky): "Success" with x = [x_r; x_e], or "NotPositiveDefinite"."""
+    def solve(self, solver="auto", stream=None):
+        """solve!(ls, solver) for solver in ("auto", "cholesky", "ldl"): "Success" or "Indefinite" with
+        x = [x_r; x_e] solved, or "NotPositiveDefinite"."""
+        if solver not in ("auto", "cholesky", "ldl"):
+            raise ValueError(f"{solver!r} is not a reduced solver")
         nr = self.nr
         if self.cond.on:
             s = dev.prepare_scale_(self.cond, lambda d: dev.hessian_diag_(d, self.layout, self.E_data, self.R,
@@ -77,10 +83,16 @@ class DenseSchurLinearSystem:
                                      self.E_data, self.C_data, self.ws, s_r=s_r, s_e=s_e, stream=stream)
         if not ok:
             return NOT_POSITIVE_DEFINITE
-        ok, _ = self.chol.factor(self.S, stream=stream)
-        if not ok:
-            return NOT_POSITIVE_DEFINITE
-        self.chol.solve(self.g_S, self.x_r, stream=stream)
+        status = SUCCESS
+        ok = solver != "ldl" and self.chol.factor(self.S, stream=stream)[0]
+        if ok:
+            self.chol.solve(self.g_S, self.x_r, stream=stream)
+        else:   # Bunch-Kaufman on a copy of S (:ldl; the fallback of :auto and :cholesky)
+            ok, neg = self.ldl.factor(self.S, stream=stream)
+            if not ok:
+                return NOT_POSITIVE_DEFINITE
+            status = INDEFINITE if neg > 0 else SUCCESS
+            self.ldl.solve(self.g_S, self.x_r, stream=stream)
         dev.backsubstitute_(self.x_e, self.x_r, self.handle, self.C_data, self.ws, s_r=s_r, s_e=s_e, stream=stream)
         dev.unscale_(self.cond, self.x, s, self._nonfinite, stream=stream)
-        return SUCCESS if int(self._nonfinite.item()) == 0 else NOT_POSITIVE_DEFINITE
+        return status if int(self._nonfinite.item()) == 0 else NOT_POSITIVE_DEFINITE

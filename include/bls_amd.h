@@ -530,6 +530,21 @@ int bls_chol_solve_f64(bls_chol *h, const double *g, double *x, void *stream);
 /* The factor U (upper triangle of an n x n column-major array; the lower triangle is unspecified), for tests. */
 int bls_chol_factor_copy_f64(bls_chol *h, double *U, void *stream);
 
+/* The :ldl branch of solve_reduced_dense! (src/schur_linear_systems.jl:312-319): Bunch-Kaufman of a copy of the
+ * dense symmetric S (its upper triangle read, n x n column-major), the fallback when Cholesky fails. The reference
+ * reaches it through DampedLinearSolve's bunchkaufman (LAPACK sytrf); this is LAPACK's unblocked dsytf2 on the lower
+ * triangle of the copy (1 x 1 and 2 x 2 pivots, symmetric interchanges, LAPACK's ipiv). bls_ldl_factor_f64
+ * synchronizes: BLS_OK with *negative the number of negative eigenvalues of D (those of S; > 0 is the reference's
+ * Indefinite status), or BLS_FAILED with *zero_col the first zero pivot (1-based; S singular). bls_ldl_solve_f64:
+ * x = S^-1 g with the last factor (dsytrs), queued on the stream; x may be g. bls_ldl_factor_copy_f64: the factor
+ * (L below the diagonal, D on it and, for a 2 x 2 pivot, below it) and ipiv, device to device. */
+typedef struct bls_ldl bls_ldl;
+int bls_ldl_create(int64_t n, bls_ldl **out);
+void bls_ldl_destroy(bls_ldl *h);
+int bls_ldl_factor_f64(bls_ldl *h, const double *S, int64_t *zero_col, int64_t *negative, void *stream);
+int bls_ldl_solve_f64(bls_ldl *h, const double *g, double *x, void *stream);
+int bls_ldl_factor_copy_f64(bls_ldl *h, double *A, int64_t *ipiv, void *stream);
+
 /* ---- The conditioning scale (src/conditioning.jl:24-87) ----------------------------------------------------
  * bls_cond holds EliminationConditioning's state (:47-55) except its vectors: whether the next prepare computes the
  * scale afresh (stale; set at creation and by bls_cond_markstale, as zero! and markstale! set it) and the number of

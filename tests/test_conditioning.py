@@ -153,3 +153,39 @@ def test_dense_cholesky_oracle_failing_column():
     assert D.cholesky_upper(S)[1] == 3
     with pytest.raises(np.linalg.LinAlgError):
         np.linalg.cholesky(np.diag([1.0, 2.0, 3.0, -1.0, 5.0]))
+
+
+def test_bunch_kaufman_oracle_equals_lapack_dsytrf():
+    """dense_ref.bunch_kaufman_lower restates LAPACK's dsytf2 (the reference's :ldl fallback is Julia's bunchkaufman,
+    LAPACK sytrf): on matrices of up to 60 rows scipy's dsytrf runs dsytf2 itself, so ipiv and info are equal and the
+    factor agrees to rounding; 2 x 2 pivots are forced with zero leading blocks and small diagonals. The inertia of D
+    equals that of S (Sylvester's law) and ldl_solve solves."""
+    import scipy.linalg.lapack as L
+    import dense_ref as D
+    rng = np.random.default_rng(0)
+    n2 = 0
+    for trial in range(200):
+        n = int(rng.integers(1, 61))
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        if trial % 3 == 0:
+            A[np.diag_indices(n)] *= 1e-3
+        if trial % 5 == 0 and n > 3:
+            A[:2, :2] = 0.0
+        F, ipiv, info = D.bunch_kaufman_lower(np.triu(A))
+        lu, ipiv_l, info_l = L.dsytrf(A, lower=1)
+        assert np.array_equal(ipiv, ipiv_l) and info == info_l
+        assert np.abs(np.tril(lu) - F).max() <= 1e-12 * max(1.0, np.abs(F).max())
+        n2 += int((ipiv < 0).sum())
+        assert D.bk_inertia(F, ipiv) == int((np.linalg.eigvalsh(A) < 0).sum())
+        g = rng.standard_normal(n)
+        x = D.ldl_solve(F, ipiv, g)
+        assert np.linalg.norm(A @ x - g) <= 1e-10 * np.linalg.norm(A, 2) * np.linalg.norm(x) + 1e-12
+    assert n2 > 0   # 2 x 2 pivots were exercised
+    # a zero column: info is its index and the factorization continues (dsytf2)
+    A = rng.standard_normal((5, 5))
+    A = A + A.T
+    A[2, :] = A[:, 2] = 0.0
+    _, ipiv, info = D.bunch_kaufman_lower(np.triu(A))
+    _, ipiv_l, info_l = L.dsytrf(A, lower=1)
+    assert info == info_l > 0 and np.array_equal(ipiv, ipiv_l)

@@ -2,7 +2,8 @@
 schur_solve! of DenseSchurLinearSystem, src/schur_linear_systems.jl:292-335) against numpy's dense solve of
 (H + lambda Diagonal(scaling)) x = b: within 1e-10 relative. Conditioned and unconditioned solves agree bit for bit
 in Float64, as the reference's test requires (test/conditioning.jl:64-67: powers of two commute with every
-operation); an indefinite reduced system returns NotPositiveDefinite."""
+operation); an indefinite reduced system falls back to Bunch-Kaufman and returns Indefinite with the whole x solved
+(test/dampedlinearsolve_schurhessian.jl:150-170); a singular one returns NotPositiveDefinite."""
 import numpy as np
 import pytest
 
@@ -45,13 +46,41 @@ def test_dense_schur_solve_matches_numpy(blsamd, lam):
     assert np.array_equal(xs[0], xs[1])
 
 
-def test_dense_schur_solve_not_positive_definite(blsamd):
-    """A retained diagonal entry made very negative: the reduction succeeds (E is fine), the reduced system is not
-    positive definite, and the solve says so."""
+def test_dense_schur_solve_indefinite_falls_back_to_bunch_kaufman(blsamd):
+    """test/dampedlinearsolve_schurhessian.jl:150-170: a retained diagonal entry made very negative makes S
+    indefinite; Cholesky fails and the :ldl fallback (also :ldl asked for directly) solves the whole system, x_e
+    included (checked on a stale x), within 1e-10 of the dense solve, with status Indefinite; with enough damping
+    the system is positive definite again and Cholesky solves it (Success)."""
     rng = np.random.default_rng(4)
     r_dims, e_dims, couplings = _dims(rng)
     H, b = schur_test_problem(rng, r_dims, e_dims, couplings)
     H[1, 1] = -50.0
     p = problem_from_dense(blsamd, H, b, r_dims, e_dims, couplings, 0.0, dense=True)
+    x_full = np.linalg.solve(H, b)
+    for solver in ("auto", "cholesky", "ldl"):
+        for condition in (True, False):
+            ls = _system(blsamd, p, condition)
+            ls.dampen(0.0)
+            ls.x.fill_(float("nan"))
+            assert ls.solve(solver) == "Indefinite"
+            assert np.linalg.norm(ls.x.cpu().numpy() - x_full) <= 1e-10 * np.linalg.norm(x_full)
     ls = _system(blsamd, p, True)
-    assert ls.solve() == "NotPositiveDefinite"
+    ls.dampen(100.0)
+    assert ls.solve() == "Success"
+    ref = np.linalg.solve(H + 100.0 * np.eye(len(b)), b)
+    assert np.linalg.norm(ls.x.cpu().numpy() - ref) <= 1e-10 * np.linalg.norm(ref)
+
+
+def test_dense_schur_solve_singular_not_positive_definite(blsamd):
+    """A retained variable with no entries at all (its row and column of H zero): the reduced S is singular, Cholesky
+    and Bunch-Kaufman both stop at a zero pivot, and the solve returns NotPositiveDefinite."""
+    rng = np.random.default_rng(5)
+    r_dims, e_dims, couplings = _dims(rng)
+    H, b = schur_test_problem(rng, r_dims, e_dims, couplings)
+    H[0, :] = 0.0
+    H[:, 0] = 0.0
+    p = problem_from_dense(blsamd, H, b, r_dims, e_dims, couplings, 0.0, dense=True)
+    for solver in ("auto", "ldl"):
+        ls = _system(blsamd, p, False)
+        ls.dampen(0.0)
+        assert ls.solve(solver) == "NotPositiveDefinite"
