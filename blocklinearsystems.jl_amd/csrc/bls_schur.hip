@@ -248,9 +248,11 @@ __global__ __launch_bounds__(256) void k_phase1(int64_t first, int64_t count, in
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
-__global__ void k_finalize_flag(const unsigned long long *flag, int64_t *out) {
+// The verdict of a reduction (0, or the lowest failing block); the flag is left at NO_FAILURE for the next one.
+__global__ void k_finalize_flag(unsigned long long *flag, int64_t *out) {
     const unsigned long long f = *flag;
     *out = f == (unsigned long long)NO_FAILURE ? 0 : (int64_t)f;
+    *flag = (unsigned long long)NO_FAILURE;
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -822,6 +824,39 @@ __global__ void k_scale_sparse(TS *nzval, const int64_t *S_colptr, const int64_t
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nr) return;
     for (int64_t k = S_colptr[j] - 1; k < S_colptr[j + 1] - 1; k++) nzval[k] = (TS)((double)nzval[k] * (s_r[S_rowval[k] - 1] * s_r[j]));
+}
+
+// reset_schur_sparse! in one pass over the stored entries (src/schur_reduction.jl:416-436): entry k takes R[q]
+// (q + 1 = code[k] >> 1: the last R entry r_to_s_map sends there; 0 where none does), scaled(., s_r[row] s_r[col])
+// when conditioned ((row, col) from rc, read only then), and the damping when it is a diagonal entry (code bit 0;
+// indices nnz + i of the loop: diagonal entry i), with the same operations per entry as fill!, the scatter,
+// scale_sparse! and the damping loop one after another.
+template <class TS, class TR>
+__global__ void k_reset_sparse1(TS *nzval, const TR *R, const int32_t *code, const int2 *rc, const int64_t *diag_inds,
+                                int64_t nnz, int64_t nr, const double *scaling_r, double lambda, const double *s_r) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nnz + nr; x += (int64_t)gridDim.x * blockDim.x) {
+        if (x < nnz) {
+            const int32_t c = code[x];
+            if (c & 1) continue;   // a diagonal entry: below
+            const int32_t q = (c >> 1) - 1;
+            TS v = q >= 0 ? (TS)R[q] : (TS)0;
+            if (s_r) {
+                const int2 p = rc[x];
+                v = (TS)((double)v * (s_r[p.x] * s_r[p.y]));
+            }
+            nzval[x] = v;
+        } else {
+            const int64_t i = x - nnz, k = diag_inds[i] - 1;
+            const int32_t q = (code[k] >> 1) - 1;
+            TS v = q >= 0 ? (TS)R[q] : (TS)0;
+            if (s_r) v = (TS)((double)v * (s_r[i] * s_r[i]));
+            {
+#pragma clang fp contract(off)
+                const double si = s_r ? s_r[i] : 1.0;
+                nzval[k] = (TS)((double)v + lambda * scaling_r[i] * (si * si));
+            }
+        }
+    }
 }
 
 template <class TS, class TR>
@@ -1401,6 +1436,10 @@ struct bls_schur {
     DevArray<int64_t> d_eoffsets, d_boffsets_e, d_coffsets, d_eliminated_ptr, d_boffsets_r;
     DevArray<int32_t> d_e_dims, d_r_dims, d_entry_retained;
     DevArray<int64_t> d_r_to_s, d_diag_inds, d_S_colptr, d_S_rowval;
+    // the one-pass reset: per stored entry its R entry (-1 none) and its (row, column); empty when the registered
+    // diag_inds are not the entries with row == column (the reset then runs fill, scatter, scale, damping)
+    DevArray<int32_t> d_rst_code;   // (R entry + 1) << 1 | diagonal
+    DevArray<int2> d_rst_rc;
     DevArray<P2Item> d_items;
     DevArray<P2Entry> d_entries;
     DevArray<P2Task> d_tasks;
@@ -1421,6 +1460,7 @@ struct bls_schur {
     DevArray<int2> d_sy_tiles;
     int64_t sy_ntiles = 0;
     DevArray<unsigned long long> d_flag;
+    bool flag_dirty = true;   // d_flag may not hold NO_FAILURE (before the first reduction, or after one that errored)
     DevArray<int64_t> d_failed_out;
     int64_t nitems = 0, nentries = 0;
     Timing timing;
@@ -1815,7 +1855,9 @@ int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, cons
     }
     const P2Scratch ws = scratch ? *scratch : P2Scratch{h->d_flag.ptr, h->d_partial.ptr};
     h->timing.mark(1, s);
-    k_fill_u64<<<1, 1, 0, s>>>(ws.flag, (unsigned long long)NO_FAILURE);
+    const bool own_flag = ws.flag == h->d_flag.ptr;
+    if (!own_flag || h->flag_dirty) k_fill_u64<<<1, 1, 0, s>>>(ws.flag, (unsigned long long)NO_FAILURE);
+    if (own_flag) h->flag_dirty = true;   // until the finalize below is queued
     int rc = phase1<P>(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, ws.flag, s);
     if (rc) return rc;
     if ((s_r || s_e) && h->neb) {
@@ -1836,6 +1878,7 @@ int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, cons
     }
     k_finalize_flag<<<1, 1, 0, s>>>(ws.flag, d_failed);
     BLS_HIP(hipGetLastError());
+    if (own_flag) h->flag_dirty = false;
     return BLS_OK;
 }
 
@@ -2025,6 +2068,38 @@ extern "C" int bls_schur_set_sparse(bls_schur *h, int64_t nnz, const int64_t *S_
     if ((rc = h->d_r_to_s.upload(rts)) || (rc = h->d_diag_inds.upload(di)) || (rc = h->d_S_colptr.upload(Scp)) ||
         (rc = h->d_S_rowval.upload(Srv)))
         return rc;
+    {
+        std::vector<int32_t> src((size_t)nnz, -1);
+        std::vector<int2> rcv((size_t)nnz, make_int2(-1, -2));
+        bool ok = nnz <= INT32_MAX && nR < (1 << 30);
+        for (int64_t i = 0; ok && i < nR; i++)   // the last R entry mapped to an S entry wins, as the scatter loop's
+            if (rts[i] > 0) {
+                if (rts[i] > nnz) return fail(BLS_ERR_ARG, "r_to_s_map points past the stored entries");
+                src[rts[i] - 1] = (int32_t)i;
+            }
+        for (int64_t j = 0; ok && j < h->nr; j++)
+            for (int64_t k = Scp[j] - 1; k < Scp[j + 1] - 1; k++) rcv[k] = make_int2((int)(Srv[k] - 1), (int)j);
+        for (int64_t i = 0; ok && i < h->nr; i++) {
+            const int64_t k = di[i] - 1;
+            if (k < 0 || k >= nnz || rcv[k].x != i || rcv[k].y != i) ok = false;
+        }
+        if (ok) {
+            int64_t ndiag = 0;
+            for (int64_t k = 0; k < nnz; k++) {
+                if (rcv[k].x < 0) ok = false;   // an entry no column holds
+                ndiag += rcv[k].x == rcv[k].y;
+            }
+            ok = ok && ndiag == h->nr;
+        }
+        if (ok) {
+            std::vector<int32_t> code((size_t)nnz);
+            for (int64_t k = 0; k < nnz; k++) code[k] = ((src[k] + 1) << 1) | (rcv[k].x == rcv[k].y ? 1 : 0);
+            if ((rc = h->d_rst_code.upload(code)) || (rc = h->d_rst_rc.upload(rcv))) return rc;
+        } else {
+            h->d_rst_code.reset();
+            h->d_rst_rc.reset();
+        }
+    }
     h->nnz = nnz;
     h->nR = nR;
     h->mode = 1;
@@ -2050,6 +2125,14 @@ int reset_sparse(bls_schur *h, TS *nzval, const TR *R_nzval, const double *scali
         return fail(BLS_ERR_ARG, "lambda must be finite and non-negative, got " + std::to_string(lambda));
     if (!nzval || !scaling_r || (h->nR && !R_nzval)) return fail(BLS_ERR_ARG, "reset_schur_sparse: a required array is NULL");
     h->timing.mark(0, s);
+    if (h->nnz && h->d_rst_code.ptr) {
+        k_reset_sparse1<TS, TR><<<std::min<int64_t>(grid_for(h->nnz + h->nr, 256), 8192), 256, 0, s>>>(
+            nzval, R_nzval, h->d_rst_code.ptr, h->d_rst_rc.ptr, h->d_diag_inds.ptr, h->nnz, h->nr, scaling_r, lambda,
+            s_r);
+        h->timing.mark(1, s);
+        BLS_HIP(hipGetLastError());
+        return BLS_OK;
+    }
     k_fill<TS><<<std::min<int64_t>(grid_for(h->nnz, 256), 4096), 256, 0, s>>>(nzval, h->nnz, (TS)0);
     if (h->nR)
         k_scatter_r<TS, TR><<<std::min<int64_t>(grid_for(h->nR, 256), 4096), 256, 0, s>>>(nzval, R_nzval, h->d_r_to_s.ptr,
@@ -2157,10 +2240,12 @@ extern "C" int bls_schur_factor_f64(bls_schur *h, const double *E_data, const do
     if (h->neb && (!E_data || !b_e || !scaling_e || !E_inv || !z))
         return fail(BLS_ERR_ARG, "factor_all_eliminated: a required array is NULL");
     hipStream_t s = (hipStream_t)stream;
+    h->flag_dirty = true;
     k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
     int rc = phase1<PrecF64>(h, E_data, b_e, scaling_e, lambda, s_e, E_inv, z, h->d_flag.ptr, s);
     if (rc) return rc;
     k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, h->d_failed_out.ptr);
+    h->flag_dirty = false;
     BLS_HIP(hipGetLastError());
     int64_t fb = 0;
     BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -2218,10 +2303,12 @@ int reduce_dense_mfma(bls_schur *h, double *S, double *g, const double *b_r, con
     if (h->wide)
         return fail(BLS_ERR_UNSUPPORTED, "MFMA dense reduction: blocks up to d_e 8 and d_r " + std::to_string(MAXDR));
     // phase 1 (the reference's landmark factorisation and acceptance), then its verdict
+    h->flag_dirty = true;
     k_fill_u64<<<1, 1, 0, s>>>(h->d_flag.ptr, (unsigned long long)NO_FAILURE);
     int rc = phase1<PrecF64>(h, E, b_e, sc, lambda, nullptr, E_inv, z, h->d_flag.ptr, s);
     if (rc) return rc;
     k_finalize_flag<<<1, 1, 0, s>>>(h->d_flag.ptr, h->d_failed_out.ptr);
+    h->flag_dirty = false;
     int64_t fb = 0;
     BLS_HIP(hipMemcpyAsync(&fb, h->d_failed_out.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     BLS_HIP(hipStreamSynchronize(s));
