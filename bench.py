@@ -55,9 +55,9 @@ def main():
                     help="landmark scale of the reduce_qr! leg (500 cameras d 6/9, 500000 landmarks x scale); 0 skips")
     ap.add_argument("--config3-scale", type=float, default=1.0,
                     help="landmark scale of the dense leg (256 cameras, 1000000 landmarks x scale); 0 skips it")
-    ap.add_argument("--config3-valu-scale", type=float, default=0.1,
-                    help="fraction of the dense leg's landmarks at which the VALU kernel and the CPU oracle run "
-                         "(their host work list and the oracle are slow at the full size)")
+    ap.add_argument("--config3-valu-scale", type=float, default=1.0,
+                    help="fraction of the dense leg's landmarks at which the VALU kernel, the CPU oracle and the "
+                         "parity check run (1: the same full-size instance as the MFMA path)")
     ap.add_argument("--config5-scale", type=float, default=1.0,
                     help="landmark scale of the batched 64-problem leg (cameras at full size); 0 skips it")
     ap.add_argument("--tuning", default="",
@@ -295,7 +295,7 @@ def parity_record(st, pattern, S_dev, g_dev, S_ref, g_ref):
 FP32_MFMA_PEAK_TF = 157.3
 
 
-def config3_leg(torch, dev, b, scale, valu_scale=0.1, cpu=True, reps=3):
+def config3_leg(torch, dev, b, scale, valu_scale=1.0, cpu=True, reps=3):
     """The dense configuration: 256 cameras (d 6), 1000000 x scale landmarks (d 3), k ~ U{2..64} random cameras,
     Jacobian blocks scaled per camera by 10^U(-2, 2), lambda 1e-4, dense reduce_schur_dense!.
     - The SYRK on the matrix cores (phase 1 in Float64, S -= Y'Y with Y_e = chol(E_inv_e)' C_e): Float64 C and FP64
@@ -303,8 +303,9 @@ def config3_leg(torch, dev, b, scale, valu_scale=0.1, cpu=True, reps=3):
       MFMA rate against the peaks, and the useful rate (108 k^2 + 162 k + 68 flop per landmark).
     - The mixed S against the Float64 one, and the dense solve through the mixed S's Cholesky factor with one
       Float64 refinement step (its residual from the Float64 blocks), judged against the Float64 system.
-    - The owner-computes VALU kernel (the default dense reduction) at valu_scale of the landmarks (its host work
-      list grows with landmarks x retained blocks), with the MFMA path at the same size, and the CPU oracle there."""
+    - The owner-computes VALU kernel (the default dense reduction) at valu_scale of the landmarks (the full instance
+      by default; the MFMA path is timed again there when it is smaller), the CPU oracle on the same instance and the
+      VALU result's parity against it."""
     t0 = time.time()
     p = b.generators.config3(scale=scale)
     st = p.structure
@@ -386,7 +387,7 @@ def config3_leg(torch, dev, b, scale, valu_scale=0.1, cpu=True, reps=3):
     del Ss, gs, S64, Sm, A64, Am, SR, rd, E, h, ws
     torch.cuda.empty_cache()
     # the VALU owner-computes kernel, the MFMA path and the CPU oracle at valu_scale of the landmarks
-    q = b.generators.config3(scale=scale * valu_scale)
+    q = p if valu_scale == 1.0 else b.generators.config3(scale=scale * valu_scale)
     sq = q.structure
     t0 = time.time()
     hq = dev.SchurHandle(sq, q.ws_index, dense=True)
@@ -399,8 +400,11 @@ def config3_leg(torch, dev, b, scale, valu_scale=0.1, cpu=True, reps=3):
     kq = np.diff(sq.eliminated_ptr).astype(np.float64)
     useful_q = float(np.sum(108 * kq * kq + 162 * kq + 68))
     small = {"landmarks": int(sq.neblocks), "valu_work_list_s": plan_s}
-    for name, fn in (("valu_f64", lambda: dev.reduce_schur_dense_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq, check=False)),
-                     ("mfma_f64", lambda: dev.reduce_schur_dense_mfma_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq, check=False))):
+    legs = [("valu_f64", lambda: dev.reduce_schur_dense_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq, check=False))]
+    if valu_scale != 1.0:
+        legs.append(("mfma_f64", lambda: dev.reduce_schur_dense_mfma_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq,
+                                                                      check=False)))
+    for name, fn in legs:
         ts = []
         for rep_ in range(reps + 1):
             dev.reset_schur_dense_(Sq, Rq, hq, srq, q.lam, check=False)
@@ -436,7 +440,7 @@ def config3_leg(torch, dev, b, scale, valu_scale=0.1, cpu=True, reps=3):
                                           "rel_g": float(np.linalg.norm(gq.cpu().numpy() - go) / np.linalg.norm(go))}
         log(f"dense leg at {sq.neblocks} landmarks, CPU oracle ({threads} threads): {cms:.1f} ms; device S within "
             f"{small['parity_valu_vs_oracle']['rel_S']:.2e} of it (norm-wise)")
-    out["reduced_size"] = small
+    out["owner_computes"] = small
     return out
 
 

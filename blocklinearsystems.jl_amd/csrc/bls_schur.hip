@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <type_traits>
 
 #include "bls_internal.h"
@@ -1551,14 +1553,9 @@ int build_wide_pairs(bls_schur *h, Locate locate) {
     return BLS_OK;
 }
 
-// Build the phase-2 work: items (slot runs of block columns), their landmark entries, and the lanes' tasks.
-// `slots_of(r2)` gives the block rows of column r2 (ascending, last == r2), `rowbase_of(r2, k, r1)` the row base of
-// its k-th block, and base_of the column's addressing. An item with more than P2_PART landmarks is split into
-// parts over consecutive landmark ranges (one workgroup each, summed in part order by k_phase2_merge). Within a
-// part, a slot touched by many landmarks gets several lane groups (chunks), which share each batch's landmarks of
-// the slot in contiguous runs and are merged in chunk order: every sum is taken in a fixed order.
-template <typename Slots, typename RowBase, typename Base>
-int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of) {
+// One thread's share of the work list: block columns [c0, c1), with offsets local to it (fixed when the shares are
+// joined in column order).
+struct P2Share {
     std::vector<P2Item> items;
     std::vector<P2Entry> entries;
     std::vector<P2Task> tasks;
@@ -1566,25 +1563,88 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
     std::vector<int32_t> slot_dr, slot_roff;
     std::vector<P2Merge> merges;
     int64_t npartial = 0;
-    std::vector<int64_t> owner(h->ncoup);
-    for (int64_t e = 0; e < h->neb; e++)
-        for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++) owner[c] = e;
+    std::string error;
+};
+
+template <typename Slots, typename RowBase, typename Base>
+void build_phase2_columns(const bls_schur *h, const std::vector<int32_t> &owner, int64_t c0, int64_t c1,
+                          Slots slots_of, RowBase rowbase_of, Base base_of, P2Share &out) {
+    std::vector<P2Item> &items = out.items;
+    std::vector<P2Entry> &entries = out.entries;
+    std::vector<P2Task> &tasks = out.tasks;
+    std::vector<int64_t> &slot_rowbase = out.slot_rowbase;
+    std::vector<int32_t> &slot_dr = out.slot_dr, &slot_roff = out.slot_roff;
+    std::vector<P2Merge> &merges = out.merges;
+    int64_t &npartial = out.npartial;
+    const int64_t *er = h->entry_retained.data();
     std::vector<int64_t> col;
-    std::vector<int64_t> hits(64);   // landmarks touching each slot of the current item
-    for (int64_t r2 = 0; r2 < h->nrb; r2++) {
+    std::vector<int32_t> pos((size_t)h->nrb, -1);   // slot of each block row in the current column, -1 none
+    std::vector<int32_t> item_of;                   // item (within the column) of each slot
+    std::vector<std::vector<P2Entry>> item_entries;
+    std::vector<int64_t> hits(64);                  // landmarks touching each slot of the current item
+    for (int64_t r2 = c0; r2 < c1; r2++) {
         col.clear();
         slots_of(r2, col);
-        if (col.empty() || col.back() != r2)
-            return fail(BLS_ERR_ARG, "the reduced-system pattern has no diagonal block " + std::to_string(r2 + 1));
+        if (col.empty() || col.back() != r2) {
+            out.error = "the reduced-system pattern has no diagonal block " + std::to_string(r2 + 1);
+            return;
+        }
         const int dr2 = (int)h->r_dims[r2];
         int64_t base, colstride;
         base_of(r2, base, colstride);
-        size_t k0 = 0;
-        while (k0 < col.size()) {
+        // the column's items: runs of up to 64 slots and about p2_lanes rows
+        std::vector<std::pair<size_t, size_t>> runs;
+        item_of.assign(col.size(), 0);
+        for (size_t k0 = 0; k0 < col.size();) {
             size_t k1 = k0;
             int rows = 0;
             while (k1 < col.size() && k1 - k0 < 64 && (k1 == k0 || rows + h->r_dims[col[k1]] <= h->p2_lanes))
                 rows += (int)h->r_dims[col[k1++]];
+            for (size_t k = k0; k < k1; k++) item_of[k] = (int32_t)runs.size();
+            runs.emplace_back(k0, k1);
+            k0 = k1;
+        }
+        for (size_t k = 0; k < col.size(); k++) pos[col[k]] = (int32_t)k;
+        item_entries.resize(runs.size());
+        for (auto &v : item_entries) v.clear();
+        // each landmark of the column once: its couplings with block rows <= r2 (ascending), one entry per item they
+        // fall in, in the column's landmark order
+        for (int64_t kk = h->retained_ptr[r2]; kk < h->retained_ptr[r2 + 1]; kk++) {
+            const int64_t c2 = h->retained_entry[kk];
+            const int64_t e = owner[c2];
+            for (int64_t c = h->eliminated_ptr[e]; c <= c2;) {
+                const int32_t k = pos[er[c]];
+                if (k < 0) {
+                    out.error = "The sparse pattern has no block (" + std::to_string(er[c] + 1) + ", " +
+                                std::to_string(r2 + 1) + ")";
+                    return;
+                }
+                const int32_t it = item_of[k];
+                const size_t k0 = runs[it].first, k1 = runs[it].second;
+                P2Entry en{};
+                en.e = (int32_t)e;
+                en.c_first = (int32_t)c;
+                en.c2 = (int32_t)c2;
+                unsigned long long mask = 0;
+                for (; c <= c2; c++) {
+                    const int32_t kc = pos[er[c]];
+                    if (kc < 0) {
+                        out.error = "The sparse pattern has no block (" + std::to_string(er[c] + 1) + ", " +
+                                    std::to_string(r2 + 1) + ")";
+                        return;
+                    }
+                    if ((size_t)kc < k0 || (size_t)kc >= k1) break;
+                    mask |= 1ull << (kc - (int32_t)k0);
+                }
+                en.mask = mask;
+                item_entries[it].push_back(en);
+            }
+        }
+        for (size_t k = 0; k < col.size(); k++) pos[col[k]] = -1;
+        for (size_t it_i = 0; it_i < runs.size(); it_i++) {
+            const size_t k0 = runs[it_i].first, k1 = runs[it_i].second;
+            int rows = 0;
+            for (size_t k = k0; k < k1; k++) rows += (int)h->r_dims[col[k]];
             P2Item it{};
             it.base = base;
             it.colstride = colstride;
@@ -1604,33 +1664,8 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
                 if (h->r_dims[col[k]] != udr) udr = 0;
             }
             it.uniform_dr = udr;
-            const int64_t r1_lo = col[k0], r1_hi = col[k1 - 1];
             const int64_t e_begin = (int64_t)entries.size();
-            for (int64_t kk = h->retained_ptr[r2]; kk < h->retained_ptr[r2 + 1]; kk++) {
-                const int64_t c2 = h->retained_entry[kk];
-                const int64_t e = owner[c2];
-                const int64_t cs = h->eliminated_ptr[e];
-                // couplings of e with retained block in [r1_lo, r1_hi] (entries ascending, all <= r2)
-                const int64_t *er = h->entry_retained.data();
-                const int64_t ca = std::lower_bound(er + cs, er + c2 + 1, r1_lo) - er;
-                const int64_t cb = std::upper_bound(er + cs, er + c2 + 1, r1_hi) - er;   // exclusive
-                if (ca >= cb) continue;
-                unsigned long long mask = 0;
-                size_t k = k0;
-                for (int64_t c = ca; c < cb; c++) {
-                    while (k < k1 && col[k] < er[c]) k++;
-                    if (k == k1 || col[k] != er[c])
-                        return fail(BLS_ERR_ARG, "The sparse pattern has no block (" + std::to_string(er[c] + 1) + ", " +
-                                                     std::to_string(r2 + 1) + ")");
-                    mask |= 1ull << (k - k0);
-                }
-                P2Entry en{};
-                en.e = (int32_t)e;
-                en.c_first = (int32_t)ca;
-                en.c2 = (int32_t)c2;
-                en.mask = mask;
-                entries.push_back(en);
-            }
+            entries.insert(entries.end(), item_entries[it_i].begin(), item_entries[it_i].end());
             const int64_t e_end = (int64_t)entries.size();
             const int64_t ne = e_end - e_begin;
             const int np = ne > h->p2_part ? (int)((ne + h->p2_part - 1) / h->p2_part) : 1;
@@ -1692,10 +1727,87 @@ int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of)
                 items.push_back(pt);
             }
             if (np > 1) npartial += (int64_t)np * rows * (dr2 + 1);
-            k0 = k1;
         }
     }
-    if (entries.size() > (size_t)INT32_MAX) return fail(BLS_ERR_UNSUPPORTED, "too many phase-2 entries");
+}
+
+// Build the phase-2 work: items (slot runs of block columns), their landmark entries, and the lanes' tasks.
+// `slots_of(r2)` gives the block rows of column r2 (ascending, last == r2), `rowbase_of(r2, k, r1)` the row base of
+// its k-th block, and base_of the column's addressing. An item with more than P2_PART landmarks is split into
+// parts over consecutive landmark ranges (one workgroup each, summed in part order by k_phase2_merge). Within a
+// part, a slot touched by many landmarks gets several lane groups (chunks), which share each batch's landmarks of
+// the slot in contiguous runs and are merged in chunk order: every sum is taken in a fixed order. The block columns
+// are shared out to host threads by their coupling counts and the shares joined in column order, so the list is
+// the same whatever the thread count.
+template <typename Slots, typename RowBase, typename Base>
+int build_phase2(bls_schur *h, Slots slots_of, RowBase rowbase_of, Base base_of) {
+    std::vector<int32_t> owner((size_t)h->ncoup);
+    for (int64_t e = 0; e < h->neb; e++)
+        for (int64_t c = h->eliminated_ptr[e]; c < h->eliminated_ptr[e + 1]; c++) owner[c] = (int32_t)e;
+    // the work of column r2 grows with its landmarks times their couplings below it; share columns by coupling count
+    const int64_t nrb = h->nrb;
+    const int64_t total = h->ncoup;
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (total < (1 << 16)) nth = 1;
+    std::vector<int64_t> cut{0};
+    for (unsigned t = 1; t < nth; t++) {
+        const int64_t target = total * t / nth;
+        int64_t r = cut.back();
+        while (r < nrb && h->retained_ptr[r] < target) r++;
+        cut.push_back(r);
+    }
+    cut.push_back(nrb);
+    std::vector<P2Share> shares(cut.size() - 1);
+    {
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t + 1 < cut.size(); t++)
+            pool.emplace_back([&, t] {
+                build_phase2_columns(h, owner, cut[t], cut[t + 1], slots_of, rowbase_of, base_of, shares[t]);
+            });
+        for (auto &th : pool) th.join();
+    }
+    for (auto &sh : shares)
+        if (!sh.error.empty()) return fail(BLS_ERR_ARG, sh.error);
+    std::vector<P2Item> items;
+    std::vector<P2Entry> entries;
+    std::vector<P2Task> tasks;
+    std::vector<int64_t> slot_rowbase;
+    std::vector<int32_t> slot_dr, slot_roff;
+    std::vector<P2Merge> merges;
+    int64_t npartial = 0;
+    size_t n_it = 0, n_en = 0, n_tk = 0, n_sl = 0, n_mg = 0;
+    for (auto &sh : shares) {
+        n_it += sh.items.size(); n_en += sh.entries.size(); n_tk += sh.tasks.size(); n_sl += sh.slot_dr.size();
+        n_mg += sh.merges.size();
+    }
+    items.reserve(n_it); entries.reserve(n_en + 1); tasks.reserve(n_tk); slot_rowbase.reserve(n_sl);
+    slot_dr.reserve(n_sl); slot_roff.reserve(n_sl); merges.reserve(n_mg);
+    for (auto &sh : shares) {
+        const int64_t eo = (int64_t)entries.size(), to = (int64_t)tasks.size(), so = (int64_t)slot_dr.size();
+        const int64_t po = npartial;
+        for (P2Item it : sh.items) {
+            it.entry_lo += eo;
+            it.entry_hi += eo;
+            it.task_lo += (int32_t)to;
+            it.slot_lo += (int32_t)so;
+            if (it.nparts > 1) it.pbase += po;
+            items.push_back(it);
+        }
+        for (P2Merge mg : sh.merges) {
+            mg.pbase += po;
+            mg.slot_lo += (int32_t)so;
+            merges.push_back(mg);
+        }
+        entries.insert(entries.end(), sh.entries.begin(), sh.entries.end());
+        tasks.insert(tasks.end(), sh.tasks.begin(), sh.tasks.end());
+        slot_rowbase.insert(slot_rowbase.end(), sh.slot_rowbase.begin(), sh.slot_rowbase.end());
+        slot_dr.insert(slot_dr.end(), sh.slot_dr.begin(), sh.slot_dr.end());
+        slot_roff.insert(slot_roff.end(), sh.slot_roff.begin(), sh.slot_roff.end());
+        npartial += sh.npartial;
+        sh = P2Share{};
+    }
+    if (entries.size() > (size_t)INT32_MAX || tasks.size() > (size_t)INT32_MAX || slot_dr.size() > (size_t)INT32_MAX)
+        return fail(BLS_ERR_UNSUPPORTED, "too many phase-2 entries");
     // longest items first (their landmark count sets the workgroup's run time): the dispatcher then fills the
     // tail with short items. Items are independent, so the order does not change any result.
     std::stable_sort(items.begin(), items.end(), [](const P2Item &x, const P2Item &y) {
