@@ -10,8 +10,8 @@
 # The device is selected by the workspace argument (a DeviceSchurWorkspace / DeviceQRWorkspace wrapping the handle of
 # the C ABI, include/bls_amd.h) or, for copyto!, by a DeviceBlockData source. Host arrays are staged through device
 # buffers owned by the workspace; results are copied back into the caller's arrays.
-# Further down: dot(x, H, y), getdiag! and prepare_scale! over a device copy of a filled Hessian, and the Cholesky
-# branch of the dense reduced solve.
+# Further down: dot(x, H, y), getdiag! and prepare_scale! over a device copy of a filled Hessian, the dense reduced
+# solve (Cholesky with its Bunch-Kaufman fallback), and the tables' fill from row-block Jacobian blocks.
 # Written against the C ABI; not executed in this repository (no Julia toolchain where it was built).
 module BLSAmd
 
@@ -401,10 +401,9 @@ function device_prepare_scale!(c::DeviceConditioning, l::DeviceHessianLayout, la
     return c.s                       # device pointer: pass as s_r / s_e (offset by nr) to the reduction
 end
 
-# ---- The dense reduced solve: the :cholesky branch of solve_reduced_dense! (src/schur_linear_systems.jl:312-319) --
+# ---- The dense reduced solve: solve_reduced_dense! (src/schur_linear_systems.jl:312-319) --------------------------
 
-# x_r = S \ g_S for the dense reduced S (upper triangle, as reduce_schur_dense! writes it): Success, or
-# NotPositiveDefinite with S left as built (the reference then tries :ldl, on the host)
+# x_r = S \ g_S by the Cholesky branch alone: Success, or NotPositiveDefinite with S left as built
 function device_cholesky_solve!(x_r::Vector{Float64}, S::Matrix{Float64}, g_S::Vector{Float64})
     n = size(S, 1)
     h = Ref{Ptr{Cvoid}}(C_NULL)
@@ -424,6 +423,83 @@ function device_cholesky_solve!(x_r::Vector{Float64}, S::Matrix{Float64}, g_S::V
     finally
         ccall((:bls_chol_destroy, libblsamd), Cvoid, (Ptr{Cvoid},), h[])
     end
+end
+
+# x_r = S \ g_S for the dense reduced S (upper triangle) as solve_reduced_dense! does it: :auto and :cholesky try
+# Cholesky and fall back to Bunch-Kaufman, :ldl goes to Bunch-Kaufman; :Success, :Indefinite (S has a negative
+# eigenvalue; x_r is solved) or :NotPositiveDefinite (a zero pivot: S singular)
+function device_solve_reduced_dense!(x_r::Vector{Float64}, S::Matrix{Float64}, g_S::Vector{Float64},
+                                     solver::Symbol=:auto)
+    solver in (:auto, :cholesky, :ldl) || throw(ArgumentError("$solver is not a reduced solver"))
+    solver === :ldl || device_cholesky_solve!(x_r, S, g_S) === :Success && return :Success
+    n = size(S, 1)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:bls_ldl_create, libblsamd), Cint, (Int64, Ptr{Ptr{Cvoid}}), n, h))
+    try
+        dS = DevBuf{Float64}(n * n); upload!(dS, vec(S))
+        zc, neg = Ref{Int64}(0), Ref{Int64}(0)
+        rc = check(ccall((:bls_ldl_factor_f64, libblsamd), Cint,
+                         (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}), h[], dS.ptr, zc, neg, C_NULL))
+        rc == BLS_FAILED && return :NotPositiveDefinite
+        dg = DevBuf{Float64}(n); upload!(dg, g_S)
+        dx = DevBuf{Float64}(n)
+        check(ccall((:bls_ldl_solve_f64, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                    h[], dg.ptr, dx.ptr, C_NULL))
+        download!(x_r, dx)
+        return neg[] > 0 ? :Indefinite : :Success
+    finally
+        ccall((:bls_ldl_destroy, libblsamd), Cvoid, (Ptr{Cvoid},), h[])
+    end
+end
+
+# ---- The tables' fill: block(J, k, blockentry(jst, k, j)) .= J_kj, residual_block(J, k) .= r_k (src/schur_jacobian.jl:264-387)
+
+"""
+    DeviceJacobianFill(jst::SchurJacobianStructure, rowblocks)
+
+The row blocks of a structure built from a description (rowblocks: row_table, row_first, row_nrows, row_ptr,
+row_blocks, as SchurJacobianStructure(blocksizes, rowblocks, rowblocksizes; eliminated) lays them out) registered
+with the device; `nvals` / `nres` are the lengths `device_fill!` takes.
+"""
+mutable struct DeviceJacobianFill
+    handle::Ptr{Cvoid}
+    nvals::Int
+    nres::Int
+end
+
+function DeviceJacobianFill(jst::BLS.SchurJacobianStructure, row_table::Vector{Int}, row_first::Vector{Int},
+                            row_nrows::Vector{Int}, row_ptr::Vector{Int}, row_blocks::Vector{Int})
+    st = jst.schur
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    nv, nr = Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:bls_jfill_create, libblsamd), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Ptr{Cvoid}}),
+                st.nrblocks, st.neblocks, collect(Int64, st.r_dims), collect(Int64, st.e_dims), jst.ntables,
+                jst.table_nrows, jst.table_ncols, jst.table_top, jst.raw_offsets, jst.residual_offsets, jst.table_ptr,
+                jst.entry_retained, jst.entry_col, length(row_table), row_table, row_first, row_nrows, row_ptr,
+                row_blocks, nv, nr, h))
+    f = DeviceJacobianFill(h[], nv[], nr[])
+    finalizer(x -> ccall((:bls_jfill_destroy, libblsamd), Cvoid, (Ptr{Cvoid},), x.handle), f)
+    return f
+end
+
+# every row block's blocks (vals: row block after row block, its blocks in ascending column block, each m x d
+# column-major) and residual rows (res) written into J.data and J.r through the device
+function device_fill!(J::BLS.SchurBlockJacobian{Float64}, f::DeviceJacobianFill, vals::Vector{Float64},
+                      res::Vector{Float64})
+    length(vals) == f.nvals || throw(DimensionMismatch("vals has length $(length(vals)), expected $(f.nvals)"))
+    length(res) == f.nres || throw(DimensionMismatch("res has length $(length(res)), expected $(f.nres)"))
+    draw = DevBuf{Float64}(length(J.data)); upload!(draw, J.data)
+    dr = DevBuf{Float64}(length(J.r)); upload!(dr, J.r)
+    dv = DevBuf{Float64}(length(vals)); upload!(dv, vals)
+    dres = DevBuf{Float64}(length(res)); upload!(dres, res)
+    check(ccall((:bls_jfill_f64, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                f.handle, draw.ptr, dr.ptr, dv.ptr, dres.ptr, C_NULL))
+    download!(J.data, draw)
+    download!(J.r, dr)
+    return J
 end
 
 end # module BLSAmd
