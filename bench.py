@@ -400,6 +400,12 @@ def config3_leg(torch, dev, b, scale, valu_scale=1.0, cpu=True, reps=3):
     kq = np.diff(sq.eliminated_ptr).astype(np.float64)
     useful_q = float(np.sum(108 * kq * kq + 162 * kq + 68))
     small = {"landmarks": int(sq.neblocks), "valu_work_list_s": plan_s}
+    # the work-list shape on this instance (columns of 256 block rows: wider items stage each landmark fewer times)
+    t0 = time.time()
+    dev.reset_schur_dense_(Sq, Rq, hq, srq, q.lam, check=False)
+    tuned = hq.autotune([(384, 96), (1536, 96), (1536, 192), (1536, 256)], Sq, gq, brq, beq, seq, q.lam, Eq, Cq, wq)
+    small["tuning"] = {"part": tuned[0], "lanes": tuned[1]}
+    log(f"dense leg autotune: part {tuned[0]}, lanes {tuned[1]} ({tuned[2]:.3f} ms reduce) ({time.time() - t0:.1f}s)")
     legs = [("valu_f64", lambda: dev.reduce_schur_dense_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq, check=False))]
     if valu_scale != 1.0:
         legs.append(("mfma_f64", lambda: dev.reduce_schur_dense_mfma_(Sq, gq, brq, beq, seq, q.lam, hq, Eq, Cq, wq,

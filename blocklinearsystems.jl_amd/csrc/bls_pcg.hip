@@ -409,6 +409,49 @@ __global__ void k_pcg_factor_groups(PcgArgs a, int64_t nblocks, double *Lout, co
     }
 }
 
+// The same, one wavefront per group (blocks b that start one; the others return): lane c holds column c of the
+// upper triangle in registers, and at step j the pivot and row j of U are broadcast by readlane; each entry receives
+// its updates in the order the column-by-column form above subtracts them, and is divided by its pivot, so the
+// factor is bitwise that of k_pcg_factor_groups (and of k_pcg_factor's for a block-diagonal group).
+constexpr int PCW = PC_GMAX * MAXD;   // largest group dimension
+__global__ __launch_bounds__(64) void k_pcg_factor_groups_w(PcgArgs a, int64_t nblocks, double *Lout,
+                                                           const double *Mblocks) {
+    const int64_t b = blockIdx.x;
+    if (b >= nblocks) return;
+    const int gd = group_dim(a, b);
+    if (!gd) return;
+    const int lane = threadIdx.x;
+    const double *M = Mblocks + a.gloff[b];
+    double col[PCW];
+#pragma unroll
+    for (int r = 0; r < PCW; r++) col[r] = (lane < gd && r <= lane) ? M[lane * gd + r] : 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < PCW; j++) {
+        if (j < gd && ok) {   // uniform
+            const double piv = __shfl(col[j], j, 64);
+            if (!(piv > 0.0) || !isfinite(piv)) {
+                ok = false;
+            } else {
+                const double ljj = sqrt(piv);
+                const double lij = lane == j ? ljj : col[j] / ljj;   // L(lane, j) for lanes > j
+                col[j] = lij;
+#pragma unroll
+                for (int r = j + 1; r < PCW; r++)
+                    if (r < gd) col[r] -= __shfl(lij, r, 64) * lij;   // entry (r, lane): L(r, j) L(lane, j)
+            }
+        }
+    }
+    if (!ok) {
+        if (lane == 0) a.st[a.prob_of_block[b]].failed = 1;
+        return;
+    }
+    double *L = Lout + a.gloff[b];
+#pragma unroll
+    for (int j = 0; j < PCW; j++)
+        if (lane < gd && j <= lane) L[j * gd + lane] = col[j];
+}
+
 // The group correction of block-Jacobi (the learned correction's family; no reference counterpart): the group of
 // consecutive blocks b .. b + gs - 1 that block b starts gets the block matrix with S's diagonal blocks on its
 // diagonal and w S_ij (w = omega[b] in [0, 1]; zero where the pattern has no block (i, j)) off it: w = 0 is plain
@@ -449,6 +492,51 @@ __global__ void k_pcg_group_blocks(PcgArgs a, const double *S, const int64_t *ma
     }
 }
 
+// The same, one wavefront per group: lanes 0 .. gs^2 - 1 find the pattern's blocks (i, j) of the group in their
+// block rows, then every lane writes entries of the gd x gd block (the values as k_pcg_group_blocks writes them).
+__global__ __launch_bounds__(64) void k_pcg_group_blocks_w(PcgArgs a, const double *S, const int64_t *map,
+                                                          const int64_t *diag_voff, const float *omega, double *M) {
+    __shared__ int64_t voff[PC_GMAX * PC_GMAX];
+    __shared__ int boff[PC_GMAX + 1];
+    const int64_t b = blockIdx.x;
+    if (b >= a.nrb) return;
+    const int gd = group_dim(a, b);
+    if (!gd) return;
+    const int gs = a.gsize[b];
+    const int lane = threadIdx.x;
+    const double w = fmin(1.0, fmax(0.0, (double)omega[b]));
+    if (lane < gs * gs) {
+        const int i = lane / gs, j = lane % gs;
+        int64_t v = -1;
+        if (i == j) {
+            v = diag_voff[b + i];
+        } else if (w != 0.0) {
+            for (int k = a.bsr_rowptr[b + i]; k < a.bsr_rowptr[b + i + 1]; k++)
+                if (a.bsr_col[k] == b + j) v = a.bsr_voff[k];
+        }
+        voff[lane] = v;
+    }
+    if (lane == 0) {
+        int o = 0;
+        for (int i = 0; i < gs; i++) { boff[i] = o; o += a.dims[b + i]; }
+        boff[gs] = o;
+    }
+    __syncthreads();
+    double *out = M + a.gloff[b];
+    for (int q = lane; q < gd * gd; q += 64) {
+        const int row = q % gd, cl = q / gd;
+        int i = 0, j = 0;
+        while (row >= boff[i + 1]) i++;
+        while (cl >= boff[j + 1]) j++;
+        const int r = row - boff[i], c = cl - boff[j];
+        const int64_t v = voff[i * gs + j];
+        double x = 0.0;
+        if (i == j) x = S[map[v + r * a.dims[b + i] + c] - 1];
+        else if (v >= 0) x = w * S[map[v + r * a.dims[b + j] + c] - 1];
+        out[q] = x;
+    }
+}
+
 // The learned correction of the block-Jacobi preconditioner (no reference counterpart): per retained block,
 // features for a small MLP, and the corrected blocks M_b = D_b + mu_b m_b I (D_b the diagonal block of S, m_b the
 // mean of its diagonal), read from S through the handle's BSR map.
@@ -458,22 +546,22 @@ constexpr int PC_NF = 26;
 // Features of block row i (dimension 6; other dimensions get zeros): the 21 upper entries of D / m, log10 m,
 // log(1 + number of off-diagonal blocks), log10 lambda, log10(m / lambda), log10(off-diagonal Frobenius mass of the
 // block row / ||D||_F). One wavefront per block row; the off-diagonal sum is a fixed-order wave reduction.
-__global__ __launch_bounds__(256) void k_pcg_features(PcgArgs a, const double *S, const int64_t *map,
-                                                       const int64_t *diag_voff, double lambda, float *F) {
+__global__ __launch_bounds__(256) void k_pcg_features(PcgArgs a, const double *vals, const int64_t *diag_voff,
+                                                       double lambda, float *F) {
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (i >= a.nrb) return;
     const int di = a.dims[i];
+    // the block row's values lie one after another in the gathered block-row copy: the off-diagonal ones, the lanes
+    // over the run (coalesced), each lane adding its values in order
+    const int kl = a.bsr_rowptr[i + 1] - 1;
+    const int64_t v0 = a.bsr_voff[a.bsr_rowptr[i]], v1 = a.bsr_voff[kl] + (int64_t)di * a.dims[a.bsr_col[kl]];
+    const int64_t d0 = diag_voff[i], d1 = d0 + (int64_t)di * di;
     double off = 0.0;
-    for (int k = a.bsr_rowptr[i] + lane; k < a.bsr_rowptr[i + 1]; k += 64) {
-        const int j = a.bsr_col[k];
-        if (j == i) continue;
-        const int64_t v0 = a.bsr_voff[k];
-        const int n = di * a.dims[j];
-        for (int q = 0; q < n; q++) {
-            const double x = S[map[v0 + q] - 1];
-            off += x * x;
-        }
+#pragma unroll 4
+    for (int64_t v = v0 + lane; v < v1; v += 64) {
+        const double x = vals[v];
+        off += (v >= d0 && v < d1) ? 0.0 : x * x;
     }
     off = wave_sum64(off);
     if (lane != 0) return;
@@ -482,7 +570,7 @@ __global__ __launch_bounds__(256) void k_pcg_features(PcgArgs a, const double *S
     if (di != 6) return;
     double D[36], m = 0.0, dd = 0.0;
     for (int q = 0; q < 36; q++) {
-        D[q] = S[map[diag_voff[i] + q] - 1];   // row-major
+        D[q] = vals[d0 + q];   // row-major
         dd += D[q] * D[q];
     }
     for (int r = 0; r < 6; r++) m += D[r * 6 + r];
@@ -762,8 +850,12 @@ extern "C" int bls_pcg_features_f32(bls_pcg *h, const double *S_nzval, double la
                                     void *stream) {
     if (!h || !S_nzval || !features) return fail(BLS_ERR_ARG, "bls_pcg_features_f32: bad arguments");
     PcgArgs a = h->args(1.0, 0);
-    k_pcg_features<<<(unsigned)((h->nrb + 3) / 4), 256, 0, (hipStream_t)stream>>>(a, S_nzval, h->d_map.ptr,
-                                                                                 h->d_diag_voff.ptr, lambda, features);
+    hipStream_t s = (hipStream_t)stream;
+    // S gathered into the block-row copy (as a solve starts), then read row by row
+    k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval,
+                                                                                 h->d_map.ptr, h->nvals);
+    k_pcg_features<<<(unsigned)((h->nrb + 3) / 4), 256, 0, s>>>(a, h->d_vals.ptr, h->d_diag_voff.ptr, lambda,
+                                                               features);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
@@ -834,7 +926,7 @@ static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double 
     k_copy_f64<<<std::min<int64_t>(grid_for(h->nr, 256), 4096), 256, 0, s>>>(h->d_g.ptr, g, h->nr);
     k_pcg_init_state<<<grid_for(nprob, 256), 256, 0, s>>>(h->d_st.ptr, nprob, h->d_nactive.ptr);
     if (groups)
-        k_pcg_factor_groups<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_Lp.ptr, precond_blocks);
+        k_pcg_factor_groups_w<<<(unsigned)h->nrb, 64, 0, s>>>(a, h->nrb, h->d_Lp.ptr, precond_blocks);
     else
         k_pcg_factor<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_diag_voff.ptr, h->d_L.ptr, precond_blocks);
     k_pcg_restart<<<nprob, UPD_THREADS, 0, s>>>(a, 1);
@@ -922,8 +1014,8 @@ extern "C" int bls_pcg_group_blocks_f64(bls_pcg *h, const double *S_nzval, const
     if (!h || !S_nzval || !omega || !blocks) return fail(BLS_ERR_ARG, "bls_pcg_group_blocks_f64: bad arguments");
     if (h->group == 0) return fail(BLS_ERR_STATE, "no preconditioner groups (bls_pcg_set_groups)");
     PcgArgs a = h->args(1.0, 0, true);
-    k_pcg_group_blocks<<<grid_for(h->nrb, 256), 256, 0, (hipStream_t)stream>>>(a, S_nzval, h->d_map.ptr,
-                                                                            h->d_diag_voff.ptr, omega, blocks);
+    k_pcg_group_blocks_w<<<(unsigned)h->nrb, 64, 0, (hipStream_t)stream>>>(a, S_nzval, h->d_map.ptr,
+                                                                        h->d_diag_voff.ptr, omega, blocks);
     BLS_HIP(hipGetLastError());
     return BLS_OK;
 }
