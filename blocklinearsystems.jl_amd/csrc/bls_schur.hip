@@ -119,6 +119,11 @@ template <class P> struct P2Args {
     const unsigned long long *flag;
     int uniform_offsets;   // one run of d_e == DE and one d_r: E/z/C offsets are e*DE^2, e*DE, c*cstride
     int64_t cstride;
+    // the reduction's verdict from the last workgroup to finish (no split items): the lowest failing block into
+    // fin_out, the flag re-armed; fin_counter counts finished workgroups and is left at 0 (NULL: a separate kernel)
+    unsigned int *fin_counter;
+    unsigned long long *fin_flag;
+    int64_t *fin_out;
 };
 
 // ---------------------------------------------------------------------------------------------------------
@@ -219,25 +224,58 @@ __global__ __launch_bounds__(256) void k_phase1(int64_t first, int64_t count, in
                                                 const double *__restrict__ s_e, typename P::TEI *__restrict__ E_inv,
                                                 double *__restrict__ z, unsigned long long *flag) {
     using F = typename std::conditional<P::F32_FACTOR, float, double>::type;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int64_t eo = eoff0 + i * DE * DE, bo = boff0 + i * DE;
+    using TEI = typename P::TEI;
+    // blocks of up to 4 x 4: the workgroup's E blocks are read, and its E_inv blocks written, as one contiguous
+    // run through LDS (coalesced), each lane then taking its own block from LDS
+    constexpr bool STG = DE <= 4;
+    constexpr int NQ = DE * DE;
+    __shared__ double sbuf[STG ? 256 * NQ : 1];
+    __shared__ unsigned char sok[STG ? 256 : 1];
+    const int t = threadIdx.x;
+    const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t i = w0 + t;
+    const bool valid = i < count;
+    const int64_t nw = count - w0 < (int64_t)blockDim.x ? count - w0 : (int64_t)blockDim.x;
+    if constexpr (STG) {
+        for (int q = t; q < nw * NQ; q += 256) sbuf[q] = (double)E[eoff0 + w0 * NQ + q];
+        __syncthreads();
+    } else {
+        if (!valid) return;
+    }
+    const int64_t eo = eoff0 + i * NQ, bo = boff0 + i * DE;
     double se[DE];
-    F damp[DE], H[DE * DE], inv[DE * DE];
+    F damp[DE], H[NQ], inv[NQ];
+    bool ok = false;
+    if (valid) {
 #pragma unroll
-    for (int k = 0; k < DE; k++) {
-        se[k] = s_e ? s_e[bo + k] : 1.0;
-        damp[k] = (F)((lambda * scaling_e[bo + k]) * (se[k] * se[k]));
+        for (int k = 0; k < DE; k++) {
+            se[k] = s_e ? s_e[bo + k] : 1.0;
+            damp[k] = (F)((lambda * scaling_e[bo + k]) * (se[k] * se[k]));
+        }
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            const typename P::TE ev = STG ? (typename P::TE)sbuf[t * NQ + k] : E[eo + k];
+            H[k] = s_e ? (F)((double)ev * (se[k % DE] * se[k / DE])) : (F)ev;
+        }
+        ok = inv_sym<DE, F>(H, damp, inv);
+        if (!ok) atomicMin(flag, (unsigned long long)(first + i + 1));
     }
+    if constexpr (STG) {
+        __syncthreads();   // every lane has read its E block
+        if (valid && ok) {
 #pragma unroll
-    for (int k = 0; k < DE * DE; k++)
-        H[k] = s_e ? (F)((double)E[eo + k] * (se[k % DE] * se[k / DE])) : (F)E[eo + k];
-    if (!inv_sym<DE, F>(H, damp, inv)) {
-        atomicMin(flag, (unsigned long long)(first + i + 1));
-        return;
+            for (int k = 0; k < NQ; k++) sbuf[t * NQ + k] = (double)(TEI)inv[k];
+        }
+        sok[t] = valid && ok;
+        __syncthreads();
+        for (int q = t; q < nw * NQ; q += 256)
+            if (sok[q / NQ]) E_inv[eoff0 + w0 * NQ + q] = (TEI)sbuf[q];   // a failing block's E_inv is not written
+    } else {
+        if (!ok) return;
+#pragma unroll
+        for (int k = 0; k < NQ; k++) E_inv[eo + k] = (TEI)inv[k];
     }
-#pragma unroll
-    for (int k = 0; k < DE * DE; k++) E_inv[eo + k] = (typename P::TEI)inv[k];
+    if (!valid || !ok) return;
 #pragma unroll
     for (int r = 0; r < DE; r++) {
         double acc = 0.0;
@@ -570,11 +608,10 @@ __global__ __launch_bounds__(P2_THREADS, (DE > 0 && DRM <= 6) ? 4 : (DRM <= 9 ? 
     __shared__ P2Meta meta[3];
     __shared__ int s_slot_dr[64];
 
-    if (*a.flag != (unsigned long long)NO_FAILURE) return;   // phase 1 failed: S and g_S stay untouched
-    // the dispatcher deals workgroups to the 8 XCDs round robin, so with items longest first every XCD gets a
-    // longest-first share of the work
+    // phase 1 failed: S and g_S stay untouched. The dispatcher deals workgroups to the 8 XCDs round robin, so
+    // with items longest first every XCD gets a longest-first share of the work
     const int64_t idx = blockIdx.x;
-    if (idx >= a.nitems) return;
+    if (*a.flag == (unsigned long long)NO_FAILURE && idx < a.nitems) {
     const P2Item it = a.items[idx];
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -757,6 +794,16 @@ __global__ __launch_bounds__(P2_THREADS, (DE > 0 && DRM <= 6) ? 4 : (DRM <= 9 ? 
             for (int c = 0; c < DRM; c++)
                 if (c < dr2) prow[c] = acc[c];
             prow[dr2] = gacc;
+        }
+    }
+    }
+    if (a.fin_counter) {
+        __syncthreads();
+        if (threadIdx.x == 0 && atomicAdd(a.fin_counter, 1u) == gridDim.x - 1) {   // every workgroup is done
+            const unsigned long long f = *a.fin_flag;
+            *a.fin_out = f == (unsigned long long)NO_FAILURE ? 0 : (int64_t)f;
+            *a.fin_flag = (unsigned long long)NO_FAILURE;
+            *a.fin_counter = 0u;
         }
     }
 }
@@ -1462,6 +1509,7 @@ struct bls_schur {
     DevArray<int2> d_sy_tiles;
     int64_t sy_ntiles = 0;
     DevArray<unsigned long long> d_flag;
+    DevArray<unsigned int> d_fin_counter;   // phase 2's finished workgroups (the folded verdict), 0 between calls
     bool flag_dirty = true;   // d_flag may not hold NO_FAILURE (before the first reduction, or after one that errored)
     DevArray<int64_t> d_failed_out;
     int64_t nitems = 0, nentries = 0;
@@ -1888,9 +1936,9 @@ template <class P> void launch_phase2(int de, int max_dr, unsigned grid, hipStre
 }
 
 template <class Q>
-void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, const typename Q::TC *C,
+bool run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, const typename Q::TC *C,
                 const double *b_r, const double *s_r, typename Q::TS *S, double *g, const P2Scratch &ws,
-                hipStream_t s) {
+                int64_t *d_failed, hipStream_t s) {
     P2Args<Q> a{};
     a.items = h->d_items.ptr;
     a.nitems = h->nitems;
@@ -1914,6 +1962,14 @@ void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, con
     a.flag = ws.flag;
     a.uniform_offsets = h->uniform_de > 0 && h->uniform_dr_all > 0 && h->runs.size() == 1;
     a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
+    // the verdict folded into the kernel's last workgroup when nothing runs after it (no split items) and the
+    // handle's own flag is used
+    const bool fold = h->nitems > 0 && h->nmerges == 0 && ws.flag == h->d_flag.ptr && h->d_fin_counter.ptr;
+    if (fold) {
+        a.fin_counter = h->d_fin_counter.ptr;
+        a.fin_flag = ws.flag;
+        a.fin_out = d_failed;
+    }
     if (h->nitems > 0) {
         launch_phase2<Q>(h->uniform_de, h->max_dr, (unsigned)h->nitems, s, a);
         h->timing.mark(3, s);
@@ -1923,6 +1979,7 @@ void run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, con
     }
     if (h->nitems == 0) h->timing.mark(3, s);
     h->timing.mark(4, s);
+    return fold;
 }
 
 // Wide mode's W, S and g_S (every block size): see k_phase2_wide.
@@ -1972,6 +2029,7 @@ int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, cons
     if (own_flag) h->flag_dirty = true;   // until the finalize below is queued
     int rc = phase1<P>(h, E, b_e, scaling_e, lambda, s_e, E_inv, z, ws.flag, s);
     if (rc) return rc;
+    bool folded = false;
     if ((s_r || s_e) && h->neb) {
         using Q = typename ScaledPolicy<P>::type;
         using TSC = typename P::TSC;
@@ -1982,13 +2040,13 @@ int reduce_async(bls_schur *h, int want_mode, typename P::TS *S, double *g, cons
                                                                   h->d_boffsets_r.ptr, C, s_r, s_e, Cs);
         h->timing.mark(2, s);
         if (h->wide) run_phase2_wide<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, ws, s);
-        else run_phase2<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, ws, s);
+        else folded = run_phase2<Q>(h, E_inv, z, Cs, b_r, s_r, S, g, ws, d_failed, s);
     } else {
         h->timing.mark(2, s);
         if (h->wide) run_phase2_wide<P>(h, E_inv, z, C, b_r, s_r, S, g, ws, s);
-        else run_phase2<P>(h, E_inv, z, C, b_r, s_r, S, g, ws, s);
+        else folded = run_phase2<P>(h, E_inv, z, C, b_r, s_r, S, g, ws, d_failed, s);
     }
-    k_finalize_flag<<<1, 1, 0, s>>>(ws.flag, d_failed);
+    if (!folded) k_finalize_flag<<<1, 1, 0, s>>>(ws.flag, d_failed);
     BLS_HIP(hipGetLastError());
     if (own_flag) h->flag_dirty = false;
     return BLS_OK;
@@ -2121,7 +2179,10 @@ extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, 
         (rc = h->d_coffsets.upload(h->coffsets)) || (rc = h->d_eliminated_ptr.upload(h->eliminated_ptr)) ||
         (rc = h->d_boffsets_r.upload(h->boffsets_r)) || (rc = h->d_e_dims.upload(narrow32(h->e_dims))) ||
         (rc = h->d_r_dims.upload(narrow32(h->r_dims))) || (rc = h->d_entry_retained.upload(narrow32(h->entry_retained))) ||
-        (rc = h->d_flag.alloc(1)) || (rc = h->d_failed_out.alloc(1))) {
+        (rc = h->d_flag.alloc(1)) || (rc = h->d_failed_out.alloc(1)) || (rc = h->d_fin_counter.alloc(1)) ||
+        (rc = hipMemset(h->d_fin_counter.ptr, 0, sizeof(unsigned int)) == hipSuccess
+                  ? BLS_OK
+                  : fail(BLS_ERR_HIP, "hipMemset of the phase-2 counter failed"))) {
         delete h;
         return rc;
     }
