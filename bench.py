@@ -9,7 +9,7 @@ A step is one such pass with every input resident in HBM. Prints one JSON line.
 
 --calibration-copy first runs one device copy of 256 MiB (a kernel of known traffic) so that a hardware-counter
 run of this command can convert FETCH_SIZE / WRITE_SIZE into bytes. roofline.traffic is read from the committed
-counter summary profiles/k_phase2_slam_traffic.md when present.
+counter summary profiles/<phase-2 kernel>_slam_traffic.md when present.
 """
 import argparse
 import json
@@ -24,13 +24,12 @@ sys.path.insert(0, ROOT)
 METRIC = "landmarks eliminated/s + time-to-1e-8 PCG residual, FP64 plain vs mixed/learned"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TF = 78.6
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_phase2_slam_traffic.md")
 
 
-def measured_traffic():
-    """HBM bytes per k_phase2 launch from the counter summary (its line `traffic_bytes_per_launch: N`)."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of the phase-2 kernel from its counter summary (line `traffic_bytes_per_launch: N`)."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(os.path.join(ROOT, "profiles", f"{kernel}_slam_traffic.md")) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -125,8 +124,11 @@ def main():
         tuned = cands[0] + (float("nan"),)
     else:
         tuned = h.autotune(cands, S, g, b_r, b_e, se, lam, E, C, ws)
+    info = h.phase2_info()
+    p2_kernel = "k_tiles" if info[0] == 2 else "k_phase2"
     log(f"autotune: part {tuned[0]}, lanes {tuned[1]} ({tuned[2]:.4f} ms reduce; default 384 / 96) "
-        f"over {len(cands)} shapes ({time.time() - t0:.1f}s)")
+        f"over {len(cands)} shapes and the tiles ({time.time() - t0:.1f}s); phase 2 runs {p2_kernel} "
+        f"(tile plan: {info[1]} tiles, {info[2]} landmark entries, {info[3]} matrix-core products)")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -211,14 +213,18 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded PCG64 generator)",
         "graph_ms_per_step": graph_ms, "step": "reset + reduce + copyto!, device-resident inputs",
-        "tuning": {"part": tuned[0], "lanes": tuned[1], "candidates": len(cands)},
+        "tuning": {"part": tuned[0], "lanes": tuned[1], "candidates": len(cands), "phase2_kernel": p2_kernel,
+                   "tiles": info[1], "tile_entries": info[2], "tile_mfma": info[3]},
         "config": {"workload": "reduce_schur_sparse! + copyto!, sequential SLAM, 1000 cams (d6) x 200000 landmarks (d3), "
                                "FP64", "landmarks": int(st.neblocks), "cameras": int(st.nrblocks),
                    "couplings": int(st.ncoup), "S_nnz": int(nnz), "csc_nnz": int(len(smap))},
         "kernel_ms": {"reset": kt[0], "phase1": kt[1], "phase2": kt[2], "phase2_merge": kt[3]},
-        "roofline": {"bound": "hbm", "kernel": "k_phase2", "achieved": p2_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": p2_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "step_achieved": whole_bytes / (ms * 1e-3) / 1e9},
+        "roofline": {"bound": "hbm", "kernel": p2_kernel, "achieved": p2_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": p2_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(p2_kernel),
+                     "step_achieved": whole_bytes / (ms * 1e-3) / 1e9,
+                     # the tiles' issued FP64 matrix-core rate (16 x 16 x 4 products, K padded from d_e)
+                     "mfma_issued_TFs": (info[3] * 2048.0 / (kt[2] * 1e-3) / 1e12) if info[0] == 2 else None,
+                     "mfma_peak_TFs": FP64_MFMA_PEAK_TF},
         "pcie_inclusive": pcie,
         "mixed": mixed,
         "assembly": assembly,

@@ -54,6 +54,39 @@ template <typename T> struct DevArray {
 
 constexpr int64_t NO_FAILURE = INT64_MAX;
 
+// Phase 2 on 16 x 16 tiles of S (bls_tiles.hip). A unit is one tile (ti, tj) of S; its landmarks are the first `cnt`
+// entries of tile row ti's list (elo), whose couplings lie in [cbase, cbase + spanc).
+struct TileUnit {
+    int32_t ti, tj, cnt, has_g;
+    int32_t rb0, spanc;   // first retained block of the tile row's rows; couplings the row's landmarks reach
+    int64_t elo, cbase;
+};
+struct TileEntry {
+    int32_t e;      // landmark (0-based)
+    int32_t cw;     // its first coupling at or after block rb0, relative to cbase
+    int32_t qfql;   // that coupling's block - rb0 (low 16 bits), its last coupling's block - rb0 (high 16 bits)
+    int32_t pad;
+};
+struct TilePlan {
+    DevArray<TileUnit> units;
+    DevArray<TileEntry> entries;
+    DevArray<int32_t> tabs;   // per unit: offsets in S's columns of (row block q, column block p), 4 x 4, -1 none
+    int64_t nunits = 0, nentries = 0, nmfma = 0;
+    int de = 0, dr = 0;
+};
+// What the plan is built from (0-based host arrays of the handle and its registered sparse layout).
+struct TileStructure {
+    int64_t nr, nrb, neb;
+    const std::vector<int64_t> &e_dims, &r_dims, &eoffsets, &boffsets_e, &boffsets_r, &coffsets, &eptr, &eret;
+    const std::vector<int64_t> *bcp, *brv, *bro;
+};
+int tiles_build(TilePlan &P, const TileStructure &T, std::string *why);
+// variant 0: Float64 (E_inv, C, S double); 1: Float32 (float E_inv, C, S); 2: mixed (double E_inv, float C, double S)
+int tiles_launch(const TilePlan &P, int variant, const void *E_inv, const double *z, const void *C, const double *b_r,
+                 const double *s_r, void *S, double *g, const int64_t *S_colptr, int64_t nr,
+                 const unsigned long long *flag, unsigned *fin_counter, unsigned long long *fin_flag, int64_t *fin_out,
+                 hipStream_t s);
+
 // Blocks per grid for n items at `threads` per block.
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 

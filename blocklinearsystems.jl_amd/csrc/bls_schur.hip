@@ -1527,6 +1527,13 @@ struct bls_schur {
     DevArray<WidePair> d_wpairs;     // stored blocks of S with coupling pairs
     DevArray<int2> d_wcc;            // their (c1, c2) lists
     int64_t nwpairs = 0;
+    // phase 2 on 16 x 16 tiles of S (matrix cores), for sparse layouts whose structure fits (bls_tiles.hip); p2_mode:
+    // 0 the tiles when they fit, else the owner-computes work list; 1 always the work list; 2 the tiles (an error
+    // when they do not fit)
+    TilePlan tiles;
+    std::string tiles_why;
+    int p2_mode = 0;
+    bool use_tiles() const { return mode == 1 && !wide && tiles.nunits > 0 && p2_mode != 1; }
 };
 
 namespace {
@@ -1964,6 +1971,19 @@ bool run_phase2(bls_schur *h, const typename Q::TEI *E_inv, const double *z, con
     a.cstride = a.uniform_offsets ? (int64_t)h->uniform_de * h->uniform_dr_all : 0;
     // the verdict folded into the kernel's last workgroup when nothing runs after it (no split items) and the
     // handle's own flag is used
+    if (h->use_tiles()) {
+        // the tile plan: no split items, so the verdict is always folded into its last workgroup when the handle's
+        // own flag is used
+        const bool tfold = ws.flag == h->d_flag.ptr && h->d_fin_counter.ptr;
+        const int variant = all_f64<Q>() ? 0 : (sizeof(typename Q::TS) == 4 ? 1 : 2);
+        if (tiles_launch(h->tiles, variant, E_inv, z, C, b_r, s_r, S, g, h->d_S_colptr.ptr, h->nr, ws.flag,
+                         tfold ? h->d_fin_counter.ptr : nullptr, tfold ? ws.flag : nullptr, tfold ? d_failed : nullptr,
+                         s))
+            return false;
+        h->timing.mark(3, s);
+        h->timing.mark(4, s);
+        return tfold;
+    }
     const bool fold = h->nitems > 0 && h->nmerges == 0 && ws.flag == h->d_flag.ptr && h->d_fin_counter.ptr;
     if (fold) {
         a.fin_counter = h->d_fin_counter.ptr;
@@ -2223,6 +2243,13 @@ extern "C" int bls_schur_set_sparse(bls_schur *h, int64_t nnz, const int64_t *S_
     h->lay_ch.assign(col_heights, col_heights + h->nrb);
     int rc = build_sparse_phase2(h);
     if (rc) return rc;
+    if (!h->wide) {
+        const TileStructure T{h->nr, h->nrb, h->neb, h->e_dims, h->r_dims, h->eoffsets, h->boffsets_e, h->boffsets_r,
+                              h->coffsets, h->eliminated_ptr, h->entry_retained, &h->lay_bcp, &h->lay_brv, &h->lay_bro};
+        if ((rc = tiles_build(h->tiles, T, &h->tiles_why))) return rc;
+    } else {
+        h->tiles_why = "blocks above the register path";
+    }
     std::vector<int64_t> rts(r_to_s_map, r_to_s_map + nR), di(diag_inds, diag_inds + h->nr);
     // scalar row of every stored entry (1-based), for the conditioning scale of reset_schur_sparse!: a scalar column
     // of block column r2 holds, in order, the rows of the blocks (r1, r2) of the pattern
@@ -2733,6 +2760,27 @@ extern "C" int bls_schur_kernel_times(bls_schur *h, float *times, int n) {
     return BLS_OK;
 }
 
+// ---- which phase-2 kernel runs --------------------------------------------------------------------------------------
+extern "C" int bls_schur_set_phase2_mode(bls_schur *h, int mode) {
+    if (!h) return fail(BLS_ERR_ARG, "null handle");
+    if (mode < 0 || mode > 2) return fail(BLS_ERR_ARG, "phase-2 mode must be 0 (automatic), 1 (work list) or 2 (tiles)");
+    if (mode == 2 && !(h->mode == 1 && !h->wide && h->tiles.nunits > 0))
+        return fail(BLS_ERR_UNSUPPORTED, "the tiled phase 2 does not fit this structure: " +
+                                             (h->mode != 1 ? std::string("no sparse layout") : h->tiles_why));
+    h->p2_mode = mode;
+    return BLS_OK;
+}
+
+// info[0]: the kernel in use (1 work list, 2 tiles); info[1..3]: the tile plan's tiles, landmark entries and MFMAs
+extern "C" int bls_schur_phase2_info(bls_schur *h, int64_t *info) {
+    if (!h || !info) return fail(BLS_ERR_ARG, "bad arguments");
+    info[0] = h->use_tiles() ? 2 : 1;
+    info[1] = h->tiles.nunits;
+    info[2] = h->tiles.nentries;
+    info[3] = h->tiles.nmfma;
+    return BLS_OK;
+}
+
 // ---- the shape of the phase-2 work list ------------------------------------------------------------------------
 extern "C" int bls_schur_set_tuning(bls_schur *h, int64_t part, int64_t lanes) {
     if (!h) return fail(BLS_ERR_ARG, "null handle");
@@ -2768,8 +2816,8 @@ extern "C" int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t
     float best = -1.0f;
     int64_t bp = h->p2_part, bl = h->p2_lanes;
     int rc = BLS_OK;
-    for (int64_t c = 0; c < ncand && rc == BLS_OK; c++) {
-        if ((rc = bls_schur_set_tuning(h, parts[c], lanes[c]))) break;
+    // the median of three timed reductions after an untimed one
+    auto time_one = [&](float *ms) {
         float t[3] = {0.0f, 0.0f, 0.0f};
         for (int rep = 0; rep < 4 && rc == BLS_OK; rep++) {
             if (rep > 0 && hipEventRecord(e0, st) != hipSuccess) rc = fail(BLS_ERR_HIP, "hipEventRecord");
@@ -2782,16 +2830,36 @@ extern "C" int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t
                 rc = fail(BLS_ERR_HIP, "autotune timing");
         }
         std::sort(t, t + 3);
-        const float ms = t[1];
+        *ms = t[1];
+    };
+    // the work-list shapes, then (when the structure fits them) the tiles
+    const int mode0 = h->p2_mode;
+    h->p2_mode = 1;
+    for (int64_t c = 0; c < ncand && rc == BLS_OK; c++) {
+        if ((rc = bls_schur_set_tuning(h, parts[c], lanes[c]))) break;
+        float ms = 0.0f;
+        time_one(&ms);
         if (rc == BLS_OK && (best < 0 || ms < best)) {
             best = ms;
             bp = parts[c];
             bl = lanes[c];
         }
     }
+    int mode = mode0 == 2 ? 2 : 1;
+    if (rc == BLS_OK && mode0 != 1 && h->mode == 1 && !h->wide && h->tiles.nunits > 0) {
+        h->p2_mode = 2;
+        float ms = 0.0f;
+        time_one(&ms);
+        if (rc == BLS_OK && ms < best) {
+            best = ms;
+            mode = mode0;
+        }
+    }
+    h->p2_mode = mode0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (best_ms) *best_ms = best;
+    h->p2_mode = mode;
     return bls_schur_set_tuning(h, bp, bl);
 }

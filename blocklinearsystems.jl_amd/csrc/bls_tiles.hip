@@ -1,0 +1,400 @@
+// Phase 2 of the Schur reduction on the FP64 matrix cores, for structures whose landmarks couple contiguous windows
+// of retained blocks (every block of uniform size d_r >= 6, every landmark block of uniform size d_e <= 4).
+//
+// S is cut into 16 x 16 scalar tiles. A workgroup owns one tile (ti, tj), tj >= ti, and sums into it, for every
+// landmark e that couples a block of tile row ti and a block of tile column tj,
+//     D[i][j] += sum_k W1[k][i] C2[k][j],   W1 = E_inv_e C_e(rows of the row block of i),  C2 = C_e(column j)
+// which is the reference's S[r1, r2] -= W_{c1}' C_{c2} (src/schur_reduction.jl:315-380) for the block pairs the tile
+// covers, as one v_mfma_f64_16x16x4f64 per landmark (K = d_e components, padded to 4). The tile's 16 rows and columns
+// are scalar rows and columns, so a tile spans parts of up to four blocks on each side; a landmark adds zero where it
+// does not couple a block (buffer loads outside the window return 0).
+//
+// The landmarks of tile row ti are one list, sorted by the last tile column they reach (descending, then landmark
+// order): the landmarks reaching tile column tj are a prefix of that list, so every tile of the row reads the same
+// list. The four waves of a workgroup take equal contiguous shares of the prefix; their partial tiles are added in
+// wave order through LDS and wave 0 writes S. Each S entry is owned by one tile: no atomics, and the sums are taken in
+// a fixed order (bitwise reproducible). g_S = b_r s_r - sum C' z is formed by the diagonal tile (tj == ti), whose
+// prefix is the whole list.
+//
+// Per landmark a lane loads one value of the row side (its column i of the row block's coupling, component k) and one
+// of the column side; three lane-row swaps (v_permlane16/32_swap) give every lane the d_e components of its column,
+// and E_inv (with z) comes from one coalesced load per group of landmarks through LDS.
+#include "bls_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace bls {
+
+namespace {
+
+constexpr int TW = 4;   // waves per workgroup
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// every lane gets x of lane rows 0..3 (same column) in fixed order: three permlane swaps per 32-bit half
+__device__ __forceinline__ void lane_rows(double x, double *xo) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto ah = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    auto c = __builtin_amdgcn_permlane32_swap(a[0], a[0], false, false);
+    auto ch = __builtin_amdgcn_permlane32_swap(ah[0], ah[0], false, false);
+    auto e = __builtin_amdgcn_permlane32_swap(a[1], a[1], false, false);
+    auto eh = __builtin_amdgcn_permlane32_swap(ah[1], ah[1], false, false);
+    xo[0] = __hiloint2double((int)ch[0], (int)c[0]);
+    xo[1] = __hiloint2double((int)eh[0], (int)e[0]);
+    xo[2] = __hiloint2double((int)ch[1], (int)c[1]);
+    xo[3] = __hiloint2double((int)eh[1], (int)e[1]);
+}
+
+template <int DE> constexpr int tile_group() { return 64 / (DE * DE + DE) < 5 ? 64 / (DE * DE + DE) : 5; }
+
+template <int U> struct TileGroup {
+    int4 ev;     // lane u < U: record of slot u
+    double ez;   // lane l < U (DE^2 + DE): E_inv value l % DE^2 of slot l / DE^2, then the z values
+    double x[U], b[U];
+};
+
+template <int DE, class TC, class TEI, class TS>
+__global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ units, const TileEntry *__restrict__ entries,
+                                                  const int32_t *__restrict__ tabs, const TEI *__restrict__ Einv,
+                                                  const double *__restrict__ z, const TC *__restrict__ C,
+                                                  const double *__restrict__ b_r, const double *__restrict__ s_r,
+                                                  TS *S, double *g, const int64_t *__restrict__ S_colptr, int64_t nr,
+                                                  int dr, const unsigned long long *flag, unsigned *fin_counter,
+                                                  unsigned long long *fin_flag, int64_t *fin_out) {
+    constexpr int U = tile_group<DE>();
+    constexpr int EZN = DE * DE + DE;
+    __shared__ double red[TW - 1][256];
+    __shared__ double gred[TW - 1][16];
+    __shared__ double ezs[TW][2 * U * EZN];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, k = lane >> 4, i = lane & 15;
+    if (*flag == (unsigned long long)NO_FAILURE) {
+        const TileUnit *up = units + blockIdx.x;
+        const int ti = __builtin_amdgcn_readfirstlane(up->ti), tj = __builtin_amdgcn_readfirstlane(up->tj);
+        const int cnt = __builtin_amdgcn_readfirstlane(up->cnt), has_g = __builtin_amdgcn_readfirstlane(up->has_g);
+        const int rb0 = __builtin_amdgcn_readfirstlane(up->rb0), spanc = __builtin_amdgcn_readfirstlane(up->spanc);
+        const int cst = DE * dr;
+        const int lo = cnt * w / TW, hi = cnt * (w + 1) / TW;
+        const TileEntry *ep = entries + up->elo;
+        // the lane's row (16 ti + i) and column (16 tj + i): block relative to rb0 and byte offset of its value
+        // (component k) inside a coupling block, d_e x d_r column-major
+        const int64_t R = 16ll * ti + i, Cc = 16ll * tj + i;
+        const bool rv = R < nr && k < DE, cv = Cc < nr && k < DE;
+        const int rb = (int)(R / dr), cb = (int)(Cc / dr);
+        const int qr = rv ? rb - rb0 : -1, qc = cv ? cb - rb0 : -1;
+        const int aoff = rv ? ((k + DE * (int)(R - (int64_t)rb * dr)) + qr * cst) * (int)sizeof(TC) : 0;
+        const int boff = cv ? ((k + DE * (int)(Cc - (int64_t)cb * dr)) + qc * cst) * (int)sizeof(TC) : 0;
+        const bool diag = tj == ti;
+        const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(C + up->cbase * cst), (short)0, spanc * cst * (int)sizeof(TC), 0x00020000);
+        auto ldc = [&](int off) -> double {
+            if constexpr (sizeof(TC) == 8) {
+                auto v = __builtin_amdgcn_raw_buffer_load_b64(crs, off, 0, 0);
+                return __hiloint2double((int)v[1], (int)v[0]);
+            } else {
+                return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(crs, off, 0, 0));
+            }
+        };
+        d4 D = {0.0, 0.0, 0.0, 0.0};
+        double gacc = 0.0;
+        if (lo < hi) {
+            const int ngroups = (hi - lo + U - 1) / U;
+            const int ez_slot = lane < U * DE * DE ? lane / (DE * DE) : (lane - U * DE * DE) / DE;
+            const int ez_q = lane < U * DE * DE ? lane % (DE * DE) : (lane - U * DE * DE) % DE;
+            const bool ez_e = lane < U * DE * DE, ez_on = lane < U * EZN;
+            double *buf0 = ezs[w], *buf1 = ezs[w] + U * EZN;
+            auto load_ev = [&](int gi, TileGroup<U> &G) {
+                const int t = lo + gi * U + lane < hi ? lo + gi * U + lane : hi - 1;
+                G.ev = *reinterpret_cast<const int4 *>(ep + t);
+            };
+            // the group's E_inv / z (one coalesced load) and, per landmark, the row- and column-side values
+            auto issue = [&](int gi, TileGroup<U> &G) {
+                const int e_l = __shfl(G.ev.x, ez_on ? ez_slot : 0, 64);
+                const TEI *pe = Einv + (int64_t)e_l * (DE * DE) + ez_q;
+                const double *pz = z + (int64_t)e_l * DE + ez_q;
+                G.ez = ez_on ? (ez_e ? (double)*pe : *pz) : 0.0;
+                const int nvalid = hi - (lo + gi * U);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int cw = __builtin_amdgcn_readlane(G.ev.y, u);
+                    const int qq = __builtin_amdgcn_readlane(G.ev.z, u);
+                    const int qf = qq & 0xffff;
+                    const unsigned wl = (unsigned)((qq >> 16) - qf);   // window: blocks qf .. qf + wl
+                    const int base = (cw - qf) * (cst * (int)sizeof(TC));
+                    const bool pr = (unsigned)(qr - qf) <= wl && u < nvalid;
+                    G.x[u] = ldc(pr ? base + aoff : (int)0x80000000);
+                    if (!diag) {
+                        const bool pc = (unsigned)(qc - qf) <= wl && u < nvalid;
+                        G.b[u] = ldc(pc ? base + boff : (int)0x80000000);
+                    } else {
+                        G.b[u] = 0.0;
+                    }
+                }
+            };
+            auto compute = [&](const TileGroup<U> &G, double *buf) {
+                if (lane < U * EZN) buf[lane] = G.ez;
+                __builtin_amdgcn_wave_barrier();
+                const int kk = k < DE ? k : 0;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    double xr[4];
+                    lane_rows(G.x[u], xr);
+                    // A[i][k] = W1[k][i] = sum_n E_inv[k][n] x_n (E_inv symmetric: row k is column k)
+                    double a = 0.0;
+#pragma unroll
+                    for (int n = 0; n < DE; n++) a += buf[u * DE * DE + DE * kk + n] * xr[n];
+                    a = k < DE ? a : 0.0;
+                    if (has_g) {
+#pragma unroll
+                        for (int n = 0; n < DE; n++) gacc += xr[n] * buf[U * DE * DE + u * DE + n];
+                    }
+                    D = __builtin_amdgcn_mfma_f64_16x16x4f64(a, diag ? G.x[u] : G.b[u], D, 0, 0, 0);
+                }
+                __builtin_amdgcn_wave_barrier();
+            };
+            TileGroup<U> GA, GB;
+            load_ev(0, GA);
+            if (ngroups > 1) load_ev(1, GB);
+            issue(0, GA);
+            for (int gi = 0; gi < ngroups; gi += 2) {
+                if (gi + 1 < ngroups) issue(gi + 1, GB);
+                if (gi + 2 < ngroups) load_ev(gi + 2, GA);
+                compute(GA, buf0);
+                if (gi + 1 >= ngroups) break;
+                if (gi + 2 < ngroups) issue(gi + 2, GA);
+                if (gi + 3 < ngroups) load_ev(gi + 3, GB);
+                compute(GB, buf1);
+            }
+        }
+        // the waves' partial tiles added in wave order; wave 0 writes S (upper part) and g
+        if (w > 0) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[w - 1][lane * 4 + v] = D[v];
+            if (k == 0) gred[w - 1][i] = gacc;
+        }
+        __syncthreads();
+        if (w == 0) {
+            if (has_g) {
+#pragma unroll
+                for (int q = 0; q < TW - 1; q++) gacc += gred[q][i];
+                if (k == 0 && R < nr) g[R] = b_r[R] * (s_r ? s_r[R] : 1.0) - gacc;
+            }
+            if (Cc < nr && cnt > 0) {
+                const int cl = (int)(Cc - (int64_t)cb * dr);
+                const int cbl = (int)((16ll * tj) / dr);
+                const int64_t colbase = S_colptr[Cc] - 1;
+                const int32_t *tab = tabs + (int64_t)blockIdx.x * 16;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double dv = D[v];
+#pragma unroll
+                    for (int q = 0; q < TW - 1; q++) dv += red[q][lane * 4 + v];
+                    // D layout of v_mfma_f64_16x16x4f64: lane l holds rows l / 16 + 4 v of column l % 16
+                    const int64_t Rr = 16ll * ti + k + 4 * v;
+                    if (Rr >= nr) continue;
+                    const int rbb = (int)(Rr / dr);
+                    const int rll = (int)(Rr - (int64_t)rbb * dr);
+                    if (rbb > cb || (rbb == cb && cl < rll)) continue;   // below the diagonal block
+                    const int off = tab[(rbb - rb0) * 4 + (cb - cbl)];
+                    if (off < 0) continue;
+                    TS *d = S + colbase + off + rll;
+                    *d = (TS)((double)*d - dv);
+                }
+            }
+        }
+    }
+    // the reduction's verdict from the last workgroup to finish: the lowest failing block, the flag re-armed
+    if (fin_counter) {
+        __syncthreads();
+        if (threadIdx.x == 0 && atomicAdd(fin_counter, 1u) == gridDim.x - 1) {
+            const unsigned long long f = *fin_flag;
+            *fin_out = f == (unsigned long long)NO_FAILURE ? 0 : (int64_t)f;
+            *fin_flag = (unsigned long long)NO_FAILURE;
+            *fin_counter = 0u;
+        }
+    }
+}
+
+template <int DE, class TC, class TEI, class TS>
+void launch_de(const TilePlan &P, const void *E_inv, const double *z, const void *C, const double *b_r,
+               const double *s_r, void *S, double *g, const int64_t *S_colptr, int64_t nr,
+               const unsigned long long *flag, unsigned *fin_counter, unsigned long long *fin_flag, int64_t *fin_out,
+               hipStream_t s) {
+    k_tiles<DE, TC, TEI, TS><<<(unsigned)P.nunits, 64 * TW, 0, s>>>(
+        P.units.ptr, P.entries.ptr, P.tabs.ptr, (const TEI *)E_inv, z, (const TC *)C, b_r, s_r, (TS *)S, g, S_colptr,
+        nr, P.dr, flag, fin_counter, fin_flag, fin_out);
+}
+
+template <class TC, class TEI, class TS>
+void launch_variant(const TilePlan &P, const void *E_inv, const double *z, const void *C, const double *b_r,
+                    const double *s_r, void *S, double *g, const int64_t *S_colptr, int64_t nr,
+                    const unsigned long long *flag, unsigned *fin_counter, unsigned long long *fin_flag,
+                    int64_t *fin_out, hipStream_t s) {
+    switch (P.de) {
+    case 1: launch_de<1, TC, TEI, TS>(P, E_inv, z, C, b_r, s_r, S, g, S_colptr, nr, flag, fin_counter, fin_flag, fin_out, s); break;
+    case 2: launch_de<2, TC, TEI, TS>(P, E_inv, z, C, b_r, s_r, S, g, S_colptr, nr, flag, fin_counter, fin_flag, fin_out, s); break;
+    case 3: launch_de<3, TC, TEI, TS>(P, E_inv, z, C, b_r, s_r, S, g, S_colptr, nr, flag, fin_counter, fin_flag, fin_out, s); break;
+    default: launch_de<4, TC, TEI, TS>(P, E_inv, z, C, b_r, s_r, S, g, S_colptr, nr, flag, fin_counter, fin_flag, fin_out, s); break;
+    }
+}
+
+}  // namespace
+
+int tiles_launch(const TilePlan &P, int variant, const void *E_inv, const double *z, const void *C, const double *b_r,
+                 const double *s_r, void *S, double *g, const int64_t *S_colptr, int64_t nr,
+                 const unsigned long long *flag, unsigned *fin_counter, unsigned long long *fin_flag, int64_t *fin_out,
+                 hipStream_t s) {
+    if (P.nunits <= 0) return fail(BLS_ERR_STATE, "no tile plan");
+    switch (variant) {
+    case 0: launch_variant<double, double, double>(P, E_inv, z, C, b_r, s_r, S, g, S_colptr, nr, flag, fin_counter, fin_flag, fin_out, s); break;
+    case 1: launch_variant<float, float, float>(P, E_inv, z, C, b_r, s_r, S, g, S_colptr, nr, flag, fin_counter, fin_flag, fin_out, s); break;
+    default: launch_variant<float, double, double>(P, E_inv, z, C, b_r, s_r, S, g, S_colptr, nr, flag, fin_counter, fin_flag, fin_out, s); break;
+    }
+    BLS_HIP(hipGetLastError());
+    return BLS_OK;
+}
+
+// The tile plan of a sparse layout. Leaves P.nunits == 0 (and says why) when the structure does not fit: blocks not
+// uniform in size (d_e 1..4, d_r 6..16), storage offsets that are not the uniform ones, a landmark whose couplings
+// are not a contiguous window of blocks, or a tile row whose couplings span more than the 32-bit buffer offsets allow.
+int tiles_build(TilePlan &P, const TileStructure &T, std::string *why) {
+    P.units.reset();
+    P.entries.reset();
+    P.tabs.reset();
+    P.nunits = P.nentries = P.nmfma = 0;
+    auto no = [&](const std::string &m) {
+        if (why) *why = m;
+        return BLS_OK;
+    };
+    const int64_t nr = T.nr, nrb = T.nrb, neb = T.neb;
+    if (neb == 0 || nrb == 0) return no("no landmarks");
+    const int de = (int)T.e_dims[0], dr = (int)T.r_dims[0];
+    if (de < 1 || de > 4) return no("landmark blocks above 4");
+    if (dr < 6 || dr > 16) return no("retained blocks outside 6..16");
+    for (int64_t e = 0; e < neb; e++)
+        if (T.e_dims[e] != de || T.eoffsets[e] != e * de * de || T.boffsets_e[e] != e * de)
+            return no("landmark blocks not uniform");
+    for (int64_t r = 0; r < nrb; r++)
+        if (T.r_dims[r] != dr || T.boffsets_r[r] != r * dr) return no("retained blocks not uniform");
+    const int64_t cst = (int64_t)de * dr;
+    for (int64_t c = 0; c < (int64_t)T.coffsets.size(); c++)
+        if (T.coffsets[c] != c * cst) return no("coupling offsets not uniform");
+    struct Raw {
+        int64_t ti;
+        int32_t e, c0, clast, qf, ql, last;   // last: last tile column reached, relative to ti
+    };
+    std::vector<Raw> raw;
+    raw.reserve((size_t)(T.eptr[neb] * 2));
+    for (int64_t e = 0; e < neb; e++) {
+        const int64_t p0 = T.eptr[e], p1 = T.eptr[e + 1];
+        if (p1 == p0) continue;
+        for (int64_t c = p0 + 1; c < p1; c++)
+            if (T.eret[c] != T.eret[c - 1] + 1) return no("a landmark's couplings are not a contiguous window");
+        const int64_t first_row = T.eret[p0] * dr, last_col = T.eret[p1 - 1] * dr + dr - 1;
+        for (int64_t ti = first_row / 16; ti <= last_col / 16; ti++) {
+            const int64_t rb0 = (16 * ti) / dr;
+            const int64_t rbh = std::min(16 * ti + 15, nr - 1) / dr;
+            if (T.eret[p1 - 1] < rb0 || T.eret[p0] > rbh) continue;   // no block in the tile row's rows
+            int64_t c0 = p0;
+            while (T.eret[c0] < rb0) c0++;
+            const int64_t qf = T.eret[c0] - rb0, ql = T.eret[p1 - 1] - rb0;
+            if (ql >= 0x7fff) return no("window too long");
+            raw.push_back(Raw{ti, (int32_t)e, (int32_t)c0, (int32_t)(p1 - 1), (int32_t)qf, (int32_t)ql,
+                              (int32_t)(last_col / 16 - ti)});
+        }
+    }
+    std::stable_sort(raw.begin(), raw.end(), [](const Raw &a, const Raw &b) {
+        return a.ti != b.ti ? a.ti < b.ti : a.last > b.last;
+    });
+    struct Pending {
+        int64_t weight, tab;
+        TileUnit u;
+    };
+    std::vector<Pending> us;
+    std::vector<TileEntry> ents;
+    std::vector<int32_t> tabs;
+    ents.reserve(raw.size());
+    const int64_t max_bytes = 0x7fffff00ll;
+    const int64_t ntr = (nr + 15) / 16;
+    size_t r0 = 0;
+    for (int64_t ti = 0; ti < ntr; ti++) {
+        const int64_t rb0 = (16 * ti) / dr;
+        if (r0 == raw.size() || raw[r0].ti != ti) {
+            // no landmark couples the tile row's blocks: one unit that only writes g = b_r s_r
+            TileUnit u{};
+            u.ti = u.tj = (int32_t)ti;
+            u.has_g = 1;
+            u.rb0 = (int32_t)rb0;
+            u.elo = (int64_t)ents.size();
+            tabs.resize(tabs.size() + 16, -1);
+            us.push_back(Pending{0, (int64_t)tabs.size() / 16 - 1, u});
+            continue;
+        }
+        size_t r1 = r0;
+        int64_t cb = INT64_MAX, cm = -1;
+        while (r1 < raw.size() && raw[r1].ti == ti) {
+            cb = std::min<int64_t>(cb, raw[r1].c0);
+            cm = std::max<int64_t>(cm, raw[r1].clast);
+            r1++;
+        }
+        const int64_t spanc = cm + 1 - cb;
+        if (spanc * cst * 8 >= max_bytes) return no("a tile row's couplings span more than 2 GB");
+        const int64_t elo = (int64_t)ents.size();
+        for (size_t q = r0; q < r1; q++)
+            ents.push_back(TileEntry{raw[q].e, (int32_t)(raw[q].c0 - cb), raw[q].qf | (raw[q].ql << 16), 0});
+        const int maxlast = raw[r0].last;
+        int64_t cnt = (int64_t)(r1 - r0);
+        for (int s = 0; s <= maxlast; s++) {
+            while (cnt > 0 && raw[r0 + cnt - 1].last < s) cnt--;   // the prefix reaching tile column ti + s
+            const int64_t tj = ti + s;
+            if (cnt == 0 || 16 * tj >= nr) break;
+            const int64_t cbl = (16 * tj) / dr, cbh = std::min(16 * tj + 15, nr - 1) / dr;
+            if (cbh - rb0 >= 64) return no("a tile column more than 64 blocks from its row");
+            if (cnt > INT32_MAX) return no("too many landmarks in a tile row");
+            TileUnit u{};
+            u.ti = (int32_t)ti;
+            u.tj = (int32_t)tj;
+            u.cnt = (int32_t)cnt;
+            u.has_g = s == 0;
+            u.rb0 = (int32_t)rb0;
+            u.spanc = (int32_t)spanc;
+            u.elo = elo;
+            u.cbase = cb;
+            // offsets in S's scalar column of each (row block, column block) the tile covers; -1: not stored
+            const int64_t t0 = (int64_t)tabs.size();
+            tabs.resize(tabs.size() + 16, -1);
+            for (int q = 0; q < 4; q++)
+                for (int p = 0; p < 4; p++) {
+                    const int64_t rb = rb0 + q, c = cbl + p;
+                    if (rb >= nrb || c >= nrb || rb > c) continue;
+                    for (int64_t kk = (*T.bcp)[c]; kk < (*T.bcp)[c + 1]; kk++)
+                        if ((*T.brv)[kk] == rb) tabs[t0 + q * 4 + p] = (int32_t)(*T.bro)[kk];
+                }
+            us.push_back(Pending{cnt, t0 / 16, u});
+            P.nmfma += cnt;
+        }
+        r0 = r1;
+    }
+    // the heaviest tiles first: the dispatcher deals workgroups in order
+    std::stable_sort(us.begin(), us.end(), [](const Pending &a, const Pending &b) { return a.weight > b.weight; });
+    std::vector<TileUnit> units;
+    std::vector<int32_t> otabs;
+    units.reserve(us.size());
+    otabs.reserve(tabs.size());
+    for (const Pending &x : us) {
+        units.push_back(x.u);
+        otabs.insert(otabs.end(), tabs.begin() + x.tab * 16, tabs.begin() + x.tab * 16 + 16);
+    }
+    if (units.size() > (size_t)INT32_MAX) return no("too many tiles");
+    int rc;
+    if ((rc = P.units.upload(units)) || (rc = P.entries.upload(ents)) || (rc = P.tabs.upload(otabs))) return rc;
+    P.nunits = (int64_t)units.size();
+    P.nentries = (int64_t)ents.size();
+    P.de = de;
+    P.dr = dr;
+    if (why) why->clear();
+    return BLS_OK;
+}
+
+}  // namespace bls

@@ -52,6 +52,8 @@ SIGNATURES = {
     "bls_schur_set_timing": (ct.c_int, [VP, ct.c_int]),
     "bls_schur_set_tuning": (ct.c_int, [VP, ct.c_int64, ct.c_int64]),
     "bls_schur_get_tuning": (ct.c_int, [VP, P64, P64]),
+    "bls_schur_set_phase2_mode": (ct.c_int, [VP, ct.c_int]),
+    "bls_schur_phase2_info": (ct.c_int, [VP, P64]),
     "bls_schur_autotune_f64": (ct.c_int, [VP, ct.c_int64, P64, P64] + [VP] * 5 + [ct.c_double] + [VP] * 6
                                + [ct.POINTER(ct.c_float), VP]),
     "bls_pcg_create": (ct.c_int, [ct.c_int64, ct.c_int64, P64, P64, ct.c_int64, P64, ct.c_int64, P64, P64, P64, P64,
@@ -266,6 +268,17 @@ class SchurHandle:
         a, b = ct.c_int64(0), ct.c_int64(0)
         _check(lib().bls_schur_get_tuning(self.h, ct.byref(a), ct.byref(b)))
         return int(a.value), int(b.value)
+
+    def set_phase2_mode(self, mode):
+        """0: the tiles of S on the matrix cores when the layout fits them, else the work list; 1: the work list;
+        2: the tiles (an error when the layout does not fit)."""
+        _check(lib().bls_schur_set_phase2_mode(self.h, int(mode)))
+
+    def phase2_info(self):
+        """(kernel in use: 1 work list / 2 tiles, tiles, landmark entries, matrix-core products)."""
+        arr = (ct.c_int64 * 4)()
+        _check(lib().bls_schur_phase2_info(self.h, arr))
+        return tuple(int(v) for v in arr)
 
     def autotune(self, candidates, S, g_S, b_r, b_e, scaling_e, lam, E_data, C_data, ws, s_r=None, s_e=None,
                  stream=None):

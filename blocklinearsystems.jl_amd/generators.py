@@ -158,6 +158,29 @@ def config2(seed=2, scale=1.0, nchunks=1, ncam=None, nland=None, dense=False):
     return _hessian_from_tracks(rng, ncam, np.full(ncam, 6, dtype=I64), nland, k, ptr, cams, 1e-4, dense, nchunks=nchunks)
 
 
+def window_problem(seed=7, ncam=60, nland=1500, de=3, dr=6, kmin=2, kmax=40, unobserved=(), lam=1e-4, dense=False):
+    """Sequential-SLAM tracks (contiguous camera windows, k = clip(Geometric(0.2), kmin, kmax)) with landmark blocks
+    of size `de` and camera blocks of size `dr`. Cameras listed in `unobserved` (0-based) are seen by no landmark:
+    windows that would cover one are moved to the longest free run of cameras."""
+    rng = np.random.default_rng(seed)
+    bad = np.zeros(ncam, bool)
+    bad[list(unobserved)] = True
+    free = np.nonzero(~bad)[0]
+    # maximal runs of observed cameras
+    runs = np.split(free, np.nonzero(np.diff(free) != 1)[0] + 1) if len(free) else []
+    longest = max(len(r) for r in runs)
+    k = np.clip(rng.geometric(0.2, size=nland), kmin, min(kmax, longest)).astype(I64)
+    ptr = np.zeros(nland + 1, dtype=I64)
+    np.cumsum(k, out=ptr[1:])
+    cams = np.empty(int(ptr[-1]), dtype=I64)
+    for l in range(nland):
+        fits = [r for r in runs if len(r) >= k[l]]
+        r = fits[int(rng.integers(len(fits)))]
+        st = r[0] + int(rng.integers(len(r) - k[l] + 1))
+        cams[ptr[l]:ptr[l + 1]] = np.arange(st, st + k[l])
+    return _hessian_from_tracks(rng, ncam, np.full(ncam, dr, dtype=I64), nland, k, ptr, cams, lam, dense, de=de)
+
+
 def config3(seed=3, scale=1.0, nchunks=1, ncam=256, nland=None, kmax=64):
     rng = np.random.default_rng(seed)
     nland = nland or max(4, int(round(1000000 * scale)))

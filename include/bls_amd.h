@@ -219,9 +219,20 @@ int bls_scatter_map_create(int64_t n, const int64_t *scatter_map, int64_t **d_ma
  * shapes differ from each other by rounding only. */
 int bls_schur_set_tuning(bls_schur *h, int64_t part, int64_t lanes);
 int bls_schur_get_tuning(bls_schur *h, int64_t *part, int64_t *lanes);
+/* Which phase-2 kernel a sparse handle runs (no reference counterpart; the reference has one loop nest,
+ * src/schur_reduction.jl:315-380). mode 0 (default): the 16 x 16 tiles of S on the matrix cores when the layout fits
+ * them (every landmark block of one size d_e <= 4, every retained block of one size 6..16, the default storage
+ * offsets, each landmark coupling a contiguous run of retained blocks), else the work list; 1: always the work list;
+ * 2: always the tiles (BLS_ERR_UNSUPPORTED when the layout does not fit). The kernels sum in different fixed orders:
+ * each is deterministic, the two differ by rounding only. */
+int bls_schur_set_phase2_mode(bls_schur *h, int mode);
+/* info[0]: the phase-2 kernel in use (1 work list, 2 tiles); info[1..3]: the tile plan's tiles, landmark entries and
+ * matrix-core products (0 when the layout does not fit). */
+int bls_schur_phase2_info(bls_schur *h, int64_t *info);
 /* The autotuner: times three Float64 reductions (the arguments of bls_reduce_schur_{sparse,dense}_f64, on the
  * handle's layout) for each of the ncand (parts[i], lanes[i]) shapes, after one untimed run each, and keeps the shape
- * with the fastest median (*best_ms: that median). S is reduced repeatedly without a reset, so S, g_S, E_inv and z are scratch for the timing:
+ * with the fastest median (*best_ms: that median). In phase-2 mode 0 it also times the tiles when the layout fits
+ * them and sets mode 1 when a work-list shape was faster. S is reduced repeatedly without a reset, so S, g_S, E_inv and z are scratch for the timing:
  * reset and reduce again after tuning. Synchronizes `stream`. */
 int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t *parts, const int64_t *lanes, double *S,
                            double *g_S, const double *b_r, const double *b_e, const double *scaling_e, double lambda,
@@ -229,7 +240,7 @@ int bls_schur_autotune_f64(bls_schur *h, int64_t ncand, const int64_t *parts, co
                            const double *s_e, float *best_ms, void *stream);
 
 /* Optional per-kernel timing of the last call on a handle (HIP events on the launch stream), in ms:
- * times[0] reset, [1] phase 1, [2] phase 2 (k_phase2), [3] the merge of split block columns. Enabled by
+ * times[0] reset, [1] phase 1, [2] phase 2 (k_phase2, or k_tiles), [3] the merge of split block columns. Enabled by
  * bls_schur_set_timing(h, 1). */
 int bls_schur_set_timing(bls_schur *h, int enable);
 int bls_schur_kernel_times(bls_schur *h, float *times, int n);

@@ -44,13 +44,15 @@ def _rounded(p):
     return q
 
 
-def run_variant(blsamd, p, variant, lam=None, s_r=None, s_e=None):
-    """reset + reduce with the f32 or mixed entry points; returns host copies."""
+def run_variant(blsamd, p, variant, lam=None, s_r=None, s_e=None, mode=None):
+    """reset + reduce with the f32 or mixed entry points; returns host copies. `mode`: the phase-2 kernel."""
     torch = _torch()
     dev = blsamd.device
     st = p.structure
     lam = p.lam if lam is None else lam
     h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=getattr(p, "r_to_s", None), dense=p.dense)
+    if mode is not None:
+        h.set_phase2_mode(mode)
     ws = dev.SchurReductionWorkspace(h, variant)
     sdt = torch.float32 if variant == "f32" else torch.float64
     T64 = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()

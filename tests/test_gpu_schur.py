@@ -20,13 +20,16 @@ def _torch():
     return torch
 
 
-def run_gpu(blsamd, p, lam=None, repeat=1):
-    """reset + reduce on the device; returns host copies of S, g, E_inv, z, status, failed_block."""
+def run_gpu(blsamd, p, lam=None, repeat=1, mode=None):
+    """reset + reduce on the device; returns host copies of S, g, E_inv, z, status, failed_block. `mode`: the
+    phase-2 kernel (bls_schur_set_phase2_mode), default automatic."""
     torch = _torch()
     dev = blsamd.device
     st = p.structure
     lam = p.lam if lam is None else lam
     h = dev.SchurHandle(st, p.ws_index, pattern=p.pattern, r_to_s=getattr(p, "r_to_s", None), dense=p.dense)
+    if mode is not None:
+        h.set_phase2_mode(mode)
     ws = dev.SchurReductionWorkspace(h)
     T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
     E, C, b_r, b_e = T(p.E_data), T(p.C_data), T(p.b_r), T(p.b_e)
@@ -386,6 +389,7 @@ def test_autotune_shapes_agree(blsamd):
     S = torch.empty(p.pattern.nnz, dtype=torch.float64, device="cuda")
     g = torch.empty(st.nr, dtype=torch.float64, device="cuda")
     cands = [(384, 96), (128, 48), (1024, 256), (64, 16)]
+    h.set_phase2_mode(1)   # the work list (the shapes are its shapes)
     for part, lanes in cands:
         h.set_tuning(part, lanes)
         outs = []
@@ -400,6 +404,17 @@ def test_autotune_shapes_agree(blsamd):
     dev.reset_schur_sparse_(S, Rd, h, sr, p.lam)
     part, lanes, ms = h.autotune(cands, S, g, b_r, b_e, se, p.lam, E, C, ws)
     assert (part, lanes) in cands and ms > 0
+    assert h.phase2_info()[0] == 1   # mode 1 set by the caller: the tiles are not timed
+    # in automatic mode the tiles are timed as well; the kernel kept reproduces the oracle
+    h.set_phase2_mode(0)
+    dev.reset_schur_sparse_(S, Rd, h, sr, p.lam)
+    part, lanes, ms = h.autotune(cands, S, g, b_r, b_e, se, p.lam, E, C, ws)
+    assert (part, lanes) in cands and ms > 0 and h.phase2_info()[0] in (1, 2)
+    dev.reset_schur_sparse_(S, Rd, h, sr, p.lam)
+    assert dev.reduce_schur_sparse_(S, g, b_r, b_e, se, p.lam, h, E, C, ws)
+    torch.cuda.synchronize()
+    compare(p, dict(S=S.cpu().numpy(), g=g.cpu().numpy(), E_inv=ws.E_inv_data.cpu().numpy(),
+                    z=ws.z_data.cpu().numpy(), ok=True), orc)
 
 
 @pytest.mark.parametrize("shape", [None, (1536, 96), (2304, 64)])
