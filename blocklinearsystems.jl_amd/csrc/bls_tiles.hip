@@ -5,7 +5,7 @@
 // landmark e that couples a block of tile row ti and a block of tile column tj,
 //     D[i][j] += sum_k W1[k][i] C2[k][j],   W1 = E_inv_e C_e(rows of the row block of i),  C2 = C_e(column j)
 // which is the reference's S[r1, r2] -= W_{c1}' C_{c2} (src/schur_reduction.jl:315-380) for the block pairs the tile
-// covers, as one v_mfma_f64_16x16x4f64 per landmark (K = d_e components, padded to 4). The tile's 16 rows and columns
+// covers, as a v_mfma_f64_16x16x4f64 per landmark (K = d_e components, padded to 4). The tile's 16 rows and columns
 // are scalar rows and columns, so a tile spans parts of up to four blocks on each side; a landmark adds zero where it
 // does not couple a block (buffer loads outside the window return 0).
 //
@@ -16,13 +16,15 @@
 // a fixed order (bitwise reproducible). g_S = b_r s_r - sum C' z is formed by the diagonal tile (tj == ti), whose
 // prefix is the whole list.
 //
-// Per landmark a lane loads one value of the row side (its column i of the row block's coupling, component k) and one
-// of the column side; three lane-row swaps (v_permlane16/32_swap) give every lane the d_e components of its column,
-// and E_inv (with z) comes from one coalesced load per group of landmarks through LDS.
+// Per landmark a lane loads one value of the row side (component k of its row i's coupling column) and one of the
+// column side; W1 = E_inv C1 is itself one 16 x 16 x 4 product (A' = E_inv in rows 0..d_e-1 and z in row 4, from
+// LDS), whose output lands in the layout of the tile product's A operand and carries the g_S part C1' z in row 4.
+// E_inv and z come from one coalesced load per group of landmarks through LDS.
 #include "bls_internal.h"
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 namespace bls {
 
@@ -31,26 +33,10 @@ namespace {
 constexpr int TW = 4;   // waves per workgroup
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// every lane gets x of lane rows 0..3 (same column) in fixed order: three permlane swaps per 32-bit half
-__device__ __forceinline__ void lane_rows(double x, double *xo) {
-    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
-    auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-    auto ah = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-    auto c = __builtin_amdgcn_permlane32_swap(a[0], a[0], false, false);
-    auto ch = __builtin_amdgcn_permlane32_swap(ah[0], ah[0], false, false);
-    auto e = __builtin_amdgcn_permlane32_swap(a[1], a[1], false, false);
-    auto eh = __builtin_amdgcn_permlane32_swap(ah[1], ah[1], false, false);
-    xo[0] = __hiloint2double((int)ch[0], (int)c[0]);
-    xo[1] = __hiloint2double((int)eh[0], (int)e[0]);
-    xo[2] = __hiloint2double((int)ch[1], (int)c[1]);
-    xo[3] = __hiloint2double((int)eh[1], (int)e[1]);
-}
-
 template <int DE> constexpr int tile_group() { return 64 / (DE * DE + DE) < 5 ? 64 / (DE * DE + DE) : 5; }
 
 template <int U> struct TileGroup {
-    int4 ev;     // lane u < U: record of slot u
-    double ez;   // lane l < U (DE^2 + DE): E_inv value l % DE^2 of slot l / DE^2, then the z values
+    double ez;   // lane l < U (DE^2 + DE): value l % (DE^2 + DE) of slot l / (DE^2 + DE): E_inv, then z
     double x[U], b[U];
 };
 
@@ -63,16 +49,16 @@ __global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ 
                                                   int dr, const unsigned long long *flag, unsigned *fin_counter,
                                                   unsigned long long *fin_flag, int64_t *fin_out) {
     constexpr int U = tile_group<DE>();
-    constexpr int EZN = DE * DE + DE;
+    constexpr int EZN = DE * DE + DE, EZS = EZN + 1;
     __shared__ double red[TW - 1][256];
     __shared__ double gred[TW - 1][16];
-    __shared__ double ezs[TW][2 * U * EZN];
+    __shared__ double ezs[TW][2 * U * EZS];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, k = lane >> 4, i = lane & 15;
     if (*flag == (unsigned long long)NO_FAILURE) {
         const TileUnit *up = units + blockIdx.x;
         const int ti = __builtin_amdgcn_readfirstlane(up->ti), tj = __builtin_amdgcn_readfirstlane(up->tj);
-        const int cnt = __builtin_amdgcn_readfirstlane(up->cnt), has_g = __builtin_amdgcn_readfirstlane(up->has_g);
+        const int cnt = __builtin_amdgcn_readfirstlane(up->cnt);
         const int rb0 = __builtin_amdgcn_readfirstlane(up->rb0), spanc = __builtin_amdgcn_readfirstlane(up->spanc);
         const int cst = DE * dr;
         const int lo = cnt * w / TW, hi = cnt * (w + 1) / TW;
@@ -98,75 +84,87 @@ __global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ 
         };
         d4 D = {0.0, 0.0, 0.0, 0.0};
         double gacc = 0.0;
-        if (lo < hi) {
+        // the group's E_inv and z in LDS, slot u at u * EZS: E_inv (column-major), z, then a zero
+        const int ez_slot = lane / EZN, ez_q = lane - ez_slot * EZN;
+        const bool ez_on = lane < U * EZN;
+        double *buf0 = ezs[w], *buf1 = ezs[w] + U * EZS;
+        if (lane < 2 * U) ezs[w][lane * EZS + EZN] = 0.0;
+        // the lane's value of the first product's A operand, A'[i][k] (lane 16 k + i): E_inv[i][k] in rows i < d_e,
+        // z[k] in row 4, else zero
+        const int aq = (i < DE && k < DE) ? i + DE * k : (i == 4 && k < DE) ? DE * DE + k : EZN;
+        auto run = [&](auto diag_c) {
+            constexpr bool DIAG = decltype(diag_c)::value;
+            if (lo >= hi) return;
             const int ngroups = (hi - lo + U - 1) / U;
-            const int ez_slot = lane < U * DE * DE ? lane / (DE * DE) : (lane - U * DE * DE) / DE;
-            const int ez_q = lane < U * DE * DE ? lane % (DE * DE) : (lane - U * DE * DE) % DE;
-            const bool ez_e = lane < U * DE * DE, ez_on = lane < U * EZN;
-            double *buf0 = ezs[w], *buf1 = ezs[w] + U * EZN;
-            auto load_ev = [&](int gi, TileGroup<U> &G) {
+            auto load_ev = [&](int gi) {
                 const int t = lo + gi * U + lane < hi ? lo + gi * U + lane : hi - 1;
-                G.ev = *reinterpret_cast<const int4 *>(ep + t);
+                return *reinterpret_cast<const int4 *>(ep + t);
             };
-            // the group's E_inv / z (one coalesced load) and, per landmark, the row- and column-side values
-            auto issue = [&](int gi, TileGroup<U> &G) {
-                const int e_l = __shfl(G.ev.x, ez_on ? ez_slot : 0, 64);
-                const TEI *pe = Einv + (int64_t)e_l * (DE * DE) + ez_q;
-                const double *pz = z + (int64_t)e_l * DE + ez_q;
-                G.ez = ez_on ? (ez_e ? (double)*pe : *pz) : 0.0;
+            // the group's E_inv / z (one coalesced load; every lane's address valid, only lanes < U (d_e^2 + d_e)
+            // keep theirs) and, per landmark, the row- and column-side values
+            auto issue = [&](int gi, TileGroup<U> &G, const int4 ev) {
+                const int e_l = __shfl(ev.x, ez_slot < U ? ez_slot : 0, 64);
+                const TEI *pe = Einv + (int64_t)e_l * (DE * DE) + (ez_q < DE * DE ? ez_q : 0);
+                const double *pz = z + (int64_t)e_l * DE + (ez_q < DE * DE ? 0 : ez_q - DE * DE);
+                if constexpr (sizeof(TEI) == sizeof(double)) {
+                    G.ez = *(ez_q < DE * DE ? reinterpret_cast<const double *>(pe) : pz);
+                } else {
+                    const double ve = (double)*pe, vz = *pz;
+                    G.ez = ez_q < DE * DE ? ve : vz;
+                }
                 const int nvalid = hi - (lo + gi * U);
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const int cw = __builtin_amdgcn_readlane(G.ev.y, u);
-                    const int qq = __builtin_amdgcn_readlane(G.ev.z, u);
+                    const int cw = __builtin_amdgcn_readlane(ev.y, u);
+                    const int qq = __builtin_amdgcn_readlane(ev.z, u);
                     const int qf = qq & 0xffff;
                     const unsigned wl = (unsigned)((qq >> 16) - qf);   // window: blocks qf .. qf + wl
                     const int base = (cw - qf) * (cst * (int)sizeof(TC));
                     const bool pr = (unsigned)(qr - qf) <= wl && u < nvalid;
                     G.x[u] = ldc(pr ? base + aoff : (int)0x80000000);
-                    if (!diag) {
+                    if constexpr (!DIAG) {
                         const bool pc = (unsigned)(qc - qf) <= wl && u < nvalid;
                         G.b[u] = ldc(pc ? base + boff : (int)0x80000000);
-                    } else {
-                        G.b[u] = 0.0;
                     }
                 }
             };
+            // per landmark two products: W = A' x (rows k < d_e: W1[k][i] = sum_n E_inv[k][n] C1[n][i], the A operand
+            // of the second in its layout; row 4: the g_S part sum_n z[n] C1[n][i]), then D += W1' C2
             auto compute = [&](const TileGroup<U> &G, double *buf) {
-                if (lane < U * EZN) buf[lane] = G.ez;
+                if (ez_on) buf[ez_slot * EZS + ez_q] = G.ez;
                 __builtin_amdgcn_wave_barrier();
-                const int kk = k < DE ? k : 0;
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    double xr[4];
-                    lane_rows(G.x[u], xr);
-                    // A[i][k] = W1[k][i] = sum_n E_inv[k][n] x_n (E_inv symmetric: row k is column k)
-                    double a = 0.0;
-#pragma unroll
-                    for (int n = 0; n < DE; n++) a += buf[u * DE * DE + DE * kk + n] * xr[n];
-                    a = k < DE ? a : 0.0;
-                    if (has_g) {
-#pragma unroll
-                        for (int n = 0; n < DE; n++) gacc += xr[n] * buf[U * DE * DE + u * DE + n];
+                    const d4 W = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[u * EZS + aq], G.x[u],
+                                                                      d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+                    if constexpr (DIAG) {
+                        gacc += W[1];
+                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(W[0], G.x[u], D, 0, 0, 0);
+                    } else {
+                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(W[0], G.b[u], D, 0, 0, 0);
                     }
-                    D = __builtin_amdgcn_mfma_f64_16x16x4f64(a, diag ? G.x[u] : G.b[u], D, 0, 0, 0);
                 }
                 __builtin_amdgcn_wave_barrier();
             };
+            // two groups in flight, their records loaded two groups ahead; the loads of a group past the last are
+            // issued all the same (records clamped to the last one, every C load outside the buffer: zeros), so
+            // that the loop body has no branch
             TileGroup<U> GA, GB;
-            load_ev(0, GA);
-            if (ngroups > 1) load_ev(1, GB);
-            issue(0, GA);
+            int4 evA = load_ev(0), evB = load_ev(1);
+            issue(0, GA, evA);
+            evA = load_ev(2);
             for (int gi = 0; gi < ngroups; gi += 2) {
-                if (gi + 1 < ngroups) issue(gi + 1, GB);
-                if (gi + 2 < ngroups) load_ev(gi + 2, GA);
+                issue(gi + 1, GB, evB);
+                evB = load_ev(gi + 3);
                 compute(GA, buf0);
                 if (gi + 1 >= ngroups) break;
-                if (gi + 2 < ngroups) issue(gi + 2, GA);
-                if (gi + 3 < ngroups) load_ev(gi + 3, GB);
+                issue(gi + 2, GA, evA);
+                evA = load_ev(gi + 4);
                 compute(GB, buf1);
             }
-        }
+        };
+        if (diag) run(std::true_type{});
+        else run(std::false_type{});
         // the waves' partial tiles added in wave order; wave 0 writes S (upper part) and g
         if (w > 0) {
 #pragma unroll
@@ -175,7 +173,7 @@ __global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ 
         }
         __syncthreads();
         if (w == 0) {
-            if (has_g) {
+            if (diag) {
 #pragma unroll
                 for (int q = 0; q < TW - 1; q++) gacc += gred[q][i];
                 if (k == 0 && R < nr) g[R] = b_r[R] * (s_r ? s_r[R] : 1.0) - gacc;
@@ -376,16 +374,46 @@ int tiles_build(TilePlan &P, const TileStructure &T, std::string *why) {
         }
         r0 = r1;
     }
-    // the heaviest tiles first: the dispatcher deals workgroups in order
-    std::stable_sort(us.begin(), us.end(), [](const Pending &a, const Pending &b) { return a.weight > b.weight; });
+    // Dispatch order. The dispatcher places workgroup p on XCD p % 8, and the tiles of one tile row (and of
+    // neighbouring rows) read the same coupling values: each XCD gets a contiguous range of tile rows of about equal
+    // weight, in its own L2, and runs its tiles heaviest first. Positions of an XCD past its last tile hold empty
+    // units (no landmarks, not diagonal: they write nothing).
+    std::vector<std::vector<Pending>> xcd(8);
+    {
+        int64_t wsum = 0;
+        for (const Pending &x : us) wsum += x.weight + 64;   // 64: a tile's fixed cost in landmark units
+        int64_t acc = 0;
+        for (size_t q = 0; q < us.size();) {   // us is in tile-row order
+            size_t q1 = q;
+            int64_t rw = 0;
+            while (q1 < us.size() && us[q1].u.ti == us[q].u.ti) rw += us[q1++].weight + 64;
+            const int x = (int)std::min<int64_t>(7, (acc + rw / 2) * 8 / std::max<int64_t>(wsum, 1));
+            xcd[x].insert(xcd[x].end(), us.begin() + q, us.begin() + q1);
+            acc += rw;
+            q = q1;
+        }
+    }
+    size_t per = 0;
+    for (auto &v : xcd) {
+        std::stable_sort(v.begin(), v.end(), [](const Pending &a, const Pending &b) { return a.weight > b.weight; });
+        per = std::max(per, v.size());
+    }
     std::vector<TileUnit> units;
     std::vector<int32_t> otabs;
-    units.reserve(us.size());
-    otabs.reserve(tabs.size());
-    for (const Pending &x : us) {
-        units.push_back(x.u);
-        otabs.insert(otabs.end(), tabs.begin() + x.tab * 16, tabs.begin() + x.tab * 16 + 16);
-    }
+    units.reserve(per * 8);
+    otabs.reserve(per * 8 * 16);
+    for (size_t j = 0; j < per; j++)
+        for (int x = 0; x < 8; x++) {
+            if (j < xcd[x].size()) {
+                units.push_back(xcd[x][j].u);
+                otabs.insert(otabs.end(), tabs.begin() + xcd[x][j].tab * 16, tabs.begin() + xcd[x][j].tab * 16 + 16);
+            } else {
+                TileUnit e{};
+                e.tj = 1;   // ti 0, tj 1: not diagonal; cnt 0: no landmarks
+                units.push_back(e);
+                otabs.insert(otabs.end(), 16, -1);
+            }
+        }
     if (units.size() > (size_t)INT32_MAX) return no("too many tiles");
     int rc;
     if ((rc = P.units.upload(units)) || (rc = P.entries.upload(ents)) || (rc = P.tabs.upload(otabs))) return rc;
