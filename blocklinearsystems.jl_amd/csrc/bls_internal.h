@@ -54,23 +54,25 @@ template <typename T> struct DevArray {
 
 constexpr int64_t NO_FAILURE = INT64_MAX;
 
-// Phase 2 on 16 x 16 tiles of S (bls_tiles.hip). A unit is one tile (ti, tj) of S; its landmarks are the first `cnt`
-// entries of tile row ti's list (elo), whose couplings lie in [cbase, cbase + spanc).
+// Phase 2 on 16 x 16 tiles of S (bls_tiles.hip). A unit is a run of nt <= 4 tiles of tile row ti, tile columns
+// tj .. tj + nt - 1; its landmarks are the first cnt[0] entries of the row's list (elo), cnt[t] of them reaching tile
+// t, whose couplings lie in [cbase, cbase + spanc).
 struct TileUnit {
-    int32_t ti, tj, cnt, has_g;
-    int32_t rb0, spanc;   // first retained block of the tile row's rows; couplings the row's landmarks reach
+    int32_t ti, tj, nt, rb0;   // rb0: first retained block of the tile row's rows
+    int32_t cnt[4];
+    int32_t spanc, pad;        // spanc: couplings the row's landmarks reach
     int64_t elo, cbase;
 };
 struct TileEntry {
     int32_t e;      // landmark (0-based)
     int32_t cw;     // its first coupling at or after block rb0, relative to cbase
     int32_t qfql;   // that coupling's block - rb0 (low 16 bits), its last coupling's block - rb0 (high 16 bits)
-    int32_t pad;
+    int32_t last;   // the last tile column it reaches, relative to ti
 };
 struct TilePlan {
     DevArray<TileUnit> units;
     DevArray<TileEntry> entries;
-    DevArray<int32_t> tabs;   // per unit: offsets in S's columns of (row block q, column block p), 4 x 4, -1 none
+    DevArray<int32_t> tabs;   // per unit and tile: offsets in S's columns of (row block q, column block p), 4 x 4, -1 none
     int64_t nunits = 0, nentries = 0, nmfma = 0;
     int de = 0, dr = 0;
 };

@@ -1,25 +1,27 @@
 // Phase 2 of the Schur reduction on the FP64 matrix cores, for structures whose landmarks couple contiguous windows
 // of retained blocks (every block of uniform size d_r >= 6, every landmark block of uniform size d_e <= 4).
 //
-// S is cut into 16 x 16 scalar tiles. A workgroup owns one tile (ti, tj), tj >= ti, and sums into it, for every
-// landmark e that couples a block of tile row ti and a block of tile column tj,
-//     D[i][j] += sum_k W1[k][i] C2[k][j],   W1 = E_inv_e C_e(rows of the row block of i),  C2 = C_e(column j)
+// S is cut into 16 x 16 scalar tiles. A workgroup owns a run of up to four tiles of one tile row, (ti, tj .. tj+3),
+// tj >= ti, and sums into each tile, for every landmark e that couples a block of tile row ti and a block of that
+// tile's column,
+//     D[i][j] += sum_k W1[k][i] C2[k][j],   W1 = E_inv_e C_e(row i),  C2 = C_e(column j)
 // which is the reference's S[r1, r2] -= W_{c1}' C_{c2} (src/schur_reduction.jl:315-380) for the block pairs the tile
-// covers, as a v_mfma_f64_16x16x4f64 per landmark (K = d_e components, padded to 4). The tile's 16 rows and columns
-// are scalar rows and columns, so a tile spans parts of up to four blocks on each side; a landmark adds zero where it
-// does not couple a block (buffer loads outside the window return 0).
+// covers, as a v_mfma_f64_16x16x4f64 per landmark and tile (K = d_e components, padded to 4). The tile's 16 rows and
+// columns are scalar rows and columns, so a tile spans parts of up to four blocks on each side; a landmark adds zero
+// where it does not couple a block (buffer loads outside its window return 0).
 //
 // The landmarks of tile row ti are one list, sorted by the last tile column they reach (descending, then landmark
-// order): the landmarks reaching tile column tj are a prefix of that list, so every tile of the row reads the same
-// list. The four waves of a workgroup take equal contiguous shares of the prefix; their partial tiles are added in
-// wave order through LDS and wave 0 writes S. Each S entry is owned by one tile: no atomics, and the sums are taken in
-// a fixed order (bitwise reproducible). g_S = b_r s_r - sum C' z is formed by the diagonal tile (tj == ti), whose
-// prefix is the whole list.
+// order): the landmarks reaching tile column tj + t are a prefix of that list, and the run's list splits into
+// segments of landmarks reaching 4, 3, 2 and 1 of its tiles. Per landmark a lane loads one value of the row side
+// (component k of row i's coupling) once for all the tiles it reaches, and W1 = E_inv C1 is one 16 x 16 x 4 product
+// (A' = E_inv in rows 0..d_e-1 and z in row 4, from LDS) whose output lands in the layout of the tile products' A
+// operand and carries the g_S part C1' z in row 4; then per tile one column-side load and one product. E_inv and z
+// come from one coalesced load per group of landmarks through LDS.
 //
-// Per landmark a lane loads one value of the row side (component k of its row i's coupling column) and one of the
-// column side; W1 = E_inv C1 is itself one 16 x 16 x 4 product (A' = E_inv in rows 0..d_e-1 and z in row 4, from
-// LDS), whose output lands in the layout of the tile product's A operand and carries the g_S part C1' z in row 4.
-// E_inv and z come from one coalesced load per group of landmarks through LDS.
+// The four waves of a workgroup take equal contiguous shares of each segment; their partial tiles are added in wave
+// order through LDS, and wave t writes tile t. Each S entry is owned by one tile: no atomics, and the sums are taken
+// in a fixed order (bitwise reproducible). g_S = b_r s_r - sum C' z is formed by the run starting at the diagonal
+// tile (tj == ti), whose list is the whole row's.
 #include "bls_internal.h"
 
 #include <algorithm>
@@ -31,13 +33,20 @@ namespace bls {
 namespace {
 
 constexpr int TW = 4;   // waves per workgroup
+constexpr int NT = 4;   // tiles per run
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-template <int DE> constexpr int tile_group() { return 64 / (DE * DE + DE) < 5 ? 64 / (DE * DE + DE) : 5; }
+// landmarks per group: what one cooperative E_inv / z load covers (U (d_e^2 + d_e) <= 64 lanes), fewer when each
+// landmark holds the values of several tiles (registers)
+template <int DE, int SEG> constexpr int tile_group() {
+    constexpr int cap = 64 / (DE * DE + DE);
+    constexpr int want = SEG >= 3 ? 3 : (SEG == 2 ? 4 : 5);
+    return cap < want ? cap : want;
+}
 
-template <int U> struct TileGroup {
+template <int U, int SEG> struct TileGroup {
     double ez;   // lane l < U (DE^2 + DE): value l % (DE^2 + DE) of slot l / (DE^2 + DE): E_inv, then z
-    double x[U], b[U];
+    double x[U], b[U][SEG];
 };
 
 template <int DE, class TC, class TEI, class TS>
@@ -48,29 +57,40 @@ __global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ 
                                                   TS *S, double *g, const int64_t *__restrict__ S_colptr, int64_t nr,
                                                   int dr, const unsigned long long *flag, unsigned *fin_counter,
                                                   unsigned long long *fin_flag, int64_t *fin_out) {
-    constexpr int U = tile_group<DE>();
     constexpr int EZN = DE * DE + DE, EZS = EZN + 1;
-    __shared__ double red[TW - 1][256];
+    constexpr int UMAX = tile_group<DE, 1>();
+    __shared__ double red[TW][NT][256];
     __shared__ double gred[TW - 1][16];
-    __shared__ double ezs[TW][2 * U * EZS];
+    __shared__ double ezs[TW][2 * UMAX * EZS];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, k = lane >> 4, i = lane & 15;
     if (*flag == (unsigned long long)NO_FAILURE) {
         const TileUnit *up = units + blockIdx.x;
         const int ti = __builtin_amdgcn_readfirstlane(up->ti), tj = __builtin_amdgcn_readfirstlane(up->tj);
-        const int cnt = __builtin_amdgcn_readfirstlane(up->cnt);
+        const int nt = __builtin_amdgcn_readfirstlane(up->nt);
         const int rb0 = __builtin_amdgcn_readfirstlane(up->rb0), spanc = __builtin_amdgcn_readfirstlane(up->spanc);
+        int cnt[NT + 1];
+#pragma unroll
+        for (int t = 0; t < NT; t++) cnt[t] = __builtin_amdgcn_readfirstlane(up->cnt[t]);
+        cnt[NT] = 0;
         const int cst = DE * dr;
-        const int lo = cnt * w / TW, hi = cnt * (w + 1) / TW;
         const TileEntry *ep = entries + up->elo;
-        // the lane's row (16 ti + i) and column (16 tj + i): block relative to rb0 and byte offset of its value
-        // (component k) inside a coupling block, d_e x d_r column-major
-        const int64_t R = 16ll * ti + i, Cc = 16ll * tj + i;
-        const bool rv = R < nr && k < DE, cv = Cc < nr && k < DE;
-        const int rb = (int)(R / dr), cb = (int)(Cc / dr);
-        const int qr = rv ? rb - rb0 : -1, qc = cv ? cb - rb0 : -1;
+        // the lane's row (16 ti + i) and, per tile, column (16 (tj + t) + i): block relative to rb0 and byte offset of
+        // its value (component k) inside a coupling block, d_e x d_r column-major
+        const int64_t R = 16ll * ti + i;
+        const bool rv = R < nr && k < DE;
+        const int rb = (int)(R / dr);
+        const int qr = rv ? rb - rb0 : -1;
         const int aoff = rv ? ((k + DE * (int)(R - (int64_t)rb * dr)) + qr * cst) * (int)sizeof(TC) : 0;
-        const int boff = cv ? ((k + DE * (int)(Cc - (int64_t)cb * dr)) + qc * cst) * (int)sizeof(TC) : 0;
+        int qc[NT], boff[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int64_t Cc = 16ll * (tj + t) + i;
+            const bool cv = Cc < nr && k < DE && t < nt;
+            const int cb = (int)(Cc / dr);
+            qc[t] = cv ? cb - rb0 : -1;
+            boff[t] = cv ? ((k + DE * (int)(Cc - (int64_t)cb * dr)) + qc[t] * cst) * (int)sizeof(TC) : 0;
+        }
         const bool diag = tj == ti;
         const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(C + up->cbase * cst), (short)0, spanc * cst * (int)sizeof(TC), 0x00020000);
@@ -82,27 +102,31 @@ __global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ 
                 return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(crs, off, 0, 0));
             }
         };
-        d4 D = {0.0, 0.0, 0.0, 0.0};
+        d4 D[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) D[t] = d4{0.0, 0.0, 0.0, 0.0};
         double gacc = 0.0;
         // the group's E_inv and z in LDS, slot u at u * EZS: E_inv (column-major), z, then a zero
         const int ez_slot = lane / EZN, ez_q = lane - ez_slot * EZN;
-        const bool ez_on = lane < U * EZN;
-        double *buf0 = ezs[w], *buf1 = ezs[w] + U * EZS;
-        if (lane < 2 * U) ezs[w][lane * EZS + EZN] = 0.0;
+        if (lane < 2 * UMAX) ezs[w][lane * EZS + EZN] = 0.0;
         // the lane's value of the first product's A operand, A'[i][k] (lane 16 k + i): E_inv[i][k] in rows i < d_e,
         // z[k] in row 4, else zero
         const int aq = (i < DE && k < DE) ? i + DE * k : (i == 4 && k < DE) ? DE * DE + k : EZN;
-        auto run = [&](auto diag_c) {
+        // the landmarks [lo, hi) of the list, each reaching the run's first SEG tiles
+        auto run = [&](auto diag_c, auto seg_c, int lo, int hi) {
             constexpr bool DIAG = decltype(diag_c)::value;
+            constexpr int SEG = decltype(seg_c)::value;
+            constexpr int U = tile_group<DE, SEG>();
             if (lo >= hi) return;
             const int ngroups = (hi - lo + U - 1) / U;
+            double *buf0 = ezs[w], *buf1 = ezs[w] + U * EZS;
             auto load_ev = [&](int gi) {
                 const int t = lo + gi * U + lane < hi ? lo + gi * U + lane : hi - 1;
                 return *reinterpret_cast<const int4 *>(ep + t);
             };
             // the group's E_inv / z (one coalesced load; every lane's address valid, only lanes < U (d_e^2 + d_e)
-            // keep theirs) and, per landmark, the row- and column-side values
-            auto issue = [&](int gi, TileGroup<U> &G, const int4 ev) {
+            // keep theirs) and, per landmark, the row side and each tile's column side
+            auto issue = [&](int gi, TileGroup<U, SEG> &G, const int4 ev) {
                 const int e_l = __shfl(ev.x, ez_slot < U ? ez_slot : 0, 64);
                 const TEI *pe = Einv + (int64_t)e_l * (DE * DE) + (ez_q < DE * DE ? ez_q : 0);
                 const double *pz = z + (int64_t)e_l * DE + (ez_q < DE * DE ? 0 : ez_q - DE * DE);
@@ -122,34 +146,34 @@ __global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ 
                     const int base = (cw - qf) * (cst * (int)sizeof(TC));
                     const bool pr = (unsigned)(qr - qf) <= wl && u < nvalid;
                     G.x[u] = ldc(pr ? base + aoff : (int)0x80000000);
-                    if constexpr (!DIAG) {
-                        const bool pc = (unsigned)(qc - qf) <= wl && u < nvalid;
-                        G.b[u] = ldc(pc ? base + boff : (int)0x80000000);
+#pragma unroll
+                    for (int t = DIAG ? 1 : 0; t < SEG; t++) {
+                        const bool pc = (unsigned)(qc[t] - qf) <= wl && u < nvalid;
+                        G.b[u][t] = ldc(pc ? base + boff[t] : (int)0x80000000);
                     }
                 }
             };
-            // per landmark two products: W = A' x (rows k < d_e: W1[k][i] = sum_n E_inv[k][n] C1[n][i], the A operand
-            // of the second in its layout; row 4: the g_S part sum_n z[n] C1[n][i]), then D += W1' C2
-            auto compute = [&](const TileGroup<U> &G, double *buf) {
-                if (ez_on) buf[ez_slot * EZS + ez_q] = G.ez;
+            // per landmark: W = A' x (rows k < d_e: W1[k][i] = sum_n E_inv[k][n] C1[n][i], the A operand of the tile
+            // products in their layout; row 4: the g_S part sum_n z[n] C1[n][i]), then D[t] += W1' C2 per tile
+            auto compute = [&](const TileGroup<U, SEG> &G, double *buf) {
+                if (lane < U * EZN) buf[ez_slot * EZS + ez_q] = G.ez;
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const d4 W = __builtin_amdgcn_mfma_f64_16x16x4f64(buf[u * EZS + aq], G.x[u],
                                                                       d4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
-                    if constexpr (DIAG) {
-                        gacc += W[1];
-                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(W[0], G.x[u], D, 0, 0, 0);
-                    } else {
-                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(W[0], G.b[u], D, 0, 0, 0);
-                    }
+                    if constexpr (DIAG) gacc += W[1];
+#pragma unroll
+                    for (int t = 0; t < SEG; t++)
+                        D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(W[0], (DIAG && t == 0) ? G.x[u] : G.b[u][t], D[t],
+                                                                    0, 0, 0);
                 }
                 __builtin_amdgcn_wave_barrier();
             };
             // two groups in flight, their records loaded two groups ahead; the loads of a group past the last are
             // issued all the same (records clamped to the last one, every C load outside the buffer: zeros), so
             // that the loop body has no branch
-            TileGroup<U> GA, GB;
+            TileGroup<U, SEG> GA, GB;
             int4 evA = load_ev(0), evB = load_ev(1);
             issue(0, GA, evA);
             evA = load_ev(2);
@@ -163,42 +187,56 @@ __global__ __launch_bounds__(64 * TW) void k_tiles(const TileUnit *__restrict__ 
                 compute(GB, buf1);
             }
         };
-        if (diag) run(std::true_type{});
-        else run(std::false_type{});
-        // the waves' partial tiles added in wave order; wave 0 writes S (upper part) and g
-        if (w > 0) {
+        // segment s: the landmarks reaching exactly the first s tiles, [cnt[s], cnt[s - 1]); the wave's share of it
+        auto seg = [&](auto diag_c, auto seg_c) {
+            constexpr int SEG = decltype(seg_c)::value;
+            const int a = cnt[SEG], n = cnt[SEG - 1] - cnt[SEG];
+            run(diag_c, seg_c, a + n * w / TW, a + n * (w + 1) / TW);
+        };
+        auto all_segs = [&](auto diag_c) {
+            seg(diag_c, std::integral_constant<int, 4>{});
+            seg(diag_c, std::integral_constant<int, 3>{});
+            seg(diag_c, std::integral_constant<int, 2>{});
+            seg(diag_c, std::integral_constant<int, 1>{});
+        };
+        if (diag) all_segs(std::true_type{});
+        else all_segs(std::false_type{});
+        // the waves' partial tiles added in wave order: wave t writes tile t (upper part of S), wave 0 also g
 #pragma unroll
-            for (int v = 0; v < 4; v++) red[w - 1][lane * 4 + v] = D[v];
-            if (k == 0) gred[w - 1][i] = gacc;
-        }
+        for (int t = 0; t < NT; t++)
+            if (t < nt)
+#pragma unroll
+                for (int v = 0; v < 4; v++) red[w][t][lane * 4 + v] = D[t][v];
+        if (w > 0 && k == 0) gred[w - 1][i] = gacc;
         __syncthreads();
-        if (w == 0) {
-            if (diag) {
+        if (w == 0 && diag) {
 #pragma unroll
-                for (int q = 0; q < TW - 1; q++) gacc += gred[q][i];
-                if (k == 0 && R < nr) g[R] = b_r[R] * (s_r ? s_r[R] : 1.0) - gacc;
-            }
-            if (Cc < nr && cnt > 0) {
-                const int cl = (int)(Cc - (int64_t)cb * dr);
-                const int cbl = (int)((16ll * tj) / dr);
-                const int64_t colbase = S_colptr[Cc] - 1;
-                const int32_t *tab = tabs + (int64_t)blockIdx.x * 16;
+            for (int q = 0; q < TW - 1; q++) gacc += gred[q][i];
+            if (k == 0 && R < nr) g[R] = b_r[R] * (s_r ? s_r[R] : 1.0) - gacc;
+        }
+        const int t = w;
+        const int64_t Cc = 16ll * (tj + t) + i;
+        if (t < nt && cnt[t] > 0 && Cc < nr) {
+            const int cb = (int)(Cc / dr);
+            const int cl = (int)(Cc - (int64_t)cb * dr);
+            const int cbl = (int)((16ll * (tj + t)) / dr);
+            const int64_t colbase = S_colptr[Cc] - 1;
+            const int32_t *tab = tabs + ((int64_t)blockIdx.x * NT + t) * 16;
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    double dv = D[v];
+            for (int v = 0; v < 4; v++) {
+                double dv = 0.0;
 #pragma unroll
-                    for (int q = 0; q < TW - 1; q++) dv += red[q][lane * 4 + v];
-                    // D layout of v_mfma_f64_16x16x4f64: lane l holds rows l / 16 + 4 v of column l % 16
-                    const int64_t Rr = 16ll * ti + k + 4 * v;
-                    if (Rr >= nr) continue;
-                    const int rbb = (int)(Rr / dr);
-                    const int rll = (int)(Rr - (int64_t)rbb * dr);
-                    if (rbb > cb || (rbb == cb && cl < rll)) continue;   // below the diagonal block
-                    const int off = tab[(rbb - rb0) * 4 + (cb - cbl)];
-                    if (off < 0) continue;
-                    TS *d = S + colbase + off + rll;
-                    *d = (TS)((double)*d - dv);
-                }
+                for (int q = 0; q < TW; q++) dv += red[q][t][lane * 4 + v];
+                // D layout of v_mfma_f64_16x16x4f64: lane l holds rows l / 16 + 4 v of column l % 16
+                const int64_t Rr = 16ll * ti + k + 4 * v;
+                if (Rr >= nr) continue;
+                const int rbb = (int)(Rr / dr);
+                const int rll = (int)(Rr - (int64_t)rbb * dr);
+                if (rbb > cb || (rbb == cb && cl < rll)) continue;   // below the diagonal block
+                const int off = tab[(rbb - rb0) * 4 + (cb - cbl)];
+                if (off < 0) continue;
+                TS *d = S + colbase + off + rll;
+                *d = (TS)((double)*d - dv);
             }
         }
     }
@@ -315,18 +353,33 @@ int tiles_build(TilePlan &P, const TileStructure &T, std::string *why) {
     ents.reserve(raw.size());
     const int64_t max_bytes = 0x7fffff00ll;
     const int64_t ntr = (nr + 15) / 16;
+    // the 4 x 4 offsets in S's scalar columns of the (row block, column block) pairs of tile (ti, tj); -1: not stored
+    auto push_tab = [&](int64_t ti, int64_t tj, bool any) {
+        const int64_t t0 = (int64_t)tabs.size();
+        tabs.resize(tabs.size() + 16, -1);
+        if (!any || 16 * tj >= nr) return;
+        const int64_t rb0 = (16 * ti) / dr, cbl = (16 * tj) / dr;
+        for (int q = 0; q < 4; q++)
+            for (int p = 0; p < 4; p++) {
+                const int64_t rb = rb0 + q, c = cbl + p;
+                if (rb >= nrb || c >= nrb || rb > c) continue;
+                for (int64_t kk = (*T.bcp)[c]; kk < (*T.bcp)[c + 1]; kk++)
+                    if ((*T.brv)[kk] == rb) tabs[t0 + q * 4 + p] = (int32_t)(*T.bro)[kk];
+            }
+    };
     size_t r0 = 0;
     for (int64_t ti = 0; ti < ntr; ti++) {
         const int64_t rb0 = (16 * ti) / dr;
         if (r0 == raw.size() || raw[r0].ti != ti) {
-            // no landmark couples the tile row's blocks: one unit that only writes g = b_r s_r
+            // no landmark couples the tile row's blocks: one run that only writes g = b_r s_r
             TileUnit u{};
             u.ti = u.tj = (int32_t)ti;
-            u.has_g = 1;
+            u.nt = 1;
             u.rb0 = (int32_t)rb0;
             u.elo = (int64_t)ents.size();
-            tabs.resize(tabs.size() + 16, -1);
-            us.push_back(Pending{0, (int64_t)tabs.size() / 16 - 1, u});
+            const int64_t t0 = (int64_t)tabs.size() / 16;
+            for (int t = 0; t < NT; t++) push_tab(ti, ti + t, false);
+            us.push_back(Pending{0, t0, u});
             continue;
         }
         size_t r1 = r0;
@@ -338,50 +391,50 @@ int tiles_build(TilePlan &P, const TileStructure &T, std::string *why) {
         }
         const int64_t spanc = cm + 1 - cb;
         if (spanc * cst * 8 >= max_bytes) return no("a tile row's couplings span more than 2 GB");
+        if ((int64_t)(r1 - r0) > INT32_MAX) return no("too many landmarks in a tile row");
         const int64_t elo = (int64_t)ents.size();
         for (size_t q = r0; q < r1; q++)
-            ents.push_back(TileEntry{raw[q].e, (int32_t)(raw[q].c0 - cb), raw[q].qf | (raw[q].ql << 16), 0});
+            ents.push_back(TileEntry{raw[q].e, (int32_t)(raw[q].c0 - cb), raw[q].qf | (raw[q].ql << 16), raw[q].last});
+        // prefix[s]: the landmarks reaching tile column ti + s (the list is sorted by reach, descending)
         const int maxlast = raw[r0].last;
-        int64_t cnt = (int64_t)(r1 - r0);
-        for (int s = 0; s <= maxlast; s++) {
-            while (cnt > 0 && raw[r0 + cnt - 1].last < s) cnt--;   // the prefix reaching tile column ti + s
-            const int64_t tj = ti + s;
-            if (cnt == 0 || 16 * tj >= nr) break;
-            const int64_t cbl = (16 * tj) / dr, cbh = std::min(16 * tj + 15, nr - 1) / dr;
-            if (cbh - rb0 >= 64) return no("a tile column more than 64 blocks from its row");
-            if (cnt > INT32_MAX) return no("too many landmarks in a tile row");
+        std::vector<int64_t> prefix(maxlast + 2, 0);
+        for (size_t q = r0; q < r1; q++) prefix[raw[q].last]++;
+        for (int s = maxlast - 1; s >= 0; s--) prefix[s] += prefix[s + 1];
+        for (int s0 = 0; s0 <= maxlast && 16 * (ti + s0) < nr; s0 += NT) {
             TileUnit u{};
             u.ti = (int32_t)ti;
-            u.tj = (int32_t)tj;
-            u.cnt = (int32_t)cnt;
-            u.has_g = s == 0;
+            u.tj = (int32_t)(ti + s0);
             u.rb0 = (int32_t)rb0;
             u.spanc = (int32_t)spanc;
             u.elo = elo;
             u.cbase = cb;
-            // offsets in S's scalar column of each (row block, column block) the tile covers; -1: not stored
-            const int64_t t0 = (int64_t)tabs.size();
-            tabs.resize(tabs.size() + 16, -1);
-            for (int q = 0; q < 4; q++)
-                for (int p = 0; p < 4; p++) {
-                    const int64_t rb = rb0 + q, c = cbl + p;
-                    if (rb >= nrb || c >= nrb || rb > c) continue;
-                    for (int64_t kk = (*T.bcp)[c]; kk < (*T.bcp)[c + 1]; kk++)
-                        if ((*T.brv)[kk] == rb) tabs[t0 + q * 4 + p] = (int32_t)(*T.bro)[kk];
-                }
-            us.push_back(Pending{cnt, t0 / 16, u});
-            P.nmfma += cnt;
+            int nt = 0;
+            int64_t weight = prefix[s0];   // the W products
+            for (int t = 0; t < NT; t++) {
+                const int s = s0 + t;
+                if (s > maxlast || 16 * (ti + s) >= nr) break;
+                const int64_t cbh = std::min(16 * (ti + s) + 15, nr - 1) / dr;
+                if (cbh - rb0 >= 64) return no("a tile column more than 64 blocks from its row");
+                u.cnt[t] = (int32_t)prefix[s];
+                weight += prefix[s];
+                nt = t + 1;
+            }
+            u.nt = nt;
+            const int64_t t0 = (int64_t)tabs.size() / 16;
+            for (int t = 0; t < NT; t++) push_tab(ti, ti + s0 + t, t < nt);
+            us.push_back(Pending{weight, t0, u});
+            P.nmfma += weight;
         }
         r0 = r1;
     }
-    // Dispatch order. The dispatcher places workgroup p on XCD p % 8, and the tiles of one tile row (and of
+    // Dispatch order. The dispatcher places workgroup p on XCD p % 8, and the runs of one tile row (and of
     // neighbouring rows) read the same coupling values: each XCD gets a contiguous range of tile rows of about equal
-    // weight, in its own L2, and runs its tiles heaviest first. Positions of an XCD past its last tile hold empty
+    // weight, in its own L2, and runs its runs heaviest first. Positions of an XCD past its last run hold empty
     // units (no landmarks, not diagonal: they write nothing).
     std::vector<std::vector<Pending>> xcd(8);
     {
         int64_t wsum = 0;
-        for (const Pending &x : us) wsum += x.weight + 64;   // 64: a tile's fixed cost in landmark units
+        for (const Pending &x : us) wsum += x.weight + 64;   // 64: a run's fixed cost in landmark units
         int64_t acc = 0;
         for (size_t q = 0; q < us.size();) {   // us is in tile-row order
             size_t q1 = q;
@@ -401,17 +454,18 @@ int tiles_build(TilePlan &P, const TileStructure &T, std::string *why) {
     std::vector<TileUnit> units;
     std::vector<int32_t> otabs;
     units.reserve(per * 8);
-    otabs.reserve(per * 8 * 16);
+    otabs.reserve(per * 8 * NT * 16);
     for (size_t j = 0; j < per; j++)
         for (int x = 0; x < 8; x++) {
             if (j < xcd[x].size()) {
                 units.push_back(xcd[x][j].u);
-                otabs.insert(otabs.end(), tabs.begin() + xcd[x][j].tab * 16, tabs.begin() + xcd[x][j].tab * 16 + 16);
+                const int64_t t0 = xcd[x][j].tab * 16;
+                otabs.insert(otabs.end(), tabs.begin() + t0, tabs.begin() + t0 + NT * 16);
             } else {
                 TileUnit e{};
-                e.tj = 1;   // ti 0, tj 1: not diagonal; cnt 0: no landmarks
+                e.tj = 1;   // ti 0, tj 1: not diagonal; no tiles, no landmarks
                 units.push_back(e);
-                otabs.insert(otabs.end(), 16, -1);
+                otabs.insert(otabs.end(), NT * 16, -1);
             }
         }
     if (units.size() > (size_t)INT32_MAX) return no("too many tiles");
