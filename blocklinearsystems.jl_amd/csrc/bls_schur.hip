@@ -973,6 +973,56 @@ __global__ void k_backsub(int64_t neb, const int32_t *e_dims, const int64_t *eof
     x_e[bo + lane] = z[bo + lane] - acc;
 }
 
+// backsubstitute! (src/schur_reduction.jl:541-576) for layouts of uniform blocks (d_e = DE) with the default offsets
+// whose landmarks couple contiguous runs of retained blocks: a landmark's coupling values then form one run in which
+// value (i, q) — component i, row q of its window's scalar retained rows — sits at DE q + i, against x_r at the
+// window's first row + q. 32 lanes per landmark, a lane per window row (coalesced over C and x_r), the lanes'
+// partials added in a fixed tree, the lanes i < d_e applying E_inv.
+template <int DE, class TC, class TEI, class TSC>
+__global__ __launch_bounds__(256) void k_backsub_win(int64_t neb, const int64_t *__restrict__ eliminated_ptr,
+                                                     const int32_t *__restrict__ entry_retained, int dr,
+                                                     const TC *__restrict__ C, const TEI *__restrict__ E_inv,
+                                                     const double *__restrict__ z, const double *__restrict__ x_r,
+                                                     const double *__restrict__ s_r, const double *__restrict__ s_e,
+                                                     double *__restrict__ x_e) {
+    const int64_t e = (int64_t)blockIdx.x * 8 + threadIdx.x / 32;
+    const int lane = threadIdx.x % 32;
+    double t[DE];
+#pragma unroll
+    for (int i = 0; i < DE; i++) t[i] = 0.0;
+    if (e < neb) {
+        const int64_t c0 = eliminated_ptr[e], nq = (eliminated_ptr[e + 1] - c0) * dr;
+        if (nq > 0) {
+            const int64_t xb = (int64_t)entry_retained[c0] * dr;
+            const TC *cp = C + c0 * DE * dr;
+            double se[DE];
+#pragma unroll
+            for (int i = 0; i < DE; i++) se[i] = s_e ? s_e[e * DE + i] : 1.0;
+            for (int64_t q = lane; q < nq; q += 32) {
+                const double x = x_r[xb + q];
+                if (s_r || s_e) {
+                    // scaled(T, C, s_e s_r), held in TSC before it is promoted (src/conditioning.jl:222-235)
+                    const double f = s_r ? s_r[xb + q] : 1.0;
+#pragma unroll
+                    for (int i = 0; i < DE; i++) t[i] += (double)(TSC)((double)cp[DE * q + i] * (se[i] * f)) * x;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < DE; i++) t[i] += (double)cp[DE * q + i] * x;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DE; i++)
+#pragma unroll
+        for (int o = 16; o >= 1; o >>= 1) t[i] += __shfl_xor(t[i], o, 32);
+    if (e >= neb || lane >= DE) return;
+    double acc = 0.0;   // lane i: row i of E_inv t
+#pragma unroll
+    for (int j = 0; j < DE; j++) acc += (double)E_inv[e * DE * DE + j * DE + lane] * t[j];
+    x_e[e * DE + lane] = z[e * DE + lane] - acc;
+}
+
 // Conditioning scales (src/conditioning.jl:99-141): Cs = the value the reference reads for every coupling entry,
 // scaled(C_er[i, ir], s_e[i] s_r[ir]) = C * (s_e * s_r) in Float64, for phase 2 to use in place of C (phase 1 scales
 // E and b_e itself). One lane per eliminated block.
@@ -1499,6 +1549,9 @@ struct bls_schur {
     int64_t clen = 0;             // length of C_data
     int64_t nmerges = 0;
     int uniform_dr_all = 0;
+    // d_e and d_r when every block has one size (d_e <= 4), the storage offsets are the default ones and every
+    // landmark couples a contiguous run of retained blocks (k_backsub_win), else 0
+    int win_de = 0, win_dr = 0;
     int max_dr = 0;
     DevArray<int64_t> d_slot_rowbase;
     DevArray<int64_t> d_retained_ptr;      // retained-major index of the couplings (0-based), for k_ct_xe
@@ -2195,6 +2248,21 @@ extern "C" int bls_schur_create(int64_t nr, int64_t nrblocks, int64_t neblocks, 
     for (int64_t r = 0; r < nrblocks; r++) h->max_dr = std::max(h->max_dr, (int)h->r_dims[r]);
     for (int64_t r = 0; r < nrblocks; r++)
         if (h->r_dims[r] != h->uniform_dr_all) h->uniform_dr_all = 0;
+    {
+        const int64_t de = neblocks ? h->e_dims[0] : 0, dr = h->uniform_dr_all;
+        bool ok = de >= 1 && de <= 4 && dr >= 1;
+        for (int64_t e = 0; ok && e < neblocks; e++) {
+            ok = h->e_dims[e] == de && h->eoffsets[e] == e * de * de && h->boffsets_e[e] == e * de;
+            for (int64_t c = h->eliminated_ptr[e] + 1; ok && c < h->eliminated_ptr[e + 1]; c++)
+                ok = h->entry_retained[c] == h->entry_retained[c - 1] + 1;
+        }
+        for (int64_t r = 0; ok && r < nrblocks; r++) ok = h->boffsets_r[r] == r * dr;
+        for (int64_t c = 0; ok && c < ncoup; c++) ok = h->coffsets[c] == c * de * dr;
+        if (ok) {
+            h->win_de = (int)de;
+            h->win_dr = (int)dr;
+        }
+    }
     if ((rc = h->d_eoffsets.upload(h->eoffsets)) || (rc = h->d_boffsets_e.upload(h->boffsets_e)) ||
         (rc = h->d_coffsets.upload(h->coffsets)) || (rc = h->d_eliminated_ptr.upload(h->eliminated_ptr)) ||
         (rc = h->d_boffsets_r.upload(h->boffsets_r)) || (rc = h->d_e_dims.upload(narrow32(h->e_dims))) ||
@@ -2373,7 +2441,21 @@ int backsub(bls_schur *h, double *x_e, const double *x_r, const TC *C_data, cons
             h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr, h->d_boffsets_e.ptr, h->d_eliminated_ptr.ptr,
             h->d_entry_retained.ptr, h->d_coffsets.ptr, h->d_r_dims.ptr, h->d_boffsets_r.ptr, C_data, E_inv, z, x_r, s_r,
             s_e, x_e);
-    else if (h->neb)
+    else if (h->neb && h->win_de > 0) {
+        const unsigned grid = grid_for(h->neb, 8);
+        auto go = [&](auto de_c) {
+            constexpr int DE = decltype(de_c)::value;
+            k_backsub_win<DE, TC, TEI, TSC><<<grid, 256, 0, s>>>(h->neb, h->d_eliminated_ptr.ptr,
+                                                                 h->d_entry_retained.ptr, h->win_dr, C_data, E_inv, z,
+                                                                 x_r, s_r, s_e, x_e);
+        };
+        switch (h->win_de) {
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 3: go(std::integral_constant<int, 3>{}); break;
+        default: go(std::integral_constant<int, 4>{}); break;
+        }
+    } else if (h->neb)
         k_backsub<TC, TEI, TSC><<<grid_for(h->neb, 256 / BS_LANES), 256, 0, s>>>(h->neb, h->d_e_dims.ptr, h->d_eoffsets.ptr,
                                                                   h->d_boffsets_e.ptr, h->d_eliminated_ptr.ptr,
                                                                   h->d_entry_retained.ptr, h->d_coffsets.ptr,
