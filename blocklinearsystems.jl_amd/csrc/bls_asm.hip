@@ -14,6 +14,7 @@
 // The handle's storage layout also serves the queries that read a filled Hessian: dot(x, H, y) (src/linalg.jl:133-260,
 // per-block partials and a fixed-order sum) and getdiag! (src/schur_linear_systems.jl:239-270).
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <memory>
 
@@ -175,6 +176,46 @@ __global__ __launch_bounds__(256) void k_hdot_e(HDotArgs a, const double *__rest
     if (lane == 0 && e < a.neb) part[e] = acc;
 }
 
+// the same sum for layouts of uniform blocks (d_e = DE) with the default offsets whose landmarks couple contiguous
+// runs of retained blocks: a landmark's couplings are one run of values, value (i, q) — component i, row q of its
+// window's scalar retained rows — at DE q + i, against x_r and y_r at the window's first row + q. 32 lanes per
+// eliminated block, a lane per window row (coalesced over C, x_r and y_r), E_e by the lanes w < DE^2, the lanes'
+// sums added in a fixed tree
+template <int DE, class T>
+__global__ __launch_bounds__(256) void k_hdot_win(int64_t neb, int64_t nr, int dr, const int64_t *__restrict__ cptr,
+                                                  const int64_t *__restrict__ cro, const double *__restrict__ x,
+                                                  const double *__restrict__ y, const T *__restrict__ E,
+                                                  const T *__restrict__ C, double *__restrict__ part) {
+    const int64_t e = (int64_t)blockIdx.x * 8 + threadIdx.x / 32;
+    const int lane = threadIdx.x % 32;
+    double acc = 0.0;
+    if (e < neb) {
+        double xe[DE], ye[DE];
+#pragma unroll
+        for (int i = 0; i < DE; i++) {
+            xe[i] = x[nr + e * DE + i];
+            ye[i] = y[nr + e * DE + i];
+        }
+        if (lane < DE * DE) {
+            const int col = lane / DE, row = lane - col * DE;
+            acc += (double)E[e * DE * DE + lane] * xe[row] * ye[col];
+        }
+        const int64_t c0 = cptr[e], nq = (cptr[e + 1] - c0) * dr;
+        if (nq > 0) {
+            const int64_t xb = cro[c0];
+            const T *cp = C + c0 * DE * dr;
+            for (int64_t q = lane; q < nq; q += 32) {
+                const double xr = x[xb + q], yr = y[xb + q];
+#pragma unroll
+                for (int i = 0; i < DE; i++) acc += (double)cp[DE * q + i] * (xr * ye[i] + yr * xe[i]);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 32);
+    if (lane == 0 && e < neb) part[e] = acc;
+}
+
 // one lane per stored block of R (src/linalg.jl:153-190): a dense R is read in its upper triangle (each
 // off-diagonal entry counted both ways, a diagonal block by its upper triangle); a sparse R's blocks whole, an
 // off-diagonal block both ways
@@ -255,6 +296,9 @@ struct bls_asm {
     DevArray<int32_t> d_clen;  // per coupling: d_e d_r (dot)
     DevArray<double> d_part;
     int64_t nrblk = 0;
+    // d_e and d_r when every block has one size (d_e <= 4), the storage offsets are the default ones and every
+    // landmark couples a contiguous run of retained blocks (k_hdot_win), else 0
+    int win_de = 0, win_dr = 0;
 };
 
 namespace {
@@ -324,6 +368,21 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         }
     }
     for (int64_t r = 0; r < nrb; r++) dbr[r].blk = RGBlock{boff[r], 1, 1, (int32_t)dim[r], 0, 0};
+    {
+        const int64_t de = neb ? e_dims[0] : 0, dr = nrb ? r_dims[0] : 0;
+        bool ok = de >= 1 && de <= 4 && dr >= 1;
+        for (int64_t r = 0; ok && r < nrb; r++) ok = r_dims[r] == dr && boffsets_r[r] - 1 == r * dr;
+        for (int64_t e = 0; ok && e < neb; e++) {
+            ok = e_dims[e] == de && eoffsets[e] - 1 == e * de * de && boffsets_e[e] - 1 == e * de;
+            for (int64_t c = eliminated_ptr[e]; ok && c < eliminated_ptr[e + 1] - 1; c++)   // 1-based: c - 1 and c
+                ok = entry_retained[c] == entry_retained[c - 1] + 1;
+        }
+        for (int64_t c = 0; ok && c < ncoup; c++) ok = coffsets[c] - 1 == c * de * dr;
+        if (ok) {
+            h->win_de = (int)de;
+            h->win_dr = (int)dr;
+        }
+    }
     // R: index of the destination of retained pair (p, q)
     std::vector<int64_t> rcol;   // sparse: block column start in dR
     if (h->dense) {
@@ -620,7 +679,22 @@ int hessian_dot(bls_asm *a, const double *x, const double *y, const T *E_data, c
         return fail(BLS_ERR_ARG, "dot(x, H, y): a required array is NULL");
     const HDotArgs d = hdot_args(a);
     constexpr int per_wg = 256 / HD_LANES;
-    if (a->neb) k_hdot_e<T><<<(unsigned)((a->neb + per_wg - 1) / per_wg), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
+    if (a->neb && a->win_de > 0) {
+        const unsigned grid = (unsigned)((a->neb + 7) / 8);
+        auto go = [&](auto de_c) {
+            constexpr int DE = decltype(de_c)::value;
+            k_hdot_win<DE, T><<<grid, 256, 0, s>>>(a->neb, a->nr, a->win_dr, a->d_cptr.ptr, a->d_cro.ptr, x, y, E_data,
+                                                   C_data, a->d_part.ptr);
+        };
+        switch (a->win_de) {
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 3: go(std::integral_constant<int, 3>{}); break;
+        default: go(std::integral_constant<int, 4>{}); break;
+        }
+    } else if (a->neb) {
+        k_hdot_e<T><<<(unsigned)((a->neb + per_wg - 1) / per_wg), 256, 0, s>>>(d, x, y, E_data, C_data, a->d_part.ptr);
+    }
     if (a->nrblk) k_hdot_r<T><<<(unsigned)((a->nrblk + 255) / 256), 256, 0, s>>>(d, x, y, R, a->d_part.ptr);
     fsum_launch(a->neb + a->nrblk, a->d_part.ptr, a->d_part.ptr + a->neb + a->nrblk, result, s);   // FSUM_CHUNKS slots after the partials
     BLS_HIP(hipGetLastError());

@@ -82,3 +82,42 @@ def test_hessian_dot_matches_oracle(blsamd, sparse):
         scale = A.bilinear(Habs, np.abs(a), np.abs(b))
         assert abs(got - want) <= 1e-12 * scale
         assert dev.hessian_dot(T(a), h, T(b), E, C, R) == got
+
+
+@pytest.mark.parametrize("de,dr", [(3, 6), (2, 9), (4, 7), (1, 6)])
+def test_hessian_dot_contiguous_windows(blsamd, de, dr):
+    """dot(x, H, y) on a layout of uniform blocks whose landmarks couple contiguous camera windows (the
+    coalesced per-row path of the C ABI's dot), against x' H y with H formed densely from the same storage
+    (R's diagonal blocks, E, and each coupling block in both triangles): within 1e-12 of |x|' |H| |y|, bitwise
+    reproducible."""
+    import torch
+    dev = blsamd.device
+    p = blsamd.generators.window_problem(seed=de * 10 + dr, ncam=24, nland=300, de=de, dr=dr,
+                                         kmin=3 if de == 4 else 2, dense=True)
+    st = p.structure
+    nr, ne = st.nr, st.ne
+    n = nr + ne
+    H = np.zeros((n, n))
+    H[:nr, :nr] = p.R
+    for e in range(st.neblocks):
+        d = st.e_dims[e]
+        eo, bo = st.eoffsets[e] - 1, nr + st.boffsets_e[e] - 1
+        H[bo:bo + d, bo:bo + d] = p.E_data[eo:eo + d * d].reshape(d, d, order="F")
+        for c in range(st.eliminated_ptr[e] - 1, st.eliminated_ptr[e + 1] - 1):
+            r = st.entry_retained[c] - 1
+            ro, dr_ = st.boffsets_r[r] - 1, st.r_dims[r]
+            co = st.coffsets[c] - 1
+            blk = p.C_data[co:co + d * dr_].reshape(d, dr_, order="F")   # d_e x d_r column-major
+            H[bo:bo + d, ro:ro + dr_] = blk
+            H[ro:ro + dr_, bo:bo + d] = blk.T
+    h = dev.AssemblyHandle(st, [], [], None)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    E, C, R = T(p.E_data), T(p.C_data), T(p.R.reshape(-1, order="F"))
+    rng = np.random.default_rng(de + dr)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    for a, b in ((x, y), (x, x)):
+        got = dev.hessian_dot(T(a), h, T(b), E, C, R)
+        want = a @ H @ b
+        scale = np.abs(a) @ np.abs(H) @ np.abs(b)
+        assert abs(got - want) <= 1e-12 * scale
+        assert dev.hessian_dot(T(a), h, T(b), E, C, R) == got
