@@ -72,6 +72,18 @@ def test_block_sizes_match_oracle_and_work_list(blsamd, de, dr):
     _agree(p, a, run_gpu(blsamd, p, mode=1))
 
 
+@pytest.mark.parametrize("ncam,de,dr,nland", [(2, 3, 6, 40), (3, 2, 16, 60), (1, 2, 6, 30)])
+def test_tiny_layouts(blsamd, ncam, de, dr, nland):
+    """Layouts of one or two tile rows (fewer scalar rows than a tile, a single camera): most XCDs get only empty
+    units, every tile is clipped at nr."""
+    p = blsamd.generators.window_problem(seed=ncam + de + dr, ncam=ncam, nland=nland, de=de, dr=dr,
+                                         kmin=1 if ncam == 1 else 2, kmax=ncam)
+    a = run_gpu(blsamd, p, mode=2)
+    assert a["h"].phase2_info()[0] == 2
+    compare(p, a, oracle_run(p))
+    _agree(p, a, run_gpu(blsamd, p, mode=1))
+
+
 def test_unobserved_cameras(blsamd):
     """Cameras seen by no landmark (a run in the middle, the last one): their tile rows hold no landmark, and g_S
     there is b_r alone; S blocks no landmark reaches are left as reset wrote them."""
