@@ -83,6 +83,18 @@ function DeviceSchurWorkspace(st::BLS.SchurStructure, ws::BLS.SchurReductionWork
     return d
 end
 
+# which phase-2 kernel the workspace runs (no reference counterpart): 0 the tiles of S on the matrix cores where the
+# sparse layout fits them, else the work list; 1 the work list; 2 the tiles
+set_phase2_mode!(d::DeviceSchurWorkspace, mode::Integer) =
+    check(ccall((:bls_schur_set_phase2_mode, libblsamd), Cint, (Ptr{Cvoid}, Cint), d.handle, mode))
+
+# (kernel in use: 1 work list, 2 tiles; tiles; landmark entries; matrix-core products)
+function phase2_info(d::DeviceSchurWorkspace)
+    info = zeros(Int64, 4)
+    check(ccall((:bls_schur_phase2_info, libblsamd), Cint, (Ptr{Cvoid}, Ptr{Int64}), d.handle, info))
+    return (info[1], info[2], info[3], info[4])
+end
+
 function staged(d, key::Symbol, v::AbstractVector{Float64})
     b = get!(() -> DevBuf{Float64}(length(v)), d.bufs, key)
     b.n == length(v) || (b = d.bufs[key] = DevBuf{Float64}(length(v)))
