@@ -215,50 +215,6 @@ __device__ inline double wg_sum(double v, double *red) {
     return out;
 }
 
-// Problem-wise (re)start of CG from the current x: r = g - S x (q holds S x; first = 1: x = 0, r = g),
-// z = M^-1 r, p = z, rz; the true relative residual decides whether the problem (re)enters the iteration.
-__global__ __launch_bounds__(UPD_THREADS) void k_pcg_restart(PcgArgs a, int first) {
-    __shared__ double red[UPD_THREADS];
-    const int pb = blockIdx.x;
-    const int64_t b0 = a.prob_bptr[pb], b1 = a.prob_bptr[pb + 1];
-    const int64_t s0 = a.bo[b0], s1 = a.bo[b1 - 1] + a.dims[b1 - 1];
-    ProbState &ps = a.st[pb];
-    double gg = 0.0, rr = 0.0;
-    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
-        const double gv = a.g[s];
-        if (first) a.x[s] = 0.0;
-        const double rv = first ? gv : gv - a.q[s];
-        a.r[s] = rv;
-        gg += gv * gv;
-        rr += rv * rv;
-    }
-    gg = wg_sum(gg, red);
-    rr = wg_sum(rr, red);
-    const bool ok = !ps.failed;
-    if (ok)
-        for (int64_t b = b0 + threadIdx.x; b < b1; b += UPD_THREADS) {
-            const int gd = group_dim(a, b);   // a group's rows follow one another; its first block solves them
-            if (gd) block_solve(group_L(a, b), gd, a.r + a.bo[b], a.z + a.bo[b]);
-        }
-    __syncthreads();
-    double rz = 0.0;
-    for (int64_t s = s0 + threadIdx.x; s < s1; s += UPD_THREADS) {
-        const double zv = ok ? a.z[s] : 0.0;
-        a.p[s] = zv;
-        rz += a.r[s] * zv;
-    }
-    rz = wg_sum(rz, red);
-    if (threadIdx.x == 0) {
-        if (first) { ps.gnorm = sqrt(gg); ps.iters = 0; }
-        ps.rz2[0] = rz;
-        ps.relres = ps.gnorm > 0 ? sqrt(rr) / ps.gnorm : 0.0;
-        const int was = first ? 0 : ps.active;
-        ps.active = (ok && ps.relres > a.tol && ps.iters < a.maxit && isfinite(ps.relres)) ? 1 : 0;
-        if (ps.active && !was) atomicAdd(a.nactive, 1);
-        if (!ps.active && was) atomicSub(a.nactive, 1);
-    }
-}
-
 // fixed-order sum over the 256 threads of a group
 __device__ inline double grp_sum(double v, double *red) {
     const int t = threadIdx.x;
@@ -278,6 +234,62 @@ __device__ inline double problem_sum(const double *part, int64_t b0, int64_t b1,
     double s = 0.0;
     for (int64_t b = b0 + threadIdx.x; b < b1; b += GRP_BLOCKS) s += part[b];
     return grp_sum(s, red);
+}
+
+// (Re)start of CG from the current x, per group of up to GRP_BLOCKS blocks of one problem (the groups of
+// k_pcg_step1): r = g - S x (q holds S x; first = 1: x = 0, r = g), z = M^-1 r, p = z, and per block the partials
+// g.g (in pq_part, free between iterations), r.r and r.z; k_pcg_restart_fin forms the problem's sums in a fixed
+// order and decides from the true relative residual whether the problem (re)enters the iteration.
+__global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_restart_grp(PcgArgs a, int first) {
+    const int pb = a.grp_prob[blockIdx.x];
+    const bool ok = !a.st[pb].failed;
+    const int64_t b = a.grp_b0[blockIdx.x] + threadIdx.x;
+    const bool in = b < a.grp_b1[blockIdx.x];
+    const int64_t o = in ? a.bo[b] : 0;
+    const int d = in ? a.dims[b] : 0;
+    double gg = 0.0, rr = 0.0;
+    for (int i = 0; i < d; i++) {
+        const double gv = a.g[o + i];
+        if (first) a.x[o + i] = 0.0;
+        const double rv = first ? gv : gv - a.q[o + i];
+        a.r[o + i] = rv;
+        gg += gv * gv;
+        rr += rv * rv;
+    }
+    if (a.gsize) __syncthreads();   // a group's rows are complete before its first block solves them
+    const int gd = in ? group_dim(a, b) : 0;
+    if (ok && gd) block_solve(group_L(a, b), gd, a.r + o, a.z + o);
+    if (a.gsize) __syncthreads();   // and its z before the other blocks read theirs
+    if (!in) return;
+    double rz = 0.0;
+    for (int i = 0; i < d; i++) {
+        const double zv = ok ? a.z[o + i] : 0.0;
+        a.p[o + i] = zv;
+        rz += a.r[o + i] * zv;
+    }
+    a.pq_part[b] = gg;
+    a.rr_part[b] = rr;
+    a.rz_part[b] = rz;
+}
+
+__global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_restart_fin(PcgArgs a, int first) {
+    __shared__ double red[GRP_BLOCKS];
+    const int pb = blockIdx.x;
+    const int64_t b0 = a.prob_bptr[pb], b1 = a.prob_bptr[pb + 1];
+    ProbState &ps = a.st[pb];
+    const double gg = problem_sum(a.pq_part, b0, b1, red);
+    const double rr = problem_sum(a.rr_part, b0, b1, red);
+    const double rz = problem_sum(a.rz_part, b0, b1, red);
+    if (threadIdx.x == 0) {
+        const bool ok = !ps.failed;
+        if (first) { ps.gnorm = sqrt(gg); ps.iters = 0; }
+        ps.rz2[0] = rz;
+        ps.relres = ps.gnorm > 0 ? sqrt(rr) / ps.gnorm : 0.0;
+        const int was = first ? 0 : ps.active;
+        ps.active = (ok && ps.relres > a.tol && ps.iters < a.maxit && isfinite(ps.relres)) ? 1 : 0;
+        if (ps.active && !was) atomicAdd(a.nactive, 1);
+        if (!ps.active && was) atomicSub(a.nactive, 1);
+    }
 }
 
 // PCG iteration, after q = S p (Saad Alg. 9.1, lines 3-5): alpha, x, r, z = M^-1 r, partial r.z and r.r
@@ -929,7 +941,8 @@ static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double 
         k_pcg_factor_groups_w<<<(unsigned)h->nrb, 64, 0, s>>>(a, h->nrb, h->d_Lp.ptr, precond_blocks);
     else
         k_pcg_factor<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_diag_voff.ptr, h->d_L.ptr, precond_blocks);
-    k_pcg_restart<<<nprob, UPD_THREADS, 0, s>>>(a, 1);
+    k_pcg_restart_grp<<<ngroups, GRP_BLOCKS, 0, s>>>(a, 1);
+    k_pcg_restart_fin<<<nprob, GRP_BLOCKS, 0, s>>>(a, 1);
     BLS_HIP(hipGetLastError());
     // the iteration graph (handle-owned buffers only; rebuilt when tol / maxit change; one per preconditioner kind)
     hipGraph_t &graph = groups ? h->graph_p : h->graph;
@@ -964,7 +977,8 @@ static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double 
             BLS_HIP(hipStreamSynchronize(s));
         }
         spmv(h->d_x.ptr, h->d_q.ptr, nullptr, 0);
-        k_pcg_restart<<<nprob, UPD_THREADS, 0, s>>>(a, 0);
+        k_pcg_restart_grp<<<ngroups, GRP_BLOCKS, 0, s>>>(a, 0);
+        k_pcg_restart_fin<<<nprob, GRP_BLOCKS, 0, s>>>(a, 0);
         BLS_HIP(hipGetLastError());
         BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
         BLS_HIP(hipStreamSynchronize(s));
