@@ -63,6 +63,9 @@ def main():
                     help="PART,LANES: use this phase-2 work-list shape instead of autotuning (profiling runs)")
     ap.add_argument("--slam-only", action="store_true",
                     help="time only the sequential-SLAM step (no other legs, no CPU baseline): for profiler runs")
+    ap.add_argument("--no-dense-solve", action="store_true",
+                    help="skip the dense reduced solve leg (its Bunch-Kaufman fallback launches two kernels per "
+                         "column): for hardware-counter runs of the other legs")
     ap.add_argument("--config3-only", action="store_true",
                     help="run only the dense leg (no SLAM line, no CPU baseline): for profiler runs of its kernels")
     args = ap.parse_args()
@@ -234,7 +237,7 @@ def main():
     S_dev, g_dev = S.cpu().numpy(), g.cpu().numpy()
     del E, C, b_r, b_e, Rd, S, g, csc, dmap, h, ws
     torch.cuda.empty_cache()
-    if not args.slam_only:
+    if not args.slam_only and not args.no_dense_solve:
         result["dense_cholesky"] = dense_leg(torch, dev, int(st.nr))
     torch.cuda.empty_cache()
     if args.qr_scale > 0:
