@@ -33,17 +33,25 @@ constexpr int TINY_W = 16, TINY_M = 8;   // row blocks of at most 16 columns and
 // where the Gram pass writes it: pair block (e1, e2) of the row block's entries at Q[e2] + d2 P[e1], row-major, so
 // (a, b) -> goff + Q[e2] + (b - P[e2]) + d2 a (the columns of a row block are its entries' in order, from 0); and
 // lane c forms (J_k' r_k)[c] at eoff + c.
+// A row block of two entries, one eliminated and one retained, whose coupling block has no other contribution
+// (cw[t] = 2 o + [entry 0 is the eliminated one], o the block's offset in C_data; else -1) writes that pair block
+// straight into C_data (d_e x d_r column-major at o) as 0 + value, the ordered sum of its one contribution.
 template <class T>
 __global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs, int64_t cnt, int slot,
                                                   const int32_t *__restrict__ ecd, const T *__restrict__ F,
                                                   const double *__restrict__ Fr, double *__restrict__ G,
-                                                  double *__restrict__ gp) {
+                                                  double *__restrict__ gp, double *__restrict__ Cd,
+                                                  const int64_t *__restrict__ cw) {
     const int wl = threadIdx.x & 63, per = 64 / slot, grp = wl / slot;
     const int l = wl - grp * slot, base = grp * slot;   // idle lanes (grp == per) join the shuffles, write nothing
     const int64_t t = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * per + grp;
     const bool valid = grp < per && t < cnt;
     const RGTab g = rg_load(tabs + (valid ? t : 0));
     const int m = valid ? g.n : 0, W = valid ? g.W : 0;
+    const int64_t cwt = valid ? cw[t] : -1;
+    // a coupling written in place: entry 0's columns are [0, d0), entry 1's [d0, W)
+    const int d0 = cwt >= 0 ? (ecd[g.u0] & 0xffff) : W, efirst = (int)(cwt & 1);
+    const int64_t cbase = cwt >> 1;
     double x[TINY_M];
 #pragma unroll
     for (int i = 0; i < TINY_M; i++) x[i] = (l < W && i < m) ? (double)F[g.raw_off + (int64_t)l * m + i] : 0.0;
@@ -81,7 +89,10 @@ __global__ __launch_bounds__(256) void k_asm_tiny(const RGTab *__restrict__ tabs
 #pragma unroll
             for (int i = 0; i < TINY_M; i++)
                 if (i < mmax) acc += xa[i] * x[i];
-            G[g.goff + qb + (int64_t)db * a] = acc;
+            if (a < d0 && l >= d0)   // element (a, l) of the coupling pair block (only when d0 < W: cw >= 0)
+                Cd[efirst ? cbase + (int64_t)(l - d0) * d0 + a : cbase + (int64_t)a * (W - d0) + (l - d0)] = 0.0 + acc;
+            else
+                G[g.goff + qb + (int64_t)db * a] = acc;
         }
     }
 }
@@ -109,10 +120,17 @@ __global__ __launch_bounds__(256) void k_red_gram_sum_light(int64_t nblk, int sl
         double acc = 0.0;
         if (!lower) {
             const int wt = col * bk.d1 + row;
-            for (int64_t c = cb; c < ce; c++) {
-                const int64_t ci = contrib[c];
-                acc += G[ci >= 0 ? ci + w : ~ci + wt];
+            auto at = [&](int64_t ci) { return ci >= 0 ? ci + w : ~ci + wt; };
+            int64_t c = cb;
+            for (; c + 4 <= ce; c += 4) {   // four contributions' loads in flight, added in order
+                const int64_t i0 = contrib[c], i1 = contrib[c + 1], i2 = contrib[c + 2], i3 = contrib[c + 3];
+                const double v0 = G[at(i0)], v1 = G[at(i1)], v2 = G[at(i2)], v3 = G[at(i3)];
+                acc += v0;
+                acc += v1;
+                acc += v2;
+                acc += v3;
             }
+            for (; c < ce; c++) acc += G[at(contrib[c])];
         }
         const int64_t at = tr ? bk.dst + (int64_t)row * bk.cstride + col : bk.dst + (int64_t)col * bk.cstride + row;
         if (!(lower && full)) S[at] = (T)acc;
@@ -275,6 +293,7 @@ struct bls_asm {
     bool dense = false;
     RGPlan rg;
     DevArray<RGTab> d_tabs, d_tiny;
+    DevArray<int64_t> d_tiny_cw;   // per tiny row block: its coupling written in place (k_asm_tiny), or -1
     int64_t ntiny = 0;
     int tiny_slot = 16;   // lanes per tiny row block: the widest of them
     DevArray<int32_t> d_ecd;
@@ -306,6 +325,7 @@ namespace {
 struct Dest {   // one destination block while planning
     RGBlock blk;
     std::vector<int64_t> contrib;
+    bool direct = false;   // a coupling block written in place by k_asm_tiny (no ordered sum)
 };
 
 std::string pair_name(int64_t i, int64_t j) {
@@ -428,6 +448,8 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
     // row blocks: records of the Gram pass, entry words, and each pair's destination
     std::vector<RGTab> recs, tiny;
     std::vector<int32_t> ecd;
+    struct Direct { int64_t tiny, c; int efirst; };   // tiny two-entry row blocks over a coupling
+    std::vector<Direct> direct;
     int64_t joff = 0, roff = 0, goff = 0, eoff = 0;
     for (int64_t k = 0; k < nrowblocks; k++) {
         const int64_t u0 = rb_ptr[k] - 1, u1 = rb_ptr[k + 1] - 1, ne = u1 - u0;
@@ -491,6 +513,8 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
                         if (c < 0)
                             return fail(BLS_ERR_ARG, pair_name(p + 1, q + 1) + " share no row block: the system has no storage for the pair");
                         dC[c].contrib.push_back(p >= nrb ? off : ~off);
+                        if (ne == 2 && W <= TINY_W && m <= TINY_M)
+                            direct.push_back(Direct{(int64_t)tiny.size() - 1, c, p >= nrb ? 1 : 0});
                     } else {   // retained pair: dense stores (min, max) upper, sparse (max, min) lower
                         const int64_t b = rblock(p, q);
                         if (b < 0)
@@ -510,10 +534,18 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         joff += m * W;
         roff += m;
     }
+    std::vector<int64_t> tcw(tiny.size(), -1);
+    for (const Direct &d : direct)
+        if (dC[d.c].contrib.size() == 1) {
+            tcw[d.tiny] = 2 * (coffsets[d.c] - 1) + d.efirst;
+            dC[d.c].direct = true;
+        }
     std::vector<RGTab> tabs;
     int rc;
     if ((rc = rg_classify(recs, tabs, h->rg))) return rc;
-    if ((rc = h->d_tabs.upload(tabs)) || (rc = h->d_tiny.upload(tiny)) || (rc = h->d_ecd.upload(ecd))) return rc;
+    if ((rc = h->d_tabs.upload(tabs)) || (rc = h->d_tiny.upload(tiny)) || (rc = h->d_ecd.upload(ecd)) ||
+        (rc = h->d_tiny_cw.upload(tcw)))
+        return rc;
     h->ntiny = (int64_t)tiny.size();
     h->tiny_slot = 1;
     for (const RGTab &g : tiny) h->tiny_slot = std::max(h->tiny_slot, (int)g.W);
@@ -582,14 +614,14 @@ extern "C" int bls_asm_create(int64_t nrblocks, int64_t neblocks, const int64_t 
         int maxsz = 1;
         for (int pass = 0; pass < 2; pass++)
             for (const Dest &d : *lists[l]) {
-                if ((d.contrib.size() <= (size_t)LIGHT_MAX) != (pass == 0)) continue;
+                if (d.direct || (d.contrib.size() <= (size_t)LIGHT_MAX) != (pass == 0)) continue;
                 blocks.push_back(d.blk);
                 contrib.insert(contrib.end(), d.contrib.begin(), d.contrib.end());
                 ptr.push_back((int64_t)contrib.size());
                 if (pass == 0) maxsz = std::max(maxsz, d.blk.d1 * d.blk.d2);
             }
         int64_t nl = 0;
-        for (const Dest &d : *lists[l]) nl += d.contrib.size() <= (size_t)LIGHT_MAX;
+        for (const Dest &d : *lists[l]) nl += !d.direct && d.contrib.size() <= (size_t)LIGHT_MAX;
         h->nlight[l] = nl;
         h->slot[l] = std::min(maxsz, 64);
         if ((rc = h->d_blocks[l].upload(blocks)) || (rc = h->d_ptr[l].upload(ptr)) || (rc = h->d_contrib[l].upload(contrib)))
@@ -624,7 +656,8 @@ extern "C" int bls_asm_hessian_f64(bls_asm *a, const double *J, const double *r,
     if (a->ntiny) {
         const int64_t per_wg = 4 * (64 / a->tiny_slot);
         k_asm_tiny<double><<<(unsigned)((a->ntiny + per_wg - 1) / per_wg), 256, 0, s>>>(
-            a->d_tiny.ptr, a->ntiny, a->tiny_slot, a->d_ecd.ptr, J, r, a->d_G.ptr, a->d_gp.ptr);
+            a->d_tiny.ptr, a->ntiny, a->tiny_slot, a->d_ecd.ptr, J, r, a->d_G.ptr, a->d_gp.ptr, C_data,
+            a->d_tiny_cw.ptr);
     }
     double *dst[bls_asm::ND] = {E_data, C_data, R, b_r, b_e};
     for (int l = 0; l < bls_asm::ND; l++) {

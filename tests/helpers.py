@@ -173,12 +173,14 @@ def max_block_rel_to(A, B, Mag):
     return worst
 
 
-def assembly_problem(blsamd, seed, ncam=6, nland=40, kmax=4, npairs=4, sparse=False, tall=()):
+def assembly_problem(blsamd, seed, ncam=6, nland=40, kmax=4, npairs=4, sparse=False, tall=(), dup=0, triple=0):
     """A description with observations (2 rows over [landmark; camera], the landmark listed first or second at
     random), camera priors and camera-pair rows; its SchurStructure, the row blocks as layout block numbers in each
     Jacobian's column order, random J and r, and (sparse) the stored lower blocks of R: every retained pair that
     shares a row block plus the diagonal, as a block CSC with 1-based data offsets. tall: row counts of extra
-    row blocks over cameras 1 and 2 (which then have 9 parameters each)."""
+    row blocks over cameras 1 and 2 (which then have 9 parameters each). dup: the first `dup` observations are
+    made a second time (their coupling blocks then have two contributions); triple: landmarks 1..triple each get a
+    2-row block over themselves and two cameras."""
     rng = np.random.default_rng(seed)
     cam = np.where(rng.random(ncam) < 0.5, 6, 9)
     if tall:
@@ -190,6 +192,11 @@ def assembly_problem(blsamd, seed, ncam=6, nland=40, kmax=4, npairs=4, sparse=Fa
         for c in rng.choice(ncam, size=int(rng.integers(2, kmax + 1)), replace=False):
             rb = [ncam + l + 1, int(c) + 1]
             rows.append(rb if rng.random() < 0.5 else rb[::-1]); sizes.append(2)
+    for i in range(dup):
+        rows.append(list(rows[i])); sizes.append(2)
+    for l in range(triple):
+        a, b = rng.choice(ncam, size=2, replace=False)
+        rows.append([ncam + l + 1, int(a) + 1, int(b) + 1]); sizes.append(2)
     for c in range(ncam):
         rows.append([c + 1]); sizes.append(int(cam[c]))
     for _ in range(npairs):

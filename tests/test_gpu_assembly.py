@@ -30,14 +30,19 @@ def _run(blsamd, st, rowblocks, rb_rows, J, r, R_layout):
 
 
 @pytest.mark.parametrize("sparse", [False, True])
-@pytest.mark.parametrize("case", ["light", "heavy", "tall"])
+@pytest.mark.parametrize("case", ["light", "heavy", "tall", "shared"])
 def test_assembly_matches_oracle(blsamd, sparse, case):
     """Dense and sparse R. Observation row blocks (2 rows, <= 16 columns) take the 16-lane path, 9-row priors and
     wide pair rows the Gram pass. light: 20 landmarks, every destination block has <= 32 contributions (one
     wavefront per block); heavy: 150 landmarks, the camera blocks of R and b_r have more (runs over a workgroup);
-    tall: camera-pair row blocks of 250, 400 and 1500 rows put row blocks in every launch class of the Gram pass."""
-    nland, tall = {"light": (20, ()), "heavy": (150, ()), "tall": (150, (250, 400, 1500))}[case]
-    st, rowblocks, rb_rows, J, r, R_layout = assembly_problem(blsamd, 5 + nland, nland=nland, sparse=sparse, tall=tall)
+    tall: camera-pair row blocks of 250, 400 and 1500 rows put row blocks in every launch class of the Gram pass.
+    A coupling block with one contribution from an observation is written in place by the 16-lane path; shared:
+    couplings with two contributions (an observation made twice) and couplings from row blocks over a landmark and
+    two cameras, which take the ordered sums."""
+    nland, tall, dup, triple = {"light": (20, (), 0, 0), "heavy": (150, (), 0, 0), "tall": (150, (250, 400, 1500), 0, 0),
+                                "shared": (20, (), 6, 5)}[case]
+    st, rowblocks, rb_rows, J, r, R_layout = assembly_problem(blsamd, 5 + nland, nland=nland, sparse=sparse, tall=tall,
+                                                              dup=dup, triple=triple)
     E, C, R, br, be = _run(blsamd, st, rowblocks, rb_rows, J, r, R_layout)
     H = A.assemble(A.HessianStorage(st, *(R_layout or ())), rowblocks, rb_rows, J, r)
     Ro = H.R.reshape(-1, order="F") if R_layout is None else H.R
