@@ -92,11 +92,12 @@ int tiles_launch(const TilePlan &P, int variant, const void *E_inv, const double
 // Blocks per grid for n items at `threads` per block.
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
-// XCD-aware remap of the hardware block index: the dispatcher places block i on XCD i % 8, so consecutive
-// logical items (which share landmark data) are given to blocks of the same XCD, sharing its L2.
+// XCD-aware remap of the hardware block index, a bijection of [0, nblocks): the dispatcher places block i on XCD
+// i % 8, so XCD x is given the contiguous logical range of its blocks (the first nblocks % 8 XCDs one block more),
+// and neighbouring logical blocks, which read neighbouring data, share that XCD's L2.
 __device__ inline int64_t xcd_remap(int64_t bid, int64_t nblocks) {
-    const int64_t per = (nblocks + 7) / 8;
-    return (bid % 8) * per + bid / 8;
+    const int64_t q = nblocks / 8, r = nblocks % 8, x = bid % 8, slot = bid / 8;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
 }
 
 // Event-based timing of the kernels of one call (enabled per handle).

@@ -883,7 +883,10 @@ __global__ void k_scale_sparse(TS *nzval, const int64_t *S_colptr, const int64_t
 template <class TS, class TR>
 __global__ void k_reset_sparse1(TS *nzval, const TR *R, const int32_t *code, const int2 *rc, const int64_t *diag_inds,
                                 int64_t nnz, int64_t nr, const double *scaling_r, double lambda, const double *s_r) {
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nnz + nr; x += (int64_t)gridDim.x * blockDim.x) {
+    // blocks of one XCD take a contiguous range: the R entries an S column reads (R is stored the other way round)
+    // are shared by the neighbouring S columns, in that XCD's L2
+    const int64_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    for (int64_t x = lb * blockDim.x + threadIdx.x; x < nnz + nr; x += (int64_t)gridDim.x * blockDim.x) {
         if (x < nnz) {
             const int32_t c = code[x];
             if (c & 1) continue;   // a diagonal entry: below
@@ -926,7 +929,10 @@ __global__ void k_reset_dense(TS *S, const TR *R, int64_t nr, const double *scal
 
 template <class T>
 __global__ void k_gather(T *dest, const T *src, const int64_t *map, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    // XCD-contiguous ranges: neighbouring CSC columns gather from the same stored blocks (the lower triangle reads S
+    // the other way round), in one XCD's L2
+    const int64_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    for (int64_t i = lb * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dest[i] = src[map[i] - 1];
 }
 
