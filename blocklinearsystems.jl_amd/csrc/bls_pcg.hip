@@ -174,7 +174,39 @@ __global__ __launch_bounds__(256) void k_bsr_spmv_u(PcgArgs a, const double *xin
     }
 }
 
-// z_b = (L L')^-1 r_b for one block (solve_diagonal_blocks!, src/diagonal_blocks.jl:33-61)
+// z_b = (L L')^-1 r_b for one block of d <= DM (the kernels' UD): the loops unrolled to DM and predicated on d, so
+// y stays in registers; the same operations in the same order as block_solve
+template <int DM> __device__ inline void block_solve_upto(const double *L, int d, const double *rb, double *zb) {
+    double y[DM];
+#pragma unroll
+    for (int i = 0; i < DM; i++) y[i] = i < d ? rb[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < DM; j++) {
+        if (j < d) {
+            const double yj = y[j] / L[j * d + j];
+            y[j] = yj;
+#pragma unroll
+            for (int i = j + 1; i < DM; i++)
+                if (i < d) y[i] -= L[j * d + i] * yj;
+        }
+    }
+#pragma unroll
+    for (int j = DM - 1; j >= 0; j--) {
+        if (j < d) {
+            double t = y[j];
+#pragma unroll
+            for (int i = j + 1; i < DM; i++)
+                if (i < d) t -= L[j * d + i] * y[i];
+            y[j] = t / L[j * d + j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DM; i++)
+        if (i < d) zb[i] = y[i];
+}
+
+// z_b = (L L')^-1 r_b for one block (solve_diagonal_blocks!, src/diagonal_blocks.jl:33-61), any size up to a
+// group of PC_GMAX blocks of MAXD (y is indexed at run time, so it lives in private memory)
 __device__ inline void block_solve(const double *L, int d, const double *rb, double *zb) {
     double y[PC_GMAX * MAXD];
     for (int i = 0; i < d; i++) y[i] = rb[i];
@@ -240,6 +272,8 @@ __device__ inline double problem_sum(const double *part, int64_t b0, int64_t b1,
 // k_pcg_step1): r = g - S x (q holds S x; first = 1: x = 0, r = g), z = M^-1 r, p = z, and per block the partials
 // g.g (in pq_part, free between iterations), r.r and r.z; k_pcg_restart_fin forms the problem's sums in a fixed
 // order and decides from the true relative residual whether the problem (re)enters the iteration.
+// UD > 0: every block has dimension at most UD and there are no groups (the solve in registers); 0: any
+template <int UD>
 __global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_restart_grp(PcgArgs a, int first) {
     const int pb = a.grp_prob[blockIdx.x];
     const bool ok = !a.st[pb].failed;
@@ -257,8 +291,12 @@ __global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_restart_grp(PcgArgs a, int f
         rr += rv * rv;
     }
     if (a.gsize) __syncthreads();   // a group's rows are complete before its first block solves them
-    const int gd = in ? group_dim(a, b) : 0;
-    if (ok && gd) block_solve(group_L(a, b), gd, a.r + o, a.z + o);
+    if constexpr (UD > 0) {
+        if (ok && in) block_solve_upto<UD>(a.L + a.loff[b], d, a.r + o, a.z + o);
+    } else {
+        const int gd = in ? group_dim(a, b) : 0;
+        if (ok && gd) block_solve(group_L(a, b), gd, a.r + o, a.z + o);
+    }
     if (a.gsize) __syncthreads();   // and its z before the other blocks read theirs
     if (!in) return;
     double rz = 0.0;
@@ -292,7 +330,9 @@ __global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_restart_fin(PcgArgs a, int f
     }
 }
 
-// PCG iteration, after q = S p (Saad Alg. 9.1, lines 3-5): alpha, x, r, z = M^-1 r, partial r.z and r.r
+// PCG iteration, after q = S p (Saad Alg. 9.1, lines 3-5): alpha, x, r, z = M^-1 r, partial r.z and r.r. UD as in
+// k_pcg_restart_grp.
+template <int UD>
 __global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_step1(PcgArgs a, int parity) {
     __shared__ double red[GRP_BLOCKS];
     const int pb = a.grp_prob[blockIdx.x];
@@ -317,9 +357,10 @@ __global__ __launch_bounds__(GRP_BLOCKS) void k_pcg_step1(PcgArgs a, int parity)
     // GRP_BLOCKS: they start at multiples of their size, a divisor of GRP_BLOCKS, from their problem's first block)
     if (a.gsize) __syncthreads();
     if (!in) return;
-    const int gd = group_dim(a, b);
+    const int gd = UD > 0 ? a.dims[b] : group_dim(a, b);
     if (gd) {
-        block_solve(group_L(a, b), gd, a.r + o, a.z + o);
+        if constexpr (UD > 0) block_solve_upto<UD>(a.L + a.loff[b], gd, a.r + o, a.z + o);
+        else block_solve(group_L(a, b), gd, a.r + o, a.z + o);
         // r.z per block, as the plain preconditioner sums it (the group's other blocks' partials by this thread)
         const int gs = a.gsize ? a.gsize[b] : 1;
         int64_t oo = o;
@@ -934,6 +975,22 @@ static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double 
         launch_spmv(h, a, xin, yout, pq, only_active, s);
     };
     const unsigned ngroups = (unsigned)h->ngroups;
+    // the register-resident block solve, unrolled to the largest block, when there are no groups
+    const int ud = groups ? 0 : h->maxd <= 3 ? 3 : h->maxd <= 6 ? 6 : h->maxd <= 9 ? 9 : h->maxd <= 16 ? 16 : 0;
+    auto restart = [&](int first) {
+        if (ud == 3) k_pcg_restart_grp<3><<<ngroups, GRP_BLOCKS, 0, s>>>(a, first);
+        else if (ud == 6) k_pcg_restart_grp<6><<<ngroups, GRP_BLOCKS, 0, s>>>(a, first);
+        else if (ud == 9) k_pcg_restart_grp<9><<<ngroups, GRP_BLOCKS, 0, s>>>(a, first);
+        else if (ud == 16) k_pcg_restart_grp<16><<<ngroups, GRP_BLOCKS, 0, s>>>(a, first);
+        else k_pcg_restart_grp<0><<<ngroups, GRP_BLOCKS, 0, s>>>(a, first);
+    };
+    auto step1 = [&](int parity) {
+        if (ud == 3) k_pcg_step1<3><<<ngroups, GRP_BLOCKS, 0, s>>>(a, parity);
+        else if (ud == 6) k_pcg_step1<6><<<ngroups, GRP_BLOCKS, 0, s>>>(a, parity);
+        else if (ud == 9) k_pcg_step1<9><<<ngroups, GRP_BLOCKS, 0, s>>>(a, parity);
+        else if (ud == 16) k_pcg_step1<16><<<ngroups, GRP_BLOCKS, 0, s>>>(a, parity);
+        else k_pcg_step1<0><<<ngroups, GRP_BLOCKS, 0, s>>>(a, parity);
+    };
     k_gather_f64<<<std::min<int64_t>(grid_for(h->nvals, 256), 16384), 256, 0, s>>>(h->d_vals.ptr, S_nzval, h->d_map.ptr, h->nvals);
     k_copy_f64<<<std::min<int64_t>(grid_for(h->nr, 256), 4096), 256, 0, s>>>(h->d_g.ptr, g, h->nr);
     k_pcg_init_state<<<grid_for(nprob, 256), 256, 0, s>>>(h->d_st.ptr, nprob, h->d_nactive.ptr);
@@ -941,7 +998,7 @@ static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double 
         k_pcg_factor_groups_w<<<(unsigned)h->nrb, 64, 0, s>>>(a, h->nrb, h->d_Lp.ptr, precond_blocks);
     else
         k_pcg_factor<<<grid_for(h->nrb, 128), 128, 0, s>>>(a, h->nrb, h->d_diag_voff.ptr, h->d_L.ptr, precond_blocks);
-    k_pcg_restart_grp<<<ngroups, GRP_BLOCKS, 0, s>>>(a, 1);
+    restart(1);
     k_pcg_restart_fin<<<nprob, GRP_BLOCKS, 0, s>>>(a, 1);
     BLS_HIP(hipGetLastError());
     // the iteration graph (handle-owned buffers only; rebuilt when tol / maxit change; one per preconditioner kind)
@@ -955,7 +1012,7 @@ static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double 
         BLS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int it = 0; it < ITERS_PER_GRAPH; it++) {
             spmv(h->d_p.ptr, h->d_q.ptr, h->d_pq.ptr, 1);
-            k_pcg_step1<<<ngroups, GRP_BLOCKS, 0, s>>>(a, it & 1);
+            step1(it & 1);
             k_pcg_step2<<<ngroups, GRP_BLOCKS, 0, s>>>(a, it & 1);
         }
         hipError_t e = hipStreamEndCapture(s, &graph);
@@ -977,7 +1034,7 @@ static int pcg_solve(bls_pcg *h, const double *S_nzval, const double *g, double 
             BLS_HIP(hipStreamSynchronize(s));
         }
         spmv(h->d_x.ptr, h->d_q.ptr, nullptr, 0);
-        k_pcg_restart_grp<<<ngroups, GRP_BLOCKS, 0, s>>>(a, 0);
+        restart(0);
         k_pcg_restart_fin<<<nprob, GRP_BLOCKS, 0, s>>>(a, 0);
         BLS_HIP(hipGetLastError());
         BLS_HIP(hipMemcpyAsync(&nact, h->d_nactive.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
