@@ -72,8 +72,9 @@ def main():
     if args.slam_only:
         args.no_cpu, args.qr_scale, args.config5_scale, args.config3_scale = True, 0.0, 0.0, 0.0
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no multi-GPU path; "
-              "see the sharding design in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the library has no multi-GPU path "
+              "(the reduction shards by independent problems or by landmark ranges, one process per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
