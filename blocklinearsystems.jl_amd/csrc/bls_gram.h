@@ -64,10 +64,16 @@ __global__ __launch_bounds__(NT, 4) void k_red_gram(const RGTab *__restrict__ ta
     // (a, b) of X'X goes to pair-block offset (s_colB[b] >> 5) + (s_colB[b] & 31) s_colA[a]
     __shared__ int16_t s_colA[RG_MAXW];
     __shared__ int s_colB[RG_MAXW];
-    double pf[PF];
+    // the next table's staged values as loaded: Float32 tables keep theirs in Float32 registers, widened as they go
+    // into LDS, and a lane's one residual value (the residual column holds np4 <= 64 < NT elements) in pres at slot
+    // jres, so that no register a pending load writes is reused and no wait for the prefetch sits in its issue
+    std::conditional_t<sizeof(T) == 8, double, float> pf[PF];
+    double pres = 0.0;
+    int jres = -1;
     int my_ecd = 0;
     auto prefetch = [&](const RGTab g) {
         my_ecd = (int)threadIdx.x < g.ne ? ecd[g.u0 + threadIdx.x] : 0;
+        jres = -1;
         const int Wp = (g.W + 15) & ~15, np4 = (g.n + 3) & ~3, SW = Wp + 1;
         if ((int64_t)np4 * SW > stage_cap) return;
         const T *Xg = F + g.raw_off + g.top;
@@ -77,10 +83,18 @@ __global__ __launch_bounds__(NT, 4) void k_red_gram(const RGTab *__restrict__ ta
         int c = threadIdx.x / np4, k = threadIdx.x - c * np4;
 #pragma unroll
         for (int j = 0; j < PF; j++) {
-            double v = 0.0;
+            T v = (T)0;
             if (c < SW && k < g.n) {
-                if (c < g.W) v = (double)Xg[(int64_t)(g.c0 + c) * g.M + k];
-                else if (c == Wp) v = rg[k];
+                if (c < g.W) {
+                    v = Xg[(int64_t)(g.c0 + c) * g.M + k];
+                } else if (c == Wp) {
+                    if constexpr (sizeof(T) == 8) {
+                        v = rg[k];
+                    } else {
+                        pres = rg[k];
+                        jres = j;
+                    }
+                }
             }
             pf[j] = v;
             c += dc;
@@ -109,7 +123,12 @@ __global__ __launch_bounds__(NT, 4) void k_red_gram(const RGTab *__restrict__ ta
             int c = threadIdx.x / np4, k = threadIdx.x - c * np4;
 #pragma unroll
             for (int j = 0; j < PF; j++) {
-                if (c < SW) XL[k * SW + c] = pf[j];
+                if (c < SW) {
+                    double v = (double)pf[j];
+                    if constexpr (sizeof(T) == 4)
+                        if (j == jres) v = pres;
+                    XL[k * SW + c] = v;
+                }
                 c += dc;
                 k += dk;
                 if (k >= np4) {
